@@ -1,0 +1,97 @@
+/*
+ * pywave_amd.h -- C ABI of the MI355X (gfx950) acoustic finite-difference engine.
+ *
+ * Plain pointers and sizes only.  Every entry point returns 0 on success or a negative
+ * PW_ERR_* code; pw_last_error() then holds a message.  Host arrays are caller-owned.
+ * Arrays follow the reference's layouts: models are z-fastest ("Fortran order" of
+ * [nz,nx(,ny)]), receiver data are time-fastest.
+ *
+ * Reference interfaces replaced (pywave = github.com/chenyk1990/pywave):
+ *   afdcfun.afd2dc            pywave/src/afd.c:316-578   (module table afd.c:585-607)
+ *   afdcfun.afd3dc            pywave/src/afd.c:23-314
+ *   pfwicfun.forward_modeling_ac  pywave/src/pfwi.c:27-495
+ *   pfwicfun.fwic             pywave/src/pfwi.c:497-1028
+ */
+#ifndef PYWAVE_AMD_H
+#define PYWAVE_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PW_OK 0
+#define PW_ERR_ARG -1      /* invalid argument / dimension mismatch */
+#define PW_ERR_HIP -2      /* HIP runtime error */
+#define PW_ERR_NOMEM -3    /* device or host allocation failed */
+#define PW_ERR_STATE -4    /* call order (e.g. run before velocity was set) */
+
+typedef struct pw_ctx pw_ctx; /* one device + one stream; owns no global state */
+typedef struct pw_afd pw_afd; /* one afd modeling / time-reversal problem resident in HBM */
+
+/* ---------------------------------------------------------------- context */
+pw_ctx *pw_create(int device);
+void pw_destroy(pw_ctx *ctx);
+const char *pw_last_error(pw_ctx *ctx);
+int pw_device_count(void);
+/* Run subsequent work on a caller stream (hipStream_t passed as void*; NULL = own stream). */
+int pw_set_stream(pw_ctx *ctx, void *stream);
+int pw_synchronize(pw_ctx *ctx);
+
+/* ---------------------------------------------------------------- host setup helpers */
+/* kiss_fft_next_fast_size (pywave/src/wave_kissfft.c:412-424) */
+int pw_next_fast_size(int n);
+/* fdm/fdm2d padded extents (pywave/src/wave_fft3.c:45-60, wave_fft2.c:45-51); ny2 = 1 for ny <= 1 */
+void pw_padded_sizes(int nz, int nx, int ny, int *nz2, int *nx2, int *ny2);
+/* abc_cal weights (pywave/src/wave_abc.c:127-145) */
+void pw_abc_weights(int nb, float c, float *w);
+/* the zero-phase Ricker fdm/fdm2d build per source (pywave/src/wave_fdm.c:502-516) */
+int pw_ricker(int nt, float dt, float f0, float t0, float A, float *out);
+
+/* ---------------------------------------------------------------- afd: one-shot drop-ins */
+/* afdcfun.afd2dc (pywave/src/afd.c:316-578).  Same argument meaning and order; vel is nz*nx
+ * z-fastest; src is [sx(ns) sz(ns) f(ns) t(ns) A(ns)] or, when tri != 0, the nt*nx receiver data
+ * (time fastest).  out receives exactly the reference's return vector:
+ * nt*nx traces (time fastest) followed by nz*nx*ntsnap snapshots when jsnap > 0.
+ * verb, ifsnaps, abc, ox, oz are accepted and ignored as in the reference (abc is forced on). */
+int pw_afd2d(pw_ctx *ctx, const float *vel, const float *src, int tri, int nt, int nx, int nz,
+             int ns, int verb, int jsnap, int ifsnaps, int abc, int nbt, float ct, float dt,
+             float ox, float dx, float oz, float dz, float *out, long long out_len);
+
+/* afdcfun.afd3dc (pywave/src/afd.c:23-314); src is [sx sy sz f t A] (6*ns) or nt*nx*ny data. */
+int pw_afd3d(pw_ctx *ctx, const float *vel, const float *src, int tri, int nt, int nx, int ny,
+             int nz, int ns, int verb, int jsnap, int ifsnaps, int abc, int nbt, float ct,
+             float dt, float ox, float dx, float oy, float dy, float oz, float dz, float *out,
+             long long out_len);
+
+/* length of the afd return vector: nt*nx*ny + nz*nx*ny*ntsnap (ny = 1 in 2D) */
+long long pw_afd_output_len(int nt, int nz, int nx, int ny, int jsnap);
+
+/* ---------------------------------------------------------------- afd: resident plans
+ * The same engine split into set-up, run and read-back so that models stay in HBM and the
+ * step loop can be timed on its own.  ndim is 2 or 3 (ny, dy ignored in 2D).
+ * Sources: src_pack as in pw_afd2d/pw_afd3d; wavelet (ns*nt, row per source) overrides the
+ * built-in Ricker when non-NULL. */
+pw_afd *pw_afd_create(pw_ctx *ctx, int ndim, int nz, int nx, int ny, float dz, float dx,
+                      float dy, int nt, float dt, int nbt, float ct, int jsnap);
+int pw_afd_set_velocity(pw_afd *p, const float *vel);
+int pw_afd_set_sources(pw_afd *p, int ns, const float *src_pack, const float *wavelet);
+int pw_afd_set_tri_data(pw_afd *p, const float *dat);
+/* Enqueue the whole time loop (forward modelling, or backward time-reversal when tri != 0). */
+int pw_afd_run(pw_afd *p, int tri);
+/* As pw_afd_run, bracketed by HIP events on the launch stream: total milliseconds and the
+ * average duration of one step kernel (ms), measured by events around every step launch
+ * when per_step != 0 (adds event overhead), else total/nt. Synchronises. */
+int pw_afd_run_timed(pw_afd *p, int tri, int per_step, double *total_ms, double *step_ms);
+/* Copy the result in the reference's return layout (see pw_afd2d). Synchronises. */
+int pw_afd_output(pw_afd *p, float *out, long long out_len);
+/* Final wavefield interior after a run (nz*nx*ny, z fastest): the reference's 'img'. */
+int pw_afd_final_field(pw_afd *p, float *out, long long out_len);
+/* Kernel tile choice for the 3D step (0 = default); returns PW_ERR_ARG for unknown ids. */
+int pw_afd_set_variant(pw_afd *p, int variant, int ny_chunk);
+int pw_afd_variant_count(void);
+void pw_afd_destroy(pw_afd *p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,252 @@
+"""ctypes bindings to the oracle (oracle/liborc.so) plus its numpy pieces.
+
+TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).
+
+What pins each function (tests/test_oracle_*.py):
+  next_fast_size, padded_sizes   brute-force enumeration of 2^a 3^b 5^c; golden afd outputs
+  abc_weights                    closed form + w[nb-1] == 1; golden afd outputs (bit-exact)
+  ricker                         golden wavelets from the compiled reference (rel <= 1e-6),
+                                 zero-phase symmetry, peak at the onset index
+  afd2d, afd3d                   golden traces/snapshots from the compiled reference, bit-exact
+                                 when fed the reference's wavelet; linearity in amplitude
+  fwi_prepare geometry           closed-form integer formulas (pywave/src/wave_fwiutil.c:45-60)
+  fwi_forward, gradient_av, fwi  golden outputs from the compiled reference, bit-exact
+  source_map, gradient_smooth2   numpy convolution with the reflected triangle (independent form)
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+class OrcAcq(ctypes.Structure):
+    """Mirror of orc_acq in oracle/orc.h."""
+    _fields_ = [
+        ("nz", ctypes.c_int), ("nx", ctypes.c_int), ("nt", ctypes.c_int), ("ns", ctypes.c_int),
+        ("nb", ctypes.c_int),
+        ("dz", ctypes.c_float), ("dx", ctypes.c_float), ("dt", ctypes.c_float),
+        ("z0", ctypes.c_float), ("x0", ctypes.c_float), ("t0", ctypes.c_float),
+        ("coef", ctypes.c_float), ("acqui_type", ctypes.c_int),
+        ("ds", ctypes.c_float), ("s0", ctypes.c_float), ("r0", ctypes.c_float),
+        ("dr", ctypes.c_float), ("nr", ctypes.c_int), ("interval", ctypes.c_int),
+        ("padnz", ctypes.c_int), ("padnx", ctypes.c_int), ("sz", ctypes.c_int),
+        ("rz", ctypes.c_int), ("ds_v", ctypes.c_int), ("s0_v", ctypes.c_int),
+        ("dr_v", ctypes.c_int),
+        ("r0_v", ctypes.POINTER(ctypes.c_int)), ("r02", ctypes.POINTER(ctypes.c_int)),
+        ("nr2", ctypes.POINTER(ctypes.c_int)), ("bc", ctypes.POINTER(ctypes.c_float)),
+    ]
+
+
+def lib():
+    """Load oracle/liborc.so (built by ``make -C oracle``)."""
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liborc.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liborc.so missing: run `make -C oracle`")
+        L = ctypes.CDLL(path)
+        L.orc_next_fast_size.argtypes = [ctypes.c_int]
+        L.orc_next_fast_size.restype = ctypes.c_int
+        L.orc_abc_weights.argtypes = [ctypes.c_int, ctypes.c_float, _f32p]
+        L.orc_afd2d.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                ctypes.c_float, ctypes.c_float, ctypes.c_int, ctypes.c_float,
+                                ctypes.c_int, _i32p, _i32p, _f32p, ctypes.c_int, _f32p,
+                                ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_afd3d.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                ctypes.c_int, ctypes.c_float, ctypes.c_int, _i32p, _i32p, _i32p,
+                                _f32p, ctypes.c_int, _f32p, ctypes.c_int, ctypes.c_void_p,
+                                ctypes.c_void_p]
+        L.orc_fwi_prepare.argtypes = [ctypes.POINTER(OrcAcq)]
+        L.orc_fwi_release.argtypes = [ctypes.POINTER(OrcAcq)]
+        L.orc_fwi_forward.argtypes = [ctypes.POINTER(OrcAcq), _f32p, _f32p, _f32p]
+        L.orc_gradient_av.argtypes = [ctypes.POINTER(OrcAcq), _f32p, _f32p, _f32p, _f32p,
+                                      ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
+        L.orc_fwi.argtypes = [ctypes.POINTER(OrcAcq), _f32p, _f32p, _f32p, ctypes.c_int, _f32p]
+        L.orc_fwi.restype = ctypes.c_int
+        L.orc_source_map.argtypes = [ctypes.c_int] * 6 + [_f32p]
+        L.orc_gradient_smooth2.argtypes = [ctypes.c_int] * 5 + [ctypes.c_float, _f32p]
+        _LIB = L
+    return _LIB
+
+
+# ------------------------------------------------------------------ geometry
+
+def next_fast_size(n):
+    return lib().orc_next_fast_size(int(n))
+
+
+def padded_sizes(nz, nx, ny=1):
+    """(nz2, nx2, ny2) of pywave/src/wave_fft3.c:45-60 (first axis real-to-complex)."""
+    return 2 * next_fast_size((nz + 1) // 2), next_fast_size(nx), next_fast_size(ny)
+
+
+def abc_weights(nb, c):
+    w = np.zeros(max(nb, 1), np.float32)
+    lib().orc_abc_weights(int(nb), np.float32(c), w)
+    return w[:nb]
+
+
+def count_snaps(nt, jsnap):
+    return sum(1 for it in range(nt) if jsnap and it % jsnap == 0)
+
+
+# ------------------------------------------------------------------ wavelet
+
+def ricker(nt, dt, f0, t0, A):
+    """Zero-phase Ricker built like pywave/src/wave_ricker.c:22-59 + wave_freqfilt.c:141-164:
+    a spike of height A at int(f32(t0)/f32(dt)), circularly filtered over nfft samples by the
+    real spectrum w^2 e^(1-w^2)/nfft (w = iw/(nfft*f0*dt), formed in float32).  The transform
+    itself is numpy's double-precision FFT (the reference uses KISS FFT in float32)."""
+    nt = int(nt)
+    dt32, f32_, t32 = np.float32(dt), np.float32(f0), np.float32(t0)
+    k0 = int(t32 / dt32)
+    spike = np.zeros(nt, np.float64)
+    spike[k0] = np.float32(A)
+    freq = np.float32(f32_ * dt32)
+    nfft = 2 * next_fast_size((nt + 1) // 2)
+    nw = nfft // 2 + 1
+    dw = np.float32(1.0 / (nfft * float(freq)))
+    w = np.arange(nw, dtype=np.float32) * dw
+    w = w * w
+    shape = (w * np.exp(np.float32(1) - w) / np.float32(nfft)).astype(np.float64)
+    buf = np.zeros(nfft, np.float64)
+    buf[:nt] = spike
+    spec = np.fft.rfft(buf) * shape
+    out = np.fft.irfft(spec, nfft) * nfft
+    return out[:nt].astype(np.float32)
+
+
+def ricker_bank(nt, dt, f, t, A):
+    return np.stack([ricker(nt, dt, f[i], t[i], A[i]) for i in range(len(f))]).astype(np.float32)
+
+
+# ------------------------------------------------------------------ afd engine
+
+def afd2d(vel, nt, dt, dx, dz, nbt, ct, sx, sz, rick, tri=False, dat=None, jsnap=0):
+    """Oracle afd2d.  vel (nz,nx); rick (ns,nt).  Returns (traces[nx,nt], snaps, img) in the
+    reference's internal layouts: traces [ix][it], snaps [isnap][ix][iz], img [ix][iz]."""
+    vel = np.asfortranarray(vel, dtype=np.float32)
+    nz1, nx1 = vel.shape
+    velf = np.ascontiguousarray(vel.ravel(order="F"))
+    ns = len(sx)
+    spx = np.ascontiguousarray(np.asarray(sx, np.float32).astype(np.int32))
+    spz = np.ascontiguousarray(np.asarray(sz, np.float32).astype(np.int32))
+    rick = np.ascontiguousarray(rick if rick is not None else np.zeros((1, nt)), np.float32)
+    d = np.zeros((nx1, nt), np.float32) if not tri else np.ascontiguousarray(dat, np.float32).copy()
+    nsnap = count_snaps(nt, jsnap)
+    snaps = np.zeros((nsnap, nx1, nz1), np.float32) if jsnap > 0 else None
+    img = np.zeros((nx1, nz1), np.float32) if tri else None
+    lib().orc_afd2d(velf, nz1, nx1, nt, dt, dx, dz, nbt, ct, ns if not tri else 0,
+                    spx if ns else np.zeros(1, np.int32), spz if ns else np.zeros(1, np.int32),
+                    rick, int(bool(tri)), d, int(jsnap),
+                    snaps.ctypes.data if snaps is not None else None,
+                    img.ctypes.data if img is not None else None)
+    return d, snaps, img
+
+
+def afd3d(vel, nt, dt, dx, dy, dz, nbt, ct, sx, sy, sz, rick, tri=False, dat=None, jsnap=0):
+    """Oracle afd3d.  vel (nz,nx,ny).  traces [iy][ix][it], snaps [isnap][iy][ix][iz]."""
+    vel = np.asarray(vel, np.float32)
+    nz1, nx1, ny1 = vel.shape
+    velf = np.ascontiguousarray(vel.ravel(order="F"))
+    ns = len(sx)
+    spx = np.ascontiguousarray(np.asarray(sx, np.float32).astype(np.int32))
+    spy = np.ascontiguousarray(np.asarray(sy, np.float32).astype(np.int32))
+    spz = np.ascontiguousarray(np.asarray(sz, np.float32).astype(np.int32))
+    rick = np.ascontiguousarray(rick if rick is not None else np.zeros((1, nt)), np.float32)
+    d = np.zeros((ny1, nx1, nt), np.float32) if not tri else np.ascontiguousarray(dat, np.float32).copy()
+    nsnap = count_snaps(nt, jsnap)
+    snaps = np.zeros((nsnap, ny1, nx1, nz1), np.float32) if jsnap > 0 else None
+    img = np.zeros((ny1, nx1, nz1), np.float32) if tri else None
+    z1 = np.zeros(1, np.int32)
+    lib().orc_afd3d(velf, nz1, nx1, ny1, nt, dt, dx, dy, dz, nbt, ct, ns if not tri else 0,
+                    spx if ns else z1, spy if ns else z1, spz if ns else z1, rick,
+                    int(bool(tri)), d, int(jsnap),
+                    snaps.ctypes.data if snaps is not None else None,
+                    img.ctypes.data if img is not None else None)
+    return d, snaps, img
+
+
+# ------------------------------------------------------------------ FWI
+
+def make_acq(pararray):
+    """orc_acq from the 20-float pararray of pywave/pfwi.py:40-61 (fields as pfwi.c:697-716,
+    with the hard-coded sz=5, rz=1, nr=nx, dr=dx, r0=ox of pfwi.c:781-806)."""
+    p = np.asarray(pararray, np.float32)
+    a = OrcAcq()
+    a.nz, a.nx = int(p[0]), int(p[1])
+    a.dz, a.dx = p[2], p[3]
+    a.z0, a.x0 = p[4], p[5]
+    a.nt, a.dt, a.t0 = int(p[6]), p[7], p[8]
+    a.ns, a.ds, a.s0 = int(p[10]), p[11], p[12]
+    a.nb, a.coef = int(p[14]), p[15]
+    a.acqui_type, a.interval = int(p[17]), int(p[18])
+    a.nr, a.dr, a.r0 = a.nx, a.dx, a.x0
+    lib().orc_fwi_prepare(ctypes.byref(a))
+    return a
+
+
+def acq_geometry(a):
+    ns = a.ns
+    return dict(padnz=a.padnz, padnx=a.padnx, sz=a.sz, rz=a.rz, ds_v=a.ds_v, s0_v=a.s0_v,
+                dr_v=a.dr_v, r0_v=[a.r0_v[i] for i in range(ns)], r02=[a.r02[i] for i in range(ns)],
+                nr2=[a.nr2[i] for i in range(ns)], bc=np.array([a.bc[i] for i in range(a.nb)], np.float32))
+
+
+def fwi_forward(pararray, vel, wav):
+    """forward_modeling_a: returns data (nt, nx, ns) like pywave/pfwi.py:70-74."""
+    a = make_acq(pararray)
+    velf = np.ascontiguousarray(np.asarray(vel, np.float32).ravel(order="F"))
+    out = np.zeros(a.ns * a.nr * a.nt, np.float32)
+    lib().orc_fwi_forward(ctypes.byref(a), velf, np.ascontiguousarray(wav, np.float32), out)
+    res = out.reshape(a.nt, a.nr, a.ns, order="F")
+    lib().orc_fwi_release(ctypes.byref(a))
+    return res
+
+
+def gradient_av(pararray, vel, wav, data, scaling=0.0):
+    """One gradient_av call.  data (nt, nx, ns).  Returns (grad (nz,nx), fcost, scaling)."""
+    a = make_acq(pararray)
+    velf = np.ascontiguousarray(np.asarray(vel, np.float32).ravel(order="F"))
+    dflat = np.ascontiguousarray(np.asarray(data, np.float32).ravel(order="F"))
+    g = np.zeros(a.nz * a.nx, np.float32)
+    fc = ctypes.c_float(0)
+    sc = ctypes.c_float(scaling)
+    lib().orc_gradient_av(ctypes.byref(a), velf, np.ascontiguousarray(wav, np.float32), dflat, g,
+                          ctypes.byref(fc), ctypes.byref(sc))
+    nz, nx = a.nz, a.nx
+    lib().orc_fwi_release(ctypes.byref(a))
+    return g.reshape(nz, nx, order="F"), fc.value, sc.value
+
+
+def fwi(pararray, vel, wav, data):
+    """fwi(): returns vinv (nz, nx, niter) and the number of iterates stored."""
+    a = make_acq(pararray)
+    niter = int(np.float32(pararray[19]))
+    velf = np.ascontiguousarray(np.asarray(vel, np.float32).ravel(order="F"))
+    dflat = np.ascontiguousarray(np.asarray(data, np.float32).ravel(order="F"))
+    vinv = np.zeros(a.nz * a.nx * niter, np.float32)
+    k = lib().orc_fwi(ctypes.byref(a), velf, np.ascontiguousarray(wav, np.float32), dflat, niter, vinv)
+    nz, nx = a.nz, a.nx
+    lib().orc_fwi_release(ctypes.byref(a))
+    return vinv.reshape(nz, nx, niter, order="F"), k
+
+
+def source_map(sx, sz, rectx, rectz, padnx, padnz):
+    rr = np.zeros(padnx * padnz, np.float32)
+    lib().orc_source_map(sx, sz, rectx, rectz, padnx, padnz, rr)
+    return rr.reshape(padnz, padnx, order="F")
+
+
+def gradient_smooth2(grad, rectx, rectz, waterz, scaling):
+    nz, nx = grad.shape
+    g = np.ascontiguousarray(np.asarray(grad, np.float32).ravel(order="F")).copy()
+    lib().orc_gradient_smooth2(rectx, rectz, nx, nz, waterz, np.float32(scaling), g)
+    return g.reshape(nz, nx, order="F")

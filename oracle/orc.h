@@ -1,0 +1,86 @@
+/*
+ * orc.h -- the oracle: a plain-C CPU restatement of pywave's acoustic FD engine and
+ * FWI gradient.  TEST INFRASTRUCTURE ONLY: tests/, the smoke check and the CPU-baseline
+ * leg of bench.py may call it, as the checker; the product never does.
+ *
+ * Arrays are z-fastest ("Fortran order" of [nz,nx(,ny)]) as in the reference.
+ */
+#ifndef ORC_H
+#define ORC_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* kiss_fft_next_fast_size (pywave/src/wave_kissfft.c:412-424): smallest m >= n whose only
+ * prime factors are 2, 3 and 5. */
+int orc_next_fast_size(int n);
+
+/* Padded extents used by fdm2d/fdm (pywave/src/wave_fft2.c:45-51, wave_fft3.c:45-60):
+ * first axis 2*nfs((n+1)/2), the others nfs(n). */
+void orc_padded_sizes(int nz, int nx, int ny, int *nz2, int *nx2, int *ny2);
+
+/* abc_cal (pywave/src/wave_abc.c:127-145): w[i] = exp(-c*c*(nb-1-i)^2), f32 products,
+ * exp in double. */
+void orc_abc_weights(int nb, float c, float *w);
+
+/* afd2d (pywave/src/afd.c:316-578 + pywave/src/wave_fdm.c:397-686).
+ *   vel    : nz1*nx1 model velocity
+ *   spx/spz: ns source grid indices (already truncated to int)
+ *   rick   : ns*nt wavelets (row per source), used when tri==0
+ *   dat    : nx1*nt receiver data [ix][it]; input when tri!=0, output otherwise
+ *   snaps  : ntsnap*nz1*nx1 or NULL;  img: nz1*nx1 or NULL (tri only)
+ * Returns 0. */
+int orc_afd2d(const float *vel, int nz1, int nx1, int nt, float dt, float dx, float dz,
+              int nbt, float ct, int ns, const int *spx, const int *spz, const float *rick,
+              int tri, float *dat, int jsnap, float *snaps, float *img);
+
+/* afd3d (pywave/src/afd.c:23-314 + pywave/src/wave_fdm.c:28-381).
+ *   dat : nx1*ny1*nt receiver plane data [iy][ix][it]. */
+int orc_afd3d(const float *vel, int nz1, int nx1, int ny1, int nt, float dt, float dx,
+              float dy, float dz, int nbt, float ct, int ns, const int *spx, const int *spy,
+              const int *spz, const float *rick, int tri, float *dat, int jsnap, float *snaps,
+              float *img);
+
+/* ---- FWI discretisation (pywave/src/wave_fwiutil.c, wave_fwimodeling.c,
+ *      wave_fwigradient.c, wave_fwi.c, wave_fwilbfgs.c) ---- */
+
+typedef struct {
+    int nz, nx, nt, ns, nb;
+    float dz, dx, dt, z0, x0, t0;
+    float coef;
+    int acqui_type;
+    float ds, s0, r0, dr;
+    int nr;
+    int interval;
+    /* derived by orc_fwi_prepare (wave_fwiutil.c:19-121) */
+    int padnz, padnx, sz, rz, ds_v, s0_v, dr_v;
+    int *r0_v, *r02, *nr2;  /* per shot, allocated by orc_fwi_prepare */
+    float *bc;              /* nb sponge weights */
+} orc_acq;
+
+int orc_fwi_prepare(orc_acq *a);
+void orc_fwi_release(orc_acq *a);
+
+/* forward_modeling_a: data[is][ir][it] (ns*nr*nt) */
+int orc_fwi_forward(const orc_acq *a, const float *vel, const float *wav, float *data);
+
+/* One gradient_av call.  'state' carries the first-call scaling across calls:
+ * *scaling <= 0 on entry means "first call" (compute 0.1/max, store it). */
+int orc_gradient_av(const orc_acq *a, const float *vel, const float *wav, const float *data,
+                    float *grad, float *fcost, float *scaling);
+
+/* fwi(): L-BFGS with Wolfe line search over orc_gradient_av; vinv is niter*nz*nx,
+ * slots after an early stop are left untouched. Returns the number of stored iterates. */
+int orc_fwi(const orc_acq *a, const float *vel0, const float *wav, const float *data,
+            int niter, float *vinv);
+
+/* Triangle smoothing pieces used by source_map / gradient_smooth2 */
+void orc_source_map(int sx, int sz, int rectx, int rectz, int padnx, int padnz, float *rr);
+void orc_gradient_smooth2(int rectx, int rectz, int nx, int nz, int waterz, float scaling,
+                          float *grad);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

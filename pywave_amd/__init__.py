@@ -1,0 +1,15 @@
+"""pywave_amd -- MI355X-native acoustic finite-difference engine with pywave's Python API.
+
+Drop-ins for the reference's hot path (github.com/chenyk1990/pywave):
+  afd2d, afd3d   2D/3D acoustic FD modelling and time-reversal imaging (pywave/afd2d.py, afd3d.py)
+  pfwi, fillpar  FWI forward modelling (mode 1) and L-BFGS FWI (mode 2) (pywave/pfwi.py)
+Extensions beyond the reference's Python surface:
+  AfdPlan        resident modelling problem (set-up once, run many times)
+  tri3d_image    batched time-reversal imaging of receiver groups with a fused image
+  FwiProblem     FWI gradients (reference imaging condition or exact adjoint), batched shots
+  FwiForward     torch.autograd.Function wrapper of the FWI forward map and its adjoint
+All numerics run in hand-written HIP kernels (pywave_amd/csrc); there is no CPU fallback.
+"""
+from .afd import afd2d, afd3d, AfdPlan, count_snaps  # noqa: F401
+
+__version__ = "0.1.0"

@@ -1,0 +1,108 @@
+"""ctypes binding of libpywave_amd.so, the HIP library (declared in include/pywave_amd.h).
+
+The library is built in-tree by ``make -C pywave_amd/csrc`` (or ``__graft_entry__.build()``).
+There is no CPU fallback: every entry point raises if the library or a GPU is missing.
+"""
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpywave_amd.so")
+
+_lib = None
+_ctx = {}
+_lock = threading.Lock()
+
+c_int, c_float, c_double, c_ll, c_vp = ctypes.c_int, ctypes.c_float, ctypes.c_double, ctypes.c_longlong, ctypes.c_void_p
+_fp = ctypes.POINTER(ctypes.c_float)
+
+# name -> (restype, argtypes); every exported symbol of include/pywave_amd.h
+SIGNATURES = {
+    "pw_create": (c_vp, [c_int]),
+    "pw_destroy": (None, [c_vp]),
+    "pw_last_error": (ctypes.c_char_p, [c_vp]),
+    "pw_device_count": (c_int, []),
+    "pw_set_stream": (c_int, [c_vp, c_vp]),
+    "pw_synchronize": (c_int, [c_vp]),
+    "pw_next_fast_size": (c_int, [c_int]),
+    "pw_padded_sizes": (None, [c_int, c_int, c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int), ctypes.POINTER(c_int)]),
+    "pw_abc_weights": (None, [c_int, c_float, c_vp]),
+    "pw_ricker": (c_int, [c_int, c_float, c_float, c_float, c_float, c_vp]),
+    "pw_afd2d": (c_int, [c_vp, c_vp, c_vp, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int,
+                         c_int, c_float, c_float, c_float, c_float, c_float, c_float, c_vp, c_ll]),
+    "pw_afd3d": (c_int, [c_vp, c_vp, c_vp, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int,
+                         c_int, c_int, c_float, c_float, c_float, c_float, c_float, c_float, c_float,
+                         c_float, c_vp, c_ll]),
+    "pw_afd_output_len": (c_ll, [c_int, c_int, c_int, c_int, c_int]),
+    "pw_afd_create": (c_vp, [c_vp, c_int, c_int, c_int, c_int, c_float, c_float, c_float, c_int, c_float,
+                             c_int, c_float, c_int]),
+    "pw_afd_set_velocity": (c_int, [c_vp, c_vp]),
+    "pw_afd_set_sources": (c_int, [c_vp, c_int, c_vp, c_vp]),
+    "pw_afd_set_tri_data": (c_int, [c_vp, c_vp]),
+    "pw_afd_run": (c_int, [c_vp, c_int]),
+    "pw_afd_run_timed": (c_int, [c_vp, c_int, c_int, ctypes.POINTER(c_double), ctypes.POINTER(c_double)]),
+    "pw_afd_output": (c_int, [c_vp, c_vp, c_ll]),
+    "pw_afd_final_field": (c_int, [c_vp, c_vp, c_ll]),
+    "pw_afd_set_variant": (c_int, [c_vp, c_int, c_int]),
+    "pw_afd_variant_count": (c_int, []),
+    "pw_afd_destroy": (None, [c_vp]),
+}
+
+
+class PywaveError(RuntimeError):
+    """A HIP-side failure reported by libpywave_amd."""
+
+
+def lib():
+    """Load the in-tree library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        with _lock:
+            if _lib is None:
+                if not os.path.exists(LIB_PATH):
+                    raise ImportError("pywave_amd: %s is missing; build it with `make -C pywave_amd/csrc` "
+                                      "(there is no CPU fallback)" % LIB_PATH)
+                L = ctypes.CDLL(LIB_PATH)
+                for name, (res, args) in SIGNATURES.items():
+                    fn = getattr(L, name)
+                    fn.restype = res
+                    fn.argtypes = args
+                _lib = L
+    return _lib
+
+
+def context(device=0):
+    """Per-device library context (created on first use)."""
+    c = _ctx.get(device)
+    if c is None:
+        with _lock:
+            c = _ctx.get(device)
+            if c is None:
+                L = lib()
+                if L.pw_device_count() <= device:
+                    raise PywaveError("pywave_amd: no HIP device %d visible (the engine has no CPU path)" % device)
+                c = L.pw_create(device)
+                if not c:
+                    raise PywaveError("pywave_amd: pw_create(%d) failed" % device)
+                _ctx[device] = c
+    return c
+
+
+def check(rc, ctx, what):
+    if rc != 0:
+        msg = lib().pw_last_error(ctx).decode(errors="replace") if ctx else ""
+        if rc == -1:
+            raise ValueError("%s: %s" % (what, msg))
+        raise PywaveError("%s failed (%d): %s" % (what, rc, msg))
+
+
+def ptr(a):
+    """Raw data pointer of a C-contiguous numpy array."""
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def f32c(a):
+    return np.ascontiguousarray(a, dtype=np.float32)

@@ -1,0 +1,736 @@
+// pw_afd.hip -- afd2d/afd3d on MI355X: one fused kernel per time step.
+//
+// Reference path: pywave/src/wave_fdm.c:397-686 (fdm2d) and :28-381 (fdm), set up by
+// pywave/src/afd.c:23-578.  Per step the reference does, in separate passes,
+//   (i)   wave = 4th-order Laplacian of curr on [2, n2-2) per axis (0 on the rim)
+//   (ii)  c = curr; c += c - prev; prev = c_old; c += wave*vv      (updated domain [0,n))
+//   (iii) source 3x3(x3) injection  curr += (vv*rick)/(|dz|+|dx|+|dy|+1)   or TRI plane
+//   (iv)  damp curr and damp prev (faces applied top, bottom, lx, rx, ly, ry)
+//   (v)   record the receiver row/plane, (vi) snapshot every jsnap steps.
+// Here one kernel does all of it per cell.  The second damping of prev is applied when prev
+// is read in the next step: the kernel reads C_n (already damped) and the raw C_{n-1},
+// forms D(C_{n-1}) on the fly and writes D(C_n + (C_n - D(C_{n-1})) + lap*vv [+src]) over
+// C_{n-1}.  Every cell uses the reference's f32 operation order; the library is built with
+// -ffp-contract=off, so results are bit-identical to the reference C.
+//
+// HBM layout: z fastest, padded extents nz2 x nx2 (x ny2) of the reference with the z stride
+// rounded up to 32 floats (128 B rows), slack cells zero.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "pw_internal.h"
+
+namespace {
+
+constexpr float C0 = -30. / 12., C1 = +16. / 12., C2 = -1. / 12.;  // pywave/src/wave_fdm.c:15
+constexpr int LDZ_ALIGN = 32;
+
+struct Geo {
+    int nz, nx, ny;     // updated domain (model + 2*nb)
+    int nz2, nx2, ny2;  // padded extents
+    int ldz;            // z stride
+    long long ldxy;     // plane stride (ldz * nx2)
+    int nb;             // absorbing width, all faces
+    int nz1, nx1, ny1;  // model
+    float idz, idx, idy, cc;
+};
+
+struct Step {
+    const float* __restrict__ cur;  // C_n (damped)
+    float* __restrict__ pv;         // in: C_{n-1} undamped-once; out: C_{n+1}
+    const float* __restrict__ vv;   // v^2 dt^2, same layout
+    const float2* __restrict__ fz;  // damping (lo, hi) per index
+    const float2* __restrict__ fx;
+    const float2* __restrict__ fy;
+    const int* __restrict__ src;    // 3*ns: gz, gx, gy in padded coordinates
+    const float* __restrict__ wav;  // ns*nt
+    const float* __restrict__ tri;  // this step's receiver plane [ny1][nx1] or null
+    float* __restrict__ rec;        // this step's receiver plane or null
+    float* __restrict__ snap;       // this snapshot [ny1][nx1][nz1] or null
+    int ns, nt, it;
+};
+
+__device__ __forceinline__ float damp3(float v, float2 a, float2 b, float2 c)
+{
+    v = v * a.x; v = v * a.y;  // top, bottom
+    v = v * b.x; v = v * b.y;  // left x, right x
+    v = v * c.x; v = v * c.y;  // left y, right y
+    return v;
+}
+
+__device__ __forceinline__ float damp2(float v, float2 a, float2 b)
+{
+    v = v * a.x; v = v * a.y;
+    v = v * b.x; v = v * b.y;
+    return v;
+}
+
+// source stencil contributions at one cell, in source order
+__device__ __forceinline__ float inject(float c, float vvc, const Step& a, int z, int x, int y)
+{
+    for (int s = 0; s < a.ns; ++s) {
+        int dz = z - a.src[3 * s], dx = x - a.src[3 * s + 1], dy = y - a.src[3 * s + 2];
+        int adz = dz < 0 ? -dz : dz, adx = dx < 0 ? -dx : dx, ady = dy < 0 ? -dy : dy;
+        if (adz <= 1 && adx <= 1 && ady <= 1) {
+            float k = (float)(adz + adx + ady + 1);
+            c = c + (vvc * a.wav[(size_t)s * a.nt + a.it]) / k;
+        }
+    }
+    return c;
+}
+
+// ---------------------------------------------------------------- 3D: y-streaming 2.5D tiles
+template <int TX, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_afd3d(Geo g, Step a, int ntz, int ntx, int nyc,
+                                                      int nblk)
+{
+    constexpr int TZ = 64, RPW = TX / WAVES, LW = TZ + 4;
+    static_assert(TX % WAVES == 0, "rows must split evenly over waves");
+    __shared__ float lds[2][TX + 4][LW];
+
+    // XCD-aware order: hardware deals blocks round-robin over the 8 XCDs; give each XCD a
+    // contiguous range of logical tiles so neighbouring tiles share an L2.
+    const int per = gridDim.x >> 3;
+    const int lb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (lb >= nblk) return;
+    const int tz = lb % ntz;
+    const int rest = lb / ntz;
+    const int tx = rest % ntx;
+    const int ch = rest / ntx;
+    const int z0 = tz * TZ, x0 = tx * TX;
+    const int y0 = ch * nyc, y1 = min(y0 + nyc, g.ny);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int z = z0 + lane;
+    const bool zok = z < g.nz2;
+
+    bool blk_src = false;
+    for (int s = 0; s < a.ns; ++s) {
+        int gz = a.src[3 * s], gx = a.src[3 * s + 1], gy = a.src[3 * s + 2];
+        blk_src |= gz >= z0 - 1 && gz <= z0 + TZ && gx >= x0 - 1 && gx <= x0 + TX && gy >= y0 - 1 &&
+                   gy <= y1;
+    }
+
+    int xoff[RPW];
+    bool xok[RPW], act[RPW];
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+        int x = x0 + w * RPW + r;
+        xok[r] = x < g.nx2;
+        act[r] = (z < g.nz) && (x < g.nx);
+        xoff[r] = x * g.ldz;
+    }
+
+    float q[RPW][5];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        int yy = y0 - 2 + k;
+        bool yok = yy >= 0 && yy < g.ny2;
+        const float* P = a.cur + (long long)(yok ? yy : 0) * g.ldxy;
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) q[r][k] = (yok && zok && xok[r]) ? P[xoff[r] + z] : 0.f;
+    }
+
+    int buf = 0;
+    for (int y = y0; y < y1; ++y) {
+        const long long poff = (long long)y * g.ldxy;
+        const float* Pc = a.cur + poff;
+        {
+            int yy = y + 2;
+            bool yok = yy < g.ny2;
+            const float* P = a.cur + (long long)(yok ? yy : 0) * g.ldxy;
+#pragma unroll
+            for (int r = 0; r < RPW; ++r) q[r][4] = (yok && zok && xok[r]) ? P[xoff[r] + z] : 0.f;
+        }
+        float pr[RPW], vc[RPW];
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) {
+            pr[r] = act[r] ? a.pv[poff + xoff[r] + z] : 0.f;
+            vc[r] = act[r] ? a.vv[poff + xoff[r] + z] : 0.f;
+        }
+        float(*L)[LW] = lds[buf];
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) L[w * RPW + r + 2][lane + 2] = q[r][2];
+        if (lane < 4) {
+            int zz = lane < 2 ? z0 - 2 + lane : z0 + TZ + lane - 2;
+            int lz = lane < 2 ? lane : TZ + lane;
+            bool zzok = zz >= 0 && zz < g.nz2;
+#pragma unroll
+            for (int r = 0; r < RPW; ++r) L[w * RPW + r + 2][lz] = (zzok && xok[r]) ? Pc[xoff[r] + zz] : 0.f;
+        }
+        for (int h = w; h < 4; h += WAVES) {
+            int xx = h < 2 ? x0 - 2 + h : x0 + TX + h - 2;
+            int lrow = h < 2 ? h : TX + h;
+            L[lrow][lane + 2] = (zok && xx >= 0 && xx < g.nx2) ? Pc[xx * g.ldz + z] : 0.f;
+        }
+        __syncthreads();
+
+        const float2 fyy = a.fy[y];
+        const bool yin = y >= 2 && y < g.ny2 - 2;
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) {
+            if (!act[r]) continue;
+            const int x = x0 + w * RPW + r;
+            const int lr = w * RPW + r + 2;
+            const float u = q[r][2];
+            float lap = 0.f;
+            if (yin && z >= 2 && z < g.nz2 - 2 && x >= 2 && x < g.nx2 - 2) {
+                lap = C0 * u * g.cc
+                    + C1 * (q[r][1] + q[r][3]) * g.idy
+                    + C2 * (q[r][0] + q[r][4]) * g.idy
+                    + C1 * (L[lr - 1][lane + 2] + L[lr + 1][lane + 2]) * g.idx
+                    + C2 * (L[lr - 2][lane + 2] + L[lr + 2][lane + 2]) * g.idx
+                    + C1 * (L[lr][lane + 1] + L[lr][lane + 3]) * g.idz
+                    + C2 * (L[lr][lane] + L[lr][lane + 4]) * g.idz;
+            }
+            const float2 fzz = a.fz[z], fxx = a.fx[x];
+            const float p = damp3(pr[r], fzz, fxx, fyy);
+            float c = u + (u - p);
+            c = c + lap * vc[r];
+            if (blk_src) c = inject(c, vc[r], a, z, x, y);
+            const bool onrec = z == g.nb && x >= g.nb && x < g.nb + g.nx1 && y >= g.nb && y < g.nb + g.ny1;
+            if (a.tri && onrec) c = c + vc[r] * a.tri[(y - g.nb) * g.nx1 + (x - g.nb)];
+            c = damp3(c, fzz, fxx, fyy);
+            a.pv[poff + xoff[r] + z] = c;
+            if (a.rec && onrec) a.rec[(y - g.nb) * g.nx1 + (x - g.nb)] = c;
+            if (a.snap && z >= g.nb && z < g.nb + g.nz1 && x >= g.nb && x < g.nb + g.nx1 && y >= g.nb &&
+                y < g.nb + g.ny1)
+                a.snap[((size_t)(y - g.nb) * g.nx1 + (x - g.nb)) * g.nz1 + (z - g.nb)] = c;
+        }
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) {
+            q[r][0] = q[r][1]; q[r][1] = q[r][2]; q[r][2] = q[r][3]; q[r][3] = q[r][4];
+        }
+        buf ^= 1;
+    }
+}
+
+// ---------------------------------------------------------------- 2D: one cell per lane
+template <int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_afd2d(Geo g, Step a)
+{
+    const int z = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int x = blockIdx.y * WAVES + (threadIdx.x >> 6);
+    if (z >= g.nz || x >= g.nx) return;
+    const float* __restrict__ cur = a.cur;
+    const int L = g.ldz;
+    const int j = x * L + z;
+    const float u = cur[j];
+    float lap = 0.f;
+    if (z >= 2 && z < g.nz2 - 2 && x >= 2 && x < g.nx2 - 2) {
+        lap = C0 * u * g.cc
+            + C1 * (cur[j - L] + cur[j + L]) * g.idx
+            + C2 * (cur[j - 2 * L] + cur[j + 2 * L]) * g.idx
+            + C1 * (cur[j - 1] + cur[j + 1]) * g.idz
+            + C2 * (cur[j - 2] + cur[j + 2]) * g.idz;
+    }
+    const float vc = a.vv[j];
+    const float2 fzz = a.fz[z], fxx = a.fx[x];
+    const float p = damp2(a.pv[j], fzz, fxx);
+    float c = u + (u - p);
+    c = c + lap * vc;
+    c = inject(c, vc, a, z, x, 0);
+    const bool onrec = z == g.nb && x >= g.nb && x < g.nb + g.nx1;
+    if (a.tri && onrec) c = c + vc * a.tri[x - g.nb];
+    c = damp2(c, fzz, fxx);
+    a.pv[j] = c;
+    if (a.rec && onrec) a.rec[x - g.nb] = c;
+    if (a.snap && z >= g.nb && z < g.nb + g.nz1 && x >= g.nb && x < g.nb + g.nx1)
+        a.snap[(size_t)(x - g.nb) * g.nz1 + (z - g.nb)] = c;
+}
+
+// ---------------------------------------------------------------- helpers
+// vv = ((v*v)*dt)*dt on the padded layout from the clamp-expanded model velocity
+// (vel_expand, pywave/src/wave_abc.c:14-125, replicates the model edges into the boundary)
+__global__ void k_expand_vv(Geo g, const float* __restrict__ vel, float* __restrict__ vv, float dt)
+{
+    const int z = blockIdx.x * blockDim.x + threadIdx.x;
+    const int x = blockIdx.y;
+    const int y = blockIdx.z;
+    if (z >= g.nz) return;
+    const int sz = min(max(z - g.nb, 0), g.nz1 - 1);
+    const int sx = min(max(x - g.nb, 0), g.nx1 - 1);
+    const int nby = g.ny2 > 1 ? g.nb : 0;
+    const int sy = min(max(y - nby, 0), g.ny1 - 1);
+    const float v = vel[((size_t)sy * g.nx1 + sx) * g.nz1 + sz];
+    vv[(long long)y * g.ldxy + (long long)x * g.ldz + z] = v * v * dt * dt;
+}
+
+// out[c][r] = in[r][c] for an R x C matrix (receiver data time-major <-> time-fastest)
+__global__ void k_transpose(const float* __restrict__ in, float* __restrict__ out, int R, int C)
+{
+    __shared__ float t[32][33];
+    const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+    for (int k = threadIdx.y; k < 32; k += blockDim.y) {
+        int r = r0 + k, c = c0 + threadIdx.x;
+        if (r < R && c < C) t[k][threadIdx.x] = in[(size_t)r * C + c];
+    }
+    __syncthreads();
+    for (int k = threadIdx.y; k < 32; k += blockDim.y) {
+        int c = c0 + k, r = r0 + threadIdx.x;
+        if (r < R && c < C) out[(size_t)c * R + r] = t[threadIdx.x][k];
+    }
+}
+
+// copy the interior of a padded field (img of the reference, wave_fdm.c:356-367,666-672)
+__global__ void k_interior(Geo g, const float* __restrict__ f, float* __restrict__ out)
+{
+    const int z = blockIdx.x * blockDim.x + threadIdx.x;
+    const int x = blockIdx.y;
+    const int y = blockIdx.z;
+    if (z >= g.nz1) return;
+    const int nby = g.ny2 > 1 ? g.nb : 0;
+    out[((size_t)y * g.nx1 + x) * g.nz1 + z] =
+        f[(long long)(y + nby) * g.ldxy + (long long)(x + g.nb) * g.ldz + z + g.nb];
+}
+
+struct Variant3 {
+    int tx, waves;
+};
+const Variant3 kVariants3[] = {{16, 4}, {8, 4}, {16, 8}, {32, 8}, {8, 8}, {4, 4}};
+constexpr int kNumVariants3 = sizeof(kVariants3) / sizeof(kVariants3[0]);
+
+}  // namespace
+
+struct pw_afd {
+    pw_ctx* ctx = nullptr;
+    int ndim = 2;
+    Geo g{};
+    int nt = 0, jsnap = 0, ntsnap = 0;
+    float dt = 0.f, ct = 0.f;
+    float* d_a = nullptr;  // field buffers (ping-pong)
+    float* d_b = nullptr;
+    float* d_vv = nullptr;
+    float2* d_fz = nullptr;
+    float2* d_fx = nullptr;
+    float2* d_fy = nullptr;
+    int ns = 0;
+    int* d_src = nullptr;
+    float* d_wav = nullptr;
+    float* d_tri = nullptr;   // [nt][ny1][nx1]
+    float* d_rec = nullptr;   // [nt][ny1][nx1]
+    float* d_snap = nullptr;  // [ntsnap][ny1][nx1][nz1]
+    float* d_tmp = nullptr;   // transpose scratch (nrec*nt)
+    std::vector<float> tri_host;  // echoed back as the trace part in TRI mode
+    bool have_vel = false, have_src = false, have_tri = false;
+    int last_mode = -1;  // 0 modelling, 1 tri
+    const float* final_field = nullptr;
+    int variant = 0, ny_chunk = 0;
+    size_t field_elems = 0;
+};
+
+namespace {
+
+int afd_launch_step(pw_afd* p, const Step& a)
+{
+    pw_ctx* ctx = p->ctx;
+    hipStream_t s = ctx->s();
+    const Geo& g = p->g;
+    if (p->ndim == 2) {
+        constexpr int W = 4;
+        dim3 grid(pw::ceil_div(g.nz, 64), pw::ceil_div(g.nx, W));
+        hipLaunchKernelGGL(k_afd2d<W>, grid, dim3(64 * W), 0, s, g, a);
+    } else {
+        const Variant3 v = kVariants3[p->variant];
+        const int ntz = pw::ceil_div(g.nz, 64), ntx = pw::ceil_div(g.nx, v.tx);
+        int nyc = p->ny_chunk;
+        if (nyc <= 0) {
+            // enough blocks for ~6 per CU on 256 CUs, chunks no shorter than 16 planes
+            long long tiles = (long long)ntz * ntx;
+            int nch = (int)std::max(1LL, (6 * 256 + tiles - 1) / tiles);
+            nyc = std::max(16, pw::ceil_div(g.ny, nch));
+        }
+        const int nch = pw::ceil_div(g.ny, nyc);
+        const int nblk = ntz * ntx * nch;
+        const int grid = pw::ceil_div(nblk, 8) * 8;
+        switch (p->variant) {
+#define PW_V3(I, TX_, W_)                                                                          \
+    case I:                                                                                        \
+        hipLaunchKernelGGL((k_afd3d<TX_, W_>), dim3(grid), dim3(64 * W_), 0, s, g, a, ntz, ntx, nyc, \
+                           nblk);                                                                  \
+        break;
+            PW_V3(0, 16, 4)
+            PW_V3(1, 8, 4)
+            PW_V3(2, 16, 8)
+            PW_V3(3, 32, 8)
+            PW_V3(4, 8, 8)
+            PW_V3(5, 4, 4)
+#undef PW_V3
+            default:
+                PW_FAIL(ctx, PW_ERR_ARG, "unknown 3D kernel variant");
+        }
+    }
+    PW_HIP(ctx, hipGetLastError());
+    return PW_OK;
+}
+
+void afd_free(pw_afd* p)
+{
+    float* bufs[] = {p->d_a, p->d_b, p->d_vv, p->d_wav, p->d_tri, p->d_rec, p->d_snap, p->d_tmp};
+    for (float* b : bufs)
+        if (b) hipFree(b);
+    if (p->d_fz) hipFree(p->d_fz);
+    if (p->d_fx) hipFree(p->d_fx);
+    if (p->d_fy) hipFree(p->d_fy);
+    if (p->d_src) hipFree(p->d_src);
+}
+
+int afd_check(pw_afd* p) { return p && p->ctx ? PW_OK : PW_ERR_ARG; }
+
+}  // namespace
+
+extern "C" {
+
+long long pw_afd_output_len(int nt, int nz, int nx, int ny, int jsnap)
+{
+    if (ny < 1) ny = 1;
+    long long ntsnap = 0;
+    if (jsnap > 0) ntsnap = (nt - 1) / jsnap + 1;
+    return (long long)nt * nx * ny + (jsnap > 0 ? (long long)nz * nx * ny * ntsnap : 0);
+}
+
+int pw_afd_variant_count(void) { return kNumVariants3; }
+
+pw_afd* pw_afd_create(pw_ctx* ctx, int ndim, int nz, int nx, int ny, float dz, float dx, float dy,
+                      int nt, float dt, int nbt, float ct, int jsnap)
+{
+    if (!ctx) return nullptr;
+    if ((ndim != 2 && ndim != 3) || nz < 1 || nx < 1 || (ndim == 3 && ny < 1) || nt < 1 || nbt < 0 ||
+        jsnap < 0 || !(dz > 0.f) || !(dx > 0.f) || (ndim == 3 && !(dy > 0.f)) || !(dt > 0.f)) {
+        ctx->err = "pw_afd_create: invalid dimensions or spacings";
+        return nullptr;
+    }
+    if (hipSetDevice(ctx->device) != hipSuccess) {
+        ctx->err = "hipSetDevice failed";
+        return nullptr;
+    }
+    pw_afd* p = new pw_afd;
+    p->ctx = ctx;
+    p->ndim = ndim;
+    p->nt = nt;
+    p->dt = dt;
+    p->ct = ct;
+    p->jsnap = jsnap;
+    p->ntsnap = jsnap > 0 ? (nt - 1) / jsnap + 1 : 0;
+    Geo& g = p->g;
+    g.nz1 = nz; g.nx1 = nx; g.ny1 = ndim == 3 ? ny : 1;
+    g.nb = nbt;
+    g.nz = nz + 2 * nbt; g.nx = nx + 2 * nbt; g.ny = ndim == 3 ? ny + 2 * nbt : 1;
+    pw::padded_sizes(g.nz, g.nx, g.ny, &g.nz2, &g.nx2, &g.ny2);
+    if (ndim == 2) g.ny2 = 1;
+    g.ldz = pw::ceil_div(g.nz2, LDZ_ALIGN) * LDZ_ALIGN;
+    g.ldxy = (long long)g.ldz * g.nx2;
+    g.idz = 1 / (dz * dz);
+    g.idx = 1 / (dx * dx);
+    g.idy = ndim == 3 ? 1 / (dy * dy) : 0.f;
+    g.cc = ndim == 3 ? g.idy + g.idx + g.idz : g.idx + g.idz;
+    p->field_elems = (size_t)g.ldxy * g.ny2;
+
+    // damping pairs per axis; all six faces use nbt and ct (pywave/src/afd.c:101-116,392-402)
+    std::vector<float> w(std::max(nbt, 1));
+    pw::abc_weights(nbt, ct, w.data());
+    std::vector<float> fz, fx, fy;
+    pw::damping_pairs(g.nz2, nbt, nbt, w.data(), w.data(), fz);
+    pw::damping_pairs(g.nx2, nbt, nbt, w.data(), w.data(), fx);
+    if (ndim == 3) pw::damping_pairs(g.ny2, nbt, nbt, w.data(), w.data(), fy);
+    else fy.assign(2, 1.0f);
+
+    const size_t nrec = (size_t)g.nx1 * g.ny1;
+    bool ok = hipMalloc(&p->d_a, p->field_elems * 4) == hipSuccess &&
+              hipMalloc(&p->d_b, p->field_elems * 4) == hipSuccess &&
+              hipMalloc(&p->d_vv, p->field_elems * 4) == hipSuccess &&
+              hipMalloc(&p->d_fz, fz.size() * 4) == hipSuccess &&
+              hipMalloc(&p->d_fx, fx.size() * 4) == hipSuccess &&
+              hipMalloc(&p->d_fy, fy.size() * 4) == hipSuccess &&
+              hipMalloc(&p->d_rec, nrec * nt * 4) == hipSuccess &&
+              hipMalloc(&p->d_tmp, nrec * nt * 4) == hipSuccess;
+    if (ok && p->ntsnap > 0)
+        ok = hipMalloc(&p->d_snap, (size_t)p->ntsnap * g.nz1 * g.nx1 * g.ny1 * 4) == hipSuccess;
+    if (!ok) {
+        ctx->err = "pw_afd_create: device allocation failed";
+        afd_free(p);
+        delete p;
+        return nullptr;
+    }
+    hipStream_t s = ctx->s();
+    ok = hipMemcpyAsync(p->d_fz, fz.data(), fz.size() * 4, hipMemcpyHostToDevice, s) == hipSuccess &&
+         hipMemcpyAsync(p->d_fx, fx.data(), fx.size() * 4, hipMemcpyHostToDevice, s) == hipSuccess &&
+         hipMemcpyAsync(p->d_fy, fy.data(), fy.size() * 4, hipMemcpyHostToDevice, s) == hipSuccess &&
+         hipMemsetAsync(p->d_vv, 0, p->field_elems * 4, s) == hipSuccess &&
+         hipStreamSynchronize(s) == hipSuccess;
+    if (!ok) {
+        ctx->err = "pw_afd_create: upload failed";
+        afd_free(p);
+        delete p;
+        return nullptr;
+    }
+    return p;
+}
+
+void pw_afd_destroy(pw_afd* p)
+{
+    if (!p) return;
+    hipSetDevice(p->ctx->device);
+    hipStreamSynchronize(p->ctx->s());
+    afd_free(p);
+    delete p;
+}
+
+int pw_afd_set_velocity(pw_afd* p, const float* vel)
+{
+    if (afd_check(p) || !vel) return PW_ERR_ARG;
+    pw_ctx* ctx = p->ctx;
+    const Geo& g = p->g;
+    size_t n = (size_t)g.nz1 * g.nx1 * g.ny1;
+    float* d_vel = nullptr;
+    PW_HIP(ctx, hipMalloc(&d_vel, n * 4));
+    hipStream_t s = ctx->s();
+    hipError_t e = hipMemcpyAsync(d_vel, vel, n * 4, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) {
+        dim3 grid(pw::ceil_div(g.nz, 256), g.nx, g.ny);
+        hipLaunchKernelGGL(k_expand_vv, grid, dim3(256), 0, s, g, d_vel, p->d_vv, p->dt);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    hipFree(d_vel);
+    if (e != hipSuccess) PW_FAIL(ctx, PW_ERR_HIP, std::string("set_velocity: ") + hipGetErrorString(e));
+    p->have_vel = true;
+    return PW_OK;
+}
+
+int pw_afd_set_sources(pw_afd* p, int ns, const float* src_pack, const float* wavelet)
+{
+    if (afd_check(p)) return PW_ERR_ARG;
+    pw_ctx* ctx = p->ctx;
+    const Geo& g = p->g;
+    if (ns < 0 || (ns > 0 && !src_pack)) PW_FAIL(ctx, PW_ERR_ARG, "set_sources: bad source pack");
+    const int per = p->ndim == 3 ? 6 : 5;
+    std::vector<int> cen(3 * std::max(ns, 1), 0);
+    std::vector<float> wav((size_t)std::max(ns, 1) * p->nt, 0.f);
+    for (int i = 0; i < ns; ++i) {
+        // float -> int truncation of the grid positions (pywave/src/afd.c:136-147, 416-425)
+        int sx = (int)src_pack[i];
+        int sy = p->ndim == 3 ? (int)src_pack[ns + i] : 0;
+        int sz = (int)src_pack[(p->ndim == 3 ? 2 : 1) * ns + i];
+        float f0 = src_pack[(per - 3) * ns + i], t0 = src_pack[(per - 2) * ns + i];
+        float A = src_pack[(per - 1) * ns + i];
+        int gz = sz + g.nb, gx = sx + g.nb, gy = p->ndim == 3 ? sy + g.nb : 0;
+        bool inside = gz - 1 >= 0 && gz + 1 < g.nz && gx - 1 >= 0 && gx + 1 < g.nx &&
+                      (p->ndim == 2 || (gy - 1 >= 0 && gy + 1 < g.ny));
+        if (!inside)
+            PW_FAIL(ctx, PW_ERR_ARG,
+                    "set_sources: source " + std::to_string(i) +
+                        " stencil leaves the padded domain (the reference would write out of bounds)");
+        cen[3 * i] = gz; cen[3 * i + 1] = gx; cen[3 * i + 2] = gy;
+        if (wavelet) {
+            std::memcpy(&wav[(size_t)i * p->nt], wavelet + (size_t)i * p->nt, sizeof(float) * p->nt);
+        } else {
+            std::string e;
+            int rc = pw::ricker(p->nt, p->dt, f0, t0, A, &wav[(size_t)i * p->nt], &e);
+            if (rc) PW_FAIL(ctx, rc, "set_sources: source " + std::to_string(i) + ": " + e);
+        }
+    }
+    if (p->d_src) { hipFree(p->d_src); p->d_src = nullptr; }
+    if (p->d_wav) { hipFree(p->d_wav); p->d_wav = nullptr; }
+    PW_HIP(ctx, hipMalloc(&p->d_src, cen.size() * 4));
+    PW_HIP(ctx, hipMalloc(&p->d_wav, wav.size() * 4));
+    hipStream_t s = ctx->s();
+    PW_HIP(ctx, hipMemcpyAsync(p->d_src, cen.data(), cen.size() * 4, hipMemcpyHostToDevice, s));
+    PW_HIP(ctx, hipMemcpyAsync(p->d_wav, wav.data(), wav.size() * 4, hipMemcpyHostToDevice, s));
+    PW_HIP(ctx, hipStreamSynchronize(s));
+    p->ns = ns;
+    p->have_src = true;
+    return PW_OK;
+}
+
+int pw_afd_set_tri_data(pw_afd* p, const float* dat)
+{
+    if (afd_check(p) || !dat) return PW_ERR_ARG;
+    pw_ctx* ctx = p->ctx;
+    const Geo& g = p->g;
+    const size_t nrec = (size_t)g.nx1 * g.ny1, n = nrec * p->nt;
+    p->tri_host.assign(dat, dat + n);
+    if (!p->d_tri) PW_HIP(ctx, hipMalloc(&p->d_tri, n * 4));
+    hipStream_t s = ctx->s();
+    // host layout [iy][ix][it] (time fastest) -> device [it][iy][ix]
+    PW_HIP(ctx, hipMemcpyAsync(p->d_tmp, dat, n * 4, hipMemcpyHostToDevice, s));
+    dim3 grid(pw::ceil_div(p->nt, 32), pw::ceil_div((int)nrec, 32));
+    hipLaunchKernelGGL(k_transpose, grid, dim3(32, 8), 0, s, p->d_tmp, p->d_tri, (int)nrec, p->nt);
+    PW_HIP(ctx, hipGetLastError());
+    PW_HIP(ctx, hipStreamSynchronize(s));
+    p->have_tri = true;
+    return PW_OK;
+}
+
+static int afd_enqueue(pw_afd* p, int tri, hipEvent_t* evs)
+{
+    pw_ctx* ctx = p->ctx;
+    const Geo& g = p->g;
+    if (!p->have_vel) PW_FAIL(ctx, PW_ERR_STATE, "run: velocity not set");
+    if (tri && !p->have_tri) PW_FAIL(ctx, PW_ERR_STATE, "run: time-reversal data not set");
+    if (!tri && !p->have_src) PW_FAIL(ctx, PW_ERR_STATE, "run: sources not set");
+    hipStream_t s = ctx->s();
+    PW_HIP(ctx, hipMemsetAsync(p->d_a, 0, p->field_elems * 4, s));
+    PW_HIP(ctx, hipMemsetAsync(p->d_b, 0, p->field_elems * 4, s));
+    const size_t nrec = (size_t)g.nx1 * g.ny1;
+    const size_t snap_elems = (size_t)g.nz1 * g.nx1 * g.ny1;
+    float* A = p->d_a;
+    float* B = p->d_b;
+    const int it0 = tri ? p->nt - 1 : 0, it1 = tri ? -1 : p->nt, its = tri ? -1 : 1;
+    int k = 0;
+    for (int it = it0; it != it1; it += its, ++k) {
+        Step a{};
+        a.cur = A; a.pv = B; a.vv = p->d_vv;
+        a.fz = p->d_fz; a.fx = p->d_fx; a.fy = p->d_fy;
+        a.ns = tri ? 0 : p->ns;
+        a.src = p->d_src; a.wav = p->d_wav; a.nt = p->nt; a.it = it;
+        a.tri = tri ? p->d_tri + (size_t)it * nrec : nullptr;
+        a.rec = tri ? nullptr : p->d_rec + (size_t)it * nrec;
+        a.snap = (p->jsnap > 0 && it % p->jsnap == 0) ? p->d_snap + (size_t)(it / p->jsnap) * snap_elems : nullptr;
+        if (evs) PW_HIP(ctx, hipEventRecord(evs[2 * k], s));
+        int rc = afd_launch_step(p, a);
+        if (rc) return rc;
+        if (evs) PW_HIP(ctx, hipEventRecord(evs[2 * k + 1], s));
+        std::swap(A, B);
+    }
+    p->final_field = A;
+    p->last_mode = tri ? 1 : 0;
+    return PW_OK;
+}
+
+int pw_afd_run(pw_afd* p, int tri)
+{
+    if (afd_check(p)) return PW_ERR_ARG;
+    return afd_enqueue(p, tri, nullptr);
+}
+
+int pw_afd_run_timed(pw_afd* p, int tri, int per_step, double* total_ms, double* step_ms)
+{
+    if (afd_check(p)) return PW_ERR_ARG;
+    pw_ctx* ctx = p->ctx;
+    hipStream_t s = ctx->s();
+    hipEvent_t e0, e1;
+    PW_HIP(ctx, hipEventCreate(&e0));
+    PW_HIP(ctx, hipEventCreate(&e1));
+    std::vector<hipEvent_t> evs;
+    if (per_step) {
+        evs.resize(2 * (size_t)p->nt);
+        for (auto& e : evs) PW_HIP(ctx, hipEventCreate(&e));
+    }
+    PW_HIP(ctx, hipEventRecord(e0, s));
+    int rc = afd_enqueue(p, tri, per_step ? evs.data() : nullptr);
+    if (rc) return rc;
+    PW_HIP(ctx, hipEventRecord(e1, s));
+    PW_HIP(ctx, hipEventSynchronize(e1));
+    float ms = 0.f;
+    PW_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
+    if (total_ms) *total_ms = ms;
+    double acc = 0.0;
+    if (per_step) {
+        for (int k = 0; k < p->nt; ++k) {
+            float m = 0.f;
+            PW_HIP(ctx, hipEventElapsedTime(&m, evs[2 * k], evs[2 * k + 1]));
+            acc += m;
+        }
+        for (auto& e : evs) hipEventDestroy(e);
+    } else {
+        acc = ms;
+    }
+    if (step_ms) *step_ms = acc / p->nt;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return PW_OK;
+}
+
+int pw_afd_output(pw_afd* p, float* out, long long out_len)
+{
+    if (afd_check(p) || !out) return PW_ERR_ARG;
+    pw_ctx* ctx = p->ctx;
+    const Geo& g = p->g;
+    const size_t nrec = (size_t)g.nx1 * g.ny1, ntr = nrec * p->nt;
+    const long long need = pw_afd_output_len(p->nt, g.nz1, g.nx1, g.ny1, p->jsnap);
+    if (out_len != need) PW_FAIL(ctx, PW_ERR_ARG, "output: length mismatch");
+    if (p->last_mode < 0) PW_FAIL(ctx, PW_ERR_STATE, "output: nothing has run");
+    hipStream_t s = ctx->s();
+    if (p->last_mode == 1) {
+        std::memcpy(out, p->tri_host.data(), ntr * 4);  // TRI echoes its input data
+    } else {
+        dim3 grid(pw::ceil_div((int)nrec, 32), pw::ceil_div(p->nt, 32));
+        hipLaunchKernelGGL(k_transpose, grid, dim3(32, 8), 0, s, p->d_rec, p->d_tmp, p->nt, (int)nrec);
+        PW_HIP(ctx, hipGetLastError());
+        PW_HIP(ctx, hipMemcpyAsync(out, p->d_tmp, ntr * 4, hipMemcpyDeviceToHost, s));
+    }
+    if (p->jsnap > 0)
+        PW_HIP(ctx, hipMemcpyAsync(out + ntr, p->d_snap, (size_t)p->ntsnap * g.nz1 * nrec * 4,
+                                   hipMemcpyDeviceToHost, s));
+    PW_HIP(ctx, hipStreamSynchronize(s));
+    return PW_OK;
+}
+
+int pw_afd_final_field(pw_afd* p, float* out, long long out_len)
+{
+    if (afd_check(p) || !out) return PW_ERR_ARG;
+    pw_ctx* ctx = p->ctx;
+    const Geo& g = p->g;
+    const size_t n = (size_t)g.nz1 * g.nx1 * g.ny1;
+    if ((size_t)out_len != n) PW_FAIL(ctx, PW_ERR_ARG, "final_field: length mismatch");
+    if (!p->final_field) PW_FAIL(ctx, PW_ERR_STATE, "final_field: nothing has run");
+    hipStream_t s = ctx->s();
+    dim3 grid(pw::ceil_div(g.nz1, 256), g.nx1, g.ny1);
+    hipLaunchKernelGGL(k_interior, grid, dim3(256), 0, s, g, p->final_field, p->d_tmp);
+    PW_HIP(ctx, hipGetLastError());
+    PW_HIP(ctx, hipMemcpyAsync(out, p->d_tmp, n * 4, hipMemcpyDeviceToHost, s));
+    PW_HIP(ctx, hipStreamSynchronize(s));
+    return PW_OK;
+}
+
+int pw_afd_set_variant(pw_afd* p, int variant, int ny_chunk)
+{
+    if (afd_check(p)) return PW_ERR_ARG;
+    if (variant < 0 || variant >= kNumVariants3 || ny_chunk < 0)
+        PW_FAIL(p->ctx, PW_ERR_ARG, "set_variant: unknown variant");
+    p->variant = variant;
+    p->ny_chunk = ny_chunk;
+    return PW_OK;
+}
+
+static int afd_oneshot(pw_ctx* ctx, int ndim, const float* vel, const float* src, int tri, int nt,
+                       int nx, int ny, int nz, int ns, int jsnap, int nbt, float ct, float dt,
+                       float dx, float dy, float dz, float* out, long long out_len)
+{
+    if (!ctx) return PW_ERR_ARG;
+    if (!vel || !src || !out) PW_FAIL(ctx, PW_ERR_ARG, "afd: null array");
+    if (!tri && ns < 1) PW_FAIL(ctx, PW_ERR_ARG, "afd: ns must be >= 1 for modelling");
+    if (jsnap < 0) jsnap = 0;
+    if (out_len != pw_afd_output_len(nt, nz, nx, ny, jsnap))
+        PW_FAIL(ctx, PW_ERR_ARG, "afd: output length mismatch");
+    pw_afd* p = pw_afd_create(ctx, ndim, nz, nx, ny, dz, dx, dy, nt, dt, nbt, ct, jsnap);
+    if (!p) return PW_ERR_ARG;
+    int rc = pw_afd_set_velocity(p, vel);
+    if (!rc) rc = tri ? pw_afd_set_tri_data(p, src) : pw_afd_set_sources(p, ns, src, nullptr);
+    if (!rc) rc = pw_afd_run(p, tri);
+    if (!rc) rc = pw_afd_output(p, out, out_len);
+    pw_afd_destroy(p);
+    return rc;
+}
+
+int pw_afd2d(pw_ctx* ctx, const float* vel, const float* src, int tri, int nt, int nx, int nz,
+             int ns, int verb, int jsnap, int ifsnaps, int abc, int nbt, float ct, float dt,
+             float ox, float dx, float oz, float dz, float* out, long long out_len)
+{
+    (void)verb; (void)ifsnaps; (void)abc; (void)ox; (void)oz;
+    return afd_oneshot(ctx, 2, vel, src, tri, nt, nx, 1, nz, ns, jsnap, nbt, ct, dt, dx, 1.f, dz, out,
+                       out_len);
+}
+
+int pw_afd3d(pw_ctx* ctx, const float* vel, const float* src, int tri, int nt, int nx, int ny,
+             int nz, int ns, int verb, int jsnap, int ifsnaps, int abc, int nbt, float ct,
+             float dt, float ox, float dx, float oy, float dy, float oz, float dz, float* out,
+             long long out_len)
+{
+    (void)verb; (void)ifsnaps; (void)abc; (void)ox; (void)oy; (void)oz;
+    return afd_oneshot(ctx, 3, vel, src, tri, nt, nx, ny, nz, ns, jsnap, nbt, ct, dt, dx, dy, dz, out,
+                       out_len);
+}
+
+}  // extern "C"

@@ -1,0 +1,62 @@
+// pw_internal.h -- shared definitions of the pywave_amd HIP library (gfx950 / MI355X).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/pywave_amd.h"
+
+struct pw_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;   // library-owned stream (default priority)
+    hipStream_t ext_stream = nullptr;  // caller stream if set (torch), else null
+    std::string err;
+    hipStream_t s() const { return ext_stream ? ext_stream : stream; }
+};
+
+// Error helpers: every HIP call is checked; the message lands in ctx->err and the
+// entry point returns a negative code.
+#define PW_FAIL(ctx, code, msg)                                   \
+    do {                                                          \
+        if (ctx) (ctx)->err = (msg);                              \
+        return (code);                                            \
+    } while (0)
+
+#define PW_HIP(ctx, call)                                                                  \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            if (ctx)                                                                       \
+                (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);            \
+            return PW_ERR_HIP;                                                             \
+        }                                                                                  \
+    } while (0)
+
+namespace pw {
+
+// kiss_fft_next_fast_size (pywave/src/wave_kissfft.c:412-424)
+int next_fast_size(int n);
+
+// padded extents of fdm/fdm2d (pywave/src/wave_fft2.c:45-51, wave_fft3.c:45-60)
+void padded_sizes(int nz, int nx, int ny, int* nz2, int* nx2, int* ny2);
+
+// abc_cal weights (pywave/src/wave_abc.c:141-143)
+void abc_weights(int nb, float c, float* w);
+
+// Zero-phase Ricker of pywave/src/wave_ricker.c:22-59 + wave_freqfilt.c:141-164 evaluated as
+// the closed-form cosine sum of the filtered spike, in double precision.
+int ricker(int nt, float dt, float f0, float t0, float A, float* out, std::string* err);
+
+// Per-index damping factor pairs (lo, hi): the factor each face multiplies into index i of an
+// axis of padded length n2 with nb boundary cells (1.0 where a face does not apply).
+void damping_pairs(int n2, int nb_lo, int nb_hi, const float* w_lo, const float* w_hi,
+                   std::vector<float>& out2);
+
+template <typename T>
+inline T ceil_div(T a, T b) { return (a + b - 1) / b; }
+
+}  // namespace pw

@@ -1,0 +1,142 @@
+"""Write the golden vectors under tests/golden/ from the reference's own C.
+
+The reference ships no tests, fixtures or known-answer vectors for this path, so the golden
+vectors are produced here from the reference's engine sources compiled unchanged by
+``make -C oracle ref`` (oracle/_ref/pwref; it exists only where the reference tree is
+mounted).  Inputs are seeded synthetic models.  Each fixture stores its inputs, the
+reference outputs and the reference's own wavelet(s).
+
+Run:  make -C oracle ref && python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+from oracle import refrun  # noqa: E402
+
+
+def save(name, **arrays):
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, **arrays)
+    print("wrote", path, os.path.getsize(path), "bytes")
+
+
+def layered2d(nz, nx, vals):
+    v = np.zeros((nz, nx), np.float32)
+    edges = np.linspace(0, nz, len(vals) + 1).astype(int)
+    for k, val in enumerate(vals):
+        v[edges[k]:edges[k + 1]] = val
+    return v
+
+
+def afd2d_cases():
+    # cfg0: three flat layers 101x101, nt=1000, source (z=5,x=50) 20 Hz at 0.1 s, jsnap=999
+    vel = np.zeros((101, 101), np.float32)
+    vel[:34] = 1500; vel[34:67] = 2000; vel[67:] = 2500
+    src = np.array([50, 5, 20, 0.1, 1], np.float32)
+    out, _ = refrun.afd2d(vel, 1000, 0.001, 10, 10, 30, 0.01, src, 1, jsnap=999)
+    rick = refrun.ricker(1000, 0.001, 20, 0.1, 1)[None, :]
+    save("afd2d_cfg0", vel=vel, src=src, nt=1000, dt=0.001, dx=10., dz=10., nbt=30, ct=0.01,
+         jsnap=999, out=out, rick=rick)
+
+    # two overlapping sources, uneven dims with FFT slack on both axes, fractional positions
+    rng = np.random.default_rng(11)
+    vel = (1500 + 1500 * rng.random((57, 83))).astype(np.float32)
+    src = np.array([40.7, 41.2, 20.0, 21.9, 25, 18, 0.03, 0.05, 1.0, -2.5], np.float32)
+    nt = 300
+    out, _ = refrun.afd2d(vel, nt, 0.0008, 10, 9, 12, 0.03, src, 2, jsnap=23)
+    rick = np.stack([refrun.ricker(nt, 0.0008, 25, 0.03, 1.0), refrun.ricker(nt, 0.0008, 18, 0.05, -2.5)])
+    save("afd2d_two_sources", vel=vel, src=src, nt=nt, dt=0.0008, dx=10., dz=9., nbt=12, ct=0.03,
+         jsnap=23, out=out, rick=rick)
+
+    # padded extents equal the updated domain (no slack): the bottom/right rim has a zero Laplacian
+    vel = layered2d(44, 40, [1800, 2600]).astype(np.float32)
+    src = np.array([20, 0, 30, 0.02, 1], np.float32)  # source on the first model row
+    nt = 250
+    out, _ = refrun.afd2d(vel, nt, 0.001, 8, 8, 10, 0.02, src, 1, jsnap=0)
+    rick = refrun.ricker(nt, 0.001, 30, 0.02, 1)[None, :]
+    save("afd2d_noslack", vel=vel, src=src, nt=nt, dt=0.001, dx=8., dz=8., nbt=10, ct=0.02,
+         jsnap=0, out=out, rick=rick)
+
+    # time-reversal: back-propagate seeded receiver data
+    vel = (1600 + 900 * rng.random((40, 50))).astype(np.float32)
+    dat = rng.standard_normal((300, 50)).astype(np.float32)
+    out, _ = refrun.afd2d(vel, 300, 0.001, 10, 10, 20, 0.02, tri=True, dat=dat, jsnap=7)
+    save("afd2d_tri", vel=vel, dat=dat, nt=300, dt=0.001, dx=10., dz=10., nbt=20, ct=0.02, jsnap=7,
+         out=out)
+
+
+def afd3d_cases():
+    rng = np.random.default_rng(7)
+    vel = (1500 + 1000 * rng.random((30, 26, 22))).astype(np.float32)
+    src = np.array([10, 15, 12, 9, 8, 7, 30, 25, 0.02, 0.05, 1, 2], np.float32)
+    nt = 150
+    out, _ = refrun.afd3d(vel, nt, 0.001, 10, 12, 9, 8, 0.05, src, 2, jsnap=40)
+    rick = np.stack([refrun.ricker(nt, 0.001, 30, 0.02, 1), refrun.ricker(nt, 0.001, 25, 0.05, 2)])
+    save("afd3d_two_sources", vel=vel, src=src, nt=nt, dt=0.001, dx=10., dy=12., dz=9., nbt=8,
+         ct=0.05, jsnap=40, out=out, rick=rick)
+
+    vel = (1700 + 800 * rng.random((24, 20, 18))).astype(np.float32)
+    dat = (0.1 * rng.standard_normal((120, 20, 18))).astype(np.float32)
+    out, _ = refrun.afd3d(vel, 120, 0.001, 10, 10, 10, 6, 0.05, tri=True, dat=dat, jsnap=30)
+    save("afd3d_tri", vel=vel, dat=dat, nt=120, dt=0.001, dx=10., dy=10., dz=10., nbt=6, ct=0.05,
+         jsnap=30, out=out)
+
+
+def ricker_cases():
+    cases = [(1000, 0.001, 20, 0.1, 1), (1500, 0.001, 15, 0.1, 1), (2000, 0.001, 10, 0.25, 1),
+             (3000, 0.001, 8, 0.15, 1e6), (333, 0.002, 12, 0.35, -3), (1501, 0.001, 10, 0.2, 2)]
+    waves = [refrun.ricker(*c) for c in cases]
+    n = max(len(w) for w in waves)
+    arr = np.zeros((len(cases), n), np.float32)
+    for i, w in enumerate(waves):
+        arr[i, :len(w)] = w
+    save("ricker", params=np.array(cases, np.float64), waves=arr)
+
+
+def fwi_cases():
+    rng = np.random.default_rng(4)
+    nz, nx, nt = 40, 60, 500
+    vtrue = np.full((nz, nx), 2.0, np.float32)
+    vtrue[15:] = 2.5
+    vtrue[28:] = 3.0
+    vtrue += (0.1 * rng.standard_normal((nz, nx))).astype(np.float32)
+    vtrue = vtrue.astype(np.float32)
+    # smooth start model: separable running mean (radius 3), computed in float64
+    k = np.ones(7) / 7.0
+    vs = np.apply_along_axis(lambda c: np.convolve(np.pad(c, 3, mode="edge"), k, "valid"), 0, vtrue)
+    vs = np.apply_along_axis(lambda r: np.convolve(np.pad(r, 3, mode="edge"), k, "valid"), 1, vs)
+    v0 = vs.astype(np.float32)
+    q = np.full((nz, nx), 1e4, np.float32)
+    wav = (1e6 * refrun.ricker(nt, 0.001, 10, 0.1, 1)).astype(np.float32)
+    par = np.array([nz, nx, 0.01, 0.01, 0, 0, nt, 0.001, 0, 0, 3, 0.2, 0.05, 5, 20, 0.01, 10, 1, 1, 3],
+                   np.float32)
+    data, _ = refrun.fwimod(par, vtrue, q, wav)
+    grads, fc, _ = refrun.fwigrad(par, v0, q, wav, data, vel_second=(v0 * 1.01).astype(np.float32))
+    par4 = par.copy(); par4[18] = 4
+    grads4, fc4, _ = refrun.fwigrad(par4, v0, q, wav, data)
+    vinv, _ = refrun.fwi(par, v0, q, wav, data)
+    save("fwi_small", par=par, par4=par4, vtrue=vtrue, v0=v0, q=q, wav=wav, data=data,
+         grad1=grads[0], grad2=grads[1], fcost=fc, grad_int4=grads4[0], fcost_int4=fc4, vinv=vinv)
+
+    # marine-style acquisition (acqui_type 2), sources marching across the model
+    par2 = np.array([30, 48, 0.01, 0.01, 0, 0, 300, 0.001, 0, 0, 2, 0.15, 0.0, 5, 16, 0.01, 10, 2, 1, 1],
+                    np.float32)
+    v2 = (2.2 + 0.4 * rng.random((30, 48))).astype(np.float32)
+    q2 = np.full((30, 48), 1e4, np.float32)
+    wav2 = (1e6 * refrun.ricker(300, 0.001, 12, 0.06, 1)).astype(np.float32)
+    data2, _ = refrun.fwimod(par2, v2, q2, wav2)
+    save("fwi_marine", par=par2, vel=v2, wav=wav2, data=data2)
+
+
+if __name__ == "__main__":
+    if not refrun.available():
+        sys.exit("oracle/_ref/pwref missing: run `make -C oracle ref` where the reference tree exists")
+    ricker_cases()
+    afd2d_cases()
+    afd3d_cases()
+    fwi_cases()

@@ -1,0 +1,75 @@
+"""The C-ABI library loads, exports every symbol include/pywave_amd.h declares, and its
+host-side set-up helpers agree with the oracle.  CPU only (no device work)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+
+from conftest import ROOT
+from oracle import oracle as O
+from pywave_amd import _native as N
+
+HEADER = os.path.join(ROOT, "include", "pywave_amd.h")
+
+
+def declared_symbols():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(pw_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_header_symbols_exported_and_bound():
+    L = N.lib()
+    syms = declared_symbols()
+    assert len(syms) >= 20
+    for s in syms:
+        assert hasattr(L, s), "missing export %s" % s
+    assert set(syms) == set(N.SIGNATURES), "ctypes table and header disagree"
+
+
+def test_next_fast_size_and_padding_match_oracle():
+    L = N.lib()
+    for n in list(range(1, 400)) + [1501, 2999, 3001]:
+        assert L.pw_next_fast_size(n) == O.next_fast_size(n)
+    a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    for dims in [(161, 161, 1), (261, 261, 261), (188, 188, 188), (64, 60, 1), (77, 91, 43)]:
+        L.pw_padded_sizes(*dims, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+        expect = O.padded_sizes(*dims)
+        assert (a.value, b.value) == expect[:2]
+        if dims[2] > 1:
+            assert c.value == expect[2]
+
+
+def test_abc_weights_bit_exact():
+    L = N.lib()
+    for nb, c in [(30, 0.01), (12, 0.03), (60, 0.005), (1, 0.5)]:
+        w = np.zeros(nb, np.float32)
+        L.pw_abc_weights(nb, c, N.ptr(w))
+        assert np.array_equal(w, O.abc_weights(nb, c))
+
+
+def test_ricker_host_vs_reference(golden):
+    L = N.lib()
+    g = golden("ricker")
+    for (nt, dt, f0, t0, A), ref in zip(g["params"], g["waves"]):
+        nt = int(nt)
+        w = np.zeros(nt, np.float32)
+        assert L.pw_ricker(nt, dt, f0, t0, A, N.ptr(w)) == 0
+        ref = ref[:nt]
+        assert np.abs(w - ref).max() <= 1e-6 * np.abs(ref).max()
+        # independent f64 FFT evaluation of the same filter
+        assert np.abs(w - O.ricker(nt, dt, f0, t0, A)).max() <= 1e-6 * np.abs(ref).max()
+
+
+def test_ricker_rejects_onset_outside_trace():
+    L = N.lib()
+    w = np.zeros(100, np.float32)
+    assert L.pw_ricker(100, 0.001, 10, 0.5, 1, N.ptr(w)) == -1
+
+
+def test_output_length_formula():
+    L = N.lib()
+    assert L.pw_afd_output_len(1000, 101, 101, 1, 999) == 1000 * 101 + 101 * 101 * 2
+    assert L.pw_afd_output_len(1500, 201, 201, 201, 0) == 1500 * 201 * 201
+    assert L.pw_afd_output_len(10, 5, 6, 7, 3) == 10 * 42 + 5 * 42 * 4

@@ -1,0 +1,232 @@
+"""Pins the oracle (CPU restatement) against the compiled reference's golden vectors and
+against what the mathematics fixes.  CPU only."""
+import itertools
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+from oracle import oracle as O
+
+
+# ------------------------------------------------------------------ geometry
+
+def test_next_fast_size_brute_force():
+    smooth = sorted({2 ** a * 3 ** b * 5 ** c for a, b, c in itertools.product(range(14), range(9), range(7))})
+    for n in range(1, 3000):
+        expect = next(m for m in smooth if m >= n)
+        assert O.next_fast_size(n) == expect
+
+
+def test_padded_sizes_table():
+    # pywave/src/wave_fft3.c:45-60 applied to the configurations' updated domains
+    assert O.padded_sizes(161, 161)[:2] == (162, 162)
+    assert O.padded_sizes(261, 261, 261) == (270, 270, 270)
+    assert O.padded_sizes(188, 188, 188) == (192, 192, 192)
+    assert O.padded_sizes(141, 141)[:2] == (144, 144)
+    assert O.padded_sizes(64, 60)[:2] == (64, 60)  # no slack
+
+
+def test_abc_weights_closed_form():
+    w = O.abc_weights(30, 0.01)
+    assert w[-1] == 1.0  # innermost cell undamped
+    k = 29 - np.arange(30)
+    expect = np.exp(-(0.01 ** 2) * k.astype(np.float64) ** 2)
+    np.testing.assert_allclose(w, expect, rtol=1e-6)
+    assert abs(float(w[0]) - 0.9193) < 1e-4
+    assert np.all(np.diff(w) > 0)
+
+
+# ------------------------------------------------------------------ wavelet
+
+def test_ricker_vs_reference(golden):
+    g = golden("ricker")
+    for (nt, dt, f0, t0, A), ref in zip(g["params"], g["waves"]):
+        nt = int(nt)
+        w = O.ricker(nt, dt, f0, t0, A)
+        ref = ref[:nt]
+        # the reference filters in f32 KISS FFT, the oracle in f64: agree to f32 noise
+        assert np.abs(w - ref).max() <= 1e-6 * np.abs(ref).max()
+        k0 = int(np.float32(t0) / np.float32(dt))
+        assert int(np.argmax(np.abs(w))) == k0
+        # zero phase: symmetric about the onset where both sides fit
+        m = min(k0, nt - 1 - k0, 60)
+        np.testing.assert_allclose(w[k0 - m:k0], w[k0 + m:k0:-1], atol=1e-6 * abs(A))
+
+
+def test_ricker_matches_analytic_shape():
+    # the filtered spike approaches the continuous Ricker A(1-2x)e^{-x}, x=(pi f (t-t0))^2
+    # times the filter's peak value; check the shape (normalised) to 1e-3
+    nt, dt, f0, t0 = 2000, 0.001, 10.0, 0.5
+    w = O.ricker(nt, dt, f0, t0, 1.0).astype(np.float64)
+    k0 = int(np.float32(t0) / np.float32(dt))  # 499, not 500: f32 0.5/0.001 = 499.99997
+    assert k0 == 499
+    t = (np.arange(nt) - k0) * dt
+    x = (np.pi * f0 * t) ** 2
+    r = (1 - 2 * x) * np.exp(-x)
+    np.testing.assert_allclose(w / w.max(), r / r.max(), atol=2e-3)
+
+
+# ------------------------------------------------------------------ afd engine
+
+def _afd2d_oracle_from_golden(g):
+    src = g["src"]
+    ns = len(src) // 5
+    return O.afd2d(g["vel"], int(g["nt"]), float(g["dt"]), float(g["dx"]), float(g["dz"]), int(g["nbt"]),
+                   float(g["ct"]), src[:ns], src[ns:2 * ns], g["rick"], jsnap=int(g["jsnap"]))
+
+
+@pytest.mark.parametrize("name", ["afd2d_cfg0", "afd2d_two_sources", "afd2d_noslack"])
+def test_afd2d_bit_exact(golden, name):
+    g = golden(name)
+    nt, jsnap = int(g["nt"]), int(g["jsnap"])
+    nz, nx = g["vel"].shape
+    d, snaps, _ = _afd2d_oracle_from_golden(g)
+    out = g["out"]
+    tr = out[:nt * nx].reshape(nt, nx, order="F")
+    assert np.array_equal(tr, d.T)
+    if jsnap > 0:
+        w = out[nt * nx:].reshape(nz, nx, -1, order="F")
+        for k in range(w.shape[-1]):
+            assert np.array_equal(w[..., k], snaps[k].T)
+
+
+def test_afd2d_tri_bit_exact(golden):
+    g = golden("afd2d_tri")
+    nt = int(g["nt"])
+    nz, nx = g["vel"].shape
+    d, snaps, img = O.afd2d(g["vel"], nt, float(g["dt"]), float(g["dx"]), float(g["dz"]), int(g["nbt"]),
+                            float(g["ct"]), [], [], None, tri=True, dat=np.ascontiguousarray(g["dat"].T),
+                            jsnap=int(g["jsnap"]))
+    out = g["out"]
+    assert np.array_equal(out[:nt * nx].reshape(nt, nx, order="F"), g["dat"])  # data echoed
+    w = out[nt * nx:].reshape(nz, nx, -1, order="F")
+    assert all(np.array_equal(w[..., k], snaps[k].T) for k in range(w.shape[-1]))
+    assert np.array_equal(img.T, w[..., 0])  # final backward step is it = 0 = snapshot 0
+
+
+def test_afd3d_bit_exact(golden):
+    g = golden("afd3d_two_sources")
+    src, nt = g["src"], int(g["nt"])
+    ns = len(src) // 6
+    nz, nx, ny = g["vel"].shape
+    d, snaps, _ = O.afd3d(g["vel"], nt, float(g["dt"]), float(g["dx"]), float(g["dy"]), float(g["dz"]),
+                          int(g["nbt"]), float(g["ct"]), src[:ns], src[ns:2 * ns], src[2 * ns:3 * ns],
+                          g["rick"], jsnap=int(g["jsnap"]))
+    out = g["out"]
+    assert np.array_equal(out[:nt * nx * ny].reshape(nt, nx, ny, order="F"), d.transpose(2, 1, 0))
+    w = out[nt * nx * ny:].reshape(nz, nx, ny, -1, order="F")
+    assert all(np.array_equal(w[..., k], snaps[k].transpose(2, 1, 0)) for k in range(w.shape[-1]))
+
+
+def test_afd3d_tri_bit_exact(golden):
+    g = golden("afd3d_tri")
+    nt = int(g["nt"])
+    nz, nx, ny = g["vel"].shape
+    _, snaps, _ = O.afd3d(g["vel"], nt, float(g["dt"]), float(g["dx"]), float(g["dy"]), float(g["dz"]),
+                          int(g["nbt"]), float(g["ct"]), [], [], [], None, tri=True,
+                          dat=np.ascontiguousarray(g["dat"].transpose(2, 1, 0)), jsnap=int(g["jsnap"]))
+    w = g["out"][nt * nx * ny:].reshape(nz, nx, ny, -1, order="F")
+    assert all(np.array_equal(w[..., k], snaps[k].transpose(2, 1, 0)) for k in range(w.shape[-1]))
+
+
+def test_afd2d_own_wavelet_within_tolerance(golden):
+    g = golden("afd2d_cfg0")
+    nt = int(g["nt"])
+    rick = O.ricker(nt, float(g["dt"]), 20, 0.1, 1)[None, :]
+    d, _, _ = O.afd2d(g["vel"], nt, float(g["dt"]), 10, 10, 30, 0.01, [50], [5], rick, jsnap=0)
+    tr = g["out"][:nt * 101].reshape(nt, 101, order="F")
+    assert rel_l2(d.T, tr) <= 1e-5
+
+
+def test_afd_linear_in_amplitude():
+    rng = np.random.default_rng(3)
+    vel = (1500 + 1000 * rng.random((30, 34))).astype(np.float32)
+    r1 = O.ricker(200, 0.001, 25, 0.03, 1.0)[None, :]
+    d1, _, _ = O.afd2d(vel, 200, 0.001, 10, 10, 8, 0.03, [15], [12], r1)
+    # a power-of-two scale commutes with every f32 operation (up to denormal rounding)
+    d2, _, _ = O.afd2d(vel, 200, 0.001, 10, 10, 8, 0.03, [15], [12], (2 * r1).astype(np.float32))
+    assert rel_l2(d2, 2 * d1) < 1e-12
+    d3, _, _ = O.afd2d(vel, 200, 0.001, 10, 10, 8, 0.03, [15], [12], (3 * r1).astype(np.float32))
+    assert rel_l2(d3, 3 * d1) < 1e-5
+    d0, _, _ = O.afd2d(vel, 200, 0.001, 10, 10, 8, 0.03, [15], [12], 0 * r1)
+    assert not d0.any()
+
+
+# ------------------------------------------------------------------ FWI
+
+def test_fwi_forward_bit_exact(golden):
+    g = golden("fwi_small")
+    d = O.fwi_forward(g["par"], g["vtrue"], g["wav"])
+    assert np.array_equal(d, g["data"])
+    m = golden("fwi_marine")
+    assert np.array_equal(O.fwi_forward(m["par"], m["vel"], m["wav"]), m["data"])
+
+
+def test_fwi_geometry_closed_form():
+    # preparation (pywave/src/wave_fwiutil.c:45-60) for the demo-like geometry
+    par = np.array([400, 800, 0.01, 0.01, 0, 0, 3000, 0.001, 0, 0, 32, 0.24, 0.2, 5, 60, 0.005, 8, 1, 1, 5],
+                   np.float32)
+    a = O.make_acq(par)
+    geo = O.acq_geometry(a)
+    assert (geo["padnz"], geo["padnx"]) == (520, 920)
+    assert geo["sz"] == 65 and geo["rz"] == 61
+    assert geo["ds_v"] == 24 and geo["s0_v"] == 80 and geo["dr_v"] == 1
+    assert geo["r0_v"] == [60] * 32 and geo["nr2"] == [800] * 32 and geo["r02"] == [0] * 32
+    np.testing.assert_allclose(geo["bc"], np.exp(-(0.005 * (60 - np.arange(60))) ** 2), rtol=1e-6)
+
+
+def test_gradient_av_bit_exact_and_scaling(golden):
+    g = golden("fwi_small")
+    g1, f1, sc = O.gradient_av(g["par"], g["v0"], g["wav"], g["data"])
+    assert np.array_equal(g1, g["grad1"])
+    assert np.float32(f1) == g["fcost"][0]
+    assert abs(np.abs(g1).max() - 0.1) < 0.1  # scaled to 0.1/max before smoothing
+    # a second call reuses the first call's scaling (static 'first' in wave_fwigradient.c:245)
+    g2, f2, _ = O.gradient_av(g["par"], (g["v0"] * 1.01).astype(np.float32), g["wav"], g["data"], scaling=sc)
+    assert np.array_equal(g2, g["grad2"]) and np.float32(f2) == g["fcost"][1]
+    g4, f4, _ = O.gradient_av(g["par4"], g["v0"], g["wav"], g["data"])
+    assert np.array_equal(g4, g["grad_int4"])
+
+
+def test_fwi_lbfgs_bit_exact(golden):
+    g = golden("fwi_small")
+    vinv, k = O.fwi(g["par"], g["v0"], g["wav"], g["data"])
+    assert k == 3
+    assert np.array_equal(vinv, g["vinv"])
+
+
+def _tri_adj_1d(x, r):
+    """Adjoint triangle smoothing, independent form: full convolution with the triangle
+    (r-|k|)/r^2 then half-sample reflection of the overhang back into the line."""
+    nx = len(x)
+    T = (r - np.abs(np.arange(-r + 1, r))) / float(r * r)
+    full = np.convolve(np.asarray(x, np.float64), T)
+    y = full[r - 1:r - 1 + nx].copy()
+    for m in range(r - 1):
+        y[m] += full[r - 2 - m]
+        y[nx - 1 - m] += full[r - 1 + nx + m]
+    return y
+
+
+def test_source_map_is_separable_121():
+    rr = O.source_map(10, 7, 2, 2, 20, 16)
+    expect = np.zeros((16, 20))
+    k = np.array([1, 2, 1]) / 4.0
+    expect[6:9, 9:12] = np.outer(k, k)
+    np.testing.assert_allclose(rr, expect, atol=1e-7)
+    # at an edge the overhang folds back (reflection)
+    rr = O.source_map(0, 0, 2, 2, 8, 8)
+    assert abs(float(rr.sum()) - 1.0) < 1e-6
+
+
+def test_gradient_smooth2_matches_convolution():
+    rng = np.random.default_rng(5)
+    g = rng.standard_normal((13, 17)).astype(np.float32)
+    out = O.gradient_smooth2(g, 3, 3, 0, 2.5)
+    ref = np.apply_along_axis(_tri_adj_1d, 0, g.astype(np.float64), 3)
+    ref = np.apply_along_axis(_tri_adj_1d, 1, ref, 3) * 2.5
+    np.testing.assert_allclose(out, ref, rtol=2e-5, atol=2e-6)
+    # the smoother preserves the sum (adjoint of a normalised reflecting smoother)
+    s = O.gradient_smooth2(g, 3, 3, 0, 1.0)
+    assert abs(float(s.sum()) - float(g.sum())) < 1e-4
