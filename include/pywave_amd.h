@@ -91,6 +91,48 @@ int pw_afd_set_variant(pw_afd *p, int variant, int ny_chunk);
 int pw_afd_variant_count(void);
 void pw_afd_destroy(pw_afd *p);
 
+/* ---------------------------------------------------------------- FWI (pfwi discretisation)
+ * pararray: the 20 floats of pywave/pfwi.py:86-107 (nz,nx,dz,dx,oz,ox,nt,dt,ot,inv,ns,ds,os,sz,
+ * nb,coef,f0,acqui_type,interval,niter); as in pfwic (pywave/src/pfwi.c:796-806) sz=5, rz=1,
+ * nr=nx, dr=dx, r0=ox are fixed.  batch = shots advanced per launch (<= 0: all);
+ * precision = 4 (f32, reference arithmetic) or 8 (f64, for finite-difference checks).
+ * Data arrays are (nt, nr, ns) Fortran order unless noted; device variants use [ns][nt][nr]. */
+typedef struct pw_fwi pw_fwi;
+pw_fwi *pw_fwi_create(pw_ctx *ctx, const float *pararray, int batch, int precision);
+void pw_fwi_destroy(pw_fwi *p);
+/* checkpoint segment length in steps (0 = one segment when it fits half the free HBM, else 50) */
+int pw_fwi_set_checkpointing(pw_fwi *p, int steps);
+/* padnz, padnx, batch, K, nseg, ns, nt, nr */
+int pw_fwi_info(pw_fwi *p, int *ints8);
+int pw_fwi_set_wavelet(pw_fwi *p, const float *wav);
+int pw_fwi_set_observed(pw_fwi *p, const float *data);
+/* forward_modeling_a (pywave/src/wave_fwimodeling.c:21-114) */
+int pw_fwi_forward(pw_fwi *p, const float *vel, float *data_out);
+/* one gradient over all shots: mode 0 = gradient_av (pywave/src/wave_fwigradient.c:93-262:
+ * imaging condition, windowed residual, first-call 0.1/max scaling (scaling_io <= 0 on input ->
+ * computed; NULL -> remembered by the plan), triangle smoothing); mode 1 = exact adjoint of
+ * 0.5*||F(v)-d||^2.  fcost = 0.5*sum (d-F(v))^2 reduced in f64. */
+int pw_fwi_gradient(pw_fwi *p, const float *vel, int mode, float *grad, double *fcost, float *scaling_io);
+/* f64 plans only: exact-adjoint gradient / misfit with f64 model and output */
+int pw_fwi_gradient_f64(pw_fwi *p, const double *vel, int mode, double *grad, double *fcost);
+int pw_fwi_misfit_f64(pw_fwi *p, const double *vel, double *fcost);
+/* device pointers (element type = plan precision) on the context's stream: model (nz*nx),
+ * traces / cotangent [ns][nt][nr], gradient (nz*nx) */
+int pw_fwi_forward_dev(pw_fwi *p, const void *d_vel, void *d_traces);
+int pw_fwi_adjoint_dev(pw_fwi *p, const void *d_vel, const void *d_gtraces, void *d_gvel);
+/* event-timed milliseconds of the last gradient: forward sweeps, backward sweeps */
+int pw_fwi_timing(pw_fwi *p, double *forward_ms, double *backward_ms);
+/* fwi() L-BFGS (pywave/src/wave_fwi.c:24-119, wave_fwilbfgs.c:9-165) with f64 dot products;
+ * vinv: niter*nz*nx, *nstored = iterates written (the rest untouched) */
+int pw_fwi_lbfgs(pw_fwi *p, const float *vel0, int niter, float *vinv, int *nstored);
+
+/* pfwicfun.forward_modeling_ac (pywave/src/pfwi.c:27-495): out = nt*nx*ns, (nt,nx,ns) Fortran */
+int pw_forward_modeling_ac(pw_ctx *ctx, const float *vel, const float *q, const float *wav,
+                           const float *pararray, float *out, long long out_len);
+/* pfwicfun.fwic (pywave/src/pfwi.c:497-1028): out = nz*nx*niter, (nz,nx,niter) Fortran */
+int pw_fwic(pw_ctx *ctx, const float *vel, const float *q, const float *wav, const float *data,
+            const float *pararray, float *out, long long out_len);
+
 #ifdef __cplusplus
 }
 #endif

@@ -49,6 +49,22 @@ SIGNATURES = {
     "pw_afd_set_variant": (c_int, [c_vp, c_int, c_int]),
     "pw_afd_variant_count": (c_int, []),
     "pw_afd_destroy": (None, [c_vp]),
+    "pw_fwi_create": (c_vp, [c_vp, c_vp, c_int, c_int]),
+    "pw_fwi_destroy": (None, [c_vp]),
+    "pw_fwi_set_checkpointing": (c_int, [c_vp, c_int]),
+    "pw_fwi_info": (c_int, [c_vp, c_vp]),
+    "pw_fwi_set_wavelet": (c_int, [c_vp, c_vp]),
+    "pw_fwi_set_observed": (c_int, [c_vp, c_vp]),
+    "pw_fwi_forward": (c_int, [c_vp, c_vp, c_vp]),
+    "pw_fwi_gradient": (c_int, [c_vp, c_vp, c_int, c_vp, ctypes.POINTER(c_double), ctypes.POINTER(c_float)]),
+    "pw_fwi_gradient_f64": (c_int, [c_vp, c_vp, c_int, c_vp, ctypes.POINTER(c_double)]),
+    "pw_fwi_misfit_f64": (c_int, [c_vp, c_vp, ctypes.POINTER(c_double)]),
+    "pw_fwi_forward_dev": (c_int, [c_vp, c_vp, c_vp]),
+    "pw_fwi_adjoint_dev": (c_int, [c_vp, c_vp, c_vp, c_vp]),
+    "pw_fwi_timing": (c_int, [c_vp, ctypes.POINTER(c_double), ctypes.POINTER(c_double)]),
+    "pw_fwi_lbfgs": (c_int, [c_vp, c_vp, c_int, c_vp, ctypes.POINTER(c_int)]),
+    "pw_forward_modeling_ac": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_ll]),
+    "pw_fwic": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_ll]),
 }
 
 

@@ -35,6 +35,213 @@ void abc_weights(int nb, float c, float* w)
     }
 }
 
+// ---------------------------------------------------------------------------------------------
+// f32 mixed-radix FFT following the KISS FFT algorithm vendored by the reference
+// (pywave/src/wave_kissfft.c:37-560): the same factorisation (radix 4 first, then 2, 3, 5,
+// odd primes), the same recursive decimation, the same butterfly arithmetic in f32 and the same
+// double-precision twiddle generation rounded to f32.  Reproducing its rounding makes the
+// wavelet, and therefore every trace and wavefield, bit-identical to the reference.
+namespace {
+
+struct cpx {
+    float r, i;
+};
+
+inline cpx cmul(cpx a, cpx b) { return {a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r}; }
+inline cpx cadd(cpx a, cpx b) { return {a.r + b.r, a.i + b.i}; }
+inline cpx csub(cpx a, cpx b) { return {a.r - b.r, a.i - b.i}; }
+
+struct CFft {
+    int n = 0;
+    bool inverse = false;
+    std::vector<cpx> tw;
+    std::vector<int> fac;  // p0, m0, p1, m1, ...
+
+    CFft(int n_, bool inv) : n(n_), inverse(inv), tw(n_)
+    {
+        const double pi = 3.141592653589793238462643383279502884197169399375105820974944;
+        for (int i = 0; i < n; ++i) {
+            double phase = -2 * pi * i / n;
+            if (inverse) phase *= -1;
+            tw[i] = {(float)std::cos(phase), (float)std::sin(phase)};
+        }
+        int m = n, p = 4;
+        double fsq = std::floor(std::sqrt((double)m));
+        do {
+            while (m % p) {
+                p = p == 4 ? 2 : (p == 2 ? 3 : p + 2);
+                if (p > fsq) p = m;
+            }
+            m /= p;
+            fac.push_back(p);
+            fac.push_back(m);
+        } while (m > 1);
+    }
+
+    void bfly2(cpx* F, size_t fs, int m) const
+    {
+        for (int k = 0; k < m; ++k) {
+            cpx t = cmul(F[m + k], tw[k * fs]);
+            F[m + k] = csub(F[k], t);
+            F[k] = cadd(F[k], t);
+        }
+    }
+
+    void bfly3(cpx* F, size_t fs, size_t m) const
+    {
+        const cpx epi3 = tw[fs * m];
+        for (size_t k = 0; k < m; ++k) {
+            cpx s1 = cmul(F[k + m], tw[k * fs]);
+            cpx s2 = cmul(F[k + 2 * m], tw[2 * k * fs]);
+            cpx s3 = cadd(s1, s2);
+            cpx s0 = csub(s1, s2);
+            F[k + m].r = (float)((double)F[k].r - (double)s3.r * .5);  // HALF_OF promotes to double
+            F[k + m].i = (float)((double)F[k].i - (double)s3.i * .5);
+            s0.r *= epi3.i;
+            s0.i *= epi3.i;
+            F[k] = cadd(F[k], s3);
+            F[k + 2 * m].r = F[k + m].r + s0.i;
+            F[k + 2 * m].i = F[k + m].i - s0.r;
+            F[k + m].r -= s0.i;
+            F[k + m].i += s0.r;
+        }
+    }
+
+    void bfly4(cpx* F, size_t fs, size_t m) const
+    {
+        for (size_t k = 0; k < m; ++k) {
+            cpx s0 = cmul(F[k + m], tw[k * fs]);
+            cpx s1 = cmul(F[k + 2 * m], tw[2 * k * fs]);
+            cpx s2 = cmul(F[k + 3 * m], tw[3 * k * fs]);
+            cpx s5 = csub(F[k], s1);
+            F[k] = cadd(F[k], s1);
+            cpx s3 = cadd(s0, s2);
+            cpx s4 = csub(s0, s2);
+            F[k + 2 * m] = csub(F[k], s3);
+            F[k] = cadd(F[k], s3);
+            if (inverse) {
+                F[k + m] = {s5.r - s4.i, s5.i + s4.r};
+                F[k + 3 * m] = {s5.r + s4.i, s5.i - s4.r};
+            } else {
+                F[k + m] = {s5.r + s4.i, s5.i - s4.r};
+                F[k + 3 * m] = {s5.r - s4.i, s5.i + s4.r};
+            }
+        }
+    }
+
+    void bfly5(cpx* F, size_t fs, int m) const
+    {
+        const cpx ya = tw[fs * m], yb = tw[fs * 2 * m];
+        cpx *F0 = F, *F1 = F + m, *F2 = F + 2 * m, *F3 = F + 3 * m, *F4 = F + 4 * m;
+        for (int u = 0; u < m; ++u) {
+            cpx s0 = F0[u];
+            cpx s1 = cmul(F1[u], tw[u * fs]);
+            cpx s2 = cmul(F2[u], tw[2 * u * fs]);
+            cpx s3 = cmul(F3[u], tw[3 * u * fs]);
+            cpx s4 = cmul(F4[u], tw[4 * u * fs]);
+            cpx s7 = cadd(s1, s4), s10 = csub(s1, s4), s8 = cadd(s2, s3), s9 = csub(s2, s3);
+            F0[u].r += s7.r + s8.r;
+            F0[u].i += s7.i + s8.i;
+            cpx s5 = {s0.r + s7.r * ya.r + s8.r * yb.r, s0.i + s7.i * ya.r + s8.i * yb.r};
+            cpx s6 = {s10.i * ya.i + s9.i * yb.i, -(s10.r * ya.i) - s9.r * yb.i};
+            F1[u] = csub(s5, s6);
+            F4[u] = cadd(s5, s6);
+            cpx s11 = {s0.r + s7.r * yb.r + s8.r * ya.r, s0.i + s7.i * yb.r + s8.i * ya.r};
+            cpx s12 = {-(s10.i * yb.i) + s9.i * ya.i, s10.r * yb.i - s9.r * ya.i};
+            F2[u] = cadd(s11, s12);
+            F3[u] = csub(s11, s12);
+        }
+    }
+
+    void bfly_generic(cpx* F, size_t fs, int m, int p) const
+    {
+        std::vector<cpx> s(p);
+        for (int u = 0; u < m; ++u) {
+            for (int q1 = 0, k = u; q1 < p; ++q1, k += m) s[q1] = F[k];
+            for (int q1 = 0, k = u; q1 < p; ++q1, k += m) {
+                size_t twidx = 0;
+                F[k] = s[0];
+                for (int q = 1; q < p; ++q) {
+                    twidx += fs * k;
+                    if (twidx >= (size_t)n) twidx -= n;
+                    F[k] = cadd(F[k], cmul(s[q], tw[twidx]));
+                }
+            }
+        }
+    }
+
+    void work(cpx* Fout, const cpx* f, size_t fs, const int* fp) const
+    {
+        const int p = fp[0], m = fp[1];
+        if (m == 1) {
+            for (int j = 0; j < p; ++j) Fout[j] = f[j * fs];
+        } else {
+            for (int j = 0; j < p; ++j) work(Fout + j * m, f + j * fs, fs * p, fp + 2);
+        }
+        switch (p) {
+            case 2: bfly2(Fout, fs, m); break;
+            case 3: bfly3(Fout, fs, m); break;
+            case 4: bfly4(Fout, fs, m); break;
+            case 5: bfly5(Fout, fs, m); break;
+            default: bfly_generic(Fout, fs, m, p); break;
+        }
+    }
+
+    void run(const cpx* in, cpx* out) const { work(out, in, 1, fac.data()); }
+};
+
+// real transforms of even length via a half-length complex transform and split twiddles
+struct RFft {
+    int half;
+    CFft sub;
+    std::vector<cpx> sup;
+
+    RFft(int nfft, bool inv) : half(nfft / 2), sub(nfft / 2, inv), sup(nfft / 4)
+    {
+        for (int i = 0; i < half / 2; ++i) {
+            double phase = -3.14159265358979323846264338327 * ((double)(i + 1) / half + .5);
+            if (inv) phase *= -1;
+            sup[i] = {(float)std::cos(phase), (float)std::sin(phase)};
+        }
+    }
+
+    void forward(const float* x, cpx* X) const  // X has half+1 bins
+    {
+        std::vector<cpx> t(half);
+        sub.run(reinterpret_cast<const cpx*>(x), t.data());
+        X[0].r = t[0].r + t[0].i;
+        X[half].r = t[0].r - t[0].i;
+        X[0].i = X[half].i = 0;
+        for (int k = 1; k <= half / 2; ++k) {
+            cpx fpk = t[k], fpnk = {t[half - k].r, -t[half - k].i};
+            cpx f1k = cadd(fpk, fpnk), f2k = csub(fpk, fpnk);
+            cpx w = cmul(f2k, sup[k - 1]);
+            X[k].r = (float)((double)(f1k.r + w.r) * .5);
+            X[k].i = (float)((double)(f1k.i + w.i) * .5);
+            X[half - k].r = (float)((double)(f1k.r - w.r) * .5);
+            X[half - k].i = (float)((double)(w.i - f1k.i) * .5);
+        }
+    }
+
+    void inverse(const cpx* X, float* x) const
+    {
+        std::vector<cpx> t(half);
+        t[0].r = X[0].r + X[half].r;
+        t[0].i = X[0].r - X[half].r;
+        for (int k = 1; k <= half / 2; ++k) {
+            cpx fk = X[k], fnkc = {X[half - k].r, -X[half - k].i};
+            cpx fek = cadd(fk, fnkc), tmp = csub(fk, fnkc);
+            cpx fok = cmul(tmp, sup[k - 1]);
+            t[k] = cadd(fek, fok);
+            t[half - k] = csub(fek, fok);
+            t[half - k].i *= -1;
+        }
+        sub.run(t.data(), reinterpret_cast<cpx*>(x));
+    }
+};
+
+}  // namespace
+
 int ricker(int nt, float dt, float f0, float t0, float A, float* out, std::string* err)
 {
     if (nt <= 0 || !(dt > 0.f)) {
@@ -47,29 +254,26 @@ int ricker(int nt, float dt, float f0, float t0, float A, float* out, std::strin
         if (err) *err = "ricker: onset index int(t/dt) must lie in [0, nt)";
         return PW_ERR_ARG;
     }
+    // spike, then zero-phase filtering by w^2 e^(1-w^2)/nfft over nfft = 2*nfs((nt+1)/2) samples
+    // (pywave/src/wave_ricker.c:22-59, wave_freqfilt.c:141-164, wave_fdm.c:505-513)
     const int nfft = 2 * next_fast_size((nt + 1) / 2);
     const int nw = nfft / 2 + 1;
-    // spectral shape in f32 as the reference forms it (pywave/src/wave_ricker.c:33-51)
     float freq = f0 * dt;
     float dw = 1. / (nfft * freq);
-    std::vector<double> S(nw);
+    std::vector<cpx> shape(nw);
     for (int iw = 0; iw < nw; ++iw) {
         float w = iw * dw;
         w *= w;
-        S[iw] = (double)(w * expf(1 - w) / nfft);
+        shape[iw] = {w * expf(1 - w) / nfft, 0.f};
     }
-    // The filtered spike is A * g[(n - k0) mod nfft] with the real, even kernel
-    //   g[m] = S0 + 2 sum_{k=1}^{nfft/2-1} S_k cos(2 pi k m / nfft) + S_{nfft/2} (-1)^m,
-    // i.e. irfft(rfft(spike) * S) with the unnormalised inverse; evaluated in double.
-    std::vector<double> cs(nfft);
-    for (int k = 0; k < nfft; ++k) cs[k] = std::cos(2.0 * M_PI * (double)k / (double)nfft);
-    for (int n = 0; n < nt; ++n) {
-        long long m = ((long long)n - k0) % nfft;
-        if (m < 0) m += nfft;
-        double g = S[0] + S[nw - 1] * ((m & 1) ? -1.0 : 1.0);
-        for (int k = 1; k < nw - 1; ++k) g += 2.0 * S[k] * cs[(k * m) % nfft];
-        out[n] = (float)((double)A * g);
-    }
+    std::vector<float> buf(nfft, 0.f);
+    buf[k0] = A;
+    std::vector<cpx> spec(nw);
+    RFft fw(nfft, false), iv(nfft, true);
+    fw.forward(buf.data(), spec.data());
+    for (int iw = 0; iw < nw; ++iw) spec[iw] = cmul(spec[iw], shape[iw]);
+    iv.inverse(spec.data(), buf.data());
+    for (int n = 0; n < nt; ++n) out[n] = buf[n];
     return PW_OK;
 }
 

@@ -1,2 +1,1773 @@
-// pw_fwi.hip -- FWI discretisation kernels (filled in below).
+// pw_fwi.hip -- pfwi's acoustic FWI discretisation on MI355X.
+//
+// Reference path (pywave = github.com/chenyk1990/pywave):
+//   geometry/sponge   pywave/src/wave_fwiutil.c:19-121 (preparation), :201-223 (apply_sponge)
+//   8th-order stencil pywave/src/wave_fwiutil.c:17,179-199 (laplace)
+//   source map        pywave/src/wave_fwiutil.c:149-177, pywave/src/wave_triangle.c:177-238
+//   modelling         pywave/src/wave_fwimodeling.c:21-114 (forward_modeling_a)
+//   gradient          pywave/src/wave_fwigradient.c:20-262 (gradient_init, gradient_av)
+//   smoothing         pywave/src/wave_fwiutil.c:298-335 (gradient_smooth2)
+//   optimiser         pywave/src/wave_fwi.c:24-119, pywave/src/wave_fwilbfgs.c:9-165
+//
+// One fused kernel per time step and shot batch.  As in the afd engine the reference's separate
+// passes (record, snapshot, Laplacian, source add, leapfrog, rotate, sponge both levels) become
+// one pass per cell: the kernel reads p1 (sponged) and the previous p1 (sponged once), forms the
+// second sponge of p0 on read, writes S(2 p1 - S(p0raw) + (v v dt^2)(lap p1 + rr w)) over p0raw.
+// Per-cell f32 arithmetic follows the reference's order (built without FP contraction), so traces
+// and the imaging-condition gradient reproduce the reference C bit for bit for one shot per
+// launch; batched launches differ only in the order per-shot gradients are summed.
+//
+// Two gradients are provided:
+//   mode 0  the reference's imaging condition: -2/v^3 * (second time difference of forward
+//           snapshots) * adjoint field, residual windowed, first-call 0.1/max scaling, triangle
+//           smoothing (gradient_av);
+//   mode 1  the exact discrete adjoint of the modelling map (the gradient of 0.5||F(v)-d||^2,
+//           also used by the torch.autograd.Function), checked against finite differences.
+// Both use forward checkpoints every K steps and recompute each segment during the backward
+// sweep; when all segment buffers fit the memory budget the whole run is one segment.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
 #include "pw_internal.h"
+
+namespace {
+
+constexpr double PI_D = 3.14159265358979323846264338328;
+constexpr int PMAX = 7;  // largest source-map patch edge (rect <= 4)
+constexpr int XC = 16;   // x columns streamed per block
+
+template <typename T>
+struct Coef {
+    // pywave/src/wave_fwiutil.c:17 (f32 constants from double literals; f64 for T = double)
+    static constexpr T K0 = (T)(-205. / 72), K1 = (T)(8. / 5), K2 = (T)(-1. / 5), K3 = (T)(8. / 315),
+                       K4 = (T)(-1. / 560);
+};
+
+template <typename T>
+struct FGeo {
+    int pnz, pnx, ldz;  // padded extents, z stride
+    int nz, nx, nb;
+    int nt, nr, rz, dr_v;
+    T dx2, dz2;
+    double wdt2;  // imaging denominator, f32 value promoted (reference promotes to double)
+    long long fs;  // field stride
+};
+
+struct ShotInfo {
+    int sx0, sz0, pw, ph;  // source-map patch origin (padded coords) and extent
+    int r0_v, r02, nr2, shot;
+};
+
+template <typename T>
+struct StepArgs {
+    const T* X;     // p1 (sponged)
+    T* Y;           // in: previous p1; out: new p1
+    const T* m;     // (v*v)*dt^2 on the padded grid
+    const T* sfz;   // sponge factor pairs per z (top, bottom)
+    const T* sfx;   // per x (left, right)
+    const ShotInfo* shots;
+    const T* patch;     // per slot PMAX*PMAX, x-major
+    const T* wav;       // nt
+    T* rec;             // forward: recorded p1 at receivers [slot][nt][nr]
+    const T* inj;       // backward: injected receiver signal [slot][nt][nr]
+    long long trs;      // trace stride per slot (nt*nr)
+    T* snap;            // interior snapshot of p1 for this step: [slot][nx][nz] or null
+    long long snaps;    // snapshot stride per slot
+    T* full;            // full-field copy of p1 for this step [slot][fs] or null
+    long long fulls;
+    int it;
+};
+
+template <typename T>
+__device__ __forceinline__ T sponge(T v, T zl, T zh, T xl, T xh)
+{
+    v = zl * v; v = zh * v;  // top, bottom faces (apply_sponge z loop)
+    v = xl * v; v = xh * v;  // left, right faces (x loop)
+    return v;
+}
+
+// ---------------------------------------------------------------- forward step (+ modelling)
+template <typename T>
+__global__ __launch_bounds__(64) void k_fwd(FGeo<T> g, StepArgs<T> a, int ntz)
+{
+    using C = Coef<T>;
+    const int slot = blockIdx.y;
+    const int tz = blockIdx.x % ntz, xc = blockIdx.x / ntz;
+    const int lane = threadIdx.x, z0 = tz * 64, z = z0 + lane;
+    const int x0 = xc * XC, x1 = min(x0 + XC, g.pnx);
+    __shared__ T row[2][72];
+    const T* __restrict__ X = a.X + slot * g.fs;
+    T* __restrict__ Y = a.Y + slot * g.fs;
+    const ShotInfo sh = a.shots[slot];
+    const bool zok = z < g.pnz;
+    const bool zin = z >= 4 && z < g.pnz - 4;
+    const T zl = zok ? a.sfz[2 * z] : T(1), zh = zok ? a.sfz[2 * z + 1] : T(1);
+    const T wv = a.wav[a.it];
+    T q[9];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        int xx = x0 - 4 + k;
+        q[k] = (zok && xx >= 0 && xx < g.pnx) ? X[(long long)xx * g.ldz + z] : T(0);
+    }
+    int buf = 0;
+    for (int x = x0; x < x1; ++x) {
+        const long long col = (long long)x * g.ldz;
+        q[8] = (zok && x + 4 < g.pnx) ? X[col + 4LL * g.ldz + z] : T(0);
+        const T yv = zok ? Y[col + z] : T(0);
+        const T mv = zok ? a.m[col + z] : T(0);
+        row[buf][lane + 4] = q[4];
+        if (lane < 4) {
+            int za = z0 - 4 + lane, zb = z0 + 64 + lane;
+            row[buf][lane] = za >= 0 ? X[col + za] : T(0);
+            row[buf][68 + lane] = zb < g.pnz ? X[col + zb] : T(0);
+        }
+        __syncthreads();
+        if (zok) {
+            const T p1 = q[4];
+            if (a.rec && z == g.rz) {  // record before the update (wave_fwigradient.c:136-139)
+                T* r = a.rec + slot * a.trs + (long long)a.it * g.nr;
+                if (g.dr_v > 0) {
+                    int d = x - sh.r0_v;
+                    if (d >= 0 && d % g.dr_v == 0 && d / g.dr_v < sh.nr2) {
+                        int k = sh.r02 + d / g.dr_v;
+                        if (k >= 0 && k < g.nr) r[k] = p1;
+                    }
+                } else if (x == sh.r0_v) {
+                    for (int ir = 0; ir < sh.nr2; ++ir)
+                        if (sh.r02 + ir >= 0 && sh.r02 + ir < g.nr) r[sh.r02 + ir] = p1;
+                }
+            }
+            if (a.snap && z >= g.nb && z < g.nb + g.nz && x >= g.nb && x < g.nb + g.nx)
+                a.snap[slot * a.snaps + (long long)(x - g.nb) * g.nz + (z - g.nb)] = p1;
+            if (a.full) a.full[slot * a.fulls + col + z] = p1;
+            T term = T(0);
+            if (zin && x >= 4 && x < g.pnx - 4) {
+                const T* R = &row[buf][lane + 4];
+                term = (C::K0 * p1 + C::K1 * (q[5] + q[3]) + C::K2 * (q[6] + q[2]) + C::K3 * (q[7] + q[1]) +
+                        C::K4 * (q[8] + q[0])) / g.dx2 +
+                       (C::K0 * p1 + C::K1 * (R[1] + R[-1]) + C::K2 * (R[2] + R[-2]) + C::K3 * (R[3] + R[-3]) +
+                        C::K4 * (R[4] + R[-4])) / g.dz2;
+            }
+            const int px = x - sh.sx0, pz = z - sh.sz0;
+            if (px >= 0 && px < sh.pw && pz >= 0 && pz < sh.ph)
+                term = term + a.patch[slot * PMAX * PMAX + px * PMAX + pz] * wv;
+            const T xl = a.sfx[2 * x], xh = a.sfx[2 * x + 1];
+            const T p0 = sponge(yv, zl, zh, xl, xh);
+            T p2 = 2 * p1 - p0 + mv * term;
+            Y[col + z] = sponge(p2, zl, zh, xl, xh);
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) q[k] = q[k + 1];
+        buf ^= 1;
+    }
+}
+
+// ---------------------------------------------------------------- reference imaging-condition adjoint
+template <typename T>
+struct ImgArgs {
+    const T* w_prev;  // snapshots at it-I, it, it+I: [slot][nx][nz] (null when no imaging this step)
+    const T* w_cur;
+    const T* w_next;
+    long long snaps;
+    const float* tmpv;  // -2/v^3 per interior cell (f32 as the reference)
+    T* grad;            // accumulator [slot][nx][nz] (slot stride gstride; 0 = shared)
+    long long gstride;
+};
+
+template <typename T>
+__global__ __launch_bounds__(64) void k_adj_img(FGeo<T> g, StepArgs<T> a, ImgArgs<T> im, int ntz)
+{
+    using C = Coef<T>;
+    const int slot = blockIdx.y;
+    const int tz = blockIdx.x % ntz, xc = blockIdx.x / ntz;
+    const int lane = threadIdx.x, z0 = tz * 64, z = z0 + lane;
+    const int x0 = xc * XC, x1 = min(x0 + XC, g.pnx);
+    __shared__ T row[2][72];
+    const T* __restrict__ X = a.X + slot * g.fs;
+    T* __restrict__ Y = a.Y + slot * g.fs;
+    const ShotInfo sh = a.shots[slot];
+    const bool zok = z < g.pnz;
+    const bool zin = z >= 4 && z < g.pnz - 4;
+    const T zl = zok ? a.sfz[2 * z] : T(1), zh = zok ? a.sfz[2 * z + 1] : T(1);
+    const T* inj = a.inj + slot * a.trs + (long long)a.it * g.nr;
+    T q[9];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        int xx = x0 - 4 + k;
+        q[k] = (zok && xx >= 0 && xx < g.pnx) ? X[(long long)xx * g.ldz + z] : T(0);
+    }
+    int buf = 0;
+    for (int x = x0; x < x1; ++x) {
+        const long long col = (long long)x * g.ldz;
+        q[8] = (zok && x + 4 < g.pnx) ? X[col + 4LL * g.ldz + z] : T(0);
+        const T yv = zok ? Y[col + z] : T(0);
+        const T mv = zok ? a.m[col + z] : T(0);
+        row[buf][lane + 4] = q[4];
+        if (lane < 4) {
+            int za = z0 - 4 + lane, zb = z0 + 64 + lane;
+            row[buf][lane] = za >= 0 ? X[col + za] : T(0);
+            row[buf][68 + lane] = zb < g.pnz ? X[col + zb] : T(0);
+        }
+        __syncthreads();
+        if (zok) {
+            const T p1 = q[4];
+            T term = T(0);
+            if (zin && x >= 4 && x < g.pnx - 4) {
+                const T* R = &row[buf][lane + 4];
+                term = (C::K0 * p1 + C::K1 * (q[5] + q[3]) + C::K2 * (q[6] + q[2]) + C::K3 * (q[7] + q[1]) +
+                        C::K4 * (q[8] + q[0])) / g.dx2 +
+                       (C::K0 * p1 + C::K1 * (R[1] + R[-1]) + C::K2 * (R[2] + R[-2]) + C::K3 * (R[3] + R[-3]) +
+                        C::K4 * (R[4] + R[-4])) / g.dz2;
+            }
+            if (z == g.rz) {  // load the windowed residual at the receivers (:208-211)
+                if (g.dr_v > 0) {
+                    int d = x - sh.r0_v;
+                    if (d >= 0 && d % g.dr_v == 0 && d / g.dr_v < sh.nr2) {
+                        int k = sh.r02 + d / g.dr_v;
+                        if (k >= 0 && k < g.nr) term = term + inj[k];
+                    }
+                } else if (x == sh.r0_v) {
+                    for (int ir = 0; ir < sh.nr2; ++ir)
+                        if (sh.r02 + ir >= 0 && sh.r02 + ir < g.nr) term = term + inj[sh.r02 + ir];
+                }
+            }
+            const T xl = a.sfx[2 * x], xh = a.sfx[2 * x + 1];
+            const T p0 = sponge(yv, zl, zh, xl, xh);
+            T p2 = 2 * p1 - p0 + mv * term;
+            Y[col + z] = sponge(p2, zl, zh, xl, xh);
+            if (im.w_cur && z >= g.nb && z < g.nb + g.nz && x >= g.nb && x < g.nb + g.nx) {
+                const long long k = (long long)(x - g.nb) * g.nz + (z - g.nb);
+                const long long ks = slot * im.snaps + k;
+                double e = ((double)im.w_next[ks] - 2. * (double)im.w_cur[ks] + (double)im.w_prev[ks]) / g.wdt2 *
+                           (double)p1 * (double)im.tmpv[k];
+                T* gp = im.grad + slot * im.gstride + k;
+                *gp = (T)((double)*gp + e);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) q[k] = q[k + 1];
+        buf ^= 1;
+    }
+}
+
+// ---------------------------------------------------------------- exact discrete adjoint
+// q_t = S(2 q_{t+1} - S q_{t+2} + L^T(m q_{t+1}) + R^T g_t);  dJ/dm += (L X_t + r w_t) q_{t+1}
+template <typename T>
+struct ExArgs {
+    const T* F;      // forward p1 at this step, full field [slot][fs]
+    long long fsl;   // slot stride of F
+    T* gm;           // dJ/dm accumulator [slot][fs]
+};
+
+template <typename T>
+__global__ __launch_bounds__(64) void k_adj_exact(FGeo<T> g, StepArgs<T> a, ExArgs<T> ex, int ntz)
+{
+    using C = Coef<T>;
+    const int slot = blockIdx.y;
+    const int tz = blockIdx.x % ntz, xc = blockIdx.x / ntz;
+    const int lane = threadIdx.x, z0 = tz * 64, z = z0 + lane;
+    const int x0 = xc * XC, x1 = min(x0 + XC, g.pnx);
+    __shared__ T rowa[2][72];
+    __shared__ T rowf[2][72];
+    const T* __restrict__ A = a.X + slot * g.fs;
+    T* __restrict__ B = a.Y + slot * g.fs;
+    const T* __restrict__ F = ex.F + slot * ex.fsl;
+    const T* __restrict__ M = a.m;
+    const ShotInfo sh = a.shots[slot];
+    const bool zok = z < g.pnz;
+    const bool zin = z >= 4 && z < g.pnz - 4;
+    const T zl = zok ? a.sfz[2 * z] : T(1), zh = zok ? a.sfz[2 * z + 1] : T(1);
+    const T* inj = a.inj + slot * a.trs + (long long)a.it * g.nr;
+    const T wv = a.wav[a.it];
+    // product m*A masked to the Laplacian's rows [4, n-4) in both axes
+    auto pa = [&](int zz, int xx) -> T {
+        if (zz < 4 || zz >= g.pnz - 4 || xx < 4 || xx >= g.pnx - 4) return T(0);
+        long long j = (long long)xx * g.ldz + zz;
+        return M[j] * A[j];
+    };
+    T qa[9], qf[9];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        int xx = x0 - 4 + k;
+        bool ok = zok && xx >= 0 && xx < g.pnx;
+        qa[k] = ok ? pa(z, xx) : T(0);
+        qf[k] = ok ? F[(long long)xx * g.ldz + z] : T(0);
+    }
+    int buf = 0;
+    for (int x = x0; x < x1; ++x) {
+        const long long col = (long long)x * g.ldz;
+        const bool ok8 = zok && x + 4 < g.pnx;
+        qa[8] = ok8 ? pa(z, x + 4) : T(0);
+        qf[8] = ok8 ? F[col + 4LL * g.ldz + z] : T(0);
+        const T av = zok ? A[col + z] : T(0);
+        const T bv = zok ? B[col + z] : T(0);
+        rowa[buf][lane + 4] = qa[4];
+        rowf[buf][lane + 4] = qf[4];
+        if (lane < 4) {
+            int za = z0 - 4 + lane, zb = z0 + 64 + lane;
+            rowa[buf][lane] = za >= 0 ? pa(za, x) : T(0);
+            rowa[buf][68 + lane] = zb < g.pnz ? pa(zb, x) : T(0);
+            rowf[buf][lane] = za >= 0 ? F[col + za] : T(0);
+            rowf[buf][68 + lane] = zb < g.pnz ? F[col + zb] : T(0);
+        }
+        __syncthreads();
+        if (zok) {
+            // L^T (m q): the symmetric stencil over the masked product, at every cell
+            const T* Ra = &rowa[buf][lane + 4];
+            T lt = (C::K0 * qa[4] + C::K1 * (qa[5] + qa[3]) + C::K2 * (qa[6] + qa[2]) + C::K3 * (qa[7] + qa[1]) +
+                    C::K4 * (qa[8] + qa[0])) / g.dx2 +
+                   (C::K0 * Ra[0] + C::K1 * (Ra[1] + Ra[-1]) + C::K2 * (Ra[2] + Ra[-2]) + C::K3 * (Ra[3] + Ra[-3]) +
+                    C::K4 * (Ra[4] + Ra[-4])) / g.dz2;
+            T src = T(0);
+            if (z == g.rz) {
+                if (g.dr_v > 0) {
+                    int d = x - sh.r0_v;
+                    if (d >= 0 && d % g.dr_v == 0 && d / g.dr_v < sh.nr2) {
+                        int k = sh.r02 + d / g.dr_v;
+                        if (k >= 0 && k < g.nr) src = src + inj[k];
+                    }
+                } else if (x == sh.r0_v) {
+                    for (int ir = 0; ir < sh.nr2; ++ir)
+                        if (sh.r02 + ir >= 0 && sh.r02 + ir < g.nr) src = src + inj[sh.r02 + ir];
+                }
+            }
+            const T xl = a.sfx[2 * x], xh = a.sfx[2 * x + 1];
+            T qn = 2 * av - sponge(bv, zl, zh, xl, xh) + lt + src;
+            B[col + z] = sponge(qn, zl, zh, xl, xh);
+            // dJ/dm accumulation with the forward field's Laplacian (rim rows are zero)
+            T lf = T(0);
+            if (zin && x >= 4 && x < g.pnx - 4) {
+                const T* Rf = &rowf[buf][lane + 4];
+                lf = (C::K0 * qf[4] + C::K1 * (qf[5] + qf[3]) + C::K2 * (qf[6] + qf[2]) + C::K3 * (qf[7] + qf[1]) +
+                      C::K4 * (qf[8] + qf[0])) / g.dx2 +
+                     (C::K0 * Rf[0] + C::K1 * (Rf[1] + Rf[-1]) + C::K2 * (Rf[2] + Rf[-2]) + C::K3 * (Rf[3] + Rf[-3]) +
+                      C::K4 * (Rf[4] + Rf[-4])) / g.dz2;
+            }
+            const int px = x - sh.sx0, pz = z - sh.sz0;
+            if (px >= 0 && px < sh.pw && pz >= 0 && pz < sh.ph)
+                lf = lf + a.patch[slot * PMAX * PMAX + px * PMAX + pz] * wv;
+            ex.gm[slot * g.fs + col + z] += lf * av;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) { qa[k] = qa[k + 1]; qf[k] = qf[k + 1]; }
+        buf ^= 1;
+    }
+}
+
+// ---------------------------------------------------------------- small kernels
+template <typename T>
+__global__ void k_fill(T* p, long long n, T v)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (long long)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// pad2d (wave_fwiutil.c:123-147): clamp-to-edge replication of the model into the padded grid;
+// writes the padded velocity and m = (v*v)*dt^2
+template <typename T>
+__global__ void k_pad_model(FGeo<T> g, const T* __restrict__ vel, T* __restrict__ vpad, T* __restrict__ m, T dt2)
+{
+    const int z = blockIdx.x * blockDim.x + threadIdx.x, x = blockIdx.y;
+    if (z >= g.pnz) return;
+    const int sz = min(max(z - g.nb, 0), g.nz - 1), sx = min(max(x - g.nb, 0), g.nx - 1);
+    const T v = vel[(long long)sx * g.nz + sz];
+    const long long j = (long long)x * g.ldz + z;
+    vpad[j] = v;
+    m[j] = v * v * dt2;
+}
+
+// temp = v; temp = temp*temp*temp; temp = -2./temp  (wave_fwigradient.c:225-227), f32
+__global__ void k_tmpv(int n, const float* __restrict__ vel, float* __restrict__ out)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float t = vel[i];
+    t = t * t * t;
+    out[i] = -2. / t;
+}
+__global__ void k_tmpv_d(int n, const double* __restrict__ vel, float* __restrict__ out)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float t = (float)vel[i];
+    t = t * t * t;
+    out[i] = -2. / t;
+}
+
+// residual r = d - p (misfit terms 0.5 r^2 into per-block partials), then r *= W
+// layouts: obs [shot][nt][nr], pred/res [slot][nt][nr], W [nt][nr]
+template <typename T>
+__global__ void k_residual(int nslot, long long per, const T* __restrict__ obs, const T* __restrict__ pred,
+                           const T* __restrict__ W, T* __restrict__ res, const int* __restrict__ shot_of_slot,
+                           int weighted, int sign, double* __restrict__ partial)
+{
+    __shared__ double red[256];
+    double acc = 0.0;
+    const long long n = per * nslot;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const int s = (int)(i / per);
+        const long long k = i - (long long)s * per;
+        T r = obs[(long long)shot_of_slot[s] * per + k] - pred[i];
+        acc += 0.5 * (double)r * (double)r;
+        if (weighted) r *= W[k];
+        res[i] = sign > 0 ? r : -r;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// deterministic f64 dot / max-abs reductions (two-stage, fixed order)
+template <typename T>
+__global__ void k_dot_partial(long long n, const T* __restrict__ a, const T* __restrict__ b, double* __restrict__ partial)
+{
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        acc += (double)a[i] * (double)b[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+template <typename T>
+__global__ void k_absmax_partial(long long n, const T* __restrict__ a, double* __restrict__ partial)
+{
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        acc = fmax(acc, fabs((double)a[i]));
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ void k_sum_partials(int n, const double* __restrict__ partial, double* __restrict__ out, int is_max)
+{
+    if (threadIdx.x || blockIdx.x) return;
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s = is_max ? fmax(s, partial[i]) : s + partial[i];
+    *out = s;
+}
+
+// sum per-slot interior gradients into the total, in slot order
+template <typename T>
+__global__ void k_add_slots(long long n, int nslot, const T* __restrict__ g, long long gs, T* __restrict__ tot)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    T v = tot[i];
+    for (int s = 0; s < nslot; ++s) v = v + g[s * gs + i];
+    tot[i] = v;
+}
+
+// exact mode: dJ/dv on the model grid = sum over the cells pad2d replicates from each model cell
+// of dJ/dm * 2 v dt^2, over all slots
+template <typename T>
+__global__ void k_gm_to_gv(FGeo<T> g, int nslot, const T* __restrict__ gm, const T* __restrict__ vpad, T dt2,
+                           T* __restrict__ gv)
+{
+    const int iz = blockIdx.x * blockDim.x + threadIdx.x, ix = blockIdx.y;
+    if (iz >= g.nz) return;
+    const int za = iz == 0 ? 0 : iz + g.nb, zb = iz == g.nz - 1 ? g.pnz : iz + g.nb + 1;
+    const int xa = ix == 0 ? 0 : ix + g.nb, xb = ix == g.nx - 1 ? g.pnx : ix + g.nb + 1;
+    double acc = 0.0;
+    for (int x = xa; x < xb; ++x)
+        for (int z = za; z < zb; ++z) {
+            const long long j = (long long)x * g.ldz + z;
+            double s = 0.0;
+            for (int k = 0; k < nslot; ++k) s += (double)gm[k * g.fs + j];
+            acc += s * 2.0 * (double)vpad[j] * (double)dt2;
+        }
+    gv[(long long)ix * g.nz + iz] = (T)((double)gv[(long long)ix * g.nz + iz] + acc);
+}
+
+// adjoint triangle smoothing of one line (np_smooth2, wave_triangle.c:177-238) in the reference's
+// f32 order; one thread per line
+__global__ void k_tri_smooth2(float* __restrict__ f, int nlines, int line_stride, int d, int n, int nb,
+                              float* __restrict__ work)
+{
+    const int li = blockIdx.x * blockDim.x + threadIdx.x;
+    if (li >= nlines) return;
+    const int o = li * line_stride;
+    const int np = n + 2 * nb;
+    float* tmp = work + (long long)li * np;
+    float wt = 1.0 / (nb * nb);
+    for (int i = 0; i < np; ++i) tmp[i] = 0;
+    float a1 = -wt, a2 = 2. * wt, a3 = -wt;
+    for (int i = 0; i < n; ++i) tmp[i] += a1 * f[o + i * d];
+    for (int i = 0; i < n; ++i) tmp[nb + i] += a2 * f[o + i * d];
+    for (int i = 0; i < n; ++i) tmp[2 * nb + i] += a3 * f[o + i * d];
+    float t = 0.;
+    for (int i = 0; i < np; ++i) { t += tmp[i]; tmp[i] = t; }
+    t = 0.;
+    for (int i = np - 1; i >= 0; --i) { t += tmp[i]; tmp[i] = t; }
+    for (int i = 0; i < n; ++i) f[o + i * d] = tmp[i + nb];
+    for (int j = nb + n; j < np; j += n) {
+        for (int i = 0; i < n && i < np - j; ++i) f[o + (n - 1 - i) * d] += tmp[j + i];
+        j += n;
+        for (int i = 0; i < n && i < np - j; ++i) f[o + i * d] += tmp[j + i];
+    }
+    for (int j = nb; j >= 0; j -= n) {
+        for (int i = 0; i < n && i < j; ++i) f[o + i * d] += tmp[j - 1 - i];
+        j -= n;
+        for (int i = 0; i < n && i < j; ++i) f[o + (n - 1 - i) * d] += tmp[j - 1 - i];
+    }
+}
+
+__global__ void k_scale_f(long long n, float* __restrict__ f, float s)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) f[i] *= s;
+}
+
+// L-BFGS vector kernels (x, y in T; scalars are f64 from the host)
+template <typename T>
+__global__ void k_axpby(long long n, double a, const T* __restrict__ x, double b, const T* __restrict__ y, T* __restrict__ out)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (T)(a * (double)x[i] + b * (double)y[i]);
+}
+
+template <typename T>
+__global__ void k_clip(long long n, T* __restrict__ x, T lo, T hi)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        T v = x[i];
+        if (v < lo) v = lo;
+        if (v > hi) v = hi;
+        x[i] = v;
+    }
+}
+
+// out[c][r] = in[r][c]
+template <typename T>
+__global__ void k_transpose_t(const T* __restrict__ in, T* __restrict__ out, int R, int C)
+{
+    __shared__ T t[32][33];
+    const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+    for (int k = threadIdx.y; k < 32; k += blockDim.y) {
+        int r = r0 + k, c = c0 + threadIdx.x;
+        if (r < R && c < C) t[k][threadIdx.x] = in[(size_t)r * C + c];
+    }
+    __syncthreads();
+    for (int k = threadIdx.y; k < 32; k += blockDim.y) {
+        int c = c0 + k, r = r0 + threadIdx.x;
+        if (r < R && c < C) out[(size_t)c * R + r] = t[threadIdx.x][k];
+    }
+}
+
+template <typename S, typename D>
+__global__ void k_convert(long long n, const S* __restrict__ s, D* __restrict__ d)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = (D)s[i];
+}
+
+// ---------------------------------------------------------------- host geometry (preparation)
+struct Acq {
+    int nz, nx, nt, ns, nb, acqui_type, interval, niter, nr;
+    float dz, dx, z0, x0, dt, t0, ds, s0, coef, f0, dr, r0;
+    int padnz, padnx, sz, rz, ds_v, s0_v, dr_v;
+    std::vector<int> r0_v, r02, nr2;
+    std::vector<float> bc;
+};
+
+int acq_from_pararray(const float* p, Acq& a, std::string& err)
+{
+    a.nz = p[0]; a.nx = p[1]; a.dz = p[2]; a.dx = p[3]; a.z0 = p[4]; a.x0 = p[5];
+    a.nt = p[6]; a.dt = p[7]; a.t0 = p[8]; a.ns = p[10]; a.ds = p[11]; a.s0 = p[12];
+    a.nb = p[14]; a.coef = p[15]; a.f0 = p[16]; a.acqui_type = p[17]; a.interval = p[18]; a.niter = p[19];
+    // hard-coded by pfwic (pywave/src/pfwi.c:796-806): sz = 5, nr = nx, dr = dx, r0 = x0, rz = 1
+    a.nr = a.nx; a.dr = a.dx; a.r0 = a.x0;
+    if (a.nz < 1 || a.nx < 1 || a.nt < 2 || a.ns < 1 || a.interval < 1 || !(a.dx > 0) || !(a.dz > 0) || !(a.dt > 0)) {
+        err = "pararray: invalid dimensions (nz, nx, nt, ns, interval must be positive)";
+        return PW_ERR_ARG;
+    }
+    if (a.nb < 4) {
+        err = "pararray: nb must be >= 4 (receivers and sources must stay off the Laplacian's 4-cell rim)";
+        return PW_ERR_ARG;
+    }
+    const int nb = a.nb;
+    a.bc.resize(nb);
+    for (int i = 0; i < nb; ++i) {
+        float tmp = a.coef * (nb - i);
+        a.bc[i] = expf(-tmp * tmp);
+    }
+    a.padnx = a.nx + 2 * nb;
+    a.padnz = a.nz + 2 * nb;
+    a.ds_v = a.ds / a.dx + 0.5;
+    a.s0_v = a.s0 / a.dx + 0.5 + nb;
+    a.sz = 5 + nb;
+    a.dr_v = a.dr / a.dx + 0.5;
+    a.rz = 1 + nb;
+    a.r0_v.resize(a.ns); a.r02.resize(a.ns); a.nr2.resize(a.ns);
+    float xend = a.x0 + (a.nx - 1) * a.dx;
+    for (int i = 0; i < a.ns; ++i) {
+        if (a.acqui_type == 1) {
+            a.r0_v[i] = (a.r0 - a.x0) / a.dx + 0.5 + nb;
+            a.r02[i] = 0;
+            a.nr2[i] = a.nr;
+        } else {
+            float sx = a.s0 + a.ds * i;
+            float rbegin = (sx + a.r0 < a.x0) ? a.x0 : sx + a.r0;
+            float rend = sx + a.r0 + (a.nr - 1) * a.dr;
+            rend = (rend < xend) ? rend : xend;
+            a.r0_v[i] = rbegin / a.dx + 0.5 + nb;
+            a.r02[i] = (rbegin - sx - a.r0) / a.dx + 0.5;
+            a.nr2[i] = (rend - rbegin) / a.dx + 1.5;
+        }
+    }
+    return PW_OK;
+}
+
+// one line of the adjoint triangle smoothing (host copy for the source map)
+void tri_line_host(int o, int d, int n, int nb, float* x)
+{
+    const int np = n + 2 * nb;
+    std::vector<float> tmp(np, 0.f);
+    float wt = 1.0 / (nb * nb);
+    float a1 = -wt, a2 = 2. * wt, a3 = -wt;
+    for (int i = 0; i < n; ++i) tmp[i] += a1 * x[o + i * d];
+    for (int i = 0; i < n; ++i) tmp[nb + i] += a2 * x[o + i * d];
+    for (int i = 0; i < n; ++i) tmp[2 * nb + i] += a3 * x[o + i * d];
+    float t = 0.;
+    for (int i = 0; i < np; ++i) { t += tmp[i]; tmp[i] = t; }
+    t = 0.;
+    for (int i = np - 1; i >= 0; --i) { t += tmp[i]; tmp[i] = t; }
+    for (int i = 0; i < n; ++i) x[o + i * d] = tmp[i + nb];
+    for (int j = nb + n; j < np; j += n) {
+        for (int i = 0; i < n && i < np - j; ++i) x[o + (n - 1 - i) * d] += tmp[j + i];
+        j += n;
+        for (int i = 0; i < n && i < np - j; ++i) x[o + i * d] += tmp[j + i];
+    }
+    for (int j = nb; j >= 0; j -= n) {
+        for (int i = 0; i < n && i < j; ++i) x[o + i * d] += tmp[j - 1 - i];
+        j -= n;
+        for (int i = 0; i < n && i < j; ++i) x[o + (n - 1 - i) * d] += tmp[j - 1 - i];
+    }
+}
+
+// source_map with rect 2 x 2 (pfwi.c:805-806) -> its non-zero patch
+int source_patch(const Acq& a, int sx, int rect, ShotInfo& sh, float* patch, std::string& err)
+{
+    const int pnz = a.padnz, pnx = a.padnx;
+    if (sx - rect + 1 < 4 || sx + rect - 1 >= pnx - 4 || a.sz - rect + 1 < 4 || a.sz + rect - 1 >= pnz - 4) {
+        err = "source map support reaches the padded grid's 4-cell rim (shot at x index " + std::to_string(sx) + ")";
+        return PW_ERR_ARG;
+    }
+    std::vector<float> rr((size_t)pnx * pnz, 0.f);
+    rr[(size_t)sx * pnz + a.sz] = 1.;
+    if (rect > 1) {
+        for (int j = 0; j < pnx; ++j) tri_line_host(j * pnz, 1, pnz, rect, rr.data());
+        for (int j = 0; j < pnz; ++j) tri_line_host(j, pnz, pnx, rect, rr.data());
+    }
+    const int r = rect - 1;
+    sh.sx0 = sx - r; sh.sz0 = a.sz - r; sh.pw = 2 * r + 1; sh.ph = 2 * r + 1;
+    for (int i = 0; i < PMAX * PMAX; ++i) patch[i] = 0.f;
+    for (int px = 0; px < sh.pw; ++px)
+        for (int pz = 0; pz < sh.ph; ++pz)
+            patch[px * PMAX + pz] = rr[(size_t)(sh.sx0 + px) * pnz + sh.sz0 + pz];
+    return PW_OK;
+}
+
+void residual_weights(const Acq& a, std::vector<float>& w /* [ir][it] */)
+{
+    const int nt = a.nt, nr = a.nr;
+    // fwic's full windows (pfwi.c:847-860) through gradient_init (wave_fwigradient.c:75-80)
+    float wt1 = a.t0, wt2 = a.t0 + (nt - 1) * a.dt;
+    float woff1 = a.r0, woff2 = a.r0 + (nr - 1) * a.dr;
+    int wtn1 = (wt1 - a.t0) / a.dt + 0.5, wtn2 = (wt2 - a.t0) / a.dt + 0.5;
+    int woffn1 = (woff1 - a.r0) / a.dr + 0.5, woffn2 = (woff2 - a.r0) / a.dr + 0.5;
+    float sw[10];
+    for (int it = 0; it < 10; ++it) sw[it] = sinf(0.5 * PI_D * (it + 1) / 11.);
+    w.assign((size_t)nt * nr, 0.f);
+    auto W = [&](int ir, int it) -> float& { return w[(size_t)ir * nt + it]; };
+    for (int ir = woffn1; ir <= woffn2; ++ir)
+        for (int it = wtn1; it <= wtn2; ++it) W(ir, it) = 1.;
+    for (int ir = 0; ir < 10; ++ir)
+        for (int it = wtn1; it <= wtn2; ++it) {
+            W(woffn1 + ir, it) *= sw[ir];
+            W(woffn2 - ir, it) *= sw[ir];
+        }
+    for (int it = 0; it < 10; ++it)
+        for (int ir = woffn1; ir <= woffn2; ++ir) {
+            W(ir, wtn1 + it) *= sw[it];
+            W(ir, wtn2 - it) *= sw[it];
+        }
+}
+
+inline int nblk(long long n, int bs = 256) { return (int)std::min<long long>((n + bs - 1) / bs, 1 << 20); }
+
+}  // namespace
+
+// ==================================================================================== plan
+struct pw_fwi {
+    pw_ctx* ctx = nullptr;
+    int prec = 4;  // 4 = f32, 8 = f64
+    Acq acq;
+    int nslot = 1;        // shots per launch
+    int K = 0;            // checkpoint segment length (steps)
+    int ldz = 0;
+    long long fs = 0;     // field elements
+    long long nzx = 0;
+    // buffers (void*; typed by prec)
+    void *vel = nullptr, *vpad = nullptr, *m = nullptr, *sfz = nullptr, *sfx = nullptr, *wav = nullptr;
+    void *X = nullptr, *Y = nullptr, *A = nullptr, *B = nullptr;
+    void *rec = nullptr, *res = nullptr, *obs = nullptr, *W = nullptr;
+    void *ckpt = nullptr, *ckpt_snap = nullptr, *seg = nullptr, *gm = nullptr, *gslot = nullptr, *gtot = nullptr;
+    void *patch = nullptr;
+    float* tmpv = nullptr;
+    float* work = nullptr;  // smoothing workspace
+    ShotInfo* shots = nullptr;
+    int* slot_shot = nullptr;
+    double* partial = nullptr;
+    double* dscalar = nullptr;
+    int nseg = 0;
+    long long seg_cap = 0;  // elements of the segment buffer
+    bool have_wav = false, have_obs = false;
+    float scaling = 0.f;    // first-call gradient scaling (0 = not yet set)
+    std::vector<float> patch_host;  // per shot PMAX*PMAX
+    std::vector<ShotInfo> shot_host;
+    double last_fcost = 0.0;
+    size_t budget = 0;  // bytes allowed for checkpoints + segments
+    double t_forward_ms = 0, t_backward_ms = 0;
+};
+
+namespace {
+
+size_t esz(const pw_fwi* p) { return p->prec == 8 ? 8 : 4; }
+
+template <typename T>
+FGeo<T> geo(const pw_fwi* p)
+{
+    FGeo<T> g{};
+    const Acq& a = p->acq;
+    g.pnz = a.padnz; g.pnx = a.padnx; g.ldz = p->ldz;
+    g.nz = a.nz; g.nx = a.nx; g.nb = a.nb;
+    g.nt = a.nt; g.nr = a.nr; g.rz = a.rz; g.dr_v = a.dr_v;
+    if (sizeof(T) == 4) {
+        float dx2 = a.dx * a.dx, dz2 = a.dz * a.dz;  // wave_fwigradient.c:60-61
+        g.dx2 = dx2; g.dz2 = dz2;
+        float wdt = a.dt * a.interval;
+        float wdt2 = wdt * wdt;
+        g.wdt2 = wdt2;
+    } else {
+        g.dx2 = (T)a.dx * (T)a.dx; g.dz2 = (T)a.dz * (T)a.dz;
+        double wdt = (double)a.dt * a.interval;
+        g.wdt2 = wdt * wdt;
+    }
+    g.fs = p->fs;
+    return g;
+}
+
+template <typename T>
+T dt2_of(const pw_fwi* p)
+{
+    if (sizeof(T) == 4) {
+        float d = p->acq.dt * p->acq.dt;
+        return (T)d;
+    }
+    return (T)p->acq.dt * (T)p->acq.dt;
+}
+
+void fwi_free(pw_fwi* p)
+{
+    void* bufs[] = {p->vel, p->vpad, p->m, p->sfz, p->sfx, p->wav, p->X, p->Y, p->A, p->B, p->rec, p->res,
+                    p->obs, p->W, p->ckpt, p->ckpt_snap, p->seg, p->gm, p->gslot, p->gtot, p->patch,
+                    p->tmpv, p->work, p->shots, p->slot_shot, p->partial, p->dscalar};
+    for (void* b : bufs)
+        if (b) hipFree(b);
+}
+
+#define PW_HIPS(call)                                                                                   \
+    do {                                                                                                \
+        hipError_t e_ = (call);                                                                         \
+        if (e_ != hipSuccess) {                                                                         \
+            p->ctx->err = std::string(#call) + ": " + hipGetErrorString(e_);                            \
+            return PW_ERR_HIP;                                                                          \
+        }                                                                                               \
+    } while (0)
+
+int dmalloc(pw_fwi* p, void** ptr, size_t bytes)
+{
+    if (hipMalloc(ptr, bytes ? bytes : 8) != hipSuccess) {
+        *ptr = nullptr;
+        p->ctx->err = "pw_fwi: device allocation of " + std::to_string(bytes) + " bytes failed";
+        return PW_ERR_NOMEM;
+    }
+    return PW_OK;
+}
+
+// (re)size checkpoint and segment storage for K-step segments
+int plan_storage(pw_fwi* p, int K)
+{
+    const Acq& a = p->acq;
+    const size_t e = esz(p);
+    if (K <= 0) {
+        // one segment when everything fits the budget (no recomputation), else 50 steps
+        size_t seg_all = (size_t)a.nt * p->nslot * p->fs * e;
+        K = seg_all + 2 * p->nslot * p->fs * e <= p->budget ? a.nt : 50;
+    }
+    K = std::max(1, std::min(K, a.nt));
+    if (K % a.interval) K += a.interval - K % a.interval;  // segments start at imaging steps
+    int nseg = (a.nt + K - 1) / K;
+    if (p->ckpt) { hipFree(p->ckpt); p->ckpt = nullptr; }
+    if (p->ckpt_snap) { hipFree(p->ckpt_snap); p->ckpt_snap = nullptr; }
+    if (p->seg) { hipFree(p->seg); p->seg = nullptr; }
+    int rc = dmalloc(p, &p->ckpt, (size_t)nseg * 2 * p->nslot * p->fs * e);
+    if (!rc) rc = dmalloc(p, &p->ckpt_snap, (size_t)nseg * p->nslot * p->nzx * e);
+    // segment buffer: full fields for the exact adjoint (K of them) or interior snapshots for
+    // the imaging condition (K/I + 2); the larger of the two
+    long long full_elems = (long long)K * p->nslot * p->fs;
+    long long snap_elems = (long long)(K / a.interval + 2) * p->nslot * p->nzx;
+    p->seg_cap = std::max(full_elems, snap_elems);
+    if (!rc) rc = dmalloc(p, &p->seg, (size_t)p->seg_cap * e);
+    if (rc) return rc;
+    p->K = K;
+    p->nseg = nseg;
+    return PW_OK;
+}
+
+template <typename T>
+int upload_model(pw_fwi* p, const T* d_vel_src, bool from_device)
+{
+    pw_ctx* ctx = p->ctx;
+    hipStream_t s = ctx->s();
+    const Acq& a = p->acq;
+    if (from_device)
+        PW_HIPS(hipMemcpyAsync(p->vel, d_vel_src, p->nzx * sizeof(T), hipMemcpyDeviceToDevice, s));
+    FGeo<T> g = geo<T>(p);
+    hipLaunchKernelGGL(k_pad_model<T>, dim3((a.padnz + 255) / 256, a.padnx), dim3(256), 0, s, g, (const T*)p->vel,
+                       (T*)p->vpad, (T*)p->m, dt2_of<T>(p));
+    if (sizeof(T) == 4)
+        hipLaunchKernelGGL(k_tmpv, dim3(nblk(p->nzx)), dim3(256), 0, s, (int)p->nzx, (const float*)p->vel, p->tmpv);
+    else
+        hipLaunchKernelGGL(k_tmpv_d, dim3(nblk(p->nzx)), dim3(256), 0, s, (int)p->nzx, (const double*)p->vel, p->tmpv);
+    PW_HIPS(hipGetLastError());
+    return PW_OK;
+}
+
+int upload_vel_host(pw_fwi* p, const float* vel)
+{
+    hipStream_t s = p->ctx->s();
+    if (p->prec == 4) {
+        PW_HIPS(hipMemcpyAsync(p->vel, vel, p->nzx * 4, hipMemcpyHostToDevice, s));
+        return upload_model<float>(p, nullptr, false);
+    }
+    std::vector<double> v(vel, vel + p->nzx);
+    PW_HIPS(hipMemcpyAsync(p->vel, v.data(), p->nzx * 8, hipMemcpyHostToDevice, s));
+    PW_HIPS(hipStreamSynchronize(s));
+    return upload_model<double>(p, nullptr, false);
+}
+
+// per-batch shot tables
+int set_batch(pw_fwi* p, int b0, int nb_)
+{
+    hipStream_t s = p->ctx->s();
+    std::vector<ShotInfo> sh(p->nslot);
+    std::vector<int> sos(p->nslot, 0);
+    for (int k = 0; k < nb_; ++k) {
+        sh[k] = p->shot_host[b0 + k];
+        sos[k] = b0 + k;
+    }
+    PW_HIPS(hipMemcpyAsync(p->shots, sh.data(), sizeof(ShotInfo) * p->nslot, hipMemcpyHostToDevice, s));
+    PW_HIPS(hipMemcpyAsync(p->slot_shot, sos.data(), sizeof(int) * p->nslot, hipMemcpyHostToDevice, s));
+    const size_t e = esz(p);
+    if (p->prec == 4) {
+        PW_HIPS(hipMemcpyAsync(p->patch, p->patch_host.data() + (size_t)b0 * PMAX * PMAX,
+                               sizeof(float) * nb_ * PMAX * PMAX, hipMemcpyHostToDevice, s));
+    } else {
+        std::vector<double> pd(p->patch_host.begin() + (size_t)b0 * PMAX * PMAX,
+                               p->patch_host.begin() + (size_t)(b0 + nb_) * PMAX * PMAX);
+        PW_HIPS(hipMemcpyAsync(p->patch, pd.data(), e * pd.size(), hipMemcpyHostToDevice, s));
+        PW_HIPS(hipStreamSynchronize(s));
+    }
+    PW_HIPS(hipStreamSynchronize(s));
+    return PW_OK;
+}
+
+template <typename T>
+StepArgs<T> base_args(pw_fwi* p)
+{
+    StepArgs<T> a{};
+    a.m = (const T*)p->m;
+    a.sfz = (const T*)p->sfz;
+    a.sfx = (const T*)p->sfx;
+    a.shots = p->shots;
+    a.patch = (const T*)p->patch;
+    a.wav = (const T*)p->wav;
+    a.trs = (long long)p->acq.nt * p->acq.nr;
+    a.snaps = p->nzx;
+    a.fulls = p->fs;
+    return a;
+}
+
+template <typename T>
+int fill(pw_fwi* p, void* ptr, long long n, T v)
+{
+    hipLaunchKernelGGL(k_fill<T>, dim3(nblk(n)), dim3(256), 0, p->ctx->s(), (T*)ptr, n, v);
+    PW_HIPS(hipGetLastError());
+    return PW_OK;
+}
+
+// forward sweep over one batch: traces into rec; checkpoints (and the last segment's
+// snapshots when 'store_last') when 'ckpt'
+template <typename T>
+int forward_batch(pw_fwi* p, int nb_, bool ckpt, int mode, bool store_last)
+{
+    const Acq& a = p->acq;
+    hipStream_t s = p->ctx->s();
+    FGeo<T> g = geo<T>(p);
+    const int ntz = (a.padnz + 63) / 64, nxc = (a.padnx + XC - 1) / XC;
+    const long long fld = (long long)p->nslot * p->fs;
+    int rc = fill<T>(p, p->X, fld, T(0));
+    if (!rc) rc = fill<T>(p, p->Y, fld, T(0));
+    if (!rc) rc = fill<T>(p, p->rec, (long long)p->nslot * a.nt * a.nr, T(0));
+    if (rc) return rc;
+    T* X = (T*)p->X;
+    T* Y = (T*)p->Y;
+    const int I = a.interval, K = p->K;
+    const int last = p->nseg - 1, a_last = last * K;
+    for (int it = 0; it < a.nt; ++it) {
+        StepArgs<T> s_ = base_args<T>(p);
+        s_.X = X; s_.Y = Y; s_.it = it;
+        s_.rec = (T*)p->rec;
+        if (ckpt) {
+            const int sgi = it / K;
+            if (it % K == 0) {
+                T* c = (T*)p->ckpt + (size_t)sgi * 2 * fld;
+                PW_HIPS(hipMemcpyAsync(c, X, fld * sizeof(T), hipMemcpyDeviceToDevice, s));
+                PW_HIPS(hipMemcpyAsync(c + fld, Y, fld * sizeof(T), hipMemcpyDeviceToDevice, s));
+            }
+            if (mode == 0 && (it + I) % K == 0 && it + I < a.nt)  // snapshot just before the next segment
+                s_.snap = (T*)p->ckpt_snap + (size_t)((it + I) / K) * p->nslot * p->nzx;
+            if (store_last && it >= a_last) {
+                if (mode == 1) s_.full = (T*)p->seg + (size_t)(it - a_last) * fld;
+                else if (it % I == 0) s_.snap = (T*)p->seg + (size_t)((it - a_last) / I + 1) * p->nslot * p->nzx;
+            }
+        }
+        hipLaunchKernelGGL(k_fwd<T>, dim3(ntz * nxc, nb_), dim3(64), 0, s, g, s_, ntz);
+        PW_HIPS(hipGetLastError());
+        std::swap(X, Y);
+    }
+    return PW_OK;
+}
+
+// recompute segment sg from its checkpoint into the segment buffer
+template <typename T>
+int recompute_segment(pw_fwi* p, int nb_, int sg, int mode)
+{
+    const Acq& a = p->acq;
+    hipStream_t s = p->ctx->s();
+    FGeo<T> g = geo<T>(p);
+    const int ntz = (a.padnz + 63) / 64, nxc = (a.padnx + XC - 1) / XC;
+    const long long fld = (long long)p->nslot * p->fs;
+    const int I = a.interval, K = p->K;
+    const int a0 = sg * K, b0 = std::min(a0 + K, a.nt);
+    T* c = (T*)p->ckpt + (size_t)sg * 2 * fld;
+    PW_HIPS(hipMemcpyAsync(p->X, c, fld * sizeof(T), hipMemcpyDeviceToDevice, s));
+    PW_HIPS(hipMemcpyAsync(p->Y, c + fld, fld * sizeof(T), hipMemcpyDeviceToDevice, s));
+    T* X = (T*)p->X;
+    T* Y = (T*)p->Y;
+    // imaging needs snapshots up to b0 (inclusive) for segments that are not the last
+    const int t_end = (mode == 0 && b0 < a.nt) ? b0 + 1 : b0;
+    for (int it = a0; it < t_end; ++it) {
+        StepArgs<T> s_ = base_args<T>(p);
+        s_.X = X; s_.Y = Y; s_.it = it;
+        if (mode == 1) s_.full = (T*)p->seg + (size_t)(it - a0) * fld;
+        else if (it % I == 0) s_.snap = (T*)p->seg + (size_t)((it - a0) / I + 1) * p->nslot * p->nzx;
+        hipLaunchKernelGGL(k_fwd<T>, dim3(ntz * nxc, nb_), dim3(64), 0, s, g, s_, ntz);
+        PW_HIPS(hipGetLastError());
+        std::swap(X, Y);
+    }
+    return PW_OK;
+}
+
+// backward sweep for segment sg: steps b0-1 .. a0
+template <typename T>
+int adjoint_segment(pw_fwi* p, int nb_, int sg, int mode, T** Ap, T** Bp, T* grad_ref, long long gstride)
+{
+    const Acq& a = p->acq;
+    hipStream_t s = p->ctx->s();
+    FGeo<T> g = geo<T>(p);
+    const int ntz = (a.padnz + 63) / 64, nxc = (a.padnx + XC - 1) / XC;
+    const long long fld = (long long)p->nslot * p->fs;
+    const int I = a.interval, K = p->K;
+    const int a0 = sg * K, b0 = std::min(a0 + K, a.nt);
+    const int wnt = (a.nt - 1) / I + 1;
+    T* A = *Ap;
+    T* B = *Bp;
+    if (mode == 0 && sg > 0) {  // snapshot at a0 - I into slot 0
+        PW_HIPS(hipMemcpyAsync(p->seg, (T*)p->ckpt_snap + (size_t)sg * p->nslot * p->nzx,
+                               (size_t)p->nslot * p->nzx * sizeof(T), hipMemcpyDeviceToDevice, s));
+    }
+    for (int it = b0 - 1; it >= a0; --it) {
+        StepArgs<T> s_ = base_args<T>(p);
+        s_.X = A; s_.Y = B; s_.it = it;
+        s_.inj = (const T*)p->res;
+        if (mode == 0) {
+            ImgArgs<T> im{};
+            const int wit = it / I;
+            if (it % I == 0 && wit != wnt - 1 && wit != 0) {
+                const long long ss = (long long)p->nslot * p->nzx;
+                const int k = (it - a0) / I + 1;  // slot index of snapshot 'it'
+                im.w_prev = (const T*)p->seg + (size_t)(k - 1) * ss;
+                im.w_cur = (const T*)p->seg + (size_t)k * ss;
+                im.w_next = (const T*)p->seg + (size_t)(k + 1) * ss;
+                im.snaps = p->nzx;
+                im.tmpv = p->tmpv;
+                im.grad = grad_ref;
+                im.gstride = gstride;
+            }
+            hipLaunchKernelGGL(k_adj_img<T>, dim3(ntz * nxc, nb_), dim3(64), 0, s, g, s_, im, ntz);
+        } else {
+            ExArgs<T> ex{};
+            ex.F = (const T*)p->seg + (size_t)(it - a0) * fld;
+            ex.fsl = p->fs;
+            ex.gm = (T*)p->gm;
+            hipLaunchKernelGGL(k_adj_exact<T>, dim3(ntz * nxc, nb_), dim3(64), 0, s, g, s_, ex, ntz);
+        }
+        PW_HIPS(hipGetLastError());
+        std::swap(A, B);
+    }
+    *Ap = A;
+    *Bp = B;
+    return PW_OK;
+}
+
+template <typename T>
+int reduce_sum(pw_fwi* p, double* out_host)
+{
+    (void)sizeof(T);
+    hipStream_t s = p->ctx->s();
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1), 0, s, 1024, p->partial, p->dscalar, 0);
+    PW_HIPS(hipGetLastError());
+    PW_HIPS(hipMemcpyAsync(out_host, p->dscalar, 8, hipMemcpyDeviceToHost, s));
+    PW_HIPS(hipStreamSynchronize(s));
+    return PW_OK;
+}
+
+// residual for the batch: mode 0: windowed d - p (reference); mode 1: p - d (gradient of 0.5||p-d||^2)
+template <typename T>
+int residual_batch(pw_fwi* p, int nb_, int mode, double* fc)
+{
+    const Acq& a = p->acq;
+    hipStream_t s = p->ctx->s();
+    const long long per = (long long)a.nt * a.nr;
+    PW_HIPS(hipMemsetAsync(p->partial, 0, 1024 * 8, s));
+    hipLaunchKernelGGL(k_residual<T>, dim3(1024), dim3(256), 0, s, nb_, per, (const T*)p->obs, (const T*)p->rec,
+                       (const T*)p->W, (T*)p->res, (const int*)p->slot_shot, mode == 0 ? 1 : 0, mode == 0 ? 1 : -1,
+                       p->partial);
+    PW_HIPS(hipGetLastError());
+    double v = 0;
+    int rc = reduce_sum<T>(p, &v);
+    *fc += v;
+    return rc;
+}
+
+// full gradient over all shots.  mode 0 -> float grad in p->gtot (interior, reference order);
+// mode 1 -> T grad in p->gtot.  fcost: FP64 misfit (mode 0: 0.5||d-p||^2 unweighted as the
+// reference reports; mode 1 the same).
+template <typename T>
+int gradient_all(pw_fwi* p, int mode, double* fcost)
+{
+    const Acq& a = p->acq;
+    hipStream_t s = p->ctx->s();
+    if (!p->have_wav || !p->have_obs) PW_FAIL(p->ctx, PW_ERR_STATE, "gradient: wavelet or observed data not set");
+    if (mode == 0 && p->prec != 4) PW_FAIL(p->ctx, PW_ERR_ARG, "gradient: the reference mode runs in f32");
+    const long long fld = (long long)p->nslot * p->fs;
+    int rc = 0;
+    double fc = 0.0;
+    if (mode == 0) rc = fill<float>(p, p->gtot, p->nzx, 0.f);
+    else rc = fill<T>(p, p->gtot, p->nzx, T(0));
+    if (rc) return rc;
+    hipEvent_t e0, e1, e2;
+    PW_HIPS(hipEventCreate(&e0)); PW_HIPS(hipEventCreate(&e1)); PW_HIPS(hipEventCreate(&e2));
+    float fwd_ms = 0, bwd_ms = 0;
+    const bool store_last = true;
+    for (int b0 = 0; b0 < a.ns; b0 += p->nslot) {
+        const int nb_ = std::min(p->nslot, a.ns - b0);
+        if ((rc = set_batch(p, b0, nb_))) return rc;
+        PW_HIPS(hipEventRecord(e0, s));
+        if ((rc = forward_batch<T>(p, nb_, true, mode, store_last))) return rc;
+        if ((rc = residual_batch<T>(p, nb_, mode, &fc))) return rc;
+        PW_HIPS(hipEventRecord(e1, s));
+        if ((rc = fill<T>(p, p->A, fld, T(0))) || (rc = fill<T>(p, p->B, fld, T(0)))) return rc;
+        T* grad_ref = nullptr;
+        long long gstride = 0;
+        if (mode == 0) {
+            if (p->nslot == 1) {
+                grad_ref = (T*)p->gtot;  // one shot per launch: accumulate in the reference's order
+            } else {
+                if ((rc = fill<T>(p, p->gslot, (long long)p->nslot * p->nzx, T(0)))) return rc;
+                grad_ref = (T*)p->gslot;
+                gstride = p->nzx;
+            }
+        } else {
+            if ((rc = fill<T>(p, p->gm, fld, T(0)))) return rc;
+        }
+        T* A = (T*)p->A;
+        T* B = (T*)p->B;
+        for (int sg = p->nseg - 1; sg >= 0; --sg) {
+            if (!(store_last && sg == p->nseg - 1))
+                if ((rc = recompute_segment<T>(p, nb_, sg, mode))) return rc;
+            if ((rc = adjoint_segment<T>(p, nb_, sg, mode, &A, &B, grad_ref, gstride))) return rc;
+        }
+        if (mode == 0 && p->nslot > 1) {
+            hipLaunchKernelGGL(k_add_slots<T>, dim3(nblk(p->nzx)), dim3(256), 0, s, p->nzx, nb_, (const T*)p->gslot,
+                               p->nzx, (T*)p->gtot);
+        } else if (mode == 1) {
+            FGeo<T> g = geo<T>(p);
+            hipLaunchKernelGGL(k_gm_to_gv<T>, dim3((a.nz + 255) / 256, a.nx), dim3(256), 0, s, g, nb_,
+                               (const T*)p->gm, (const T*)p->vpad, dt2_of<T>(p), (T*)p->gtot);
+        }
+        PW_HIPS(hipGetLastError());
+        PW_HIPS(hipEventRecord(e2, s));
+        PW_HIPS(hipEventSynchronize(e2));
+        float m1 = 0, m2 = 0;
+        hipEventElapsedTime(&m1, e0, e1);
+        hipEventElapsedTime(&m2, e1, e2);
+        fwd_ms += m1;
+        bwd_ms += m2;
+    }
+    hipEventDestroy(e0); hipEventDestroy(e1); hipEventDestroy(e2);
+    p->t_forward_ms = fwd_ms;
+    p->t_backward_ms = bwd_ms;
+    *fcost = fc;
+    p->last_fcost = fc;
+    return PW_OK;
+}
+
+// reference post-processing of the mode-0 gradient: first-call scaling, smoothing, *scaling
+int finish_reference_gradient(pw_fwi* p, float* scaling_io)
+{
+    const Acq& a = p->acq;
+    hipStream_t s = p->ctx->s();
+    float* g = (float*)p->gtot;
+    float sc = scaling_io ? *scaling_io : p->scaling;
+    if (!(sc > 0.f)) {
+        hipLaunchKernelGGL(k_absmax_partial<float>, dim3(1024), dim3(256), 0, s, p->nzx, g, p->partial);
+        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1), 0, s, 1024, p->partial, p->dscalar, 1);
+        PW_HIPS(hipGetLastError());
+        double dmax = 0;
+        PW_HIPS(hipMemcpyAsync(&dmax, p->dscalar, 8, hipMemcpyDeviceToHost, s));
+        PW_HIPS(hipStreamSynchronize(s));
+        float fd = (float)dmax;  // max |g| is exact in f32
+        sc = 0.1 / fd;
+        p->scaling = sc;
+    }
+    if (scaling_io) *scaling_io = sc;
+    // gradient_smooth2 with rect 3 x 3 (pfwi.c:858-859): along z for every column, then along x
+    const int r = 3;
+    hipLaunchKernelGGL(k_tri_smooth2, dim3((a.nx + 63) / 64), dim3(64), 0, s, g, a.nx, a.nz, 1, a.nz, r, p->work);
+    hipLaunchKernelGGL(k_tri_smooth2, dim3((a.nz + 63) / 64), dim3(64), 0, s, g, a.nz, 1, a.nz, a.nx, r, p->work);
+    hipLaunchKernelGGL(k_scale_f, dim3(nblk(p->nzx)), dim3(256), 0, s, p->nzx, g, sc);
+    PW_HIPS(hipGetLastError());
+    return PW_OK;
+}
+
+template <typename T>
+int dot_dev(pw_fwi* p, const void* x, const void* y, double* out)
+{
+    hipStream_t s = p->ctx->s();
+    hipLaunchKernelGGL(k_dot_partial<T>, dim3(1024), dim3(256), 0, s, p->nzx, (const T*)x, (const T*)y, p->partial);
+    PW_HIPS(hipGetLastError());
+    return reduce_sum<T>(p, out);
+}
+
+}  // namespace
+
+// ==================================================================================== C ABI
+extern "C" {
+
+pw_fwi* pw_fwi_create(pw_ctx* ctx, const float* pararray, int batch, int precision)
+{
+    if (!ctx || !pararray) return nullptr;
+    if (precision != 4 && precision != 8) {
+        ctx->err = "pw_fwi_create: precision must be 4 (f32) or 8 (f64)";
+        return nullptr;
+    }
+    if (hipSetDevice(ctx->device) != hipSuccess) {
+        ctx->err = "hipSetDevice failed";
+        return nullptr;
+    }
+    pw_fwi* p = new pw_fwi;
+    p->ctx = ctx;
+    p->prec = precision;
+    std::string err;
+    if (acq_from_pararray(pararray, p->acq, err)) {
+        ctx->err = err;
+        delete p;
+        return nullptr;
+    }
+    Acq& a = p->acq;
+    p->nslot = std::max(1, std::min(batch <= 0 ? a.ns : batch, a.ns));
+    p->ldz = (a.padnz + 31) / 32 * 32;
+    p->fs = (long long)p->ldz * a.padnx;
+    p->nzx = (long long)a.nz * a.nx;
+    // geometry per shot
+    p->shot_host.resize(a.ns);
+    p->patch_host.assign((size_t)a.ns * PMAX * PMAX, 0.f);
+    for (int is = 0; is < a.ns; ++is) {
+        ShotInfo& sh = p->shot_host[is];
+        const int sx = a.s0_v + is * a.ds_v;
+        if (source_patch(a, sx, 2, sh, &p->patch_host[(size_t)is * PMAX * PMAX], err)) {
+            ctx->err = "pw_fwi_create: " + err;
+            delete p;
+            return nullptr;
+        }
+        sh.r0_v = a.r0_v[is]; sh.r02 = a.r02[is]; sh.nr2 = a.nr2[is]; sh.shot = is;
+        int rlast = sh.r0_v + (sh.nr2 - 1) * a.dr_v;
+        if (sh.nr2 > 0 && (sh.r0_v < 4 || rlast >= a.padnx - 4)) {
+            ctx->err = "pw_fwi_create: receivers reach the padded grid's 4-cell rim";
+            delete p;
+            return nullptr;
+        }
+    }
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = (size_t)8 << 30;
+    p->budget = free_b / 2;
+    const size_t e = precision;
+    const long long fld = (long long)p->nslot * p->fs;
+    const long long tr = (long long)a.nt * a.nr;
+    int rc = 0;
+    rc = rc ? rc : dmalloc(p, &p->vel, p->nzx * e);
+    rc = rc ? rc : dmalloc(p, &p->vpad, p->fs * e);
+    rc = rc ? rc : dmalloc(p, &p->m, p->fs * e);
+    rc = rc ? rc : dmalloc(p, &p->sfz, 2 * (size_t)a.padnz * e);
+    rc = rc ? rc : dmalloc(p, &p->sfx, 2 * (size_t)a.padnx * e);
+    rc = rc ? rc : dmalloc(p, &p->wav, a.nt * e);
+    rc = rc ? rc : dmalloc(p, &p->X, fld * e);
+    rc = rc ? rc : dmalloc(p, &p->Y, fld * e);
+    rc = rc ? rc : dmalloc(p, &p->A, fld * e);
+    rc = rc ? rc : dmalloc(p, &p->B, fld * e);
+    rc = rc ? rc : dmalloc(p, &p->rec, p->nslot * tr * e);
+    rc = rc ? rc : dmalloc(p, &p->res, p->nslot * tr * e);
+    rc = rc ? rc : dmalloc(p, &p->obs, (size_t)a.ns * tr * e);
+    rc = rc ? rc : dmalloc(p, &p->W, tr * e);
+    rc = rc ? rc : dmalloc(p, &p->gm, fld * e);
+    rc = rc ? rc : dmalloc(p, &p->gslot, (size_t)p->nslot * p->nzx * e);
+    rc = rc ? rc : dmalloc(p, &p->gtot, p->nzx * 8);
+    rc = rc ? rc : dmalloc(p, &p->patch, (size_t)p->nslot * PMAX * PMAX * e);
+    rc = rc ? rc : dmalloc(p, (void**)&p->tmpv, p->nzx * 4);
+    rc = rc ? rc : dmalloc(p, (void**)&p->work, (size_t)std::max(a.nz, a.nx) * (std::max(a.nz, a.nx) + 8) * 4);
+    rc = rc ? rc : dmalloc(p, (void**)&p->shots, sizeof(ShotInfo) * p->nslot);
+    rc = rc ? rc : dmalloc(p, (void**)&p->slot_shot, sizeof(int) * p->nslot);
+    rc = rc ? rc : dmalloc(p, (void**)&p->partial, 1024 * 8);
+    rc = rc ? rc : dmalloc(p, (void**)&p->dscalar, 8);
+    if (!rc) rc = plan_storage(p, 0);
+    if (rc) {
+        fwi_free(p);
+        delete p;
+        return nullptr;
+    }
+    // sponge pairs, residual weights
+    std::vector<double> sz(2 * (size_t)a.padnz, 1.0), sx(2 * (size_t)a.padnx, 1.0);
+    for (int i = 0; i < a.padnz; ++i) {
+        if (i < a.nb) sz[2 * i] = a.bc[i];
+        if (i >= a.padnz - a.nb) sz[2 * i + 1] = a.bc[a.padnz - i - 1];
+    }
+    for (int i = 0; i < a.padnx; ++i) {
+        if (i < a.nb) sx[2 * i] = a.bc[i];
+        if (i >= a.padnx - a.nb) sx[2 * i + 1] = a.bc[a.padnx - i - 1];
+    }
+    std::vector<float> w;
+    residual_weights(a, w);
+    std::vector<float> wt((size_t)a.nt * a.nr);  // [it][ir]
+    for (int ir = 0; ir < a.nr; ++ir)
+        for (int it = 0; it < a.nt; ++it) wt[(size_t)it * a.nr + ir] = w[(size_t)ir * a.nt + it];
+    hipStream_t s = ctx->s();
+    bool ok;
+    if (precision == 4) {
+        std::vector<float> fz(sz.begin(), sz.end()), fx(sx.begin(), sx.end());
+        ok = hipMemcpyAsync(p->sfz, fz.data(), fz.size() * 4, hipMemcpyHostToDevice, s) == hipSuccess &&
+             hipMemcpyAsync(p->sfx, fx.data(), fx.size() * 4, hipMemcpyHostToDevice, s) == hipSuccess &&
+             hipMemcpyAsync(p->W, wt.data(), wt.size() * 4, hipMemcpyHostToDevice, s) == hipSuccess &&
+             hipStreamSynchronize(s) == hipSuccess;
+    } else {
+        std::vector<double> wd(wt.begin(), wt.end());
+        ok = hipMemcpyAsync(p->sfz, sz.data(), sz.size() * 8, hipMemcpyHostToDevice, s) == hipSuccess &&
+             hipMemcpyAsync(p->sfx, sx.data(), sx.size() * 8, hipMemcpyHostToDevice, s) == hipSuccess &&
+             hipMemcpyAsync(p->W, wd.data(), wd.size() * 8, hipMemcpyHostToDevice, s) == hipSuccess &&
+             hipStreamSynchronize(s) == hipSuccess;
+    }
+    if (!ok) {
+        ctx->err = "pw_fwi_create: upload failed";
+        fwi_free(p);
+        delete p;
+        return nullptr;
+    }
+    return p;
+}
+
+void pw_fwi_destroy(pw_fwi* p)
+{
+    if (!p) return;
+    hipSetDevice(p->ctx->device);
+    hipStreamSynchronize(p->ctx->s());
+    fwi_free(p);
+    delete p;
+}
+
+int pw_fwi_set_checkpointing(pw_fwi* p, int steps)
+{
+    if (!p || steps < 0) return PW_ERR_ARG;
+    return plan_storage(p, steps);
+}
+
+int pw_fwi_info(pw_fwi* p, int* ints /* 8 */)
+{
+    if (!p || !ints) return PW_ERR_ARG;
+    const Acq& a = p->acq;
+    ints[0] = a.padnz; ints[1] = a.padnx; ints[2] = p->nslot; ints[3] = p->K;
+    ints[4] = p->nseg; ints[5] = a.ns; ints[6] = a.nt; ints[7] = a.nr;
+    return PW_OK;
+}
+
+int pw_fwi_set_wavelet(pw_fwi* p, const float* wav)
+{
+    if (!p || !wav) return PW_ERR_ARG;
+    hipStream_t s = p->ctx->s();
+    if (p->prec == 4) {
+        PW_HIPS(hipMemcpyAsync(p->wav, wav, p->acq.nt * 4, hipMemcpyHostToDevice, s));
+    } else {
+        std::vector<double> w(wav, wav + p->acq.nt);
+        PW_HIPS(hipMemcpyAsync(p->wav, w.data(), w.size() * 8, hipMemcpyHostToDevice, s));
+    }
+    PW_HIPS(hipStreamSynchronize(s));
+    p->have_wav = true;
+    return PW_OK;
+}
+
+// data: (nt, nr, ns) Fortran order = [is][ir][it]; stored time-major [is][it][ir]
+int pw_fwi_set_observed(pw_fwi* p, const float* data)
+{
+    if (!p || !data) return PW_ERR_ARG;
+    const Acq& a = p->acq;
+    hipStream_t s = p->ctx->s();
+    const long long per = (long long)a.nt * a.nr;
+    std::vector<float> t((size_t)per);
+    std::vector<double> td;
+    for (int is = 0; is < a.ns; ++is) {
+        const float* d = data + (size_t)is * per;
+        for (int ir = 0; ir < a.nr; ++ir)
+            for (int it = 0; it < a.nt; ++it) t[(size_t)it * a.nr + ir] = d[(size_t)ir * a.nt + it];
+        if (p->prec == 4) {
+            PW_HIPS(hipMemcpyAsync((float*)p->obs + (size_t)is * per, t.data(), per * 4, hipMemcpyHostToDevice, s));
+            PW_HIPS(hipStreamSynchronize(s));
+        } else {
+            td.assign(t.begin(), t.end());
+            PW_HIPS(hipMemcpyAsync((double*)p->obs + (size_t)is * per, td.data(), per * 8, hipMemcpyHostToDevice, s));
+            PW_HIPS(hipStreamSynchronize(s));
+        }
+    }
+    p->have_obs = true;
+    return PW_OK;
+}
+
+// forward_modeling_a: data_out (nt, nr, ns) Fortran order
+int pw_fwi_forward(pw_fwi* p, const float* vel, float* data_out)
+{
+    if (!p || !vel || !data_out) return PW_ERR_ARG;
+    if (!p->have_wav) PW_FAIL(p->ctx, PW_ERR_STATE, "forward: wavelet not set");
+    const Acq& a = p->acq;
+    hipStream_t s = p->ctx->s();
+    int rc = upload_vel_host(p, vel);
+    if (rc) return rc;
+    const long long per = (long long)a.nt * a.nr;
+    std::vector<float> t((size_t)per * p->nslot);
+    std::vector<double> td;
+    for (int b0 = 0; b0 < a.ns; b0 += p->nslot) {
+        const int nb_ = std::min(p->nslot, a.ns - b0);
+        if ((rc = set_batch(p, b0, nb_))) return rc;
+        rc = p->prec == 4 ? forward_batch<float>(p, nb_, false, 0, false) : forward_batch<double>(p, nb_, false, 0, false);
+        if (rc) return rc;
+        if (p->prec == 4) {
+            PW_HIPS(hipMemcpyAsync(t.data(), p->rec, (size_t)nb_ * per * 4, hipMemcpyDeviceToHost, s));
+            PW_HIPS(hipStreamSynchronize(s));
+        } else {
+            td.resize((size_t)nb_ * per);
+            PW_HIPS(hipMemcpyAsync(td.data(), p->rec, (size_t)nb_ * per * 8, hipMemcpyDeviceToHost, s));
+            PW_HIPS(hipStreamSynchronize(s));
+            for (size_t i = 0; i < td.size(); ++i) t[i] = (float)td[i];
+        }
+        for (int k = 0; k < nb_; ++k) {
+            float* d = data_out + (size_t)(b0 + k) * per;
+            const float* src = t.data() + (size_t)k * per;
+            for (int it = 0; it < a.nt; ++it)
+                for (int ir = 0; ir < a.nr; ++ir) d[(size_t)ir * a.nt + it] = src[(size_t)it * a.nr + ir];
+        }
+    }
+    return PW_OK;
+}
+
+// one gradient over all shots.  mode 0: gradient_av (scaling_io: in <= 0 -> first call, out the
+// scaling used; NULL -> the plan remembers its first-call scaling).  mode 1: exact adjoint of
+// 0.5||F(v)-d||^2.  fcost: FP64 misfit 0.5 sum (d-p)^2.
+int pw_fwi_gradient(pw_fwi* p, const float* vel, int mode, float* grad, double* fcost, float* scaling_io)
+{
+    if (!p || !vel || !grad || (mode != 0 && mode != 1)) return PW_ERR_ARG;
+    hipStream_t s = p->ctx->s();
+    int rc = upload_vel_host(p, vel);
+    if (rc) return rc;
+    double fc = 0;
+    if (p->prec == 4) rc = gradient_all<float>(p, mode, &fc);
+    else rc = gradient_all<double>(p, mode, &fc);
+    if (rc) return rc;
+    if (mode == 0) {
+        if ((rc = finish_reference_gradient(p, scaling_io))) return rc;
+        PW_HIPS(hipMemcpyAsync(grad, p->gtot, p->nzx * 4, hipMemcpyDeviceToHost, s));
+    } else if (p->prec == 4) {
+        PW_HIPS(hipMemcpyAsync(grad, p->gtot, p->nzx * 4, hipMemcpyDeviceToHost, s));
+    } else {
+        std::vector<double> gd(p->nzx);
+        PW_HIPS(hipMemcpyAsync(gd.data(), p->gtot, p->nzx * 8, hipMemcpyDeviceToHost, s));
+        PW_HIPS(hipStreamSynchronize(s));
+        for (long long i = 0; i < p->nzx; ++i) grad[i] = (float)gd[i];
+    }
+    PW_HIPS(hipStreamSynchronize(s));
+    if (fcost) *fcost = fc;
+    return PW_OK;
+}
+
+// f64 gradient output (for the finite-difference check)
+int pw_fwi_gradient_f64(pw_fwi* p, const double* vel, int mode, double* grad, double* fcost)
+{
+    if (!p || !vel || !grad || p->prec != 8 || mode != 1) return PW_ERR_ARG;
+    hipStream_t s = p->ctx->s();
+    PW_HIPS(hipMemcpyAsync(p->vel, vel, p->nzx * 8, hipMemcpyHostToDevice, s));
+    int rc = upload_model<double>(p, nullptr, false);
+    double fc = 0;
+    if (!rc) rc = gradient_all<double>(p, mode, &fc);
+    if (rc) return rc;
+    PW_HIPS(hipMemcpyAsync(grad, p->gtot, p->nzx * 8, hipMemcpyDeviceToHost, s));
+    PW_HIPS(hipStreamSynchronize(s));
+    if (fcost) *fcost = fc;
+    return PW_OK;
+}
+
+// misfit only (forward sweep + FP64 reduction), f64 model input
+int pw_fwi_misfit_f64(pw_fwi* p, const double* vel, double* fcost)
+{
+    if (!p || !vel || !fcost) return PW_ERR_ARG;
+    if (!p->have_wav || !p->have_obs) PW_FAIL(p->ctx, PW_ERR_STATE, "misfit: wavelet or observed data not set");
+    hipStream_t s = p->ctx->s();
+    int rc;
+    if (p->prec == 8) {
+        PW_HIPS(hipMemcpyAsync(p->vel, vel, p->nzx * 8, hipMemcpyHostToDevice, s));
+        rc = upload_model<double>(p, nullptr, false);
+    } else {
+        std::vector<float> v(vel, vel + p->nzx);
+        rc = upload_vel_host(p, v.data());
+    }
+    if (rc) return rc;
+    double fc = 0;
+    const Acq& a = p->acq;
+    for (int b0 = 0; b0 < a.ns; b0 += p->nslot) {
+        const int nb_ = std::min(p->nslot, a.ns - b0);
+        if ((rc = set_batch(p, b0, nb_))) return rc;
+        rc = p->prec == 4 ? forward_batch<float>(p, nb_, false, 1, false) : forward_batch<double>(p, nb_, false, 1, false);
+        if (!rc) rc = p->prec == 4 ? residual_batch<float>(p, nb_, 1, &fc) : residual_batch<double>(p, nb_, 1, &fc);
+        if (rc) return rc;
+    }
+    *fcost = fc;
+    return PW_OK;
+}
+
+// device-pointer variants for torch (T = the plan precision); traces [ns][nt][nr] time-major
+int pw_fwi_forward_dev(pw_fwi* p, const void* d_vel, void* d_traces)
+{
+    if (!p || !d_vel || !d_traces) return PW_ERR_ARG;
+    if (!p->have_wav) PW_FAIL(p->ctx, PW_ERR_STATE, "forward: wavelet not set");
+    const Acq& a = p->acq;
+    hipStream_t s = p->ctx->s();
+    int rc = p->prec == 4 ? upload_model<float>(p, (const float*)d_vel, true) : upload_model<double>(p, (const double*)d_vel, true);
+    if (rc) return rc;
+    const long long per = (long long)a.nt * a.nr;
+    const size_t e = esz(p);
+    for (int b0 = 0; b0 < a.ns; b0 += p->nslot) {
+        const int nb_ = std::min(p->nslot, a.ns - b0);
+        if ((rc = set_batch(p, b0, nb_))) return rc;
+        rc = p->prec == 4 ? forward_batch<float>(p, nb_, false, 1, false) : forward_batch<double>(p, nb_, false, 1, false);
+        if (rc) return rc;
+        PW_HIPS(hipMemcpyAsync((char*)d_traces + (size_t)b0 * per * e, p->rec, (size_t)nb_ * per * e,
+                               hipMemcpyDeviceToDevice, s));
+    }
+    return PW_OK;
+}
+
+// gradient of <g, F(v)> w.r.t. v for traces cotangent g [ns][nt][nr] (exact adjoint)
+int pw_fwi_adjoint_dev(pw_fwi* p, const void* d_vel, const void* d_gtraces, void* d_gvel)
+{
+    if (!p || !d_vel || !d_gtraces || !d_gvel) return PW_ERR_ARG;
+    if (!p->have_wav) PW_FAIL(p->ctx, PW_ERR_STATE, "adjoint: wavelet not set");
+    const Acq& a = p->acq;
+    hipStream_t s = p->ctx->s();
+    int rc = p->prec == 4 ? upload_model<float>(p, (const float*)d_vel, true) : upload_model<double>(p, (const double*)d_vel, true);
+    if (rc) return rc;
+    const size_t e = esz(p);
+    const long long per = (long long)a.nt * a.nr;
+    const long long fld = (long long)p->nslot * p->fs;
+    if (p->prec == 4) rc = fill<float>(p, p->gtot, p->nzx, 0.f);
+    else rc = fill<double>(p, p->gtot, p->nzx, 0.0);
+    if (rc) return rc;
+    for (int b0 = 0; b0 < a.ns; b0 += p->nslot) {
+        const int nb_ = std::min(p->nslot, a.ns - b0);
+        if ((rc = set_batch(p, b0, nb_))) return rc;
+        // the cotangent is the injected signal of the adjoint sweep
+        PW_HIPS(hipMemcpyAsync(p->res, (const char*)d_gtraces + (size_t)b0 * per * e, (size_t)nb_ * per * e,
+                               hipMemcpyDeviceToDevice, s));
+        if (p->prec == 4) {
+            rc = forward_batch<float>(p, nb_, true, 1, true);
+            if (!rc) rc = fill<float>(p, p->A, fld, 0.f);
+            if (!rc) rc = fill<float>(p, p->B, fld, 0.f);
+            if (!rc) rc = fill<float>(p, p->gm, fld, 0.f);
+            float *A = (float*)p->A, *B = (float*)p->B;
+            for (int sg = p->nseg - 1; sg >= 0 && !rc; --sg) {
+                if (sg != p->nseg - 1) rc = recompute_segment<float>(p, nb_, sg, 1);
+                if (!rc) rc = adjoint_segment<float>(p, nb_, sg, 1, &A, &B, nullptr, 0);
+            }
+            if (!rc) {
+                FGeo<float> g = geo<float>(p);
+                hipLaunchKernelGGL(k_gm_to_gv<float>, dim3((a.nz + 255) / 256, a.nx), dim3(256), 0, s, g, nb_,
+                                   (const float*)p->gm, (const float*)p->vpad, dt2_of<float>(p), (float*)p->gtot);
+            }
+        } else {
+            rc = forward_batch<double>(p, nb_, true, 1, true);
+            if (!rc) rc = fill<double>(p, p->A, fld, 0.0);
+            if (!rc) rc = fill<double>(p, p->B, fld, 0.0);
+            if (!rc) rc = fill<double>(p, p->gm, fld, 0.0);
+            double *A = (double*)p->A, *B = (double*)p->B;
+            for (int sg = p->nseg - 1; sg >= 0 && !rc; --sg) {
+                if (sg != p->nseg - 1) rc = recompute_segment<double>(p, nb_, sg, 1);
+                if (!rc) rc = adjoint_segment<double>(p, nb_, sg, 1, &A, &B, nullptr, 0);
+            }
+            if (!rc) {
+                FGeo<double> g = geo<double>(p);
+                hipLaunchKernelGGL(k_gm_to_gv<double>, dim3((a.nz + 255) / 256, a.nx), dim3(256), 0, s, g, nb_,
+                                   (const double*)p->gm, (const double*)p->vpad, dt2_of<double>(p), (double*)p->gtot);
+            }
+        }
+        if (rc) return rc;
+        PW_HIPS(hipGetLastError());
+    }
+    PW_HIPS(hipMemcpyAsync(d_gvel, p->gtot, p->nzx * e, hipMemcpyDeviceToDevice, s));
+    return PW_OK;
+}
+
+int pw_fwi_timing(pw_fwi* p, double* forward_ms, double* backward_ms)
+{
+    if (!p) return PW_ERR_ARG;
+    if (forward_ms) *forward_ms = p->t_forward_ms;
+    if (backward_ms) *backward_ms = p->t_backward_ms;
+    return PW_OK;
+}
+
+// fwi() (pywave/src/wave_fwi.c:24-119) with the Wolfe line search of wave_fwilbfgs.c:108-165 and
+// the two-loop recursion of :38-95; vectors stay on the device and every dot product is an f64
+// reduction.  vinv: niter * nz*nx; returns the number of stored iterates in *nstored (slots after
+// an early stop are left untouched).
+int pw_fwi_lbfgs(pw_fwi* p, const float* vel0, int niter, float* vinv, int* nstored)
+{
+    if (!p || !vel0 || !vinv || niter < 0) return PW_ERR_ARG;
+    if (p->prec != 4) PW_FAIL(p->ctx, PW_ERR_ARG, "lbfgs: the reference optimiser runs in f32");
+    hipStream_t s = p->ctx->s();
+    const long long n = p->nzx;
+    const int npair = 20, nls = 20;
+    const double c1 = (float)1e-4, c2 = (float)0.9, factor = 10, conv = (float)0.01;
+    const float v1 = 0.f, v2 = 10.f;
+    float *x = nullptr, *g = nullptr, *dir = nullptr, *xk = nullptr, *q = nullptr, *sk = nullptr, *yk = nullptr;
+    size_t bytes = n * 4;
+    int rc = 0;
+    rc = rc ? rc : dmalloc(p, (void**)&x, bytes);
+    rc = rc ? rc : dmalloc(p, (void**)&g, bytes);
+    rc = rc ? rc : dmalloc(p, (void**)&dir, bytes);
+    rc = rc ? rc : dmalloc(p, (void**)&xk, bytes);
+    rc = rc ? rc : dmalloc(p, (void**)&q, bytes);
+    rc = rc ? rc : dmalloc(p, (void**)&sk, bytes * npair);
+    rc = rc ? rc : dmalloc(p, (void**)&yk, bytes * npair);
+    auto cleanup = [&]() {
+        float* bs[] = {x, g, dir, xk, q, sk, yk};
+        for (float* b : bs)
+            if (b) hipFree(b);
+    };
+    if (rc) { cleanup(); return rc; }
+    auto grad_at = [&](const float* dx, double* f) -> int {
+        // gradient of the current device model (copied to the plan's model buffer)
+        PW_HIPS(hipMemcpyAsync(p->vel, dx, bytes, hipMemcpyDeviceToDevice, s));
+        int r = upload_model<float>(p, nullptr, false);
+        if (!r) r = gradient_all<float>(p, 0, f);
+        if (!r) r = finish_reference_gradient(p, nullptr);
+        if (!r) PW_HIPS(hipMemcpyAsync(g, p->gtot, bytes, hipMemcpyDeviceToDevice, s));
+        return r;
+    };
+    auto axpby = [&](double al, const float* xa, double be, const float* yb, float* out) {
+        hipLaunchKernelGGL(k_axpby<float>, dim3(nblk(n)), dim3(256), 0, s, n, al, xa, be, yb, out);
+    };
+    auto dot = [&](const float* a_, const float* b_, double* o) { return dot_dev<float>(p, a_, b_, o); };
+    int stored = 0;
+    do {
+        if ((rc = hipMemcpyAsync(x, vel0, bytes, hipMemcpyHostToDevice, s) != hipSuccess ? PW_ERR_HIP : 0)) break;
+        double fk = 0;
+        p->scaling = 0.f;  // gradient_init: first call sets the scaling
+        if ((rc = grad_at(x, &fk))) break;
+        const double f0 = fk;
+        int ipair = 0;
+        double alpha = 1.0;
+        for (int iter = 0; iter < niter; ++iter) {
+            if (iter == 0) {
+                axpby(-1.0, g, 0.0, g, dir);
+            } else {
+                // lbfgs_update: s = x - s_last, y = g - y_last
+                float* sl = sk + (size_t)(ipair - 1) * n;
+                float* yl = yk + (size_t)(ipair - 1) * n;
+                axpby(1.0, x, -1.0, sl, sl);
+                axpby(1.0, g, -1.0, yl, yl);
+                // lbfgs_direction
+                double ns_ = 0, ny_ = 0;
+                if ((rc = dot(sl, sl, &ns_)) || (rc = dot(yl, yl, &ny_))) break;
+                if (ns_ == 0.0 || ny_ == 0.0) {
+                    axpby(-1.0, g, 0.0, g, dir);
+                } else {
+                    std::vector<double> rho(ipair), al(ipair);
+                    PW_HIPS(hipMemcpyAsync(q, g, bytes, hipMemcpyDeviceToDevice, s));
+                    for (int i = ipair - 1; i >= 0 && !rc; --i) {
+                        float* si = sk + (size_t)i * n;
+                        float* yi = yk + (size_t)i * n;
+                        double t = 0;
+                        if ((rc = dot(yi, si, &t))) break;
+                        rho[i] = 1.0 / t;
+                        if ((rc = dot(si, q, &t))) break;
+                        al[i] = rho[i] * t;
+                        axpby(1.0, q, -al[i], yi, q);
+                    }
+                    if (rc) break;
+                    double yy = 0;
+                    if ((rc = dot(yl, yl, &yy))) break;
+                    const double gamma = 1.0 / yy / rho[ipair - 1];
+                    axpby(gamma, q, 0.0, q, dir);
+                    for (int i = 0; i < ipair && !rc; ++i) {
+                        float* si = sk + (size_t)i * n;
+                        float* yi = yk + (size_t)i * n;
+                        double t = 0;
+                        if ((rc = dot(yi, dir, &t))) break;
+                        const double beta = t * rho[i];
+                        axpby(1.0, dir, al[i] - beta, si, dir);
+                    }
+                    if (rc) break;
+                    axpby(-1.0, dir, 0.0, dir, dir);
+                }
+            }
+            // lbfgs_save
+            if (ipair < npair) {
+                PW_HIPS(hipMemcpyAsync(sk + (size_t)ipair * n, x, bytes, hipMemcpyDeviceToDevice, s));
+                PW_HIPS(hipMemcpyAsync(yk + (size_t)ipair * n, g, bytes, hipMemcpyDeviceToDevice, s));
+                ipair++;
+            } else {
+                for (int i = 0; i < npair - 1; ++i) {
+                    PW_HIPS(hipMemcpyAsync(sk + (size_t)i * n, sk + (size_t)(i + 1) * n, bytes, hipMemcpyDeviceToDevice, s));
+                    PW_HIPS(hipMemcpyAsync(yk + (size_t)i * n, yk + (size_t)(i + 1) * n, bytes, hipMemcpyDeviceToDevice, s));
+                }
+                PW_HIPS(hipMemcpyAsync(sk + (size_t)(npair - 1) * n, x, bytes, hipMemcpyDeviceToDevice, s));
+                PW_HIPS(hipMemcpyAsync(yk + (size_t)(npair - 1) * n, g, bytes, hipMemcpyDeviceToDevice, s));
+            }
+            // Wolfe line search
+            PW_HIPS(hipMemcpyAsync(xk, x, bytes, hipMemcpyDeviceToDevice, s));
+            double m1 = 0;
+            if ((rc = dot(g, dir, &m1))) break;
+            const double m2 = m1 * c2;
+            m1 = m1 * c1;
+            double a1 = 0., a2 = 0., fcost = 0.;
+            int flag = 0, i;
+            for (i = 0; i < nls; ++i) {
+                axpby(1.0, xk, alpha, dir, x);
+                hipLaunchKernelGGL(k_clip<float>, dim3(nblk(n)), dim3(256), 0, s, n, x, v1, v2);
+                if ((rc = grad_at(x, &fcost))) break;
+                double m3 = 0;
+                if ((rc = dot(g, dir, &m3))) break;
+                if (fcost <= fk + alpha * m1 && m3 >= m2) {
+                    fk = fcost;
+                    flag = 0;
+                    break;
+                } else if (fcost > fk + alpha * m1) {
+                    a2 = alpha;
+                    alpha = 0.5 * (a1 + a2);
+                } else {
+                    a1 = alpha;
+                    if (a2 == 0.) alpha *= factor;
+                    else alpha = 0.5 * (a1 + a2);
+                }
+            }
+            if (rc) break;
+            if (i == nls) flag = fcost <= fk ? 1 : 2;
+            if (flag == 2) break;
+            if (fk / f0 < conv) break;
+            PW_HIPS(hipMemcpyAsync(vinv + (size_t)iter * n, x, bytes, hipMemcpyDeviceToHost, s));
+            PW_HIPS(hipStreamSynchronize(s));
+            stored++;
+        }
+    } while (0);
+    hipStreamSynchronize(s);
+    cleanup();
+    if (nstored) *nstored = stored;
+    return rc;
+}
+
+// pfwicfun.forward_modeling_ac (pywave/src/pfwi.c:27-495); q is accepted and unused (acoustic)
+int pw_forward_modeling_ac(pw_ctx* ctx, const float* vel, const float* q, const float* wav,
+                           const float* pararray, float* out, long long out_len)
+{
+    (void)q;
+    if (!ctx || !vel || !wav || !pararray || !out) return PW_ERR_ARG;
+    Acq a;
+    std::string err;
+    if (acq_from_pararray(pararray, a, err)) PW_FAIL(ctx, PW_ERR_ARG, err);
+    if (out_len != (long long)a.nt * a.nx * a.ns) PW_FAIL(ctx, PW_ERR_ARG, "forward_modeling_ac: output length");
+    pw_fwi* p = pw_fwi_create(ctx, pararray, 0, 4);
+    if (!p) return PW_ERR_ARG;
+    int rc = pw_fwi_set_wavelet(p, wav);
+    if (!rc) rc = pw_fwi_forward(p, vel, out);
+    pw_fwi_destroy(p);
+    return rc;
+}
+
+// pfwicfun.fwic (pywave/src/pfwi.c:497-1028): returns vinv (niter * nz*nx); slots after an early
+// stop are zero (the reference leaves them uninitialised)
+int pw_fwic(pw_ctx* ctx, const float* vel, const float* q, const float* wav, const float* data,
+            const float* pararray, float* out, long long out_len)
+{
+    (void)q;
+    if (!ctx || !vel || !wav || !data || !pararray || !out) return PW_ERR_ARG;
+    Acq a;
+    std::string err;
+    if (acq_from_pararray(pararray, a, err)) PW_FAIL(ctx, PW_ERR_ARG, err);
+    if (out_len != (long long)a.nz * a.nx * a.niter) PW_FAIL(ctx, PW_ERR_ARG, "fwic: output length");
+    std::memset(out, 0, sizeof(float) * out_len);
+    pw_fwi* p = pw_fwi_create(ctx, pararray, 0, 4);
+    if (!p) return PW_ERR_ARG;
+    int rc = pw_fwi_set_wavelet(p, wav);
+    if (!rc) rc = pw_fwi_set_observed(p, data);
+    int k = 0;
+    if (!rc) rc = pw_fwi_lbfgs(p, vel, a.niter, out, &k);
+    pw_fwi_destroy(p);
+    return rc;
+}
+
+}  // extern "C"

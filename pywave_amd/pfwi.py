@@ -1,0 +1,89 @@
+"""pfwi / fillpar: drop-ins for pywave.pfwi (pywave/pfwi.py:5-301) on the MI355X engine.
+
+mode 1  forward modelling (pfwicfun.forward_modeling_ac, pywave/src/pfwi.c:27-495):
+        returns (data (nt, nx, ns), [], [], [], [])
+mode 2  active-source FWI (pfwicfun.fwic, pywave/src/pfwi.c:497-1028): L-BFGS over gradient_av;
+        returns (data flattened in Fortran order, vinv (nz, nx, niter), None, None, None)
+mode 3  the reference's RTM branch is empty and its return then fails (UnboundLocalError)
+mode 4  passive-source FWI / least-squares TRI: not part of this engine (NotImplementedError)
+other   prints "Wrong mode" and returns 0
+media != 1 in mode 1 reaches an undefined function in the reference (NameError); the same error
+is raised here.  Units follow the reference demos: km and km/s (the optimiser clips to [0, 10]).
+Slots of vinv after an early stop (line-search failure or convergence) are zero; the reference
+leaves them uninitialised.
+"""
+import numpy as np
+
+from . import _native as N
+from .fwi import make_pararray
+
+__all__ = ["pfwi", "fillpar"]
+
+
+def pfwi(vel, q, wav, src, data=None, mode=1, media=1, inv=0, verb=1, nb=60, coef=0.005, acq=1, ss=[0, 0.2, 1],
+         f0=10, waterz=-30, niter=30, conv_e=0.01, par=None):
+    par = fillpar(par)
+    if mode == 1:
+        vel = vel.flatten(order="F").astype(np.float32)
+        q = q.flatten(order="F").astype(np.float32)
+        wav = wav.flatten(order="F").astype(np.float32)
+        pararray = make_pararray(par)
+        if media != 1:
+            raise NameError("name 'forward_modelingc' is not defined")
+        nt, nx, ns = int(par["nt"]), int(par["nx"]), int(par["ns"])
+        out = np.empty(nt * nx * ns, np.float32)
+        L, ctx = N.lib(), N.context()
+        N.check(L.pw_forward_modeling_ac(ctx, N.ptr(vel), N.ptr(q), N.ptr(wav), N.ptr(pararray), N.ptr(out),
+                                         out.size), ctx, "forward_modeling_ac")
+        data = out.reshape(par["nt"], par["nx"], par["ns"], order="F")
+        vinv = []; grad = []; mwt = []; src = []
+    elif mode == 2:
+        vel = vel.flatten(order="F").astype(np.float32)
+        q = q.flatten(order="F").astype(np.float32)
+        wav = wav.flatten(order="F").astype(np.float32)
+        data = data.flatten(order="F").astype(np.float32)
+        pararray = make_pararray(par)
+        nz, nx, niter = int(par["nz"]), int(par["nx"]), int(par["niter"])
+        out = np.empty(nz * nx * niter, np.float32)
+        L, ctx = N.lib(), N.context()
+        N.check(L.pw_fwic(ctx, N.ptr(vel), N.ptr(q), N.ptr(wav), N.ptr(data), N.ptr(pararray), N.ptr(out),
+                          out.size), ctx, "fwic")
+        vinv = out.reshape(par["nz"], par["nx"], par["niter"], order="F")
+        src = None; mwt = None; grad = None
+    elif mode == 3:
+        raise UnboundLocalError("local variable 'vinv' referenced before assignment")
+    elif mode == 4:
+        raise NotImplementedError("pfwi mode 4 (passive-source FWI / least-squares TRI) is not provided by "
+                                  "this engine")
+    else:
+        print("Wrong mode")
+        return 0
+    return data, vinv, grad, src, mwt
+
+
+def fillpar(par):
+    """Fill missing keys with the reference's defaults (pywave/pfwi.py:262-301)."""
+    kw = {
+        "f0": 10,
+        "nb": 100,
+        "coef": 0.003,
+        "acqui_type": 1,
+        "interval": 1,
+        "ns": 1,
+        "ds": 0.2,
+        "os": 0.0,
+        "sz": 5,
+        "nx": 100,
+        "nz": 100,
+        "nr": par["nx"],
+        "dr": par["dx"],
+        "r0": par["ox"],
+        "fhi": 0.5 / par["dt"],
+        "flo": 0,
+        "onlysrc": True,
+        "onlyvel": False,
+        "inv": False,
+        "niter": 10,
+    }
+    kw.update(par)
+    return kw

@@ -1,0 +1,146 @@
+"""HIP FWI engine against the compiled reference's golden vectors and the mathematics.
+
+Bars: forward modelling and the reference imaging-condition gradient (one shot per launch) are
+bit-exact against the reference C; batched launches change only the order per-shot gradients
+are summed (rel-L2 <= 1e-5); the exact adjoint matches central finite differences in f64 to
+1e-3 relative (observed ~1e-7), and the autograd path matches the direct gradient."""
+import numpy as np
+import pytest
+
+import pywave_amd
+from conftest import rel_l2
+from pywave_amd import FwiProblem
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def small(request):
+    from conftest import load_golden
+    return load_golden("fwi_small")
+
+
+@pytest.mark.parametrize("batch", [1, 2, 3])
+def test_forward_bit_exact(small, batch):
+    p = FwiProblem(small["par"], small["wav"], batch=batch)
+    d = p.forward(small["vtrue"])
+    assert d.shape == small["data"].shape
+    assert np.array_equal(d, small["data"])
+
+
+def test_forward_marine_geometry(golden):
+    m = golden("fwi_marine")
+    p = FwiProblem(m["par"], m["wav"], batch=2)
+    assert np.array_equal(p.forward(m["vel"]), m["data"])
+
+
+@pytest.mark.parametrize("ckpt", [0, 7, 50, 130])
+def test_reference_gradient_bit_exact(small, ckpt):
+    p = FwiProblem(small["par"], small["wav"], data=small["data"], batch=1, checkpoint=ckpt)
+    g1, f1, sc = p.gradient(small["v0"], "reference")
+    assert np.array_equal(g1, small["grad1"])
+    assert abs(f1 - float(small["fcost"][0])) <= 1e-3 * abs(f1)  # reference accumulates in f32
+    # later calls reuse the first call's scaling (wave_fwigradient.c:245-252)
+    g2, f2, _ = p.gradient((small["v0"] * 1.01).astype(np.float32), "reference", scaling=sc)
+    assert np.array_equal(g2, small["grad2"])
+
+
+def test_reference_gradient_interval4(small):
+    p = FwiProblem(small["par4"], small["wav"], data=small["data"], batch=1, checkpoint=12)
+    g4, _, _ = p.gradient(small["v0"], "reference")
+    assert np.array_equal(g4, small["grad_int4"])
+
+
+def test_reference_gradient_batched(small):
+    p = FwiProblem(small["par"], small["wav"], data=small["data"], batch=3, checkpoint=40)
+    g, f, _ = p.gradient(small["v0"], "reference")
+    assert rel_l2(g, small["grad1"]) <= 1e-5
+
+
+def test_pfwi_dropin_modes(small):
+    par = dict(zip(["nz", "nx", "dz", "dx", "oz", "ox", "nt", "dt", "ot", "inv", "ns", "ds", "os", "sz", "nb",
+                    "coef", "f0", "acqui_type", "interval", "niter"], [float(x) for x in small["par"]]))
+    for k in ["nz", "nx", "nt", "ns", "nb", "acqui_type", "interval", "niter"]:
+        par[k] = int(par[k])
+    q = small["q"]
+    data, vinv, grad, src, mwt = pywave_amd.pfwi(small["vtrue"], q, small["wav"], None, mode=1, par=par)
+    assert np.array_equal(data, small["data"]) and vinv == [] and grad == []
+    d2, vinv, grad, src, mwt = pywave_amd.pfwi(small["v0"], q, small["wav"], None, data=small["data"], mode=2,
+                                               par=par)
+    assert grad is None and src is None and vinv.shape == (40, 60, 3)
+    ref = small["vinv"]
+    # f64 dot products in the optimiser: same trajectory to f32 round-off of the models
+    assert np.abs(vinv - ref).max() < 1e-4
+    with pytest.raises(NameError):
+        pywave_amd.pfwi(small["vtrue"], q, small["wav"], None, mode=1, media=2, par=par)
+
+
+def test_lbfgs_decreases_misfit(small):
+    p = FwiProblem(small["par"], small["wav"], data=small["data"], batch=3)
+    vinv, k = p.lbfgs(small["v0"], 3)
+    assert k == 3
+    f0 = p.misfit(small["v0"])
+    f3 = p.misfit(vinv[:, :, -1])
+    assert f3 < f0
+
+
+def _fd_case():
+    rng = np.random.default_rng(8)
+    nz, nx, nt = 24, 36, 260
+    vt = np.full((nz, nx), 2.0)
+    vt[10:] = 2.6
+    vt += 0.08 * rng.standard_normal((nz, nx))
+    v0 = vt.copy()
+    v0[8:16, 12:24] += 0.15
+    par = np.array([nz, nx, 0.01, 0.01, 0, 0, nt, 0.001, 0, 0, 2, 0.12, 0.06, 5, 12, 0.01, 12, 1, 1, 1],
+                   np.float32)
+    from oracle import oracle as O
+    wav = (1e6 * O.ricker(nt, 0.001, 14, 0.05, 1.0)).astype(np.float32)
+    return par, wav, vt.astype(np.float32), v0
+
+
+def test_exact_adjoint_matches_finite_differences_f64():
+    par, wav, vt, v0 = _fd_case()
+    obs = FwiProblem(par, wav).forward(vt)
+    p = FwiProblem(par, wav, data=obs, precision="f64", checkpoint=40)
+    g, f, _ = p.gradient(v0, "exact")
+    rng = np.random.default_rng(4)
+    cells = [(int(rng.integers(2, 22)), int(rng.integers(2, 34))) for _ in range(8)]
+    for iz, ix in cells:
+        h = 1e-4 * v0[iz, ix]
+        vp = v0.copy(); vp[iz, ix] += h
+        vm = v0.copy(); vm[iz, ix] -= h
+        fd = (p.misfit(vp) - p.misfit(vm)) / (2 * h)
+        assert abs(fd - g[iz, ix]) <= 1e-3 * max(abs(fd), 1e-3 * np.abs(g).max()), (iz, ix, fd, g[iz, ix])
+
+
+def test_exact_gradient_checkpointing_and_batching_consistent():
+    par, wav, vt, v0 = _fd_case()
+    obs = FwiProblem(par, wav).forward(vt)
+    v0 = v0.astype(np.float32)
+    ga, fa, _ = FwiProblem(par, wav, data=obs, batch=1, checkpoint=0).gradient(v0, "exact")
+    gb, fb, _ = FwiProblem(par, wav, data=obs, batch=1, checkpoint=17).gradient(v0, "exact")
+    assert np.array_equal(ga, gb) and fa == fb
+    gc, fc, _ = FwiProblem(par, wav, data=obs, batch=2, checkpoint=50).gradient(v0, "exact")
+    assert rel_l2(gc, ga) <= 1e-6 and abs(fc - fa) <= 1e-9 * abs(fa)
+    # the f32 gradient agrees with the f64 one to f32 accuracy
+    gd, _, _ = FwiProblem(par, wav, data=obs, precision="f64").gradient(v0.astype(np.float64), "exact")
+    assert rel_l2(ga, gd) <= 1e-4
+
+
+def test_autograd_matches_direct_gradient():
+    torch = pytest.importorskip("torch")
+    par, wav, vt, v0 = _fd_case()
+    p = FwiProblem(par, wav, batch=2, checkpoint=30)
+    obs = p.forward(vt)
+    p.set_observed(obs)
+    g_direct, f_direct, _ = p.gradient(v0.astype(np.float32), "exact")
+    dev = torch.device("cuda:0")
+    v = torch.tensor(v0.astype(np.float32), device=dev, requires_grad=True)
+    d = torch.tensor(np.ascontiguousarray(obs.transpose(2, 0, 1)), device=dev)  # (ns, nt, nr)
+    tr = pywave_amd.fwi_traces(v, p)
+    loss = 0.5 * ((tr.double() - d.double()) ** 2).sum()
+    loss.backward()
+    torch.cuda.synchronize()
+    assert abs(loss.item() - f_direct) <= 1e-6 * f_direct
+    assert rel_l2(v.grad.cpu().numpy(), g_direct) <= 1e-5

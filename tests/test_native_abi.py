@@ -57,7 +57,8 @@ def test_ricker_host_vs_reference(golden):
         w = np.zeros(nt, np.float32)
         assert L.pw_ricker(nt, dt, f0, t0, A, N.ptr(w)) == 0
         ref = ref[:nt]
-        assert np.abs(w - ref).max() <= 1e-6 * np.abs(ref).max()
+        # the f32 mixed-radix transform reproduces the reference's f32 FFT rounding
+        assert np.array_equal(w, ref)
         # independent f64 FFT evaluation of the same filter
         assert np.abs(w - O.ricker(nt, dt, f0, t0, A)).max() <= 1e-6 * np.abs(ref).max()
 
