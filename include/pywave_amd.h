@@ -133,6 +133,23 @@ int pw_forward_modeling_ac(pw_ctx *ctx, const float *vel, const float *q, const 
 int pw_fwic(pw_ctx *ctx, const float *vel, const float *q, const float *wav, const float *data,
             const float *pararray, float *out, long long out_len);
 
+/* ---------------------------------------------------------------- time-reversal imaging of groups
+ * Back-propagates ngroups (1..4) receiver groups of one data set together (one launch per step,
+ * the model read once for all groups) and accumulates the source-location image
+ * sum over snapshots (it % jsnap == 0) of prod_g u_g^2 in f64 -- the image the reference's demo
+ * forms from per-group afd3d(tri=True) runs (demos/test_third_tri.py:57-77). */
+typedef struct pw_tri pw_tri;
+pw_tri *pw_tri_create(pw_ctx *ctx, int nz, int nx, int ny, float dz, float dx, float dy, int nt,
+                      float dt, int nbt, float ct, int ngroups, int jsnap);
+int pw_tri_set_velocity(pw_tri *p, const float *vel);
+/* dat: (nt, nx, ny) Fortran order; masks: ngroups*nx*ny bytes [g][iy][ix], non-zero = member */
+int pw_tri_set_data(pw_tri *p, const float *dat, const unsigned char *masks);
+int pw_tri_run(pw_tri *p);
+int pw_tri_run_timed(pw_tri *p, double *total_ms);
+/* unnormalised f64 image, nz*nx*ny z fastest */
+int pw_tri_image(pw_tri *p, double *img, long long len);
+void pw_tri_destroy(pw_tri *p);
+
 #ifdef __cplusplus
 }
 #endif

@@ -5,6 +5,7 @@ There is no CPU fallback: every entry point raises if the library or a GPU is mi
 """
 import ctypes
 import os
+import sys
 import threading
 
 import numpy as np
@@ -65,11 +66,32 @@ SIGNATURES = {
     "pw_fwi_lbfgs": (c_int, [c_vp, c_vp, c_int, c_vp, ctypes.POINTER(c_int)]),
     "pw_forward_modeling_ac": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_ll]),
     "pw_fwic": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_ll]),
+    "pw_tri_create": (c_vp, [c_vp, c_int, c_int, c_int, c_float, c_float, c_float, c_int, c_float, c_int, c_float,
+                             c_int, c_int]),
+    "pw_tri_set_velocity": (c_int, [c_vp, c_vp]),
+    "pw_tri_set_data": (c_int, [c_vp, c_vp, c_vp]),
+    "pw_tri_run": (c_int, [c_vp]),
+    "pw_tri_run_timed": (c_int, [c_vp, ctypes.POINTER(c_double)]),
+    "pw_tri_image": (c_int, [c_vp, c_vp, c_ll]),
+    "pw_tri_destroy": (None, [c_vp]),
 }
 
 
 class PywaveError(RuntimeError):
     """A HIP-side failure reported by libpywave_amd."""
+
+
+def _bind_torch_runtime():
+    """One HIP runtime per process: torch ships its own libamdhip64 with the same soname as
+    ROCm's, and whichever is loaded first serves both.  When torch is installed it is imported
+    before this library loads, so the library runs on torch's runtime and torch keeps working
+    (PYWAVE_AMD_NO_TORCH=1 skips this, for processes that never import torch)."""
+    if "torch" in sys.modules or os.environ.get("PYWAVE_AMD_NO_TORCH") == "1":
+        return
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
 
 
 def lib():
@@ -78,6 +100,7 @@ def lib():
     if _lib is None:
         with _lock:
             if _lib is None:
+                _bind_torch_runtime()
                 if not os.path.exists(LIB_PATH):
                     raise ImportError("pywave_amd: %s is missing; build it with `make -C pywave_amd/csrc` "
                                       "(there is no CPU fallback)" % LIB_PATH)

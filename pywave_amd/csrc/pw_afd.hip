@@ -48,6 +48,9 @@ struct Step {
     const float* __restrict__ tri;  // this step's receiver plane [ny1][nx1] or null
     float* __restrict__ rec;        // this step's receiver plane or null
     float* __restrict__ snap;       // this snapshot [ny1][nx1][nz1] or null
+    const float* __restrict__ fz1;  // one damping factor per index (lean kernel; faces do not overlap)
+    const float* __restrict__ fx1;
+    const float* __restrict__ fy1;
     int ns, nt, it;
 };
 
@@ -205,6 +208,115 @@ __global__ __launch_bounds__(64 * WAVES) void k_afd3d(Geo g, Step a, int ntz, in
     }
 }
 
+// ---------------------------------------------------------------- 3D lean variant
+// Same arithmetic as k_afd3d with the per-cell overhead removed: the fields carry zero guard
+// planes around the padded grid and a 64-aligned z stride, so every load is unconditional and in
+// bounds; the rim rule is a select; damping uses one factor per axis (exact when the two faces of
+// an axis do not overlap, checked on the host); offsets are 32-bit.
+template <int RPW, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_afd3d_lean(Geo g, Step a, int ntz, int ntx, int nyc, int nblk)
+{
+    constexpr int TZ = 64, TX = RPW * WAVES, LW = TZ + 4;
+    __shared__ float lds[2][TX + 4][LW];
+    const int per = gridDim.x >> 3;
+    const int lb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (lb >= nblk) return;
+    const int tz = lb % ntz;
+    const int rest = lb / ntz;
+    const int tx = rest % ntx;
+    const int ch = rest / ntx;
+    const int z0 = tz * TZ, x0 = tx * TX;
+    const int y0 = ch * nyc, y1 = min(y0 + nyc, g.ny);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int z = z0 + lane;
+    const int ldz = g.ldz, ldxy = (int)g.ldxy;
+    bool blk_src = false;
+    for (int s = 0; s < a.ns; ++s) {
+        int gz = a.src[3 * s], gx = a.src[3 * s + 1], gy = a.src[3 * s + 2];
+        blk_src |= gz >= z0 - 1 && gz <= z0 + TZ && gx >= x0 - 1 && gx <= x0 + TX && gy >= y0 - 1 && gy <= y1;
+    }
+    const float fz = a.fz1[z];
+    const bool zin = z >= 2 && z < g.nz2 - 2;
+    const bool zact = z < g.nz;
+    int xo[RPW];
+    float fx[RPW];
+    bool xin[RPW], act[RPW];
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+        const int x = x0 + w * RPW + r;
+        xo[r] = x * ldz + z;
+        fx[r] = a.fx1[x];
+        xin[r] = x >= 2 && x < g.nx2 - 2;
+        act[r] = zact && x < g.nx;
+    }
+    const float* __restrict__ cur = a.cur;
+    float q[RPW][5];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) q[r][k] = cur[(y0 - 2 + k) * ldxy + xo[r]];
+    int buf = 0;
+    for (int y = y0; y < y1; ++y) {
+        const int po = y * ldxy;
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) q[r][4] = cur[po + 2 * ldxy + xo[r]];
+        float pr[RPW], vc[RPW];
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) {
+            pr[r] = a.pv[po + xo[r]];
+            vc[r] = a.vv[po + xo[r]];
+        }
+        float(*L)[LW] = lds[buf];
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) L[w * RPW + r + 2][lane + 2] = q[r][2];
+        if (lane < 4) {
+            const int zz = lane < 2 ? z0 - 2 + lane : z0 + TZ - 2 + lane;
+            const int lz = lane < 2 ? lane : TZ + lane;
+#pragma unroll
+            for (int r = 0; r < RPW; ++r) L[w * RPW + r + 2][lz] = cur[po + xo[r] - z + zz];
+        }
+        for (int h = w; h < 4; h += WAVES) {
+            const int xx = h < 2 ? x0 - 2 + h : x0 + TX - 2 + h;
+            const int lrow = h < 2 ? h : TX + h;
+            L[lrow][lane + 2] = cur[po + xx * ldz + z];
+        }
+        __syncthreads();
+        const float fyv = a.fy1[y];
+        const bool yin = y >= 2 && y < g.ny2 - 2;
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) {
+            const int lr = w * RPW + r + 2;
+            const float u = q[r][2];
+            float lap = C0 * u * g.cc
+                      + C1 * (q[r][1] + q[r][3]) * g.idy
+                      + C2 * (q[r][0] + q[r][4]) * g.idy
+                      + C1 * (L[lr - 1][lane + 2] + L[lr + 1][lane + 2]) * g.idx
+                      + C2 * (L[lr - 2][lane + 2] + L[lr + 2][lane + 2]) * g.idx
+                      + C1 * (L[lr][lane + 1] + L[lr][lane + 3]) * g.idz
+                      + C2 * (L[lr][lane] + L[lr][lane + 4]) * g.idz;
+            lap = (yin && zin && xin[r]) ? lap : 0.f;
+            const float p = pr[r] * fz * fx[r] * fyv;
+            float c = u + (u - p);
+            c = c + lap * vc[r];
+            const int x = x0 + w * RPW + r;
+            if (blk_src) c = inject(c, vc[r], a, z, x, y);
+            const bool onrec = z == g.nb && x >= g.nb && x < g.nb + g.nx1 && y >= g.nb && y < g.nb + g.ny1;
+            if (a.tri && onrec) c = c + vc[r] * a.tri[(y - g.nb) * g.nx1 + (x - g.nb)];
+            c = c * fz * fx[r] * fyv;
+            if (act[r]) a.pv[po + xo[r]] = c;
+            if (a.rec && onrec) a.rec[(y - g.nb) * g.nx1 + (x - g.nb)] = c;
+            if (a.snap && act[r] && z >= g.nb && z < g.nb + g.nz1 && x >= g.nb && x < g.nb + g.nx1 && y >= g.nb &&
+                y < g.nb + g.ny1)
+                a.snap[((size_t)(y - g.nb) * g.nx1 + (x - g.nb)) * g.nz1 + (z - g.nb)] = c;
+        }
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) {
+            q[r][0] = q[r][1]; q[r][1] = q[r][2]; q[r][2] = q[r][3]; q[r][3] = q[r][4];
+        }
+        buf ^= 1;
+    }
+}
+
 // ---------------------------------------------------------------- 2D: one cell per lane
 template <int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void k_afd2d(Geo g, Step a)
@@ -284,11 +396,157 @@ __global__ void k_interior(Geo g, const float* __restrict__ f, float* __restrict
         f[(long long)(y + nby) * g.ldxy + (long long)(x + g.nb) * g.ldz + z + g.nb];
 }
 
-struct Variant3 {
-    int tx, waves;
+// ---------------------------------------------------------------- 3D time reversal of G groups
+// G receiver groups back-propagated together (the caller's loop over groups in
+// demos/test_third_tri.py:57-77 with afd3d(tri=True) per group): one launch per step reads vv
+// once for all G fields; at snapshot steps (it % jsnap == 0) the source-location image
+// sum_snap prod_g u_g^2 is accumulated in f64 (product formed in f32 as ((u0*u0)*u1)*u1 ...).
+template <int G>
+struct TriStep {
+    const float* cur[G];
+    float* pv[G];
+    const float* __restrict__ vv;
+    const float2* __restrict__ fz;
+    const float2* __restrict__ fx;
+    const float2* __restrict__ fy;
+    const float* __restrict__ dat;          // this step's receiver plane [ny1][nx1]
+    const unsigned char* __restrict__ mask;  // [G][ny1][nx1]: receiver belongs to group
+    double* __restrict__ img;                // [ny1][nx1][nz1] or null (no image this step)
 };
-const Variant3 kVariants3[] = {{16, 4}, {8, 4}, {16, 8}, {32, 8}, {8, 8}, {4, 4}};
+
+template <int G>
+__global__ __launch_bounds__(256) void k_tri3d(Geo g, TriStep<G> a, int ntz, int ntx, int nyc, int nblk)
+{
+    constexpr int TX = 16, WAVES = 4, TZ = 64, RPW = TX / WAVES, LW = TZ + 4;
+    __shared__ float lds[G][2][TX + 4][LW];
+    const int per = gridDim.x >> 3;
+    const int lb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (lb >= nblk) return;
+    const int tz = lb % ntz;
+    const int rest = lb / ntz;
+    const int tx = rest % ntx;
+    const int ch = rest / ntx;
+    const int z0 = tz * TZ, x0 = tx * TX;
+    const int y0 = ch * nyc, y1 = min(y0 + nyc, g.ny);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int z = z0 + lane;
+    const bool zok = z < g.nz2;
+    int xoff[RPW];
+    bool xok[RPW], act[RPW];
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+        int x = x0 + w * RPW + r;
+        xok[r] = x < g.nx2;
+        act[r] = (z < g.nz) && (x < g.nx);
+        xoff[r] = x * g.ldz;
+    }
+    float q[G][RPW][5];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        int yy = y0 - 2 + k;
+        bool yok = yy >= 0 && yy < g.ny2;
+        long long po = (long long)(yok ? yy : 0) * g.ldxy;
+#pragma unroll
+        for (int f = 0; f < G; ++f)
+#pragma unroll
+            for (int r = 0; r < RPW; ++r) q[f][r][k] = (yok && zok && xok[r]) ? a.cur[f][po + xoff[r] + z] : 0.f;
+    }
+    int buf = 0;
+    for (int y = y0; y < y1; ++y) {
+        const long long poff = (long long)y * g.ldxy;
+        {
+            int yy = y + 2;
+            bool yok = yy < g.ny2;
+            long long po = (long long)(yok ? yy : 0) * g.ldxy;
+#pragma unroll
+            for (int f = 0; f < G; ++f)
+#pragma unroll
+                for (int r = 0; r < RPW; ++r) q[f][r][4] = (yok && zok && xok[r]) ? a.cur[f][po + xoff[r] + z] : 0.f;
+        }
+        float vc[RPW];
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) vc[r] = act[r] ? a.vv[poff + xoff[r] + z] : 0.f;
+#pragma unroll
+        for (int f = 0; f < G; ++f) {
+            float(*L)[LW] = lds[f][buf];
+            const float* Pc = a.cur[f] + poff;
+#pragma unroll
+            for (int r = 0; r < RPW; ++r) L[w * RPW + r + 2][lane + 2] = q[f][r][2];
+            if (lane < 4) {
+                int zz = lane < 2 ? z0 - 2 + lane : z0 + TZ + lane - 2;
+                int lz = lane < 2 ? lane : TZ + lane;
+                bool zzok = zz >= 0 && zz < g.nz2;
+#pragma unroll
+                for (int r = 0; r < RPW; ++r) L[w * RPW + r + 2][lz] = (zzok && xok[r]) ? Pc[xoff[r] + zz] : 0.f;
+            }
+            {
+                const int h = w;  // WAVES == 4: one halo row per wave
+                int xx = h < 2 ? x0 - 2 + h : x0 + TX + h - 2;
+                int lrow = h < 2 ? h : TX + h;
+                L[lrow][lane + 2] = (zok && xx >= 0 && xx < g.nx2) ? Pc[xx * g.ldz + z] : 0.f;
+            }
+        }
+        __syncthreads();
+        const float2 fyy = a.fy[y];
+        const bool yin = y >= 2 && y < g.ny2 - 2;
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) {
+            if (!act[r]) continue;
+            const int x = x0 + w * RPW + r;
+            const int lr = w * RPW + r + 2;
+            const float2 fzz = a.fz[z], fxx = a.fx[x];
+            const bool inner = yin && z >= 2 && z < g.nz2 - 2 && x >= 2 && x < g.nx2 - 2;
+            const bool onrec = z == g.nb && x >= g.nb && x < g.nb + g.nx1 && y >= g.nb && y < g.nb + g.ny1;
+            const bool inside = z >= g.nb && z < g.nb + g.nz1 && x >= g.nb && x < g.nb + g.nx1 && y >= g.nb &&
+                                y < g.nb + g.ny1;
+            const int rk = onrec ? (y - g.nb) * g.nx1 + (x - g.nb) : 0;
+            const float dv = onrec ? a.dat[rk] : 0.f;
+            float prod = 1.f;
+#pragma unroll
+            for (int f = 0; f < G; ++f) {
+                const float(*L)[LW] = lds[f][buf];
+                const float u = q[f][r][2];
+                float lap = 0.f;
+                if (inner) {
+                    lap = C0 * u * g.cc
+                        + C1 * (q[f][r][1] + q[f][r][3]) * g.idy
+                        + C2 * (q[f][r][0] + q[f][r][4]) * g.idy
+                        + C1 * (L[lr - 1][lane + 2] + L[lr + 1][lane + 2]) * g.idx
+                        + C2 * (L[lr - 2][lane + 2] + L[lr + 2][lane + 2]) * g.idx
+                        + C1 * (L[lr][lane + 1] + L[lr][lane + 3]) * g.idz
+                        + C2 * (L[lr][lane] + L[lr][lane + 4]) * g.idz;
+                }
+                const float p = damp3(a.pv[f][poff + xoff[r] + z], fzz, fxx, fyy);
+                float c = u + (u - p);
+                c = c + lap * vc[r];
+                if (onrec && a.mask[(size_t)f * g.nx1 * g.ny1 + rk]) c = c + vc[r] * dv;
+                c = damp3(c, fzz, fxx, fyy);
+                a.pv[f][poff + xoff[r] + z] = c;
+                prod = f == 0 ? c * c : prod * c * c;
+            }
+            if (a.img && inside)
+                a.img[((size_t)(y - g.nb) * g.nx1 + (x - g.nb)) * g.nz1 + (z - g.nb)] += (double)prod;
+        }
+#pragma unroll
+        for (int f = 0; f < G; ++f)
+#pragma unroll
+            for (int r = 0; r < RPW; ++r) {
+                q[f][r][0] = q[f][r][1]; q[f][r][1] = q[f][r][2]; q[f][r][2] = q[f][r][3]; q[f][r][3] = q[f][r][4];
+            }
+        buf ^= 1;
+    }
+}
+
+// 3D step kernels: 0-5 general (bounds-checked), 6-11 lean (guarded layout, single damping
+// factor per axis).  tx = rows per block, waves per block; lean rows per thread = tx / waves.
+struct Variant3 {
+    int tx, waves, lean;
+};
+const Variant3 kVariants3[] = {{16, 4, 0}, {8, 4, 0}, {16, 8, 0}, {32, 8, 0}, {8, 8, 0}, {4, 4, 0},
+                               {16, 8, 1}, {8, 8, 1}, {16, 4, 1}, {32, 8, 1}, {16, 16, 1}, {32, 16, 1}};
 constexpr int kNumVariants3 = sizeof(kVariants3) / sizeof(kVariants3[0]);
+constexpr int kLeanDefault = 6;
+constexpr int kGeneralDefault = 2;
 
 }  // namespace
 
@@ -311,11 +569,17 @@ struct pw_afd {
     float* d_rec = nullptr;   // [nt][ny1][nx1]
     float* d_snap = nullptr;  // [ntsnap][ny1][nx1][nz1]
     float* d_tmp = nullptr;   // transpose scratch (nrec*nt)
+    float* raw[3] = {nullptr, nullptr, nullptr};  // allocations behind d_a, d_b, d_vv (with guards)
+    size_t raw_elems = 0, guard_lo = 0;
+    float* d_fz1 = nullptr;  // single damping factor per index (lean kernels)
+    float* d_fx1 = nullptr;
+    float* d_fy1 = nullptr;
+    bool lean_ok = false;
     std::vector<float> tri_host;  // echoed back as the trace part in TRI mode
     bool have_vel = false, have_src = false, have_tri = false;
     int last_mode = -1;  // 0 modelling, 1 tri
     const float* final_field = nullptr;
-    int variant = 0, ny_chunk = 0;
+    int variant = -1, ny_chunk = 0;
     size_t field_elems = 0;
 };
 
@@ -331,7 +595,9 @@ int afd_launch_step(pw_afd* p, const Step& a)
         dim3 grid(pw::ceil_div(g.nz, 64), pw::ceil_div(g.nx, W));
         hipLaunchKernelGGL(k_afd2d<W>, grid, dim3(64 * W), 0, s, g, a);
     } else {
-        const Variant3 v = kVariants3[p->variant];
+        const int vi = p->variant >= 0 ? p->variant : (p->lean_ok ? kLeanDefault : kGeneralDefault);
+        const Variant3 v = kVariants3[vi];
+        if (v.lean && !p->lean_ok) PW_FAIL(ctx, PW_ERR_ARG, "lean 3D kernel not applicable to this grid");
         const int ntz = pw::ceil_div(g.nz, 64), ntx = pw::ceil_div(g.nx, v.tx);
         int nyc = p->ny_chunk;
         if (nyc <= 0) {
@@ -343,7 +609,7 @@ int afd_launch_step(pw_afd* p, const Step& a)
         const int nch = pw::ceil_div(g.ny, nyc);
         const int nblk = ntz * ntx * nch;
         const int grid = pw::ceil_div(nblk, 8) * 8;
-        switch (p->variant) {
+        switch (vi) {
 #define PW_V3(I, TX_, W_)                                                                          \
     case I:                                                                                        \
         hipLaunchKernelGGL((k_afd3d<TX_, W_>), dim3(grid), dim3(64 * W_), 0, s, g, a, ntz, ntx, nyc, \
@@ -356,6 +622,18 @@ int afd_launch_step(pw_afd* p, const Step& a)
             PW_V3(4, 8, 8)
             PW_V3(5, 4, 4)
 #undef PW_V3
+#define PW_VL(I, TX_, W_)                                                                          \
+    case I:                                                                                        \
+        hipLaunchKernelGGL((k_afd3d_lean<TX_ / W_, W_>), dim3(grid), dim3(64 * W_), 0, s, g, a, ntz, ntx, \
+                           nyc, nblk);                                                             \
+        break;
+            PW_VL(6, 16, 8)
+            PW_VL(7, 8, 8)
+            PW_VL(8, 16, 4)
+            PW_VL(9, 32, 8)
+            PW_VL(10, 16, 16)
+            PW_VL(11, 32, 16)
+#undef PW_VL
             default:
                 PW_FAIL(ctx, PW_ERR_ARG, "unknown 3D kernel variant");
         }
@@ -366,7 +644,8 @@ int afd_launch_step(pw_afd* p, const Step& a)
 
 void afd_free(pw_afd* p)
 {
-    float* bufs[] = {p->d_a, p->d_b, p->d_vv, p->d_wav, p->d_tri, p->d_rec, p->d_snap, p->d_tmp};
+    float* bufs[] = {p->raw[0], p->raw[1], p->raw[2], p->d_wav, p->d_tri, p->d_rec, p->d_snap, p->d_tmp,
+                     p->d_fz1, p->d_fx1, p->d_fy1};
     for (float* b : bufs)
         if (b) hipFree(b);
     if (p->d_fz) hipFree(p->d_fz);
@@ -418,13 +697,20 @@ pw_afd* pw_afd_create(pw_ctx* ctx, int ndim, int nz, int nx, int ny, float dz, f
     g.nz = nz + 2 * nbt; g.nx = nx + 2 * nbt; g.ny = ndim == 3 ? ny + 2 * nbt : 1;
     pw::padded_sizes(g.nz, g.nx, g.ny, &g.nz2, &g.nx2, &g.ny2);
     if (ndim == 2) g.ny2 = 1;
-    g.ldz = pw::ceil_div(g.nz2, LDZ_ALIGN) * LDZ_ALIGN;
+    // 3D: z stride a multiple of 64 so a 64-lane z tile never leaves its row
+    g.ldz = pw::ceil_div(g.nz2, ndim == 3 ? 64 : LDZ_ALIGN) * (ndim == 3 ? 64 : LDZ_ALIGN);
     g.ldxy = (long long)g.ldz * g.nx2;
     g.idz = 1 / (dz * dz);
     g.idx = 1 / (dx * dx);
     g.idy = ndim == 3 ? 1 / (dy * dy) : 0.f;
     g.cc = ndim == 3 ? g.idy + g.idx + g.idz : g.idx + g.idz;
     p->field_elems = (size_t)g.ldxy * g.ny2;
+    // 3D fields carry zero guard planes (2 before, 3 plus a tile of rows after) so the lean
+    // kernels can load the radius-2 neighbourhood of any tile cell unconditionally
+    constexpr int kMaxTx = 32;
+    p->guard_lo = ndim == 3 ? (size_t)(2 * g.ldxy + 128) : 0;
+    const size_t guard_hi = ndim == 3 ? (size_t)(3 * g.ldxy + (kMaxTx + 4) * g.ldz + 128) : 0;
+    p->raw_elems = p->field_elems + p->guard_lo + guard_hi;
 
     // damping pairs per axis; all six faces use nbt and ct (pywave/src/afd.c:101-116,392-402)
     std::vector<float> w(std::max(nbt, 1));
@@ -434,16 +720,41 @@ pw_afd* pw_afd_create(pw_ctx* ctx, int ndim, int nz, int nx, int ny, float dz, f
     pw::damping_pairs(g.nx2, nbt, nbt, w.data(), w.data(), fx);
     if (ndim == 3) pw::damping_pairs(g.ny2, nbt, nbt, w.data(), w.data(), fy);
     else fy.assign(2, 1.0f);
+    // single factor per index when the lo and hi faces of an axis never both apply
+    auto single = [](const std::vector<float>& pr, size_t len, std::vector<float>& out) -> bool {
+        out.assign(len, 1.0f);
+        for (size_t i = 0; i < pr.size() / 2; ++i) {
+            float lo = pr[2 * i], hi = pr[2 * i + 1];
+            if (lo != 1.0f && hi != 1.0f) return false;
+            out[i] = lo != 1.0f ? lo : hi;
+        }
+        return true;
+    };
+    std::vector<float> fz1, fx1, fy1;
+    bool lean = ndim == 3;
+    lean = lean && single(fz, (size_t)g.ldz, fz1);
+    lean = lean && single(fx, (size_t)g.nx2 + kMaxTx + 8, fx1);
+    lean = lean && single(fy, (size_t)g.ny2 + 8, fy1);
+    lean = lean && (double)p->raw_elems < 2.0e9;  // 32-bit offsets
+    p->lean_ok = lean;
 
     const size_t nrec = (size_t)g.nx1 * g.ny1;
-    bool ok = hipMalloc(&p->d_a, p->field_elems * 4) == hipSuccess &&
-              hipMalloc(&p->d_b, p->field_elems * 4) == hipSuccess &&
-              hipMalloc(&p->d_vv, p->field_elems * 4) == hipSuccess &&
+    bool ok = hipMalloc(&p->raw[0], p->raw_elems * 4) == hipSuccess &&
+              hipMalloc(&p->raw[1], p->raw_elems * 4) == hipSuccess &&
+              hipMalloc(&p->raw[2], p->raw_elems * 4) == hipSuccess &&
               hipMalloc(&p->d_fz, fz.size() * 4) == hipSuccess &&
               hipMalloc(&p->d_fx, fx.size() * 4) == hipSuccess &&
               hipMalloc(&p->d_fy, fy.size() * 4) == hipSuccess &&
               hipMalloc(&p->d_rec, nrec * nt * 4) == hipSuccess &&
               hipMalloc(&p->d_tmp, nrec * nt * 4) == hipSuccess;
+    if (ok && lean)
+        ok = hipMalloc(&p->d_fz1, fz1.size() * 4) == hipSuccess && hipMalloc(&p->d_fx1, fx1.size() * 4) == hipSuccess &&
+             hipMalloc(&p->d_fy1, fy1.size() * 4) == hipSuccess;
+    if (ok) {
+        p->d_a = p->raw[0] + p->guard_lo;
+        p->d_b = p->raw[1] + p->guard_lo;
+        p->d_vv = p->raw[2] + p->guard_lo;
+    }
     if (ok && p->ntsnap > 0)
         ok = hipMalloc(&p->d_snap, (size_t)p->ntsnap * g.nz1 * g.nx1 * g.ny1 * 4) == hipSuccess;
     if (!ok) {
@@ -456,7 +767,10 @@ pw_afd* pw_afd_create(pw_ctx* ctx, int ndim, int nz, int nx, int ny, float dz, f
     ok = hipMemcpyAsync(p->d_fz, fz.data(), fz.size() * 4, hipMemcpyHostToDevice, s) == hipSuccess &&
          hipMemcpyAsync(p->d_fx, fx.data(), fx.size() * 4, hipMemcpyHostToDevice, s) == hipSuccess &&
          hipMemcpyAsync(p->d_fy, fy.data(), fy.size() * 4, hipMemcpyHostToDevice, s) == hipSuccess &&
-         hipMemsetAsync(p->d_vv, 0, p->field_elems * 4, s) == hipSuccess &&
+         hipMemsetAsync(p->raw[2], 0, p->raw_elems * 4, s) == hipSuccess &&
+         (!lean || (hipMemcpyAsync(p->d_fz1, fz1.data(), fz1.size() * 4, hipMemcpyHostToDevice, s) == hipSuccess &&
+                    hipMemcpyAsync(p->d_fx1, fx1.data(), fx1.size() * 4, hipMemcpyHostToDevice, s) == hipSuccess &&
+                    hipMemcpyAsync(p->d_fy1, fy1.data(), fy1.size() * 4, hipMemcpyHostToDevice, s) == hipSuccess)) &&
          hipStreamSynchronize(s) == hipSuccess;
     if (!ok) {
         ctx->err = "pw_afd_create: upload failed";
@@ -570,8 +884,8 @@ static int afd_enqueue(pw_afd* p, int tri, hipEvent_t* evs)
     if (tri && !p->have_tri) PW_FAIL(ctx, PW_ERR_STATE, "run: time-reversal data not set");
     if (!tri && !p->have_src) PW_FAIL(ctx, PW_ERR_STATE, "run: sources not set");
     hipStream_t s = ctx->s();
-    PW_HIP(ctx, hipMemsetAsync(p->d_a, 0, p->field_elems * 4, s));
-    PW_HIP(ctx, hipMemsetAsync(p->d_b, 0, p->field_elems * 4, s));
+    PW_HIP(ctx, hipMemsetAsync(p->raw[0], 0, p->raw_elems * 4, s));
+    PW_HIP(ctx, hipMemsetAsync(p->raw[1], 0, p->raw_elems * 4, s));
     const size_t nrec = (size_t)g.nx1 * g.ny1;
     const size_t snap_elems = (size_t)g.nz1 * g.nx1 * g.ny1;
     float* A = p->d_a;
@@ -582,6 +896,7 @@ static int afd_enqueue(pw_afd* p, int tri, hipEvent_t* evs)
         Step a{};
         a.cur = A; a.pv = B; a.vv = p->d_vv;
         a.fz = p->d_fz; a.fx = p->d_fx; a.fy = p->d_fy;
+        a.fz1 = p->d_fz1; a.fx1 = p->d_fx1; a.fy1 = p->d_fy1;
         a.ns = tri ? 0 : p->ns;
         a.src = p->d_src; a.wav = p->d_wav; a.nt = p->nt; a.it = it;
         a.tri = tri ? p->d_tri + (size_t)it * nrec : nullptr;
@@ -731,6 +1046,229 @@ int pw_afd3d(pw_ctx* ctx, const float* vel, const float* src, int tri, int nt, i
     (void)verb; (void)ifsnaps; (void)abc; (void)ox; (void)oy; (void)oz;
     return afd_oneshot(ctx, 3, vel, src, tri, nt, nx, ny, nz, ns, jsnap, nbt, ct, dt, dx, dy, dz, out,
                        out_len);
+}
+
+}  // extern "C"
+
+// ==================================================================================== TRI plans
+struct pw_tri {
+    pw_ctx* ctx = nullptr;
+    Geo g{};
+    int nt = 0, G = 0, jsnap = 0;
+    float dt = 0.f;
+    size_t field_elems = 0;
+    float* d_f[8] = {nullptr};  // 2*G fields
+    float* d_vv = nullptr;
+    float2 *d_fz = nullptr, *d_fx = nullptr, *d_fy = nullptr;
+    float* d_dat = nullptr;  // [nt][ny1][nx1]
+    float* d_tmp = nullptr;
+    unsigned char* d_mask = nullptr;
+    double* d_img = nullptr;
+    bool have_vel = false, have_dat = false, ran = false;
+};
+
+namespace {
+
+void tri_free(pw_tri* p)
+{
+    for (int i = 0; i < 8; ++i)
+        if (p->d_f[i]) hipFree(p->d_f[i]);
+    void* b[] = {p->d_vv, p->d_fz, p->d_fx, p->d_fy, p->d_dat, p->d_tmp, p->d_mask, p->d_img};
+    for (void* x : b)
+        if (x) hipFree(x);
+}
+
+template <int G>
+void tri_launch(pw_tri* p, const TriStep<G>& a, hipStream_t s)
+{
+    const Geo& g = p->g;
+    const int ntz = pw::ceil_div(g.nz, 64), ntx = pw::ceil_div(g.nx, 16);
+    long long tiles = (long long)ntz * ntx;
+    int nch = (int)std::max(1LL, (6 * 256 + tiles - 1) / tiles);
+    int nyc = std::max(16, pw::ceil_div(g.ny, nch));
+    nch = pw::ceil_div(g.ny, nyc);
+    const int nblk = ntz * ntx * nch;
+    const int grid = pw::ceil_div(nblk, 8) * 8;
+    hipLaunchKernelGGL(k_tri3d<G>, dim3(grid), dim3(256), 0, s, g, a, ntz, ntx, nyc, nblk);
+}
+
+template <int G>
+int tri_enqueue(pw_tri* p)
+{
+    pw_ctx* ctx = p->ctx;
+    hipStream_t s = ctx->s();
+    const Geo& g = p->g;
+    const size_t nrec = (size_t)g.nx1 * g.ny1;
+    for (int i = 0; i < 2 * G; ++i) PW_HIP(ctx, hipMemsetAsync(p->d_f[i], 0, p->field_elems * 4, s));
+    PW_HIP(ctx, hipMemsetAsync(p->d_img, 0, (size_t)g.nz1 * nrec * 8, s));
+    float* A[G];
+    float* B[G];
+    for (int f = 0; f < G; ++f) { A[f] = p->d_f[2 * f]; B[f] = p->d_f[2 * f + 1]; }
+    for (int it = p->nt - 1; it >= 0; --it) {
+        TriStep<G> a{};
+        for (int f = 0; f < G; ++f) { a.cur[f] = A[f]; a.pv[f] = B[f]; }
+        a.vv = p->d_vv; a.fz = p->d_fz; a.fx = p->d_fx; a.fy = p->d_fy;
+        a.dat = p->d_dat + (size_t)it * nrec;
+        a.mask = p->d_mask;
+        a.img = (p->jsnap > 0 && it % p->jsnap == 0) ? p->d_img : nullptr;
+        tri_launch<G>(p, a, s);
+        PW_HIP(ctx, hipGetLastError());
+        for (int f = 0; f < G; ++f) std::swap(A[f], B[f]);
+    }
+    p->ran = true;
+    return PW_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+pw_tri* pw_tri_create(pw_ctx* ctx, int nz, int nx, int ny, float dz, float dx, float dy, int nt, float dt,
+                      int nbt, float ct, int ngroups, int jsnap)
+{
+    if (!ctx) return nullptr;
+    if (nz < 1 || nx < 1 || ny < 1 || nt < 1 || nbt < 0 || jsnap < 1 || ngroups < 1 || ngroups > 4 ||
+        !(dz > 0.f) || !(dx > 0.f) || !(dy > 0.f) || !(dt > 0.f)) {
+        ctx->err = "pw_tri_create: invalid arguments (1 <= ngroups <= 4, jsnap >= 1)";
+        return nullptr;
+    }
+    if (hipSetDevice(ctx->device) != hipSuccess) { ctx->err = "hipSetDevice failed"; return nullptr; }
+    pw_tri* p = new pw_tri;
+    p->ctx = ctx;
+    p->nt = nt; p->dt = dt; p->G = ngroups; p->jsnap = jsnap;
+    Geo& g = p->g;
+    g.nz1 = nz; g.nx1 = nx; g.ny1 = ny; g.nb = nbt;
+    g.nz = nz + 2 * nbt; g.nx = nx + 2 * nbt; g.ny = ny + 2 * nbt;
+    pw::padded_sizes(g.nz, g.nx, g.ny, &g.nz2, &g.nx2, &g.ny2);
+    g.ldz = pw::ceil_div(g.nz2, LDZ_ALIGN) * LDZ_ALIGN;
+    g.ldxy = (long long)g.ldz * g.nx2;
+    g.idz = 1 / (dz * dz); g.idx = 1 / (dx * dx); g.idy = 1 / (dy * dy);
+    g.cc = g.idy + g.idx + g.idz;
+    p->field_elems = (size_t)g.ldxy * g.ny2;
+    std::vector<float> w(std::max(nbt, 1));
+    pw::abc_weights(nbt, ct, w.data());
+    std::vector<float> fz, fx, fy;
+    pw::damping_pairs(g.nz2, nbt, nbt, w.data(), w.data(), fz);
+    pw::damping_pairs(g.nx2, nbt, nbt, w.data(), w.data(), fx);
+    pw::damping_pairs(g.ny2, nbt, nbt, w.data(), w.data(), fy);
+    const size_t nrec = (size_t)nx * ny;
+    bool ok = true;
+    for (int i = 0; i < 2 * ngroups && ok; ++i) ok = hipMalloc(&p->d_f[i], p->field_elems * 4) == hipSuccess;
+    ok = ok && hipMalloc(&p->d_vv, p->field_elems * 4) == hipSuccess &&
+         hipMalloc(&p->d_fz, fz.size() * 4) == hipSuccess && hipMalloc(&p->d_fx, fx.size() * 4) == hipSuccess &&
+         hipMalloc(&p->d_fy, fy.size() * 4) == hipSuccess &&
+         hipMalloc(&p->d_dat, nrec * nt * 4) == hipSuccess && hipMalloc(&p->d_tmp, nrec * nt * 4) == hipSuccess &&
+         hipMalloc(&p->d_mask, nrec * ngroups) == hipSuccess &&
+         hipMalloc(&p->d_img, (size_t)nz * nrec * 8) == hipSuccess;
+    hipStream_t s = ctx->s();
+    ok = ok && hipMemcpyAsync(p->d_fz, fz.data(), fz.size() * 4, hipMemcpyHostToDevice, s) == hipSuccess &&
+         hipMemcpyAsync(p->d_fx, fx.data(), fx.size() * 4, hipMemcpyHostToDevice, s) == hipSuccess &&
+         hipMemcpyAsync(p->d_fy, fy.data(), fy.size() * 4, hipMemcpyHostToDevice, s) == hipSuccess &&
+         hipMemsetAsync(p->d_vv, 0, p->field_elems * 4, s) == hipSuccess && hipStreamSynchronize(s) == hipSuccess;
+    if (!ok) {
+        ctx->err = "pw_tri_create: device allocation/upload failed";
+        tri_free(p);
+        delete p;
+        return nullptr;
+    }
+    return p;
+}
+
+void pw_tri_destroy(pw_tri* p)
+{
+    if (!p) return;
+    hipSetDevice(p->ctx->device);
+    hipStreamSynchronize(p->ctx->s());
+    tri_free(p);
+    delete p;
+}
+
+int pw_tri_set_velocity(pw_tri* p, const float* vel)
+{
+    if (!p || !vel) return PW_ERR_ARG;
+    pw_ctx* ctx = p->ctx;
+    const Geo& g = p->g;
+    size_t n = (size_t)g.nz1 * g.nx1 * g.ny1;
+    float* d_vel = nullptr;
+    PW_HIP(ctx, hipMalloc(&d_vel, n * 4));
+    hipStream_t s = ctx->s();
+    hipError_t e = hipMemcpyAsync(d_vel, vel, n * 4, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_expand_vv, dim3(pw::ceil_div(g.nz, 256), g.nx, g.ny), dim3(256), 0, s, g, d_vel,
+                           p->d_vv, p->dt);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    hipFree(d_vel);
+    if (e != hipSuccess) PW_FAIL(ctx, PW_ERR_HIP, std::string("tri set_velocity: ") + hipGetErrorString(e));
+    p->have_vel = true;
+    return PW_OK;
+}
+
+// dat: nt*nx*ny receiver data, time fastest ((nt,nx,ny) Fortran order); masks: ngroups*nx*ny
+// bytes, [g][iy][ix], non-zero where the receiver belongs to group g
+int pw_tri_set_data(pw_tri* p, const float* dat, const unsigned char* masks)
+{
+    if (!p || !dat || !masks) return PW_ERR_ARG;
+    pw_ctx* ctx = p->ctx;
+    const Geo& g = p->g;
+    const size_t nrec = (size_t)g.nx1 * g.ny1;
+    hipStream_t s = ctx->s();
+    PW_HIP(ctx, hipMemcpyAsync(p->d_tmp, dat, nrec * p->nt * 4, hipMemcpyHostToDevice, s));
+    dim3 grid(pw::ceil_div(p->nt, 32), pw::ceil_div((int)nrec, 32));
+    hipLaunchKernelGGL(k_transpose, grid, dim3(32, 8), 0, s, p->d_tmp, p->d_dat, (int)nrec, p->nt);
+    PW_HIP(ctx, hipGetLastError());
+    PW_HIP(ctx, hipMemcpyAsync(p->d_mask, masks, nrec * p->G, hipMemcpyHostToDevice, s));
+    PW_HIP(ctx, hipStreamSynchronize(s));
+    p->have_dat = true;
+    return PW_OK;
+}
+
+int pw_tri_run(pw_tri* p)
+{
+    if (!p) return PW_ERR_ARG;
+    if (!p->have_vel || !p->have_dat) PW_FAIL(p->ctx, PW_ERR_STATE, "tri run: velocity or data not set");
+    switch (p->G) {
+        case 1: return tri_enqueue<1>(p);
+        case 2: return tri_enqueue<2>(p);
+        case 3: return tri_enqueue<3>(p);
+        case 4: return tri_enqueue<4>(p);
+    }
+    return PW_ERR_ARG;
+}
+
+int pw_tri_run_timed(pw_tri* p, double* total_ms)
+{
+    if (!p) return PW_ERR_ARG;
+    pw_ctx* ctx = p->ctx;
+    hipStream_t s = ctx->s();
+    hipEvent_t e0, e1;
+    PW_HIP(ctx, hipEventCreate(&e0));
+    PW_HIP(ctx, hipEventCreate(&e1));
+    PW_HIP(ctx, hipEventRecord(e0, s));
+    int rc = pw_tri_run(p);
+    if (rc) return rc;
+    PW_HIP(ctx, hipEventRecord(e1, s));
+    PW_HIP(ctx, hipEventSynchronize(e1));
+    float ms = 0.f;
+    PW_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
+    if (total_ms) *total_ms = ms;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return PW_OK;
+}
+
+// image (nz*nx*ny, z fastest) in f64, unnormalised
+int pw_tri_image(pw_tri* p, double* img, long long len)
+{
+    if (!p || !img) return PW_ERR_ARG;
+    const Geo& g = p->g;
+    if (len != (long long)g.nz1 * g.nx1 * g.ny1) PW_FAIL(p->ctx, PW_ERR_ARG, "tri image: length mismatch");
+    if (!p->ran) PW_FAIL(p->ctx, PW_ERR_STATE, "tri image: nothing has run");
+    hipStream_t s = p->ctx->s();
+    PW_HIP(p->ctx, hipMemcpyAsync(img, p->d_img, len * 8, hipMemcpyDeviceToHost, s));
+    PW_HIP(p->ctx, hipStreamSynchronize(s));
+    return PW_OK;
 }
 
 }  // extern "C"

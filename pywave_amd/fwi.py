@@ -97,7 +97,8 @@ class FwiProblem:
     def forward(self, vel):
         """Modelled data (nt, nr, ns)."""
         out = np.empty(self.nt * self.nr * self.ns, np.float32)
-        N.check(self._L.pw_fwi_forward(self._p, N.ptr(self._vel(vel)), N.ptr(out)), self._ctx, "forward")
+        v = self._vel(vel)  # keep the converted array alive across the call
+        N.check(self._L.pw_fwi_forward(self._p, N.ptr(v), N.ptr(out)), self._ctx, "forward")
         return out.reshape(self.nt, self.nr, self.ns, order="F")
 
     def gradient(self, vel, mode="reference", scaling=0.0):
@@ -113,8 +114,9 @@ class FwiProblem:
             N.check(self._L.pw_fwi_gradient_f64(self._p, N.ptr(v), 1, N.ptr(gd), ctypes.byref(fc)), self._ctx,
                     "gradient")
             return gd.reshape(self.nz, self.nx, order="F"), fc.value, None
-        N.check(self._L.pw_fwi_gradient(self._p, N.ptr(self._vel(vel)), m, N.ptr(g), ctypes.byref(fc),
-                                        ctypes.byref(sc)), self._ctx, "gradient")
+        v = self._vel(vel)
+        N.check(self._L.pw_fwi_gradient(self._p, N.ptr(v), m, N.ptr(g), ctypes.byref(fc), ctypes.byref(sc)),
+                self._ctx, "gradient")
         return g.reshape(self.nz, self.nx, order="F"), fc.value, (sc.value if m == 0 else None)
 
     def misfit(self, vel):
@@ -129,8 +131,8 @@ class FwiProblem:
         niter = self.niter if niter is None else int(niter)
         out = np.zeros(self.nz * self.nx * max(niter, 1), np.float32)
         k = ctypes.c_int(0)
-        N.check(self._L.pw_fwi_lbfgs(self._p, N.ptr(self._vel(vel0)), niter, N.ptr(out), ctypes.byref(k)),
-                self._ctx, "lbfgs")
+        v = self._vel(vel0)
+        N.check(self._L.pw_fwi_lbfgs(self._p, N.ptr(v), niter, N.ptr(out), ctypes.byref(k)), self._ctx, "lbfgs")
         return out[:self.nz * self.nx * niter].reshape(self.nz, self.nx, niter, order="F"), k.value
 
     # device-pointer entry points (torch)

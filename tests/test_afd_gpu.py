@@ -51,7 +51,10 @@ def test_afd2d_tri_golden_bit_exact(golden):
     assert np.array_equal(p.final_field(), w0)
 
 
-@pytest.mark.parametrize("variant", range(6))
+NVAR = 12  # pw_afd_variant_count(): 6 general + 6 lean 3D step kernels
+
+
+@pytest.mark.parametrize("variant", range(NVAR))
 def test_afd3d_golden_bit_exact_all_variants(golden, variant):
     g = golden("afd3d_two_sources")
     p = _plan3d(g)
@@ -151,7 +154,7 @@ def test_random_2d_vs_oracle(seed):
         assert np.array_equal(out[nt * nx:], snaps.ravel())
 
 
-@pytest.mark.parametrize("seed", range(4))
+@pytest.mark.parametrize("seed", range(6))
 def test_random_3d_vs_oracle(seed):
     rng = np.random.default_rng(200 + seed)
     vel, nbt, ns, (pz, px, py) = _random_case(rng, 3)
@@ -164,7 +167,7 @@ def test_random_3d_vs_oracle(seed):
     d, snaps, _ = O.afd3d(vel, nt, dt, 10.0, 8.0, 12.0, nbt, ct, px, py, pz, rick, jsnap=jsnap)
     p = AfdPlan(vel, (12.0, 10.0, 8.0), nt, dt, nbt, ct, jsnap)
     p.set_sources(np.concatenate([px, py, pz, f, t, A]), ns, wavelet=rick)
-    p.set_variant(seed % 6, int(rng.integers(0, 12)))
+    p.set_variant((seed * 5) % NVAR, int(rng.integers(0, 12)))
     p.run()
     out = p.output()
     nz, nx, ny = vel.shape
@@ -228,6 +231,11 @@ def test_deep_underflow_3d_vs_oracle():
     p.set_sources(np.array([8, 8, 8, 15, 0.03, 1.0], np.float32), 1, wavelet=rick)
     p.run()
     assert np.array_equal(p.output(), d.ravel())
+
+
+def test_variant_count_matches():
+    from pywave_amd import _native as N
+    assert N.lib().pw_afd_variant_count() == NVAR
 
 
 def test_errors_follow_reference():
