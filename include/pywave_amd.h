@@ -89,6 +89,9 @@ int pw_afd_final_field(pw_afd *p, float *out, long long out_len);
 /* Kernel tile choice for the 3D step (0 = default); returns PW_ERR_ARG for unknown ids. */
 int pw_afd_set_variant(pw_afd *p, int variant, int ny_chunk);
 int pw_afd_variant_count(void);
+/* Autotuner: times each 3D kernel variant and y-chunk on `steps` steps of this plan, keeps the
+ * fastest (cached per padded grid shape in the context); no-op for 2D plans. */
+int pw_afd_autotune(pw_afd *p, int steps, int *variant_out, int *chunk_out);
 void pw_afd_destroy(pw_afd *p);
 
 /* ---------------------------------------------------------------- FWI (pfwi discretisation)
@@ -149,6 +152,22 @@ int pw_tri_run_timed(pw_tri *p, double *total_ms);
 /* unnormalised f64 image, nz*nx*ny z fastest */
 int pw_tri_image(pw_tri *p, double *img, long long len);
 void pw_tri_destroy(pw_tri *p);
+
+/* ---------------------------------------------------------------- batched 2D modelling
+ * n shots of uneven size and length (each one afd2d modelling problem, pywave/src/afd.c:316 ->
+ * pywave/src/wave_fdm.c:397) advanced together: one launch per time step covers every running
+ * shot; with use_graph the step loop is captured once into a hipGraph and replayed.
+ * dims: n*(nz, nx, nt); d: n*(dz, dx, dt); nbt, ct shared. */
+typedef struct pw_batch pw_batch;
+pw_batch *pw_batch2d_create(pw_ctx *ctx, int n, const int *dims, const float *d, int nbt, float ct);
+/* shot i: vel nz*nx (z fastest), src_pack [sx sz f t A] (5*ns), wavelet ns*nt or NULL */
+int pw_batch2d_set_shot(pw_batch *p, int i, const float *vel, int ns, const float *src_pack,
+                        const float *wavelet);
+int pw_batch2d_run(pw_batch *p, int use_graph, double *ms);
+/* traces of shot i, (nt, nx) Fortran order (afd2d's dout) */
+int pw_batch2d_traces(pw_batch *p, int i, float *out, long long len);
+long long pw_batch2d_cell_updates(pw_batch *p);
+void pw_batch2d_destroy(pw_batch *p);
 
 #ifdef __cplusplus
 }

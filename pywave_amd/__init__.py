@@ -14,5 +14,6 @@ from .afd import afd2d, afd3d, AfdPlan, count_snaps  # noqa: F401
 from .pfwi import pfwi, fillpar  # noqa: F401
 from .fwi import FwiProblem, FwiForward, fwi_traces, make_pararray  # noqa: F401
 from .tri import tri3d_image, group_masks  # noqa: F401
+from .batch import Afd2dBatch, afd2d_batch  # noqa: F401
 
 __version__ = "0.1.0"

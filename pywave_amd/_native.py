@@ -49,6 +49,7 @@ SIGNATURES = {
     "pw_afd_final_field": (c_int, [c_vp, c_vp, c_ll]),
     "pw_afd_set_variant": (c_int, [c_vp, c_int, c_int]),
     "pw_afd_variant_count": (c_int, []),
+    "pw_afd_autotune": (c_int, [c_vp, c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int)]),
     "pw_afd_destroy": (None, [c_vp]),
     "pw_fwi_create": (c_vp, [c_vp, c_vp, c_int, c_int]),
     "pw_fwi_destroy": (None, [c_vp]),
@@ -74,6 +75,12 @@ SIGNATURES = {
     "pw_tri_run_timed": (c_int, [c_vp, ctypes.POINTER(c_double)]),
     "pw_tri_image": (c_int, [c_vp, c_vp, c_ll]),
     "pw_tri_destroy": (None, [c_vp]),
+    "pw_batch2d_create": (c_vp, [c_vp, c_int, c_vp, c_vp, c_int, c_float]),
+    "pw_batch2d_set_shot": (c_int, [c_vp, c_int, c_vp, c_int, c_vp, c_vp]),
+    "pw_batch2d_run": (c_int, [c_vp, c_int, ctypes.POINTER(c_double)]),
+    "pw_batch2d_traces": (c_int, [c_vp, c_int, c_vp, c_ll]),
+    "pw_batch2d_cell_updates": (c_ll, [c_vp]),
+    "pw_batch2d_destroy": (None, [c_vp]),
 }
 
 

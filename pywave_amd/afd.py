@@ -149,6 +149,13 @@ class AfdPlan:
     def set_variant(self, variant, ny_chunk=0):
         N.check(self._L.pw_afd_set_variant(self._p, int(variant), int(ny_chunk)), self._ctx, "set_variant")
 
+    def autotune(self, steps=20):
+        """Pick the fastest 3D step kernel and y-chunk for this grid (cached per shape)."""
+        v, c = ctypes.c_int(0), ctypes.c_int(0)
+        N.check(self._L.pw_afd_autotune(self._p, int(steps), ctypes.byref(v), ctypes.byref(c)), self._ctx,
+                "autotune")
+        return v.value, c.value
+
     def run(self, tri=False):
         N.check(self._L.pw_afd_run(self._p, int(bool(tri))), self._ctx, "run")
 

@@ -999,6 +999,72 @@ int pw_afd_final_field(pw_afd* p, float* out, long long out_len)
     return PW_OK;
 }
 
+// Time every applicable 3D kernel variant and y-chunk length on `steps` real steps of this plan
+// (fields are reset by the next run) and keep the fastest; the choice is cached in the context
+// per padded grid shape.  Returns the chosen variant in *variant_out and its chunk in *chunk_out.
+int pw_afd_autotune(pw_afd* p, int steps, int* variant_out, int* chunk_out)
+{
+    if (afd_check(p) || steps < 1) return PW_ERR_ARG;
+    pw_ctx* ctx = p->ctx;
+    const Geo& g = p->g;
+    if (p->ndim != 3) {
+        if (variant_out) *variant_out = -1;
+        if (chunk_out) *chunk_out = 0;
+        return PW_OK;
+    }
+    const auto key = std::make_tuple(g.nz2, g.nx2, g.ny2);
+    auto it = ctx->tuned.find(key);
+    if (it == ctx->tuned.end()) {
+        if (!p->have_vel) PW_FAIL(ctx, PW_ERR_STATE, "autotune: velocity not set");
+        hipStream_t s = ctx->s();
+        hipEvent_t e0, e1;
+        PW_HIP(ctx, hipEventCreate(&e0));
+        PW_HIP(ctx, hipEventCreate(&e1));
+        const int chunks[] = {0, 8, 16, 32, 64};
+        float best = 1e30f;
+        std::pair<int, int> choice(p->lean_ok ? kLeanDefault : kGeneralDefault, 0);
+        const int saved_v = p->variant, saved_c = p->ny_chunk;
+        for (int v = 0; v < kNumVariants3; ++v) {
+            if (kVariants3[v].lean && !p->lean_ok) continue;
+            for (int c : chunks) {
+                p->variant = v;
+                p->ny_chunk = c;
+                Step a{};
+                a.cur = p->d_a; a.pv = p->d_b; a.vv = p->d_vv;
+                a.fz = p->d_fz; a.fx = p->d_fx; a.fy = p->d_fy;
+                a.fz1 = p->d_fz1; a.fx1 = p->d_fx1; a.fy1 = p->d_fy1;
+                a.src = p->d_src; a.wav = p->d_wav; a.nt = p->nt;
+                int rc = afd_launch_step(p, a);  // warm-up launch
+                if (rc) return rc;
+                PW_HIP(ctx, hipEventRecord(e0, s));
+                for (int k = 0; k < steps; ++k) {
+                    a.cur = (k & 1) ? p->d_b : p->d_a;
+                    a.pv = (k & 1) ? p->d_a : p->d_b;
+                    if ((rc = afd_launch_step(p, a))) return rc;
+                }
+                PW_HIP(ctx, hipEventRecord(e1, s));
+                PW_HIP(ctx, hipEventSynchronize(e1));
+                float ms = 0.f;
+                PW_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
+                if (ms < best) {
+                    best = ms;
+                    choice = std::make_pair(v, c);
+                }
+            }
+        }
+        p->variant = saved_v;
+        p->ny_chunk = saved_c;
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        it = ctx->tuned.emplace(key, choice).first;
+    }
+    p->variant = it->second.first;
+    p->ny_chunk = it->second.second;
+    if (variant_out) *variant_out = p->variant;
+    if (chunk_out) *chunk_out = p->ny_chunk;
+    return PW_OK;
+}
+
 int pw_afd_set_variant(pw_afd* p, int variant, int ny_chunk)
 {
     if (afd_check(p)) return PW_ERR_ARG;

@@ -1,0 +1,429 @@
+// pw_batch.hip -- many afd2d shots of uneven grid size and length advanced together.
+//
+// The reference models one shot per afd2d call (pywave/afd2d.py:4-90 -> pywave/src/wave_fdm.c:
+// 397-686).  Here every shot keeps its own padded fields in one pooled allocation; shots are
+// ordered by decreasing nt so the shots still running at step it are a prefix, and one launch per
+// step covers the flat list of 64 x XC tiles of all running shots (a tile looks up its shot in a
+// table).  The step loop is captured once into a hipGraph and replayed.  Per cell the arithmetic
+// is that of the single-shot kernel (pw_afd.hip), so every shot is bit-identical to afd2d.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "pw_internal.h"
+
+namespace {
+
+constexpr float C0 = -30. / 12., C1 = +16. / 12., C2 = -1. / 12.;  // pywave/src/wave_fdm.c:15
+constexpr int XC = 16;  // x columns streamed per tile
+
+struct Shot {
+    int nz, nx, nz2, nx2, ldz, nb, nz1, nx1;
+    float idz, idx, cc;
+    int nt;
+    long long off;      // element offset of this shot's fields in the pools
+    long long fz_off;   // offset into the damping pools (z pairs), x pairs follow
+    int ns, src_off;    // sources: centres at src_pool[3*src_off ...], wavelets at wav_pool[src_off*.. ]
+    long long wav_off;
+    long long rec_off;  // traces [nt][nx1] at rec_pool + rec_off
+    int ntz, nxc, tile0;
+};
+
+struct BatchArgs {
+    const Shot* __restrict__ shots;
+    const int* __restrict__ tile_shot;
+    float* pool_a;
+    float* pool_b;
+    const float* __restrict__ vv;
+    const float2* __restrict__ damp;  // per shot: nz2 z pairs then nx2 x pairs
+    const int* __restrict__ src;
+    const float* __restrict__ wav;
+    float* __restrict__ rec;
+    int it;
+};
+
+__global__ __launch_bounds__(64) void k_afd2d_batch(BatchArgs a)
+{
+    const int tile = blockIdx.x;
+    const Shot sh = a.shots[a.tile_shot[tile]];
+    const int lt = tile - sh.tile0;
+    const int tz = lt % sh.ntz, xc = lt / sh.ntz;
+    const int lane = threadIdx.x, z0 = tz * 64, z = z0 + lane;
+    const int x0 = xc * XC, x1 = min(x0 + XC, sh.nx);
+    // ping-pong by step parity: the field holding C_n
+    const bool even = (a.it & 1) == 0;
+    const float* __restrict__ cur = (even ? a.pool_a : a.pool_b) + sh.off;
+    float* __restrict__ pv = (even ? a.pool_b : a.pool_a) + sh.off;
+    const float* __restrict__ vv = a.vv + sh.off;
+    const float2* __restrict__ dz_ = a.damp + sh.fz_off;
+    const float2* __restrict__ dx_ = dz_ + sh.nz2;
+    __shared__ float row[2][68];
+    const bool zr = z < sh.nz2;
+    const bool zact = z < sh.nz;
+    const bool zin = z >= 2 && z < sh.nz2 - 2;
+    const float2 fzz = zact ? dz_[z] : make_float2(1.f, 1.f);
+    const int L = sh.ldz;
+    float q[5];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        int xx = x0 - 2 + k;
+        q[k] = (zr && xx >= 0 && xx < sh.nx2) ? cur[xx * L + z] : 0.f;
+    }
+    int buf = 0;
+    for (int x = x0; x < x1; ++x) {
+        q[4] = (zr && x + 2 < sh.nx2) ? cur[(x + 2) * L + z] : 0.f;
+        row[buf][lane + 2] = q[2];
+        if (lane < 4) {
+            int zz = lane < 2 ? z0 - 2 + lane : z0 + 62 + lane;
+            int lz = lane < 2 ? lane : 64 + lane;
+            row[buf][lz] = (zz >= 0 && zz < sh.nz2) ? cur[x * L + zz] : 0.f;
+        }
+        __syncthreads();
+        if (zact) {
+            const int j = x * L + z;
+            const float u = q[2];
+            float lap = 0.f;
+            if (zin && x >= 2 && x < sh.nx2 - 2) {
+                const float* R = &row[buf][lane + 2];
+                lap = C0 * u * sh.cc
+                    + C1 * (q[1] + q[3]) * sh.idx
+                    + C2 * (q[0] + q[4]) * sh.idx
+                    + C1 * (R[-1] + R[1]) * sh.idz
+                    + C2 * (R[-2] + R[2]) * sh.idz;
+            }
+            const float vc = vv[j];
+            const float2 fxx = dx_[x];
+            float p = pv[j];
+            p = p * fzz.x; p = p * fzz.y; p = p * fxx.x; p = p * fxx.y;
+            float c = u + (u - p);
+            c = c + lap * vc;
+            for (int s = 0; s < sh.ns; ++s) {
+                const int* cz = a.src + 3 * (sh.src_off + s);
+                int ddz = z - cz[0], ddx = x - cz[1];
+                int az_ = ddz < 0 ? -ddz : ddz, ax_ = ddx < 0 ? -ddx : ddx;
+                if (az_ <= 1 && ax_ <= 1)
+                    c = c + (vc * a.wav[sh.wav_off + (long long)s * sh.nt + a.it]) / (float)(az_ + ax_ + 1);
+            }
+            c = c * fzz.x; c = c * fzz.y; c = c * fxx.x; c = c * fxx.y;
+            pv[j] = c;
+            if (z == sh.nb && x >= sh.nb && x < sh.nb + sh.nx1)
+                a.rec[sh.rec_off + (long long)a.it * sh.nx1 + (x - sh.nb)] = c;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) q[k] = q[k + 1];
+        buf ^= 1;
+    }
+}
+
+// clamp-expanded vv = ((v*v)*dt)*dt for one shot
+__global__ void k_batch_vv(Shot sh, const float* __restrict__ vel, float* __restrict__ vv, float dt)
+{
+    const int z = blockIdx.x * blockDim.x + threadIdx.x, x = blockIdx.y;
+    if (z >= sh.nz) return;
+    const int sz = min(max(z - sh.nb, 0), sh.nz1 - 1), sx = min(max(x - sh.nb, 0), sh.nx1 - 1);
+    const float v = vel[(size_t)sx * sh.nz1 + sz];
+    vv[sh.off + (long long)x * sh.ldz + z] = v * v * dt * dt;
+}
+
+__global__ void k_zero(float* p, long long n)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (long long)gridDim.x * blockDim.x) p[i] = 0.f;
+}
+
+}  // namespace
+
+struct pw_batch {
+    pw_ctx* ctx = nullptr;
+    int n = 0;
+    std::vector<Shot> shots;    // sorted by decreasing nt
+    std::vector<int> order;     // sorted position -> caller's shot index
+    std::vector<int> where;     // caller's shot index -> sorted position
+    std::vector<float> dts;
+    std::vector<int> tiles_active;  // number of active tiles at step it (piecewise by shot)
+    long long pool = 0, rec_pool = 0, damp_pool = 0;
+    int max_nt = 0, total_tiles = 0;
+    float *d_a = nullptr, *d_b = nullptr, *d_vv = nullptr, *d_rec = nullptr, *d_wav = nullptr;
+    float2* d_damp = nullptr;
+    int *d_src = nullptr, *d_tile_shot = nullptr;
+    Shot* d_shots = nullptr;
+    std::vector<int> src_host;    // 3 per source
+    std::vector<float> wav_host;  // per source nt
+    std::vector<int> ns_of;
+    std::vector<char> have;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool dirty = true;
+};
+
+namespace {
+
+void batch_free(pw_batch* p)
+{
+    if (p->exec) hipGraphExecDestroy(p->exec);
+    if (p->graph) hipGraphDestroy(p->graph);
+    void* b[] = {p->d_a, p->d_b, p->d_vv, p->d_rec, p->d_wav, p->d_damp, p->d_src, p->d_tile_shot, p->d_shots};
+    for (void* x : b)
+        if (x) hipFree(x);
+}
+
+}  // namespace
+
+extern "C" {
+
+// dims: n*3 ints (nz, nx, nt) per shot; d: n*3 floats (dz, dx, dt); nbt, ct shared
+pw_batch* pw_batch2d_create(pw_ctx* ctx, int n, const int* dims, const float* d, int nbt, float ct)
+{
+    if (!ctx || n < 1 || !dims || !d || nbt < 0) return nullptr;
+    if (hipSetDevice(ctx->device) != hipSuccess) { ctx->err = "hipSetDevice failed"; return nullptr; }
+    pw_batch* p = new pw_batch;
+    p->ctx = ctx;
+    p->n = n;
+    p->order.resize(n);
+    std::iota(p->order.begin(), p->order.end(), 0);
+    std::stable_sort(p->order.begin(), p->order.end(), [&](int i, int j) { return dims[3 * i + 2] > dims[3 * j + 2]; });
+    p->where.resize(n);
+    for (int k = 0; k < n; ++k) p->where[p->order[k]] = k;
+    std::vector<float> w(std::max(nbt, 1));
+    pw::abc_weights(nbt, ct, w.data());
+    std::vector<float2> damp;
+    long long off = 0, roff = 0;
+    int tile0 = 0;
+    for (int k = 0; k < n; ++k) {
+        const int i = p->order[k];
+        const int nz1 = dims[3 * i], nx1 = dims[3 * i + 1], nt = dims[3 * i + 2];
+        const float dz = d[3 * i], dx = d[3 * i + 1], dt = d[3 * i + 2];
+        if (nz1 < 1 || nx1 < 1 || nt < 1 || !(dz > 0) || !(dx > 0) || !(dt > 0)) {
+            ctx->err = "pw_batch2d_create: invalid dimensions for shot " + std::to_string(i);
+            delete p;
+            return nullptr;
+        }
+        Shot s{};
+        s.nz1 = nz1; s.nx1 = nx1; s.nb = nbt; s.nt = nt;
+        s.nz = nz1 + 2 * nbt; s.nx = nx1 + 2 * nbt;
+        pw::padded_sizes(s.nz, s.nx, 1, &s.nz2, &s.nx2, nullptr);
+        s.ldz = pw::ceil_div(s.nz2, 32) * 32;
+        s.idz = 1 / (dz * dz);
+        s.idx = 1 / (dx * dx);
+        s.cc = s.idx + s.idz;
+        s.off = off;
+        off += (long long)s.ldz * s.nx2;
+        s.fz_off = (long long)damp.size();
+        std::vector<float> fz, fx;
+        pw::damping_pairs(s.nz2, nbt, nbt, w.data(), w.data(), fz);
+        pw::damping_pairs(s.nx2, nbt, nbt, w.data(), w.data(), fx);
+        for (int t = 0; t < s.nz2; ++t) damp.push_back(make_float2(fz[2 * t], fz[2 * t + 1]));
+        for (int t = 0; t < s.nx2; ++t) damp.push_back(make_float2(fx[2 * t], fx[2 * t + 1]));
+        s.rec_off = roff;
+        roff += (long long)nt * nx1;
+        s.ntz = pw::ceil_div(s.nz, 64);
+        s.nxc = pw::ceil_div(s.nx, XC);
+        s.tile0 = tile0;
+        tile0 += s.ntz * s.nxc;
+        p->shots.push_back(s);
+        p->dts.push_back(dt);
+        p->max_nt = std::max(p->max_nt, nt);
+    }
+    p->total_tiles = tile0;
+    p->pool = off;
+    p->rec_pool = roff;
+    p->damp_pool = (long long)damp.size();
+    p->ns_of.assign(n, 0);
+    p->have.assign(n, 0);
+    std::vector<int> tile_shot(tile0);
+    for (int k = 0; k < n; ++k)
+        for (int t = 0; t < p->shots[k].ntz * p->shots[k].nxc; ++t) tile_shot[p->shots[k].tile0 + t] = k;
+    // active tiles per step: shots sorted by nt, so running shots form a prefix
+    p->tiles_active.assign(p->max_nt, 0);
+    for (int it = 0; it < p->max_nt; ++it) {
+        int k = 0;
+        while (k < n && p->shots[k].nt > it) ++k;
+        p->tiles_active[it] = k == n ? tile0 : p->shots[k].tile0;
+    }
+    bool ok = hipMalloc(&p->d_a, p->pool * 4) == hipSuccess && hipMalloc(&p->d_b, p->pool * 4) == hipSuccess &&
+              hipMalloc(&p->d_vv, p->pool * 4) == hipSuccess && hipMalloc(&p->d_rec, p->rec_pool * 4) == hipSuccess &&
+              hipMalloc(&p->d_damp, p->damp_pool * sizeof(float2)) == hipSuccess &&
+              hipMalloc(&p->d_tile_shot, sizeof(int) * tile0) == hipSuccess &&
+              hipMalloc(&p->d_shots, sizeof(Shot) * n) == hipSuccess;
+    hipStream_t st = ctx->s();
+    ok = ok && hipMemcpyAsync(p->d_damp, damp.data(), damp.size() * sizeof(float2), hipMemcpyHostToDevice, st) == hipSuccess &&
+         hipMemcpyAsync(p->d_tile_shot, tile_shot.data(), sizeof(int) * tile0, hipMemcpyHostToDevice, st) == hipSuccess &&
+         hipMemsetAsync(p->d_vv, 0, p->pool * 4, st) == hipSuccess && hipStreamSynchronize(st) == hipSuccess;
+    if (!ok) {
+        ctx->err = "pw_batch2d_create: device allocation failed";
+        batch_free(p);
+        delete p;
+        return nullptr;
+    }
+    return p;
+}
+
+void pw_batch2d_destroy(pw_batch* p)
+{
+    if (!p) return;
+    hipSetDevice(p->ctx->device);
+    hipStreamSynchronize(p->ctx->s());
+    batch_free(p);
+    delete p;
+}
+
+// shot i: vel (nz*nx, z fastest), src_pack [sx.. sz.. f.. t.. A..] (5*ns), wavelet (ns*nt) or NULL
+int pw_batch2d_set_shot(pw_batch* p, int i, const float* vel, int ns, const float* src_pack, const float* wavelet)
+{
+    if (!p || i < 0 || i >= p->n || !vel || ns < 0 || (ns > 0 && !src_pack)) return PW_ERR_ARG;
+    pw_ctx* ctx = p->ctx;
+    const int k = p->where[i];
+    Shot& s = p->shots[k];
+    const float dt = p->dts[k];
+    float* d_vel = nullptr;
+    PW_HIP(ctx, hipMalloc(&d_vel, (size_t)s.nz1 * s.nx1 * 4));
+    hipStream_t st = ctx->s();
+    hipError_t e = hipMemcpyAsync(d_vel, vel, (size_t)s.nz1 * s.nx1 * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_batch_vv, dim3(pw::ceil_div(s.nz, 256), s.nx), dim3(256), 0, st, s, d_vel, p->d_vv, dt);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    hipFree(d_vel);
+    if (e != hipSuccess) PW_FAIL(ctx, PW_ERR_HIP, std::string("batch set_shot: ") + hipGetErrorString(e));
+    // sources of this shot (validated as in pw_afd_set_sources)
+    std::vector<int> cen(3 * ns);
+    std::vector<float> wav((size_t)ns * s.nt);
+    for (int j = 0; j < ns; ++j) {
+        int sx = (int)src_pack[j], sz = (int)src_pack[ns + j];
+        float f0 = src_pack[2 * ns + j], t0 = src_pack[3 * ns + j], A = src_pack[4 * ns + j];
+        int gz = sz + s.nb, gx = sx + s.nb;
+        if (gz - 1 < 0 || gz + 1 >= s.nz || gx - 1 < 0 || gx + 1 >= s.nx)
+            PW_FAIL(ctx, PW_ERR_ARG, "batch set_shot: source stencil leaves the padded domain");
+        cen[3 * j] = gz; cen[3 * j + 1] = gx; cen[3 * j + 2] = 0;
+        if (wavelet) {
+            std::memcpy(&wav[(size_t)j * s.nt], wavelet + (size_t)j * s.nt, 4 * s.nt);
+        } else {
+            std::string err;
+            int rc = pw::ricker(s.nt, dt, f0, t0, A, &wav[(size_t)j * s.nt], &err);
+            if (rc) PW_FAIL(ctx, rc, "batch set_shot: " + err);
+        }
+    }
+    // per-shot source blocks are stored in sorted order; rebuild the pools
+    p->ns_of[k] = ns;
+    std::vector<std::vector<int>> cs(p->n);
+    std::vector<std::vector<float>> ws(p->n);
+    for (int kk = 0; kk < p->n; ++kk) {  // keep the other shots' entries
+        const Shot& t = p->shots[kk];
+        if (kk != k && t.ns > 0) {
+            cs[kk].assign(p->src_host.begin() + 3 * t.src_off, p->src_host.begin() + 3 * (t.src_off + t.ns));
+            ws[kk].assign(p->wav_host.begin() + t.wav_off, p->wav_host.begin() + t.wav_off + (long long)t.ns * t.nt);
+        }
+    }
+    cs[k] = cen;
+    ws[k] = wav;
+    p->src_host.clear();
+    p->wav_host.clear();
+    for (int kk = 0; kk < p->n; ++kk) {
+        Shot& t = p->shots[kk];
+        t.ns = kk == k ? ns : t.ns;
+        t.src_off = (int)(p->src_host.size() / 3);
+        t.wav_off = (long long)p->wav_host.size();
+        p->src_host.insert(p->src_host.end(), cs[kk].begin(), cs[kk].end());
+        p->wav_host.insert(p->wav_host.end(), ws[kk].begin(), ws[kk].end());
+    }
+    p->have[k] = 1;
+    p->dirty = true;
+    return PW_OK;
+}
+
+static int batch_upload(pw_batch* p)
+{
+    pw_ctx* ctx = p->ctx;
+    hipStream_t st = ctx->s();
+    if (p->d_src) { hipFree(p->d_src); p->d_src = nullptr; }
+    if (p->d_wav) { hipFree(p->d_wav); p->d_wav = nullptr; }
+    PW_HIP(ctx, hipMalloc(&p->d_src, std::max<size_t>(4, p->src_host.size() * 4)));
+    PW_HIP(ctx, hipMalloc(&p->d_wav, std::max<size_t>(4, p->wav_host.size() * 4)));
+    if (!p->src_host.empty())
+        PW_HIP(ctx, hipMemcpyAsync(p->d_src, p->src_host.data(), p->src_host.size() * 4, hipMemcpyHostToDevice, st));
+    if (!p->wav_host.empty())
+        PW_HIP(ctx, hipMemcpyAsync(p->d_wav, p->wav_host.data(), p->wav_host.size() * 4, hipMemcpyHostToDevice, st));
+    PW_HIP(ctx, hipMemcpyAsync(p->d_shots, p->shots.data(), sizeof(Shot) * p->n, hipMemcpyHostToDevice, st));
+    PW_HIP(ctx, hipStreamSynchronize(st));
+    if (p->exec) { hipGraphExecDestroy(p->exec); p->exec = nullptr; }
+    if (p->graph) { hipGraphDestroy(p->graph); p->graph = nullptr; }
+    p->dirty = false;
+    return PW_OK;
+}
+
+static void batch_enqueue_steps(pw_batch* p, hipStream_t st)
+{
+    // zero the fields with a fill kernel (graph-safe), then one launch per step
+    hipLaunchKernelGGL(k_zero, dim3(2048), dim3(256), 0, st, p->d_a, p->pool);
+    hipLaunchKernelGGL(k_zero, dim3(2048), dim3(256), 0, st, p->d_b, p->pool);
+    for (int it = 0; it < p->max_nt; ++it) {
+        BatchArgs a{};
+        a.shots = p->d_shots; a.tile_shot = p->d_tile_shot;
+        a.pool_a = p->d_a; a.pool_b = p->d_b; a.vv = p->d_vv; a.damp = p->d_damp;
+        a.src = p->d_src; a.wav = p->d_wav; a.rec = p->d_rec; a.it = it;
+        hipLaunchKernelGGL(k_afd2d_batch, dim3(p->tiles_active[it]), dim3(64), 0, st, a);
+    }
+}
+
+// use_graph != 0: capture the step loop once and replay it
+int pw_batch2d_run(pw_batch* p, int use_graph, double* ms)
+{
+    if (!p) return PW_ERR_ARG;
+    pw_ctx* ctx = p->ctx;
+    for (int k = 0; k < p->n; ++k)
+        if (!p->have[k]) PW_FAIL(ctx, PW_ERR_STATE, "batch run: a shot has no model/sources");
+    if (p->dirty) {
+        int rc = batch_upload(p);
+        if (rc) return rc;
+    }
+    hipStream_t st = ctx->s();
+    hipEvent_t e0, e1;
+    PW_HIP(ctx, hipEventCreate(&e0));
+    PW_HIP(ctx, hipEventCreate(&e1));
+    if (use_graph && !p->exec) {
+        PW_HIP(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        batch_enqueue_steps(p, st);
+        PW_HIP(ctx, hipStreamEndCapture(st, &p->graph));
+        PW_HIP(ctx, hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
+    }
+    PW_HIP(ctx, hipEventRecord(e0, st));
+    if (use_graph) PW_HIP(ctx, hipGraphLaunch(p->exec, st));
+    else batch_enqueue_steps(p, st);
+    PW_HIP(ctx, hipGetLastError());
+    PW_HIP(ctx, hipEventRecord(e1, st));
+    PW_HIP(ctx, hipEventSynchronize(e1));
+    float t = 0.f;
+    PW_HIP(ctx, hipEventElapsedTime(&t, e0, e1));
+    if (ms) *ms = t;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return PW_OK;
+}
+
+// traces of shot i: (nt, nx) Fortran order (time fastest), as afd2d's dout
+int pw_batch2d_traces(pw_batch* p, int i, float* out, long long len)
+{
+    if (!p || i < 0 || i >= p->n || !out) return PW_ERR_ARG;
+    const Shot& s = p->shots[p->where[i]];
+    if (len != (long long)s.nt * s.nx1) PW_FAIL(p->ctx, PW_ERR_ARG, "batch traces: length mismatch");
+    std::vector<float> t((size_t)len);
+    hipStream_t st = p->ctx->s();
+    PW_HIP(p->ctx, hipMemcpyAsync(t.data(), p->d_rec + s.rec_off, len * 4, hipMemcpyDeviceToHost, st));
+    PW_HIP(p->ctx, hipStreamSynchronize(st));
+    for (int it = 0; it < s.nt; ++it)
+        for (int ix = 0; ix < s.nx1; ++ix) out[(size_t)ix * s.nt + it] = t[(size_t)it * s.nx1 + ix];
+    return PW_OK;
+}
+
+// total cell-updates of one run (updated domains x nt, summed over shots)
+long long pw_batch2d_cell_updates(pw_batch* p)
+{
+    if (!p) return 0;
+    long long c = 0;
+    for (const Shot& s : p->shots) c += (long long)s.nz * s.nx * s.nt;
+    return c;
+}
+
+}  // extern "C"

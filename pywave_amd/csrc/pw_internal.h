@@ -5,7 +5,9 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <map>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/pywave_amd.h"
@@ -15,6 +17,8 @@ struct pw_ctx {
     hipStream_t stream = nullptr;   // library-owned stream (default priority)
     hipStream_t ext_stream = nullptr;  // caller stream if set (torch), else null
     std::string err;
+    // autotuned 3D step choice per padded grid shape: (nz2, nx2, ny2) -> (variant, y chunk)
+    std::map<std::tuple<int, int, int>, std::pair<int, int>> tuned;
     hipStream_t s() const { return ext_stream ? ext_stream : stream; }
 };
 
