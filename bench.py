@@ -52,31 +52,45 @@ def cpu_baseline(w, nt_cpu):
 
 
 def fwi_bench(plain_shots):
-    """FWI gradient throughput on the 400x800, 32-shot, nt=3000 workload: exact-adjoint gradients
-    with all 32 shots per launch (checkpoints every 50 steps) against the plain per-shot loop
-    (one shot per launch, same kernels), in shots per second."""
-    from pywave_amd import FwiProblem
+    """FWI gradient throughput on the 400x800, 32-shot, nt=3000 workload (8 Hz Ricker, 60-cell
+    sponge): the batched gradient -- all 32 shots per launch, checkpoints every 50 steps --
+    taken through torch.autograd (fwi_traces + L2 misfit + backward, i.e. the hand-written
+    adjoint kernel), against the plain per-shot loop (one shot per launch, same kernels, direct
+    C-ABI gradient), in shots per second."""
+    import torch
+    from pywave_amd import FwiProblem, fwi_traces
     from pywave_amd.workloads import fwi2d
-    from pywave_amd import _native as N
 
     w = fwi2d()
     par = w["par"]
-    nt = par["nt"]
+    nt, ns = par["nt"], par["ns"]
     wav = (w["amp"] * ricker_wavelet(nt, par["dt"], w["f0"], w["t0"])).astype(np.float32)
-    gen = FwiProblem(par, wav, batch=par["ns"])
-    obs = gen.forward(w["vtrue"])
+    gen = FwiProblem(par, wav, batch=ns)
+    obs = gen.forward(w["vtrue"])  # (nt, nr, ns)
     gen.close()
-    out = {"workload": "fwi2d 400x800 (+60 sponge = 520x920), 32 shots, nt=3000, exact-adjoint gradient"}
-    batched = FwiProblem(par, wav, data=obs, batch=par["ns"], checkpoint=50)
-    batched.gradient(w["v0"], "exact")  # warm-up
+    out = {"workload": "fwi2d 400x800 at 10 m (+60-cell sponge: 520x920), 32 shots, nt=3000, dt=1 ms, 8 Hz, "
+                       "exact-adjoint L2 gradient, checkpoints every 50 steps"}
+    dev = torch.device("cuda:0")
+    prob = FwiProblem(par, wav, batch=ns, checkpoint=50)
+    d = torch.tensor(np.ascontiguousarray(obs.transpose(2, 0, 1)), device=dev)  # (ns, nt, nr)
+
+    def grad_autograd():
+        v = torch.tensor(w["v0"], device=dev, requires_grad=True)
+        tr = fwi_traces(v, prob)
+        loss = 0.5 * ((tr.double() - d.double()) ** 2).sum()
+        loss.backward()
+        torch.cuda.synchronize()
+        return v.grad, loss.item()
+
+    grad_autograd()  # warm-up
     t = time.perf_counter()
-    g, f, _ = batched.gradient(w["v0"], "exact")
+    g, f = grad_autograd()
     tb = time.perf_counter() - t
-    fw, bw = batched.timing()
-    batched.close()
-    out["batched_shots_per_s"] = round(par["ns"] / tb, 3)
+    prob.close()
+    out["batched_shots_per_s"] = round(ns / tb, 3)
     out["batched_s_per_gradient"] = round(tb, 4)
-    out["batched_forward_ms"], out["batched_backward_ms"] = round(fw, 2), round(bw, 2)
+    out["misfit"] = f
+    out["gradient_norm"] = float(torch.linalg.vector_norm(g.double()).item())
     # plain loop: one shot per launch, timed on the first `plain_shots` shots
     par1 = dict(par)
     par1["ns"] = plain_shots
@@ -88,10 +102,8 @@ def fwi_bench(plain_shots):
     tp = time.perf_counter() - t
     plain.close()
     out["plain_shots_per_s"] = round(plain_shots / tp, 3)
-    out["batched_over_plain"] = round((par["ns"] / tb) / (plain_shots / tp), 3)
-    out["gradient_norm"] = float(np.linalg.norm(g))
-    out["misfit"] = f
-    del N
+    out["plain_sample_shots"] = plain_shots
+    out["batched_over_plain"] = round((ns / tb) / (plain_shots / tp), 3)
     return out
 
 
@@ -100,6 +112,18 @@ def ricker_wavelet(nt, dt, f0, t0):
     wv = np.zeros(nt, np.float32)
     N.check(N.lib().pw_ricker(nt, dt, f0, t0, 1.0, N.ptr(wv)), None, "ricker")
     return wv
+
+
+def calibrate_copy():
+    """One elementwise copy of 2^28 float32 (1 GiB read + 1 GiB written), three times: the known
+    traffic against which FETCH_SIZE / WRITE_SIZE counter readings are scaled."""
+    import torch
+    a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": a.numel() * 4, "copy_bytes_written": b.numel() * 4}))
 
 
 def load_traffic(name):
@@ -123,11 +147,17 @@ def main():
     ap.add_argument("--no-fwi", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--fwi-plain-shots", type=int, default=4)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (requested --gpus %d)"
               % args.gpus, file=sys.stderr)
         sys.exit(2)
+
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
 
     from pywave_amd import AfdPlan
     from pywave_amd import _native as N

@@ -33,8 +33,11 @@ pw_ctx *pw_create(int device);
 void pw_destroy(pw_ctx *ctx);
 const char *pw_last_error(pw_ctx *ctx);
 int pw_device_count(void);
-/* Run subsequent work on a caller stream (hipStream_t passed as void*; NULL = own stream). */
+/* Run subsequent work on a caller stream (hipStream_t passed as void*; NULL is the HIP null
+   stream, which is what torch reports for its default stream). */
 int pw_set_stream(pw_ctx *ctx, void *stream);
+/* Return to the context's own (non-blocking) stream. */
+int pw_use_own_stream(pw_ctx *ctx);
 int pw_synchronize(pw_ctx *ctx);
 
 /* ---------------------------------------------------------------- host setup helpers */

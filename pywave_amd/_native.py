@@ -27,6 +27,7 @@ SIGNATURES = {
     "pw_last_error": (ctypes.c_char_p, [c_vp]),
     "pw_device_count": (c_int, []),
     "pw_set_stream": (c_int, [c_vp, c_vp]),
+    "pw_use_own_stream": (c_int, [c_vp]),
     "pw_synchronize": (c_int, [c_vp]),
     "pw_next_fast_size": (c_int, [c_int]),
     "pw_padded_sizes": (None, [c_int, c_int, c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int), ctypes.POINTER(c_int)]),

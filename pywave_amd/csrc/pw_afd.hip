@@ -317,6 +317,181 @@ __global__ __launch_bounds__(64 * WAVES) void k_afd3d_lean(Geo g, Step a, int nt
     }
 }
 
+// ---------------------------------------------------------------- 3D pipelined variant
+// The lean kernel's arithmetic with a software-pipelined y loop: the loads for plane y+1 (prev,
+// vv, both halos) and for the centre column of plane y+3 are issued before plane y is computed,
+// so a wave only ever waits on memory it requested one iteration earlier.  Fields are read and
+// written through buffer resources (scalar base per plane, 32-bit lane offsets).  A lane whose
+// cell lies outside the updated domain -- where every field value is zero -- gets an offset
+// beyond the resource's range: the hardware answers such a load with 0 without touching
+// memory and drops such a store, so the z slack of the 64-aligned rows, rows x >= nx and planes
+// outside [0, ny) cost no HBM traffic and need no branches.  NTP: non-temporal cache policy on
+// the streamed-once fields (prev/next, vv), leaving L2 to the halo-shared current field.
+constexpr unsigned kOOB = 0x80000000u;  // out-of-range lane offset (resource windows are < 2 GiB)
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const float* p, unsigned bytes)
+{
+    // raw buffer: stride 0, num_records = window bytes, 32-bit data format
+    return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)bytes, 0x00020000);
+}
+
+// load through a plane window; ok == false loads 0 without a memory access
+template <int AUX>
+__device__ __forceinline__ float bld(__amdgpu_buffer_rsrc_t rs, unsigned off, bool ok)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, ok ? off : kOOB, 0, AUX));
+}
+
+template <int AUX>
+__device__ __forceinline__ void bst(float v, __amdgpu_buffer_rsrc_t rs, unsigned off)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), rs, off, 0, AUX);
+}
+
+template <int RPW, int WAVES, int NTP>
+__global__ __launch_bounds__(64 * WAVES) void k_afd3d_pipe(Geo g, Step a, int ntz, int ntx, int nyc, int nblk)
+{
+    static_assert(WAVES >= 4, "one x-halo row per wave for waves 0..3");
+    constexpr int TZ = 64, TX = RPW * WAVES, LW = TZ + 4;
+    constexpr int AUX = NTP ? 2 : 0;  // slc: non-temporal
+    __shared__ float lds[2][TX + 4][LW];
+    const int per = gridDim.x >> 3;
+    const int lb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (lb >= nblk) return;
+    const int tz = lb % ntz;
+    const int rest = lb / ntz;
+    const int tx = rest % ntx;
+    const int ch = rest / ntx;
+    const int z0 = tz * TZ, x0 = tx * TX;
+    const int y0 = ch * nyc, y1 = min(y0 + nyc, g.ny);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int z = z0 + lane;
+    const int ldz = g.ldz;
+    const long long ldxy = g.ldxy;
+    // window of one plane: from 2 rows + 2 cells before it to 2 rows + 2 cells after it
+    const int bias = 2 * ldz + 2;
+    const unsigned wbytes = (unsigned)((ldxy + 2 * bias) * 4);
+    bool blk_src = false;
+    for (int s = 0; s < a.ns; ++s) {
+        int gz = a.src[3 * s], gx = a.src[3 * s + 1], gy = a.src[3 * s + 2];
+        blk_src |= gz >= z0 - 1 && gz <= z0 + TZ && gx >= x0 - 1 && gx <= x0 + TX && gy >= y0 - 1 && gy <= y1;
+    }
+    const bool zact = z < g.nz;
+    const float fz = a.fz1[z];
+    const bool zin = z >= 2 && z < g.nz2 - 2;
+    const int zh = lane < 2 ? z0 - 2 + lane : z0 + TZ - 2 + lane;  // z halo cell of lanes 0..3
+    const int lzh = lane < 2 ? lane : TZ + lane;
+    unsigned oc[RPW], ohz[RPW];
+    float fx[RPW];
+    bool xin[RPW];
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+        const int x = x0 + w * RPW + r;
+        const bool xact = x < g.nx;
+        oc[r] = (zact && xact) ? (unsigned)(bias + x * ldz + z) * 4u : kOOB;
+        ohz[r] = (lane < 4 && zh >= 0 && zh < g.nz && xact) ? (unsigned)(bias + x * ldz + zh) * 4u : kOOB;
+        fx[r] = a.fx1[x];
+        xin[r] = x >= 2 && x < g.nx2 - 2;
+    }
+    const int xh = w < 2 ? x0 - 2 + w : x0 + TX - 2 + w;  // x halo row of waves 0..3
+    const int lrh = w < 2 ? w : TX + w;
+    const unsigned ohx = (w < 4 && xh >= 0 && xh < g.nx && zact) ? (unsigned)(bias + xh * ldz + z) * 4u : kOOB;
+
+    auto cur_plane = [&](int yy) { return plane_rsrc(a.cur + yy * ldxy - bias, wbytes); };
+    auto pv_plane = [&](int yy) { return plane_rsrc(a.pv + yy * ldxy - bias, wbytes); };
+    auto vv_plane = [&](int yy) { return plane_rsrc(a.vv + yy * ldxy - bias, wbytes); };
+
+    // prologue: centre planes y0-2 .. y0+2, and plane y0's prev, vv and halos
+    float q[RPW][5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        const int yy = y0 - 2 + k;
+        const bool yv = yy >= 0 && yy < g.ny;
+        const auto rs = cur_plane(yy);
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) q[r][k] = bld<0>(rs, oc[r], yv);
+    }
+    float pr[RPW], vc[RPW], hz[RPW], hx;
+    float fyv = a.fy1[y0];
+    {
+        const auto rc = cur_plane(y0), rp = pv_plane(y0), rv = vv_plane(y0);
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) {
+            pr[r] = bld<AUX>(rp, oc[r], true);
+            vc[r] = bld<AUX>(rv, oc[r], true);
+            hz[r] = bld<0>(rc, ohz[r], true);
+        }
+        hx = bld<0>(rc, ohx, true);
+    }
+    int buf = 0;
+    for (int y = y0; y < y1; ++y) {
+        // prefetch for the next iteration
+        const int yn = y + 1;
+        const bool nv = yn < y1;
+        const bool n3 = nv && y + 3 < g.ny;
+        float npr[RPW], nvc[RPW], nhz[RPW], nq[RPW], nhx;
+        // vector load (the field stores in this kernel keep the compiler off the scalar cache),
+        // so it is prefetched like the rest
+        const float nfy = a.fy1[yn];
+        {
+            const auto rc = cur_plane(yn), rp = pv_plane(yn), rv = vv_plane(yn), r3 = cur_plane(y + 3);
+#pragma unroll
+            for (int r = 0; r < RPW; ++r) {
+                nq[r] = bld<0>(r3, oc[r], n3);
+                npr[r] = bld<AUX>(rp, oc[r], nv);
+                nvc[r] = bld<AUX>(rv, oc[r], nv);
+                nhz[r] = bld<0>(rc, ohz[r], nv);
+            }
+            nhx = bld<0>(rc, ohx, nv);
+        }
+        float(*L)[LW] = lds[buf];
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) L[w * RPW + r + 2][lane + 2] = q[r][2];
+        if (lane < 4) {
+#pragma unroll
+            for (int r = 0; r < RPW; ++r) L[w * RPW + r + 2][lzh] = hz[r];
+        }
+        if (w < 4) L[lrh][lane + 2] = hx;
+        __syncthreads();
+        const bool yin = y >= 2 && y < g.ny2 - 2;
+        const auto wp = pv_plane(y);
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) {
+            const int lr = w * RPW + r + 2;
+            const float u = q[r][2];
+            float lap = C0 * u * g.cc
+                      + C1 * (q[r][1] + q[r][3]) * g.idy
+                      + C2 * (q[r][0] + q[r][4]) * g.idy
+                      + C1 * (L[lr - 1][lane + 2] + L[lr + 1][lane + 2]) * g.idx
+                      + C2 * (L[lr - 2][lane + 2] + L[lr + 2][lane + 2]) * g.idx
+                      + C1 * (L[lr][lane + 1] + L[lr][lane + 3]) * g.idz
+                      + C2 * (L[lr][lane] + L[lr][lane + 4]) * g.idz;
+            lap = (yin && zin && xin[r]) ? lap : 0.f;
+            const float p = pr[r] * fz * fx[r] * fyv;
+            float c = u + (u - p);
+            c = c + lap * vc[r];
+            const int x = x0 + w * RPW + r;
+            if (blk_src) c = inject(c, vc[r], a, z, x, y);
+            const bool onrec = z == g.nb && x >= g.nb && x < g.nb + g.nx1 && y >= g.nb && y < g.nb + g.ny1;
+            if (a.tri && onrec) c = c + vc[r] * a.tri[(y - g.nb) * g.nx1 + (x - g.nb)];
+            c = c * fz * fx[r] * fyv;
+            bst<AUX>(c, wp, oc[r]);
+            if (a.rec && onrec) a.rec[(y - g.nb) * g.nx1 + (x - g.nb)] = c;
+            if (a.snap && zact && x < g.nx && z >= g.nb && z < g.nb + g.nz1 && x >= g.nb && x < g.nb + g.nx1 &&
+                y >= g.nb && y < g.nb + g.ny1)
+                a.snap[((size_t)(y - g.nb) * g.nx1 + (x - g.nb)) * g.nz1 + (z - g.nb)] = c;
+        }
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) {
+            q[r][0] = q[r][1]; q[r][1] = q[r][2]; q[r][2] = q[r][3]; q[r][3] = q[r][4]; q[r][4] = nq[r];
+            pr[r] = npr[r]; vc[r] = nvc[r]; hz[r] = nhz[r];
+        }
+        hx = nhx;
+        fyv = nfy;
+        buf ^= 1;
+    }
+}
+
 // ---------------------------------------------------------------- 2D: one cell per lane
 template <int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void k_afd2d(Geo g, Step a)
@@ -538,14 +713,18 @@ __global__ __launch_bounds__(256) void k_tri3d(Geo g, TriStep<G> a, int ntz, int
 }
 
 // 3D step kernels: 0-5 general (bounds-checked), 6-11 lean (guarded layout, single damping
-// factor per axis).  tx = rows per block, waves per block; lean rows per thread = tx / waves.
+// factor per axis), 12-19 pipelined (lean arithmetic, prefetching y loop, buffer loads; 16-17
+// with non-temporal prev/vv traffic).  tx = rows per block, waves per block; rows per thread of
+// the lean and pipelined kernels = tx / waves.
 struct Variant3 {
-    int tx, waves, lean;
+    int tx, waves, lean;  // lean: 0 general, 1 lean, 2 pipelined, 3 pipelined non-temporal
 };
-const Variant3 kVariants3[] = {{16, 4, 0}, {8, 4, 0}, {16, 8, 0}, {32, 8, 0}, {8, 8, 0}, {4, 4, 0},
-                               {16, 8, 1}, {8, 8, 1}, {16, 4, 1}, {32, 8, 1}, {16, 16, 1}, {32, 16, 1}};
+const Variant3 kVariants3[] = {{16, 4, 0}, {8, 4, 0},  {16, 8, 0},  {32, 8, 0},  {8, 8, 0},  {4, 4, 0},
+                               {16, 8, 1}, {8, 8, 1},  {16, 4, 1},  {32, 8, 1},  {16, 16, 1}, {32, 16, 1},
+                               {16, 8, 2}, {16, 4, 2}, {32, 8, 2},  {8, 4, 2},   {16, 8, 3}, {16, 4, 3},
+                               {32, 16, 2}, {8, 8, 2}};
 constexpr int kNumVariants3 = sizeof(kVariants3) / sizeof(kVariants3[0]);
-constexpr int kLeanDefault = 6;
+constexpr int kLeanDefault = 12;
 constexpr int kGeneralDefault = 2;
 
 }  // namespace
@@ -581,9 +760,37 @@ struct pw_afd {
     const float* final_field = nullptr;
     int variant = -1, ny_chunk = 0;
     size_t field_elems = 0;
+    long long resident[kNumVariants3] = {};  // co-resident blocks per 3D variant (0: not queried)
 };
 
 namespace {
+
+// blocks of 3D variant vi that the device holds at once (CUs x blocks per CU), cached per plan
+long long resident_blocks(pw_afd* p, int vi)
+{
+    if (p->resident[vi] > 0) return p->resident[vi];
+    int cus = 0, per_cu = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->ctx->device) != hipSuccess || cus < 1)
+        cus = 256;
+    const Variant3 v = kVariants3[vi];
+    hipError_t e = hipErrorInvalidValue;
+    switch (vi) {
+#define PW_OCC(I, K) \
+    case I: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, K, 64 * v.waves, 0); break;
+        PW_OCC(0, (k_afd3d<16, 4>)) PW_OCC(1, (k_afd3d<8, 4>)) PW_OCC(2, (k_afd3d<16, 8>))
+        PW_OCC(3, (k_afd3d<32, 8>)) PW_OCC(4, (k_afd3d<8, 8>)) PW_OCC(5, (k_afd3d<4, 4>))
+        PW_OCC(6, (k_afd3d_lean<2, 8>)) PW_OCC(7, (k_afd3d_lean<1, 8>)) PW_OCC(8, (k_afd3d_lean<4, 4>))
+        PW_OCC(9, (k_afd3d_lean<4, 8>)) PW_OCC(10, (k_afd3d_lean<1, 16>)) PW_OCC(11, (k_afd3d_lean<2, 16>))
+        PW_OCC(12, (k_afd3d_pipe<2, 8, 0>)) PW_OCC(13, (k_afd3d_pipe<4, 4, 0>)) PW_OCC(14, (k_afd3d_pipe<4, 8, 0>))
+        PW_OCC(15, (k_afd3d_pipe<2, 4, 0>)) PW_OCC(16, (k_afd3d_pipe<2, 8, 1>)) PW_OCC(17, (k_afd3d_pipe<4, 4, 1>))
+        PW_OCC(18, (k_afd3d_pipe<2, 16, 0>)) PW_OCC(19, (k_afd3d_pipe<1, 8, 0>))
+#undef PW_OCC
+        default: break;
+    }
+    if (e != hipSuccess || per_cu < 1) per_cu = 2;
+    p->resident[vi] = (long long)cus * per_cu;
+    return p->resident[vi];
+}
 
 int afd_launch_step(pw_afd* p, const Step& a)
 {
@@ -601,10 +808,12 @@ int afd_launch_step(pw_afd* p, const Step& a)
         const int ntz = pw::ceil_div(g.nz, 64), ntx = pw::ceil_div(g.nx, v.tx);
         int nyc = p->ny_chunk;
         if (nyc <= 0) {
-            // enough blocks for ~6 per CU on 256 CUs, chunks no shorter than 16 planes
-            long long tiles = (long long)ntz * ntx;
-            int nch = (int)std::max(1LL, (6 * 256 + tiles - 1) / tiles);
-            nyc = std::max(16, pw::ceil_div(g.ny, nch));
+            // as many y chunks per tile column as fit one round of co-resident blocks: every
+            // block starts at once (no tail round) and neighbours stream in step through L2
+            const long long tiles = (long long)ntz * ntx;
+            const long long cap = resident_blocks(p, vi);
+            const int nch = (int)std::max(1LL, cap / tiles);
+            nyc = std::max(8, pw::ceil_div(g.ny, nch));
         }
         const int nch = pw::ceil_div(g.ny, nyc);
         const int nblk = ntz * ntx * nch;
@@ -634,6 +843,20 @@ int afd_launch_step(pw_afd* p, const Step& a)
             PW_VL(10, 16, 16)
             PW_VL(11, 32, 16)
 #undef PW_VL
+#define PW_VP(I, TX_, W_, NT_)                                                                      \
+    case I:                                                                                         \
+        hipLaunchKernelGGL((k_afd3d_pipe<TX_ / W_, W_, NT_>), dim3(grid), dim3(64 * W_), 0, s, g, a, ntz, ntx, \
+                           nyc, nblk);                                                              \
+        break;
+            PW_VP(12, 16, 8, 0)
+            PW_VP(13, 16, 4, 0)
+            PW_VP(14, 32, 8, 0)
+            PW_VP(15, 8, 4, 0)
+            PW_VP(16, 16, 8, 1)
+            PW_VP(17, 16, 4, 1)
+            PW_VP(18, 32, 16, 0)
+            PW_VP(19, 8, 8, 0)
+#undef PW_VP
             default:
                 PW_FAIL(ctx, PW_ERR_ARG, "unknown 3D kernel variant");
         }
@@ -736,6 +959,7 @@ pw_afd* pw_afd_create(pw_ctx* ctx, int ndim, int nz, int nx, int ny, float dz, f
     lean = lean && single(fx, (size_t)g.nx2 + kMaxTx + 8, fx1);
     lean = lean && single(fy, (size_t)g.ny2 + 8, fy1);
     lean = lean && (double)p->raw_elems < 2.0e9;  // 32-bit offsets
+    lean = lean && (double)(g.ldxy + 4 * g.ldz + 4) * 4.0 < 2147483648.0;  // pipelined plane windows
     p->lean_ok = lean;
 
     const size_t nrec = (size_t)g.nx1 * g.ny1;

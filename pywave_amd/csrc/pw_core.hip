@@ -330,6 +330,15 @@ int pw_set_stream(pw_ctx* ctx, void* stream)
 {
     if (!ctx) return PW_ERR_ARG;
     ctx->ext_stream = (hipStream_t)stream;
+    ctx->use_ext = true;
+    return PW_OK;
+}
+
+int pw_use_own_stream(pw_ctx* ctx)
+{
+    if (!ctx) return PW_ERR_ARG;
+    ctx->ext_stream = nullptr;
+    ctx->use_ext = false;
     return PW_OK;
 }
 

@@ -15,11 +15,12 @@
 struct pw_ctx {
     int device = 0;
     hipStream_t stream = nullptr;   // library-owned stream (default priority)
-    hipStream_t ext_stream = nullptr;  // caller stream if set (torch), else null
+    hipStream_t ext_stream = nullptr;  // caller stream (may be the null stream) when use_ext
+    bool use_ext = false;
     std::string err;
     // autotuned 3D step choice per padded grid shape: (nz2, nx2, ny2) -> (variant, y chunk)
     std::map<std::tuple<int, int, int>, std::pair<int, int>> tuned;
-    hipStream_t s() const { return ext_stream ? ext_stream : stream; }
+    hipStream_t s() const { return use_ext ? ext_stream : stream; }
 };
 
 // Error helpers: every HIP call is checked; the message lands in ctx->err and the

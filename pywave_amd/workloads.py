@@ -114,6 +114,6 @@ def fwi2d(nz=400, nx=800, ns=32, nt=3000, seed=4):
         v *= 1 + a * np.exp(-((zz - cz) ** 2 + (xx - cx) ** 2) / (2 * s * s))
     vtrue = v.astype(np.float32)
     v0 = smooth_gaussian(vtrue, 20.0).astype(np.float32)
-    par = dict(nz=nz, nx=nx, dz=0.01, dx=0.01, oz=0, ox=0, ns=ns, ds=0.24, os=0.2, nt=nt, dt=0.001, ot=0,
+    par = dict(nz=nz, nx=nx, dz=0.01, dx=0.01, oz=0, ox=0, ns=ns, ds=0.24, os=0.2, sz=5, nt=nt, dt=0.001, ot=0,
                nb=60, coef=0.005, acqui_type=1, interval=1, f0=8, inv=0, niter=5)
     return dict(vtrue=vtrue, v0=v0, par=par, f0=8.0, t0=0.15, amp=1e6)

@@ -70,7 +70,9 @@ def test_tri_image_matches_oracle_recipe():
 
 
 def test_tri_image_focuses_on_sources():
-    # three subsurface sources: the normalised image peaks within 2 cells of one of them
+    # three subsurface sources: the normalised image peaks within 2 cells of one of them laterally;
+    # a surface array resolves depth only to about a wavelength (15 Hz at 2000 m/s: 133 m = 13 cells)
+    # and the back-propagated energy leans toward the receivers, so depth is allowed 7 cells
     nz = nx = ny = 40
     vel = np.full((nz, nx, ny), 2000.0, np.float32)
     pos = np.array([[12, 20, 24], [28, 14, 18], [22, 30, 10]])  # (z, x, y)
@@ -85,4 +87,5 @@ def test_tri_image_focuses_on_sources():
     img = pywave_amd.tri3d_image(vel, dat, groups, dt=0.001, ax=[0, 10, nx], ay=[0, 10, ny], az=[0, 10, nz],
                                  nbt=20, ct=0.01, jsnap=2)
     peak = np.array(np.unravel_index(np.argmax(img), img.shape))
-    assert np.min(np.abs(pos - peak).max(axis=1)) <= 2, peak
+    d = np.abs(pos - peak)
+    assert np.any((d[:, 1] <= 2) & (d[:, 2] <= 2) & (d[:, 0] <= 7)), peak
