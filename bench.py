@@ -4,7 +4,9 @@ reference CPU path timed on this host's cores beside it.
 
 A "step" is one complete modelling run (all nt time steps) over the resident model.
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--variant V] [--ny-chunk C]
-                       [--cpu-steps S] [--no-cpu]
+                       [--cpu-steps S] [--no-cpu] [--no-fwi] [--no-tri] [--no-batch] [--no-tune]
+Beside the headline 3D line it reports, as sub-objects, the FWI gradient (batched through
+torch.autograd vs the plain per-shot loop), time-reversal imaging and batched ragged 2D modelling.
 Prints one JSON line.
 """
 import argparse
@@ -91,6 +93,8 @@ def fwi_bench(plain_shots):
     out["batched_s_per_gradient"] = round(tb, 4)
     out["misfit"] = f
     out["gradient_norm"] = float(torch.linalg.vector_norm(g.double()).item())
+    if plain_shots <= 0:
+        return out
     # plain loop: one shot per launch, timed on the first `plain_shots` shots
     par1 = dict(par)
     par1["ns"] = plain_shots
@@ -105,6 +109,60 @@ def fwi_bench(plain_shots):
     out["plain_sample_shots"] = plain_shots
     out["batched_over_plain"] = round((ns / tb) / (plain_shots / tp), 3)
     return out
+
+
+def tri_bench():
+    """Time-reversal imaging on the 128^3 workload (+30-cell ABC: 188^3 updated, nt=2000): the
+    forward modelling of three sources, then the four interleaved receiver groups back-propagated
+    together with the image sum_snap prod_g u_g^2 (jsnap=4) accumulated on the GPU.  Bytes per
+    step: G fields x (read u_n, u_{n-1}, write u_{n+1}) + one vv read = (12 G + 4) B per cell,
+    plus the f64 image read-modify-write (16 B per interior cell) at every snapshot step."""
+    from pywave_amd import AfdPlan, tri3d_image
+    from pywave_amd.workloads import tri3d
+
+    w = tri3d()
+    vel = w["vel"]
+    n = vel.shape[0]
+    nb = w["nbt"]
+    plan = AfdPlan(vel, w["d"], w["nt"], w["dt"], nb, w["ct"], 0)
+    plan.set_sources(w["src"], w["ns"])
+    plan.autotune(steps=5)
+    fwd_ms, _ = plan.run_timed()
+    dat = plan.output().reshape(w["nt"], n, n, order="F")
+    plan.close()
+    ax = [0, w["d"][1], n]
+    _, ms = tri3d_image(vel, dat, w["groups"], dt=w["dt"], ax=ax, ay=ax, az=ax, nbt=nb, ct=w["ct"],
+                        jsnap=w["jsnap"], return_plan_time=True)
+    G = len(w["groups"])
+    cells = (n + 2 * nb) ** 3
+    nsnap = (w["nt"] - 1) // w["jsnap"] + 1
+    alg = w["nt"] * cells * (12.0 * G + 4.0) + nsnap * 16.0 * n ** 3
+    return {"workload": "tri3d 128^3 (+30 ABC: 188^3), nt=2000, 3 sources, 4 interleaved receiver groups, jsnap=4",
+            "forward_ms": round(fwd_ms, 2), "backprop_image_ms": round(ms, 2),
+            "field_cell_updates_per_s_G": round(G * cells * w["nt"] / (ms * 1e-3) / 1e9, 2),
+            "achieved_GBs": round(alg / (ms * 1e-3) / 1e9, 1),
+            "frac_hbm": round(alg / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
+
+
+def batch_bench():
+    """96 ragged 2D shots (nz 200-600, nx 400-1600, nt 2000-4000) advanced in one launch per step,
+    the step loop replayed from a hipGraph; 16 B per cell-update (+ the trace sample)."""
+    from pywave_amd import Afd2dBatch
+    from pywave_amd.workloads import batch2d
+
+    shots = batch2d()
+    b = Afd2dBatch(shots)
+    b.run(use_graph=True)  # warm-up (graph capture)
+    ms = b.run(use_graph=True)
+    cu = b.cell_updates()
+    b.close()
+    nrec = sum(s["vel"].shape[1] * s["nt"] for s in shots)
+    alg = 16.0 * cu + 4.0 * nrec
+    return {"workload": "batch2d 96 shots, nz 200-600, nx 400-1600 (+30 ABC), nt 2000-4000, 12 Hz, hipGraph",
+            "ms": round(ms, 2), "shots_per_s": round(len(shots) / (ms * 1e-3), 2),
+            "cell_updates": cu, "Gcell_per_s": round(cu / (ms * 1e-3) / 1e9, 2),
+            "achieved_GBs": round(alg / (ms * 1e-3) / 1e9, 1),
+            "frac_hbm": round(alg / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
 
 
 def ricker_wavelet(nt, dt, f0, t0):
@@ -126,11 +184,15 @@ def calibrate_copy():
     print(json.dumps({"copy_bytes_read": a.numel() * 4, "copy_bytes_written": b.numel() * 4}))
 
 
-def load_traffic(name):
+def load_traffic(name, variant):
+    """HBM bytes per launch of the step kernel from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    runs (scaled by the copy calibration), if they were taken on the kernel variant in use."""
     path = os.path.join(ROOT, "profiles", name)
     if os.path.exists(path):
         with open(path) as f:
-            return json.load(f).get("hbm_bytes_per_launch")
+            d = json.load(f)
+        if variant is not None and list(d.get("variant", [])) == list(variant):
+            return d.get("hbm_bytes_per_launch")
     return None
 
 
@@ -146,9 +208,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fwi", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--no-tri", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--fwi-plain-shots", type=int, default=4)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (HBM counter calibration)")
+    ap.add_argument("--only", choices=["fwi", "tri", "batch"],
+                    help="run one secondary workload alone and print its JSON object (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (requested --gpus %d)"
@@ -157,6 +223,10 @@ def main():
 
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.only:
+        leg = {"fwi": lambda: fwi_bench(args.fwi_plain_shots), "tri": tri_bench, "batch": batch_bench}[args.only]
+        print(json.dumps({args.only: leg()}))
         return
 
     from pywave_amd import AfdPlan
@@ -197,8 +267,11 @@ def main():
     out = plan.output()
     assert np.isfinite(out).all()
 
+    used = [args.variant, args.ny_chunk] if args.variant >= 0 else (list(tuned) if tuned else None)
     cpu = None if args.no_cpu else cpu_baseline(w, args.cpu_steps)
     fwi = None if args.no_fwi else fwi_bench(args.fwi_plain_shots)
+    tri = None if args.no_tri else tri_bench()
+    batch = None if args.no_batch else batch_bench()
     line = {
         "metric": "Gcell-updates/s 3D FP32 FD (% HBM roofline); FWI grad shots/s batched vs plain",
         "value": round(value, 3),
@@ -218,7 +291,7 @@ def main():
                         "15 Hz Ricker at the centre, 201x201 surface receivers",
             "cells_per_time_step": cells,
             "time_steps": w["nt"],
-            "kernel_variant": args.variant if args.variant >= 0 else tuned,
+            "kernel_variant": used,
         },
         "roofline": {
             "bound": "hbm",
@@ -226,12 +299,14 @@ def main():
             "peak": PEAK_HBM_GBS,
             "unit": "GB/s",
             "frac": round(achieved / PEAK_HBM_GBS, 4),
-            "traffic": load_traffic("afd3d_step_traffic.json"),
+            "traffic": load_traffic("afd3d_step_traffic.json", used),
             "kernel_avg_ms": round(step_ms, 5),
             "algorithmic_bytes_per_launch": alg_bytes,
         },
         "cpu_baseline": cpu,
         "fwi": fwi,
+        "tri3d": tri,
+        "batch2d": batch,
     }
     print(json.dumps(line))
 

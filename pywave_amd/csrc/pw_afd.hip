@@ -19,6 +19,8 @@
 #include <cmath>
 #include <cstring>
 
+#include <hip/hip_ext.h>
+
 #include "pw_internal.h"
 
 namespace {
@@ -210,8 +212,9 @@ __global__ __launch_bounds__(64 * WAVES) void k_afd3d(Geo g, Step a, int ntz, in
 
 // ---------------------------------------------------------------- 3D lean variant
 // Same arithmetic as k_afd3d with the per-cell overhead removed: the fields carry zero guard
-// planes around the padded grid and a 64-aligned z stride, so every load is unconditional and in
-// bounds; the rim rule is a select; damping uses one factor per axis (exact when the two faces of
+// planes around the padded grid, so every load is unconditional and in bounds (lanes past the
+// end of a row read the next row or the guard; they are outside the updated domain and their
+// values only reach other such lanes); the rim rule is a select; damping uses one factor per axis (exact when the two faces of
 // an axis do not overlap, checked on the host); offsets are 32-bit.
 template <int RPW, int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void k_afd3d_lean(Geo g, Step a, int ntz, int ntx, int nyc, int nblk)
@@ -348,7 +351,7 @@ __device__ __forceinline__ void bst(float v, __amdgpu_buffer_rsrc_t rs, unsigned
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), rs, off, 0, AUX);
 }
 
-template <int RPW, int WAVES, int NTP>
+template <int RPW, int WAVES, int NTP, int ORD>
 __global__ __launch_bounds__(64 * WAVES) void k_afd3d_pipe(Geo g, Step a, int ntz, int ntx, int nyc, int nblk)
 {
     static_assert(WAVES >= 4, "one x-halo row per wave for waves 0..3");
@@ -358,10 +361,21 @@ __global__ __launch_bounds__(64 * WAVES) void k_afd3d_pipe(Geo g, Step a, int nt
     const int per = gridDim.x >> 3;
     const int lb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
     if (lb >= nblk) return;
-    const int tz = lb % ntz;
-    const int rest = lb / ntz;
-    const int tx = rest % ntx;
-    const int ch = rest / ntx;
+    // logical tile order inside an XCD's range: ORD 0 z-tile fastest, then x-tile, then y chunk;
+    // ORD 1 x-tile fastest (x neighbours, which read each other's halo rows, are dispatched
+    // together), then z-tile, then y chunk
+    int tz, tx, ch;
+    if (ORD == 0) {
+        tz = lb % ntz;
+        const int rest = lb / ntz;
+        tx = rest % ntx;
+        ch = rest / ntx;
+    } else {
+        tx = lb % ntx;
+        const int rest = lb / ntx;
+        tz = rest % ntz;
+        ch = rest / ntz;
+    }
     const int z0 = tz * TZ, x0 = tx * TX;
     const int y0 = ch * nyc, y1 = min(y0 + nyc, g.ny);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -714,15 +728,17 @@ __global__ __launch_bounds__(256) void k_tri3d(Geo g, TriStep<G> a, int ntz, int
 
 // 3D step kernels: 0-5 general (bounds-checked), 6-11 lean (guarded layout, single damping
 // factor per axis), 12-19 pipelined (lean arithmetic, prefetching y loop, buffer loads; 16-17
-// with non-temporal prev/vv traffic).  tx = rows per block, waves per block; rows per thread of
-// the lean and pipelined kernels = tx / waves.
+// with non-temporal prev/vv traffic), 20-23 pipelined with x tiles fastest in the block order.
+// tx = rows per block, waves per block; rows per thread of the lean and pipelined kernels =
+// tx / waves.
 struct Variant3 {
-    int tx, waves, lean;  // lean: 0 general, 1 lean, 2 pipelined, 3 pipelined non-temporal
+    int tx, waves, lean;  // lean: 0 general, 1 lean, 2 pipelined, 3 pipelined non-temporal,
+                          // 4 pipelined x-major block order
 };
 const Variant3 kVariants3[] = {{16, 4, 0}, {8, 4, 0},  {16, 8, 0},  {32, 8, 0},  {8, 8, 0},  {4, 4, 0},
                                {16, 8, 1}, {8, 8, 1},  {16, 4, 1},  {32, 8, 1},  {16, 16, 1}, {32, 16, 1},
                                {16, 8, 2}, {16, 4, 2}, {32, 8, 2},  {8, 4, 2},   {16, 8, 3}, {16, 4, 3},
-                               {32, 16, 2}, {8, 8, 2}};
+                               {32, 16, 2}, {8, 8, 2}, {16, 4, 4}, {8, 4, 4}, {16, 8, 4}, {32, 8, 4}};
 constexpr int kNumVariants3 = sizeof(kVariants3) / sizeof(kVariants3[0]);
 constexpr int kLeanDefault = 12;
 constexpr int kGeneralDefault = 2;
@@ -781,9 +797,12 @@ long long resident_blocks(pw_afd* p, int vi)
         PW_OCC(3, (k_afd3d<32, 8>)) PW_OCC(4, (k_afd3d<8, 8>)) PW_OCC(5, (k_afd3d<4, 4>))
         PW_OCC(6, (k_afd3d_lean<2, 8>)) PW_OCC(7, (k_afd3d_lean<1, 8>)) PW_OCC(8, (k_afd3d_lean<4, 4>))
         PW_OCC(9, (k_afd3d_lean<4, 8>)) PW_OCC(10, (k_afd3d_lean<1, 16>)) PW_OCC(11, (k_afd3d_lean<2, 16>))
-        PW_OCC(12, (k_afd3d_pipe<2, 8, 0>)) PW_OCC(13, (k_afd3d_pipe<4, 4, 0>)) PW_OCC(14, (k_afd3d_pipe<4, 8, 0>))
-        PW_OCC(15, (k_afd3d_pipe<2, 4, 0>)) PW_OCC(16, (k_afd3d_pipe<2, 8, 1>)) PW_OCC(17, (k_afd3d_pipe<4, 4, 1>))
-        PW_OCC(18, (k_afd3d_pipe<2, 16, 0>)) PW_OCC(19, (k_afd3d_pipe<1, 8, 0>))
+        PW_OCC(12, (k_afd3d_pipe<2, 8, 0, 0>)) PW_OCC(13, (k_afd3d_pipe<4, 4, 0, 0>))
+        PW_OCC(14, (k_afd3d_pipe<4, 8, 0, 0>)) PW_OCC(15, (k_afd3d_pipe<2, 4, 0, 0>))
+        PW_OCC(16, (k_afd3d_pipe<2, 8, 1, 0>)) PW_OCC(17, (k_afd3d_pipe<4, 4, 1, 0>))
+        PW_OCC(18, (k_afd3d_pipe<2, 16, 0, 0>)) PW_OCC(19, (k_afd3d_pipe<1, 8, 0, 0>))
+        PW_OCC(20, (k_afd3d_pipe<4, 4, 0, 1>)) PW_OCC(21, (k_afd3d_pipe<2, 4, 0, 1>))
+        PW_OCC(22, (k_afd3d_pipe<2, 8, 0, 1>)) PW_OCC(23, (k_afd3d_pipe<4, 8, 0, 1>))
 #undef PW_OCC
         default: break;
     }
@@ -792,7 +811,8 @@ long long resident_blocks(pw_afd* p, int vi)
     return p->resident[vi];
 }
 
-int afd_launch_step(pw_afd* p, const Step& a)
+// e0/e1 (optional): events stamped by the dispatch itself at kernel start and end
+int afd_launch_step(pw_afd* p, const Step& a, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr)
 {
     pw_ctx* ctx = p->ctx;
     hipStream_t s = ctx->s();
@@ -800,7 +820,7 @@ int afd_launch_step(pw_afd* p, const Step& a)
     if (p->ndim == 2) {
         constexpr int W = 4;
         dim3 grid(pw::ceil_div(g.nz, 64), pw::ceil_div(g.nx, W));
-        hipLaunchKernelGGL(k_afd2d<W>, grid, dim3(64 * W), 0, s, g, a);
+        hipExtLaunchKernelGGL(k_afd2d<W>, grid, dim3(64 * W), 0, s, e0, e1, 0, g, a);
     } else {
         const int vi = p->variant >= 0 ? p->variant : (p->lean_ok ? kLeanDefault : kGeneralDefault);
         const Variant3 v = kVariants3[vi];
@@ -821,8 +841,8 @@ int afd_launch_step(pw_afd* p, const Step& a)
         switch (vi) {
 #define PW_V3(I, TX_, W_)                                                                          \
     case I:                                                                                        \
-        hipLaunchKernelGGL((k_afd3d<TX_, W_>), dim3(grid), dim3(64 * W_), 0, s, g, a, ntz, ntx, nyc, \
-                           nblk);                                                                  \
+        hipExtLaunchKernelGGL((k_afd3d<TX_, W_>), dim3(grid), dim3(64 * W_), 0, s, e0, e1, 0, g, a, ntz, ntx, \
+                              nyc, nblk);                                                                  \
         break;
             PW_V3(0, 16, 4)
             PW_V3(1, 8, 4)
@@ -833,8 +853,8 @@ int afd_launch_step(pw_afd* p, const Step& a)
 #undef PW_V3
 #define PW_VL(I, TX_, W_)                                                                          \
     case I:                                                                                        \
-        hipLaunchKernelGGL((k_afd3d_lean<TX_ / W_, W_>), dim3(grid), dim3(64 * W_), 0, s, g, a, ntz, ntx, \
-                           nyc, nblk);                                                             \
+        hipExtLaunchKernelGGL((k_afd3d_lean<TX_ / W_, W_>), dim3(grid), dim3(64 * W_), 0, s, e0, e1, 0, g, a, \
+                              ntz, ntx, nyc, nblk);                                                             \
         break;
             PW_VL(6, 16, 8)
             PW_VL(7, 8, 8)
@@ -843,19 +863,23 @@ int afd_launch_step(pw_afd* p, const Step& a)
             PW_VL(10, 16, 16)
             PW_VL(11, 32, 16)
 #undef PW_VL
-#define PW_VP(I, TX_, W_, NT_)                                                                      \
+#define PW_VP(I, TX_, W_, NT_, ORD_)                                                                \
     case I:                                                                                         \
-        hipLaunchKernelGGL((k_afd3d_pipe<TX_ / W_, W_, NT_>), dim3(grid), dim3(64 * W_), 0, s, g, a, ntz, ntx, \
-                           nyc, nblk);                                                              \
+        hipExtLaunchKernelGGL((k_afd3d_pipe<TX_ / W_, W_, NT_, ORD_>), dim3(grid), dim3(64 * W_), 0, s, e0, e1, \
+                              0, g, a, ntz, ntx, nyc, nblk);                                                         \
         break;
-            PW_VP(12, 16, 8, 0)
-            PW_VP(13, 16, 4, 0)
-            PW_VP(14, 32, 8, 0)
-            PW_VP(15, 8, 4, 0)
-            PW_VP(16, 16, 8, 1)
-            PW_VP(17, 16, 4, 1)
-            PW_VP(18, 32, 16, 0)
-            PW_VP(19, 8, 8, 0)
+            PW_VP(12, 16, 8, 0, 0)
+            PW_VP(13, 16, 4, 0, 0)
+            PW_VP(14, 32, 8, 0, 0)
+            PW_VP(15, 8, 4, 0, 0)
+            PW_VP(16, 16, 8, 1, 0)
+            PW_VP(17, 16, 4, 1, 0)
+            PW_VP(18, 32, 16, 0, 0)
+            PW_VP(19, 8, 8, 0, 0)
+            PW_VP(20, 16, 4, 0, 1)
+            PW_VP(21, 8, 4, 0, 1)
+            PW_VP(22, 16, 8, 0, 1)
+            PW_VP(23, 32, 8, 0, 1)
 #undef PW_VP
             default:
                 PW_FAIL(ctx, PW_ERR_ARG, "unknown 3D kernel variant");
@@ -920,8 +944,10 @@ pw_afd* pw_afd_create(pw_ctx* ctx, int ndim, int nz, int nx, int ny, float dz, f
     g.nz = nz + 2 * nbt; g.nx = nx + 2 * nbt; g.ny = ndim == 3 ? ny + 2 * nbt : 1;
     pw::padded_sizes(g.nz, g.nx, g.ny, &g.nz2, &g.nx2, &g.ny2);
     if (ndim == 2) g.ny2 = 1;
-    // 3D: z stride a multiple of 64 so a 64-lane z tile never leaves its row
-    g.ldz = pw::ceil_div(g.nz2, ndim == 3 ? 64 : LDZ_ALIGN) * (ndim == 3 ? 64 : LDZ_ALIGN);
+    // z stride: rows start on 128 B boundaries.  A 64-lane z tile may run past the end of a row
+    // (into the next row or the guard); such lanes lie outside the updated domain and are masked
+    // or only feed other masked lanes.
+    g.ldz = pw::ceil_div(g.nz2, LDZ_ALIGN) * LDZ_ALIGN;
     g.ldxy = (long long)g.ldz * g.nx2;
     g.idz = 1 / (dz * dz);
     g.idx = 1 / (dx * dx);
@@ -955,7 +981,7 @@ pw_afd* pw_afd_create(pw_ctx* ctx, int ndim, int nz, int nx, int ny, float dz, f
     };
     std::vector<float> fz1, fx1, fy1;
     bool lean = ndim == 3;
-    lean = lean && single(fz, (size_t)g.ldz, fz1);
+    lean = lean && single(fz, (size_t)pw::ceil_div(g.nz2, 64) * 64 + 64, fz1);  // every lane of every z tile
     lean = lean && single(fx, (size_t)g.nx2 + kMaxTx + 8, fx1);
     lean = lean && single(fy, (size_t)g.ny2 + 8, fy1);
     lean = lean && (double)p->raw_elems < 2.0e9;  // 32-bit offsets
@@ -1126,10 +1152,8 @@ static int afd_enqueue(pw_afd* p, int tri, hipEvent_t* evs)
         a.tri = tri ? p->d_tri + (size_t)it * nrec : nullptr;
         a.rec = tri ? nullptr : p->d_rec + (size_t)it * nrec;
         a.snap = (p->jsnap > 0 && it % p->jsnap == 0) ? p->d_snap + (size_t)(it / p->jsnap) * snap_elems : nullptr;
-        if (evs) PW_HIP(ctx, hipEventRecord(evs[2 * k], s));
-        int rc = afd_launch_step(p, a);
+        int rc = afd_launch_step(p, a, evs ? evs[2 * k] : nullptr, evs ? evs[2 * k + 1] : nullptr);
         if (rc) return rc;
-        if (evs) PW_HIP(ctx, hipEventRecord(evs[2 * k + 1], s));
         std::swap(A, B);
     }
     p->final_field = A;
@@ -1244,7 +1268,7 @@ int pw_afd_autotune(pw_afd* p, int steps, int* variant_out, int* chunk_out)
         hipEvent_t e0, e1;
         PW_HIP(ctx, hipEventCreate(&e0));
         PW_HIP(ctx, hipEventCreate(&e1));
-        const int chunks[] = {0, 8, 16, 32, 64};
+        const int chunks[] = {0, 4, 6, 8, 12, 16, 32};
         float best = 1e30f;
         std::pair<int, int> choice(p->lean_ok ? kLeanDefault : kGeneralDefault, 0);
         const int saved_v = p->variant, saved_c = p->ny_chunk;

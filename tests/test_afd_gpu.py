@@ -51,7 +51,7 @@ def test_afd2d_tri_golden_bit_exact(golden):
     assert np.array_equal(p.final_field(), w0)
 
 
-NVAR = 12  # pw_afd_variant_count(): 6 general + 6 lean 3D step kernels
+NVAR = 24  # pw_afd_variant_count(): 6 general, 6 lean and 12 pipelined 3D step kernels
 
 
 @pytest.mark.parametrize("variant", range(NVAR))
