@@ -173,15 +173,24 @@ def ricker_wavelet(nt, dt, f0, t0):
 
 
 def calibrate_copy():
-    """One elementwise copy of 2^28 float32 (1 GiB read + 1 GiB written), three times: the known
-    traffic against which FETCH_SIZE / WRITE_SIZE counter readings are scaled."""
+    """Copies of 2^28 float32 (1 GiB read + 1 GiB written): the known traffic against which
+    FETCH_SIZE / WRITE_SIZE counter readings are scaled, and the device's practical copy
+    bandwidth (read + write bytes / time, 10 timed copies)."""
     import torch
     a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
     b = torch.empty_like(a)
     for _ in range(3):
         b.copy_(a)
     torch.cuda.synchronize()
-    print(json.dumps({"copy_bytes_read": a.numel() * 4, "copy_bytes_written": b.numel() * 4}))
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(10):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / 10
+    print(json.dumps({"copy_bytes_read": a.numel() * 4, "copy_bytes_written": b.numel() * 4,
+                      "copy_ms": round(ms, 4), "copy_GBs": round(2 * a.numel() * 4 / (ms * 1e-3) / 1e9, 1)}))
 
 
 def load_traffic(name, variant):
@@ -246,6 +255,7 @@ def main():
         tuned = plan.autotune(steps=10)
     ctx = N.context()
     L = N.lib()
+    used = [args.variant, args.ny_chunk] if args.variant >= 0 else (list(tuned) if tuned else None)
 
     for _ in range(args.warmup):
         plan.run()
@@ -262,12 +272,19 @@ def main():
     # launch on the library stream (one extra run)
     total_ms, step_ms = plan.run_timed(per_step=True)
     nrec = nx * ny
-    alg_bytes = 16.0 * cells + 4.0 * nrec  # read u_n, u_{n-1}, vv, write u_{n+1}; trace sample
-    achieved = alg_bytes / (step_ms * 1e-3) / 1e9
+    # per time step: read u_n, u_{n-1}, vv, write u_{n+1}; one trace sample per receiver
+    alg_step = 16.0 * cells + 4.0 * nrec
+    spl = 1  # time steps per launch of the step kernel in use
+    if used is not None:
+        info = np.zeros(4, np.int32)
+        N.check(L.pw_afd_variant_info(int(used[0]), N.ptr(info)), ctx, "variant_info")
+        spl = int(info[3])
+    launch_ms = step_ms * spl  # step_ms is the dispatch-stamped kernel time per time step
+    alg_bytes = alg_step * spl
+    achieved = alg_bytes / (launch_ms * 1e-3) / 1e9
     out = plan.output()
     assert np.isfinite(out).all()
 
-    used = [args.variant, args.ny_chunk] if args.variant >= 0 else (list(tuned) if tuned else None)
     cpu = None if args.no_cpu else cpu_baseline(w, args.cpu_steps)
     fwi = None if args.no_fwi else fwi_bench(args.fwi_plain_shots)
     tri = None if args.no_tri else tri_bench()
@@ -300,7 +317,8 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / PEAK_HBM_GBS, 4),
             "traffic": load_traffic("afd3d_step_traffic.json", used),
-            "kernel_avg_ms": round(step_ms, 5),
+            "kernel_avg_ms": round(launch_ms, 5),
+            "time_steps_per_launch": spl,
             "algorithmic_bytes_per_launch": alg_bytes,
         },
         "cpu_baseline": cpu,

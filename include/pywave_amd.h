@@ -92,6 +92,10 @@ int pw_afd_final_field(pw_afd *p, float *out, long long out_len);
 /* Kernel tile choice for the 3D step (0 = default); returns PW_ERR_ARG for unknown ids. */
 int pw_afd_set_variant(pw_afd *p, int variant, int ny_chunk);
 int pw_afd_variant_count(void);
+/* info[4] = {rows per block (step-1 rows for two-step kernels), waves per block, kind (0 general,
+ * 1 lean, 2 pipelined, 3 pipelined non-temporal, 4 pipelined x-major, 5 two steps per launch),
+ * time steps per launch}. No counterpart in the reference (tooling for the kernel choice). */
+int pw_afd_variant_info(int variant, int *info);
 /* Autotuner: times each 3D kernel variant and y-chunk on `steps` steps of this plan, keeps the
  * fastest (cached per padded grid shape in the context); no-op for 2D plans. */
 int pw_afd_autotune(pw_afd *p, int steps, int *variant_out, int *chunk_out);

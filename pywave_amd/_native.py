@@ -50,6 +50,7 @@ SIGNATURES = {
     "pw_afd_final_field": (c_int, [c_vp, c_vp, c_ll]),
     "pw_afd_set_variant": (c_int, [c_vp, c_int, c_int]),
     "pw_afd_variant_count": (c_int, []),
+    "pw_afd_variant_info": (c_int, [c_int, c_vp]),
     "pw_afd_autotune": (c_int, [c_vp, c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int)]),
     "pw_afd_destroy": (None, [c_vp]),
     "pw_fwi_create": (c_vp, [c_vp, c_vp, c_int, c_int]),

@@ -506,6 +506,254 @@ __global__ __launch_bounds__(64 * WAVES) void k_afd3d_pipe(Geo g, Step a, int nt
     }
 }
 
+// ---------------------------------------------------------------- 3D, two time steps per launch
+// Temporal blocking of the pipelined kernel: one launch advances C_n, C_{n-1} to C_{n+2}, C_{n+1}.
+// A block owns an output tile of 60 z x (TX1 - 4) x rows over a y chunk.  Streaming along y, at
+// plane j it computes step 1 (C_{n+1}) on the tile grown by 2 cells in z and x (64 lanes x TX1
+// rows, from C_n staged in LDS with a further 2-cell halo) and, two planes behind, step 2
+// (C_{n+2}) at plane j-2 from the step-1 values kept in an LDS ring of three planes (z, x
+// neighbours) and in a 5-plane register queue (y neighbours).  The grown ring of step 1 is
+// recomputed by neighbouring blocks instead of exchanged; every cell of both steps uses the
+// single-step arithmetic, so results are bit-identical.  C_{n+1} and C_{n+2} go to two other
+// buffers (neighbours still read C_n and C_{n-1} around the tile).  HBM traffic per two steps:
+// C_n and C_{n-1} and vv read once (plus halos), C_{n+1} and C_{n+2} written once.
+struct Step2 {
+    const float* cur;  // C_n
+    const float* pv;   // C_{n-1} as stored
+    const float* vv;
+    float* out1;       // C_{n+1}
+    float* out2;       // C_{n+2}
+    const float* __restrict__ fz1;
+    const float* __restrict__ fx1;
+    const float* __restrict__ fy1;
+    const int* __restrict__ src;
+    const float* __restrict__ wav;
+    const float* __restrict__ tri1;  // receiver planes of the two steps (time reversal) or null
+    const float* __restrict__ tri2;
+    float* __restrict__ rec1;  // receiver planes of the two steps or null
+    float* __restrict__ rec2;
+    float* __restrict__ snap1;  // snapshots of the two steps or null
+    float* __restrict__ snap2;
+    int ns, nt, it1, it2;
+};
+
+__device__ __forceinline__ float inject_at(float c, float vvc, const int* __restrict__ src,
+                                           const float* __restrict__ wav, int ns, int nt, int it, int z, int x,
+                                           int y)
+{
+    for (int s = 0; s < ns; ++s) {
+        int dz = z - src[3 * s], dx = x - src[3 * s + 1], dy = y - src[3 * s + 2];
+        int adz = dz < 0 ? -dz : dz, adx = dx < 0 ? -dx : dx, ady = dy < 0 ? -dy : dy;
+        if (adz <= 1 && adx <= 1 && ady <= 1) {
+            float k = (float)(adz + adx + ady + 1);
+            c = c + (vvc * wav[(size_t)s * nt + it]) / k;
+        }
+    }
+    return c;
+}
+
+template <int RPW, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_afd3d_2step(Geo g, Step2 a, int ntz, int ntx, int nyc, int nblk)
+{
+    static_assert(WAVES >= 4, "one x-halo row per wave for waves 0..3");
+    constexpr int TX1 = RPW * WAVES;  // step-1 rows
+    constexpr int TXO = TX1 - 4;      // output rows
+    constexpr int TZO = 60;           // output z per tile
+    static_assert(TXO >= 4, "tile too narrow");
+    __shared__ float lc[2][TX1 + 4][68];  // C_n of plane j: z from zs-4, x from xs-4
+    __shared__ float l1[3][TX1][64];      // step-1 planes j-2, j-1, j (ring)
+    const int per = gridDim.x >> 3;
+    const int lb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (lb >= nblk) return;
+    const int tz = lb % ntz;
+    const int rest = lb / ntz;
+    const int tx = rest % ntx;
+    const int ch = rest / ntx;
+    const int zs = tz * TZO, xs = tx * TXO;
+    const int y0 = ch * nyc, y1 = min(y0 + nyc, g.ny);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int z = zs - 2 + lane;
+    const int ldz = g.ldz;
+    const long long ldxy = g.ldxy;
+    const int bias = 4 * ldz + 4;  // plane windows reach 4 rows + 4 cells before the plane
+    const unsigned wbytes = (unsigned)((ldxy + 2 * bias) * 4);
+    const bool zact = z >= 0 && z < g.nz;
+    const bool zout = lane >= 2 && lane < 62 && z < g.nz;
+    const float fz = a.fz1[z < 0 ? 0 : z];
+    const bool zin = z >= 2 && z < g.nz2 - 2;
+    const int zh = lane < 2 ? zs - 4 + lane : zs + 60 + lane;  // cur z halo of lanes 0..3
+    const int lzh = lane < 2 ? lane : 64 + lane;
+    const int cl = lane < 2 ? 2 : (lane > 61 ? 61 : lane);  // step-2 z neighbours stay in the ring row
+    unsigned oc[RPW], ohz[RPW];
+    float fx[RPW];
+    bool xin[RPW], xout[RPW];
+    int xr[RPW];
+#pragma unroll
+    for (int k = 0; k < RPW; ++k) {
+        const int r = w * RPW + k;
+        const int x = xs - 2 + r;
+        xr[k] = x;
+        const bool xact = x >= 0 && x < g.nx;
+        oc[k] = (zact && xact) ? (unsigned)(bias + x * ldz + z) * 4u : kOOB;
+        ohz[k] = (lane < 4 && zh >= 0 && zh < g.nz && xact) ? (unsigned)(bias + x * ldz + zh) * 4u : kOOB;
+        fx[k] = a.fx1[x < 0 ? 0 : x];
+        xin[k] = x >= 2 && x < g.nx2 - 2;
+        xout[k] = r >= 2 && r < TX1 - 2 && x < g.nx;
+    }
+    const int xh = w < 2 ? xs - 4 + w : xs - 4 + TX1 + w;  // cur x halo row of waves 0..3
+    const int lrh = w < 2 ? w : TX1 + w;
+    const unsigned ohx = (w < 4 && xh >= 0 && xh < g.nx && zact) ? (unsigned)(bias + xh * ldz + z) * 4u : kOOB;
+    bool blk_src = false;
+    for (int s = 0; s < a.ns; ++s) {
+        int gz = a.src[3 * s], gx = a.src[3 * s + 1], gy = a.src[3 * s + 2];
+        blk_src |= gz >= zs - 3 && gz <= zs + 62 && gx >= xs - 3 && gx <= xs + TX1 - 2 && gy >= y0 - 3 && gy <= y1 + 2;
+    }
+    auto plane = [&](const float* f, int yy) { return plane_rsrc(f + yy * ldxy - bias, wbytes); };
+    auto yok = [&](int yy) { return yy >= 0 && yy < g.ny; };
+    auto fyat = [&](int yy) { return a.fy1[yy < 0 ? 0 : (yy > g.ny2 ? g.ny2 : yy)]; };
+
+    // step-1 plane j runs over [y0-2, y1+1]; step 2 trails it by two planes
+    const int j0 = y0 - 2, j1 = y1 + 1;
+    float q[RPW][5], s1q[RPW][5], vq[RPW][3];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        const int yy = j0 - 2 + k;
+        const auto rs = plane(a.cur, yy);
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) q[r][k] = bld<0>(rs, oc[r], yok(yy));
+    }
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) s1q[r][k] = 0.f;
+        vq[r][0] = vq[r][1] = vq[r][2] = 0.f;
+    }
+    float pr[RPW], vc[RPW], hz[RPW], hx;
+    float fyq0 = 1.f, fyq1 = 1.f, fyv = fyat(j0);
+    {
+        const bool v = yok(j0);
+        const auto rc = plane(a.cur, j0), rp = plane(a.pv, j0), rv = plane(a.vv, j0);
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) {
+            pr[r] = bld<0>(rp, oc[r], v);
+            vc[r] = bld<0>(rv, oc[r], v);
+            hz[r] = bld<0>(rc, ohz[r], v);
+        }
+        hx = bld<0>(rc, ohx, v);
+    }
+    int cb = 0, rb = 0;  // cur double buffer, step-1 ring slot of plane j
+    for (int j = j0; j <= j1; ++j) {
+        // prefetch plane j+1 (and the centre of j+3)
+        const int jn = j + 1;
+        const bool nv = jn <= j1 && yok(jn);
+        const bool n3 = jn <= j1 && yok(j + 3);
+        float npr[RPW], nvc[RPW], nhz[RPW], nq[RPW], nhx;
+        const float nfy = fyat(jn);
+        {
+            const auto rc = plane(a.cur, jn), rp = plane(a.pv, jn), rv = plane(a.vv, jn), r3 = plane(a.cur, j + 3);
+#pragma unroll
+            for (int r = 0; r < RPW; ++r) {
+                nq[r] = bld<0>(r3, oc[r], n3);
+                npr[r] = bld<0>(rp, oc[r], nv);
+                nvc[r] = bld<0>(rv, oc[r], nv);
+                nhz[r] = bld<0>(rc, ohz[r], nv);
+            }
+            nhx = bld<0>(rc, ohx, nv);
+        }
+        float(*C)[68] = lc[cb];
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) C[w * RPW + r + 2][lane + 2] = q[r][2];
+        if (lane < 4) {
+#pragma unroll
+            for (int r = 0; r < RPW; ++r) C[w * RPW + r + 2][lzh] = hz[r];
+        }
+        if (w < 4) C[lrh][lane + 2] = hx;
+        __syncthreads();
+        // ---- step 1 at plane j
+        const bool jown = j >= y0 && j < y1;
+        const bool yin = j >= 2 && j < g.ny2 - 2;
+        const auto w1 = plane(a.out1, j);
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) {
+            const int lr = w * RPW + r + 2;
+            const int x = xr[r];
+            const float u = q[r][2];
+            float lap = C0 * u * g.cc
+                      + C1 * (q[r][1] + q[r][3]) * g.idy
+                      + C2 * (q[r][0] + q[r][4]) * g.idy
+                      + C1 * (C[lr - 1][lane + 2] + C[lr + 1][lane + 2]) * g.idx
+                      + C2 * (C[lr - 2][lane + 2] + C[lr + 2][lane + 2]) * g.idx
+                      + C1 * (C[lr][lane + 1] + C[lr][lane + 3]) * g.idz
+                      + C2 * (C[lr][lane] + C[lr][lane + 4]) * g.idz;
+            lap = (yin && zin && xin[r]) ? lap : 0.f;
+            const float p = pr[r] * fz * fx[r] * fyv;
+            float c = u + (u - p);
+            c = c + lap * vc[r];
+            if (blk_src) c = inject_at(c, vc[r], a.src, a.wav, a.ns, a.nt, a.it1, z, x, j);
+            const bool onrec = z == g.nb && x >= g.nb && x < g.nb + g.nx1 && j >= g.nb && j < g.nb + g.ny1;
+            if (a.tri1 && onrec) c = c + vc[r] * a.tri1[(j - g.nb) * g.nx1 + (x - g.nb)];
+            c = c * fz * fx[r] * fyv;
+            l1[rb][lr - 2][lane] = c;
+            const bool own = jown && zout && xout[r];
+            bst<0>(c, w1, own ? oc[r] : kOOB);
+            if (a.rec1 && own && onrec) a.rec1[(j - g.nb) * g.nx1 + (x - g.nb)] = c;
+            if (a.snap1 && own && z >= g.nb && z < g.nb + g.nz1 && x >= g.nb && x < g.nb + g.nx1 && j >= g.nb &&
+                j < g.nb + g.ny1)
+                a.snap1[((size_t)(j - g.nb) * g.nx1 + (x - g.nb)) * g.nz1 + (z - g.nb)] = c;
+            s1q[r][0] = s1q[r][1]; s1q[r][1] = s1q[r][2]; s1q[r][2] = s1q[r][3]; s1q[r][3] = s1q[r][4];
+            s1q[r][4] = c;
+            vq[r][0] = vq[r][1]; vq[r][1] = vq[r][2]; vq[r][2] = vc[r];
+        }
+        // ---- step 2 at plane m = j-2 (step-1 planes m-2..m+2 are in the queue; the ring slot of
+        // plane m was written two iterations ago and published by the last barrier)
+        const int m = j - 2;
+        if (m >= y0) {
+            const float fym = fyq0;
+            const bool yin2 = m >= 2 && m < g.ny2 - 2;
+            const float(*S)[64] = l1[rb == 0 ? 1 : (rb == 1 ? 2 : 0)];  // slot (j-2) % 3 == (rb + 1) % 3
+            const auto w2 = plane(a.out2, m);
+#pragma unroll
+            for (int r = 0; r < RPW; ++r) {
+                const int lr = w * RPW + r;
+                if (lr < 2 || lr >= TX1 - 2) continue;  // uniform per wave
+                const int x = xr[r];
+                const float u = s1q[r][2];
+                float lap = C0 * u * g.cc
+                          + C1 * (s1q[r][1] + s1q[r][3]) * g.idy
+                          + C2 * (s1q[r][0] + s1q[r][4]) * g.idy
+                          + C1 * (S[lr - 1][cl] + S[lr + 1][cl]) * g.idx
+                          + C2 * (S[lr - 2][cl] + S[lr + 2][cl]) * g.idx
+                          + C1 * (S[lr][cl - 1] + S[lr][cl + 1]) * g.idz
+                          + C2 * (S[lr][cl - 2] + S[lr][cl + 2]) * g.idz;
+                lap = (yin2 && zin && xin[r]) ? lap : 0.f;
+                const float vm = vq[r][0];
+                const float p = q[r][0] * fz * fx[r] * fym;  // C_n at plane m
+                float c = u + (u - p);
+                c = c + lap * vm;
+                if (blk_src) c = inject_at(c, vm, a.src, a.wav, a.ns, a.nt, a.it2, z, x, m);
+                const bool onrec = z == g.nb && x >= g.nb && x < g.nb + g.nx1 && m >= g.nb && m < g.nb + g.ny1;
+                if (a.tri2 && onrec) c = c + vm * a.tri2[(m - g.nb) * g.nx1 + (x - g.nb)];
+                c = c * fz * fx[r] * fym;
+                const bool own = zout && xout[r];
+                bst<0>(c, w2, own ? oc[r] : kOOB);
+                if (a.rec2 && own && onrec) a.rec2[(m - g.nb) * g.nx1 + (x - g.nb)] = c;
+                if (a.snap2 && own && z >= g.nb && z < g.nb + g.nz1 && x >= g.nb && x < g.nb + g.nx1 && m >= g.nb &&
+                    m < g.nb + g.ny1)
+                    a.snap2[((size_t)(m - g.nb) * g.nx1 + (x - g.nb)) * g.nz1 + (z - g.nb)] = c;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) {
+            q[r][0] = q[r][1]; q[r][1] = q[r][2]; q[r][2] = q[r][3]; q[r][3] = q[r][4]; q[r][4] = nq[r];
+            pr[r] = npr[r]; vc[r] = nvc[r]; hz[r] = nhz[r];
+        }
+        hx = nhx;
+        fyq0 = fyq1; fyq1 = fyv; fyv = nfy;
+        cb ^= 1;
+        rb = rb == 2 ? 0 : rb + 1;
+    }
+}
+
 // ---------------------------------------------------------------- 2D: one cell per lane
 template <int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void k_afd2d(Geo g, Step a)
@@ -601,6 +849,10 @@ struct TriStep {
     const float* __restrict__ dat;          // this step's receiver plane [ny1][nx1]
     const unsigned char* __restrict__ mask;  // [G][ny1][nx1]: receiver belongs to group
     double* __restrict__ img;                // [ny1][nx1][nz1] or null (no image this step)
+    const unsigned char* __restrict__ mbits; // [ny1][nx1]: bit f set when the receiver is in group f
+    const float* __restrict__ fz1;           // one damping factor per index (faces do not overlap)
+    const float* __restrict__ fx1;
+    const float* __restrict__ fy1;
 };
 
 template <int G>
@@ -726,19 +978,175 @@ __global__ __launch_bounds__(256) void k_tri3d(Geo g, TriStep<G> a, int ntz, int
     }
 }
 
+// Pipelined form of k_tri3d (the layout and arithmetic of k_afd3d_pipe, G fields per cell): the
+// loads of plane y+1 (prev of every field, vv, halos, the receiver sample and group bits, the image
+// cell) and the centre columns of plane y+3 are in flight while plane y is computed; cells outside
+// the updated domain, receivers off the row and the image outside the model are masked through the
+// buffer range check.  TX = WAVES rows per block, one per wave.
+template <int G, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_tri3d_pipe(Geo g, TriStep<G> a, int ntz, int ntx, int nyc,
+                                                           int nblk)
+{
+    static_assert(WAVES >= 4, "one x-halo row per wave for waves 0..3");
+    constexpr int TZ = 64, TX = WAVES, LW = TZ + 4;
+    __shared__ float lds[G][2][TX + 4][LW];
+    const int per = gridDim.x >> 3;
+    const int lb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (lb >= nblk) return;
+    const int tz = lb % ntz;
+    const int rest = lb / ntz;
+    const int tx = rest % ntx;
+    const int ch = rest / ntx;
+    const int z0 = tz * TZ, x0 = tx * TX;
+    const int y0 = ch * nyc, y1 = min(y0 + nyc, g.ny);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int z = z0 + lane, x = x0 + w;
+    const int ldz = g.ldz;
+    const long long ldxy = g.ldxy;
+    const int bias = 2 * ldz + 2;
+    const unsigned wbytes = (unsigned)((ldxy + 2 * bias) * 4);
+    const bool zact = z < g.nz, xact = x < g.nx;
+    const unsigned oc = (zact && xact) ? (unsigned)(bias + x * ldz + z) * 4u : kOOB;
+    const int zh = lane < 2 ? z0 - 2 + lane : z0 + TZ - 2 + lane;
+    const int lzh = lane < 2 ? lane : TZ + lane;
+    const unsigned ohz = (lane < 4 && zh >= 0 && zh < g.nz && xact) ? (unsigned)(bias + x * ldz + zh) * 4u : kOOB;
+    const int xh = w < 2 ? x0 - 2 + w : x0 + TX - 2 + w;
+    const int lrh = w < 2 ? w : TX + w;
+    const unsigned ohx = (w < 4 && xh >= 0 && xh < g.nx && zact) ? (unsigned)(bias + xh * ldz + z) * 4u : kOOB;
+    const float fz = a.fz1[z], fx = a.fx1[x];
+    const bool zxin = z >= 2 && z < g.nz2 - 2 && x >= 2 && x < g.nx2 - 2;
+    // receiver row (z == nb) and image interior, per (z, x); the y range is checked per plane
+    const bool rx = x >= g.nb && x < g.nb + g.nx1;
+    const unsigned orc = (z == g.nb && rx) ? (unsigned)(x - g.nb) : kOOB;  // element index in the row
+    const bool ins = zact && rx && z >= g.nb && z < g.nb + g.nz1;
+    const unsigned oim = ins ? (unsigned)((x - g.nb) * g.nz1 + (z - g.nb)) * 8u : kOOB;
+    const unsigned rbytes = (unsigned)(g.nx1 * 4), mbytes = (unsigned)g.nx1, ibytes = (unsigned)(g.nx1 * g.nz1 * 8);
+
+    auto fplane = [&](const float* f, int yy) { return plane_rsrc(f + yy * ldxy - bias, wbytes); };
+    auto yrec = [&](int yy) { return yy >= g.nb && yy < g.nb + g.ny1; };
+    // resource bases depend on the (uniform) plane only; the lane's receiver goes in the offset
+    auto ld_dat = [&](int yy) {
+        const bool yr = yrec(yy);
+        const auto r = plane_rsrc(a.dat + (long long)(yr ? yy - g.nb : 0) * g.nx1, rbytes);
+        return bld<0>(r, orc * 4u, yr && orc != kOOB);
+    };
+    auto ld_bits = [&](int yy) -> unsigned {
+        const bool yr = yrec(yy);
+        const auto r = __builtin_amdgcn_make_buffer_rsrc((void*)(a.mbits + (long long)(yr ? yy - g.nb : 0) * g.nx1),
+                                                         (short)0, (int)mbytes, 0x00020000);
+        return __builtin_amdgcn_raw_buffer_load_b8(r, (yr && orc != kOOB) ? orc : kOOB, 0, 0);
+    };
+    auto img_rsrc = [&](int yy) {
+        return __builtin_amdgcn_make_buffer_rsrc((void*)(a.img + (long long)(yrec(yy) ? yy - g.nb : 0) * g.nx1 * g.nz1),
+                                                 (short)0, (int)ibytes, 0x00020000);
+    };
+    auto ld_img = [&](int yy) -> double {
+        const bool ok = a.img && yrec(yy) && oim != kOOB;
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(img_rsrc(yy), ok ? oim : kOOB, 0, 0));
+    };
+
+    float q[G][5], pr[G], hz[G], hx[G];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        const int yy = y0 - 2 + k;
+        const bool yv = yy >= 0 && yy < g.ny;
+#pragma unroll
+        for (int f = 0; f < G; ++f) q[f][k] = bld<0>(fplane(a.cur[f], yy), oc, yv);
+    }
+#pragma unroll
+    for (int f = 0; f < G; ++f) {
+        const auto rc = fplane(a.cur[f], y0);
+        pr[f] = bld<0>(fplane(a.pv[f], y0), oc, true);
+        hz[f] = bld<0>(rc, ohz, true);
+        hx[f] = bld<0>(rc, ohx, true);
+    }
+    float vc = bld<0>(fplane(a.vv, y0), oc, true);
+    float dv = ld_dat(y0);
+    unsigned bits = ld_bits(y0);
+    double im = ld_img(y0);
+    float fyv = a.fy1[y0];
+    int buf = 0;
+    for (int y = y0; y < y1; ++y) {
+        const int yn = y + 1;
+        const bool nv = yn < y1;
+        const bool n3 = nv && y + 3 < g.ny;
+        float nq[G], npr[G], nhz[G], nhx[G];
+#pragma unroll
+        for (int f = 0; f < G; ++f) {
+            const auto rc = fplane(a.cur[f], yn);
+            nq[f] = bld<0>(fplane(a.cur[f], y + 3), oc, n3);
+            npr[f] = bld<0>(fplane(a.pv[f], yn), oc, nv);
+            nhz[f] = bld<0>(rc, ohz, nv);
+            nhx[f] = bld<0>(rc, ohx, nv);
+        }
+        const float nvc = bld<0>(fplane(a.vv, yn), oc, nv);
+        const float ndv = ld_dat(yn);
+        const unsigned nbits = ld_bits(yn);
+        const double nim = ld_img(yn);
+        const float nfy = a.fy1[yn];
+#pragma unroll
+        for (int f = 0; f < G; ++f) {
+            float(*L)[LW] = lds[f][buf];
+            L[w + 2][lane + 2] = q[f][2];
+            if (lane < 4) L[w + 2][lzh] = hz[f];
+            if (w < 4) L[lrh][lane + 2] = hx[f];
+        }
+        __syncthreads();
+        const bool inner = zxin && y >= 2 && y < g.ny2 - 2;
+        const bool onrec = orc != kOOB && yrec(y);
+        float prod = 1.f;
+#pragma unroll
+        for (int f = 0; f < G; ++f) {
+            const float(*L)[LW] = lds[f][buf];
+            const int lr = w + 2;
+            const float u = q[f][2];
+            float lap = C0 * u * g.cc
+                      + C1 * (q[f][1] + q[f][3]) * g.idy
+                      + C2 * (q[f][0] + q[f][4]) * g.idy
+                      + C1 * (L[lr - 1][lane + 2] + L[lr + 1][lane + 2]) * g.idx
+                      + C2 * (L[lr - 2][lane + 2] + L[lr + 2][lane + 2]) * g.idx
+                      + C1 * (L[lr][lane + 1] + L[lr][lane + 3]) * g.idz
+                      + C2 * (L[lr][lane] + L[lr][lane + 4]) * g.idz;
+            lap = inner ? lap : 0.f;
+            const float p = pr[f] * fz * fx * fyv;
+            float c = u + (u - p);
+            c = c + lap * vc;
+            if (onrec && ((bits >> f) & 1u)) c = c + vc * dv;
+            c = c * fz * fx * fyv;
+            bst<0>(c, fplane(a.pv[f], y), oc);
+            prod = f == 0 ? c * c : prod * c * c;
+        }
+        if (a.img) {
+            const unsigned o = (yrec(y) && oim != kOOB) ? oim : kOOB;
+            const double v = im + (double)prod;
+            using u32x2 = unsigned __attribute__((ext_vector_type(2)));
+            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), img_rsrc(y), o, 0, 0);
+        }
+#pragma unroll
+        for (int f = 0; f < G; ++f) {
+            q[f][0] = q[f][1]; q[f][1] = q[f][2]; q[f][2] = q[f][3]; q[f][3] = q[f][4]; q[f][4] = nq[f];
+            pr[f] = npr[f]; hz[f] = nhz[f]; hx[f] = nhx[f];
+        }
+        vc = nvc; dv = ndv; bits = nbits; im = nim; fyv = nfy;
+        buf ^= 1;
+    }
+}
+
 // 3D step kernels: 0-5 general (bounds-checked), 6-11 lean (guarded layout, single damping
 // factor per axis), 12-19 pipelined (lean arithmetic, prefetching y loop, buffer loads; 16-17
-// with non-temporal prev/vv traffic), 20-23 pipelined with x tiles fastest in the block order.
+// with non-temporal prev/vv traffic), 20-23 pipelined with x tiles fastest in the block order,
+// 24-27 two time steps per launch (tx = step-1 rows; tx - 4 output rows).
 // tx = rows per block, waves per block; rows per thread of the lean and pipelined kernels =
 // tx / waves.
 struct Variant3 {
     int tx, waves, lean;  // lean: 0 general, 1 lean, 2 pipelined, 3 pipelined non-temporal,
-                          // 4 pipelined x-major block order
+                          // 4 pipelined x-major block order, 5 two steps per launch
 };
 const Variant3 kVariants3[] = {{16, 4, 0}, {8, 4, 0},  {16, 8, 0},  {32, 8, 0},  {8, 8, 0},  {4, 4, 0},
                                {16, 8, 1}, {8, 8, 1},  {16, 4, 1},  {32, 8, 1},  {16, 16, 1}, {32, 16, 1},
                                {16, 8, 2}, {16, 4, 2}, {32, 8, 2},  {8, 4, 2},   {16, 8, 3}, {16, 4, 3},
-                               {32, 16, 2}, {8, 8, 2}, {16, 4, 4}, {8, 4, 4}, {16, 8, 4}, {32, 8, 4}};
+                               {32, 16, 2}, {8, 8, 2}, {16, 4, 4}, {8, 4, 4}, {16, 8, 4}, {32, 8, 4},
+                               {16, 8, 5}, {32, 8, 5}, {24, 8, 5}, {16, 4, 5}};
 constexpr int kNumVariants3 = sizeof(kVariants3) / sizeof(kVariants3[0]);
 constexpr int kLeanDefault = 12;
 constexpr int kGeneralDefault = 2;
@@ -764,7 +1172,9 @@ struct pw_afd {
     float* d_rec = nullptr;   // [nt][ny1][nx1]
     float* d_snap = nullptr;  // [ntsnap][ny1][nx1][nz1]
     float* d_tmp = nullptr;   // transpose scratch (nrec*nt)
-    float* raw[3] = {nullptr, nullptr, nullptr};  // allocations behind d_a, d_b, d_vv (with guards)
+    float* raw[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // behind d_a, d_b, d_vv, d_c, d_d (guarded)
+    float* d_c = nullptr;  // the two extra fields of the two-step kernels (allocated on first use)
+    float* d_d = nullptr;
     size_t raw_elems = 0, guard_lo = 0;
     float* d_fz1 = nullptr;  // single damping factor per index (lean kernels)
     float* d_fx1 = nullptr;
@@ -803,6 +1213,8 @@ long long resident_blocks(pw_afd* p, int vi)
         PW_OCC(18, (k_afd3d_pipe<2, 16, 0, 0>)) PW_OCC(19, (k_afd3d_pipe<1, 8, 0, 0>))
         PW_OCC(20, (k_afd3d_pipe<4, 4, 0, 1>)) PW_OCC(21, (k_afd3d_pipe<2, 4, 0, 1>))
         PW_OCC(22, (k_afd3d_pipe<2, 8, 0, 1>)) PW_OCC(23, (k_afd3d_pipe<4, 8, 0, 1>))
+        PW_OCC(24, (k_afd3d_2step<2, 8>)) PW_OCC(25, (k_afd3d_2step<4, 8>))
+        PW_OCC(26, (k_afd3d_2step<3, 8>)) PW_OCC(27, (k_afd3d_2step<4, 4>))
 #undef PW_OCC
         default: break;
     }
@@ -889,10 +1301,45 @@ int afd_launch_step(pw_afd* p, const Step& a, hipEvent_t e0 = nullptr, hipEvent_
     return PW_OK;
 }
 
+int afd_launch_2step(pw_afd* p, int vi, const Step2& a, hipEvent_t e0, hipEvent_t e1)
+{
+    pw_ctx* ctx = p->ctx;
+    hipStream_t s = ctx->s();
+    const Geo& g = p->g;
+    const Variant3 v = kVariants3[vi];
+    const int txo = v.tx - 4;
+    const int ntz = pw::ceil_div(g.nz, 60), ntx = pw::ceil_div(g.nx, txo);
+    int nyc = p->ny_chunk;
+    if (nyc <= 0) {
+        const long long tiles = (long long)ntz * ntx;
+        const int nch = (int)std::max(1LL, resident_blocks(p, vi) / tiles);
+        nyc = std::max(8, pw::ceil_div(g.ny, nch));
+    }
+    const int nch = pw::ceil_div(g.ny, nyc);
+    const int nblk = ntz * ntx * nch;
+    const int grid = pw::ceil_div(nblk, 8) * 8;
+    switch (vi) {
+#define PW_V2(I, R_, W_)                                                                                  \
+    case I:                                                                                               \
+        hipExtLaunchKernelGGL((k_afd3d_2step<R_, W_>), dim3(grid), dim3(64 * W_), 0, s, e0, e1, 0, g, a, ntz, ntx, \
+                              nyc, nblk);                                                                 \
+        break;
+        PW_V2(24, 2, 8)
+        PW_V2(25, 4, 8)
+        PW_V2(26, 3, 8)
+        PW_V2(27, 4, 4)
+#undef PW_V2
+        default:
+            PW_FAIL(ctx, PW_ERR_ARG, "not a two-step 3D kernel variant");
+    }
+    PW_HIP(ctx, hipGetLastError());
+    return PW_OK;
+}
+
 void afd_free(pw_afd* p)
 {
-    float* bufs[] = {p->raw[0], p->raw[1], p->raw[2], p->d_wav, p->d_tri, p->d_rec, p->d_snap, p->d_tmp,
-                     p->d_fz1, p->d_fx1, p->d_fy1};
+    float* bufs[] = {p->raw[0], p->raw[1], p->raw[2], p->raw[3], p->raw[4], p->d_wav, p->d_tri, p->d_rec,
+                     p->d_snap, p->d_tmp, p->d_fz1, p->d_fx1, p->d_fy1};
     for (float* b : bufs)
         if (b) hipFree(b);
     if (p->d_fz) hipFree(p->d_fz);
@@ -916,6 +1363,17 @@ long long pw_afd_output_len(int nt, int nz, int nx, int ny, int jsnap)
 }
 
 int pw_afd_variant_count(void) { return kNumVariants3; }
+
+int pw_afd_variant_info(int variant, int* info)
+{
+    if (variant < 0 || variant >= kNumVariants3 || !info) return PW_ERR_ARG;
+    const Variant3 v = kVariants3[variant];
+    info[0] = v.tx;
+    info[1] = v.waves;
+    info[2] = v.lean;
+    info[3] = v.lean == 5 ? 2 : 1;  // time steps per launch
+    return PW_OK;
+}
 
 pw_afd* pw_afd_create(pw_ctx* ctx, int ndim, int nz, int nx, int ny, float dz, float dx, float dy,
                       int nt, float dt, int nbt, float ct, int jsnap)
@@ -985,7 +1443,7 @@ pw_afd* pw_afd_create(pw_ctx* ctx, int ndim, int nz, int nx, int ny, float dz, f
     lean = lean && single(fx, (size_t)g.nx2 + kMaxTx + 8, fx1);
     lean = lean && single(fy, (size_t)g.ny2 + 8, fy1);
     lean = lean && (double)p->raw_elems < 2.0e9;  // 32-bit offsets
-    lean = lean && (double)(g.ldxy + 4 * g.ldz + 4) * 4.0 < 2147483648.0;  // pipelined plane windows
+    lean = lean && (double)(g.ldxy + 8 * g.ldz + 8) * 4.0 < 2147483648.0;  // pipelined plane windows
     p->lean_ok = lean;
 
     const size_t nrec = (size_t)g.nx1 * g.ny1;
@@ -1126,7 +1584,10 @@ int pw_afd_set_tri_data(pw_afd* p, const float* dat)
     return PW_OK;
 }
 
-static int afd_enqueue(pw_afd* p, int tri, hipEvent_t* evs)
+// Enqueue nsteps time steps (all nt when nsteps <= 0) from zeroed fields (zero == false keeps the
+// fields, for timing).  evs: per launch a (start, end) pair stamped by the dispatch; returns the
+// number of launches in *nlaunch.
+static int afd_enqueue(pw_afd* p, int tri, hipEvent_t* evs, int nsteps = 0, bool zero = true, int* nlaunch = nullptr)
 {
     pw_ctx* ctx = p->ctx;
     const Geo& g = p->g;
@@ -1134,30 +1595,80 @@ static int afd_enqueue(pw_afd* p, int tri, hipEvent_t* evs)
     if (tri && !p->have_tri) PW_FAIL(ctx, PW_ERR_STATE, "run: time-reversal data not set");
     if (!tri && !p->have_src) PW_FAIL(ctx, PW_ERR_STATE, "run: sources not set");
     hipStream_t s = ctx->s();
-    PW_HIP(ctx, hipMemsetAsync(p->raw[0], 0, p->raw_elems * 4, s));
-    PW_HIP(ctx, hipMemsetAsync(p->raw[1], 0, p->raw_elems * 4, s));
+    const int vi = p->ndim == 3 ? (p->variant >= 0 ? p->variant : (p->lean_ok ? kLeanDefault : kGeneralDefault)) : -1;
+    const bool two = vi >= 0 && kVariants3[vi].lean == 5;
+    if (two && !p->lean_ok) PW_FAIL(ctx, PW_ERR_ARG, "two-step 3D kernel not applicable to this grid");
+    if (two && !p->d_c) {
+        PW_HIP(ctx, hipMalloc(&p->raw[3], p->raw_elems * 4));
+        PW_HIP(ctx, hipMalloc(&p->raw[4], p->raw_elems * 4));
+        PW_HIP(ctx, hipMemsetAsync(p->raw[3], 0, p->raw_elems * 4, s));
+        PW_HIP(ctx, hipMemsetAsync(p->raw[4], 0, p->raw_elems * 4, s));
+        p->d_c = p->raw[3] + p->guard_lo;
+        p->d_d = p->raw[4] + p->guard_lo;
+    }
+    if (zero) {
+        PW_HIP(ctx, hipMemsetAsync(p->raw[0], 0, p->raw_elems * 4, s));
+        PW_HIP(ctx, hipMemsetAsync(p->raw[1], 0, p->raw_elems * 4, s));
+    }
     const size_t nrec = (size_t)g.nx1 * g.ny1;
     const size_t snap_elems = (size_t)g.nz1 * g.nx1 * g.ny1;
-    float* A = p->d_a;
-    float* B = p->d_b;
-    const int it0 = tri ? p->nt - 1 : 0, it1 = tri ? -1 : p->nt, its = tri ? -1 : 1;
-    int k = 0;
-    for (int it = it0; it != it1; it += its, ++k) {
-        Step a{};
-        a.cur = A; a.pv = B; a.vv = p->d_vv;
-        a.fz = p->d_fz; a.fx = p->d_fx; a.fy = p->d_fy;
-        a.fz1 = p->d_fz1; a.fx1 = p->d_fx1; a.fy1 = p->d_fy1;
-        a.ns = tri ? 0 : p->ns;
-        a.src = p->d_src; a.wav = p->d_wav; a.nt = p->nt; a.it = it;
-        a.tri = tri ? p->d_tri + (size_t)it * nrec : nullptr;
-        a.rec = tri ? nullptr : p->d_rec + (size_t)it * nrec;
-        a.snap = (p->jsnap > 0 && it % p->jsnap == 0) ? p->d_snap + (size_t)(it / p->jsnap) * snap_elems : nullptr;
-        int rc = afd_launch_step(p, a, evs ? evs[2 * k] : nullptr, evs ? evs[2 * k + 1] : nullptr);
-        if (rc) return rc;
-        std::swap(A, B);
+    const int total = nsteps > 0 ? std::min(nsteps, p->nt) : p->nt;
+    float* A = p->d_a;  // C_n
+    float* B = p->d_b;  // C_{n-1}
+    float* Cb = p->d_c;
+    float* Db = p->d_d;
+    const int its = tri ? -1 : 1;
+    int it = tri ? p->nt - 1 : 0;
+    auto rec_of = [&](int t) { return tri ? nullptr : p->d_rec + (size_t)t * nrec; };
+    auto tri_of = [&](int t) { return tri ? p->d_tri + (size_t)t * nrec : nullptr; };
+    auto snap_of = [&](int t) {
+        return (p->jsnap > 0 && t % p->jsnap == 0) ? p->d_snap + (size_t)(t / p->jsnap) * snap_elems : nullptr;
+    };
+    int k = 0, done = 0;
+    while (done < total) {
+        if (two && total - done >= 2) {
+            Step2 a{};
+            a.cur = A; a.pv = B; a.vv = p->d_vv; a.out1 = Cb; a.out2 = Db;
+            a.fz1 = p->d_fz1; a.fx1 = p->d_fx1; a.fy1 = p->d_fy1;
+            a.ns = tri ? 0 : p->ns;
+            a.src = p->d_src; a.wav = p->d_wav; a.nt = p->nt;
+            a.it1 = it; a.it2 = it + its;
+            a.tri1 = tri_of(a.it1); a.tri2 = tri_of(a.it2);
+            a.rec1 = rec_of(a.it1); a.rec2 = rec_of(a.it2);
+            a.snap1 = snap_of(a.it1); a.snap2 = snap_of(a.it2);
+            int rc = afd_launch_2step(p, vi, a, evs ? evs[2 * k] : nullptr, evs ? evs[2 * k + 1] : nullptr);
+            if (rc) return rc;
+            // (C_{n+2}, C_{n+1}) become (C_n, C_{n-1}); the old pair is free for the next outputs
+            float* oa = A;
+            float* ob = B;
+            A = Db; B = Cb; Cb = oa; Db = ob;
+            it += 2 * its;
+            done += 2;
+        } else {
+            Step a{};
+            a.cur = A; a.pv = B; a.vv = p->d_vv;
+            a.fz = p->d_fz; a.fx = p->d_fx; a.fy = p->d_fy;
+            a.fz1 = p->d_fz1; a.fx1 = p->d_fx1; a.fy1 = p->d_fy1;
+            a.ns = tri ? 0 : p->ns;
+            a.src = p->d_src; a.wav = p->d_wav; a.nt = p->nt; a.it = it;
+            a.tri = tri_of(it);
+            a.rec = rec_of(it);
+            a.snap = snap_of(it);
+            const int saved = p->variant;
+            if (two) p->variant = kLeanDefault;  // odd remainder: one single step
+            int rc = afd_launch_step(p, a, evs ? evs[2 * k] : nullptr, evs ? evs[2 * k + 1] : nullptr);
+            p->variant = saved;
+            if (rc) return rc;
+            std::swap(A, B);
+            it += its;
+            done += 1;
+        }
+        ++k;
     }
+    // keep the buffer roles for the next run: d_a..d_d are only names of the four fields
     p->final_field = A;
     p->last_mode = tri ? 1 : 0;
+    if (nlaunch) *nlaunch = k;
     return PW_OK;
 }
 
@@ -1181,7 +1692,8 @@ int pw_afd_run_timed(pw_afd* p, int tri, int per_step, double* total_ms, double*
         for (auto& e : evs) PW_HIP(ctx, hipEventCreate(&e));
     }
     PW_HIP(ctx, hipEventRecord(e0, s));
-    int rc = afd_enqueue(p, tri, per_step ? evs.data() : nullptr);
+    int nl = 0;
+    int rc = afd_enqueue(p, tri, per_step ? evs.data() : nullptr, 0, true, &nl);
     if (rc) return rc;
     PW_HIP(ctx, hipEventRecord(e1, s));
     PW_HIP(ctx, hipEventSynchronize(e1));
@@ -1190,7 +1702,7 @@ int pw_afd_run_timed(pw_afd* p, int tri, int per_step, double* total_ms, double*
     if (total_ms) *total_ms = ms;
     double acc = 0.0;
     if (per_step) {
-        for (int k = 0; k < p->nt; ++k) {
+        for (int k = 0; k < nl; ++k) {
             float m = 0.f;
             PW_HIP(ctx, hipEventElapsedTime(&m, evs[2 * k], evs[2 * k + 1]));
             acc += m;
@@ -1268,28 +1780,22 @@ int pw_afd_autotune(pw_afd* p, int steps, int* variant_out, int* chunk_out)
         hipEvent_t e0, e1;
         PW_HIP(ctx, hipEventCreate(&e0));
         PW_HIP(ctx, hipEventCreate(&e1));
-        const int chunks[] = {0, 4, 6, 8, 12, 16, 32};
+        const int chunks[] = {0, 4, 6, 8, 12, 16, 24, 32, 48};
         float best = 1e30f;
         std::pair<int, int> choice(p->lean_ok ? kLeanDefault : kGeneralDefault, 0);
         const int saved_v = p->variant, saved_c = p->ny_chunk;
+        const int nst = std::max(2, steps + (steps & 1));  // even: two-step kernels run whole launches
+        const int mode = p->have_src ? 0 : 1;                 // time the plan's own kind of run
+        if (mode == 1 && !p->have_tri) PW_FAIL(ctx, PW_ERR_STATE, "autotune: set sources or time-reversal data first");
         for (int v = 0; v < kNumVariants3; ++v) {
             if (kVariants3[v].lean && !p->lean_ok) continue;
             for (int c : chunks) {
                 p->variant = v;
                 p->ny_chunk = c;
-                Step a{};
-                a.cur = p->d_a; a.pv = p->d_b; a.vv = p->d_vv;
-                a.fz = p->d_fz; a.fx = p->d_fx; a.fy = p->d_fy;
-                a.fz1 = p->d_fz1; a.fx1 = p->d_fx1; a.fy1 = p->d_fy1;
-                a.src = p->d_src; a.wav = p->d_wav; a.nt = p->nt;
-                int rc = afd_launch_step(p, a);  // warm-up launch
+                int rc = afd_enqueue(p, mode, nullptr, 2, false);  // warm-up (and first-use allocation)
                 if (rc) return rc;
                 PW_HIP(ctx, hipEventRecord(e0, s));
-                for (int k = 0; k < steps; ++k) {
-                    a.cur = (k & 1) ? p->d_b : p->d_a;
-                    a.pv = (k & 1) ? p->d_a : p->d_b;
-                    if ((rc = afd_launch_step(p, a))) return rc;
-                }
+                if ((rc = afd_enqueue(p, mode, nullptr, nst, false))) return rc;
                 PW_HIP(ctx, hipEventRecord(e1, s));
                 PW_HIP(ctx, hipEventSynchronize(e1));
                 float ms = 0.f;
@@ -1377,7 +1883,10 @@ struct pw_tri {
     float* d_dat = nullptr;  // [nt][ny1][nx1]
     float* d_tmp = nullptr;
     unsigned char* d_mask = nullptr;
+    unsigned char* d_mbits = nullptr;  // [ny1][nx1] group bits (pipelined kernel)
     double* d_img = nullptr;
+    float *d_fz1 = nullptr, *d_fx1 = nullptr, *d_fy1 = nullptr;
+    bool pipe_ok = false;  // single damping factor per axis and 32-bit plane windows
     bool have_vel = false, have_dat = false, ran = false;
 };
 
@@ -1387,7 +1896,8 @@ void tri_free(pw_tri* p)
 {
     for (int i = 0; i < 8; ++i)
         if (p->d_f[i]) hipFree(p->d_f[i]);
-    void* b[] = {p->d_vv, p->d_fz, p->d_fx, p->d_fy, p->d_dat, p->d_tmp, p->d_mask, p->d_img};
+    void* b[] = {p->d_vv, p->d_fz, p->d_fx, p->d_fy, p->d_dat, p->d_tmp, p->d_mask, p->d_img,
+                 p->d_mbits, p->d_fz1, p->d_fx1, p->d_fy1};
     for (void* x : b)
         if (x) hipFree(x);
 }
@@ -1396,6 +1906,22 @@ template <int G>
 void tri_launch(pw_tri* p, const TriStep<G>& a, hipStream_t s)
 {
     const Geo& g = p->g;
+    if (p->pipe_ok) {
+        constexpr int W = 8;
+        const int ntz = pw::ceil_div(g.nz, 64), ntx = pw::ceil_div(g.nx, W);
+        const long long tiles = (long long)ntz * ntx;
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->ctx->device) != hipSuccess || cus < 1)
+            cus = 256;
+        // y chunks of about 8 planes, at least two blocks per CU
+        int nch = std::max(pw::ceil_div(g.ny, 8), (int)std::max(1LL, (2LL * cus + tiles - 1) / tiles));
+        const int nyc = std::max(1, pw::ceil_div(g.ny, nch));
+        nch = pw::ceil_div(g.ny, nyc);
+        const int nblk = (int)(tiles * nch);
+        const int grid = pw::ceil_div(nblk, 8) * 8;
+        hipLaunchKernelGGL((k_tri3d_pipe<G, W>), dim3(grid), dim3(64 * W), 0, s, g, a, ntz, ntx, nyc, nblk);
+        return;
+    }
     const int ntz = pw::ceil_div(g.nz, 64), ntx = pw::ceil_div(g.nx, 16);
     long long tiles = (long long)ntz * ntx;
     int nch = (int)std::max(1LL, (6 * 256 + tiles - 1) / tiles);
@@ -1425,6 +1951,8 @@ int tri_enqueue(pw_tri* p)
         a.dat = p->d_dat + (size_t)it * nrec;
         a.mask = p->d_mask;
         a.img = (p->jsnap > 0 && it % p->jsnap == 0) ? p->d_img : nullptr;
+        a.mbits = p->d_mbits;
+        a.fz1 = p->d_fz1; a.fx1 = p->d_fx1; a.fy1 = p->d_fy1;
         tri_launch<G>(p, a, s);
         PW_HIP(ctx, hipGetLastError());
         for (int f = 0; f < G; ++f) std::swap(A[f], B[f]);
@@ -1466,8 +1994,29 @@ pw_tri* pw_tri_create(pw_ctx* ctx, int nz, int nx, int ny, float dz, float dx, f
     pw::damping_pairs(g.nx2, nbt, nbt, w.data(), w.data(), fx);
     pw::damping_pairs(g.ny2, nbt, nbt, w.data(), w.data(), fy);
     const size_t nrec = (size_t)nx * ny;
+    // single damping factor per index when the two faces of an axis never overlap
+    auto single = [](const std::vector<float>& pr, size_t len, std::vector<float>& out) -> bool {
+        out.assign(len, 1.0f);
+        for (size_t i = 0; i < pr.size() / 2; ++i) {
+            float lo = pr[2 * i], hi = pr[2 * i + 1];
+            if (lo != 1.0f && hi != 1.0f) return false;
+            out[i] = lo != 1.0f ? lo : hi;
+        }
+        return true;
+    };
+    std::vector<float> fz1, fx1, fy1;
+    bool pipe = single(fz, (size_t)pw::ceil_div(g.nz2, 64) * 64 + 64, fz1) && single(fx, (size_t)g.nx2 + 40, fx1) &&
+                single(fy, (size_t)g.ny2 + 8, fy1);
+    pipe = pipe && (double)(g.ldxy + 4 * g.ldz + 4) * 4.0 < 2147483648.0 && (double)nx * nz * 8.0 < 2147483648.0;
+    p->pipe_ok = pipe;
     bool ok = true;
     for (int i = 0; i < 2 * ngroups && ok; ++i) ok = hipMalloc(&p->d_f[i], p->field_elems * 4) == hipSuccess;
+    if (ok && pipe)
+        ok = hipMalloc(&p->d_fz1, fz1.size() * 4) == hipSuccess && hipMalloc(&p->d_fx1, fx1.size() * 4) == hipSuccess &&
+             hipMalloc(&p->d_fy1, fy1.size() * 4) == hipSuccess && hipMalloc(&p->d_mbits, nrec) == hipSuccess &&
+             hipMemcpy(p->d_fz1, fz1.data(), fz1.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
+             hipMemcpy(p->d_fx1, fx1.data(), fx1.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
+             hipMemcpy(p->d_fy1, fy1.data(), fy1.size() * 4, hipMemcpyHostToDevice) == hipSuccess;
     ok = ok && hipMalloc(&p->d_vv, p->field_elems * 4) == hipSuccess &&
          hipMalloc(&p->d_fz, fz.size() * 4) == hipSuccess && hipMalloc(&p->d_fx, fx.size() * 4) == hipSuccess &&
          hipMalloc(&p->d_fy, fy.size() * 4) == hipSuccess &&
@@ -1533,6 +2082,11 @@ int pw_tri_set_data(pw_tri* p, const float* dat, const unsigned char* masks)
     hipLaunchKernelGGL(k_transpose, grid, dim3(32, 8), 0, s, p->d_tmp, p->d_dat, (int)nrec, p->nt);
     PW_HIP(ctx, hipGetLastError());
     PW_HIP(ctx, hipMemcpyAsync(p->d_mask, masks, nrec * p->G, hipMemcpyHostToDevice, s));
+    std::vector<unsigned char> bits(nrec, 0);
+    for (int f = 0; f < p->G; ++f)
+        for (size_t k = 0; k < nrec; ++k)
+            if (masks[(size_t)f * nrec + k]) bits[k] |= (unsigned char)(1u << f);
+    if (p->d_mbits) PW_HIP(ctx, hipMemcpyAsync(p->d_mbits, bits.data(), nrec, hipMemcpyHostToDevice, s));
     PW_HIP(ctx, hipStreamSynchronize(s));
     p->have_dat = true;
     return PW_OK;

@@ -43,6 +43,24 @@ struct BatchArgs {
     int it;
 };
 
+constexpr unsigned kOOB = 0x80000000u;  // out-of-range lane offset: load returns 0, store dropped
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t field_rsrc(const float* p, unsigned bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)bytes, 0x00020000);
+}
+
+__device__ __forceinline__ float bld(__amdgpu_buffer_rsrc_t r, unsigned off, bool ok)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, ok ? off : kOOB, 0, 0));
+}
+
+// One 64 x XC tile of one shot per block (one wave), streamed along x with a 5-column register
+// queue and an LDS z row.  Loads for column x+1 (prev, vv, z halo, x damping pair) and the centre
+// of column x+3 are issued before column x is computed.  Field accesses go through one buffer
+// resource per shot field with 32-bit offsets; cells outside the shot's updated domain (zero in
+// every field) get an out-of-range offset, so they are neither read nor written and need no
+// branches.
 __global__ __launch_bounds__(64) void k_afd2d_batch(BatchArgs a)
 {
     const int tile = blockIdx.x;
@@ -53,51 +71,65 @@ __global__ __launch_bounds__(64) void k_afd2d_batch(BatchArgs a)
     const int x0 = xc * XC, x1 = min(x0 + XC, sh.nx);
     // ping-pong by step parity: the field holding C_n
     const bool even = (a.it & 1) == 0;
-    const float* __restrict__ cur = (even ? a.pool_a : a.pool_b) + sh.off;
-    float* __restrict__ pv = (even ? a.pool_b : a.pool_a) + sh.off;
-    const float* __restrict__ vv = a.vv + sh.off;
+    const float* cur = (even ? a.pool_a : a.pool_b) + sh.off;
+    float* pv = (even ? a.pool_b : a.pool_a) + sh.off;
+    const float* vv = a.vv + sh.off;
     const float2* __restrict__ dz_ = a.damp + sh.fz_off;
     const float2* __restrict__ dx_ = dz_ + sh.nz2;
     __shared__ float row[2][68];
-    const bool zr = z < sh.nz2;
+    const int L = sh.ldz;
+    const int bias = 2 * L + 2;  // resources start 2 columns + 2 cells before the field
+    const unsigned wbytes = (unsigned)(((long long)L * sh.nx2 + 2 * bias) * 4);
+    const auto rc = field_rsrc(cur - bias, wbytes), rp = field_rsrc(pv - bias, wbytes),
+               rv = field_rsrc(vv - bias, wbytes);
     const bool zact = z < sh.nz;
     const bool zin = z >= 2 && z < sh.nz2 - 2;
     const float2 fzz = zact ? dz_[z] : make_float2(1.f, 1.f);
-    const int L = sh.ldz;
+    const int zh = lane < 2 ? z0 - 2 + lane : z0 + 62 + lane;  // z halo of lanes 0..3
+    const int lzh = lane < 2 ? lane : 64 + lane;
+    const bool zhv = lane < 4 && zh >= 0 && zh < sh.nz;
+    const unsigned oz = (unsigned)(bias + z) * 4u, ozh = (unsigned)(bias + zh) * 4u, ocol = (unsigned)L * 4u;
+    auto xv = [&](int x) { return x >= 0 && x < sh.nx; };
+    bool blk_src = false;
+    for (int s = 0; s < sh.ns; ++s) {
+        const int* cz = a.src + 3 * (sh.src_off + s);
+        blk_src |= cz[0] >= z0 - 1 && cz[0] <= z0 + 64 && cz[1] >= x0 - 1 && cz[1] <= x1;
+    }
     float q[5];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        int xx = x0 - 2 + k;
-        q[k] = (zr && xx >= 0 && xx < sh.nx2) ? cur[xx * L + z] : 0.f;
+    for (int k = 0; k < 5; ++k) {
+        const int xx = x0 - 2 + k;
+        q[k] = bld(rc, oz + (unsigned)xx * ocol, zact && xv(xx));
     }
+    float pr = bld(rp, oz + (unsigned)x0 * ocol, zact);
+    float vc = bld(rv, oz + (unsigned)x0 * ocol, zact);
+    float hz = bld(rc, ozh + (unsigned)x0 * ocol, zhv);
+    float2 fxx = dx_[x0];
     int buf = 0;
     for (int x = x0; x < x1; ++x) {
-        q[4] = (zr && x + 2 < sh.nx2) ? cur[(x + 2) * L + z] : 0.f;
+        const int xn = x + 1;
+        const bool nv = xn < x1;
+        const float nq = bld(rc, oz + (unsigned)(x + 3) * ocol, zact && nv && x + 3 < sh.nx);
+        const float npr = bld(rp, oz + (unsigned)xn * ocol, zact && nv);
+        const float nvc = bld(rv, oz + (unsigned)xn * ocol, zact && nv);
+        const float nhz = bld(rc, ozh + (unsigned)xn * ocol, zhv && nv);
+        const float2 nfx = dx_[min(xn, sh.nx2 - 1)];  // xn may equal nx == nx2 on the last column
         row[buf][lane + 2] = q[2];
-        if (lane < 4) {
-            int zz = lane < 2 ? z0 - 2 + lane : z0 + 62 + lane;
-            int lz = lane < 2 ? lane : 64 + lane;
-            row[buf][lz] = (zz >= 0 && zz < sh.nz2) ? cur[x * L + zz] : 0.f;
-        }
+        if (lane < 4) row[buf][lzh] = hz;
         __syncthreads();
-        if (zact) {
-            const int j = x * L + z;
-            const float u = q[2];
-            float lap = 0.f;
-            if (zin && x >= 2 && x < sh.nx2 - 2) {
-                const float* R = &row[buf][lane + 2];
-                lap = C0 * u * sh.cc
-                    + C1 * (q[1] + q[3]) * sh.idx
-                    + C2 * (q[0] + q[4]) * sh.idx
-                    + C1 * (R[-1] + R[1]) * sh.idz
-                    + C2 * (R[-2] + R[2]) * sh.idz;
-            }
-            const float vc = vv[j];
-            const float2 fxx = dx_[x];
-            float p = pv[j];
-            p = p * fzz.x; p = p * fzz.y; p = p * fxx.x; p = p * fxx.y;
-            float c = u + (u - p);
-            c = c + lap * vc;
+        const float u = q[2];
+        const float* R = &row[buf][lane + 2];
+        float lap = C0 * u * sh.cc
+                  + C1 * (q[1] + q[3]) * sh.idx
+                  + C2 * (q[0] + q[4]) * sh.idx
+                  + C1 * (R[-1] + R[1]) * sh.idz
+                  + C2 * (R[-2] + R[2]) * sh.idz;
+        lap = (zin && x >= 2 && x < sh.nx2 - 2) ? lap : 0.f;
+        float p = pr;
+        p = p * fzz.x; p = p * fzz.y; p = p * fxx.x; p = p * fxx.y;
+        float c = u + (u - p);
+        c = c + lap * vc;
+        if (blk_src) {
             for (int s = 0; s < sh.ns; ++s) {
                 const int* cz = a.src + 3 * (sh.src_off + s);
                 int ddz = z - cz[0], ddx = x - cz[1];
@@ -105,13 +137,14 @@ __global__ __launch_bounds__(64) void k_afd2d_batch(BatchArgs a)
                 if (az_ <= 1 && ax_ <= 1)
                     c = c + (vc * a.wav[sh.wav_off + (long long)s * sh.nt + a.it]) / (float)(az_ + ax_ + 1);
             }
-            c = c * fzz.x; c = c * fzz.y; c = c * fxx.x; c = c * fxx.y;
-            pv[j] = c;
-            if (z == sh.nb && x >= sh.nb && x < sh.nb + sh.nx1)
-                a.rec[sh.rec_off + (long long)a.it * sh.nx1 + (x - sh.nb)] = c;
         }
-#pragma unroll
-        for (int k = 0; k < 4; ++k) q[k] = q[k + 1];
+        c = c * fzz.x; c = c * fzz.y; c = c * fxx.x; c = c * fxx.y;
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, c), rp,
+                                              zact ? oz + (unsigned)x * ocol : kOOB, 0, 0);
+        if (z == sh.nb && x >= sh.nb && x < sh.nb + sh.nx1)
+            a.rec[sh.rec_off + (long long)a.it * sh.nx1 + (x - sh.nb)] = c;
+        q[0] = q[1]; q[1] = q[2]; q[2] = q[3]; q[3] = q[4]; q[4] = nq;
+        pr = npr; vc = nvc; hz = nhz; fxx = nfx;
         buf ^= 1;
     }
 }

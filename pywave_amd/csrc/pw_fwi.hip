@@ -88,7 +88,44 @@ __device__ __forceinline__ T sponge(T v, T zl, T zh, T xl, T xh)
     return v;
 }
 
+// ---------------------------------------------------------------- buffer-resource access
+// Fields are read and written through buffer resources (scalar base, 32-bit lane offsets); a
+// lane whose cell is outside the padded grid gets an out-of-range offset, answered with 0 (the
+// value the reference's zero boundary gives) without a memory access, and its stores are dropped.
+constexpr unsigned kOOB = 0x80000000u;
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t win(const void* base, unsigned bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
+}
+
+template <typename T>
+__device__ __forceinline__ T bld(__amdgpu_buffer_rsrc_t r, unsigned off, bool ok);
+template <>
+__device__ __forceinline__ float bld<float>(__amdgpu_buffer_rsrc_t r, unsigned off, bool ok)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, ok ? off : kOOB, 0, 0));
+}
+template <>
+__device__ __forceinline__ double bld<double>(__amdgpu_buffer_rsrc_t r, unsigned off, bool ok)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, ok ? off : kOOB, 0, 0));
+}
+
+__device__ __forceinline__ void bst(__amdgpu_buffer_rsrc_t r, unsigned off, float v)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, 0, 0);
+}
+__device__ __forceinline__ void bst(__amdgpu_buffer_rsrc_t r, unsigned off, double v)
+{
+    using u32x2 = unsigned __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, off, 0, 0);
+}
+
 // ---------------------------------------------------------------- forward step (+ modelling)
+// One 64 x XC tile of one shot per block (one wave), streamed along x with a 9-column register
+// queue (radius 4) and an LDS z row; the loads of column x+1 (previous level, model, z halo,
+// sponge pair) and the centre of column x+5 are in flight while column x is computed.
 template <typename T>
 __global__ __launch_bounds__(64) void k_fwd(FGeo<T> g, StepArgs<T> a, int ntz)
 {
@@ -98,33 +135,47 @@ __global__ __launch_bounds__(64) void k_fwd(FGeo<T> g, StepArgs<T> a, int ntz)
     const int lane = threadIdx.x, z0 = tz * 64, z = z0 + lane;
     const int x0 = xc * XC, x1 = min(x0 + XC, g.pnx);
     __shared__ T row[2][72];
-    const T* __restrict__ X = a.X + slot * g.fs;
-    T* __restrict__ Y = a.Y + slot * g.fs;
     const ShotInfo sh = a.shots[slot];
+    const int L = g.ldz;
+    const int bias = 4 * L + 4;  // windows start 4 columns + 4 cells before the field
+    const unsigned wb = (unsigned)((g.fs + 2 * bias) * (long long)sizeof(T));
+    const auto rX = win(a.X + slot * g.fs - bias, wb), rY = win(a.Y + slot * g.fs - bias, wb),
+               rM = win(a.m - bias, wb);
     const bool zok = z < g.pnz;
     const bool zin = z >= 4 && z < g.pnz - 4;
     const T zl = zok ? a.sfz[2 * z] : T(1), zh = zok ? a.sfz[2 * z + 1] : T(1);
+    const int zhz = lane < 4 ? z0 - 4 + lane : z0 + 60 + lane;  // z halo of lanes 0..7
+    const int lzh = lane < 4 ? lane : 64 + lane;
+    const bool zhv = lane < 8 && zhz >= 0 && zhz < g.pnz;
+    constexpr unsigned E = sizeof(T);
+    const unsigned oz = (unsigned)(bias + z) * E, ozh = (unsigned)(bias + zhz) * E, oc = (unsigned)L * E;
+    auto xv = [&](int x) { return x >= 0 && x < g.pnx; };
     const T wv = a.wav[a.it];
     T q[9];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        int xx = x0 - 4 + k;
-        q[k] = (zok && xx >= 0 && xx < g.pnx) ? X[(long long)xx * g.ldz + z] : T(0);
+    for (int k = 0; k < 9; ++k) {
+        const int xx = x0 - 4 + k;
+        q[k] = bld<T>(rX, oz + (unsigned)xx * oc, zok && xv(xx));
     }
+    T yv = bld<T>(rY, oz + (unsigned)x0 * oc, zok);
+    T mv = bld<T>(rM, oz + (unsigned)x0 * oc, zok);
+    T hz = bld<T>(rX, ozh + (unsigned)x0 * oc, zhv);
+    T xl = a.sfx[2 * x0], xh = a.sfx[2 * x0 + 1];
     int buf = 0;
     for (int x = x0; x < x1; ++x) {
-        const long long col = (long long)x * g.ldz;
-        q[8] = (zok && x + 4 < g.pnx) ? X[col + 4LL * g.ldz + z] : T(0);
-        const T yv = zok ? Y[col + z] : T(0);
-        const T mv = zok ? a.m[col + z] : T(0);
+        const int xn = x + 1;
+        const bool nv = xn < x1;
+        const T nq = bld<T>(rX, oz + (unsigned)(x + 5) * oc, zok && nv && x + 5 < g.pnx);
+        const T nyv = bld<T>(rY, oz + (unsigned)xn * oc, zok && nv);
+        const T nmv = bld<T>(rM, oz + (unsigned)xn * oc, zok && nv);
+        const T nhz = bld<T>(rX, ozh + (unsigned)xn * oc, zhv && nv);
+        const int xs = xn < g.pnx ? xn : g.pnx - 1;
+        const T nxl = a.sfx[2 * xs], nxh = a.sfx[2 * xs + 1];
         row[buf][lane + 4] = q[4];
-        if (lane < 4) {
-            int za = z0 - 4 + lane, zb = z0 + 64 + lane;
-            row[buf][lane] = za >= 0 ? X[col + za] : T(0);
-            row[buf][68 + lane] = zb < g.pnz ? X[col + zb] : T(0);
-        }
+        if (lane < 8) row[buf][lzh] = hz;
         __syncthreads();
         if (zok) {
+            const long long col = (long long)x * g.ldz;
             const T p1 = q[4];
             if (a.rec && z == g.rz) {  // record before the update (wave_fwigradient.c:136-139)
                 T* r = a.rec + slot * a.trs + (long long)a.it * g.nr;
@@ -153,13 +204,14 @@ __global__ __launch_bounds__(64) void k_fwd(FGeo<T> g, StepArgs<T> a, int ntz)
             const int px = x - sh.sx0, pz = z - sh.sz0;
             if (px >= 0 && px < sh.pw && pz >= 0 && pz < sh.ph)
                 term = term + a.patch[slot * PMAX * PMAX + px * PMAX + pz] * wv;
-            const T xl = a.sfx[2 * x], xh = a.sfx[2 * x + 1];
             const T p0 = sponge(yv, zl, zh, xl, xh);
             T p2 = 2 * p1 - p0 + mv * term;
-            Y[col + z] = sponge(p2, zl, zh, xl, xh);
+            bst(rY, oz + (unsigned)x * oc, sponge(p2, zl, zh, xl, xh));
         }
 #pragma unroll
         for (int k = 0; k < 8; ++k) q[k] = q[k + 1];
+        q[8] = nq;
+        yv = nyv; mv = nmv; hz = nhz; xl = nxl; xh = nxh;
         buf ^= 1;
     }
 }
@@ -261,6 +313,9 @@ struct ExArgs {
     T* gm;           // dJ/dm accumulator [slot][fs]
 };
 
+// Exact adjoint step (pipelined like k_fwd): q_t = S(2 q_{t+1} - S q_{t+2} + L^T(m q_{t+1}) + R^T g_t)
+// with dJ/dm += (L X_t + source) q_{t+1}.  A = q_{t+1}, B = q_{t+2} (overwritten by q_t), F = the
+// forward field X_t of this step; the m*A product is zero outside the Laplacian's rows [4, n-4).
 template <typename T>
 __global__ __launch_bounds__(64) void k_adj_exact(FGeo<T> g, StepArgs<T> a, ExArgs<T> ex, int ntz)
 {
@@ -271,46 +326,79 @@ __global__ __launch_bounds__(64) void k_adj_exact(FGeo<T> g, StepArgs<T> a, ExAr
     const int x0 = xc * XC, x1 = min(x0 + XC, g.pnx);
     __shared__ T rowa[2][72];
     __shared__ T rowf[2][72];
-    const T* __restrict__ A = a.X + slot * g.fs;
-    T* __restrict__ B = a.Y + slot * g.fs;
-    const T* __restrict__ F = ex.F + slot * ex.fsl;
-    const T* __restrict__ M = a.m;
     const ShotInfo sh = a.shots[slot];
+    const int L = g.ldz;
+    const int bias = 4 * L + 4;
+    const unsigned wb = (unsigned)((g.fs + 2 * bias) * (long long)sizeof(T));
+    const auto rA = win(a.X + slot * g.fs - bias, wb), rB = win(a.Y + slot * g.fs - bias, wb),
+               rM = win(a.m - bias, wb), rF = win(ex.F + slot * ex.fsl - bias, wb),
+               rG = win(ex.gm + slot * g.fs - bias, wb);
     const bool zok = z < g.pnz;
     const bool zin = z >= 4 && z < g.pnz - 4;
     const T zl = zok ? a.sfz[2 * z] : T(1), zh = zok ? a.sfz[2 * z + 1] : T(1);
+    const int zhz = lane < 4 ? z0 - 4 + lane : z0 + 60 + lane;  // z halo of lanes 0..7
+    const int lzh = lane < 4 ? lane : 64 + lane;
+    const bool zhv = lane < 8 && zhz >= 0 && zhz < g.pnz;
+    const bool zhin = zhz >= 4 && zhz < g.pnz - 4;
+    constexpr unsigned E = sizeof(T);
+    const unsigned oz = (unsigned)(bias + z) * E, ozh = (unsigned)(bias + zhz) * E, oc = (unsigned)L * E;
+    auto xv = [&](int x) { return x >= 0 && x < g.pnx; };
+    auto xin = [&](int x) { return x >= 4 && x < g.pnx - 4; };
+    // receiver injection: lane z == rz of column x takes trace sample k(x) (regular spread)
     const T* inj = a.inj + slot * a.trs + (long long)a.it * g.nr;
-    const T wv = a.wav[a.it];
-    // product m*A masked to the Laplacian's rows [4, n-4) in both axes
-    auto pa = [&](int zz, int xx) -> T {
-        if (zz < 4 || zz >= g.pnz - 4 || xx < 4 || xx >= g.pnx - 4) return T(0);
-        long long j = (long long)xx * g.ldz + zz;
-        return M[j] * A[j];
+    const auto rI = win(inj, (unsigned)(g.nr * E));
+    const bool rlane = z == g.rz;
+    auto rk = [&](int x) -> int {
+        if (!rlane || g.dr_v <= 0) return -1;
+        const int d = x - sh.r0_v;
+        if (d < 0 || d % g.dr_v != 0 || d / g.dr_v >= sh.nr2) return -1;
+        const int k = sh.r02 + d / g.dr_v;
+        return (k >= 0 && k < g.nr) ? k : -1;
     };
+    const T wv = a.wav[a.it];
     T qa[9], qf[9];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        int xx = x0 - 4 + k;
-        bool ok = zok && xx >= 0 && xx < g.pnx;
-        qa[k] = ok ? pa(z, xx) : T(0);
-        qf[k] = ok ? F[(long long)xx * g.ldz + z] : T(0);
+    for (int k = 0; k < 9; ++k) {
+        const int xx = x0 - 4 + k;
+        const bool ok = zok && xv(xx) && zin && xin(xx);
+        const T av_ = bld<T>(rA, oz + (unsigned)xx * oc, ok), mv_ = bld<T>(rM, oz + (unsigned)xx * oc, ok);
+        qa[k] = mv_ * av_;
+        qf[k] = bld<T>(rF, oz + (unsigned)xx * oc, zok && xv(xx));
     }
+    T av = bld<T>(rA, oz + (unsigned)x0 * oc, zok), bv = bld<T>(rB, oz + (unsigned)x0 * oc, zok);
+    T gv = bld<T>(rG, oz + (unsigned)x0 * oc, zok);
+    T ha, hf;
+    {
+        const bool ok = zhv && zhin && xin(x0);
+        ha = bld<T>(rM, ozh + (unsigned)x0 * oc, ok) * bld<T>(rA, ozh + (unsigned)x0 * oc, ok);
+        hf = bld<T>(rF, ozh + (unsigned)x0 * oc, zhv);
+    }
+    T xl = a.sfx[2 * x0], xh = a.sfx[2 * x0 + 1];
+    int k0 = rk(x0);
+    T iv = bld<T>(rI, (unsigned)(k0 < 0 ? 0 : k0) * E, k0 >= 0);
     int buf = 0;
     for (int x = x0; x < x1; ++x) {
-        const long long col = (long long)x * g.ldz;
-        const bool ok8 = zok && x + 4 < g.pnx;
-        qa[8] = ok8 ? pa(z, x + 4) : T(0);
-        qf[8] = ok8 ? F[col + 4LL * g.ldz + z] : T(0);
-        const T av = zok ? A[col + z] : T(0);
-        const T bv = zok ? B[col + z] : T(0);
+        const int xn = x + 1, x5 = x + 5;
+        const bool nv = xn < x1;
+        const bool ok5 = zok && nv && x5 < g.pnx;
+        const bool okp5 = ok5 && zin && xin(x5);
+        const T na5 = bld<T>(rA, oz + (unsigned)x5 * oc, okp5), nm5 = bld<T>(rM, oz + (unsigned)x5 * oc, okp5);
+        const T nf5 = bld<T>(rF, oz + (unsigned)x5 * oc, ok5);
+        const T nav = bld<T>(rA, oz + (unsigned)xn * oc, zok && nv);
+        const T nbv = bld<T>(rB, oz + (unsigned)xn * oc, zok && nv);
+        const T ngv = bld<T>(rG, oz + (unsigned)xn * oc, zok && nv);
+        const bool okh = zhv && nv && zhin && xin(xn);
+        const T nha = bld<T>(rM, ozh + (unsigned)xn * oc, okh) * bld<T>(rA, ozh + (unsigned)xn * oc, okh);
+        const T nhf = bld<T>(rF, ozh + (unsigned)xn * oc, zhv && nv);
+        const int xs = xn < g.pnx ? xn : g.pnx - 1;
+        const T nxl = a.sfx[2 * xs], nxh = a.sfx[2 * xs + 1];
+        const int kn = nv ? rk(xn) : -1;
+        const T niv = bld<T>(rI, (unsigned)(kn < 0 ? 0 : kn) * E, kn >= 0);
         rowa[buf][lane + 4] = qa[4];
         rowf[buf][lane + 4] = qf[4];
-        if (lane < 4) {
-            int za = z0 - 4 + lane, zb = z0 + 64 + lane;
-            rowa[buf][lane] = za >= 0 ? pa(za, x) : T(0);
-            rowa[buf][68 + lane] = zb < g.pnz ? pa(zb, x) : T(0);
-            rowf[buf][lane] = za >= 0 ? F[col + za] : T(0);
-            rowf[buf][68 + lane] = zb < g.pnz ? F[col + zb] : T(0);
+        if (lane < 8) {
+            rowa[buf][lzh] = ha;
+            rowf[buf][lzh] = hf;
         }
         __syncthreads();
         if (zok) {
@@ -321,24 +409,19 @@ __global__ __launch_bounds__(64) void k_adj_exact(FGeo<T> g, StepArgs<T> a, ExAr
                    (C::K0 * Ra[0] + C::K1 * (Ra[1] + Ra[-1]) + C::K2 * (Ra[2] + Ra[-2]) + C::K3 * (Ra[3] + Ra[-3]) +
                     C::K4 * (Ra[4] + Ra[-4])) / g.dz2;
             T src = T(0);
-            if (z == g.rz) {
+            if (rlane) {
                 if (g.dr_v > 0) {
-                    int d = x - sh.r0_v;
-                    if (d >= 0 && d % g.dr_v == 0 && d / g.dr_v < sh.nr2) {
-                        int k = sh.r02 + d / g.dr_v;
-                        if (k >= 0 && k < g.nr) src = src + inj[k];
-                    }
+                    src = src + iv;  // iv is 0 off the receivers
                 } else if (x == sh.r0_v) {
                     for (int ir = 0; ir < sh.nr2; ++ir)
                         if (sh.r02 + ir >= 0 && sh.r02 + ir < g.nr) src = src + inj[sh.r02 + ir];
                 }
             }
-            const T xl = a.sfx[2 * x], xh = a.sfx[2 * x + 1];
             T qn = 2 * av - sponge(bv, zl, zh, xl, xh) + lt + src;
-            B[col + z] = sponge(qn, zl, zh, xl, xh);
+            bst(rB, oz + (unsigned)x * oc, sponge(qn, zl, zh, xl, xh));
             // dJ/dm accumulation with the forward field's Laplacian (rim rows are zero)
             T lf = T(0);
-            if (zin && x >= 4 && x < g.pnx - 4) {
+            if (zin && xin(x)) {
                 const T* Rf = &rowf[buf][lane + 4];
                 lf = (C::K0 * qf[4] + C::K1 * (qf[5] + qf[3]) + C::K2 * (qf[6] + qf[2]) + C::K3 * (qf[7] + qf[1]) +
                       C::K4 * (qf[8] + qf[0])) / g.dx2 +
@@ -348,10 +431,13 @@ __global__ __launch_bounds__(64) void k_adj_exact(FGeo<T> g, StepArgs<T> a, ExAr
             const int px = x - sh.sx0, pz = z - sh.sz0;
             if (px >= 0 && px < sh.pw && pz >= 0 && pz < sh.ph)
                 lf = lf + a.patch[slot * PMAX * PMAX + px * PMAX + pz] * wv;
-            ex.gm[slot * g.fs + col + z] += lf * av;
+            bst(rG, oz + (unsigned)x * oc, gv + lf * av);
         }
 #pragma unroll
         for (int k = 0; k < 8; ++k) { qa[k] = qa[k + 1]; qf[k] = qf[k + 1]; }
+        qa[8] = nm5 * na5;
+        qf[8] = nf5;
+        av = nav; bv = nbv; gv = ngv; ha = nha; hf = nhf; xl = nxl; xh = nxh; iv = niv;
         buf ^= 1;
     }
 }
@@ -746,6 +832,11 @@ struct pw_fwi {
     std::vector<float> patch_host;  // per shot PMAX*PMAX
     std::vector<ShotInfo> shot_host;
     double last_fcost = 0.0;
+    // checkpoints left by pw_fwi_forward_dev for the model at dev_ckpt_vel (all shots in one
+    // batch), reused by the next pw_fwi_adjoint_dev on the same model; any other call that
+    // uploads a model or changes the set-up clears the flag
+    bool dev_ckpt_valid = false;
+    const void* dev_ckpt_vel = nullptr;
     size_t budget = 0;  // bytes allowed for checkpoints + segments
     double t_forward_ms = 0, t_backward_ms = 0;
 };
@@ -849,6 +940,7 @@ template <typename T>
 int upload_model(pw_fwi* p, const T* d_vel_src, bool from_device)
 {
     pw_ctx* ctx = p->ctx;
+    p->dev_ckpt_valid = false;
     hipStream_t s = ctx->s();
     const Acq& a = p->acq;
     if (from_device)
@@ -1222,6 +1314,11 @@ pw_fwi* pw_fwi_create(pw_ctx* ctx, const float* pararray, int batch, int precisi
     p->ldz = (a.padnz + 31) / 32 * 32;
     p->fs = (long long)p->ldz * a.padnx;
     p->nzx = (long long)a.nz * a.nx;
+    if ((double)(p->fs + 8 * p->ldz + 8) * precision >= 2147483648.0) {  // 32-bit buffer windows per field
+        ctx->err = "pw_fwi_create: padded model too large for one field window (2 GiB)";
+        delete p;
+        return nullptr;
+    }
     // geometry per shot
     p->shot_host.resize(a.ns);
     p->patch_host.assign((size_t)a.ns * PMAX * PMAX, 0.f);
@@ -1328,6 +1425,7 @@ void pw_fwi_destroy(pw_fwi* p)
 
 int pw_fwi_set_checkpointing(pw_fwi* p, int steps)
 {
+    if (p) p->dev_ckpt_valid = false;
     if (!p || steps < 0) return PW_ERR_ARG;
     return plan_storage(p, steps);
 }
@@ -1343,6 +1441,7 @@ int pw_fwi_info(pw_fwi* p, int* ints /* 8 */)
 
 int pw_fwi_set_wavelet(pw_fwi* p, const float* wav)
 {
+    if (p) p->dev_ckpt_valid = false;
     if (!p || !wav) return PW_ERR_ARG;
     hipStream_t s = p->ctx->s();
     if (p->prec == 4) {
@@ -1502,14 +1601,18 @@ int pw_fwi_forward_dev(pw_fwi* p, const void* d_vel, void* d_traces)
     if (rc) return rc;
     const long long per = (long long)a.nt * a.nr;
     const size_t e = esz(p);
+    // one batch holds every shot: keep its checkpoints for the adjoint of the same model
+    const bool keep = p->nslot >= a.ns;
     for (int b0 = 0; b0 < a.ns; b0 += p->nslot) {
         const int nb_ = std::min(p->nslot, a.ns - b0);
         if ((rc = set_batch(p, b0, nb_))) return rc;
-        rc = p->prec == 4 ? forward_batch<float>(p, nb_, false, 1, false) : forward_batch<double>(p, nb_, false, 1, false);
+        rc = p->prec == 4 ? forward_batch<float>(p, nb_, keep, 1, keep) : forward_batch<double>(p, nb_, keep, 1, keep);
         if (rc) return rc;
         PW_HIPS(hipMemcpyAsync((char*)d_traces + (size_t)b0 * per * e, p->rec, (size_t)nb_ * per * e,
                                hipMemcpyDeviceToDevice, s));
     }
+    p->dev_ckpt_valid = keep;
+    p->dev_ckpt_vel = d_vel;
     return PW_OK;
 }
 
@@ -1520,7 +1623,12 @@ int pw_fwi_adjoint_dev(pw_fwi* p, const void* d_vel, const void* d_gtraces, void
     if (!p->have_wav) PW_FAIL(p->ctx, PW_ERR_STATE, "adjoint: wavelet not set");
     const Acq& a = p->acq;
     hipStream_t s = p->ctx->s();
-    int rc = p->prec == 4 ? upload_model<float>(p, (const float*)d_vel, true) : upload_model<double>(p, (const double*)d_vel, true);
+    // checkpoints of this very model from the preceding forward_dev (one batch): no forward sweep
+    const bool reuse = p->dev_ckpt_valid && p->dev_ckpt_vel == d_vel && p->nslot >= a.ns;
+    int rc = 0;
+    if (!reuse)
+        rc = p->prec == 4 ? upload_model<float>(p, (const float*)d_vel, true)
+                          : upload_model<double>(p, (const double*)d_vel, true);
     if (rc) return rc;
     const size_t e = esz(p);
     const long long per = (long long)a.nt * a.nr;
@@ -1535,7 +1643,7 @@ int pw_fwi_adjoint_dev(pw_fwi* p, const void* d_vel, const void* d_gtraces, void
         PW_HIPS(hipMemcpyAsync(p->res, (const char*)d_gtraces + (size_t)b0 * per * e, (size_t)nb_ * per * e,
                                hipMemcpyDeviceToDevice, s));
         if (p->prec == 4) {
-            rc = forward_batch<float>(p, nb_, true, 1, true);
+            if (!reuse) rc = forward_batch<float>(p, nb_, true, 1, true);
             if (!rc) rc = fill<float>(p, p->A, fld, 0.f);
             if (!rc) rc = fill<float>(p, p->B, fld, 0.f);
             if (!rc) rc = fill<float>(p, p->gm, fld, 0.f);
@@ -1550,7 +1658,7 @@ int pw_fwi_adjoint_dev(pw_fwi* p, const void* d_vel, const void* d_gtraces, void
                                    (const float*)p->gm, (const float*)p->vpad, dt2_of<float>(p), (float*)p->gtot);
             }
         } else {
-            rc = forward_batch<double>(p, nb_, true, 1, true);
+            if (!reuse) rc = forward_batch<double>(p, nb_, true, 1, true);
             if (!rc) rc = fill<double>(p, p->A, fld, 0.0);
             if (!rc) rc = fill<double>(p, p->B, fld, 0.0);
             if (!rc) rc = fill<double>(p, p->gm, fld, 0.0);
@@ -1569,6 +1677,7 @@ int pw_fwi_adjoint_dev(pw_fwi* p, const void* d_vel, const void* d_gtraces, void
         PW_HIPS(hipGetLastError());
     }
     PW_HIPS(hipMemcpyAsync(d_gvel, p->gtot, p->nzx * e, hipMemcpyDeviceToDevice, s));
+    p->dev_ckpt_valid = false;  // the sweep reused the segment buffer: a second adjoint recomputes
     return PW_OK;
 }
 

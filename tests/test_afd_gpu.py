@@ -51,7 +51,8 @@ def test_afd2d_tri_golden_bit_exact(golden):
     assert np.array_equal(p.final_field(), w0)
 
 
-NVAR = 24  # pw_afd_variant_count(): 6 general, 6 lean and 12 pipelined 3D step kernels
+NVAR = 28  # pw_afd_variant_count(): 6 general, 6 lean, 12 pipelined and 4 two-step 3D step kernels
+TWO_STEP = (24, 25, 26, 27)  # variants that advance two time steps per launch
 
 
 @pytest.mark.parametrize("variant", range(NVAR))
@@ -65,12 +66,45 @@ def test_afd3d_golden_bit_exact_all_variants(golden, variant):
         assert np.array_equal(p.output(), g["out"]), (variant, ny_chunk)
 
 
-def test_afd3d_tri_golden_bit_exact(golden):
+@pytest.mark.parametrize("variant", [-1, 2, 6, 12, 13, 20] + list(TWO_STEP))
+def test_afd3d_tri_golden_bit_exact(golden, variant):
     g = golden("afd3d_tri")
     p = _plan3d(g)
     p.set_tri_data(g["dat"])
+    if variant >= 0:
+        p.set_variant(variant, 7)
     p.run(tri=True)
     assert np.array_equal(p.output(), g["out"])
+
+
+@pytest.mark.parametrize("variant", TWO_STEP)
+@pytest.mark.parametrize("tri", [False, True])
+def test_two_step_odd_nt_vs_oracle(variant, tri):
+    # odd nt: the last step runs on the single-step kernel; snapshots fall on both steps of a launch
+    rng = np.random.default_rng(300 + variant + 7 * tri)
+    vel, nbt, ns, (pz, px, py) = _random_case(rng, 3)
+    nz, nx, ny = vel.shape
+    nt, dt, ct = 71, 0.0008, float(rng.uniform(0.005, 0.06))
+    jsnap = 5
+    p = AfdPlan(vel, (12.0, 10.0, 8.0), nt, dt, nbt, ct, jsnap)
+    if tri:
+        dat = rng.standard_normal((nt, nx, ny)).astype(np.float32)
+        d, snaps, _ = O.afd3d(vel, nt, dt, 10.0, 8.0, 12.0, nbt, ct, [], [], [], None, tri=True,
+                              dat=np.ascontiguousarray(dat.transpose(2, 1, 0)), jsnap=jsnap)
+        p.set_tri_data(dat)
+    else:
+        f = rng.uniform(10, 40, ns).astype(np.float32)
+        t = rng.uniform(0.0, 0.05, ns).astype(np.float32)
+        A = rng.uniform(-2, 2, ns).astype(np.float32)
+        rick = O.ricker_bank(nt, dt, f, t, A)
+        d, snaps, _ = O.afd3d(vel, nt, dt, 10.0, 8.0, 12.0, nbt, ct, px, py, pz, rick, jsnap=jsnap)
+        p.set_sources(np.concatenate([px, py, pz, f, t, A]), ns, wavelet=rick)
+    p.set_variant(variant, int(rng.integers(0, 12)))
+    p.run(tri=tri)
+    out = p.output()
+    if not tri:
+        assert np.array_equal(out[:nt * nx * ny], d.ravel())
+    assert np.array_equal(out[nt * nx * ny:], snaps.ravel())
 
 
 def test_dropin_afd2d_cfg0(golden):
@@ -180,13 +214,22 @@ def _cfg1_model():
     return plain3d_model(201)
 
 
-def test_cfg1_size_early_steps_vs_oracle():
+@pytest.fixture(scope="module")
+def cfg1_early():
     vel = _cfg1_model()
-    nt = 12
+    nt = 13
     rick = O.ricker_bank(nt, 0.001, [15], [0.0], [1.0])
     d, _, _ = O.afd3d(vel, nt, 0.001, 10, 10, 10, 30, 0.01, [100], [100], [100], rick)
+    return vel, nt, rick, d
+
+
+@pytest.mark.parametrize("variant", [-1, 12, 13, 24, 25])
+def test_cfg1_size_early_steps_vs_oracle(cfg1_early, variant):
+    vel, nt, rick, d = cfg1_early
     p = AfdPlan(vel, (10, 10, 10), nt, 0.001, 30, 0.01, 0)
     p.set_sources(np.array([100, 100, 100, 15, 0.0, 1.0], np.float32), 1, wavelet=rick)
+    if variant >= 0:
+        p.set_variant(variant, 0)
     p.run()
     assert np.array_equal(p.output(), d.ravel())
 

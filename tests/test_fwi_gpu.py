@@ -162,3 +162,27 @@ def test_autograd_on_side_stream_matches_host_forward():
         host = tr.cpu().numpy()
     torch.cuda.synchronize()
     assert np.array_equal(host, np.ascontiguousarray(ref.transpose(2, 0, 1)))
+
+
+def test_autograd_checkpoint_reuse_and_repeat():
+    # one batch holds both shots: the backward reuses the forward's checkpoints; a second backward
+    # through the same graph recomputes them and must give the same gradient; with one shot per
+    # batch nothing is reused
+    torch = pytest.importorskip("torch")
+    par, wav, vt, v0 = _fd_case()
+    obs = FwiProblem(par, wav).forward(vt)
+    g_direct, _, _ = FwiProblem(par, wav, data=obs, batch=2, checkpoint=30).gradient(v0.astype(np.float32), "exact")
+    dev = torch.device("cuda:0")
+    d = torch.tensor(np.ascontiguousarray(obs.transpose(2, 0, 1)), device=dev)
+    for batch in (2, 1):
+        p = FwiProblem(par, wav, batch=batch, checkpoint=30)
+        v = torch.tensor(v0.astype(np.float32), device=dev, requires_grad=True)
+        loss = 0.5 * ((pywave_amd.fwi_traces(v, p).double() - d.double()) ** 2).sum()
+        loss.backward(retain_graph=True)
+        g1 = v.grad.clone()
+        v.grad = None
+        loss.backward()
+        torch.cuda.synchronize()
+        g2 = v.grad
+        assert torch.equal(g1, g2), batch
+        assert rel_l2(g1.cpu().numpy(), g_direct) <= 1e-5, batch
