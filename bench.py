@@ -93,6 +93,7 @@ def fwi_bench(plain_shots):
     out["batched_s_per_gradient"] = round(tb, 4)
     out["misfit"] = f
     out["gradient_norm"] = float(torch.linalg.vector_norm(g.double()).item())
+    out["cpu_baseline"] = fwi_cpu_baseline(par, w)
     if plain_shots <= 0:
         return out
     # plain loop: one shot per launch, timed on the first `plain_shots` shots
@@ -109,6 +110,52 @@ def fwi_bench(plain_shots):
     out["plain_sample_shots"] = plain_shots
     out["batched_over_plain"] = round((ns / tb) / (plain_shots / tp), 3)
     return out
+
+
+def fwi_cpu_baseline(par, w, nt_cpu=300):
+    """Reference C gradient_av (pywave/src/wave_fwigradient.c:93-262, single thread) for one shot
+    of the same 400x800 geometry over the first nt_cpu of 3000 steps (its per-step cost does not
+    depend on the step), extrapolated to shots/s at nt=3000.  It is the reference's own gradient
+    (imaging condition, every forward snapshot kept), the CPU path this engine replaces."""
+    try:
+        from oracle import refrun
+        if not refrun.available():
+            return None
+        from pywave_amd.fwi import make_pararray
+        p1 = dict(par)
+        p1["ns"], p1["nt"] = 1, nt_cpu
+        pa = make_pararray(p1)
+        wav = (w["amp"] * ricker_wavelet(nt_cpu, par["dt"], w["f0"], w["t0"])).astype(np.float32)
+        data = np.zeros((nt_cpu, par["nx"], 1), np.float32)
+        q = np.ones_like(w["v0"])
+        _, _, el = refrun.fwigrad(pa, w["v0"], q, wav, data)
+        per_shot = el * par["nt"] / nt_cpu
+        return dict(value=round(1.0 / per_shot, 5), unit="shots/s", cores=1, kind="reference",
+                    sample="reference C gradient_av, one shot of the 400x800 (+60 sponge) geometry, %d of %d steps "
+                           "in %.1f s, extrapolated to %.1f s per shot; single thread, -O2 -fwrapv"
+                           % (nt_cpu, par["nt"], el, per_shot))
+    except Exception as e:  # reported, never required
+        return dict(value=None, unit="shots/s", cores=1, kind="reference", sample="failed: %s" % e)
+
+
+def batch_cpu_baseline(shots, total_cu, nt_cpu=400):
+    """Reference C afd2d (fdm2d, pywave/src/wave_fdm.c:397-686, single thread) on the first shot of
+    the batch over nt_cpu steps, as cell-updates/s, extrapolated to the whole 96-shot workload."""
+    try:
+        from oracle import refrun
+        if not refrun.available():
+            return None
+        s0 = shots[0]
+        nz, nx = s0["vel"].shape
+        src = s0["src"].copy()
+        _, el = refrun.afd2d(s0["vel"], nt_cpu, s0["dt"], s0["d"][1], s0["d"][0], 30, 0.01, src, 1)
+        cells = (nz + 60) * (nx + 60) * nt_cpu
+        rate = cells / el
+        return dict(value=round(len(shots) / (total_cu / rate), 5), unit="shots/s", cores=1, kind="reference",
+                    sample="reference C afd2d on shot 0 (%dx%d +30 ABC), %d steps in %.2f s = %.3f Gcell/s, "
+                           "extrapolated to the 96 shots; single thread" % (nz, nx, nt_cpu, el, rate / 1e9))
+    except Exception as e:
+        return dict(value=None, unit="shots/s", cores=1, kind="reference", sample="failed: %s" % e)
 
 
 def tri_bench():
@@ -158,11 +205,13 @@ def batch_bench():
     b.close()
     nrec = sum(s["vel"].shape[1] * s["nt"] for s in shots)
     alg = 16.0 * cu + 4.0 * nrec
+    cpu = batch_cpu_baseline(shots, cu)
     return {"workload": "batch2d 96 shots, nz 200-600, nx 400-1600 (+30 ABC), nt 2000-4000, 12 Hz, hipGraph",
             "ms": round(ms, 2), "shots_per_s": round(len(shots) / (ms * 1e-3), 2),
             "cell_updates": cu, "Gcell_per_s": round(cu / (ms * 1e-3) / 1e9, 2),
             "achieved_GBs": round(alg / (ms * 1e-3) / 1e9, 1),
-            "frac_hbm": round(alg / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
+            "frac_hbm": round(alg / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+            "cpu_baseline": cpu}
 
 
 def ricker_wavelet(nt, dt, f0, t0):
@@ -200,8 +249,9 @@ def load_traffic(name, variant):
     if os.path.exists(path):
         with open(path) as f:
             d = json.load(f)
-        if variant is not None and list(d.get("variant", [])) == list(variant):
-            return d.get("hbm_bytes_per_launch")
+        for e in d.get("variants", []):
+            if variant is not None and list(e["variant"]) == list(variant):
+                return e["hbm_bytes_per_launch"]
     return None
 
 

@@ -381,10 +381,15 @@ __global__ __launch_bounds__(64 * WAVES) void k_afd3d_pipe(Geo g, Step a, int nt
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int z = z0 + lane;
     const int ldz = g.ldz;
-    const long long ldxy = g.ldxy;
-    // window of one plane: from 2 rows + 2 cells before it to 2 rows + 2 cells after it
+    // one resource per field from 2 rows + 2 cells before the grid to as far after it; lane
+    // offsets (within a plane, constant over the loop) are vector, the plane offset is the
+    // scalar soffset; a plane outside [0, ny) or outside this block's needs uses a resource of
+    // size 0, so none of its loads touches memory
     const int bias = 2 * ldz + 2;
-    const unsigned wbytes = (unsigned)((ldxy + 2 * bias) * 4);
+    const unsigned fbytes = (unsigned)((g.ldxy * g.ny2 + 2 * bias) * 4);
+    const unsigned pstep = (unsigned)(g.ldxy * 4);  // bytes per plane
+    const auto rC = plane_rsrc(a.cur - bias, fbytes), rP = plane_rsrc(a.pv - bias, fbytes),
+               rV = plane_rsrc(a.vv - bias, fbytes), rN = plane_rsrc(a.cur - bias, 0u);
     bool blk_src = false;
     for (int s = 0; s < a.ns; ++s) {
         int gz = a.src[3 * s], gx = a.src[3 * s + 1], gy = a.src[3 * s + 2];
@@ -393,8 +398,10 @@ __global__ __launch_bounds__(64 * WAVES) void k_afd3d_pipe(Geo g, Step a, int nt
     const bool zact = z < g.nz;
     const float fz = a.fz1[z];
     const bool zin = z >= 2 && z < g.nz2 - 2;
-    const int zh = lane < 2 ? z0 - 2 + lane : z0 + TZ - 2 + lane;  // z halo cell of lanes 0..3
-    const int lzh = lane < 2 ? lane : TZ + lane;
+    // z halo: lanes 0..3 stage cells z0-2, z0-1, z0+64, z0+65; the other lanes rewrite their own
+    // centre value, so the halo store needs no branch
+    const int zh = lane < 2 ? z0 - 2 + lane : z0 + TZ - 2 + lane;
+    const int lzh = lane < 4 ? (lane < 2 ? lane : TZ + lane) : lane + 2;
     unsigned oc[RPW], ohz[RPW];
     float fx[RPW];
     bool xin[RPW];
@@ -410,65 +417,68 @@ __global__ __launch_bounds__(64 * WAVES) void k_afd3d_pipe(Geo g, Step a, int nt
     const int xh = w < 2 ? x0 - 2 + w : x0 + TX - 2 + w;  // x halo row of waves 0..3
     const int lrh = w < 2 ? w : TX + w;
     const unsigned ohx = (w < 4 && xh >= 0 && xh < g.nx && zact) ? (unsigned)(bias + xh * ldz + z) * 4u : kOOB;
-
-    auto cur_plane = [&](int yy) { return plane_rsrc(a.cur + yy * ldxy - bias, wbytes); };
-    auto pv_plane = [&](int yy) { return plane_rsrc(a.pv + yy * ldxy - bias, wbytes); };
-    auto vv_plane = [&](int yy) { return plane_rsrc(a.vv + yy * ldxy - bias, wbytes); };
+    auto ld = [](__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+    };
+    auto ldn = [](__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, AUX));
+    };
+    auto yok = [&](int yy) { return yy >= 0 && yy < g.ny; };
 
     // prologue: centre planes y0-2 .. y0+2, and plane y0's prev, vv and halos
     float q[RPW][5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
         const int yy = y0 - 2 + k;
-        const bool yv = yy >= 0 && yy < g.ny;
-        const auto rs = cur_plane(yy);
+        const auto rs = yok(yy) ? rC : rN;
+        const unsigned so = yok(yy) ? (unsigned)yy * pstep : 0u;
 #pragma unroll
-        for (int r = 0; r < RPW; ++r) q[r][k] = bld<0>(rs, oc[r], yv);
+        for (int r = 0; r < RPW; ++r) q[r][k] = ld(rs, oc[r], so);
     }
     float pr[RPW], vc[RPW], hz[RPW], hx;
     float fyv = a.fy1[y0];
     {
-        const auto rc = cur_plane(y0), rp = pv_plane(y0), rv = vv_plane(y0);
+        const unsigned so = (unsigned)y0 * pstep;
 #pragma unroll
         for (int r = 0; r < RPW; ++r) {
-            pr[r] = bld<AUX>(rp, oc[r], true);
-            vc[r] = bld<AUX>(rv, oc[r], true);
-            hz[r] = bld<0>(rc, ohz[r], true);
+            pr[r] = ldn(rP, oc[r], so);
+            vc[r] = ldn(rV, oc[r], so);
+            hz[r] = ld(rC, ohz[r], so);
         }
-        hx = bld<0>(rc, ohx, true);
+        hx = ld(rC, ohx, so);
     }
+    // block-uniform: does this z tile hold the receiver plane (z == nb)?
+    const bool rtile = g.nb >= z0 && g.nb < z0 + TZ;
+    unsigned s0 = (unsigned)y0 * pstep;  // plane y
     int buf = 0;
     for (int y = y0; y < y1; ++y) {
-        // prefetch for the next iteration
-        const int yn = y + 1;
-        const bool nv = yn < y1;
+        // prefetch for the next iteration: plane y+1 (prev, vv, halos) and the centre of y+3
+        const bool nv = y + 1 < y1;
         const bool n3 = nv && y + 3 < g.ny;
+        const unsigned s1 = s0 + pstep, s3 = s0 + 3 * pstep;
+        const auto rc1 = nv ? rC : rN, rp1 = nv ? rP : rN, rv1 = nv ? rV : rN, rc3 = n3 ? rC : rN;
         float npr[RPW], nvc[RPW], nhz[RPW], nq[RPW], nhx;
         // vector load (the field stores in this kernel keep the compiler off the scalar cache),
         // so it is prefetched like the rest
-        const float nfy = a.fy1[yn];
-        {
-            const auto rc = cur_plane(yn), rp = pv_plane(yn), rv = vv_plane(yn), r3 = cur_plane(y + 3);
+        const float nfy = a.fy1[y + 1];
 #pragma unroll
-            for (int r = 0; r < RPW; ++r) {
-                nq[r] = bld<0>(r3, oc[r], n3);
-                npr[r] = bld<AUX>(rp, oc[r], nv);
-                nvc[r] = bld<AUX>(rv, oc[r], nv);
-                nhz[r] = bld<0>(rc, ohz[r], nv);
-            }
-            nhx = bld<0>(rc, ohx, nv);
+        for (int r = 0; r < RPW; ++r) {
+            nq[r] = ld(rc3, oc[r], s3);
+            npr[r] = ldn(rp1, oc[r], s1);
+            nvc[r] = ldn(rv1, oc[r], s1);
+            nhz[r] = ld(rc1, ohz[r], s1);
         }
+        nhx = ld(rc1, ohx, s1);
         float(*L)[LW] = lds[buf];
 #pragma unroll
-        for (int r = 0; r < RPW; ++r) L[w * RPW + r + 2][lane + 2] = q[r][2];
-        if (lane < 4) {
-#pragma unroll
-            for (int r = 0; r < RPW; ++r) L[w * RPW + r + 2][lzh] = hz[r];
+        for (int r = 0; r < RPW; ++r) {
+            L[w * RPW + r + 2][lane + 2] = q[r][2];
+            L[w * RPW + r + 2][lzh] = lane < 4 ? hz[r] : q[r][2];
         }
         if (w < 4) L[lrh][lane + 2] = hx;
         __syncthreads();
         const bool yin = y >= 2 && y < g.ny2 - 2;
-        const auto wp = pv_plane(y);
+        float c[RPW];
 #pragma unroll
         for (int r = 0; r < RPW; ++r) {
             const int lr = w * RPW + r + 2;
@@ -482,18 +492,41 @@ __global__ __launch_bounds__(64 * WAVES) void k_afd3d_pipe(Geo g, Step a, int nt
                       + C2 * (L[lr][lane] + L[lr][lane + 4]) * g.idz;
             lap = (yin && zin && xin[r]) ? lap : 0.f;
             const float p = pr[r] * fz * fx[r] * fyv;
-            float c = u + (u - p);
-            c = c + lap * vc[r];
-            const int x = x0 + w * RPW + r;
-            if (blk_src) c = inject(c, vc[r], a, z, x, y);
-            const bool onrec = z == g.nb && x >= g.nb && x < g.nb + g.nx1 && y >= g.nb && y < g.nb + g.ny1;
-            if (a.tri && onrec) c = c + vc[r] * a.tri[(y - g.nb) * g.nx1 + (x - g.nb)];
-            c = c * fz * fx[r] * fyv;
-            bst<AUX>(c, wp, oc[r]);
-            if (a.rec && onrec) a.rec[(y - g.nb) * g.nx1 + (x - g.nb)] = c;
-            if (a.snap && zact && x < g.nx && z >= g.nb && z < g.nb + g.nz1 && x >= g.nb && x < g.nb + g.nx1 &&
-                y >= g.nb && y < g.nb + g.ny1)
-                a.snap[((size_t)(y - g.nb) * g.nx1 + (x - g.nb)) * g.nz1 + (z - g.nb)] = c;
+            c[r] = u + (u - p);
+            c[r] = c[r] + lap * vc[r];
+        }
+        if (blk_src) {
+#pragma unroll
+            for (int r = 0; r < RPW; ++r) c[r] = inject(c[r], vc[r], a, z, x0 + w * RPW + r, y);
+        }
+        const bool yint = y >= g.nb && y < g.nb + g.ny1;
+        if (a.tri && rtile && yint) {  // time-reversal injection on the receiver plane
+#pragma unroll
+            for (int r = 0; r < RPW; ++r) {
+                const int x = x0 + w * RPW + r;
+                if (z == g.nb && x >= g.nb && x < g.nb + g.nx1)
+                    c[r] = c[r] + vc[r] * a.tri[(y - g.nb) * g.nx1 + (x - g.nb)];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) {
+            c[r] = c[r] * fz * fx[r] * fyv;
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, c[r]), rP, oc[r], s0, AUX);
+        }
+        if (a.rec && rtile && yint) {
+#pragma unroll
+            for (int r = 0; r < RPW; ++r) {
+                const int x = x0 + w * RPW + r;
+                if (z == g.nb && x >= g.nb && x < g.nb + g.nx1) a.rec[(y - g.nb) * g.nx1 + (x - g.nb)] = c[r];
+            }
+        }
+        if (a.snap && yint) {
+#pragma unroll
+            for (int r = 0; r < RPW; ++r) {
+                const int x = x0 + w * RPW + r;
+                if (zact && x < g.nx && z >= g.nb && z < g.nb + g.nz1 && x >= g.nb && x < g.nb + g.nx1)
+                    a.snap[((size_t)(y - g.nb) * g.nx1 + (x - g.nb)) * g.nz1 + (z - g.nb)] = c[r];
+            }
         }
 #pragma unroll
         for (int r = 0; r < RPW; ++r) {
@@ -503,6 +536,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_afd3d_pipe(Geo g, Step a, int nt
         hx = nhx;
         fyv = nfy;
         buf ^= 1;
+        s0 = s1;
     }
 }
 
@@ -1149,6 +1183,7 @@ const Variant3 kVariants3[] = {{16, 4, 0}, {8, 4, 0},  {16, 8, 0},  {32, 8, 0}, 
                                {16, 8, 5}, {32, 8, 5}, {24, 8, 5}, {16, 4, 5}};
 constexpr int kNumVariants3 = sizeof(kVariants3) / sizeof(kVariants3[0]);
 constexpr int kLeanDefault = 12;
+constexpr int kLeanOnlyDefault = 6;  // lean kernel when fields exceed the pipelined kernels' 2 GiB windows
 constexpr int kGeneralDefault = 2;
 
 }  // namespace
@@ -1179,7 +1214,8 @@ struct pw_afd {
     float* d_fz1 = nullptr;  // single damping factor per index (lean kernels)
     float* d_fx1 = nullptr;
     float* d_fy1 = nullptr;
-    bool lean_ok = false;
+    bool lean_ok = false;  // faces do not overlap, 32-bit offsets
+    bool pipe_ok = false;  // lean_ok and every field fits one 2 GiB buffer window
     std::vector<float> tri_host;  // echoed back as the trace part in TRI mode
     bool have_vel = false, have_src = false, have_tri = false;
     int last_mode = -1;  // 0 modelling, 1 tri
@@ -1190,6 +1226,19 @@ struct pw_afd {
 };
 
 namespace {
+
+// the 3D kernel used when none was chosen
+int default_variant(const pw_afd* p)
+{
+    return p->pipe_ok ? kLeanDefault : (p->lean_ok ? kLeanOnlyDefault : kGeneralDefault);
+}
+
+// whether 3D variant vi can run on this plan's grid
+bool variant_ok(const pw_afd* p, int vi)
+{
+    const int k = kVariants3[vi].lean;
+    return k == 0 || (k == 1 && p->lean_ok) || (k >= 2 && p->pipe_ok);
+}
 
 // blocks of 3D variant vi that the device holds at once (CUs x blocks per CU), cached per plan
 long long resident_blocks(pw_afd* p, int vi)
@@ -1234,9 +1283,9 @@ int afd_launch_step(pw_afd* p, const Step& a, hipEvent_t e0 = nullptr, hipEvent_
         dim3 grid(pw::ceil_div(g.nz, 64), pw::ceil_div(g.nx, W));
         hipExtLaunchKernelGGL(k_afd2d<W>, grid, dim3(64 * W), 0, s, e0, e1, 0, g, a);
     } else {
-        const int vi = p->variant >= 0 ? p->variant : (p->lean_ok ? kLeanDefault : kGeneralDefault);
+        const int vi = p->variant >= 0 ? p->variant : default_variant(p);
         const Variant3 v = kVariants3[vi];
-        if (v.lean && !p->lean_ok) PW_FAIL(ctx, PW_ERR_ARG, "lean 3D kernel not applicable to this grid");
+        if (!variant_ok(p, vi)) PW_FAIL(ctx, PW_ERR_ARG, "3D kernel variant not applicable to this grid");
         const int ntz = pw::ceil_div(g.nz, 64), ntx = pw::ceil_div(g.nx, v.tx);
         int nyc = p->ny_chunk;
         if (nyc <= 0) {
@@ -1445,6 +1494,7 @@ pw_afd* pw_afd_create(pw_ctx* ctx, int ndim, int nz, int nx, int ny, float dz, f
     lean = lean && (double)p->raw_elems < 2.0e9;  // 32-bit offsets
     lean = lean && (double)(g.ldxy + 8 * g.ldz + 8) * 4.0 < 2147483648.0;  // pipelined plane windows
     p->lean_ok = lean;
+    p->pipe_ok = lean && (double)p->raw_elems * 4.0 < 2147483648.0;
 
     const size_t nrec = (size_t)g.nx1 * g.ny1;
     bool ok = hipMalloc(&p->raw[0], p->raw_elems * 4) == hipSuccess &&
@@ -1595,9 +1645,9 @@ static int afd_enqueue(pw_afd* p, int tri, hipEvent_t* evs, int nsteps = 0, bool
     if (tri && !p->have_tri) PW_FAIL(ctx, PW_ERR_STATE, "run: time-reversal data not set");
     if (!tri && !p->have_src) PW_FAIL(ctx, PW_ERR_STATE, "run: sources not set");
     hipStream_t s = ctx->s();
-    const int vi = p->ndim == 3 ? (p->variant >= 0 ? p->variant : (p->lean_ok ? kLeanDefault : kGeneralDefault)) : -1;
+    const int vi = p->ndim == 3 ? (p->variant >= 0 ? p->variant : default_variant(p)) : -1;
     const bool two = vi >= 0 && kVariants3[vi].lean == 5;
-    if (two && !p->lean_ok) PW_FAIL(ctx, PW_ERR_ARG, "two-step 3D kernel not applicable to this grid");
+    if (two && !variant_ok(p, vi)) PW_FAIL(ctx, PW_ERR_ARG, "two-step 3D kernel not applicable to this grid");
     if (two && !p->d_c) {
         PW_HIP(ctx, hipMalloc(&p->raw[3], p->raw_elems * 4));
         PW_HIP(ctx, hipMalloc(&p->raw[4], p->raw_elems * 4));
@@ -1782,13 +1832,13 @@ int pw_afd_autotune(pw_afd* p, int steps, int* variant_out, int* chunk_out)
         PW_HIP(ctx, hipEventCreate(&e1));
         const int chunks[] = {0, 4, 6, 8, 12, 16, 24, 32, 48};
         float best = 1e30f;
-        std::pair<int, int> choice(p->lean_ok ? kLeanDefault : kGeneralDefault, 0);
+        std::pair<int, int> choice(default_variant(p), 0);
         const int saved_v = p->variant, saved_c = p->ny_chunk;
         const int nst = std::max(2, steps + (steps & 1));  // even: two-step kernels run whole launches
         const int mode = p->have_src ? 0 : 1;                 // time the plan's own kind of run
         if (mode == 1 && !p->have_tri) PW_FAIL(ctx, PW_ERR_STATE, "autotune: set sources or time-reversal data first");
         for (int v = 0; v < kNumVariants3; ++v) {
-            if (kVariants3[v].lean && !p->lean_ok) continue;
+            if (!variant_ok(p, v)) continue;
             for (int c : chunks) {
                 p->variant = v;
                 p->ny_chunk = c;
