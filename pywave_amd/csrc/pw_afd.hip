@@ -608,19 +608,25 @@ __global__ __launch_bounds__(64 * WAVES) void k_afd3d_2step(Geo g, Step2 a, int 
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int z = zs - 2 + lane;
     const int ldz = g.ldz;
-    const long long ldxy = g.ldxy;
-    const int bias = 4 * ldz + 4;  // plane windows reach 4 rows + 4 cells before the plane
-    const unsigned wbytes = (unsigned)((ldxy + 2 * bias) * 4);
+    // whole-field resources reaching 4 rows + 4 cells around the grid; lane offsets are vector,
+    // the plane offset is the scalar soffset; planes outside the grid use a size-0 resource
+    const int bias = 4 * ldz + 4;
+    const unsigned fbytes = (unsigned)((g.ldxy * g.ny2 + 2 * bias) * 4);
+    const unsigned pstep = (unsigned)(g.ldxy * 4);
+    const auto rC = plane_rsrc(a.cur - bias, fbytes), rP = plane_rsrc(a.pv - bias, fbytes),
+               rV = plane_rsrc(a.vv - bias, fbytes), rN = plane_rsrc(a.cur - bias, 0u),
+               rO1 = plane_rsrc(a.out1 - bias, fbytes), rO2 = plane_rsrc(a.out2 - bias, fbytes);
     const bool zact = z >= 0 && z < g.nz;
     const bool zout = lane >= 2 && lane < 62 && z < g.nz;
     const float fz = a.fz1[z < 0 ? 0 : z];
     const bool zin = z >= 2 && z < g.nz2 - 2;
-    const int zh = lane < 2 ? zs - 4 + lane : zs + 60 + lane;  // cur z halo of lanes 0..3
-    const int lzh = lane < 2 ? lane : 64 + lane;
+    // cur z halo: lanes 0..3 stage zs-4, zs-3, zs+62, zs+63; the others rewrite their centre
+    const int zh = lane < 2 ? zs - 4 + lane : zs + 60 + lane;
+    const int lzh = lane < 4 ? (lane < 2 ? lane : 64 + lane) : lane + 2;
     const int cl = lane < 2 ? 2 : (lane > 61 ? 61 : lane);  // step-2 z neighbours stay in the ring row
-    unsigned oc[RPW], ohz[RPW];
+    unsigned oc[RPW], ohz[RPW], oown[RPW];
     float fx[RPW];
-    bool xin[RPW], xout[RPW];
+    bool xin[RPW];
     int xr[RPW];
 #pragma unroll
     for (int k = 0; k < RPW; ++k) {
@@ -630,9 +636,9 @@ __global__ __launch_bounds__(64 * WAVES) void k_afd3d_2step(Geo g, Step2 a, int 
         const bool xact = x >= 0 && x < g.nx;
         oc[k] = (zact && xact) ? (unsigned)(bias + x * ldz + z) * 4u : kOOB;
         ohz[k] = (lane < 4 && zh >= 0 && zh < g.nz && xact) ? (unsigned)(bias + x * ldz + zh) * 4u : kOOB;
+        oown[k] = (zout && r >= 2 && r < TX1 - 2 && x < g.nx) ? oc[k] : kOOB;  // cells this block writes
         fx[k] = a.fx1[x < 0 ? 0 : x];
         xin[k] = x >= 2 && x < g.nx2 - 2;
-        xout[k] = r >= 2 && r < TX1 - 2 && x < g.nx;
     }
     const int xh = w < 2 ? xs - 4 + w : xs - 4 + TX1 + w;  // cur x halo row of waves 0..3
     const int lrh = w < 2 ? w : TX1 + w;
@@ -642,7 +648,13 @@ __global__ __launch_bounds__(64 * WAVES) void k_afd3d_2step(Geo g, Step2 a, int 
         int gz = a.src[3 * s], gx = a.src[3 * s + 1], gy = a.src[3 * s + 2];
         blk_src |= gz >= zs - 3 && gz <= zs + 62 && gx >= xs - 3 && gx <= xs + TX1 - 2 && gy >= y0 - 3 && gy <= y1 + 2;
     }
-    auto plane = [&](const float* f, int yy) { return plane_rsrc(f + yy * ldxy - bias, wbytes); };
+    const bool rtile = g.nb >= zs - 2 && g.nb < zs + 62;  // the receiver plane z == nb is in the tile
+    auto ld = [](__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+    };
+    auto st = [](float v, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
+    };
     auto yok = [&](int yy) { return yy >= 0 && yy < g.ny; };
     auto fyat = [&](int yy) { return a.fy1[yy < 0 ? 0 : (yy > g.ny2 ? g.ny2 : yy)]; };
 
@@ -652,9 +664,10 @@ __global__ __launch_bounds__(64 * WAVES) void k_afd3d_2step(Geo g, Step2 a, int 
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
         const int yy = j0 - 2 + k;
-        const auto rs = plane(a.cur, yy);
+        const auto rs = yok(yy) ? rC : rN;
+        const unsigned so = yok(yy) ? (unsigned)yy * pstep : 0u;
 #pragma unroll
-        for (int r = 0; r < RPW; ++r) q[r][k] = bld<0>(rs, oc[r], yok(yy));
+        for (int r = 0; r < RPW; ++r) q[r][k] = ld(rs, oc[r], so);
     }
 #pragma unroll
     for (int r = 0; r < RPW; ++r) {
@@ -666,14 +679,15 @@ __global__ __launch_bounds__(64 * WAVES) void k_afd3d_2step(Geo g, Step2 a, int 
     float fyq0 = 1.f, fyq1 = 1.f, fyv = fyat(j0);
     {
         const bool v = yok(j0);
-        const auto rc = plane(a.cur, j0), rp = plane(a.pv, j0), rv = plane(a.vv, j0);
+        const unsigned so = v ? (unsigned)j0 * pstep : 0u;
+        const auto rc = v ? rC : rN, rp = v ? rP : rN, rv = v ? rV : rN;
 #pragma unroll
         for (int r = 0; r < RPW; ++r) {
-            pr[r] = bld<0>(rp, oc[r], v);
-            vc[r] = bld<0>(rv, oc[r], v);
-            hz[r] = bld<0>(rc, ohz[r], v);
+            pr[r] = ld(rp, oc[r], so);
+            vc[r] = ld(rv, oc[r], so);
+            hz[r] = ld(rc, ohz[r], so);
         }
-        hx = bld<0>(rc, ohx, v);
+        hx = ld(rc, ohx, so);
     }
     int cb = 0, rb = 0;  // cur double buffer, step-1 ring slot of plane j
     for (int j = j0; j <= j1; ++j) {
@@ -681,36 +695,36 @@ __global__ __launch_bounds__(64 * WAVES) void k_afd3d_2step(Geo g, Step2 a, int 
         const int jn = j + 1;
         const bool nv = jn <= j1 && yok(jn);
         const bool n3 = jn <= j1 && yok(j + 3);
+        const unsigned s1 = nv ? (unsigned)jn * pstep : 0u, s3 = n3 ? (unsigned)(j + 3) * pstep : 0u;
+        const auto rc1 = nv ? rC : rN, rp1 = nv ? rP : rN, rv1 = nv ? rV : rN, rc3 = n3 ? rC : rN;
         float npr[RPW], nvc[RPW], nhz[RPW], nq[RPW], nhx;
         const float nfy = fyat(jn);
-        {
-            const auto rc = plane(a.cur, jn), rp = plane(a.pv, jn), rv = plane(a.vv, jn), r3 = plane(a.cur, j + 3);
 #pragma unroll
-            for (int r = 0; r < RPW; ++r) {
-                nq[r] = bld<0>(r3, oc[r], n3);
-                npr[r] = bld<0>(rp, oc[r], nv);
-                nvc[r] = bld<0>(rv, oc[r], nv);
-                nhz[r] = bld<0>(rc, ohz[r], nv);
-            }
-            nhx = bld<0>(rc, ohx, nv);
+        for (int r = 0; r < RPW; ++r) {
+            nq[r] = ld(rc3, oc[r], s3);
+            npr[r] = ld(rp1, oc[r], s1);
+            nvc[r] = ld(rv1, oc[r], s1);
+            nhz[r] = ld(rc1, ohz[r], s1);
         }
+        nhx = ld(rc1, ohx, s1);
         float(*C)[68] = lc[cb];
 #pragma unroll
-        for (int r = 0; r < RPW; ++r) C[w * RPW + r + 2][lane + 2] = q[r][2];
-        if (lane < 4) {
-#pragma unroll
-            for (int r = 0; r < RPW; ++r) C[w * RPW + r + 2][lzh] = hz[r];
+        for (int r = 0; r < RPW; ++r) {
+            C[w * RPW + r + 2][lane + 2] = q[r][2];
+            C[w * RPW + r + 2][lzh] = lane < 4 ? hz[r] : q[r][2];
         }
         if (w < 4) C[lrh][lane + 2] = hx;
         __syncthreads();
         // ---- step 1 at plane j
         const bool jown = j >= y0 && j < y1;
+        const bool jint = j >= g.nb && j < g.nb + g.ny1;
         const bool yin = j >= 2 && j < g.ny2 - 2;
-        const auto w1 = plane(a.out1, j);
+        const unsigned sj = jown ? (unsigned)j * pstep : 0u;
+        const auto rw1 = jown ? rO1 : rN;
+        float c1[RPW];
 #pragma unroll
         for (int r = 0; r < RPW; ++r) {
             const int lr = w * RPW + r + 2;
-            const int x = xr[r];
             const float u = q[r][2];
             float lap = C0 * u * g.cc
                       + C1 * (q[r][1] + q[r][3]) * g.idy
@@ -721,22 +735,41 @@ __global__ __launch_bounds__(64 * WAVES) void k_afd3d_2step(Geo g, Step2 a, int 
                       + C2 * (C[lr][lane] + C[lr][lane + 4]) * g.idz;
             lap = (yin && zin && xin[r]) ? lap : 0.f;
             const float p = pr[r] * fz * fx[r] * fyv;
-            float c = u + (u - p);
-            c = c + lap * vc[r];
-            if (blk_src) c = inject_at(c, vc[r], a.src, a.wav, a.ns, a.nt, a.it1, z, x, j);
-            const bool onrec = z == g.nb && x >= g.nb && x < g.nb + g.nx1 && j >= g.nb && j < g.nb + g.ny1;
-            if (a.tri1 && onrec) c = c + vc[r] * a.tri1[(j - g.nb) * g.nx1 + (x - g.nb)];
-            c = c * fz * fx[r] * fyv;
-            l1[rb][lr - 2][lane] = c;
-            const bool own = jown && zout && xout[r];
-            bst<0>(c, w1, own ? oc[r] : kOOB);
-            if (a.rec1 && own && onrec) a.rec1[(j - g.nb) * g.nx1 + (x - g.nb)] = c;
-            if (a.snap1 && own && z >= g.nb && z < g.nb + g.nz1 && x >= g.nb && x < g.nb + g.nx1 && j >= g.nb &&
-                j < g.nb + g.ny1)
-                a.snap1[((size_t)(j - g.nb) * g.nx1 + (x - g.nb)) * g.nz1 + (z - g.nb)] = c;
+            c1[r] = u + (u - p);
+            c1[r] = c1[r] + lap * vc[r];
+        }
+        if (blk_src) {
+#pragma unroll
+            for (int r = 0; r < RPW; ++r) c1[r] = inject_at(c1[r], vc[r], a.src, a.wav, a.ns, a.nt, a.it1, z, xr[r], j);
+        }
+        if (a.tri1 && rtile && jint) {
+#pragma unroll
+            for (int r = 0; r < RPW; ++r)
+                if (z == g.nb && xr[r] >= g.nb && xr[r] < g.nb + g.nx1)
+                    c1[r] = c1[r] + vc[r] * a.tri1[(j - g.nb) * g.nx1 + (xr[r] - g.nb)];
+        }
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) {
+            c1[r] = c1[r] * fz * fx[r] * fyv;
+            l1[rb][w * RPW + r][lane] = c1[r];
+            st(c1[r], rw1, oown[r], sj);
             s1q[r][0] = s1q[r][1]; s1q[r][1] = s1q[r][2]; s1q[r][2] = s1q[r][3]; s1q[r][3] = s1q[r][4];
-            s1q[r][4] = c;
+            s1q[r][4] = c1[r];
             vq[r][0] = vq[r][1]; vq[r][1] = vq[r][2]; vq[r][2] = vc[r];
+        }
+        if (a.rec1 && rtile && jown && jint) {
+#pragma unroll
+            for (int r = 0; r < RPW; ++r)
+                if (oown[r] != kOOB && z == g.nb && xr[r] >= g.nb && xr[r] < g.nb + g.nx1)
+                    a.rec1[(j - g.nb) * g.nx1 + (xr[r] - g.nb)] = c1[r];
+        }
+        if (a.snap1 && jown && jint) {
+#pragma unroll
+            for (int r = 0; r < RPW; ++r) {
+                const int x = xr[r];
+                if (oown[r] != kOOB && z >= g.nb && z < g.nb + g.nz1 && x >= g.nb && x < g.nb + g.nx1)
+                    a.snap1[((size_t)(j - g.nb) * g.nx1 + (x - g.nb)) * g.nz1 + (z - g.nb)] = c1[r];
+            }
         }
         // ---- step 2 at plane m = j-2 (step-1 planes m-2..m+2 are in the queue; the ring slot of
         // plane m was written two iterations ago and published by the last barrier)
@@ -744,13 +777,14 @@ __global__ __launch_bounds__(64 * WAVES) void k_afd3d_2step(Geo g, Step2 a, int 
         if (m >= y0) {
             const float fym = fyq0;
             const bool yin2 = m >= 2 && m < g.ny2 - 2;
+            const bool mint = m >= g.nb && m < g.nb + g.ny1;
+            const unsigned sm = (unsigned)m * pstep;
             const float(*S)[64] = l1[rb == 0 ? 1 : (rb == 1 ? 2 : 0)];  // slot (j-2) % 3 == (rb + 1) % 3
-            const auto w2 = plane(a.out2, m);
+            float c2[RPW];
 #pragma unroll
             for (int r = 0; r < RPW; ++r) {
                 const int lr = w * RPW + r;
-                if (lr < 2 || lr >= TX1 - 2) continue;  // uniform per wave
-                const int x = xr[r];
+                if (lr < 2 || lr >= TX1 - 2) { c2[r] = 0.f; continue; }  // uniform per wave
                 const float u = s1q[r][2];
                 float lap = C0 * u * g.cc
                           + C1 * (s1q[r][1] + s1q[r][3]) * g.idy
@@ -760,20 +794,39 @@ __global__ __launch_bounds__(64 * WAVES) void k_afd3d_2step(Geo g, Step2 a, int 
                           + C1 * (S[lr][cl - 1] + S[lr][cl + 1]) * g.idz
                           + C2 * (S[lr][cl - 2] + S[lr][cl + 2]) * g.idz;
                 lap = (yin2 && zin && xin[r]) ? lap : 0.f;
-                const float vm = vq[r][0];
                 const float p = q[r][0] * fz * fx[r] * fym;  // C_n at plane m
-                float c = u + (u - p);
-                c = c + lap * vm;
-                if (blk_src) c = inject_at(c, vm, a.src, a.wav, a.ns, a.nt, a.it2, z, x, m);
-                const bool onrec = z == g.nb && x >= g.nb && x < g.nb + g.nx1 && m >= g.nb && m < g.nb + g.ny1;
-                if (a.tri2 && onrec) c = c + vm * a.tri2[(m - g.nb) * g.nx1 + (x - g.nb)];
-                c = c * fz * fx[r] * fym;
-                const bool own = zout && xout[r];
-                bst<0>(c, w2, own ? oc[r] : kOOB);
-                if (a.rec2 && own && onrec) a.rec2[(m - g.nb) * g.nx1 + (x - g.nb)] = c;
-                if (a.snap2 && own && z >= g.nb && z < g.nb + g.nz1 && x >= g.nb && x < g.nb + g.nx1 && m >= g.nb &&
-                    m < g.nb + g.ny1)
-                    a.snap2[((size_t)(m - g.nb) * g.nx1 + (x - g.nb)) * g.nz1 + (z - g.nb)] = c;
+                c2[r] = u + (u - p);
+                c2[r] = c2[r] + lap * vq[r][0];
+            }
+            if (blk_src) {
+#pragma unroll
+                for (int r = 0; r < RPW; ++r)
+                    c2[r] = inject_at(c2[r], vq[r][0], a.src, a.wav, a.ns, a.nt, a.it2, z, xr[r], m);
+            }
+            if (a.tri2 && rtile && mint) {
+#pragma unroll
+                for (int r = 0; r < RPW; ++r)
+                    if (z == g.nb && xr[r] >= g.nb && xr[r] < g.nb + g.nx1)
+                        c2[r] = c2[r] + vq[r][0] * a.tri2[(m - g.nb) * g.nx1 + (xr[r] - g.nb)];
+            }
+#pragma unroll
+            for (int r = 0; r < RPW; ++r) {
+                c2[r] = c2[r] * fz * fx[r] * fym;
+                st(c2[r], rO2, oown[r], sm);
+            }
+            if (a.rec2 && rtile && mint) {
+#pragma unroll
+                for (int r = 0; r < RPW; ++r)
+                    if (oown[r] != kOOB && z == g.nb && xr[r] >= g.nb && xr[r] < g.nb + g.nx1)
+                        a.rec2[(m - g.nb) * g.nx1 + (xr[r] - g.nb)] = c2[r];
+            }
+            if (a.snap2 && mint) {
+#pragma unroll
+                for (int r = 0; r < RPW; ++r) {
+                    const int x = xr[r];
+                    if (oown[r] != kOOB && z >= g.nb && z < g.nb + g.nz1 && x >= g.nb && x < g.nb + g.nx1)
+                        a.snap2[((size_t)(m - g.nb) * g.nx1 + (x - g.nb)) * g.nz1 + (z - g.nb)] = c2[r];
+                }
             }
         }
 #pragma unroll
@@ -1180,7 +1233,7 @@ const Variant3 kVariants3[] = {{16, 4, 0}, {8, 4, 0},  {16, 8, 0},  {32, 8, 0}, 
                                {16, 8, 1}, {8, 8, 1},  {16, 4, 1},  {32, 8, 1},  {16, 16, 1}, {32, 16, 1},
                                {16, 8, 2}, {16, 4, 2}, {32, 8, 2},  {8, 4, 2},   {16, 8, 3}, {16, 4, 3},
                                {32, 16, 2}, {8, 8, 2}, {16, 4, 4}, {8, 4, 4}, {16, 8, 4}, {32, 8, 4},
-                               {16, 8, 5}, {32, 8, 5}, {24, 8, 5}, {16, 4, 5}};
+                               {16, 16, 5}, {16, 8, 5}, {32, 16, 5}, {24, 8, 5}};
 constexpr int kNumVariants3 = sizeof(kVariants3) / sizeof(kVariants3[0]);
 constexpr int kLeanDefault = 12;
 constexpr int kLeanOnlyDefault = 6;  // lean kernel when fields exceed the pipelined kernels' 2 GiB windows
@@ -1262,8 +1315,8 @@ long long resident_blocks(pw_afd* p, int vi)
         PW_OCC(18, (k_afd3d_pipe<2, 16, 0, 0>)) PW_OCC(19, (k_afd3d_pipe<1, 8, 0, 0>))
         PW_OCC(20, (k_afd3d_pipe<4, 4, 0, 1>)) PW_OCC(21, (k_afd3d_pipe<2, 4, 0, 1>))
         PW_OCC(22, (k_afd3d_pipe<2, 8, 0, 1>)) PW_OCC(23, (k_afd3d_pipe<4, 8, 0, 1>))
-        PW_OCC(24, (k_afd3d_2step<2, 8>)) PW_OCC(25, (k_afd3d_2step<4, 8>))
-        PW_OCC(26, (k_afd3d_2step<3, 8>)) PW_OCC(27, (k_afd3d_2step<4, 4>))
+        PW_OCC(24, (k_afd3d_2step<1, 16>)) PW_OCC(25, (k_afd3d_2step<2, 8>))
+        PW_OCC(26, (k_afd3d_2step<2, 16>)) PW_OCC(27, (k_afd3d_2step<3, 8>))
 #undef PW_OCC
         default: break;
     }
@@ -1373,10 +1426,10 @@ int afd_launch_2step(pw_afd* p, int vi, const Step2& a, hipEvent_t e0, hipEvent_
         hipExtLaunchKernelGGL((k_afd3d_2step<R_, W_>), dim3(grid), dim3(64 * W_), 0, s, e0, e1, 0, g, a, ntz, ntx, \
                               nyc, nblk);                                                                 \
         break;
-        PW_V2(24, 2, 8)
-        PW_V2(25, 4, 8)
-        PW_V2(26, 3, 8)
-        PW_V2(27, 4, 4)
+        PW_V2(24, 1, 16)
+        PW_V2(25, 2, 8)
+        PW_V2(26, 2, 16)
+        PW_V2(27, 3, 8)
 #undef PW_V2
         default:
             PW_FAIL(ctx, PW_ERR_ARG, "not a two-step 3D kernel variant");
@@ -1837,6 +1890,7 @@ int pw_afd_autotune(pw_afd* p, int steps, int* variant_out, int* chunk_out)
         const int nst = std::max(2, steps + (steps & 1));  // even: two-step kernels run whole launches
         const int mode = p->have_src ? 0 : 1;                 // time the plan's own kind of run
         if (mode == 1 && !p->have_tri) PW_FAIL(ctx, PW_ERR_STATE, "autotune: set sources or time-reversal data first");
+        std::vector<std::tuple<int, int, float>> timed;
         for (int v = 0; v < kNumVariants3; ++v) {
             if (!variant_ok(p, v)) continue;
             for (int c : chunks) {
@@ -1850,12 +1904,17 @@ int pw_afd_autotune(pw_afd* p, int steps, int* variant_out, int* chunk_out)
                 PW_HIP(ctx, hipEventSynchronize(e1));
                 float ms = 0.f;
                 PW_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
-                if (ms < best) {
-                    best = ms;
-                    choice = std::make_pair(v, c);
-                }
+                timed.emplace_back(v, c, ms);
+                best = std::min(best, ms);
             }
         }
+        // the first configuration (table order) within 1.5% of the fastest: timing noise between
+        // near-equal kernels does not flip the choice from run to run
+        for (const auto& t : timed)
+            if (std::get<2>(t) <= best * 1.015f) {
+                choice = std::make_pair(std::get<0>(t), std::get<1>(t));
+                break;
+            }
         p->variant = saved_v;
         p->ny_chunk = saved_c;
         hipEventDestroy(e0);

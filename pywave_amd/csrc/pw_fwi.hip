@@ -565,21 +565,32 @@ __global__ void k_add_slots(long long n, int nslot, const T* __restrict__ g, lon
 // exact mode: dJ/dv on the model grid = sum over the cells pad2d replicates from each model cell
 // of dJ/dm * 2 v dt^2, over all slots
 template <typename T>
-__global__ void k_gm_to_gv(FGeo<T> g, int nslot, const T* __restrict__ gm, const T* __restrict__ vpad, T dt2,
-                           T* __restrict__ gv)
+__global__ void k_gm_fold_z(FGeo<T> g, int nslot, const T* __restrict__ gm, const T* __restrict__ vpad, T dt2,
+                            double* __restrict__ t)
+{
+    // t[x][iz] = sum over the padded rows that replicate model row iz (pad2d: the first and last
+    // rows absorb the sponge) of sum_slots dJ/dm * 2 v dt^2, for every padded column x
+    const int iz = blockIdx.x * blockDim.x + threadIdx.x, x = blockIdx.y;
+    if (iz >= g.nz) return;
+    const int za = iz == 0 ? 0 : iz + g.nb, zb = iz == g.nz - 1 ? g.pnz : iz + g.nb + 1;
+    double acc = 0.0;
+    for (int z = za; z < zb; ++z) {
+        const long long j = (long long)x * g.ldz + z;
+        double s = 0.0;
+        for (int k = 0; k < nslot; ++k) s += (double)gm[k * g.fs + j];
+        acc += s * 2.0 * (double)vpad[j] * (double)dt2;
+    }
+    t[(long long)x * g.nz + iz] = acc;
+}
+
+template <typename T>
+__global__ void k_gm_fold_x(FGeo<T> g, const double* __restrict__ t, T* __restrict__ gv)
 {
     const int iz = blockIdx.x * blockDim.x + threadIdx.x, ix = blockIdx.y;
     if (iz >= g.nz) return;
-    const int za = iz == 0 ? 0 : iz + g.nb, zb = iz == g.nz - 1 ? g.pnz : iz + g.nb + 1;
     const int xa = ix == 0 ? 0 : ix + g.nb, xb = ix == g.nx - 1 ? g.pnx : ix + g.nb + 1;
     double acc = 0.0;
-    for (int x = xa; x < xb; ++x)
-        for (int z = za; z < zb; ++z) {
-            const long long j = (long long)x * g.ldz + z;
-            double s = 0.0;
-            for (int k = 0; k < nslot; ++k) s += (double)gm[k * g.fs + j];
-            acc += s * 2.0 * (double)vpad[j] * (double)dt2;
-        }
+    for (int x = xa; x < xb; ++x) acc += t[(long long)x * g.nz + iz];
     gv[(long long)ix * g.nz + iz] = (T)((double)gv[(long long)ix * g.nz + iz] + acc);
 }
 
@@ -825,6 +836,7 @@ struct pw_fwi {
     int* slot_shot = nullptr;
     double* partial = nullptr;
     double* dscalar = nullptr;
+    double* gfold = nullptr;  // dJ/dv folded over padded rows: [padnx][nz]
     int nseg = 0;
     long long seg_cap = 0;  // elements of the segment buffer
     bool have_wav = false, have_obs = false;
@@ -882,7 +894,7 @@ void fwi_free(pw_fwi* p)
 {
     void* bufs[] = {p->vel, p->vpad, p->m, p->sfz, p->sfx, p->wav, p->X, p->Y, p->A, p->B, p->rec, p->res,
                     p->obs, p->W, p->ckpt, p->ckpt_snap, p->seg, p->gm, p->gslot, p->gtot, p->patch,
-                    p->tmpv, p->work, p->shots, p->slot_shot, p->partial, p->dscalar};
+                    p->tmpv, p->work, p->shots, p->slot_shot, p->partial, p->dscalar, p->gfold};
     for (void* b : bufs)
         if (b) hipFree(b);
 }
@@ -1227,8 +1239,10 @@ int gradient_all(pw_fwi* p, int mode, double* fcost)
                                p->nzx, (T*)p->gtot);
         } else if (mode == 1) {
             FGeo<T> g = geo<T>(p);
-            hipLaunchKernelGGL(k_gm_to_gv<T>, dim3((a.nz + 255) / 256, a.nx), dim3(256), 0, s, g, nb_,
-                               (const T*)p->gm, (const T*)p->vpad, dt2_of<T>(p), (T*)p->gtot);
+            hipLaunchKernelGGL(k_gm_fold_z<T>, dim3((a.nz + 255) / 256, a.padnx), dim3(256), 0, s, g, nb_,
+                               (const T*)p->gm, (const T*)p->vpad, dt2_of<T>(p), p->gfold);
+            hipLaunchKernelGGL(k_gm_fold_x<T>, dim3((a.nz + 255) / 256, a.nx), dim3(256), 0, s, g,
+                               (const double*)p->gfold, (T*)p->gtot);
         }
         PW_HIPS(hipGetLastError());
         PW_HIPS(hipEventRecord(e2, s));
@@ -1362,6 +1376,7 @@ pw_fwi* pw_fwi_create(pw_ctx* ctx, const float* pararray, int batch, int precisi
     rc = rc ? rc : dmalloc(p, &p->gm, fld * e);
     rc = rc ? rc : dmalloc(p, &p->gslot, (size_t)p->nslot * p->nzx * e);
     rc = rc ? rc : dmalloc(p, &p->gtot, p->nzx * 8);
+    rc = rc ? rc : dmalloc(p, (void**)&p->gfold, (size_t)a.nz * a.padnx * 8);
     rc = rc ? rc : dmalloc(p, &p->patch, (size_t)p->nslot * PMAX * PMAX * e);
     rc = rc ? rc : dmalloc(p, (void**)&p->tmpv, p->nzx * 4);
     rc = rc ? rc : dmalloc(p, (void**)&p->work, (size_t)std::max(a.nz, a.nx) * (std::max(a.nz, a.nx) + 8) * 4);
@@ -1654,8 +1669,10 @@ int pw_fwi_adjoint_dev(pw_fwi* p, const void* d_vel, const void* d_gtraces, void
             }
             if (!rc) {
                 FGeo<float> g = geo<float>(p);
-                hipLaunchKernelGGL(k_gm_to_gv<float>, dim3((a.nz + 255) / 256, a.nx), dim3(256), 0, s, g, nb_,
-                                   (const float*)p->gm, (const float*)p->vpad, dt2_of<float>(p), (float*)p->gtot);
+                hipLaunchKernelGGL(k_gm_fold_z<float>, dim3((a.nz + 255) / 256, a.padnx), dim3(256), 0, s, g, nb_,
+                               (const float*)p->gm, (const float*)p->vpad, dt2_of<float>(p), p->gfold);
+            hipLaunchKernelGGL(k_gm_fold_x<float>, dim3((a.nz + 255) / 256, a.nx), dim3(256), 0, s, g,
+                               (const double*)p->gfold, (float*)p->gtot);
             }
         } else {
             if (!reuse) rc = forward_batch<double>(p, nb_, true, 1, true);
@@ -1669,8 +1686,10 @@ int pw_fwi_adjoint_dev(pw_fwi* p, const void* d_vel, const void* d_gtraces, void
             }
             if (!rc) {
                 FGeo<double> g = geo<double>(p);
-                hipLaunchKernelGGL(k_gm_to_gv<double>, dim3((a.nz + 255) / 256, a.nx), dim3(256), 0, s, g, nb_,
-                                   (const double*)p->gm, (const double*)p->vpad, dt2_of<double>(p), (double*)p->gtot);
+                hipLaunchKernelGGL(k_gm_fold_z<double>, dim3((a.nz + 255) / 256, a.padnx), dim3(256), 0, s, g, nb_,
+                               (const double*)p->gm, (const double*)p->vpad, dt2_of<double>(p), p->gfold);
+            hipLaunchKernelGGL(k_gm_fold_x<double>, dim3((a.nz + 255) / 256, a.nx), dim3(256), 0, s, g,
+                               (const double*)p->gfold, (double*)p->gtot);
             }
         }
         if (rc) return rc;
