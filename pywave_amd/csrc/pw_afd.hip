@@ -940,6 +940,9 @@ struct TriStep {
     const float* __restrict__ fz1;           // one damping factor per index (faces do not overlap)
     const float* __restrict__ fx1;
     const float* __restrict__ fy1;
+    const float* pool;                       // the 2G fields back to back (pipelined kernel)
+    unsigned pool_bytes;
+    int off_cur[G], off_pv[G];               // element offsets of cur[f], pv[f] in the pool
 };
 
 template <int G>
@@ -1089,13 +1092,17 @@ __global__ __launch_bounds__(64 * WAVES) void k_tri3d_pipe(Geo g, TriStep<G> a, 
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int z = z0 + lane, x = x0 + w;
     const int ldz = g.ldz;
-    const long long ldxy = g.ldxy;
+    // one resource over the pool of all fields (field and plane go in the scalar offset), one
+    // over vv, and a size-0 resource for planes outside the grid
     const int bias = 2 * ldz + 2;
-    const unsigned wbytes = (unsigned)((ldxy + 2 * bias) * 4);
+    const unsigned pstep = (unsigned)(g.ldxy * 4);
+    const auto rF = plane_rsrc(a.pool - bias, a.pool_bytes),
+               rV = plane_rsrc(a.vv - bias, (unsigned)((g.ldxy * g.ny2 + 2 * bias) * 4)),
+               rN = plane_rsrc(a.pool - bias, 0u);
     const bool zact = z < g.nz, xact = x < g.nx;
     const unsigned oc = (zact && xact) ? (unsigned)(bias + x * ldz + z) * 4u : kOOB;
     const int zh = lane < 2 ? z0 - 2 + lane : z0 + TZ - 2 + lane;
-    const int lzh = lane < 2 ? lane : TZ + lane;
+    const int lzh = lane < 4 ? (lane < 2 ? lane : TZ + lane) : lane + 2;  // others rewrite their centre
     const unsigned ohz = (lane < 4 && zh >= 0 && zh < g.nz && xact) ? (unsigned)(bias + x * ldz + zh) * 4u : kOOB;
     const int xh = w < 2 ? x0 - 2 + w : x0 + TX - 2 + w;
     const int lrh = w < 2 ? w : TX + w;
@@ -1103,70 +1110,79 @@ __global__ __launch_bounds__(64 * WAVES) void k_tri3d_pipe(Geo g, TriStep<G> a, 
     const float fz = a.fz1[z], fx = a.fx1[x];
     const bool zxin = z >= 2 && z < g.nz2 - 2 && x >= 2 && x < g.nx2 - 2;
     // receiver row (z == nb) and image interior, per (z, x); the y range is checked per plane
+    const bool rtile = g.nb >= z0 && g.nb < z0 + TZ;  // block-uniform
     const bool rx = x >= g.nb && x < g.nb + g.nx1;
     const unsigned orc = (z == g.nb && rx) ? (unsigned)(x - g.nb) : kOOB;  // element index in the row
     const bool ins = zact && rx && z >= g.nb && z < g.nb + g.nz1;
     const unsigned oim = ins ? (unsigned)((x - g.nb) * g.nz1 + (z - g.nb)) * 8u : kOOB;
     const unsigned rbytes = (unsigned)(g.nx1 * 4), mbytes = (unsigned)g.nx1, ibytes = (unsigned)(g.nx1 * g.nz1 * 8);
-
-    auto fplane = [&](const float* f, int yy) { return plane_rsrc(f + yy * ldxy - bias, wbytes); };
+    auto ld = [](__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+    };
+    auto yok = [&](int yy) { return yy >= 0 && yy < g.ny; };
     auto yrec = [&](int yy) { return yy >= g.nb && yy < g.nb + g.ny1; };
     // resource bases depend on the (uniform) plane only; the lane's receiver goes in the offset
     auto ld_dat = [&](int yy) {
-        const bool yr = yrec(yy);
-        const auto r = plane_rsrc(a.dat + (long long)(yr ? yy - g.nb : 0) * g.nx1, rbytes);
-        return bld<0>(r, orc * 4u, yr && orc != kOOB);
+        const bool yr = rtile && yrec(yy);
+        const auto r = plane_rsrc(a.dat + (long long)(yr ? yy - g.nb : 0) * g.nx1, yr ? rbytes : 0u);
+        return ld(r, orc != kOOB ? orc * 4u : kOOB, 0u);
     };
     auto ld_bits = [&](int yy) -> unsigned {
-        const bool yr = yrec(yy);
+        const bool yr = rtile && yrec(yy);
         const auto r = __builtin_amdgcn_make_buffer_rsrc((void*)(a.mbits + (long long)(yr ? yy - g.nb : 0) * g.nx1),
-                                                         (short)0, (int)mbytes, 0x00020000);
-        return __builtin_amdgcn_raw_buffer_load_b8(r, (yr && orc != kOOB) ? orc : kOOB, 0, 0);
+                                                         (short)0, (int)(yr ? mbytes : 0u), 0x00020000);
+        return __builtin_amdgcn_raw_buffer_load_b8(r, orc, 0, 0);
     };
     auto img_rsrc = [&](int yy) {
-        return __builtin_amdgcn_make_buffer_rsrc((void*)(a.img + (long long)(yrec(yy) ? yy - g.nb : 0) * g.nx1 * g.nz1),
-                                                 (short)0, (int)ibytes, 0x00020000);
+        const bool ok = a.img && yrec(yy);
+        return __builtin_amdgcn_make_buffer_rsrc((void*)(a.img + (long long)(ok ? yy - g.nb : 0) * g.nx1 * g.nz1),
+                                                 (short)0, (int)(ok ? ibytes : 0u), 0x00020000);
     };
     auto ld_img = [&](int yy) -> double {
-        const bool ok = a.img && yrec(yy) && oim != kOOB;
-        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(img_rsrc(yy), ok ? oim : kOOB, 0, 0));
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(img_rsrc(yy), oim, 0, 0));
     };
 
     float q[G][5], pr[G], hz[G], hx[G];
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
         const int yy = y0 - 2 + k;
-        const bool yv = yy >= 0 && yy < g.ny;
+        const auto rs = yok(yy) ? rF : rN;
+        const unsigned so = yok(yy) ? (unsigned)yy * pstep : 0u;
 #pragma unroll
-        for (int f = 0; f < G; ++f) q[f][k] = bld<0>(fplane(a.cur[f], yy), oc, yv);
+        for (int f = 0; f < G; ++f) q[f][k] = ld(rs, oc, so + (unsigned)a.off_cur[f] * 4u);
     }
+    {
+        const unsigned so = (unsigned)y0 * pstep;
 #pragma unroll
-    for (int f = 0; f < G; ++f) {
-        const auto rc = fplane(a.cur[f], y0);
-        pr[f] = bld<0>(fplane(a.pv[f], y0), oc, true);
-        hz[f] = bld<0>(rc, ohz, true);
-        hx[f] = bld<0>(rc, ohx, true);
+        for (int f = 0; f < G; ++f) {
+            pr[f] = ld(rF, oc, so + (unsigned)a.off_pv[f] * 4u);
+            hz[f] = ld(rF, ohz, so + (unsigned)a.off_cur[f] * 4u);
+            hx[f] = ld(rF, ohx, so + (unsigned)a.off_cur[f] * 4u);
+        }
     }
-    float vc = bld<0>(fplane(a.vv, y0), oc, true);
+    float vc = ld(rV, oc, (unsigned)y0 * pstep);
     float dv = ld_dat(y0);
     unsigned bits = ld_bits(y0);
     double im = ld_img(y0);
     float fyv = a.fy1[y0];
+    unsigned s0 = (unsigned)y0 * pstep;
     int buf = 0;
     for (int y = y0; y < y1; ++y) {
         const int yn = y + 1;
         const bool nv = yn < y1;
         const bool n3 = nv && y + 3 < g.ny;
+        const unsigned s1 = s0 + pstep, s3 = s0 + 3 * pstep;
+        const auto r1 = nv ? rF : rN, r3 = n3 ? rF : rN;
         float nq[G], npr[G], nhz[G], nhx[G];
 #pragma unroll
         for (int f = 0; f < G; ++f) {
-            const auto rc = fplane(a.cur[f], yn);
-            nq[f] = bld<0>(fplane(a.cur[f], y + 3), oc, n3);
-            npr[f] = bld<0>(fplane(a.pv[f], yn), oc, nv);
-            nhz[f] = bld<0>(rc, ohz, nv);
-            nhx[f] = bld<0>(rc, ohx, nv);
+            const unsigned oc_f = (unsigned)a.off_cur[f] * 4u, op_f = (unsigned)a.off_pv[f] * 4u;
+            nq[f] = ld(r3, oc, s3 + oc_f);
+            npr[f] = ld(r1, oc, s1 + op_f);
+            nhz[f] = ld(r1, ohz, s1 + oc_f);
+            nhx[f] = ld(r1, ohx, s1 + oc_f);
         }
-        const float nvc = bld<0>(fplane(a.vv, yn), oc, nv);
+        const float nvc = ld(nv ? rV : rN, oc, s1);
         const float ndv = ld_dat(yn);
         const unsigned nbits = ld_bits(yn);
         const double nim = ld_img(yn);
@@ -1175,13 +1191,13 @@ __global__ __launch_bounds__(64 * WAVES) void k_tri3d_pipe(Geo g, TriStep<G> a, 
         for (int f = 0; f < G; ++f) {
             float(*L)[LW] = lds[f][buf];
             L[w + 2][lane + 2] = q[f][2];
-            if (lane < 4) L[w + 2][lzh] = hz[f];
+            L[w + 2][lzh] = lane < 4 ? hz[f] : q[f][2];
             if (w < 4) L[lrh][lane + 2] = hx[f];
         }
         __syncthreads();
         const bool inner = zxin && y >= 2 && y < g.ny2 - 2;
-        const bool onrec = orc != kOOB && yrec(y);
-        float prod = 1.f;
+        const bool rplane = rtile && yrec(y);  // uniform
+        float c[G];
 #pragma unroll
         for (int f = 0; f < G; ++f) {
             const float(*L)[LW] = lds[f][buf];
@@ -1196,18 +1212,26 @@ __global__ __launch_bounds__(64 * WAVES) void k_tri3d_pipe(Geo g, TriStep<G> a, 
                       + C2 * (L[lr][lane] + L[lr][lane + 4]) * g.idz;
             lap = inner ? lap : 0.f;
             const float p = pr[f] * fz * fx * fyv;
-            float c = u + (u - p);
-            c = c + lap * vc;
-            if (onrec && ((bits >> f) & 1u)) c = c + vc * dv;
-            c = c * fz * fx * fyv;
-            bst<0>(c, fplane(a.pv[f], y), oc);
-            prod = f == 0 ? c * c : prod * c * c;
+            c[f] = u + (u - p);
+            c[f] = c[f] + lap * vc;
+        }
+        if (rplane) {
+#pragma unroll
+            for (int f = 0; f < G; ++f)
+                if (orc != kOOB && ((bits >> f) & 1u)) c[f] = c[f] + vc * dv;
+        }
+        float prod = 1.f;
+#pragma unroll
+        for (int f = 0; f < G; ++f) {
+            c[f] = c[f] * fz * fx * fyv;
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, c[f]), rF, oc,
+                                                  s0 + (unsigned)a.off_pv[f] * 4u, 0);
+            prod = f == 0 ? c[f] * c[f] : prod * c[f] * c[f];
         }
         if (a.img) {
-            const unsigned o = (yrec(y) && oim != kOOB) ? oim : kOOB;
             const double v = im + (double)prod;
             using u32x2 = unsigned __attribute__((ext_vector_type(2)));
-            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), img_rsrc(y), o, 0, 0);
+            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), img_rsrc(y), oim, 0, 0);
         }
 #pragma unroll
         for (int f = 0; f < G; ++f) {
@@ -1215,6 +1239,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_tri3d_pipe(Geo g, TriStep<G> a, 
             pr[f] = npr[f]; hz[f] = nhz[f]; hx[f] = nhx[f];
         }
         vc = nvc; dv = ndv; bits = nbits; im = nim; fyv = nfy;
+        s0 = s1;
         buf ^= 1;
     }
 }
@@ -1986,7 +2011,9 @@ struct pw_tri {
     int nt = 0, G = 0, jsnap = 0;
     float dt = 0.f;
     size_t field_elems = 0;
-    float* d_f[8] = {nullptr};  // 2*G fields
+    float* d_f[8] = {nullptr};  // 2*G fields (inside d_pool when the pipelined kernel runs)
+    float* d_pool = nullptr;
+    unsigned pool_bytes = 0;  // resource size over the pool (pipelined kernel)
     float* d_vv = nullptr;
     float2 *d_fz = nullptr, *d_fx = nullptr, *d_fy = nullptr;
     float* d_dat = nullptr;  // [nt][ny1][nx1]
@@ -2003,8 +2030,12 @@ namespace {
 
 void tri_free(pw_tri* p)
 {
-    for (int i = 0; i < 8; ++i)
-        if (p->d_f[i]) hipFree(p->d_f[i]);
+    if (p->d_pool) {
+        hipFree(p->d_pool);
+    } else {
+        for (int i = 0; i < 8; ++i)
+            if (p->d_f[i]) hipFree(p->d_f[i]);
+    }
     void* b[] = {p->d_vv, p->d_fz, p->d_fx, p->d_fy, p->d_dat, p->d_tmp, p->d_mask, p->d_img,
                  p->d_mbits, p->d_fz1, p->d_fx1, p->d_fy1};
     for (void* x : b)
@@ -2062,6 +2093,13 @@ int tri_enqueue(pw_tri* p)
         a.img = (p->jsnap > 0 && it % p->jsnap == 0) ? p->d_img : nullptr;
         a.mbits = p->d_mbits;
         a.fz1 = p->d_fz1; a.fx1 = p->d_fx1; a.fy1 = p->d_fy1;
+        a.pool = p->d_pool;
+        a.pool_bytes = p->pool_bytes;
+        if (p->d_pool)
+            for (int f = 0; f < G; ++f) {
+                a.off_cur[f] = (int)(A[f] - p->d_pool);
+                a.off_pv[f] = (int)(B[f] - p->d_pool);
+            }
         tri_launch<G>(p, a, s);
         PW_HIP(ctx, hipGetLastError());
         for (int f = 0; f < G; ++f) std::swap(A[f], B[f]);
@@ -2119,7 +2157,17 @@ pw_tri* pw_tri_create(pw_ctx* ctx, int nz, int nx, int ny, float dz, float dx, f
     pipe = pipe && (double)(g.ldxy + 4 * g.ldz + 4) * 4.0 < 2147483648.0 && (double)nx * nz * 8.0 < 2147483648.0;
     p->pipe_ok = pipe;
     bool ok = true;
-    for (int i = 0; i < 2 * ngroups && ok; ++i) ok = hipMalloc(&p->d_f[i], p->field_elems * 4) == hipSuccess;
+    // the pipelined kernel addresses all 2G fields through one buffer resource (< 2 GiB)
+    const double pool_b = ((double)2 * ngroups * p->field_elems + 4.0 * g.ldz + 4) * 4.0;
+    pipe = pipe && pool_b < 2147483648.0;
+    p->pipe_ok = pipe;
+    if (pipe) {
+        ok = hipMalloc(&p->d_pool, (size_t)2 * ngroups * p->field_elems * 4) == hipSuccess;
+        for (int i = 0; i < 2 * ngroups && ok; ++i) p->d_f[i] = p->d_pool + (size_t)i * p->field_elems;
+        p->pool_bytes = (unsigned)((2LL * ngroups * (long long)p->field_elems + 2LL * (2 * g.ldz + 2)) * 4);
+    } else {
+        for (int i = 0; i < 2 * ngroups && ok; ++i) ok = hipMalloc(&p->d_f[i], p->field_elems * 4) == hipSuccess;
+    }
     if (ok && pipe)
         ok = hipMalloc(&p->d_fz1, fz1.size() * 4) == hipSuccess && hipMalloc(&p->d_fx1, fx1.size() * 4) == hipSuccess &&
              hipMalloc(&p->d_fy1, fy1.size() * 4) == hipSuccess && hipMalloc(&p->d_mbits, nrec) == hipSuccess &&

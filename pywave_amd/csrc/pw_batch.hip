@@ -50,17 +50,13 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t field_rsrc(const float* p, uns
     return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)bytes, 0x00020000);
 }
 
-__device__ __forceinline__ float bld(__amdgpu_buffer_rsrc_t r, unsigned off, bool ok)
-{
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, ok ? off : kOOB, 0, 0));
-}
-
 // One 64 x XC tile of one shot per block (one wave), streamed along x with a 5-column register
 // queue and an LDS z row.  Loads for column x+1 (prev, vv, z halo, x damping pair) and the centre
 // of column x+3 are issued before column x is computed.  Field accesses go through one buffer
-// resource per shot field with 32-bit offsets; cells outside the shot's updated domain (zero in
-// every field) get an out-of-range offset, so they are neither read nor written and need no
-// branches.
+// resource per shot field: the lane's cell in the column is a fixed vector offset, the column a
+// scalar offset; lanes outside the shot's updated domain (zero in every field), and every lane
+// of a column outside it, get an out-of-range offset, so such cells are neither read nor
+// written and need no branches.
 __global__ __launch_bounds__(64) void k_afd2d_batch(BatchArgs a)
 {
     const int tile = blockIdx.x;
@@ -86,36 +82,48 @@ __global__ __launch_bounds__(64) void k_afd2d_batch(BatchArgs a)
     const bool zin = z >= 2 && z < sh.nz2 - 2;
     const float2 fzz = zact ? dz_[z] : make_float2(1.f, 1.f);
     const int zh = lane < 2 ? z0 - 2 + lane : z0 + 62 + lane;  // z halo of lanes 0..3
-    const int lzh = lane < 2 ? lane : 64 + lane;
-    const bool zhv = lane < 4 && zh >= 0 && zh < sh.nz;
-    const unsigned oz = (unsigned)(bias + z) * 4u, ozh = (unsigned)(bias + zh) * 4u, ocol = (unsigned)L * 4u;
+    // lanes 0..3 stage the z halo, the others rewrite their own centre (no branch)
+    const int lzh = lane < 4 ? (lane < 2 ? lane : 64 + lane) : lane + 2;
+    // lane offsets are fixed; the column goes in the scalar offset; a column outside the shot
+    // gives every lane the out-of-range offset
+    const unsigned oz = zact ? (unsigned)(bias + z) * 4u : kOOB;
+    const unsigned ozh = (lane < 4 && zh >= 0 && zh < sh.nz) ? (unsigned)(bias + zh) * 4u : kOOB;
+    const unsigned ocol = (unsigned)L * 4u;
     auto xv = [&](int x) { return x >= 0 && x < sh.nx; };
+    auto ld = [](__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+    };
     bool blk_src = false;
     for (int s = 0; s < sh.ns; ++s) {
         const int* cz = a.src + 3 * (sh.src_off + s);
         blk_src |= cz[0] >= z0 - 1 && cz[0] <= z0 + 64 && cz[1] >= x0 - 1 && cz[1] <= x1;
     }
+    const bool rtile = sh.nb >= z0 && sh.nb < z0 + 64;  // the receiver row is in this tile
     float q[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
         const int xx = x0 - 2 + k;
-        q[k] = bld(rc, oz + (unsigned)xx * ocol, zact && xv(xx));
+        q[k] = ld(rc, xv(xx) ? oz : kOOB, xv(xx) ? (unsigned)xx * ocol : 0u);
     }
-    float pr = bld(rp, oz + (unsigned)x0 * ocol, zact);
-    float vc = bld(rv, oz + (unsigned)x0 * ocol, zact);
-    float hz = bld(rc, ozh + (unsigned)x0 * ocol, zhv);
+    float pr = ld(rp, oz, (unsigned)x0 * ocol);
+    float vc = ld(rv, oz, (unsigned)x0 * ocol);
+    float hz = ld(rc, ozh, (unsigned)x0 * ocol);
     float2 fxx = dx_[x0];
     int buf = 0;
     for (int x = x0; x < x1; ++x) {
         const int xn = x + 1;
         const bool nv = xn < x1;
-        const float nq = bld(rc, oz + (unsigned)(x + 3) * ocol, zact && nv && x + 3 < sh.nx);
-        const float npr = bld(rp, oz + (unsigned)xn * ocol, zact && nv);
-        const float nvc = bld(rv, oz + (unsigned)xn * ocol, zact && nv);
-        const float nhz = bld(rc, ozh + (unsigned)xn * ocol, zhv && nv);
+        const bool n3 = nv && x + 3 < sh.nx;
+        // column validity is uniform: it selects the lane offset (one VALU op per load) rather
+        // than the resource (several scalar ops), the scalar side being the busier one here
+        const unsigned sn = nv ? (unsigned)xn * ocol : 0u, s3 = n3 ? (unsigned)(x + 3) * ocol : 0u;
+        const float nq = ld(rc, n3 ? oz : kOOB, s3);
+        const float npr = ld(rp, nv ? oz : kOOB, sn);
+        const float nvc = ld(rv, nv ? oz : kOOB, sn);
+        const float nhz = ld(rc, nv ? ozh : kOOB, sn);
         const float2 nfx = dx_[min(xn, sh.nx2 - 1)];  // xn may equal nx == nx2 on the last column
         row[buf][lane + 2] = q[2];
-        if (lane < 4) row[buf][lzh] = hz;
+        row[buf][lzh] = lane < 4 ? hz : q[2];
         __syncthreads();
         const float u = q[2];
         const float* R = &row[buf][lane + 2];
@@ -139,10 +147,10 @@ __global__ __launch_bounds__(64) void k_afd2d_batch(BatchArgs a)
             }
         }
         c = c * fzz.x; c = c * fzz.y; c = c * fxx.x; c = c * fxx.y;
-        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, c), rp,
-                                              zact ? oz + (unsigned)x * ocol : kOOB, 0, 0);
-        if (z == sh.nb && x >= sh.nb && x < sh.nb + sh.nx1)
-            a.rec[sh.rec_off + (long long)a.it * sh.nx1 + (x - sh.nb)] = c;
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, c), rp, oz, (unsigned)x * ocol, 0);
+        if (rtile && x >= sh.nb && x < sh.nb + sh.nx1) {
+            if (z == sh.nb) a.rec[sh.rec_off + (long long)a.it * sh.nx1 + (x - sh.nb)] = c;
+        }
         q[0] = q[1]; q[1] = q[2]; q[2] = q[3]; q[3] = q[4]; q[4] = nq;
         pr = npr; vc = nvc; hz = nhz; fxx = nfx;
         buf ^= 1;

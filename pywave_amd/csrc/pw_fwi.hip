@@ -112,15 +112,29 @@ __device__ __forceinline__ double bld<double>(__amdgpu_buffer_rsrc_t r, unsigned
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, ok ? off : kOOB, 0, 0));
 }
 
-__device__ __forceinline__ void bst(__amdgpu_buffer_rsrc_t r, unsigned off, float v)
+// the same with a scalar offset added to the lane offset
+template <typename T>
+__device__ __forceinline__ T bldo(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff);
+template <>
+__device__ __forceinline__ float bldo<float>(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
 {
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, 0, 0);
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
 }
-__device__ __forceinline__ void bst(__amdgpu_buffer_rsrc_t r, unsigned off, double v)
+template <>
+__device__ __forceinline__ double bldo<double>(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+__device__ __forceinline__ void bsto(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, float v)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
+}
+__device__ __forceinline__ void bsto(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, double v)
 {
     using u32x2 = unsigned __attribute__((ext_vector_type(2)));
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, off, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, 0);
 }
+
 
 // ---------------------------------------------------------------- forward step (+ modelling)
 // One 64 x XC tile of one shot per block (one wave), streamed along x with a 9-column register
@@ -148,27 +162,33 @@ __global__ __launch_bounds__(64) void k_fwd(FGeo<T> g, StepArgs<T> a, int ntz)
     const int lzh = lane < 4 ? lane : 64 + lane;
     const bool zhv = lane < 8 && zhz >= 0 && zhz < g.pnz;
     constexpr unsigned E = sizeof(T);
-    const unsigned oz = (unsigned)(bias + z) * E, ozh = (unsigned)(bias + zhz) * E, oc = (unsigned)L * E;
+    // lane offsets are fixed (out of range for lanes outside the grid); the column is the scalar
+    // offset; a column outside the grid gives every lane the out-of-range offset
+    const unsigned oz = zok ? (unsigned)(bias + z) * E : kOOB, ozh = zhv ? (unsigned)(bias + zhz) * E : kOOB;
+    const unsigned oc = (unsigned)L * E;
     auto xv = [&](int x) { return x >= 0 && x < g.pnx; };
+    auto ldc = [&](__amdgpu_buffer_rsrc_t r, unsigned voff, int x, bool ok) {  // column x of a field
+        return bldo<T>(r, ok ? voff : kOOB, ok ? (unsigned)x * oc : 0u);
+    };
     const T wv = a.wav[a.it];
     T q[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
         const int xx = x0 - 4 + k;
-        q[k] = bld<T>(rX, oz + (unsigned)xx * oc, zok && xv(xx));
+        q[k] = ldc(rX, oz, xx, xv(xx));
     }
-    T yv = bld<T>(rY, oz + (unsigned)x0 * oc, zok);
-    T mv = bld<T>(rM, oz + (unsigned)x0 * oc, zok);
-    T hz = bld<T>(rX, ozh + (unsigned)x0 * oc, zhv);
+    T yv = ldc(rY, oz, x0, true);
+    T mv = ldc(rM, oz, x0, true);
+    T hz = ldc(rX, ozh, x0, true);
     T xl = a.sfx[2 * x0], xh = a.sfx[2 * x0 + 1];
     int buf = 0;
     for (int x = x0; x < x1; ++x) {
         const int xn = x + 1;
         const bool nv = xn < x1;
-        const T nq = bld<T>(rX, oz + (unsigned)(x + 5) * oc, zok && nv && x + 5 < g.pnx);
-        const T nyv = bld<T>(rY, oz + (unsigned)xn * oc, zok && nv);
-        const T nmv = bld<T>(rM, oz + (unsigned)xn * oc, zok && nv);
-        const T nhz = bld<T>(rX, ozh + (unsigned)xn * oc, zhv && nv);
+        const T nq = ldc(rX, oz, x + 5, nv && x + 5 < g.pnx);
+        const T nyv = ldc(rY, oz, xn, nv);
+        const T nmv = ldc(rM, oz, xn, nv);
+        const T nhz = ldc(rX, ozh, xn, nv);
         const int xs = xn < g.pnx ? xn : g.pnx - 1;
         const T nxl = a.sfx[2 * xs], nxh = a.sfx[2 * xs + 1];
         row[buf][lane + 4] = q[4];
@@ -206,7 +226,7 @@ __global__ __launch_bounds__(64) void k_fwd(FGeo<T> g, StepArgs<T> a, int ntz)
                 term = term + a.patch[slot * PMAX * PMAX + px * PMAX + pz] * wv;
             const T p0 = sponge(yv, zl, zh, xl, xh);
             T p2 = 2 * p1 - p0 + mv * term;
-            bst(rY, oz + (unsigned)x * oc, sponge(p2, zl, zh, xl, xh));
+            bsto(rY, oz, (unsigned)x * oc, sponge(p2, zl, zh, xl, xh));
         }
 #pragma unroll
         for (int k = 0; k < 8; ++k) q[k] = q[k + 1];
@@ -341,9 +361,15 @@ __global__ __launch_bounds__(64) void k_adj_exact(FGeo<T> g, StepArgs<T> a, ExAr
     const bool zhv = lane < 8 && zhz >= 0 && zhz < g.pnz;
     const bool zhin = zhz >= 4 && zhz < g.pnz - 4;
     constexpr unsigned E = sizeof(T);
-    const unsigned oz = (unsigned)(bias + z) * E, ozh = (unsigned)(bias + zhz) * E, oc = (unsigned)L * E;
+    // lane offsets fixed (out of range outside the grid / the Laplacian rows), column = scalar offset
+    const unsigned oz = zok ? (unsigned)(bias + z) * E : kOOB, ozh = zhv ? (unsigned)(bias + zhz) * E : kOOB;
+    const unsigned ozi = (zok && zin) ? oz : kOOB, ozhi = (zhv && zhin) ? ozh : kOOB;
+    const unsigned oc = (unsigned)L * E;
     auto xv = [&](int x) { return x >= 0 && x < g.pnx; };
     auto xin = [&](int x) { return x >= 4 && x < g.pnx - 4; };
+    auto ldc = [&](__amdgpu_buffer_rsrc_t r, unsigned voff, int x, bool ok) {  // column x of a field
+        return bldo<T>(r, ok ? voff : kOOB, ok ? (unsigned)x * oc : 0u);
+    };
     // receiver injection: lane z == rz of column x takes trace sample k(x) (regular spread)
     const T* inj = a.inj + slot * a.trs + (long long)a.it * g.nr;
     const auto rI = win(inj, (unsigned)(g.nr * E));
@@ -360,18 +386,17 @@ __global__ __launch_bounds__(64) void k_adj_exact(FGeo<T> g, StepArgs<T> a, ExAr
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
         const int xx = x0 - 4 + k;
-        const bool ok = zok && xv(xx) && zin && xin(xx);
-        const T av_ = bld<T>(rA, oz + (unsigned)xx * oc, ok), mv_ = bld<T>(rM, oz + (unsigned)xx * oc, ok);
-        qa[k] = mv_ * av_;
-        qf[k] = bld<T>(rF, oz + (unsigned)xx * oc, zok && xv(xx));
+        const bool okp = xv(xx) && xin(xx);
+        qa[k] = ldc(rM, ozi, xx, okp) * ldc(rA, ozi, xx, okp);
+        qf[k] = ldc(rF, oz, xx, xv(xx));
     }
-    T av = bld<T>(rA, oz + (unsigned)x0 * oc, zok), bv = bld<T>(rB, oz + (unsigned)x0 * oc, zok);
-    T gv = bld<T>(rG, oz + (unsigned)x0 * oc, zok);
+    T av = ldc(rA, oz, x0, true), bv = ldc(rB, oz, x0, true);
+    T gv = ldc(rG, oz, x0, true);
     T ha, hf;
     {
-        const bool ok = zhv && zhin && xin(x0);
-        ha = bld<T>(rM, ozh + (unsigned)x0 * oc, ok) * bld<T>(rA, ozh + (unsigned)x0 * oc, ok);
-        hf = bld<T>(rF, ozh + (unsigned)x0 * oc, zhv);
+        const bool okp = xin(x0);
+        ha = ldc(rM, ozhi, x0, okp) * ldc(rA, ozhi, x0, okp);
+        hf = ldc(rF, ozh, x0, true);
     }
     T xl = a.sfx[2 * x0], xh = a.sfx[2 * x0 + 1];
     int k0 = rk(x0);
@@ -380,16 +405,16 @@ __global__ __launch_bounds__(64) void k_adj_exact(FGeo<T> g, StepArgs<T> a, ExAr
     for (int x = x0; x < x1; ++x) {
         const int xn = x + 1, x5 = x + 5;
         const bool nv = xn < x1;
-        const bool ok5 = zok && nv && x5 < g.pnx;
-        const bool okp5 = ok5 && zin && xin(x5);
-        const T na5 = bld<T>(rA, oz + (unsigned)x5 * oc, okp5), nm5 = bld<T>(rM, oz + (unsigned)x5 * oc, okp5);
-        const T nf5 = bld<T>(rF, oz + (unsigned)x5 * oc, ok5);
-        const T nav = bld<T>(rA, oz + (unsigned)xn * oc, zok && nv);
-        const T nbv = bld<T>(rB, oz + (unsigned)xn * oc, zok && nv);
-        const T ngv = bld<T>(rG, oz + (unsigned)xn * oc, zok && nv);
-        const bool okh = zhv && nv && zhin && xin(xn);
-        const T nha = bld<T>(rM, ozh + (unsigned)xn * oc, okh) * bld<T>(rA, ozh + (unsigned)xn * oc, okh);
-        const T nhf = bld<T>(rF, ozh + (unsigned)xn * oc, zhv && nv);
+        const bool ok5 = nv && x5 < g.pnx;
+        const bool okp5 = ok5 && xin(x5);
+        const T na5 = ldc(rA, ozi, x5, okp5), nm5 = ldc(rM, ozi, x5, okp5);
+        const T nf5 = ldc(rF, oz, x5, ok5);
+        const T nav = ldc(rA, oz, xn, nv);
+        const T nbv = ldc(rB, oz, xn, nv);
+        const T ngv = ldc(rG, oz, xn, nv);
+        const bool okh = nv && xin(xn);
+        const T nha = ldc(rM, ozhi, xn, okh) * ldc(rA, ozhi, xn, okh);
+        const T nhf = ldc(rF, ozh, xn, nv);
         const int xs = xn < g.pnx ? xn : g.pnx - 1;
         const T nxl = a.sfx[2 * xs], nxh = a.sfx[2 * xs + 1];
         const int kn = nv ? rk(xn) : -1;
@@ -418,7 +443,7 @@ __global__ __launch_bounds__(64) void k_adj_exact(FGeo<T> g, StepArgs<T> a, ExAr
                 }
             }
             T qn = 2 * av - sponge(bv, zl, zh, xl, xh) + lt + src;
-            bst(rB, oz + (unsigned)x * oc, sponge(qn, zl, zh, xl, xh));
+            bsto(rB, oz, (unsigned)x * oc, sponge(qn, zl, zh, xl, xh));
             // dJ/dm accumulation with the forward field's Laplacian (rim rows are zero)
             T lf = T(0);
             if (zin && xin(x)) {
@@ -431,7 +456,7 @@ __global__ __launch_bounds__(64) void k_adj_exact(FGeo<T> g, StepArgs<T> a, ExAr
             const int px = x - sh.sx0, pz = z - sh.sz0;
             if (px >= 0 && px < sh.pw && pz >= 0 && pz < sh.ph)
                 lf = lf + a.patch[slot * PMAX * PMAX + px * PMAX + pz] * wv;
-            bst(rG, oz + (unsigned)x * oc, gv + lf * av);
+            bsto(rG, oz, (unsigned)x * oc, gv + lf * av);
         }
 #pragma unroll
         for (int k = 0; k < 8; ++k) { qa[k] = qa[k + 1]; qf[k] = qf[k + 1]; }
