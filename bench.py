@@ -158,6 +158,45 @@ def batch_cpu_baseline(shots, total_cu, nt_cpu=400):
         return dict(value=None, unit="shots/s", cores=1, kind="reference", sample="failed: %s" % e)
 
 
+def afd2d_small_bench():
+    """The reference's own CPU-sized 2D case (101x101 three-layer model + 30-cell ABC = 161x161,
+    nt=1000, 20 Hz source, jsnap=999): launch-bound on a GPU, so reported as the latency of the
+    whole modelling run (kernels only, fields resident) and of the drop-in afd2d call (host arrays
+    in and out), beside the reference C on one core."""
+    import pywave_amd
+    from pywave_amd import AfdPlan
+    from pywave_amd.workloads import plain2d
+
+    w = plain2d()
+    vel = w["vel"]
+    plan = AfdPlan(vel, w["d"], w["nt"], w["dt"], w["nbt"], w["ct"], w["jsnap"])
+    plan.set_sources(w["src"], w["ns"])
+    plan.run()
+    total_ms, step_ms = plan.run_timed(per_step=True)
+    plan.close()
+    src = w["src"]
+    kw = dict(nt=w["nt"], dt=w["dt"], ax=[0, 10, 101], az=[0, 10, 101], ns=1, sx=[src[0]], sz=[src[1]], f=[src[2]],
+              t=[src[3]], A=[src[4]], nbt=w["nbt"], ct=w["ct"], jsnap=w["jsnap"], verb=False)
+    pywave_amd.afd2d(vel, **kw)
+    t = time.perf_counter()
+    pywave_amd.afd2d(vel, **kw)
+    call_ms = (time.perf_counter() - t) * 1e3
+    cells = 161 * 161 * w["nt"]
+    out = {"workload": "afd2d 101x101 three layers (+30 ABC: 161x161), nt=1000, jsnap=999",
+           "run_ms": round(total_ms, 3), "kernel_us_per_step": round(step_ms * 1e3, 3),
+           "dropin_call_ms": round(call_ms, 3), "Gcell_per_s_run": round(cells / (total_ms * 1e-3) / 1e9, 3)}
+    try:
+        from oracle import refrun
+        if refrun.available():
+            pack = np.array(w["src"], np.float32)
+            _, el = refrun.afd2d(vel, w["nt"], w["dt"], 10.0, 10.0, w["nbt"], w["ct"], pack, 1, jsnap=w["jsnap"])
+            out["cpu_baseline"] = dict(value=round(el * 1e3, 3), unit="ms per run", cores=1, kind="reference",
+                                       sample="reference C afd2d (fdm2d), the whole 1000-step run, single thread")
+    except Exception as e:
+        out["cpu_baseline"] = dict(value=None, unit="ms per run", cores=1, kind="reference", sample="failed: %s" % e)
+    return out
+
+
 def tri_bench():
     """Time-reversal imaging on the 128^3 workload (+30-cell ABC: 188^3 updated, nt=2000): the
     forward modelling of three sources, then the four interleaved receiver groups back-propagated
@@ -269,14 +308,15 @@ def main():
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-tri", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-small", action="store_true")
     ap.add_argument("--fwi-plain-shots", type=int, default=4)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (HBM counter calibration)")
-    ap.add_argument("--only", choices=["fwi", "tri", "batch"],
+    ap.add_argument("--only", choices=["fwi", "tri", "batch", "small"],
                     help="run one secondary workload alone and print its JSON object (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path (requested --gpus %d)"
+        print("bench.py runs on one GPU only: pywave_amd has no multi-GPU path (requested --gpus %d)"
               % args.gpus, file=sys.stderr)
         sys.exit(2)
 
@@ -284,7 +324,8 @@ def main():
         calibrate_copy()
         return
     if args.only:
-        leg = {"fwi": lambda: fwi_bench(args.fwi_plain_shots), "tri": tri_bench, "batch": batch_bench}[args.only]
+        leg = {"fwi": lambda: fwi_bench(args.fwi_plain_shots), "tri": tri_bench, "batch": batch_bench,
+               "small": afd2d_small_bench}[args.only]
         print(json.dumps({args.only: leg()}))
         return
 
@@ -339,6 +380,7 @@ def main():
     fwi = None if args.no_fwi else fwi_bench(args.fwi_plain_shots)
     tri = None if args.no_tri else tri_bench()
     batch = None if args.no_batch else batch_bench()
+    small = None if args.no_small else afd2d_small_bench()
     line = {
         "metric": "Gcell-updates/s 3D FP32 FD (% HBM roofline); FWI grad shots/s batched vs plain",
         "value": round(value, 3),
@@ -375,6 +417,7 @@ def main():
         "fwi": fwi,
         "tri3d": tri,
         "batch2d": batch,
+        "afd2d_small": small,
     }
     print(json.dumps(line))
 

@@ -2,7 +2,7 @@
 
 TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).
 
-What pins each function (tests/test_oracle_*.py):
+What pins each function (tests/test_oracle.py):
   next_fast_size, padded_sizes   brute-force enumeration of 2^a 3^b 5^c; golden afd outputs
   abc_weights                    closed form + w[nb-1] == 1; golden afd outputs (bit-exact)
   ricker                         golden wavelets from the compiled reference (rel <= 1e-6),
@@ -11,6 +11,9 @@ What pins each function (tests/test_oracle_*.py):
                                  when fed the reference's wavelet; linearity in amplitude
   fwi_prepare geometry           closed-form integer formulas (pywave/src/wave_fwiutil.c:45-60)
   fwi_forward, gradient_av, fwi  golden outputs from the compiled reference, bit-exact
+  the same with src= (passive)   compiled reference's gradient_pas_av / pfwi, bit-exact; a field
+                                 equal to the point-source term reproduces the active golden;
+                                 zero misfit of the reference at the true model; linearity
   source_map, gradient_smooth2   numpy convolution with the reflected triangle (independent form)
 """
 import ctypes
@@ -40,6 +43,7 @@ class OrcAcq(ctypes.Structure):
         ("dr_v", ctypes.c_int),
         ("r0_v", ctypes.POINTER(ctypes.c_int)), ("r02", ctypes.POINTER(ctypes.c_int)),
         ("nr2", ctypes.POINTER(ctypes.c_int)), ("bc", ctypes.POINTER(ctypes.c_float)),
+        ("srcf", ctypes.c_void_p),
     ]
 
 
@@ -176,11 +180,17 @@ def afd3d(vel, nt, dt, dx, dy, dz, nbt, ct, sx, sy, sz, rick, tri=False, dat=Non
 
 # ------------------------------------------------------------------ FWI
 
-def make_acq(pararray):
+def make_acq(pararray, src=None):
     """orc_acq from the 20-float pararray of pywave/pfwi.py:40-61 (fields as pfwi.c:697-716,
-    with the hard-coded sz=5, rz=1, nr=nx, dr=dx, r0=ox of pfwi.c:781-806)."""
+    with the hard-coded sz=5, rz=1, nr=nx, dr=dx, r0=ox of pfwi.c:781-806).  src: the passive
+    source field (nz, nx, nt, ns) of pfwi mode 4, or None; the caller keeps a reference to the
+    returned float32 copy (a._src) alive while a is used."""
     p = np.asarray(pararray, np.float32)
     a = OrcAcq()
+    a._src = None
+    if src is not None:
+        a._src = np.ascontiguousarray(np.asarray(src, np.float32).ravel(order="F"))
+        a.srcf = a._src.ctypes.data
     a.nz, a.nx = int(p[0]), int(p[1])
     a.dz, a.dx = p[2], p[3]
     a.z0, a.x0 = p[4], p[5]
@@ -200,9 +210,10 @@ def acq_geometry(a):
                 nr2=[a.nr2[i] for i in range(ns)], bc=np.array([a.bc[i] for i in range(a.nb)], np.float32))
 
 
-def fwi_forward(pararray, vel, wav):
-    """forward_modeling_a: returns data (nt, nx, ns) like pywave/pfwi.py:70-74."""
-    a = make_acq(pararray)
+def fwi_forward(pararray, vel, wav, src=None):
+    """forward_modeling_a: returns data (nt, nx, ns) like pywave/pfwi.py:70-74.  With src (nz, nx,
+    nt, ns) the passive source field replaces the point sources."""
+    a = make_acq(pararray, src)
     velf = np.ascontiguousarray(np.asarray(vel, np.float32).ravel(order="F"))
     out = np.zeros(a.ns * a.nr * a.nt, np.float32)
     lib().orc_fwi_forward(ctypes.byref(a), velf, np.ascontiguousarray(wav, np.float32), out)
@@ -211,9 +222,10 @@ def fwi_forward(pararray, vel, wav):
     return res
 
 
-def gradient_av(pararray, vel, wav, data, scaling=0.0):
-    """One gradient_av call.  data (nt, nx, ns).  Returns (grad (nz,nx), fcost, scaling)."""
-    a = make_acq(pararray)
+def gradient_av(pararray, vel, wav, data, scaling=0.0, src=None):
+    """One gradient_av call (gradient_pas_av with src).  data (nt, nx, ns).  Returns (grad (nz,nx),
+    fcost, scaling)."""
+    a = make_acq(pararray, src)
     velf = np.ascontiguousarray(np.asarray(vel, np.float32).ravel(order="F"))
     dflat = np.ascontiguousarray(np.asarray(data, np.float32).ravel(order="F"))
     g = np.zeros(a.nz * a.nx, np.float32)
@@ -226,9 +238,9 @@ def gradient_av(pararray, vel, wav, data, scaling=0.0):
     return g.reshape(nz, nx, order="F"), fc.value, sc.value
 
 
-def fwi(pararray, vel, wav, data):
-    """fwi(): returns vinv (nz, nx, niter) and the number of iterates stored."""
-    a = make_acq(pararray)
+def fwi(pararray, vel, wav, data, src=None):
+    """fwi() (pfwi() with src): returns vinv (nz, nx, niter) and the number of iterates stored."""
+    a = make_acq(pararray, src)
     niter = int(np.float32(pararray[19]))
     velf = np.ascontiguousarray(np.asarray(vel, np.float32).ravel(order="F"))
     dflat = np.ascontiguousarray(np.asarray(data, np.float32).ravel(order="F"))

@@ -57,6 +57,10 @@ typedef struct {
     int padnz, padnx, sz, rz, ds_v, s0_v, dr_v;
     int *r0_v, *r02, *nr2;  /* per shot, allocated by orc_fwi_prepare */
     float *bc;              /* nb sponge weights */
+    /* passive path (pywave/src/wave_fwigradient.c:528-870, wave_fwi.c:212-322): when set, a
+     * space-time source field [is][it][ix][iz] (nz*nx*nt*ns) replaces the point sources, all
+     * shots record at r0_v[0] + ir*dr_v, ir < nr, and fwi resets the step every 5 iterations */
+    const float *srcf;
 } orc_acq;
 
 int orc_fwi_prepare(orc_acq *a);

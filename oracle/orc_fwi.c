@@ -175,14 +175,29 @@ static void sponge(float *p, const float *bc, int pnx, int pnz, int nb)
 }
 
 /* one leapfrog step: p2 = 2 p1 - p0 + v*v*dt2*(lap p1 + rr*w) over the whole padded grid */
+/* fsrc: the passive source field of this step on the model interior (nx*nz) or NULL
+ * (wave_fwigradient.c:733-737: term[ix+nb][iz+nb] += ww3[is][it][ix][iz]) */
 static void fwi_step(const float *p0, const float *p1, float *p2, float *term, const float *vv,
-                     const float *rr, float wsrc, int pnx, int pnz, float dx2, float dz2, float dt2)
+                     const float *rr, float wsrc, const float *fsrc, int nz, int nx, int nb,
+                     int pnx, int pnz, float dx2, float dz2, float dt2)
 {
     size_t n = (size_t)pnx * pnz;
     laplace8(p1, term, pnx, pnz, dx2, dz2);
     if (rr)
         for (size_t j = 0; j < n; j++) term[j] += rr[j] * wsrc;
+    if (fsrc)
+        for (int ix = 0; ix < nx; ix++)
+            for (int iz = 0; iz < nz; iz++)
+                term[(size_t)(ix + nb) * pnz + iz + nb] += fsrc[(size_t)ix * nz + iz];
     for (size_t j = 0; j < n; j++) p2[j] = 2 * p1[j] - p0[j] + vv[j] * vv[j] * dt2 * term[j];
+}
+
+/* receivers of shot is: (first index in the trace panel, count, first column); the passive
+ * path records every shot at r0_v[0] + ir*dr_v, ir < nr (wave_fwigradient.c:710-713) */
+static void receivers(const orc_acq *a, int is, int *r02, int *nr2, int *r0v)
+{
+    if (a->srcf) { *r02 = 0; *nr2 = a->nr; *r0v = a->r0_v[0]; }
+    else { *r02 = a->r02[is]; *nr2 = a->nr2[is]; *r0v = a->r0_v[is]; }
 }
 
 int orc_fwi_forward(const orc_acq *a, const float *vel, const float *wav, float *data)
@@ -200,12 +215,16 @@ int orc_fwi_forward(const orc_acq *a, const float *vel, const float *wav, float 
         memset(p0, 0, sizeof(float) * n); memset(p1, 0, sizeof(float) * n); memset(p2, 0, sizeof(float) * n);
         int sx = a->s0_v + is * a->ds_v;
         orc_source_map(sx, a->sz, 2, 2, pnx, pnz, rr);
+        int r02, nr2, r0v;
+        receivers(a, is, &r02, &nr2, &r0v);
         for (int it = 0; it < nt; it++) {
-            for (int ir = 0; ir < a->nr2[is]; ir++) {
-                int rx = a->r0_v[is] + ir * a->dr_v;
-                dd[(size_t)(a->r02[is] + ir) * nt + it] = p1[(size_t)rx * pnz + a->rz];
+            for (int ir = 0; ir < nr2; ir++) {
+                int rx = r0v + ir * a->dr_v;
+                dd[(size_t)(r02 + ir) * nt + it] = p1[(size_t)rx * pnz + a->rz];
             }
-            fwi_step(p0, p1, p2, term, vv, rr, wav[it], pnx, pnz, dx2, dz2, dt2);
+            const float *fs = a->srcf ? a->srcf + ((size_t)is * nt + it) * a->nz * a->nx : NULL;
+            fwi_step(p0, p1, p2, term, vv, a->srcf ? NULL : rr, wav[it], fs, a->nz, a->nx, a->nb, pnx, pnz, dx2,
+                     dz2, dt2);
             float *t = p0; p0 = p1; p1 = p2; p2 = t;
             sponge(p0, a->bc, pnx, pnz, a->nb);
             sponge(p1, a->bc, pnx, pnz, a->nb);
@@ -266,11 +285,13 @@ int orc_gradient_av(const orc_acq *a, const float *vel, const float *wav, const 
         memset(pp, 0, sizeof(float) * (size_t)nr * nt);
         int sx = a->s0_v + is * a->ds_v;
         orc_source_map(sx, a->sz, 2, 2, pnx, pnz, rr);
+        int r02, nr2, r0v;
+        receivers(a, is, &r02, &nr2, &r0v);
         int wit = 0;
         for (int it = 0; it < nt; it++) {
-            for (int ir = 0; ir < a->nr2[is]; ir++) {
-                int rx = a->r0_v[is] + ir * a->dr_v;
-                pp[(size_t)(a->r02[is] + ir) * nt + it] = p1[(size_t)rx * pnz + a->rz];
+            for (int ir = 0; ir < nr2; ir++) {
+                int rx = r0v + ir * a->dr_v;
+                pp[(size_t)(r02 + ir) * nt + it] = p1[(size_t)rx * pnz + a->rz];
             }
             if (it % interval == 0) {
                 for (int ix = 0; ix < nx; ix++)
@@ -278,7 +299,8 @@ int orc_gradient_av(const orc_acq *a, const float *vel, const float *wav, const 
                         wave[(size_t)wit * nzx + (size_t)ix * nz + iz] = p1[(size_t)(ix + nb) * pnz + iz + nb];
                 wit++;
             }
-            fwi_step(p0, p1, p2, term, vv, rr, wav[it], pnx, pnz, dx2, dz2, dt2);
+            const float *fs = a->srcf ? a->srcf + ((size_t)is * nt + it) * nzx : NULL;
+            fwi_step(p0, p1, p2, term, vv, a->srcf ? NULL : rr, wav[it], fs, nz, nx, nb, pnx, pnz, dx2, dz2, dt2);
             float *t = p0; p0 = p1; p1 = p2; p2 = t;
             sponge(p0, a->bc, pnx, pnz, nb);
             sponge(p1, a->bc, pnx, pnz, nb);
@@ -297,9 +319,9 @@ int orc_gradient_av(const orc_acq *a, const float *vel, const float *wav, const 
         memset(p2, 0, sizeof(float) * n); memset(term, 0, sizeof(float) * n);
         for (int it = nt - 1; it >= 0; it--) {
             laplace8(p1, term, pnx, pnz, dx2, dz2);
-            for (int ir = 0; ir < a->nr2[is]; ir++) {
-                int rx = a->r0_v[is] + ir * a->dr_v;
-                term[(size_t)rx * pnz + a->rz] += pp[(size_t)(a->r02[is] + ir) * nt + it];
+            for (int ir = 0; ir < nr2; ir++) {
+                int rx = r0v + ir * a->dr_v;
+                term[(size_t)rx * pnz + a->rz] += pp[(size_t)(r02 + ir) * nt + it];
             }
             for (size_t j = 0; j < n; j++) p2[j] = 2 * p1[j] - p0[j] + vv[j] * vv[j] * dt2 * term[j];
             if (it % interval == 0) {
@@ -421,6 +443,7 @@ int orc_fwi(const orc_acq *a, const float *vel0, const float *wav, const float *
     o.ipair = 0; o.alpha = 1.; o.fk = fcost; o.f0 = fcost;
     int stored = 0, iter;
     for (iter = 0; iter < niter; iter++) {
+        if (a->srcf && iter % 5 == 0) o.alpha = 1.;  /* pfwi: optpar->repeat, pywave/src/wave_fwi.c:282 */
         if (iter == 0) {
             for (int i = 0; i < n; i++) dir[i] = -g[i];
         } else {

@@ -1,5 +1,5 @@
 /*
- * ref_harness.c -- command-line driver for the compiled pywave reference C.
+ * ref_harness.c -- command-line front end for the compiled pywave reference C.
  *
  * TEST INFRASTRUCTURE ONLY.  This program is linked against the reference's own
  * engine sources (compiled in place from the read-only reference tree by
@@ -15,6 +15,9 @@
  *   fwigrad: pywave/src/pfwi.c:497-1028 setup -> gradient_init/gradient_av
  *            (wave_fwigradient.c:20,93), called 1 or 2 times
  *   fwi   : pywave/src/pfwi.c:497-1028 -> fwi (wave_fwi.c:24)
+ *   pfwigrad: pywave/src/pfwi.c:1507-2034 setup (passive, only velocity, known source field)
+ *            -> gradient_pas_init/gradient_pas_av (wave_fwigradient.c:528,672), 1 or 2 calls
+ *   pfwi  : pywave/src/pfwi.c:1507-2034 -> pfwi (wave_fwi.c:212)
  *   ricker: the wavelet construction inside fdm2d (wave_fdm.c:502-516)
  *
  * All arrays are raw little-endian float32 files.  Uninitialised heap memory is
@@ -372,12 +375,108 @@ static int run_fwi(int argc, char **argv)
     return 0;
 }
 
+/* the parameter blocks pfwic fills for the passive path (pywave/src/pfwi.c:1819-1880) */
+static np_pas passive_params(np_acqui acpar, int niter)
+{
+    np_pas paspar = passive_init(acpar);
+    paspar->inv = true;
+    paspar->onlysrc = false;
+    paspar->onlyvel = true;
+    paspar->niter = niter;
+    return paspar;
+}
+
+static float ****read_src_field(const char *fname, const fwiargs *a)
+{
+    size_t n = (size_t)a->nz * a->nx * a->nt * a->ns;
+    float ****src = np_floatalloc4(a->nz, a->nx, a->nt, a->ns);
+    float *in = read_f32(fname, n);
+    memcpy(src[0][0][0], in, sizeof(float) * n);
+    return src;
+}
+
+/* pfwigrad pararray vel q wav data src grad_out fcost_out [vel_second] */
+static int run_pfwigrad(int argc, char **argv)
+{
+    if (argc < 10) { fprintf(stderr, "pfwigrad: bad args\n"); return 2; }
+    float *p = read_f32(argv[2], 20);
+    fwiargs a; parse_pararray(p, &a);
+    size_t nzx = (size_t)a.nz * a.nx;
+    float *vel = read_f32(argv[3], nzx);
+    float *q = read_f32(argv[4], nzx);
+    float *wav = read_f32(argv[5], a.nt);
+    float ***data = np_floatalloc3(a.nt, a.nx, a.ns);
+    float *din = read_f32(argv[6], (size_t)a.nt * a.nx * a.ns);
+    memcpy(data[0][0], din, sizeof(float) * (size_t)a.nt * a.nx * a.ns);
+    float ****src = read_src_field(argv[7], &a);
+    float *vel_b = argc > 10 ? read_f32(argv[10], nzx) : NULL;
+
+    np_sou soupar = (np_sou)np_alloc(1, sizeof(*soupar));
+    np_acqui acpar = (np_acqui)np_alloc(1, sizeof(*acpar));
+    np_vec array = (np_vec)np_alloc(1, sizeof(*array));
+    np_fwi fwipar = (np_fwi)np_alloc(1, sizeof(*fwipar));
+    fill_acpar(&a, soupar, acpar);
+    np_pas paspar = passive_params(acpar, a.niter);
+    preparation(vel, q, wav, acpar, soupar, array);
+    fill_fwipar(acpar, fwipar);
+    gradient_pas_init(data, src, NULL, soupar, acpar, array, fwipar, paspar, true);
+
+    int ncall = vel_b ? 2 : 1;
+    float *grads = (float *)malloc(sizeof(float) * nzx * ncall);
+    float fc[2] = {0, 0};
+    double t_start = now_s();
+    gradient_pas_av(vel, &fc[0], grads);
+    fprintf(stderr, "PWREF_ELAPSED_S=%.6f\n", now_s() - t_start);
+    if (vel_b) gradient_pas_av(vel_b, &fc[1], grads + nzx);
+    write_f32(argv[8], grads, nzx * ncall);
+    write_f32(argv[9], fc, ncall);
+    return 0;
+}
+
+/* pfwi pararray vel q wav data src vinv_out : the pfwic() path (inv, only velocity) */
+static int run_pfwi(int argc, char **argv)
+{
+    if (argc < 9) { fprintf(stderr, "pfwi: bad args\n"); return 2; }
+    float *p = read_f32(argv[2], 20);
+    fwiargs a; parse_pararray(p, &a);
+    size_t nzx = (size_t)a.nz * a.nx;
+    float *vel = read_f32(argv[3], nzx);
+    float *q = read_f32(argv[4], nzx);
+    float *wav = read_f32(argv[5], a.nt);
+    float ***data = np_floatalloc3(a.nt, a.nx, a.ns);
+    float *din = read_f32(argv[6], (size_t)a.nt * a.nx * a.ns);
+    memcpy(data[0][0], din, sizeof(float) * (size_t)a.nt * a.nx * a.ns);
+    float ****src = read_src_field(argv[7], &a);
+    float **vinv = np_floatalloc2(nzx, a.niter);
+    float *vgrad = np_floatalloc(nzx);
+
+    np_sou soupar = (np_sou)np_alloc(1, sizeof(*soupar));
+    np_acqui acpar = (np_acqui)np_alloc(1, sizeof(*acpar));
+    np_vec array = (np_vec)np_alloc(1, sizeof(*array));
+    fill_acpar(&a, soupar, acpar);
+    np_pas paspar = passive_params(acpar, a.niter);
+    preparation(vel, q, wav, acpar, soupar, array);
+
+    np_optim optpar = (np_optim)np_alloc(1, sizeof(*optpar));
+    optpar->niter = a.niter; optpar->repeat = 5; optpar->conv_error = 0.01;
+    optpar->npair = 20; optpar->nls = 20; optpar->c1 = 1e-4; optpar->c2 = 0.9;
+    optpar->factor = 10; optpar->v1 = 0.; optpar->v2 = 10.;
+    np_fwi fwipar = (np_fwi)np_alloc(1, sizeof(*fwipar));
+    fill_fwipar(acpar, fwipar);
+
+    double t_start = now_s();
+    pfwi(data, vinv, vgrad, NULL, src, soupar, acpar, array, fwipar, optpar, paspar, true);
+    fprintf(stderr, "PWREF_ELAPSED_S=%.6f\n", now_s() - t_start);
+    write_f32(argv[8], vinv[0], nzx * a.niter);
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
     /* every allocation through mmap: fresh, zero-filled pages */
     mallopt(M_MMAP_THRESHOLD, 0);
     mallopt(M_MMAP_MAX, 1 << 30);
-    if (argc < 2) { fprintf(stderr, "usage: pwref {afd2d|afd3d|ricker|fwimod|fwigrad|fwi} ...\n"); return 2; }
+    if (argc < 2) { fprintf(stderr, "usage: pwref {afd2d|afd3d|ricker|fwimod|fwigrad|fwi|pfwigrad|pfwi} ...\n"); return 2; }
     /* the engines print progress on stdout; keep it off the caller's stdout */
     if (!getenv("PWREF_KEEP_STDOUT")) { if (!freopen("/dev/null", "w", stdout)) return 4; }
     if (!strcmp(argv[1], "afd2d")) return run_afd2d(argc, argv);
@@ -386,6 +485,8 @@ int main(int argc, char **argv)
     if (!strcmp(argv[1], "fwimod")) return run_fwimod(argc, argv);
     if (!strcmp(argv[1], "fwigrad")) return run_fwigrad(argc, argv);
     if (!strcmp(argv[1], "fwi")) return run_fwi(argc, argv);
+    if (!strcmp(argv[1], "pfwigrad")) return run_pfwigrad(argc, argv);
+    if (!strcmp(argv[1], "pfwi")) return run_pfwi(argc, argv);
     fprintf(stderr, "unknown command %s\n", argv[1]);
     return 2;
 }

@@ -121,3 +121,37 @@ def fwi(pararray, vel, q, wav, data):
         el = _run(args, tmp)
         v = np.fromfile(args[-1], np.float32)
     return v.reshape(nz, nx, niter, order="F"), el
+
+
+def pfwigrad(pararray, vel, q, wav, data, src, vel_second=None):
+    """Passive path (known source field, velocity only): gradient_pas_init + gradient_pas_av
+    (x1 or x2).  src is (nz, nx, nt, ns).  Returns (grads [(nz,nx)...], fcosts, elapsed)."""
+    p = np.asarray(pararray, np.float32)
+    nz, nx = int(p[0]), int(p[1])
+    with tempfile.TemporaryDirectory() as tmp:
+        args = ["pfwigrad", _put(tmp, "par.f32", p), _put(tmp, "vel.f32", np.asarray(vel).ravel(order="F")),
+                _put(tmp, "q.f32", np.asarray(q).ravel(order="F")), _put(tmp, "wav.f32", wav),
+                _put(tmp, "data.f32", np.asarray(data).ravel(order="F")),
+                _put(tmp, "src.f32", np.asarray(src).ravel(order="F")),
+                os.path.join(tmp, "grad.f32"), os.path.join(tmp, "fc.f32")]
+        if vel_second is not None:
+            args.append(_put(tmp, "vel2.f32", np.asarray(vel_second).ravel(order="F")))
+        el = _run(args, tmp)
+        g = np.fromfile(args[7], np.float32)
+        fc = np.fromfile(args[8], np.float32)
+    k = g.size // (nz * nx)
+    return [g[i * nz * nx:(i + 1) * nz * nx].reshape(nz, nx, order="F") for i in range(k)], fc, el
+
+
+def pfwi(pararray, vel, q, wav, data, src):
+    """pfwic (passive FWI, velocity only): returns (vinv (nz,nx,niter), elapsed)."""
+    p = np.asarray(pararray, np.float32)
+    nz, nx, niter = int(p[0]), int(p[1]), int(p[19])
+    with tempfile.TemporaryDirectory() as tmp:
+        args = ["pfwi", _put(tmp, "par.f32", p), _put(tmp, "vel.f32", np.asarray(vel).ravel(order="F")),
+                _put(tmp, "q.f32", np.asarray(q).ravel(order="F")), _put(tmp, "wav.f32", wav),
+                _put(tmp, "data.f32", np.asarray(data).ravel(order="F")),
+                _put(tmp, "src.f32", np.asarray(src).ravel(order="F")), os.path.join(tmp, "vinv.f32")]
+        el = _run(args, tmp)
+        v = np.fromfile(args[-1], np.float32)
+    return v.reshape(nz, nx, niter, order="F"), el

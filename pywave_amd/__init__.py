@@ -15,5 +15,6 @@ from .pfwi import pfwi, fillpar  # noqa: F401
 from .fwi import FwiProblem, FwiForward, fwi_traces, make_pararray  # noqa: F401
 from .tri import tri3d_image, group_masks  # noqa: F401
 from .batch import Afd2dBatch, afd2d_batch  # noqa: F401
+from .io import binread, binwrite  # noqa: F401
 
 __version__ = "0.1.0"

@@ -16,7 +16,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 
-from oracle import refrun  # noqa: E402
+from oracle import oracle, refrun  # noqa: E402
 
 
 def save(name, **arrays):
@@ -133,6 +133,43 @@ def fwi_cases():
     save("fwi_marine", par=par2, vel=v2, wav=wav2, data=data2)
 
 
+def passive_field(nz, nx, nt, events, dt=0.001):
+    """Space-time source field (nz, nx, nt, ns): each event is a list of buried point sources
+    (iz, ix, f0, t0, amp), each radiating a Ricker wavelet."""
+    src = np.zeros((nz, nx, nt, len(events)), np.float32)
+    for e, pts in enumerate(events):
+        for iz, ix, f0, t0, amp in pts:
+            src[iz, ix, :, e] += (1e6 * refrun.ricker(nt, dt, f0, t0, amp)).astype(np.float32)
+    return src
+
+
+def passive_cases():
+    """pfwi mode 4 (passive, velocity only, known source field): pywave/src/pfwi.c:1507-2034."""
+    rng = np.random.default_rng(11)
+    nz, nx, nt = 30, 40, 400
+    vtrue = np.full((nz, nx), 2.0, np.float32)
+    vtrue[12:] = 2.4
+    vtrue[22:] = 2.8
+    vtrue = (vtrue + 0.05 * rng.standard_normal((nz, nx))).astype(np.float32)
+    v0 = np.repeat(np.linspace(2.0, 2.8, nz, dtype=np.float32)[:, None], nx, 1).astype(np.float32)
+    q = np.full((nz, nx), 1e4, np.float32)
+    wav = (1e6 * refrun.ricker(nt, 0.001, 10, 0.1, 1)).astype(np.float32)
+    events = [[(24, 9, 15, 0.05, 1.0), (20, 27, 12, 0.08, -0.7)],
+              [(26, 33, 14, 0.04, 0.8), (17, 15, 11, 0.1, 1.2), (25, 21, 16, 0.06, -0.5)]]
+    src = passive_field(nz, nx, nt, events)
+    par = np.array([nz, nx, 0.01, 0.01, 0, 0, nt, 0.001, 0, 1, 2, 0.2, 0.05, 5, 16, 0.01, 10, 1, 1, 7],
+                   np.float32)
+    # observed data: the oracle's passive forward at the true model (pinned below by a zero misfit
+    # of the reference's own gradient_pas_av at that model)
+    data = oracle.fwi_forward(par, vtrue, wav, src=src)
+    _, fc_true, _ = refrun.pfwigrad(par, vtrue, q, wav, data, src)
+    assert fc_true[0] == 0.0, fc_true
+    grads, fc, _ = refrun.pfwigrad(par, v0, q, wav, data, src, vel_second=(v0 * 1.01).astype(np.float32))
+    vinv, _ = refrun.pfwi(par, v0, q, wav, data, src)
+    save("pfwi_small", par=par, vtrue=vtrue, v0=v0, q=q, wav=wav, src=src, data=data,
+         grad1=grads[0], grad2=grads[1], fcost=fc, vinv=vinv)
+
+
 if __name__ == "__main__":
     if not refrun.available():
         sys.exit("oracle/_ref/pwref missing: run `make -C oracle ref` where the reference tree exists")
@@ -140,3 +177,4 @@ if __name__ == "__main__":
     afd2d_cases()
     afd3d_cases()
     fwi_cases()
+    passive_cases()

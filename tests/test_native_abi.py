@@ -5,7 +5,9 @@ import os
 import re
 
 import numpy as np
+import pytest
 
+import pywave_amd
 from conftest import ROOT
 from oracle import oracle as O
 from pywave_amd import _native as N
@@ -74,3 +76,19 @@ def test_output_length_formula():
     assert L.pw_afd_output_len(1000, 101, 101, 1, 999) == 1000 * 101 + 101 * 101 * 2
     assert L.pw_afd_output_len(1500, 201, 201, 201, 0) == 1500 * 201 * 201
     assert L.pw_afd_output_len(10, 5, 6, 7, 3) == 10 * 42 + 5 * 42 * 4
+
+
+def test_binread_binwrite_round_trip(tmp_path):
+    # first axis fastest, as the demos' binread(n1=..., n2=...) reads model files
+    rng = np.random.default_rng(0)
+    a = rng.standard_normal((7, 5)).astype(np.float32)
+    f = tmp_path / "m.bin"
+    pywave_amd.binwrite(str(f), a)
+    raw = np.fromfile(str(f), dtype="<f4")
+    assert np.array_equal(raw[:7], a[:, 0])  # n1 contiguous
+    assert np.array_equal(pywave_amd.binread(str(f), n1=7, n2=5), a)
+    b = rng.standard_normal((3, 4, 2)).astype(np.float32)
+    pywave_amd.binwrite(str(f), b)
+    assert np.array_equal(pywave_amd.binread(str(f), 3, 4, 2), b)
+    with pytest.raises(ValueError):
+        pywave_amd.binread(str(f), 5, 5)

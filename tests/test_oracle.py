@@ -196,6 +196,63 @@ def test_fwi_lbfgs_bit_exact(golden):
     assert np.array_equal(vinv, g["vinv"])
 
 
+def _point_source_field(par, wav):
+    """The active point-source term written as a passive source field (nz, nx, nt, ns):
+    source_map's 3x3 stencil at each shot position times the wavelet."""
+    a = O.make_acq(par)
+    nz, nx, nt, ns, nb = int(par[0]), int(par[1]), int(par[6]), int(par[10]), int(par[14])
+    src = np.zeros((nz, nx, nt, ns), np.float32)
+    for s in range(ns):
+        rr = O.source_map(a.s0_v + s * a.ds_v, a.sz, 2, 2, a.padnx, a.padnz)
+        inner = rr[nb:nb + nz, nb:nb + nx]
+        assert abs(float(np.abs(rr).sum()) - float(np.abs(inner).sum())) == 0.0
+        src[:, :, :, s] = inner[:, :, None] * np.asarray(wav, np.float32)[None, None, :]
+    return src
+
+
+def test_passive_field_equal_to_point_sources_reproduces_active(golden):
+    # gradient_pas_av adds the field on the interior where gradient_av adds the 3x3 point source;
+    # with the same term both must give the active golden bit for bit (indexing of iz/ix/it/is)
+    g = golden("fwi_small")
+    src = _point_source_field(g["par"], g["wav"])
+    assert np.array_equal(O.fwi_forward(g["par"], g["vtrue"], g["wav"], src=src), g["data"])
+    g1, f1, _ = O.gradient_av(g["par"], g["v0"], g["wav"], g["data"], src=src)
+    assert np.array_equal(g1, g["grad1"]) and np.float32(f1) == g["fcost"][0]
+    # a field shifted by one sample in z is not the same term
+    sh = np.roll(src, 1, axis=0)
+    assert not np.array_equal(O.fwi_forward(g["par"], g["vtrue"], g["wav"], src=sh), g["data"])
+
+
+def test_passive_forward_linear_in_field(golden):
+    g = golden("pfwi_small")
+    d1 = O.fwi_forward(g["par"], g["vtrue"], g["wav"], src=g["src"])
+    assert np.array_equal(d1, g["data"])
+    d2 = O.fwi_forward(g["par"], g["vtrue"], g["wav"], src=2 * g["src"])
+    assert np.array_equal(d2, 2 * d1)  # power-of-two scaling is exact in binary floating point
+    assert not O.fwi_forward(g["par"], g["vtrue"], g["wav"], src=0 * g["src"]).any()
+
+
+def test_passive_gradient_bit_exact(golden):
+    g = golden("pfwi_small")
+    g1, f1, sc = O.gradient_av(g["par"], g["v0"], g["wav"], g["data"], src=g["src"])
+    assert np.array_equal(g1, g["grad1"]) and np.float32(f1) == g["fcost"][0]
+    g2, f2, _ = O.gradient_av(g["par"], (g["v0"] * 1.01).astype(np.float32), g["wav"], g["data"],
+                              scaling=sc, src=g["src"])
+    assert np.array_equal(g2, g["grad2"]) and np.float32(f2) == g["fcost"][1]
+    # at the true model the residual vanishes exactly
+    _, f0, _ = O.gradient_av(g["par"], g["vtrue"], g["wav"], g["data"], src=g["src"])
+    assert f0 == 0.0
+
+
+def test_passive_fwi_lbfgs_bit_exact(golden):
+    # pfwi resets the trial step to 1 every 5 iterations (pywave/src/wave_fwi.c:282, repeat=5 from
+    # pywave/src/pfwi.c:1846); 7 iterations run through one reset
+    g = golden("pfwi_small")
+    vinv, k = O.fwi(g["par"], g["v0"], g["wav"], g["data"], src=g["src"])
+    assert k == 7
+    assert np.array_equal(vinv, g["vinv"])
+
+
 def _tri_adj_1d(x, r):
     """Adjoint triangle smoothing, independent form: full convolution with the triangle
     (r-|k|)/r^2 then half-sample reflection of the overhang back into the line."""
