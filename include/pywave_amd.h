@@ -116,6 +116,15 @@ int pw_fwi_set_checkpointing(pw_fwi *p, int steps);
 int pw_fwi_info(pw_fwi *p, int *ints8);
 int pw_fwi_set_wavelet(pw_fwi *p, const float *wav);
 int pw_fwi_set_observed(pw_fwi *p, const float *data);
+/* passive source field (pfwi mode 4: gradient_pas_init / gradient_pas_av,
+ * pywave/src/wave_fwigradient.c:528-870): src (nz,nx,nt,ns) Fortran order, added to the Laplacian
+ * on the model interior at every step in place of the point sources; every shot then records and
+ * injects at r0_v[0] + ir*dr_v (ir < nr).  NULL switches back to the point sources.  Applies to
+ * forward, gradient (both modes), device variants and L-BFGS (which then resets its trial step to
+ * 1 every 5 iterations as pfwi does, pywave/src/wave_fwi.c:282). */
+int pw_fwi_set_source_field(pw_fwi *p, const float *src);
+/* the same from device memory of the plan's precision, [ns][nt][nx][nz] */
+int pw_fwi_set_source_field_dev(pw_fwi *p, const void *d_src);
 /* forward_modeling_a (pywave/src/wave_fwimodeling.c:21-114) */
 int pw_fwi_forward(pw_fwi *p, const float *vel, float *data_out);
 /* one gradient over all shots: mode 0 = gradient_av (pywave/src/wave_fwigradient.c:93-262:
@@ -142,6 +151,12 @@ int pw_forward_modeling_ac(pw_ctx *ctx, const float *vel, const float *q, const 
 /* pfwicfun.fwic (pywave/src/pfwi.c:497-1028): out = nz*nx*niter, (nz,nx,niter) Fortran */
 int pw_fwic(pw_ctx *ctx, const float *vel, const float *q, const float *wav, const float *data,
             const float *pararray, float *out, long long out_len);
+/* pfwicfun.pfwic (pywave/src/pfwi.c:1507-2034; called by pywave/pfwi.py:196 for mode 4 with
+ * inv and not onlysrc): passive FWI for velocity with a known source field.  datasrc = data
+ * (nt*nx*ns) followed by src (nz*nx*nt*ns) when pararray[9] (inv) != 0, else src alone (zero
+ * data); datasrc_len its length in floats.  out = nz*nx*niter, (nz,nx,niter) Fortran. */
+int pw_pfwic(pw_ctx *ctx, const float *vel, const float *q, const float *wav, const float *datasrc,
+             const float *pararray, float *out, long long out_len, long long datasrc_len);
 
 /* ---------------------------------------------------------------- time-reversal imaging of groups
  * Back-propagates ngroups (1..4) receiver groups of one data set together (one launch per step,

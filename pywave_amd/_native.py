@@ -59,6 +59,8 @@ SIGNATURES = {
     "pw_fwi_info": (c_int, [c_vp, c_vp]),
     "pw_fwi_set_wavelet": (c_int, [c_vp, c_vp]),
     "pw_fwi_set_observed": (c_int, [c_vp, c_vp]),
+    "pw_fwi_set_source_field": (c_int, [c_vp, c_vp]),
+    "pw_fwi_set_source_field_dev": (c_int, [c_vp, c_vp]),
     "pw_fwi_forward": (c_int, [c_vp, c_vp, c_vp]),
     "pw_fwi_gradient": (c_int, [c_vp, c_vp, c_int, c_vp, ctypes.POINTER(c_double), ctypes.POINTER(c_float)]),
     "pw_fwi_gradient_f64": (c_int, [c_vp, c_vp, c_int, c_vp, ctypes.POINTER(c_double)]),
@@ -69,6 +71,7 @@ SIGNATURES = {
     "pw_fwi_lbfgs": (c_int, [c_vp, c_vp, c_int, c_vp, ctypes.POINTER(c_int)]),
     "pw_forward_modeling_ac": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_ll]),
     "pw_fwic": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_ll]),
+    "pw_pfwic": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_ll, c_ll]),
     "pw_tri_create": (c_vp, [c_vp, c_int, c_int, c_int, c_float, c_float, c_float, c_int, c_float, c_int, c_float,
                              c_int, c_int]),
     "pw_tri_set_velocity": (c_int, [c_vp, c_vp]),

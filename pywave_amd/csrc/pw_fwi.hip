@@ -77,6 +77,8 @@ struct StepArgs {
     long long snaps;    // snapshot stride per slot
     T* full;            // full-field copy of p1 for this step [slot][fs] or null
     long long fulls;
+    const T* fsrc;      // passive source field of this step on the interior, [slot][nx][nz], or null
+    long long fss;      // its slot stride (nt*nz*nx)
     int it;
 };
 
@@ -171,6 +173,12 @@ __global__ __launch_bounds__(64) void k_fwd(FGeo<T> g, StepArgs<T> a, int ntz)
         return bldo<T>(r, ok ? voff : kOOB, ok ? (unsigned)x * oc : 0u);
     };
     const T wv = a.wav[a.it];
+    // passive source field (gradient_pas_av's ww3): added to the Laplacian on the model interior
+    const bool fzin = a.fsrc && z >= g.nb && z < g.nb + g.nz;
+    const auto rS = win(a.fsrc ? a.fsrc + slot * a.fss : nullptr, a.fsrc ? (unsigned)(g.nz * g.nx * sizeof(T)) : 0u);
+    const unsigned ofz = fzin ? (unsigned)(z - g.nb) * E : kOOB;
+    auto fxin = [&](int x) { return x >= g.nb && x < g.nb + g.nx; };
+    auto ldf = [&](int x) { return bldo<T>(rS, fxin(x) ? ofz : kOOB, fxin(x) ? (unsigned)((x - g.nb) * g.nz) * E : 0u); };
     T q[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
@@ -180,6 +188,7 @@ __global__ __launch_bounds__(64) void k_fwd(FGeo<T> g, StepArgs<T> a, int ntz)
     T yv = ldc(rY, oz, x0, true);
     T mv = ldc(rM, oz, x0, true);
     T hz = ldc(rX, ozh, x0, true);
+    T fv = ldf(x0);
     T xl = a.sfx[2 * x0], xh = a.sfx[2 * x0 + 1];
     int buf = 0;
     for (int x = x0; x < x1; ++x) {
@@ -189,6 +198,7 @@ __global__ __launch_bounds__(64) void k_fwd(FGeo<T> g, StepArgs<T> a, int ntz)
         const T nyv = ldc(rY, oz, xn, nv);
         const T nmv = ldc(rM, oz, xn, nv);
         const T nhz = ldc(rX, ozh, xn, nv);
+        const T nfv = ldf(xn);
         const int xs = xn < g.pnx ? xn : g.pnx - 1;
         const T nxl = a.sfx[2 * xs], nxh = a.sfx[2 * xs + 1];
         row[buf][lane + 4] = q[4];
@@ -224,6 +234,7 @@ __global__ __launch_bounds__(64) void k_fwd(FGeo<T> g, StepArgs<T> a, int ntz)
             const int px = x - sh.sx0, pz = z - sh.sz0;
             if (px >= 0 && px < sh.pw && pz >= 0 && pz < sh.ph)
                 term = term + a.patch[slot * PMAX * PMAX + px * PMAX + pz] * wv;
+            if (fzin && fxin(x)) term = term + fv;
             const T p0 = sponge(yv, zl, zh, xl, xh);
             T p2 = 2 * p1 - p0 + mv * term;
             bsto(rY, oz, (unsigned)x * oc, sponge(p2, zl, zh, xl, xh));
@@ -231,7 +242,7 @@ __global__ __launch_bounds__(64) void k_fwd(FGeo<T> g, StepArgs<T> a, int ntz)
 #pragma unroll
         for (int k = 0; k < 8; ++k) q[k] = q[k + 1];
         q[8] = nq;
-        yv = nyv; mv = nmv; hz = nhz; xl = nxl; xh = nxh;
+        yv = nyv; mv = nmv; hz = nhz; fv = nfv; xl = nxl; xh = nxh;
         buf ^= 1;
     }
 }
@@ -382,6 +393,11 @@ __global__ __launch_bounds__(64) void k_adj_exact(FGeo<T> g, StepArgs<T> a, ExAr
         return (k >= 0 && k < g.nr) ? k : -1;
     };
     const T wv = a.wav[a.it];
+    const bool fzin = a.fsrc && z >= g.nb && z < g.nb + g.nz;
+    const auto rS = win(a.fsrc ? a.fsrc + slot * a.fss : nullptr, a.fsrc ? (unsigned)(g.nz * g.nx * sizeof(T)) : 0u);
+    const unsigned ofz = fzin ? (unsigned)(z - g.nb) * E : kOOB;
+    auto fxin = [&](int x) { return x >= g.nb && x < g.nb + g.nx; };
+    auto ldf = [&](int x) { return bldo<T>(rS, fxin(x) ? ofz : kOOB, fxin(x) ? (unsigned)((x - g.nb) * g.nz) * E : 0u); };
     T qa[9], qf[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
@@ -401,6 +417,7 @@ __global__ __launch_bounds__(64) void k_adj_exact(FGeo<T> g, StepArgs<T> a, ExAr
     T xl = a.sfx[2 * x0], xh = a.sfx[2 * x0 + 1];
     int k0 = rk(x0);
     T iv = bld<T>(rI, (unsigned)(k0 < 0 ? 0 : k0) * E, k0 >= 0);
+    T fv = ldf(x0);
     int buf = 0;
     for (int x = x0; x < x1; ++x) {
         const int xn = x + 1, x5 = x + 5;
@@ -419,6 +436,7 @@ __global__ __launch_bounds__(64) void k_adj_exact(FGeo<T> g, StepArgs<T> a, ExAr
         const T nxl = a.sfx[2 * xs], nxh = a.sfx[2 * xs + 1];
         const int kn = nv ? rk(xn) : -1;
         const T niv = bld<T>(rI, (unsigned)(kn < 0 ? 0 : kn) * E, kn >= 0);
+        const T nfv = ldf(xn);
         rowa[buf][lane + 4] = qa[4];
         rowf[buf][lane + 4] = qf[4];
         if (lane < 8) {
@@ -456,13 +474,14 @@ __global__ __launch_bounds__(64) void k_adj_exact(FGeo<T> g, StepArgs<T> a, ExAr
             const int px = x - sh.sx0, pz = z - sh.sz0;
             if (px >= 0 && px < sh.pw && pz >= 0 && pz < sh.ph)
                 lf = lf + a.patch[slot * PMAX * PMAX + px * PMAX + pz] * wv;
+            if (fzin && fxin(x)) lf = lf + fv;
             bsto(rG, oz, (unsigned)x * oc, gv + lf * av);
         }
 #pragma unroll
         for (int k = 0; k < 8; ++k) { qa[k] = qa[k + 1]; qf[k] = qf[k + 1]; }
         qa[8] = nm5 * na5;
         qf[8] = nf5;
-        av = nav; bv = nbv; gv = ngv; ha = nha; hf = nhf; xl = nxl; xh = nxh; iv = niv;
+        av = nav; bv = nbv; gv = ngv; ha = nha; hf = nhf; xl = nxl; xh = nxh; iv = niv; fv = nfv;
         buf ^= 1;
     }
 }
@@ -855,6 +874,11 @@ struct pw_fwi {
     void *rec = nullptr, *res = nullptr, *obs = nullptr, *W = nullptr;
     void *ckpt = nullptr, *ckpt_snap = nullptr, *seg = nullptr, *gm = nullptr, *gslot = nullptr, *gtot = nullptr;
     void *patch = nullptr;
+    void* fsrc = nullptr;   // passive source field [ns][nt][nx][nz] (plan precision) or null
+    bool passive = false;   // shots are the source field (gradient_pas_av), not point sources
+    int cur_b0 = 0;         // first shot of the batch set by set_batch
+    std::string active_err; // why the point-source geometry is unusable (empty: usable)
+    std::vector<ShotInfo> shot_active;
     float* tmpv = nullptr;
     float* work = nullptr;  // smoothing workspace
     ShotInfo* shots = nullptr;
@@ -919,7 +943,7 @@ void fwi_free(pw_fwi* p)
 {
     void* bufs[] = {p->vel, p->vpad, p->m, p->sfz, p->sfx, p->wav, p->X, p->Y, p->A, p->B, p->rec, p->res,
                     p->obs, p->W, p->ckpt, p->ckpt_snap, p->seg, p->gm, p->gslot, p->gtot, p->patch,
-                    p->tmpv, p->work, p->shots, p->slot_shot, p->partial, p->dscalar, p->gfold};
+                    p->tmpv, p->work, p->shots, p->slot_shot, p->partial, p->dscalar, p->gfold, p->fsrc};
     for (void* b : bufs)
         if (b) hipFree(b);
 }
@@ -1009,6 +1033,8 @@ int upload_vel_host(pw_fwi* p, const float* vel)
 // per-batch shot tables
 int set_batch(pw_fwi* p, int b0, int nb_)
 {
+    if (!p->passive && !p->active_err.empty()) PW_FAIL(p->ctx, PW_ERR_ARG, p->active_err);
+    p->cur_b0 = b0;
     hipStream_t s = p->ctx->s();
     std::vector<ShotInfo> sh(p->nslot);
     std::vector<int> sos(p->nslot, 0);
@@ -1045,7 +1071,16 @@ StepArgs<T> base_args(pw_fwi* p)
     a.trs = (long long)p->acq.nt * p->acq.nr;
     a.snaps = p->nzx;
     a.fulls = p->fs;
+    a.fss = (long long)p->acq.nt * p->nzx;
     return a;
+}
+
+// the batch's passive source field at step it (slot 0), or null for point sources
+template <typename T>
+const T* step_src(const pw_fwi* p, int it)
+{
+    if (!p->passive) return nullptr;
+    return (const T*)p->fsrc + ((size_t)p->cur_b0 * p->acq.nt + it) * (size_t)p->nzx;
 }
 
 template <typename T>
@@ -1077,6 +1112,7 @@ int forward_batch(pw_fwi* p, int nb_, bool ckpt, int mode, bool store_last)
     for (int it = 0; it < a.nt; ++it) {
         StepArgs<T> s_ = base_args<T>(p);
         s_.X = X; s_.Y = Y; s_.it = it;
+        s_.fsrc = step_src<T>(p, it);
         s_.rec = (T*)p->rec;
         if (ckpt) {
             const int sgi = it / K;
@@ -1120,6 +1156,7 @@ int recompute_segment(pw_fwi* p, int nb_, int sg, int mode)
     for (int it = a0; it < t_end; ++it) {
         StepArgs<T> s_ = base_args<T>(p);
         s_.X = X; s_.Y = Y; s_.it = it;
+        s_.fsrc = step_src<T>(p, it);
         if (mode == 1) s_.full = (T*)p->seg + (size_t)(it - a0) * fld;
         else if (it % I == 0) s_.snap = (T*)p->seg + (size_t)((it - a0) / I + 1) * p->nslot * p->nzx;
         hipLaunchKernelGGL(k_fwd<T>, dim3(ntz * nxc, nb_), dim3(64), 0, s, g, s_, ntz);
@@ -1150,6 +1187,7 @@ int adjoint_segment(pw_fwi* p, int nb_, int sg, int mode, T** Ap, T** Bp, T* gra
     for (int it = b0 - 1; it >= a0; --it) {
         StepArgs<T> s_ = base_args<T>(p);
         s_.X = A; s_.Y = B; s_.it = it;
+        s_.fsrc = step_src<T>(p, it);
         s_.inj = (const T*)p->res;
         if (mode == 0) {
             ImgArgs<T> im{};
@@ -1361,22 +1399,23 @@ pw_fwi* pw_fwi_create(pw_ctx* ctx, const float* pararray, int batch, int precisi
     // geometry per shot
     p->shot_host.resize(a.ns);
     p->patch_host.assign((size_t)a.ns * PMAX * PMAX, 0.f);
+    // point-source geometry; an unusable one is reported by the first call that would use it
+    // (a passive source field set with pw_fwi_set_source_field replaces it)
     for (int is = 0; is < a.ns; ++is) {
         ShotInfo& sh = p->shot_host[is];
         const int sx = a.s0_v + is * a.ds_v;
         if (source_patch(a, sx, 2, sh, &p->patch_host[(size_t)is * PMAX * PMAX], err)) {
-            ctx->err = "pw_fwi_create: " + err;
-            delete p;
-            return nullptr;
+            if (p->active_err.empty()) p->active_err = "pw_fwi: " + err;
+            sh.pw = sh.ph = 0;
         }
         sh.r0_v = a.r0_v[is]; sh.r02 = a.r02[is]; sh.nr2 = a.nr2[is]; sh.shot = is;
         int rlast = sh.r0_v + (sh.nr2 - 1) * a.dr_v;
         if (sh.nr2 > 0 && (sh.r0_v < 4 || rlast >= a.padnx - 4)) {
-            ctx->err = "pw_fwi_create: receivers reach the padded grid's 4-cell rim";
-            delete p;
-            return nullptr;
+            if (p->active_err.empty()) p->active_err = "pw_fwi: receivers reach the padded grid's 4-cell rim";
+            sh.nr2 = 0;
         }
     }
+    p->shot_active = p->shot_host;
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = (size_t)8 << 30;
     p->budget = free_b / 2;
@@ -1519,6 +1558,83 @@ int pw_fwi_set_observed(pw_fwi* p, const float* data)
     }
     p->have_obs = true;
     return PW_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// switch the plan to the passive source field already in p->fsrc (or back to point sources)
+int use_source_field(pw_fwi* p, bool on)
+{
+    p->dev_ckpt_valid = false;
+    const Acq& a = p->acq;
+    if (!on) {
+        p->passive = false;
+        p->shot_host = p->shot_active;
+        return PW_OK;
+    }
+    // gradient_pas_av (pywave/src/wave_fwigradient.c:710-713, :807-810): every shot records and
+    // injects at r0_v[0] + ir*dr_v, ir < nr; no point source
+    const int rlast = a.r0_v[0] + (a.nr - 1) * a.dr_v;
+    if (a.r0_v[0] < 4 || rlast >= a.padnx - 4)
+        PW_FAIL(p->ctx, PW_ERR_ARG, "pw_fwi_set_source_field: receivers reach the padded grid's 4-cell rim");
+    for (int is = 0; is < a.ns; ++is) {
+        ShotInfo& sh = p->shot_host[is];
+        sh.sx0 = sh.sz0 = 0; sh.pw = sh.ph = 0;
+        sh.r0_v = a.r0_v[0]; sh.r02 = 0; sh.nr2 = a.nr; sh.shot = is;
+    }
+    p->passive = true;
+    return PW_OK;
+}
+
+int alloc_source_field(pw_fwi* p)
+{
+    if ((double)p->nzx * esz(p) >= 2147483648.0)
+        PW_FAIL(p->ctx, PW_ERR_ARG, "pw_fwi_set_source_field: one source-field plane exceeds 2 GiB");
+    if (!p->fsrc) return dmalloc(p, &p->fsrc, (size_t)p->acq.ns * p->acq.nt * p->nzx * esz(p));
+    return PW_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+// passive source field (pfwi mode 4, gradient_pas_init's ww3): src (nz, nx, nt, ns) Fortran order,
+// i.e. [is][it][ix][iz]; NULL returns the plan to its point sources
+int pw_fwi_set_source_field(pw_fwi* p, const float* src)
+{
+    if (!p) return PW_ERR_ARG;
+    if (!src) return use_source_field(p, false);
+    int rc = alloc_source_field(p);
+    if (rc) return rc;
+    const Acq& a = p->acq;
+    hipStream_t s = p->ctx->s();
+    const size_t per = (size_t)a.nt * p->nzx;
+    if (p->prec == 4) {
+        PW_HIPS(hipMemcpyAsync(p->fsrc, src, per * a.ns * 4, hipMemcpyHostToDevice, s));
+    } else {
+        std::vector<double> d(per);
+        for (int is = 0; is < a.ns; ++is) {
+            for (size_t i = 0; i < per; ++i) d[i] = src[is * per + i];
+            PW_HIPS(hipMemcpyAsync((double*)p->fsrc + is * per, d.data(), per * 8, hipMemcpyHostToDevice, s));
+            PW_HIPS(hipStreamSynchronize(s));
+        }
+    }
+    PW_HIPS(hipStreamSynchronize(s));
+    return use_source_field(p, true);
+}
+
+// the same from device memory of the plan's precision (stream-ordered, no host synchronisation)
+int pw_fwi_set_source_field_dev(pw_fwi* p, const void* d_src)
+{
+    if (!p) return PW_ERR_ARG;
+    if (!d_src) return use_source_field(p, false);
+    int rc = alloc_source_field(p);
+    if (rc) return rc;
+    PW_HIPS(hipMemcpyAsync(p->fsrc, d_src, (size_t)p->acq.ns * p->acq.nt * p->nzx * esz(p),
+                           hipMemcpyDeviceToDevice, p->ctx->s()));
+    return use_source_field(p, true);
 }
 
 // forward_modeling_a: data_out (nt, nr, ns) Fortran order
@@ -1785,6 +1901,7 @@ int pw_fwi_lbfgs(pw_fwi* p, const float* vel0, int niter, float* vinv, int* nsto
         int ipair = 0;
         double alpha = 1.0;
         for (int iter = 0; iter < niter; ++iter) {
+            if (p->passive && iter % 5 == 0) alpha = 1.0;  // pfwi: optpar->repeat = 5 (wave_fwi.c:282)
             if (iter == 0) {
                 axpby(-1.0, g, 0.0, g, dir);
             } else {
@@ -1917,6 +2034,42 @@ int pw_fwic(pw_ctx* ctx, const float* vel, const float* q, const float* wav, con
     if (!p) return PW_ERR_ARG;
     int rc = pw_fwi_set_wavelet(p, wav);
     if (!rc) rc = pw_fwi_set_observed(p, data);
+    int k = 0;
+    if (!rc) rc = pw_fwi_lbfgs(p, vel, a.niter, out, &k);
+    pw_fwi_destroy(p);
+    return rc;
+}
+
+// pfwicfun.pfwic (pywave/src/pfwi.c:1507-2034): passive FWI for velocity with a known source
+// field (paspar onlyvel).  datasrc = [data (nt*nx*ns) | src (nz*nx*nt*ns)] when pararray's inv
+// flag is set, else src alone with zero data (pfwi.c:1753-1766).  out: vinv (niter * nz*nx); slots
+// after an early stop are zero.
+int pw_pfwic(pw_ctx* ctx, const float* vel, const float* q, const float* wav, const float* datasrc,
+             const float* pararray, float* out, long long out_len, long long datasrc_len)
+{
+    (void)q;
+    if (!ctx || !vel || !wav || !datasrc || !pararray || !out) return PW_ERR_ARG;
+    Acq a;
+    std::string err;
+    if (acq_from_pararray(pararray, a, err)) PW_FAIL(ctx, PW_ERR_ARG, err);
+    if (out_len != (long long)a.nz * a.nx * a.niter) PW_FAIL(ctx, PW_ERR_ARG, "pfwic: output length");
+    const bool inv = pararray[9] != 0.f;
+    const long long nd = (long long)a.nt * a.nx * a.ns, nsrc = (long long)a.nz * a.nx * a.nt * a.ns;
+    if (datasrc_len != (inv ? nd : 0) + nsrc) PW_FAIL(ctx, PW_ERR_ARG, "pfwic: datasrc length");
+    std::memset(out, 0, sizeof(float) * out_len);
+    pw_fwi* p = pw_fwi_create(ctx, pararray, 0, 4);
+    if (!p) return PW_ERR_ARG;
+    int rc = pw_fwi_set_wavelet(p, wav);
+    std::vector<float> zero;
+    if (!rc) {
+        if (inv) {
+            rc = pw_fwi_set_observed(p, datasrc);
+        } else {
+            zero.assign((size_t)nd, 0.f);
+            rc = pw_fwi_set_observed(p, zero.data());
+        }
+    }
+    if (!rc) rc = pw_fwi_set_source_field(p, datasrc + (inv ? nd : 0));
     int k = 0;
     if (!rc) rc = pw_fwi_lbfgs(p, vel, a.niter, out, &k);
     pw_fwi_destroy(p);

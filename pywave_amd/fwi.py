@@ -7,6 +7,9 @@ FwiProblem wraps one pw_fwi plan (pywave_amd/csrc/pw_fwi.hip):
                                   triangle smoothing -- the reference's gradient
   gradient(vel, mode="exact")     exact discrete adjoint of 0.5*||F(v) - d||^2
   lbfgs(vel0, niter)              fwi() (pywave/src/wave_fwi.c:24-119) with f64 dot products
+  set_source_field(src)           passive shots (pfwi mode 4): gradient_pas_av
+                                  (pywave/src/wave_fwigradient.c:672-870) and pfwi()
+                                  (pywave/src/wave_fwi.c:212-322) for a known source field
 Shots advance `batch` at a time in one launch per time step; forward wavefields are checkpointed
 every `checkpoint` steps (0 = automatic: no recomputation when everything fits in half the free
 HBM, else 50) and recomputed segment by segment during the backward sweep.
@@ -37,7 +40,7 @@ def make_pararray(par):
 
 
 class FwiProblem:
-    def __init__(self, par, wav, data=None, batch=0, precision="f32", checkpoint=0, device=0):
+    def __init__(self, par, wav, data=None, batch=0, precision="f32", checkpoint=0, device=0, src=None):
         self.par = make_pararray(par)
         self.nz, self.nx = int(self.par[0]), int(self.par[1])
         self.nt, self.ns = int(self.par[6]), int(self.par[10])
@@ -65,6 +68,8 @@ class FwiProblem:
         N.check(self._L.pw_fwi_set_wavelet(self._p, N.ptr(w)), self._ctx, "set_wavelet")
         if data is not None:
             self.set_observed(data)
+        if src is not None:
+            self.set_source_field(src)
 
     # ------------------------------------------------------------------ set-up
     def set_observed(self, data):
@@ -72,6 +77,28 @@ class FwiProblem:
         if d.size != self.nt * self.nr * self.ns:
             raise ValueError("observed data must hold nt*nx*ns samples")
         N.check(self._L.pw_fwi_set_observed(self._p, N.ptr(d)), self._ctx, "set_observed")
+
+    def set_source_field(self, src):
+        """Passive source field (nz, nx, nt, ns) added on the model interior at every step in place
+        of the point sources (gradient_pas_init's ww3); receivers of every shot at r0_v[0] + ir*dr_v.
+        A torch device tensor is taken as [ns][nt][nx][nz] of the plan's precision, copied on the
+        current torch stream.  None returns to the point sources."""
+        if src is None:
+            N.check(self._L.pw_fwi_set_source_field(self._p, None), self._ctx, "set_source_field")
+            return
+        n = self.nz * self.nx * self.nt * self.ns
+        if hasattr(src, "data_ptr") and getattr(src, "is_cuda", False):
+            if src.numel() != n or not src.is_contiguous() or src.element_size() != self.prec:
+                raise ValueError("device source field must be contiguous [ns][nt][nx][nz] of the plan precision")
+            torch = _torch()
+            self._bind_stream(torch.cuda.current_stream(src.device).cuda_stream)
+            N.check(self._L.pw_fwi_set_source_field_dev(self._p, ctypes.c_void_p(src.data_ptr())), self._ctx,
+                    "set_source_field")
+            return
+        f = np.ascontiguousarray(np.asarray(src, np.float32).ravel(order="F"))
+        if f.size != n:
+            raise ValueError("source field must hold nz*nx*nt*ns samples")
+        N.check(self._L.pw_fwi_set_source_field(self._p, N.ptr(f)), self._ctx, "set_source_field")
 
     def set_checkpointing(self, steps):
         N.check(self._L.pw_fwi_set_checkpointing(self._p, int(steps)), self._ctx, "set_checkpointing")

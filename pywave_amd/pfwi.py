@@ -5,7 +5,13 @@ mode 1  forward modelling (pfwicfun.forward_modeling_ac, pywave/src/pfwi.c:27-49
 mode 2  active-source FWI (pfwicfun.fwic, pywave/src/pfwi.c:497-1028): L-BFGS over gradient_av;
         returns (data flattened in Fortran order, vinv (nz, nx, niter), None, None, None)
 mode 3  the reference's RTM branch is empty and its return then fails (UnboundLocalError)
-mode 4  passive-source FWI / least-squares TRI: not part of this engine (NotImplementedError)
+mode 4  inv and not onlysrc: passive FWI for velocity with the known source field src
+        (pfwicfun.pfwic, pywave/src/pfwi.c:1507-2034, through pywave/pfwi.py:168-204): with
+        onlyvel returns (data flattened in Fortran order, vinv (nz, nx, niter), None, None, None);
+        without onlyvel the reference's return reaches unassigned grad/mwt (pywave/pfwi.py:205-209)
+        and raises UnboundLocalError, raised here before the inversion runs
+        other mode-4 branches (passive modelling and source inversion by least-squares TRI,
+        pfwicfun.lstric): NotImplementedError
 other   prints "Wrong mode" and returns 0
 media != 1 in mode 1 reaches an undefined function in the reference (NameError); the same error
 is raised here.  Units follow the reference demos: km and km/s (the optimiser clips to [0, 10]).
@@ -53,8 +59,26 @@ def pfwi(vel, q, wav, src, data=None, mode=1, media=1, inv=0, verb=1, nb=60, coe
     elif mode == 3:
         raise UnboundLocalError("local variable 'vinv' referenced before assignment")
     elif mode == 4:
-        raise NotImplementedError("pfwi mode 4 (passive-source FWI / least-squares TRI) is not provided by "
-                                  "this engine")
+        if not (par["inv"] and not par["onlysrc"]):
+            raise NotImplementedError("pfwi mode 4 passive modelling / source inversion (lstric) is not provided "
+                                      "by this engine")
+        if not par["onlyvel"]:
+            raise UnboundLocalError("local variable 'grad' referenced before assignment")
+        vel = vel.flatten(order="F").astype(np.float32)
+        q = q.flatten(order="F").astype(np.float32)
+        wav = wav.flatten(order="F").astype(np.float32)
+        par["inv"] = 1
+        pararray = make_pararray(par)
+        data = data.flatten(order="F").astype(np.float32)
+        src = src.flatten(order="F").astype(np.float32)
+        datasrc = np.concatenate([data, src], axis=0)
+        nz, nx, niter = int(par["nz"]), int(par["nx"]), int(par["niter"])
+        out = np.empty(nz * nx * niter, np.float32)
+        L, ctx = N.lib(), N.context()
+        N.check(L.pw_pfwic(ctx, N.ptr(vel), N.ptr(q), N.ptr(wav), N.ptr(datasrc), N.ptr(pararray), N.ptr(out),
+                           out.size, datasrc.size), ctx, "pfwic")
+        vinv = out.reshape(par["nz"], par["nx"], par["niter"], order="F")
+        src = None; mwt = None; grad = None
     else:
         print("Wrong mode")
         return 0

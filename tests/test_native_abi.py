@@ -92,3 +92,19 @@ def test_binread_binwrite_round_trip(tmp_path):
     assert np.array_equal(pywave_amd.binread(str(f), 3, 4, 2), b)
     with pytest.raises(ValueError):
         pywave_amd.binread(str(f), 5, 5)
+
+
+def test_pfwi_mode4_branches_without_device():
+    # branches decided on the host before any device work (pywave/pfwi.py:120-256)
+    par = {"nz": 8, "nx": 8, "dz": 0.01, "dx": 0.01, "oz": 0, "ox": 0, "nt": 10, "dt": 0.001, "ot": 0,
+           "ns": 1, "nb": 8, "niter": 1}
+    v = np.full((8, 8), 2.0, np.float32)
+    w = np.zeros(10, np.float32)
+    src = np.zeros((8, 8, 10, 1), np.float32)
+    with pytest.raises(NotImplementedError):  # passive modelling (lstric, inv=0)
+        pywave_amd.pfwi(v, v, w, src, mode=4, par=dict(par, inv=False))
+    with pytest.raises(NotImplementedError):  # source-only inversion (lstric, onlysrc)
+        pywave_amd.pfwi(v, v, w, src, data=np.zeros((10, 8, 1)), mode=4, par=dict(par, inv=True, onlysrc=True))
+    with pytest.raises(UnboundLocalError):  # joint inversion: the reference's return fails
+        pywave_amd.pfwi(v, v, w, src, data=np.zeros((10, 8, 1)), mode=4,
+                        par=dict(par, inv=True, onlysrc=False, onlyvel=False))
