@@ -15,6 +15,10 @@ What pins each function (tests/test_oracle.py):
                                  equal to the point-source term reproduces the active golden;
                                  zero misfit of the reference at the true model; linearity
   source_map, gradient_smooth2   numpy convolution with the reflected triangle (independent form)
+  timerev, lstri                 golden outputs of the compiled reference's lstric (modelling
+                                 bit-exact; CG inversion to 1e-6); the adjoint identity
+                                 <A m, d> = <m, A^T d>; linearity; a point source against afd2d-free
+                                 closed checks (zero field -> zero data, causality)
 """
 import ctypes
 import os
@@ -76,6 +80,10 @@ def lib():
         L.orc_fwi.restype = ctypes.c_int
         L.orc_source_map.argtypes = [ctypes.c_int] * 6 + [_f32p]
         L.orc_gradient_smooth2.argtypes = [ctypes.c_int] * 5 + [ctypes.c_float, _f32p]
+        L.orc_timerev.argtypes = [ctypes.c_int] * 4 + [ctypes.c_float] * 4 + [_f32p, ctypes.c_int, ctypes.c_int,
+                                                                               _f32p, _f32p]
+        L.orc_lstri.argtypes = [ctypes.c_int] * 5 + [ctypes.c_float] * 4 + [_f32p, ctypes.c_int, ctypes.c_int,
+                                                                              _f32p, _f32p]
         _LIB = L
     return _LIB
 
@@ -262,3 +270,46 @@ def gradient_smooth2(grad, rectx, rectz, waterz, scaling):
     g = np.ascontiguousarray(np.asarray(grad, np.float32).ravel(order="F")).copy()
     lib().orc_gradient_smooth2(rectx, rectz, nx, nz, waterz, np.float32(scaling), g)
     return g.reshape(nz, nx, order="F")
+
+
+def _tri_args(pararray):
+    p = np.asarray(pararray, np.float32)
+    return dict(nz=int(p[0]), nx=int(p[1]), nt=int(p[6]), ns=int(p[10]), nb=int(p[14]), dt=float(p[7]),
+                dx=float(p[3]), dz=float(p[2]), coef=float(p[15]), niter=int(p[19]), inv=int(p[9] != 0))
+
+
+def timerev(pararray, vel, arr, adj):
+    """timerev_lop for one shot: adj=False maps a source field (nz, nx, nt) to data (nt, nx);
+    adj=True maps data (nt, nx) to an image field (nz, nx, nt)."""
+    t = _tri_args(pararray)
+    v = np.ascontiguousarray(np.asarray(vel, np.float32).ravel(order="F"))
+    mod = np.zeros(t["nz"] * t["nx"] * t["nt"], np.float32)
+    dat = np.zeros(t["nx"] * t["nt"], np.float32)
+    if adj:
+        dat[:] = np.asarray(arr, np.float32).ravel(order="F")
+    else:
+        mod[:] = np.asarray(arr, np.float32).ravel(order="F")
+    lib().orc_timerev(t["nz"], t["nx"], t["nt"], t["nb"], t["dt"], t["dx"], t["dz"], t["coef"], v, int(adj), 0,
+                       mod, dat)
+    if adj:
+        return mod.reshape(t["nz"], t["nx"], t["nt"], order="F")
+    return dat.reshape(t["nt"], t["nx"], order="F")
+
+
+def lstri(pararray, vel, arr):
+    """lstric: with pararray's inv flag, arr = data (nt, nx, ns) -> source field (nz, nx, nt, ns);
+    else arr = source field -> data (nt, nx, ns)."""
+    t = _tri_args(pararray)
+    v = np.ascontiguousarray(np.asarray(vel, np.float32).ravel(order="F"))
+    nm, nd = t["nz"] * t["nx"] * t["nt"] * t["ns"], t["nx"] * t["nt"] * t["ns"]
+    src = np.zeros(nm, np.float32)
+    dat = np.zeros(nd, np.float32)
+    if t["inv"]:
+        dat[:] = np.asarray(arr, np.float32).ravel(order="F")
+    else:
+        src[:] = np.asarray(arr, np.float32).ravel(order="F")
+    lib().orc_lstri(t["nz"], t["nx"], t["nt"], t["ns"], t["nb"], t["dt"], t["dx"], t["dz"], t["coef"], v, t["inv"],
+                     t["niter"], dat, src)
+    if t["inv"]:
+        return src.reshape(t["nz"], t["nx"], t["nt"], t["ns"], order="F")
+    return dat.reshape(t["nt"], t["nx"], t["ns"], order="F")

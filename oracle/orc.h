@@ -84,6 +84,21 @@ void orc_source_map(int sx, int sz, int rectx, int rectz, int padnx, int padnz, 
 void orc_gradient_smooth2(int rectx, int rectz, int nx, int nz, int waterz, float scaling,
                           float *grad);
 
+/* ---- least-squares time-reversal imaging (pfwi mode 4 lstric: pywave/src/wave_triutil.c,
+ *      wave_fwi.c:121-210, wave_fwigradient.c:454-525, wave_bigsolver.c, wave_cgstep.c) ---- */
+
+/* timerev_lop: adj = 0 models dat[ix][it] (nx*nt) from mod[it][ix][iz] (nt*nx*nz); adj = 1 the
+ * adjoint; add = 0 clears the output first.  Receivers at every column, one cell below the top
+ * of the model; decay boundary with coefficient coef over nb cells. */
+int orc_timerev(int nz, int nx, int nt, int nb, float dt, float dx, float dz, float coef,
+                const float *vel, int adj, int add, float *mod, float *dat);
+
+/* lstri over ns shots: inv = 0 models data[is][ix][it] from src[is][it][ix][iz]; inv = 1 solves
+ * for src with niter conjugate-gradient iterations per shot (the step state carries from one
+ * shot to the next, restarting at the first shot). */
+int orc_lstri(int nz, int nx, int nt, int ns, int nb, float dt, float dx, float dz, float coef,
+              const float *vel, int inv, int niter, float *data, float *src);
+
 #ifdef __cplusplus
 }
 #endif

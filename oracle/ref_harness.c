@@ -18,6 +18,9 @@
  *   pfwigrad: pywave/src/pfwi.c:1507-2034 setup (passive, only velocity, known source field)
  *            -> gradient_pas_init/gradient_pas_av (wave_fwigradient.c:528,672), 1 or 2 calls
  *   pfwi  : pywave/src/pfwi.c:1507-2034 -> pfwi (wave_fwi.c:212)
+ *   lstri : pywave/src/pfwi.c:1031-1505 -> lstri (wave_fwi.c:121) -> lstri_op
+ *           (wave_fwigradient.c:454): passive modelling (inv = 0) or least-squares
+ *           time-reversal source inversion (inv = 1, CG with np_cgstep)
  *   ricker: the wavelet construction inside fdm2d (wave_fdm.c:502-516)
  *
  * All arrays are raw little-endian float32 files.  Uninitialised heap memory is
@@ -471,12 +474,48 @@ static int run_pfwi(int argc, char **argv)
     return 0;
 }
 
+/* lstri pararray vel q wav in out : lstric; inv (pararray[9]) != 0: in = data (nt*nx*ns),
+ * out = src (nz*nx*nt*ns); else in = src, out = data */
+static int run_lstri(int argc, char **argv)
+{
+    if (argc < 8) { fprintf(stderr, "lstri: bad args\n"); return 2; }
+    float *p = read_f32(argv[2], 20);
+    fwiargs a; parse_pararray(p, &a);
+    size_t nzx = (size_t)a.nz * a.nx;
+    size_t nd = (size_t)a.nt * a.nx * a.ns, nsrc = nzx * a.nt * a.ns;
+    float *vel = read_f32(argv[3], nzx);
+    float *q = read_f32(argv[4], nzx);
+    float *wav = read_f32(argv[5], a.nt);
+    float ***data = np_floatalloc3(a.nt, a.nx, a.ns);
+    float ****src = np_floatalloc4(a.nz, a.nx, a.nt, a.ns);
+    float *in = read_f32(argv[6], a.inv ? nd : nsrc);
+    if (a.inv) memcpy(data[0][0], in, sizeof(float) * nd);
+    else memcpy(src[0][0][0], in, sizeof(float) * nsrc);
+
+    np_sou soupar = (np_sou)np_alloc(1, sizeof(*soupar));
+    np_acqui acpar = (np_acqui)np_alloc(1, sizeof(*acpar));
+    np_vec array = (np_vec)np_alloc(1, sizeof(*array));
+    fill_acpar(&a, soupar, acpar);
+    np_pas paspar = passive_init(acpar);
+    paspar->inv = a.inv;
+    paspar->onlysrc = true;
+    paspar->niter = a.niter;
+    preparation(vel, q, wav, acpar, soupar, array);
+
+    double t_start = now_s();
+    lstri(data, NULL, src, acpar, array, paspar, true);
+    fprintf(stderr, "PWREF_ELAPSED_S=%.6f\n", now_s() - t_start);
+    if (a.inv) write_f32(argv[7], src[0][0][0], nsrc);
+    else write_f32(argv[7], data[0][0], nd);
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
     /* every allocation through mmap: fresh, zero-filled pages */
     mallopt(M_MMAP_THRESHOLD, 0);
     mallopt(M_MMAP_MAX, 1 << 30);
-    if (argc < 2) { fprintf(stderr, "usage: pwref {afd2d|afd3d|ricker|fwimod|fwigrad|fwi|pfwigrad|pfwi} ...\n"); return 2; }
+    if (argc < 2) { fprintf(stderr, "usage: pwref {afd2d|afd3d|ricker|fwimod|fwigrad|fwi|pfwigrad|pfwi|lstri} ...\n"); return 2; }
     /* the engines print progress on stdout; keep it off the caller's stdout */
     if (!getenv("PWREF_KEEP_STDOUT")) { if (!freopen("/dev/null", "w", stdout)) return 4; }
     if (!strcmp(argv[1], "afd2d")) return run_afd2d(argc, argv);
@@ -487,6 +526,7 @@ int main(int argc, char **argv)
     if (!strcmp(argv[1], "fwi")) return run_fwi(argc, argv);
     if (!strcmp(argv[1], "pfwigrad")) return run_pfwigrad(argc, argv);
     if (!strcmp(argv[1], "pfwi")) return run_pfwi(argc, argv);
+    if (!strcmp(argv[1], "lstri")) return run_lstri(argc, argv);
     fprintf(stderr, "unknown command %s\n", argv[1]);
     return 2;
 }

@@ -155,3 +155,19 @@ def pfwi(pararray, vel, q, wav, data, src):
         el = _run(args, tmp)
         v = np.fromfile(args[-1], np.float32)
     return v.reshape(nz, nx, niter, order="F"), el
+
+
+def lstri(pararray, vel, q, wav, arr):
+    """lstric: pararray[9] (inv) != 0 -> arr = data (nt, nx, ns), returns the inverted source
+    field (nz, nx, nt, ns); else arr = source field, returns data (nt, nx, ns).  (out, elapsed)"""
+    p = np.asarray(pararray, np.float32)
+    nz, nx, nt, ns = int(p[0]), int(p[1]), int(p[6]), int(p[10])
+    with tempfile.TemporaryDirectory() as tmp:
+        args = ["lstri", _put(tmp, "par.f32", p), _put(tmp, "vel.f32", np.asarray(vel).ravel(order="F")),
+                _put(tmp, "q.f32", np.asarray(q).ravel(order="F")), _put(tmp, "wav.f32", wav),
+                _put(tmp, "in.f32", np.asarray(arr).ravel(order="F")), os.path.join(tmp, "out.f32")]
+        el = _run(args, tmp)
+        o = np.fromfile(args[-1], np.float32)
+    if p[9]:
+        return o.reshape(nz, nx, nt, ns, order="F"), el
+    return o.reshape(nt, nx, ns, order="F"), el

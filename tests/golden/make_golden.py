@@ -170,6 +170,28 @@ def passive_cases():
          grad1=grads[0], grad2=grads[1], fcost=fc, vinv=vinv)
 
 
+def lstri_cases():
+    """pfwi mode 4 lstric: passive modelling (inv=0) and least-squares time-reversal source
+    inversion (inv=1, onlysrc) through the tri2d propagator (pywave/src/wave_triutil.c:14-180,
+    wave_fwi.c:121-210, wave_fwigradient.c:454-525)."""
+    nz, nx, nt = 16, 24, 120
+    vel = np.full((nz, nx), 2.0, np.float32)
+    vel[6:] = 2.3
+    vel[11:] = 2.7
+    q = np.full((nz, nx), 1e4, np.float32)
+    wav = np.zeros(nt, np.float32)  # unused by lstric
+    events = [[(12, 7, 25, 0.03, 1.0), (9, 16, 20, 0.05, -0.6)],
+              [(13, 18, 22, 0.04, 0.9)]]
+    src = passive_field(nz, nx, nt, events)
+    par = np.array([nz, nx, 0.01, 0.01, 0, 0, nt, 0.001, 0, 0, 2, 0.2, 0.05, 5, 8, 0.02, 10, 1, 1, 6],
+                   np.float32)
+    data, _ = refrun.lstri(par, vel, q, wav, src)
+    par_inv = par.copy()
+    par_inv[9] = 1
+    src_inv, _ = refrun.lstri(par_inv, vel, q, wav, data)
+    save("lstri_small", par=par, par_inv=par_inv, vel=vel, src=src, data=data, src_inv=src_inv)
+
+
 if __name__ == "__main__":
     if not refrun.available():
         sys.exit("oracle/_ref/pwref missing: run `make -C oracle ref` where the reference tree exists")
@@ -178,3 +200,4 @@ if __name__ == "__main__":
     afd3d_cases()
     fwi_cases()
     passive_cases()
+    lstri_cases()

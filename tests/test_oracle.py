@@ -287,3 +287,53 @@ def test_gradient_smooth2_matches_convolution():
     # the smoother preserves the sum (adjoint of a normalised reflecting smoother)
     s = O.gradient_smooth2(g, 3, 3, 0, 1.0)
     assert abs(float(s.sum()) - float(g.sum())) < 1e-4
+
+
+# ------------------------------------------------------------------ least-squares TRI (lstric)
+
+def test_lstri_modelling_bit_exact(golden):
+    g = golden("lstri_small")
+    assert np.array_equal(O.lstri(g["par"], g["vel"], g["src"]), g["data"])
+
+
+def test_lstri_inversion_bit_exact(golden):
+    # six CG iterations per shot; the second shot starts from the first shot's last step
+    # (np_cgstep keeps its step vectors between calls, pywave/src/wave_cgstep.c:26-31)
+    g = golden("lstri_small")
+    assert np.array_equal(O.lstri(g["par_inv"], g["vel"], g["data"]), g["src_inv"])
+
+
+def test_timerev_adjoint_identity(golden):
+    # <A m, d> == <m, A^T d>: the migration branch is the transpose of the modelling branch
+    g = golden("lstri_small")
+    rng = np.random.default_rng(2)
+    par = g["par"]
+    m = rng.standard_normal(g["src"].shape[:3]).astype(np.float32)
+    d = rng.standard_normal(g["data"].shape[:2]).astype(np.float32)
+    am = O.timerev(par, g["vel"], m, adj=False).astype(np.float64)
+    atd = O.timerev(par, g["vel"], d, adj=True).astype(np.float64)
+    lhs, rhs = float((am * d).sum()), float((m * atd).sum())
+    assert abs(lhs - rhs) <= 1e-5 * max(abs(lhs), abs(rhs))
+    # a planted slip (the data injected one sample late) breaks the identity
+    d_late = np.roll(d, 1, axis=0)
+    bad = float((m * O.timerev(par, g["vel"], d_late, adj=True).astype(np.float64)).sum())
+    assert abs(lhs - bad) > 1e-2 * abs(lhs)
+
+
+def test_lstri_modelling_linear_and_causal(golden):
+    g = golden("lstri_small")
+    d1 = O.lstri(g["par"], g["vel"], g["src"])
+    assert np.array_equal(O.lstri(g["par"], g["vel"], 2 * g["src"]), 2 * d1)
+    assert not O.lstri(g["par"], g["vel"], 0 * g["src"]).any()
+    # a field that starts at sample k gives no data before k + 1 (record, then step, then inject)
+    late = g["src"].copy()
+    late[:, :, :60, :] = 0
+    d = O.lstri(g["par"], g["vel"], late)
+    assert not d[:61].any() and d[61:].any()
+
+
+def test_lstri_inversion_reduces_residual(golden):
+    g = golden("lstri_small")
+    d_fit = O.lstri(g["par"], g["vel"], g["src_inv"])
+    r = np.linalg.norm(d_fit - g["data"]) / np.linalg.norm(g["data"])
+    assert r < 0.5
