@@ -158,6 +158,36 @@ int pw_fwic(pw_ctx *ctx, const float *vel, const float *q, const float *wav, con
 int pw_pfwic(pw_ctx *ctx, const float *vel, const float *q, const float *wav, const float *datasrc,
              const float *pararray, float *out, long long out_len, long long datasrc_len);
 
+/* ---------------------------------------------------------------- passive modelling and LS-TRI
+ * pfwi mode 4's lstric (pywave/src/pfwi.c:1031-1505 -> lstri, pywave/src/wave_fwi.c:121-210 ->
+ * lstri_op, pywave/src/wave_fwigradient.c:454-525): the time-reversal operator timerev_lop
+ * (pywave/src/wave_triutil.c:48-180) maps a space-time source field (nz,nx,nt) per shot to data
+ * (nt,nx) recorded at every column one cell below the top of the model; its adjoint maps data to
+ * an image field; the inversion is the reference's conjugate-gradient solver (np_solver with
+ * np_cgstep, pywave/src/wave_bigsolver.c:1170-1450, wave_cgstep.c:17-67).  Geometry from the
+ * 20-float pfwi pararray (nz, nx, dz, dx, nt, dt, ns, nb, coef; niter for pw_lstric). */
+typedef struct pw_lstri pw_lstri;
+pw_lstri *pw_lstri_create(pw_ctx *ctx, const float *pararray);
+void pw_lstri_destroy(pw_lstri *p);
+/* vel (nz,nx) Fortran order */
+int pw_lstri_set_model(pw_lstri *p, const float *vel);
+/* modelling of every shot (all shots advance in one launch per step):
+ * src (nz,nx,nt,ns) Fortran -> data (nt,nx,ns) Fortran */
+int pw_lstri_model(pw_lstri *p, const float *src, float *data);
+/* adjoint (time-reversal image) of every shot: data (nt,nx,ns) -> field (nz,nx,nt,ns) */
+int pw_lstri_image(pw_lstri *p, const float *data, float *img);
+/* niter CG iterations per shot from a zero field; the CG step vectors carry over from one shot to
+ * the next as the reference's np_cgstep state does (restart at the first shot of the call).
+ * iters_out: ns ints or NULL (iterations run per shot). */
+int pw_lstri_invert(pw_lstri *p, const float *data, int niter, float *src, int *iters_out);
+/* device milliseconds of the last pw_lstri_model / pw_lstri_invert call */
+int pw_lstri_timing(pw_lstri *p, double *ms);
+/* pfwicfun.lstric (called by pywave/pfwi.py:160 and :247): pararray[9] (inv) != 0 -> datasrc =
+ * data (nt*nx*ns), out = source field (nz*nx*nt*ns) after pararray[19] CG iterations per shot;
+ * else datasrc = source field, out = data.  in_len / out_len in floats. */
+int pw_lstric(pw_ctx *ctx, const float *vel, const float *q, const float *wav, const float *datasrc,
+              const float *pararray, float *out, long long out_len, long long in_len);
+
 /* ---------------------------------------------------------------- time-reversal imaging of groups
  * Back-propagates ngroups (1..4) receiver groups of one data set together (one launch per step,
  * the model read once for all groups) and accumulates the source-location image

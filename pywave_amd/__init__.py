@@ -2,12 +2,14 @@
 
 Drop-ins for the reference's hot path (github.com/chenyk1990/pywave):
   afd2d, afd3d   2D/3D acoustic FD modelling and time-reversal imaging (pywave/afd2d.py, afd3d.py)
-  pfwi, fillpar  FWI forward modelling (mode 1) and L-BFGS FWI (mode 2) (pywave/pfwi.py)
+  pfwi, fillpar  FWI forward modelling (mode 1), L-BFGS FWI (mode 2), passive modelling, LS-TRI
+                 source inversion and passive velocity FWI (mode 4) (pywave/pfwi.py)
 Extensions beyond the reference's Python surface:
   AfdPlan        resident modelling problem (set-up once, run many times)
   tri3d_image    batched time-reversal imaging of receiver groups with a fused image
   FwiProblem     FWI gradients (reference imaging condition or exact adjoint), batched shots
   FwiForward     torch.autograd.Function wrapper of the FWI forward map and its adjoint
+  LstriProblem   resident passive modelling / time-reversal image / LS-TRI inversion
 All numerics run in hand-written HIP kernels (pywave_amd/csrc); there is no CPU fallback.
 """
 from .afd import afd2d, afd3d, AfdPlan, count_snaps  # noqa: F401
@@ -15,6 +17,7 @@ from .pfwi import pfwi, fillpar  # noqa: F401
 from .fwi import FwiProblem, FwiForward, fwi_traces, make_pararray  # noqa: F401
 from .tri import tri3d_image, group_masks  # noqa: F401
 from .batch import Afd2dBatch, afd2d_batch  # noqa: F401
+from .lstri import LstriProblem  # noqa: F401
 from .io import binread, binwrite  # noqa: F401
 
 __version__ = "0.1.0"

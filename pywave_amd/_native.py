@@ -72,6 +72,14 @@ SIGNATURES = {
     "pw_forward_modeling_ac": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_ll]),
     "pw_fwic": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_ll]),
     "pw_pfwic": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_ll, c_ll]),
+    "pw_lstri_create": (c_vp, [c_vp, c_vp]),
+    "pw_lstri_destroy": (None, [c_vp]),
+    "pw_lstri_set_model": (c_int, [c_vp, c_vp]),
+    "pw_lstri_model": (c_int, [c_vp, c_vp, c_vp]),
+    "pw_lstri_image": (c_int, [c_vp, c_vp, c_vp]),
+    "pw_lstri_invert": (c_int, [c_vp, c_vp, c_int, c_vp, c_vp]),
+    "pw_lstri_timing": (c_int, [c_vp, c_vp]),
+    "pw_lstric": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_ll, c_ll]),
     "pw_tri_create": (c_vp, [c_vp, c_int, c_int, c_int, c_float, c_float, c_float, c_int, c_float, c_int, c_float,
                              c_int, c_int]),
     "pw_tri_set_velocity": (c_int, [c_vp, c_vp]),

@@ -30,6 +30,7 @@
 #include <cstring>
 #include <vector>
 
+#include "pw_buffer.h"
 #include "pw_internal.h"
 
 namespace {
@@ -91,51 +92,13 @@ __device__ __forceinline__ T sponge(T v, T zl, T zh, T xl, T xh)
 }
 
 // ---------------------------------------------------------------- buffer-resource access
-// Fields are read and written through buffer resources (scalar base, 32-bit lane offsets); a
-// lane whose cell is outside the padded grid gets an out-of-range offset, answered with 0 (the
-// value the reference's zero boundary gives) without a memory access, and its stores are dropped.
-constexpr unsigned kOOB = 0x80000000u;
-
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t win(const void* base, unsigned bytes)
-{
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
-}
-
-template <typename T>
-__device__ __forceinline__ T bld(__amdgpu_buffer_rsrc_t r, unsigned off, bool ok);
-template <>
-__device__ __forceinline__ float bld<float>(__amdgpu_buffer_rsrc_t r, unsigned off, bool ok)
-{
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, ok ? off : kOOB, 0, 0));
-}
-template <>
-__device__ __forceinline__ double bld<double>(__amdgpu_buffer_rsrc_t r, unsigned off, bool ok)
-{
-    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, ok ? off : kOOB, 0, 0));
-}
-
-// the same with a scalar offset added to the lane offset
-template <typename T>
-__device__ __forceinline__ T bldo(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff);
-template <>
-__device__ __forceinline__ float bldo<float>(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
-{
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
-}
-template <>
-__device__ __forceinline__ double bldo<double>(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
-{
-    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
-}
-__device__ __forceinline__ void bsto(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, float v)
-{
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
-}
-__device__ __forceinline__ void bsto(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, double v)
-{
-    using u32x2 = unsigned __attribute__((ext_vector_type(2)));
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, 0);
-}
+// (pw_buffer.h): a lane whose cell is outside the padded grid gets an out-of-range offset,
+// answered with 0 without a memory access; its stores are dropped.
+using pwbuf::kOOB;
+using pwbuf::win;
+using pwbuf::bld;
+using pwbuf::bldo;
+using pwbuf::bsto;
 
 
 // ---------------------------------------------------------------- forward step (+ modelling)

@@ -1,0 +1,845 @@
+// pw_lstri.hip -- pfwi mode 4's passive modelling and least-squares time-reversal imaging
+// (lstric) on MI355X.
+//
+// Reference path (pywave = github.com/chenyk1990/pywave):
+//   entry        pywave/pfwi.py:120-165, :212-250 -> pfwicfun.lstric (pywave/src/pfwi.c:1031-1505)
+//   driver       pywave/src/wave_fwi.c:121-210 (lstri), wave_fwigradient.c:454-525 (lstri_op)
+//   operator     pywave/src/wave_triutil.c:14-180 (tri2d_make, timerev_init, timerev_lop), the
+//                8th-order Lap/LapT of wave_triutil.h, decay boundary pfwiabc (wave_triutil.c:264-317)
+//   solver       pywave/src/wave_bigsolver.c:1170-1450 (np_solver, unit model weight) with
+//                wave_cgstep.c:17-67 (np_cgstep)
+//
+// The operator maps a space-time source field w[it][ix][iz] to data d[ix][it] recorded one cell
+// below the top of the model: per step, record u1 at the receiver row, u2 = (Lx/dx^2 + Lz/dz^2)
+// v^2dt^2 + 2 u1 - u0 on the padded interior, add w v^2dt^2 on the model, decay both levels at the
+// edges.  One fused kernel per step and shot batch does all of it per cell: the stored level is
+// already decayed, the previous one is decayed again on read (the reference decays both levels
+// after every step).  The adjoint kernel runs the transposed recursion (LapT of u1 v^2dt^2, image
+// = u1 v^2dt^2, data injected at the receiver row).  Per-cell f32 arithmetic follows the
+// reference's order, so modelled data and images are bit-identical to the reference C.
+//
+// The solver is the reference's conjugate-gradient iteration on the device: model vectors
+// (nt*nz*nx) and data vectors (nt*nx) stay in HBM; dot products are f64 reductions.  Its step
+// vectors carry over from one shot to the next as np_cgstep's static state does within a call
+// (restart only at the call's first shot), so shots of an inversion run one after the other.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "pw_buffer.h"
+#include "pw_internal.h"
+
+namespace {
+
+using pwbuf::bld;
+using pwbuf::bldo;
+using pwbuf::bsto;
+using pwbuf::kOOB;
+using pwbuf::win;
+
+// stencil weights as float quotients (wave_triutil.h)
+constexpr float W0 = -205.0f / 72.0f, W1 = 8.0f / 5.0f, W2 = -1.0f / 5.0f, W3 = 8.0f / 315.0f,
+                W4 = -1.0f / 560.0f;
+
+struct TriGeo {
+    int pz, px, ldz;        // padded extents, z stride of a field
+    int nz, nx, nb, depth;  // model extents, decay width, receiver row (padded index)
+    float sx, sz;           // 1/dx^2, 1/dz^2
+    long long fs;           // elements per field
+};
+
+struct TriArgs {
+    const float* X;   // modelling: u1 (decayed); adjoint: the current level
+    float* Y;         // in: the previous level (decayed once); out: the new level (decayed)
+    const float* v2;  // v*v*dt^2 on the padded grid, edge-replicated
+    const float* sfz; // decay factor pairs (top, bottom) per padded z
+    const float* sfx; // per padded x (left, right)
+    float* plane;     // modelling: source plane of this step (read); adjoint: image plane (written)
+    long long pss;    // slot stride of the plane ([slot][nx][nz] within a step)
+    float* row;       // modelling: recorded row of this step (written); adjoint: injected row (read)
+    long long rss;    // slot stride of the row
+};
+
+__device__ __forceinline__ float decay(float v, float zl, float zh, float xl, float xh)
+{
+    v = zl * v; v = zh * v;  // top and bottom rows (pfwiabc_apply's first loop)
+    v = xl * v; v = xh * v;  // left and right columns
+    return v;
+}
+
+// One modelling step: one 64 x XCT tile of one shot per block (one wave), streamed along x with a
+// 9-column register queue and an LDS z row; the loads of column x+1 and the centre of column x+5
+// are issued before column x is computed.
+template <int XCT>
+__global__ __launch_bounds__(64) void k_trm(TriGeo g, TriArgs a, int ntz)
+{
+    const int slot = blockIdx.y;
+    const int tz = blockIdx.x % ntz, xc = blockIdx.x / ntz;
+    const int lane = threadIdx.x, z0 = tz * 64, z = z0 + lane;
+    const int x0 = xc * XCT, x1 = min(x0 + XCT, g.px);
+    __shared__ float row[2][72];
+    const int L = g.ldz;
+    const int bias = 4 * L + 4;  // windows start 4 columns + 4 cells before the field
+    const unsigned wb = (unsigned)((g.fs + 2 * bias) * 4);
+    const auto rX = win(a.X + slot * g.fs - bias, wb), rY = win(a.Y + slot * g.fs - bias, wb),
+               rV = win(a.v2 - bias, wb);
+    const bool zok = z < g.pz;
+    const bool zin = z >= 4 && z < g.pz - 4;
+    const float zl = zok ? a.sfz[2 * z] : 1.f, zh = zok ? a.sfz[2 * z + 1] : 1.f;
+    const int zhz = lane < 4 ? z0 - 4 + lane : z0 + 60 + lane;  // z halo of lanes 0..7
+    const int lzh = lane < 4 ? lane : 64 + lane;
+    const bool zhv = lane < 8 && zhz >= 0 && zhz < g.pz;
+    const unsigned oz = zok ? (unsigned)(bias + z) * 4 : kOOB, ozh = zhv ? (unsigned)(bias + zhz) * 4 : kOOB;
+    const unsigned oc = (unsigned)L * 4;
+    auto xv = [&](int x) { return x >= 0 && x < g.px; };
+    auto ldc = [&](__amdgpu_buffer_rsrc_t r, unsigned voff, int x, bool ok) {
+        return bldo<float>(r, ok ? voff : kOOB, ok ? (unsigned)x * oc : 0u);
+    };
+    // the source plane covers the model interior [nb, nb+nz) x [nb, nb+nx)
+    const bool mz = z >= g.nb && z < g.nb + g.nz;
+    const auto rS = win(a.plane + slot * a.pss, (unsigned)(g.nz * g.nx * 4));
+    const unsigned ofz = mz ? (unsigned)(z - g.nb) * 4 : kOOB;
+    auto mx = [&](int x) { return x >= g.nb && x < g.nb + g.nx; };
+    auto lds = [&](int x) { return bldo<float>(rS, mx(x) ? ofz : kOOB, mx(x) ? (unsigned)((x - g.nb) * g.nz) * 4 : 0u); };
+    const bool rlane = z == g.depth;
+    float* rec = a.row + slot * a.rss;
+    float q[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        const int xx = x0 - 4 + k;
+        q[k] = ldc(rX, oz, xx, xv(xx));
+    }
+    float yv = ldc(rY, oz, x0, true), vv = ldc(rV, oz, x0, true), hz = ldc(rX, ozh, x0, true), sv = lds(x0);
+    float xl = a.sfx[2 * x0], xh = a.sfx[2 * x0 + 1];
+    int buf = 0;
+    for (int x = x0; x < x1; ++x) {
+        const int xn = x + 1;
+        const bool nv = xn < x1;
+        const float nq = ldc(rX, oz, x + 5, nv && x + 5 < g.px);
+        const float nyv = ldc(rY, oz, xn, nv);
+        const float nvv = ldc(rV, oz, xn, nv);
+        const float nhz = ldc(rX, ozh, xn, nv);
+        const float nsv = lds(xn);
+        const int xs = xn < g.px ? xn : g.px - 1;
+        const float nxl = a.sfx[2 * xs], nxh = a.sfx[2 * xs + 1];
+        row[buf][lane + 4] = q[4];
+        if (lane < 8) row[buf][lzh] = hz;
+        __syncthreads();
+        if (zok) {
+            const float u1 = q[4];
+            if (rlane && mx(x)) rec[x - g.nb] = 0.f + u1;  // record before the step
+            float u2 = 0.f;                              // the 4-cell rim stays zero
+            if (zin && x >= 4 && x < g.px - 4) {
+                const float* R = &row[buf][lane + 4];
+                const float lx = W4 * (q[8] + q[0]) + W3 * (q[7] + q[1]) + W2 * (q[6] + q[2]) + W1 * (q[5] + q[3]) +
+                                 W0 * q[4];
+                const float lz = W4 * (R[4] + R[-4]) + W3 * (R[3] + R[-3]) + W2 * (R[2] + R[-2]) +
+                                 W1 * (R[1] + R[-1]) + W0 * R[0];
+                u2 = (lx * g.sx + lz * g.sz) * vv + 2.0f * u1 - decay(yv, zl, zh, xl, xh);
+            }
+            if (mz && mx(x)) u2 = u2 + sv * vv;  // inject the source after the step
+            bsto(rY, oz, (unsigned)x * oc, decay(u2, zl, zh, xl, xh));
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) q[k] = q[k + 1];
+        q[8] = nq;
+        yv = nyv; vv = nvv; hz = nhz; sv = nsv; xl = nxl; xh = nxh;
+        buf ^= 1;
+    }
+}
+
+// One adjoint step: image = A v^2dt^2 on the model, new level = decay(LapT(A v^2dt^2) + 2A -
+// decay(B) + injected data at the receiver row).  Queues of the raw level and of its product with
+// v^2dt^2 (the transposed stencil weights the neighbours' products).
+template <int XCT>
+__global__ __launch_bounds__(64) void k_tra(TriGeo g, TriArgs a, int ntz)
+{
+    const int slot = blockIdx.y;
+    const int tz = blockIdx.x % ntz, xc = blockIdx.x / ntz;
+    const int lane = threadIdx.x, z0 = tz * 64, z = z0 + lane;
+    const int x0 = xc * XCT, x1 = min(x0 + XCT, g.px);
+    __shared__ float row[2][72];
+    const int L = g.ldz;
+    const int bias = 4 * L + 4;
+    const unsigned wb = (unsigned)((g.fs + 2 * bias) * 4);
+    const auto rA = win(a.X + slot * g.fs - bias, wb), rB = win(a.Y + slot * g.fs - bias, wb),
+               rV = win(a.v2 - bias, wb);
+    const bool zok = z < g.pz;
+    const bool zin = z >= 4 && z < g.pz - 4;
+    const float zl = zok ? a.sfz[2 * z] : 1.f, zh = zok ? a.sfz[2 * z + 1] : 1.f;
+    const int zhz = lane < 4 ? z0 - 4 + lane : z0 + 60 + lane;
+    const int lzh = lane < 4 ? lane : 64 + lane;
+    const bool zhv = lane < 8 && zhz >= 0 && zhz < g.pz;
+    const unsigned oz = zok ? (unsigned)(bias + z) * 4 : kOOB, ozh = zhv ? (unsigned)(bias + zhz) * 4 : kOOB;
+    const unsigned oc = (unsigned)L * 4;
+    auto xv = [&](int x) { return x >= 0 && x < g.px; };
+    auto ldc = [&](__amdgpu_buffer_rsrc_t r, unsigned voff, int x, bool ok) {
+        return bldo<float>(r, ok ? voff : kOOB, ok ? (unsigned)x * oc : 0u);
+    };
+    const bool mz = z >= g.nb && z < g.nb + g.nz;
+    const auto rI = win(a.plane + slot * a.pss, (unsigned)(g.nz * g.nx * 4));
+    const unsigned ofz = mz ? (unsigned)(z - g.nb) * 4 : kOOB;
+    auto mx = [&](int x) { return x >= g.nb && x < g.nb + g.nx; };
+    const bool rlane = z == g.depth;
+    const auto rD = win(a.row + slot * a.rss, (unsigned)(g.nx * 4));
+    auto ldd = [&](int x) { return bld<float>(rD, (unsigned)(x - g.nb) * 4, rlane && mx(x)); };
+    float qr[9], qp[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        const int xx = x0 - 4 + k;
+        qr[k] = ldc(rA, oz, xx, xv(xx));
+        qp[k] = qr[k] * ldc(rV, oz, xx, xv(xx));
+    }
+    float bv = ldc(rB, oz, x0, true);
+    float hp = ldc(rA, ozh, x0, true) * ldc(rV, ozh, x0, true);
+    float dv = ldd(x0);
+    float xl = a.sfx[2 * x0], xh = a.sfx[2 * x0 + 1];
+    int buf = 0;
+    for (int x = x0; x < x1; ++x) {
+        const int xn = x + 1, x5 = x + 5;
+        const bool nv = xn < x1;
+        const bool ok5 = nv && x5 < g.px;
+        const float na = ldc(rA, oz, x5, ok5);
+        const float nvp = ldc(rV, oz, x5, ok5);
+        const float nbv = ldc(rB, oz, xn, nv);
+        const float nhp = ldc(rA, ozh, xn, nv) * ldc(rV, ozh, xn, nv);
+        const float ndv = ldd(xn);
+        const int xs = xn < g.px ? xn : g.px - 1;
+        const float nxl = a.sfx[2 * xs], nxh = a.sfx[2 * xs + 1];
+        row[buf][lane + 4] = qp[4];
+        if (lane < 8) row[buf][lzh] = hp;
+        __syncthreads();
+        if (zok) {
+            const float u1 = qr[4];
+            if (mz && mx(x))  // image of this step, u1 v^2dt^2 on the model
+                bsto(rI, ofz, (unsigned)((x - g.nb) * g.nz) * 4, 0.f + qp[4]);
+            float u2 = 0.f;
+            if (zin && x >= 4 && x < g.px - 4) {
+                const float* R = &row[buf][lane + 4];
+                const float lx = W4 * (qp[8] + qp[0]) + W3 * (qp[7] + qp[1]) + W2 * (qp[6] + qp[2]) +
+                                 W1 * (qp[5] + qp[3]) + W0 * qp[4];
+                const float lz = W4 * (R[4] + R[-4]) + W3 * (R[3] + R[-3]) + W2 * (R[2] + R[-2]) +
+                                 W1 * (R[1] + R[-1]) + W0 * R[0];
+                u2 = (lx * g.sx + lz * g.sz) + 2.0f * u1 - decay(bv, zl, zh, xl, xh);
+            }
+            if (rlane && mx(x)) u2 = u2 + dv;  // inject the data of this step after it
+            bsto(rB, oz, (unsigned)x * oc, decay(u2, zl, zh, xl, xh));
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) { qr[k] = qr[k + 1]; qp[k] = qp[k + 1]; }
+        qr[8] = na;
+        qp[8] = na * nvp;
+        bv = nbv; hp = nhp; dv = ndv; xl = nxl; xh = nxh;
+        buf ^= 1;
+    }
+}
+
+// ---------------------------------------------------------------- small kernels
+__global__ void k_tri_pad_v2(TriGeo g, const float* __restrict__ vel, float dt2, float* __restrict__ v2)
+{
+    const int iz = blockIdx.x * blockDim.x + threadIdx.x, ix = blockIdx.y;
+    if (iz >= g.pz || ix >= g.px) return;
+    const int jx = min(max(ix, g.nb), g.nb + g.nx - 1) - g.nb;
+    const int jz = min(max(iz, g.nb), g.nb + g.nz - 1) - g.nb;
+    const float v = vel[(long long)jx * g.nz + jz];
+    v2[(long long)ix * g.ldz + iz] = v * v * dt2;
+}
+
+__global__ void k_zero(long long n, float* __restrict__ p)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        p[i] = 0.f;
+}
+
+__global__ void k_negate(long long n, const float* __restrict__ a, float* __restrict__ out)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        out[i] = -a[i];
+}
+
+// [R][C] -> [C][R]
+__global__ void k_transpose32(const float* __restrict__ in, float* __restrict__ out, int R, int C)
+{
+    __shared__ float t[32][33];
+    const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+    for (int k = threadIdx.y; k < 32; k += blockDim.y) {
+        const int r = r0 + k, c = c0 + threadIdx.x;
+        if (r < R && c < C) t[k][threadIdx.x] = in[(long long)r * C + c];
+    }
+    __syncthreads();
+    for (int k = threadIdx.y; k < 32; k += blockDim.y) {
+        const int c = c0 + k, r = r0 + threadIdx.x;
+        if (r < R && c < C) out[(long long)c * R + r] = t[threadIdx.x][k];
+    }
+}
+
+constexpr int NRED = 512;  // blocks of the partial reductions
+
+// partial sums over data vectors: gg.gg, Ss.Ss, gg.Ss, gg.rr, Ss.rr, rr.rr
+__global__ __launch_bounds__(256) void k_dots_data(long long n, const float* __restrict__ gg,
+                                                   const float* __restrict__ ss, const float* __restrict__ rr,
+                                                   double* __restrict__ part)
+{
+    double s[6] = {0, 0, 0, 0, 0, 0};
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        const double g = gg[i], t = ss[i], r = rr[i];
+        s[0] += g * g; s[1] += t * t; s[2] += g * t; s[3] += g * r; s[4] += t * r; s[5] += r * r;
+    }
+    __shared__ double sh[6][256];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) sh[k][threadIdx.x] = s[k];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+#pragma unroll
+            for (int k = 0; k < 6; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) part[threadIdx.x * NRED + blockIdx.x] = sh[threadIdx.x][0];
+}
+
+// partial sum of squares over a model vector
+__global__ __launch_bounds__(256) void k_sumsq(long long n, const float* __restrict__ a, double* __restrict__ part)
+{
+    double s = 0;
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        const double v = a[i];
+        s += v * v;
+    }
+    __shared__ double sh[256];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// k sums of NRED partials each -> out[k]
+__global__ __launch_bounds__(256) void k_reduce_parts(int k, const double* __restrict__ part, double* __restrict__ out)
+{
+    __shared__ double sh[256];
+    for (int j = 0; j < k; ++j) {
+        double s = 0;
+        for (int i = threadIdx.x; i < NRED; i += 256) s += part[j * NRED + i];
+        sh[threadIdx.x] = s;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[j] = sh[0];
+        __syncthreads();
+    }
+}
+
+// S = S*beta + alpha*g; x += S (model space) -- cblas_sscal, cblas_saxpy, then the model update
+__global__ void k_cg_model(long long n, float alpha, float beta, const float* __restrict__ g, float* __restrict__ S,
+                           float* __restrict__ x)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        float s = S[i] * beta;
+        s = s + alpha * g[i];
+        S[i] = s;
+        x[i] = x[i] + s;
+    }
+}
+
+// Ss = Ss*beta + alpha*gg; rr += Ss (data space)
+__global__ void k_cg_data(long long n, float alpha, float beta, const float* __restrict__ gg, float* __restrict__ Ss,
+                          float* __restrict__ rr)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        float s = Ss[i] * beta;
+        s = s + alpha * gg[i];
+        Ss[i] = s;
+        rr[i] = rr[i] + s;
+    }
+}
+
+inline int nblk(long long n, int bs = 256) { return (int)std::min<long long>((n + bs - 1) / bs, 1 << 16); }
+
+}  // namespace
+
+// ==================================================================================== plan
+struct pw_lstri {
+    pw_ctx* ctx = nullptr;
+    int nz = 0, nx = 0, nt = 0, ns = 0, nb = 0;
+    float dz = 0, dx = 0, dt = 0, coef = 0;
+    TriGeo g{};
+    int xct = 16, ntz = 0, nxc = 0;
+    long long nzx = 0, nm = 0, nd = 0;  // model plane, model (per shot), data (per shot) sizes
+    float* v2 = nullptr;
+    float *sfz = nullptr, *sfx = nullptr;
+    float* fields = nullptr;  // 2 * cap_slots fields
+    int cap_slots = 0;
+    float *src = nullptr, *dat = nullptr, *stage = nullptr;  // batch buffers (modelling / imaging)
+    int cap_src_slots = 0;
+    float *x = nullptr, *gvec = nullptr, *S = nullptr, *rr = nullptr, *gg = nullptr, *Ss = nullptr;
+    double* part = nullptr;
+    double* sums = nullptr;
+    bool have_model = false;
+    double t_ms = 0;
+};
+
+namespace {
+
+#define LS_HIP(call)                                                                                    \
+    do {                                                                                                \
+        hipError_t e_ = (call);                                                                         \
+        if (e_ != hipSuccess) {                                                                         \
+            p->ctx->err = std::string(#call) + ": " + hipGetErrorString(e_);                            \
+            return PW_ERR_HIP;                                                                          \
+        }                                                                                               \
+    } while (0)
+
+int ls_alloc(pw_lstri* p, float** ptr, long long n)
+{
+    if (hipMalloc((void**)ptr, (size_t)std::max<long long>(n, 1) * sizeof(float)) != hipSuccess) {
+        *ptr = nullptr;
+        p->ctx->err = "pw_lstri: device allocation of " + std::to_string(n * 4) + " bytes failed";
+        return PW_ERR_NOMEM;
+    }
+    return PW_OK;
+}
+
+int zero(pw_lstri* p, float* ptr, long long n)
+{
+    hipLaunchKernelGGL(k_zero, dim3(nblk(n)), dim3(256), 0, p->ctx->s(), n, ptr);
+    LS_HIP(hipGetLastError());
+    return PW_OK;
+}
+
+int ensure_fields(pw_lstri* p, int slots)
+{
+    if (slots <= p->cap_slots) return PW_OK;
+    if (p->fields) hipFree(p->fields);
+    p->fields = nullptr;
+    p->cap_slots = 0;
+    int rc = ls_alloc(p, &p->fields, 2LL * slots * p->g.fs);
+    if (!rc) p->cap_slots = slots;
+    return rc;
+}
+
+template <int XCT>
+void launch_step(pw_lstri* p, bool adj, int nslot, const TriArgs& a)
+{
+    dim3 grid(p->ntz * ((p->g.px + XCT - 1) / XCT), nslot);
+    if (adj) hipLaunchKernelGGL(k_tra<XCT>, grid, dim3(64), 0, p->ctx->s(), p->g, a, p->ntz);
+    else hipLaunchKernelGGL(k_trm<XCT>, grid, dim3(64), 0, p->ctx->s(), p->g, a, p->ntz);
+}
+
+void step(pw_lstri* p, bool adj, int nslot, const TriArgs& a)
+{
+    switch (p->xct) {
+        case 4: launch_step<4>(p, adj, nslot, a); break;
+        case 8: launch_step<8>(p, adj, nslot, a); break;
+        default: launch_step<16>(p, adj, nslot, a); break;
+    }
+}
+
+// modelling sweep: w [slot][nt][nx][nz] -> d [slot][nt][nx] (time-major), nslot shots at once
+int sweep_model(pw_lstri* p, int nslot, const float* w, float* d)
+{
+    int rc = ensure_fields(p, nslot);
+    if (!rc) rc = zero(p, p->fields, 2LL * nslot * p->g.fs);
+    if (rc) return rc;
+    float* X = p->fields;
+    float* Y = p->fields + (long long)nslot * p->g.fs;
+    for (int it = 0; it < p->nt; ++it) {
+        TriArgs a{};
+        a.X = X; a.Y = Y; a.v2 = p->v2; a.sfz = p->sfz; a.sfx = p->sfx;
+        a.plane = const_cast<float*>(w) + (long long)it * p->nzx;
+        a.pss = p->nm;
+        a.row = d + (long long)it * p->nx;
+        a.rss = p->nd;
+        step(p, false, nslot, a);
+        std::swap(X, Y);
+    }
+    LS_HIP(hipGetLastError());
+    return PW_OK;
+}
+
+// adjoint sweep: d [slot][nt][nx] -> image w [slot][nt][nx][nz] (every plane written)
+int sweep_adjoint(pw_lstri* p, int nslot, const float* d, float* w)
+{
+    int rc = ensure_fields(p, nslot);
+    if (!rc) rc = zero(p, p->fields, 2LL * nslot * p->g.fs);
+    if (rc) return rc;
+    float* A = p->fields;
+    float* B = p->fields + (long long)nslot * p->g.fs;
+    for (int it = p->nt - 1; it >= 0; --it) {
+        TriArgs a{};
+        a.X = A; a.Y = B; a.v2 = p->v2; a.sfz = p->sfz; a.sfx = p->sfx;
+        a.plane = w + (long long)it * p->nzx;
+        a.pss = p->nm;
+        a.row = const_cast<float*>(d) + (long long)it * p->nx;
+        a.rss = p->nd;
+        step(p, true, nslot, a);
+        std::swap(A, B);
+    }
+    LS_HIP(hipGetLastError());
+    return PW_OK;
+}
+
+// data (nt, nx, ns) Fortran = [is][ix][it] on the host <-> [is][it][ix] on the device
+int upload_data(pw_lstri* p, const float* host, int s0, int nslot, float* dev)
+{
+    hipStream_t s = p->ctx->s();
+    const long long nd = p->nd;
+    LS_HIP(hipMemcpyAsync(p->stage, host + (size_t)s0 * nd, (size_t)nslot * nd * 4, hipMemcpyHostToDevice, s));
+    for (int k = 0; k < nslot; ++k)
+        hipLaunchKernelGGL(k_transpose32, dim3((p->nt + 31) / 32, (p->nx + 31) / 32), dim3(32, 8), 0, s,
+                           p->stage + k * nd, dev + k * nd, p->nx, p->nt);
+    LS_HIP(hipGetLastError());
+    return PW_OK;
+}
+
+int download_data(pw_lstri* p, const float* dev, int s0, int nslot, float* host)
+{
+    hipStream_t s = p->ctx->s();
+    const long long nd = p->nd;
+    for (int k = 0; k < nslot; ++k)
+        hipLaunchKernelGGL(k_transpose32, dim3((p->nx + 31) / 32, (p->nt + 31) / 32), dim3(32, 8), 0, s,
+                           dev + k * nd, p->stage + k * nd, p->nt, p->nx);
+    LS_HIP(hipGetLastError());
+    LS_HIP(hipMemcpyAsync(host + (size_t)s0 * nd, p->stage, (size_t)nslot * nd * 4, hipMemcpyDeviceToHost, s));
+    LS_HIP(hipStreamSynchronize(s));
+    return PW_OK;
+}
+
+// batch buffers for nslot shots of modelling or imaging
+int ensure_batch(pw_lstri* p, int nslot)
+{
+    if (nslot <= p->cap_src_slots) return PW_OK;
+    float* bufs[] = {p->src, p->dat, p->stage};
+    for (float* b : bufs)
+        if (b) hipFree(b);
+    p->src = p->dat = p->stage = nullptr;
+    p->cap_src_slots = 0;
+    int rc = ls_alloc(p, &p->src, (long long)nslot * p->nm);
+    if (!rc) rc = ls_alloc(p, &p->dat, (long long)nslot * p->nd);
+    if (!rc) rc = ls_alloc(p, &p->stage, (long long)nslot * p->nd);
+    if (!rc) p->cap_src_slots = nslot;
+    return rc;
+}
+
+// shots per batch so that the source/image batch stays within about a quarter of free HBM
+int batch_slots(pw_lstri* p)
+{
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) fr = (size_t)8 << 30;
+    const double per = (double)(p->nm + 2 * p->nd + 2 * p->g.fs) * 4;
+    long long k = (long long)(fr / 4 / std::max(per, 1.0));
+    return (int)std::max<long long>(1, std::min<long long>(k, p->ns));
+}
+
+int reduce(pw_lstri* p, int k, double* out_host)
+{
+    hipStream_t s = p->ctx->s();
+    hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(256), 0, s, k, p->part, p->sums);
+    LS_HIP(hipGetLastError());
+    LS_HIP(hipMemcpyAsync(out_host, p->sums, k * sizeof(double), hipMemcpyDeviceToHost, s));
+    LS_HIP(hipStreamSynchronize(s));
+    return PW_OK;
+}
+
+void ls_free(pw_lstri* p)
+{
+    float* bufs[] = {p->v2, p->sfz, p->sfx, p->fields, p->src, p->dat, p->stage, p->x, p->gvec, p->S, p->rr, p->gg, p->Ss};
+    for (float* b : bufs)
+        if (b) hipFree(b);
+    if (p->part) hipFree(p->part);
+    if (p->sums) hipFree(p->sums);
+}
+
+}  // namespace
+
+// ==================================================================================== C ABI
+extern "C" {
+
+pw_lstri* pw_lstri_create(pw_ctx* ctx, const float* pararray)
+{
+    if (!ctx || !pararray) return nullptr;
+    if (hipSetDevice(ctx->device) != hipSuccess) {
+        ctx->err = "hipSetDevice failed";
+        return nullptr;
+    }
+    const float* q = pararray;
+    pw_lstri* p = new pw_lstri;
+    p->ctx = ctx;
+    p->nz = (int)q[0]; p->nx = (int)q[1]; p->dz = q[2]; p->dx = q[3];
+    p->nt = (int)q[6]; p->dt = q[7]; p->ns = (int)q[10]; p->nb = (int)q[14]; p->coef = q[15];
+    if (p->nz < 1 || p->nx < 1 || p->nt < 1 || p->ns < 1 || !(p->dx > 0) || !(p->dz > 0) || !(p->dt > 0)) {
+        ctx->err = "pw_lstri_create: invalid dimensions (nz, nx, nt, ns, dx, dz, dt must be positive)";
+        delete p;
+        return nullptr;
+    }
+    if (p->nb < 4) {
+        ctx->err = "pw_lstri_create: nb must be >= 4 (sources and receivers must stay off the 4-cell rim)";
+        delete p;
+        return nullptr;
+    }
+    TriGeo& g = p->g;
+    g.nz = p->nz; g.nx = p->nx; g.nb = p->nb;
+    g.pz = p->nz + 2 * p->nb; g.px = p->nx + 2 * p->nb;
+    g.ldz = (g.pz + 31) / 32 * 32;
+    g.depth = 1 + p->nb;  // tri2d_make: depth (rz - nb = 1) + nb
+    g.sx = 1.0f / (p->dx * p->dx);
+    g.sz = 1.0f / (p->dz * p->dz);
+    g.fs = (long long)g.ldz * g.px;
+    p->nzx = (long long)p->nz * p->nx;
+    p->nm = p->nzx * p->nt;
+    p->nd = (long long)p->nx * p->nt;
+    if ((double)(g.fs + 8 * g.ldz + 8) * 4 >= 2147483648.0 || (double)p->nzx * 4 >= 2147483648.0) {
+        ctx->err = "pw_lstri_create: padded model too large for one field window (2 GiB)";
+        delete p;
+        return nullptr;
+    }
+    if (g.depth >= g.pz - 4) {
+        ctx->err = "pw_lstri_create: receiver row reaches the 4-cell rim (nz too small)";
+        delete p;
+        return nullptr;
+    }
+    p->ntz = (g.pz + 63) / 64;
+    // narrow column chunks when the grid is small, so that a step still has several blocks per CU
+    p->xct = 16;
+    while (p->xct > 4 && (long long)p->ntz * ((g.px + p->xct - 1) / p->xct) < 1024) p->xct /= 2;
+    int rc = ls_alloc(p, &p->v2, g.fs);
+    if (!rc) rc = ls_alloc(p, &p->sfz, 2LL * g.pz);
+    if (!rc) rc = ls_alloc(p, &p->sfx, 2LL * g.px);
+    if (!rc && hipMalloc((void**)&p->part, 6 * NRED * sizeof(double)) != hipSuccess) rc = PW_ERR_NOMEM;
+    if (!rc && hipMalloc((void**)&p->sums, 8 * sizeof(double)) != hipSuccess) rc = PW_ERR_NOMEM;
+    // decay weights (pfwiabc_cal): w[i] = exp(-c*c*(nb-1-i)^2), f32 products, exp in double
+    std::vector<float> dec(p->nb), fz(2 * (size_t)g.pz, 1.f), fx(2 * (size_t)g.px, 1.f);
+    for (int i = 0; i < p->nb; ++i) {
+        const int k = p->nb - 1 - i;
+        dec[i] = (float)std::exp((double)(-p->coef * p->coef * (float)k * (float)k));
+    }
+    for (int i = 0; i < p->nb; ++i) {
+        fz[2 * i] = dec[i];
+        fz[2 * (size_t)(g.pz - 1 - i) + 1] = dec[i];
+        fx[2 * i] = dec[i];
+        fx[2 * (size_t)(g.px - 1 - i) + 1] = dec[i];
+    }
+    hipStream_t s = ctx->s();
+    if (!rc && (hipMemcpyAsync(p->sfz, fz.data(), fz.size() * 4, hipMemcpyHostToDevice, s) != hipSuccess ||
+                hipMemcpyAsync(p->sfx, fx.data(), fx.size() * 4, hipMemcpyHostToDevice, s) != hipSuccess ||
+                hipStreamSynchronize(s) != hipSuccess))
+        rc = PW_ERR_HIP;
+    if (!rc) rc = zero(p, p->v2, g.fs);
+    if (rc) {
+        if (ctx->err.empty()) ctx->err = "pw_lstri_create: set-up failed";
+        ls_free(p);
+        delete p;
+        return nullptr;
+    }
+    return p;
+}
+
+void pw_lstri_destroy(pw_lstri* p)
+{
+    if (!p) return;
+    hipSetDevice(p->ctx->device);
+    hipStreamSynchronize(p->ctx->s());
+    ls_free(p);
+    delete p;
+}
+
+// vel (nz, nx) Fortran order, km/s or any unit consistent with dx, dz, dt
+int pw_lstri_set_model(pw_lstri* p, const float* vel)
+{
+    if (!p || !vel) return PW_ERR_ARG;
+    hipStream_t s = p->ctx->s();
+    float* dv = nullptr;
+    int rc = ls_alloc(p, &dv, p->nzx);
+    if (rc) return rc;
+    const float dt2 = p->dt * p->dt;
+    if (hipMemcpyAsync(dv, vel, p->nzx * 4, hipMemcpyHostToDevice, s) != hipSuccess) rc = PW_ERR_HIP;
+    if (!rc) {
+        hipLaunchKernelGGL(k_tri_pad_v2, dim3((p->g.pz + 255) / 256, p->g.px), dim3(256), 0, s, p->g, dv, dt2, p->v2);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess) rc = PW_ERR_HIP;
+    }
+    hipFree(dv);
+    if (rc) PW_FAIL(p->ctx, rc, "pw_lstri_set_model: upload failed");
+    p->have_model = true;
+    return PW_OK;
+}
+
+// timerev_lop modelling for every shot: src (nz, nx, nt, ns) Fortran -> data (nt, nx, ns) Fortran
+int pw_lstri_model(pw_lstri* p, const float* src, float* data)
+{
+    if (!p || !src || !data) return PW_ERR_ARG;
+    if (!p->have_model) PW_FAIL(p->ctx, PW_ERR_STATE, "pw_lstri_model: model not set");
+    hipStream_t s = p->ctx->s();
+    const int bs = batch_slots(p);
+    int rc = ensure_batch(p, bs);
+    if (rc) return rc;
+    hipEvent_t e0, e1;
+    LS_HIP(hipEventCreate(&e0)); LS_HIP(hipEventCreate(&e1));
+    float ms_tot = 0;
+    for (int s0 = 0; s0 < p->ns; s0 += bs) {
+        const int k = std::min(bs, p->ns - s0);
+        LS_HIP(hipMemcpyAsync(p->src, src + (size_t)s0 * p->nm, (size_t)k * p->nm * 4, hipMemcpyHostToDevice, s));
+        LS_HIP(hipEventRecord(e0, s));
+        if ((rc = sweep_model(p, k, p->src, p->dat))) break;
+        LS_HIP(hipEventRecord(e1, s));
+        if ((rc = download_data(p, p->dat, s0, k, data))) break;
+        float ms = 0;
+        hipEventElapsedTime(&ms, e0, e1);
+        ms_tot += ms;
+    }
+    hipEventDestroy(e0); hipEventDestroy(e1);
+    p->t_ms = ms_tot;
+    return rc;
+}
+
+// timerev_lop adjoint for every shot: data (nt, nx, ns) Fortran -> image field (nz, nx, nt, ns)
+int pw_lstri_image(pw_lstri* p, const float* data, float* img)
+{
+    if (!p || !data || !img) return PW_ERR_ARG;
+    if (!p->have_model) PW_FAIL(p->ctx, PW_ERR_STATE, "pw_lstri_image: model not set");
+    hipStream_t s = p->ctx->s();
+    const int bs = batch_slots(p);
+    int rc = ensure_batch(p, bs);
+    if (rc) return rc;
+    for (int s0 = 0; s0 < p->ns && !rc; s0 += bs) {
+        const int k = std::min(bs, p->ns - s0);
+        if ((rc = upload_data(p, data, s0, k, p->dat))) break;
+        if ((rc = sweep_adjoint(p, k, p->dat, p->src))) break;
+        LS_HIP(hipMemcpyAsync(img + (size_t)s0 * p->nm, p->src, (size_t)k * p->nm * 4, hipMemcpyDeviceToHost, s));
+        LS_HIP(hipStreamSynchronize(s));
+    }
+    return rc;
+}
+
+// lstri with inversion: for each shot, niter conjugate-gradient iterations of timerev(w) ~= d
+// from w = 0 (np_solver + np_cgstep).  The step vectors carry over from one shot to the next;
+// the first shot of the call restarts.  src out: (nz, nx, nt, ns) Fortran.  iters_out (ns ints,
+// may be NULL): iterations run per shot (fewer than niter after convergence).
+int pw_lstri_invert(pw_lstri* p, const float* data, int niter, float* src, int* iters_out)
+{
+    if (!p || !data || !src || niter < 0) return PW_ERR_ARG;
+    if (!p->have_model) PW_FAIL(p->ctx, PW_ERR_STATE, "pw_lstri_invert: model not set");
+    hipStream_t s = p->ctx->s();
+    int rc = 0;
+    if (!p->x) {
+        rc = ls_alloc(p, &p->x, p->nm);
+        if (!rc) rc = ls_alloc(p, &p->gvec, p->nm);
+        if (!rc) rc = ls_alloc(p, &p->S, p->nm);
+        if (!rc) rc = ls_alloc(p, &p->rr, p->nd);
+        if (!rc) rc = ls_alloc(p, &p->gg, p->nd);
+        if (!rc) rc = ls_alloc(p, &p->Ss, p->nd);
+        if (!rc) rc = zero(p, p->S, p->nm);
+        if (!rc) rc = zero(p, p->Ss, p->nd);
+        if (rc) return rc;
+    }
+    if ((rc = ensure_batch(p, 1))) return rc;
+    const long long nm = p->nm, nd = p->nd;
+    bool started = false;
+    hipEvent_t e0, e1;
+    LS_HIP(hipEventCreate(&e0)); LS_HIP(hipEventCreate(&e1));
+    LS_HIP(hipEventRecord(e0, s));
+    for (int is = 0; is < p->ns && !rc; ++is) {
+        // rr = -d, x = 0
+        if ((rc = upload_data(p, data, is, 1, p->dat))) break;
+        hipLaunchKernelGGL(k_negate, dim3(nblk(nd)), dim3(256), 0, s, nd, p->dat, p->rr);
+        if ((rc = zero(p, p->x, nm))) break;
+        double sums[7];
+        hipLaunchKernelGGL(k_sumsq, dim3(NRED), dim3(256), 0, s, nd, p->rr, p->part);
+        if ((rc = reduce(p, 1, sums))) break;
+        const double r0 = sums[0];
+        double g0 = 1.0;
+        int iter = 0;
+        for (; iter < niter; ++iter) {
+            if ((rc = sweep_adjoint(p, 1, p->rr, p->gvec))) break;
+            if ((rc = sweep_model(p, 1, p->gvec, p->gg))) break;
+            // data-space dots (gg.gg, Ss.Ss, gg.Ss, gg.rr, Ss.rr, rr.rr) and |g|^2
+            hipLaunchKernelGGL(k_dots_data, dim3(NRED), dim3(256), 0, s, nd, p->gg, p->Ss, p->rr, p->part);
+            hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(256), 0, s, 6, p->part, p->sums);
+            hipLaunchKernelGGL(k_sumsq, dim3(NRED), dim3(256), 0, s, nm, p->gvec, p->part);
+            hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(256), 0, s, 1, p->part, p->sums + 6);
+            LS_HIP(hipGetLastError());
+            LS_HIP(hipMemcpyAsync(sums, p->sums, 7 * sizeof(double), hipMemcpyDeviceToHost, s));
+            LS_HIP(hipStreamSynchronize(s));
+            // convergence test of np_solver (squared norms relative to the first iteration)
+            double dpr = 1.0, dpg = 1.0;
+            if (iter == 0) g0 = sums[6];
+            else { dpr = sums[5] / r0; dpg = sums[6] / g0; }
+            if (dpr < 1.e-12 || dpg < 1.e-12) break;
+            // np_cgstep
+            double alfa, beta;
+            bool skip = false;
+            if (!started) {
+                started = true;
+                if ((rc = zero(p, p->S, nm)) || (rc = zero(p, p->Ss, nd))) break;
+                beta = 0.0;
+                if (sums[0] <= 0.) { skip = true; alfa = 0; }
+                else alfa = -sums[3] / sums[0];
+            } else {
+                const double gdg = sums[0], sds = sums[1], gds = sums[2];
+                if (gdg == 0. || sds == 0.) {
+                    skip = true; alfa = beta = 0;
+                } else {
+                    double det = 1.0 - (gds / gdg) * (gds / sds);
+                    if (det > (double)1.e-12f) det *= gdg * sds;
+                    else det = gdg * sds * (double)1.e-12f;
+                    const double gdr = -sums[3], sdr = -sums[4];
+                    alfa = (sds * gdr - gds * sdr) / det;
+                    beta = (-gds * gdr + gdg * sdr) / det;
+                }
+            }
+            if (skip) continue;
+            hipLaunchKernelGGL(k_cg_model, dim3(nblk(nm)), dim3(256), 0, s, nm, (float)alfa, (float)beta, p->gvec, p->S,
+                               p->x);
+            hipLaunchKernelGGL(k_cg_data, dim3(nblk(nd)), dim3(256), 0, s, nd, (float)alfa, (float)beta, p->gg, p->Ss,
+                               p->rr);
+            LS_HIP(hipGetLastError());
+        }
+        if (rc) break;
+        if (iters_out) iters_out[is] = iter;
+        LS_HIP(hipMemcpyAsync(src + (size_t)is * nm, p->x, (size_t)nm * 4, hipMemcpyDeviceToHost, s));
+        LS_HIP(hipStreamSynchronize(s));
+    }
+    LS_HIP(hipEventRecord(e1, s));
+    LS_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    hipEventElapsedTime(&ms, e0, e1);
+    p->t_ms = ms;
+    hipEventDestroy(e0); hipEventDestroy(e1);
+    return rc;
+}
+
+// elapsed milliseconds of the last model / invert call (device work; invert includes its transfers)
+int pw_lstri_timing(pw_lstri* p, double* ms)
+{
+    if (!p || !ms) return PW_ERR_ARG;
+    *ms = p->t_ms;
+    return PW_OK;
+}
+
+// pfwicfun.lstric (pywave/src/pfwi.c:1031-1505): pararray[9] (inv) != 0 -> in = data (nt*nx*ns),
+// out = source field (nz*nx*nt*ns) after niter = pararray[19] CG iterations per shot; else in =
+// source field, out = data.  q and wav are accepted and unused, as in the reference.
+int pw_lstric(pw_ctx* ctx, const float* vel, const float* q, const float* wav, const float* datasrc,
+              const float* pararray, float* out, long long out_len, long long in_len)
+{
+    (void)q; (void)wav;
+    if (!ctx || !vel || !datasrc || !pararray || !out) return PW_ERR_ARG;
+    const long long nz = (long long)pararray[0], nx = (long long)pararray[1], nt = (long long)pararray[6],
+                    ns = (long long)pararray[10];
+    const bool inv = pararray[9] != 0.f;
+    const long long nd = nt * nx * ns, nm = nz * nx * nt * ns;
+    if (in_len != (inv ? nd : nm)) PW_FAIL(ctx, PW_ERR_ARG, "lstric: input length");
+    if (out_len != (inv ? nm : nd)) PW_FAIL(ctx, PW_ERR_ARG, "lstric: output length");
+    pw_lstri* p = pw_lstri_create(ctx, pararray);
+    if (!p) return PW_ERR_ARG;
+    int rc = pw_lstri_set_model(p, vel);
+    if (!rc) rc = inv ? pw_lstri_invert(p, datasrc, (int)pararray[19], out, nullptr) : pw_lstri_model(p, datasrc, out);
+    pw_lstri_destroy(p);
+    return rc;
+}
+
+}  // extern "C"

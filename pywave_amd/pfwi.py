@@ -10,8 +10,12 @@ mode 4  inv and not onlysrc: passive FWI for velocity with the known source fiel
         onlyvel returns (data flattened in Fortran order, vinv (nz, nx, niter), None, None, None);
         without onlyvel the reference's return reaches unassigned grad/mwt (pywave/pfwi.py:205-209)
         and raises UnboundLocalError, raised here before the inversion runs
-        other mode-4 branches (passive modelling and source inversion by least-squares TRI,
-        pfwicfun.lstric): NotImplementedError
+        inv and onlysrc: least-squares time-reversal source inversion (pfwicfun.lstric,
+        pywave/src/pfwi.c:1031-1505, through pywave/pfwi.py:122-165): returns
+        ([], None, None, src (nz, nx, nt, ns), [])
+        not inv: passive modelling of the source field src (lstric, pywave/pfwi.py:212-250):
+        returns (data (nt, nx, ns), [], [], [], [])
+        The branch tests compare with True as the reference does (par['inv'] == True).
 other   prints "Wrong mode" and returns 0
 media != 1 in mode 1 reaches an undefined function in the reference (NameError); the same error
 is raised here.  Units follow the reference demos: km and km/s (the optimiser clips to [0, 10]).
@@ -59,10 +63,11 @@ def pfwi(vel, q, wav, src, data=None, mode=1, media=1, inv=0, verb=1, nb=60, coe
     elif mode == 3:
         raise UnboundLocalError("local variable 'vinv' referenced before assignment")
     elif mode == 4:
-        if not (par["inv"] and not par["onlysrc"]):
-            raise NotImplementedError("pfwi mode 4 passive modelling / source inversion (lstric) is not provided "
-                                      "by this engine")
-        if not par["onlyvel"]:
+        if par["inv"] != True:  # noqa: E712 (the reference's test)
+            return _lstric_modelling(vel, q, wav, src, par)
+        if par["onlysrc"] == True:  # noqa: E712
+            return _lstric_inversion(vel, q, wav, data, par)
+        if par["onlyvel"] != True:  # noqa: E712
             raise UnboundLocalError("local variable 'grad' referenced before assignment")
         vel = vel.flatten(order="F").astype(np.float32)
         q = q.flatten(order="F").astype(np.float32)
@@ -83,6 +88,33 @@ def pfwi(vel, q, wav, src, data=None, mode=1, media=1, inv=0, verb=1, nb=60, coe
         print("Wrong mode")
         return 0
     return data, vinv, grad, src, mwt
+
+
+def _lstric(vel, q, wav, datasrc, par, out_len):
+    vel = vel.flatten(order="F").astype(np.float32)
+    q = q.flatten(order="F").astype(np.float32)
+    wav = wav.flatten(order="F").astype(np.float32)
+    datasrc = np.ascontiguousarray(datasrc.flatten(order="F").astype(np.float32))
+    pararray = make_pararray(par)
+    out = np.empty(out_len, np.float32)
+    L, ctx = N.lib(), N.context()
+    N.check(L.pw_lstric(ctx, N.ptr(vel), N.ptr(q), N.ptr(wav), N.ptr(datasrc), N.ptr(pararray), N.ptr(out),
+                        out.size, datasrc.size), ctx, "lstric")
+    return out
+
+
+def _lstric_modelling(vel, q, wav, src, par):
+    nt, nx, ns = int(par["nt"]), int(par["nx"]), int(par["ns"])
+    out = _lstric(vel, q, wav, src, par, nt * nx * ns)
+    return out.reshape(par["nt"], par["nx"], par["ns"], order="F"), [], [], [], []
+
+
+def _lstric_inversion(vel, q, wav, data, par):
+    par["inv"] = 1
+    nz, nx, nt, ns = int(par["nz"]), int(par["nx"]), int(par["nt"]), int(par["ns"])
+    out = _lstric(vel, q, wav, data, par, nz * nx * nt * ns)
+    src = out.reshape(par["nz"], par["nx"], par["nt"], par["ns"], order="F")
+    return [], None, None, src, []
 
 
 def fillpar(par):

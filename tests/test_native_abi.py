@@ -101,10 +101,6 @@ def test_pfwi_mode4_branches_without_device():
     v = np.full((8, 8), 2.0, np.float32)
     w = np.zeros(10, np.float32)
     src = np.zeros((8, 8, 10, 1), np.float32)
-    with pytest.raises(NotImplementedError):  # passive modelling (lstric, inv=0)
-        pywave_amd.pfwi(v, v, w, src, mode=4, par=dict(par, inv=False))
-    with pytest.raises(NotImplementedError):  # source-only inversion (lstric, onlysrc)
-        pywave_amd.pfwi(v, v, w, src, data=np.zeros((10, 8, 1)), mode=4, par=dict(par, inv=True, onlysrc=True))
     with pytest.raises(UnboundLocalError):  # joint inversion: the reference's return fails
         pywave_amd.pfwi(v, v, w, src, data=np.zeros((10, 8, 1)), mode=4,
                         par=dict(par, inv=True, onlysrc=False, onlyvel=False))
