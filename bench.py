@@ -198,6 +198,72 @@ def afd2d_small_bench():
     return out
 
 
+def lstri_bench():
+    """Passive modelling and least-squares time-reversal source inversion (pfwi mode 4 lstric) on
+    the geometry of the reference's passive demos: 130x334 (+60-cell decay: 250x454), nt=3001,
+    3 events.  Modelling advances the 3 shots in one launch per step; the inversion runs 30 CG
+    iterations per shot (an adjoint and a modelling sweep each, replayed from hipGraphs), shots
+    one after the other as the reference's carried CG state requires.  Bytes per step and shot:
+    16 B per padded cell (two levels and v^2dt^2 read, one level written) + 4 B per model cell
+    (source plane read, or image plane written) + 4 B per receiver."""
+    from pywave_amd import LstriProblem
+    from pywave_amd.workloads import lstri2d
+
+    w = lstri2d()
+    par = w["par"]
+    nz, nx, nt, ns, nb, niter = par["nz"], par["nx"], par["nt"], par["ns"], par["nb"], par["niter"]
+    cells = (nz + 2 * nb) * (nx + 2 * nb)
+    step_bytes = 16.0 * cells + 4.0 * nz * nx + 4.0 * nx
+    p = LstriProblem(par, w["vel"])
+    p.model(w["src"])
+    data = p.model(w["src"])
+    model_ms = p.timing()
+    model_step_us = model_ms * 1e3 / nt
+    p.invert(data, niter=1)  # captures the sweeps' hipGraphs
+    t = time.perf_counter()
+    src_inv, its = p.invert(data, return_iters=True)
+    inv_s = time.perf_counter() - t
+    inv_dev_ms = p.timing()
+    fit = p.model(src_inv)
+    resid = float(np.linalg.norm(fit - data) / np.linalg.norm(data))
+    n_sweep_steps = 2 * nt * int(np.sum(its))
+    out = {"workload": "lstri2d 130x334 (+60 decay: 250x454), nt=3001, 3 events x 6 buried sources, "
+                       "30 CG iterations per shot",
+           "model_ms": round(model_ms, 3), "model_step_us": round(model_step_us, 3),
+           "model_achieved_GBs": round(ns * step_bytes / (model_step_us * 1e-6) / 1e9, 1),
+           "invert_s": round(inv_s, 3), "invert_device_ms": round(inv_dev_ms, 1),
+           "invert_ms_per_cg_iteration": round(inv_dev_ms / max(int(np.sum(its)), 1), 2),
+           "invert_us_per_sweep_step": round(inv_dev_ms * 1e3 / max(n_sweep_steps, 1), 3),
+           "cg_iterations": [int(x) for x in its], "relative_residual": round(resid, 4)}
+    p.close()
+    try:
+        from oracle import refrun
+        if refrun.available():
+            pr = np.array([par[k] for k in ["nz", "nx", "dz", "dx", "oz", "ox", "nt", "dt", "ot", "inv", "ns", "ds",
+                                            "os", "sz", "nb", "coef", "f0", "acqui_type", "interval", "niter"]],
+                          np.float32)
+            pr[10] = 1  # one shot
+            q = np.full((nz, nx), 1e4, np.float32)
+            wav = np.zeros(nt, np.float32)
+            _, el_mod = refrun.lstri(pr, w["vel"], q, wav, w["src"][..., :1])
+            pr_i = pr.copy()
+            pr_i[9], pr_i[19] = 1, 1
+            _, el_it = refrun.lstri(pr_i, w["vel"], q, wav, data[..., :1])
+            cpu_inv = el_it * niter * ns
+            out["cpu_baseline"] = dict(
+                value=round(cpu_inv, 1), unit="s per inversion (3 shots x 30 CG iterations)", cores=1,
+                kind="reference",
+                sample="reference C lstri: one shot's modelling (%.2f s) and one CG iteration of one shot "
+                       "(%.2f s, adjoint + modelling sweep + CG step), extrapolated x30 iterations x3 shots; "
+                       "single thread, -O2 -fwrapv" % (el_mod, el_it),
+                model_s_per_shot=round(el_mod, 3))
+            out["invert_speedup_vs_cpu"] = round(cpu_inv / inv_s, 1)
+    except Exception as e:
+        out["cpu_baseline"] = dict(value=None, unit="s per inversion", cores=1, kind="reference",
+                                   sample="failed: %s" % e)
+    return out
+
+
 def tri_bench():
     """Time-reversal imaging on the 128^3 workload (+30-cell ABC: 188^3 updated, nt=2000): the
     forward modelling of three sources, then the four interleaved receiver groups back-propagated
@@ -310,10 +376,11 @@ def main():
     ap.add_argument("--no-tri", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-small", action="store_true")
+    ap.add_argument("--no-lstri", action="store_true")
     ap.add_argument("--fwi-plain-shots", type=int, default=4)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (HBM counter calibration)")
-    ap.add_argument("--only", choices=["fwi", "tri", "batch", "small"],
+    ap.add_argument("--only", choices=["fwi", "tri", "batch", "small", "lstri"],
                     help="run one secondary workload alone and print its JSON object (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -326,7 +393,7 @@ def main():
         return
     if args.only:
         leg = {"fwi": lambda: fwi_bench(args.fwi_plain_shots), "tri": tri_bench, "batch": batch_bench,
-               "small": afd2d_small_bench}[args.only]
+               "small": afd2d_small_bench, "lstri": lstri_bench}[args.only]
         print(json.dumps({args.only: leg()}))
         return
 
@@ -382,6 +449,7 @@ def main():
     tri = None if args.no_tri else tri_bench()
     batch = None if args.no_batch else batch_bench()
     small = None if args.no_small else afd2d_small_bench()
+    lstri = None if args.no_lstri else lstri_bench()
     line = {
         "metric": "Gcell-updates/s 3D FP32 FD (% HBM roofline); FWI grad shots/s batched vs plain",
         "value": round(value, 3),
@@ -419,6 +487,7 @@ def main():
         "tri3d": tri,
         "batch2d": batch,
         "afd2d_small": small,
+        "lstri2d": lstri,
     }
     print(json.dumps(line))
 

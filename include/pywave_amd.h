@@ -180,6 +180,9 @@ int pw_lstri_image(pw_lstri *p, const float *data, float *img);
  * the next as the reference's np_cgstep state does (restart at the first shot of the call).
  * iters_out: ns ints or NULL (iterations run per shot). */
 int pw_lstri_invert(pw_lstri *p, const float *data, int niter, float *src, int *iters_out);
+/* on != 0 (default): the inversion replays its two sweeps from hipGraphs captured on the first
+ * call; 0 launches them step by step.  Results are identical.  No counterpart in the reference. */
+int pw_lstri_use_graph(pw_lstri *p, int on);
 /* device milliseconds of the last pw_lstri_model / pw_lstri_invert call */
 int pw_lstri_timing(pw_lstri *p, double *ms);
 /* pfwicfun.lstric (called by pywave/pfwi.py:160 and :247): pararray[9] (inv) != 0 -> datasrc =

@@ -382,6 +382,10 @@ struct pw_lstri {
     double* sums = nullptr;
     bool have_model = false;
     double t_ms = 0;
+    // the CG's two sweeps (rr -> g adjoint, g -> gg modelling) captured once and replayed
+    bool use_graph = true;
+    hipGraph_t g_adj = nullptr, g_fwd = nullptr;
+    hipGraphExec_t x_adj = nullptr, x_fwd = nullptr;
 };
 
 namespace {
@@ -412,9 +416,20 @@ int zero(pw_lstri* p, float* ptr, long long n)
     return PW_OK;
 }
 
+void drop_graphs(pw_lstri* p)
+{
+    if (p->x_adj) hipGraphExecDestroy(p->x_adj);
+    if (p->x_fwd) hipGraphExecDestroy(p->x_fwd);
+    if (p->g_adj) hipGraphDestroy(p->g_adj);
+    if (p->g_fwd) hipGraphDestroy(p->g_fwd);
+    p->x_adj = p->x_fwd = nullptr;
+    p->g_adj = p->g_fwd = nullptr;
+}
+
 int ensure_fields(pw_lstri* p, int slots)
 {
     if (slots <= p->cap_slots) return PW_OK;
+    drop_graphs(p);  // captured sweeps refer to the old field buffers
     if (p->fields) hipFree(p->fields);
     p->fields = nullptr;
     p->cap_slots = 0;
@@ -548,6 +563,7 @@ int reduce(pw_lstri* p, int k, double* out_host)
 
 void ls_free(pw_lstri* p)
 {
+    drop_graphs(p);
     float* bufs[] = {p->v2, p->sfz, p->sfx, p->fields, p->src, p->dat, p->stage, p->x, p->gvec, p->S, p->rr, p->gg, p->Ss};
     for (float* b : bufs)
         if (b) hipFree(b);
@@ -736,8 +752,23 @@ int pw_lstri_invert(pw_lstri* p, const float* data, int niter, float* src, int* 
         if (!rc) rc = zero(p, p->Ss, p->nd);
         if (rc) return rc;
     }
-    if ((rc = ensure_batch(p, 1))) return rc;
+    if ((rc = ensure_batch(p, 1)) || (rc = ensure_fields(p, 1))) return rc;
     const long long nm = p->nm, nd = p->nd;
+    const bool graph = p->use_graph && s != nullptr;
+    if (graph && !p->x_adj) {
+        LS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        rc = sweep_adjoint(p, 1, p->rr, p->gvec);
+        hipError_t e = hipStreamEndCapture(s, &p->g_adj);
+        if (rc) return rc;
+        LS_HIP(e);
+        LS_HIP(hipGraphInstantiate(&p->x_adj, p->g_adj, nullptr, nullptr, 0));
+        LS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        rc = sweep_model(p, 1, p->gvec, p->gg);
+        e = hipStreamEndCapture(s, &p->g_fwd);
+        if (rc) return rc;
+        LS_HIP(e);
+        LS_HIP(hipGraphInstantiate(&p->x_fwd, p->g_fwd, nullptr, nullptr, 0));
+    }
     bool started = false;
     hipEvent_t e0, e1;
     LS_HIP(hipEventCreate(&e0)); LS_HIP(hipEventCreate(&e1));
@@ -754,8 +785,13 @@ int pw_lstri_invert(pw_lstri* p, const float* data, int niter, float* src, int* 
         double g0 = 1.0;
         int iter = 0;
         for (; iter < niter; ++iter) {
-            if ((rc = sweep_adjoint(p, 1, p->rr, p->gvec))) break;
-            if ((rc = sweep_model(p, 1, p->gvec, p->gg))) break;
+            if (graph) {
+                LS_HIP(hipGraphLaunch(p->x_adj, s));
+                LS_HIP(hipGraphLaunch(p->x_fwd, s));
+            } else {
+                if ((rc = sweep_adjoint(p, 1, p->rr, p->gvec))) break;
+                if ((rc = sweep_model(p, 1, p->gvec, p->gg))) break;
+            }
             // data-space dots (gg.gg, Ss.Ss, gg.Ss, gg.rr, Ss.rr, rr.rr) and |g|^2
             hipLaunchKernelGGL(k_dots_data, dim3(NRED), dim3(256), 0, s, nd, p->gg, p->Ss, p->rr, p->part);
             hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(256), 0, s, 6, p->part, p->sums);
@@ -810,6 +846,14 @@ int pw_lstri_invert(pw_lstri* p, const float* data, int niter, float* src, int* 
     p->t_ms = ms;
     hipEventDestroy(e0); hipEventDestroy(e1);
     return rc;
+}
+
+// replay the CG's sweeps from captured hipGraphs (default) or launch them step by step
+int pw_lstri_use_graph(pw_lstri* p, int on)
+{
+    if (!p) return PW_ERR_ARG;
+    p->use_graph = on != 0;
+    return PW_OK;
 }
 
 // elapsed milliseconds of the last model / invert call (device work; invert includes its transfers)

@@ -86,6 +86,10 @@ class LstriProblem:
         src = out.reshape(self.nz, self.nx, self.nt, self.ns, order="F")
         return (src, its) if return_iters else src
 
+    def use_graph(self, on=True):
+        """Replay the inversion's sweeps from captured hipGraphs (default) or launch step by step."""
+        N.check(self._L.pw_lstri_use_graph(self._p, int(bool(on))), self._ctx, "use_graph")
+
     def timing(self):
         """Device milliseconds of the last model() / invert() call."""
         ms = ctypes.c_double()

@@ -10,6 +10,11 @@
              layered velocities 1500-4500 m/s, 12 Hz surface sources
   fwi2d      400x800 at 10 m (km units), 4 layers 1.5-3.5 km/s plus 6 Gaussian anomalies; start
              model smoothed with a 200 m Gaussian; 32 shots, nt=3000, 8 Hz Ricker at 0.15 s
+  lstri2d    the geometry of the reference's passive demos (demos/test_pfwi_mod2d.py): 130x334 at
+             8 m x 12 m (km units), 60-cell decay boundary, nt=3001, 3 events of 6 buried point
+             sources each (10 Hz Ricker, amplitude 1e6, onsets 0.1-1.1 s at the demo's positions),
+             30 CG iterations; 5 layers 1.5-4.0 km/s plus 5 Gaussian anomalies in place of the
+             demo's Marmousi cut (the model file is external data)
 """
 import numpy as np
 
@@ -117,3 +122,39 @@ def fwi2d(nz=400, nx=800, ns=32, nt=3000, seed=4):
     par = dict(nz=nz, nx=nx, dz=0.01, dx=0.01, oz=0, ox=0, ns=ns, ds=0.24, os=0.2, sz=5, nt=nt, dt=0.001, ot=0,
                nb=60, coef=0.005, acqui_type=1, interval=1, f0=8, inv=0, niter=5)
     return dict(vtrue=vtrue, v0=v0, par=par, f0=8.0, t0=0.15, amp=1e6)
+
+
+def ricker_np(nt, dt, f0, t0, amp=1.0):
+    """Analytic Ricker wavelet (1 - 2 a) exp(-a), a = (pi f0 (t - t0))^2."""
+    t = np.arange(nt, dtype=np.float64) * dt - t0
+    a = (np.pi * f0 * t) ** 2
+    return amp * (1.0 - 2.0 * a) * np.exp(-a)
+
+
+def lstri2d(nz=130, nx=334, nt=3001, seed=5):
+    """Velocity (km/s), source field (nz, nx, nt, 3) and the pfwi parameter set of the passive
+    workload."""
+    rng = np.random.default_rng(seed)
+    v = np.zeros((nz, nx), np.float64)
+    edges = np.linspace(0, nz, 6).astype(int)
+    for k, val in enumerate([1.5, 2.0, 2.6, 3.2, 4.0]):
+        v[edges[k]:edges[k + 1]] = val
+    zz, xx = np.meshgrid(np.arange(nz), np.arange(nx), indexing="ij")
+    for _ in range(5):
+        cz, cx = rng.uniform(0.2 * nz, 0.9 * nz), rng.uniform(0.1 * nx, 0.9 * nx)
+        sg = rng.uniform(6, 12)
+        a = rng.uniform(-0.06, 0.06)
+        v *= 1 + a * np.exp(-((zz - cz) ** 2 + (xx - cx) ** 2) / (2 * sg * sg))
+    vel = np.clip(v, 1.5, 4.0).astype(np.float32)
+    # the demo's event positions (z, x, onset sample) of demos/test_pfwi_mod2d.py
+    ev = [[(110, 100, 100), (120, 110, 300), (117, 165, 500), (120, 205, 700), (112, 220, 900), (115, 260, 1100)],
+          [(115, 95, 100), (111, 120, 300), (120, 170, 500), (122, 200, 700), (118, 230, 900), (115, 255, 1100)],
+          [(112, 90, 100), (118, 110, 300), (111, 145, 500), (120, 180, 700), (116, 200, 900), (110, 240, 1100)]]
+    dt = 0.001
+    src = np.zeros((nz, nx, nt, len(ev)), np.float32)
+    for e, pts in enumerate(ev):
+        for iz, ix, it in pts:
+            src[iz, ix, :, e] += ricker_np(nt, dt, 10.0, it * dt, 1e6).astype(np.float32)
+    par = dict(nz=nz, nx=nx, dz=0.008, dx=0.012, oz=0, ox=0, ns=len(ev), ds=0.2, os=0.0, sz=5, nt=nt, dt=dt, ot=0,
+               nb=60, coef=0.005, acqui_type=1, interval=1, f0=10, inv=0, niter=30)
+    return dict(vel=vel, src=src, par=par)

@@ -81,3 +81,18 @@ def test_pfwi_mode4_lstric_dropins(ls):
     assert d2 == [] and vinv is None and grad is None and mwt == []
     assert src.shape == (16, 24, 120, 2)
     assert rel_l2(src, ls["src_inv"]) <= 1e-5
+
+
+def test_lstri_graph_replay_equals_step_launches(ls):
+    # the inversion's sweeps replayed from captured hipGraphs (with the CG's eager work and host
+    # synchronisations between replays) give the step-by-step launches' result bit for bit, also
+    # on a second call and after a modelling call has re-sized the field buffers
+    p = LstriProblem(ls["par_inv"], ls["vel"])
+    p.use_graph(False)
+    eager = p.invert(ls["data"])
+    q = LstriProblem(ls["par_inv"], ls["vel"])
+    g1 = q.invert(ls["data"])
+    g2 = q.invert(ls["data"])
+    assert np.array_equal(g1, eager) and np.array_equal(g2, eager)
+    q.model(ls["src"])  # two shots in one launch: larger field buffers, captured sweeps dropped
+    assert np.array_equal(q.invert(ls["data"]), eager)
