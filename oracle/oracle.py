@@ -71,6 +71,10 @@ def lib():
                                 ctypes.c_int, ctypes.c_float, ctypes.c_int, _i32p, _i32p, _i32p,
                                 _f32p, ctypes.c_int, _f32p, ctypes.c_int, ctypes.c_void_p,
                                 ctypes.c_void_p]
+        L.orc_aps2d.argtypes = L.orc_afd2d.argtypes
+        L.orc_aps3d.argtypes = L.orc_afd3d.argtypes
+        L.orc_cfft.argtypes = [ctypes.c_int, ctypes.c_int, _f32p, _f32p]
+        L.orc_rfft.argtypes = [ctypes.c_int, ctypes.c_int, _f32p, _f32p]
         L.orc_fwi_prepare.argtypes = [ctypes.POINTER(OrcAcq)]
         L.orc_fwi_release.argtypes = [ctypes.POINTER(OrcAcq)]
         L.orc_fwi_forward.argtypes = [ctypes.POINTER(OrcAcq), _f32p, _f32p, _f32p]
@@ -141,9 +145,9 @@ def ricker_bank(nt, dt, f, t, A):
 
 # ------------------------------------------------------------------ afd engine
 
-def afd2d(vel, nt, dt, dx, dz, nbt, ct, sx, sz, rick, tri=False, dat=None, jsnap=0):
-    """Oracle afd2d.  vel (nz,nx); rick (ns,nt).  Returns (traces[nx,nt], snaps, img) in the
-    reference's internal layouts: traces [ix][it], snaps [isnap][ix][iz], img [ix][iz]."""
+def afd2d(vel, nt, dt, dx, dz, nbt, ct, sx, sz, rick, tri=False, dat=None, jsnap=0, ps=False):
+    """Oracle afd2d (ps: aps2d).  vel (nz,nx); rick (ns,nt).  Returns (traces[nx,nt], snaps, img) in
+    the reference's internal layouts: traces [ix][it], snaps [isnap][ix][iz], img [ix][iz]."""
     vel = np.asfortranarray(vel, dtype=np.float32)
     nz1, nx1 = vel.shape
     velf = np.ascontiguousarray(vel.ravel(order="F"))
@@ -155,7 +159,8 @@ def afd2d(vel, nt, dt, dx, dz, nbt, ct, sx, sz, rick, tri=False, dat=None, jsnap
     nsnap = count_snaps(nt, jsnap)
     snaps = np.zeros((nsnap, nx1, nz1), np.float32) if jsnap > 0 else None
     img = np.zeros((nx1, nz1), np.float32) if tri else None
-    lib().orc_afd2d(velf, nz1, nx1, nt, dt, dx, dz, nbt, ct, ns if not tri else 0,
+    fn = lib().orc_aps2d if ps else lib().orc_afd2d
+    fn(velf, nz1, nx1, nt, dt, dx, dz, nbt, ct, ns if not tri else 0,
                     spx if ns else np.zeros(1, np.int32), spz if ns else np.zeros(1, np.int32),
                     rick, int(bool(tri)), d, int(jsnap),
                     snaps.ctypes.data if snaps is not None else None,
@@ -163,8 +168,8 @@ def afd2d(vel, nt, dt, dx, dz, nbt, ct, sx, sz, rick, tri=False, dat=None, jsnap
     return d, snaps, img
 
 
-def afd3d(vel, nt, dt, dx, dy, dz, nbt, ct, sx, sy, sz, rick, tri=False, dat=None, jsnap=0):
-    """Oracle afd3d.  vel (nz,nx,ny).  traces [iy][ix][it], snaps [isnap][iy][ix][iz]."""
+def afd3d(vel, nt, dt, dx, dy, dz, nbt, ct, sx, sy, sz, rick, tri=False, dat=None, jsnap=0, ps=False):
+    """Oracle afd3d (ps: aps3d).  vel (nz,nx,ny).  traces [iy][ix][it], snaps [isnap][iy][ix][iz]."""
     vel = np.asarray(vel, np.float32)
     nz1, nx1, ny1 = vel.shape
     velf = np.ascontiguousarray(vel.ravel(order="F"))
@@ -178,7 +183,8 @@ def afd3d(vel, nt, dt, dx, dy, dz, nbt, ct, sx, sy, sz, rick, tri=False, dat=Non
     snaps = np.zeros((nsnap, ny1, nx1, nz1), np.float32) if jsnap > 0 else None
     img = np.zeros((ny1, nx1, nz1), np.float32) if tri else None
     z1 = np.zeros(1, np.int32)
-    lib().orc_afd3d(velf, nz1, nx1, ny1, nt, dt, dx, dy, dz, nbt, ct, ns if not tri else 0,
+    fn = lib().orc_aps3d if ps else lib().orc_afd3d
+    fn(velf, nz1, nx1, ny1, nt, dt, dx, dy, dz, nbt, ct, ns if not tri else 0,
                     spx if ns else z1, spy if ns else z1, spz if ns else z1, rick,
                     int(bool(tri)), d, int(jsnap),
                     snaps.ctypes.data if snaps is not None else None,
@@ -313,3 +319,27 @@ def lstri(pararray, vel, arr):
     if t["inv"]:
         return src.reshape(t["nz"], t["nx"], t["nt"], t["ns"], order="F")
     return dat.reshape(t["nt"], t["nx"], t["ns"], order="F")
+
+
+def cfft(x, inverse=False):
+    """Complex transform (unscaled) of a complex64 vector, the KISS algorithm in f32."""
+    x = np.ascontiguousarray(np.asarray(x, np.complex64))
+    out = np.zeros_like(x)
+    lib().orc_cfft(x.size, int(bool(inverse)), x.view(np.float32), out.view(np.float32))
+    return out
+
+
+def rfft(x):
+    """Real transform of an even-length float32 vector: n/2+1 bins (complex64)."""
+    x = np.ascontiguousarray(np.asarray(x, np.float32))
+    out = np.zeros(x.size // 2 + 1, np.complex64)
+    lib().orc_rfft(x.size, 0, x, out.view(np.float32))
+    return out
+
+
+def irfft(X, n):
+    """Inverse real transform (unscaled) of n/2+1 bins to n reals."""
+    X = np.ascontiguousarray(np.asarray(X, np.complex64))
+    out = np.zeros(n, np.float32)
+    lib().orc_rfft(n, 1, X.view(np.float32), out)
+    return out

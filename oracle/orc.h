@@ -42,6 +42,31 @@ int orc_afd3d(const float *vel, int nz1, int nx1, int ny1, int nt, float dt, flo
               const int *spz, const float *rick, int tri, float *dat, int jsnap, float *snaps,
               float *img);
 
+/* aps2d / aps3d: the same set-up and time loop with the pseudo-spectral Laplacian of
+ * psm2d / psm (pywave/src/wave_psp.c:367-635, :25-365); arguments as orc_afd2d / orc_afd3d. */
+int orc_aps2d(const float *vel, int nz1, int nx1, int nt, float dt, float dx, float dz,
+              int nbt, float ct, int ns, const int *spx, const int *spz, const float *rick,
+              int tri, float *dat, int jsnap, float *snaps, float *img);
+int orc_aps3d(const float *vel, int nz1, int nx1, int ny1, int nt, float dt, float dx,
+              float dy, float dz, int nbt, float ct, int ns, const int *spx, const int *spy,
+              const int *spz, const float *rick, int tri, float *dat, int jsnap, float *snaps,
+              float *img);
+
+/* ---- transforms (pywave/src/wave_kissfft.c, wave_fft2.c, wave_fft3.c) ---- */
+
+/* complex transform of n points (interleaved re, im); inverse = 1 for the positive exponent
+ * (unscaled) */
+int orc_cfft(int n, int inverse, const float *in, float *out);
+/* real transform of even n: forward n reals -> n/2+1 bins; inverse n/2+1 bins -> n reals
+ * (unscaled) */
+int orc_rfft(int n, int inverse, const float *in, float *out);
+
+/* the spectral Laplacian of a padded [n3][n2][n1] field (n3 = 1 in 2D) */
+typedef struct orc_psl orc_psl;
+orc_psl *orc_psl_new(int n1, int n2, int n3, float dz, float dx, float dy);
+void orc_psl_free(orc_psl *P);
+void orc_psl_apply(orc_psl *P, const float *in, float *out);
+
 /* ---- FWI discretisation (pywave/src/wave_fwiutil.c, wave_fwimodeling.c,
  *      wave_fwigradient.c, wave_fwi.c, wave_fwilbfgs.c) ---- */
 

@@ -79,11 +79,13 @@ static void damp3(float *a, int nz2, int nx2, int ny2, int nb, int nby, const fl
             for (int iz = 0; iz < nz2; iz++) a[pl * (ny2 - 1 - iy) + (size_t)nz2 * ix + iz] *= w[iy];
 }
 
-/* shared 2D/3D engine; 2D is ny1 == 1 with no y boundary */
+/* shared 2D/3D engine; 2D is ny1 == 1 with no y boundary.  ps != 0: the Laplacian is the
+ * pseudo-spectral one of psm/psm2d (pywave/src/wave_psp.c:25-365, 367-635) over the whole padded
+ * grid; everything else (set-up, leapfrog, injection, boundary, receivers) is shared. */
 static int afd_engine(const float *vel, int nz1, int nx1, int ny1, int is3d, int nt, float dt,
                       float dx, float dy, float dz, int nbt, float ct, int ns, const int *spx,
                       const int *spy, const int *spz, const float *rick, int tri, float *dat,
-                      int jsnap, float *snaps, float *img)
+                      int jsnap, float *snaps, float *img, int ps)
 {
     int nby = is3d ? nbt : 0;
     int nz = nz1 + 2 * nbt, nx = nx1 + 2 * nbt, ny = ny1 + 2 * nby;
@@ -109,13 +111,16 @@ static int afd_engine(const float *vel, int nz1, int nx1, int ny1, int is3d, int
                 vv[(size_t)nz * nx * iy + (size_t)nz * ix + iz] = v * v * dt * dt;
             }
 
+    orc_psl *P = ps ? orc_psl_new(nz2, nx2, ny2, dz, dx, dy) : NULL;
     float idz = 1 / (dz * dz), idx = 1 / (dx * dx), idy = 1 / (dy * dy);
     float cc3 = idy + idx + idz, cc2 = idx + idz;
     int it0 = tri ? nt - 1 : 0, it1 = tri ? -1 : nt, its = tri ? -1 : 1;
 
     for (int it = it0; it != it1; it += its) {
-        /* Laplacian on [2,n2-2) per axis */
-        if (is3d) {
+        /* Laplacian on [2,n2-2) per axis (finite differences) or spectral on the padded grid */
+        if (ps) {
+            orc_psl_apply(P, curr, lap);
+        } else if (is3d) {
             for (int iy = 2; iy < ny2 - 2; iy++)
                 for (int ix = 2; ix < nx2 - 2; ix++)
                     for (int iz = 2; iz < nz2 - 2; iz++) {
@@ -209,6 +214,7 @@ static int afd_engine(const float *vel, int nz1, int nx1, int ny1, int is3d, int
                 }
     }
     free(vv); free(curr); free(prev); free(lap); free(w);
+    orc_psl_free(P);
     return 0;
 }
 
@@ -217,7 +223,15 @@ int orc_afd2d(const float *vel, int nz1, int nx1, int nt, float dt, float dx, fl
               int tri, float *dat, int jsnap, float *snaps, float *img)
 {
     return afd_engine(vel, nz1, nx1, 1, 0, nt, dt, dx, 1.0f, dz, nbt, ct, ns, spx, NULL, spz, rick,
-                      tri, dat, jsnap, snaps, img);
+                      tri, dat, jsnap, snaps, img, 0);
+}
+
+int orc_aps2d(const float *vel, int nz1, int nx1, int nt, float dt, float dx, float dz,
+              int nbt, float ct, int ns, const int *spx, const int *spz, const float *rick,
+              int tri, float *dat, int jsnap, float *snaps, float *img)
+{
+    return afd_engine(vel, nz1, nx1, 1, 0, nt, dt, dx, 1.0f, dz, nbt, ct, ns, spx, NULL, spz, rick,
+                      tri, dat, jsnap, snaps, img, 1);
 }
 
 int orc_afd3d(const float *vel, int nz1, int nx1, int ny1, int nt, float dt, float dx,
@@ -226,5 +240,14 @@ int orc_afd3d(const float *vel, int nz1, int nx1, int ny1, int nt, float dt, flo
               float *img)
 {
     return afd_engine(vel, nz1, nx1, ny1, 1, nt, dt, dx, dy, dz, nbt, ct, ns, spx, spy, spz, rick,
-                      tri, dat, jsnap, snaps, img);
+                      tri, dat, jsnap, snaps, img, 0);
+}
+
+int orc_aps3d(const float *vel, int nz1, int nx1, int ny1, int nt, float dt, float dx,
+              float dy, float dz, int nbt, float ct, int ns, const int *spx, const int *spy,
+              const int *spz, const float *rick, int tri, float *dat, int jsnap, float *snaps,
+              float *img)
+{
+    return afd_engine(vel, nz1, nx1, ny1, 1, nt, dt, dx, dy, dz, nbt, ct, ns, spx, spy, spz, rick,
+                      tri, dat, jsnap, snaps, img, 1);
 }

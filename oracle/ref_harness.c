@@ -21,6 +21,8 @@
  *   lstri : pywave/src/pfwi.c:1031-1505 -> lstri (wave_fwi.c:121) -> lstri_op
  *           (wave_fwigradient.c:454): passive modelling (inv = 0) or least-squares
  *           time-reversal source inversion (inv = 1, CG with np_cgstep)
+ *   aps2d : pywave/src/aps.c:330-560 (the afd2d set-up) -> psm2d (pywave/src/wave_psp.c:367)
+ *   aps3d : pywave/src/aps.c:23-329 (the afd3d set-up)  -> psm   (pywave/src/wave_psp.c:25)
  *   ricker: the wavelet construction inside fdm2d (wave_fdm.c:502-516)
  *
  * All arrays are raw little-endian float32 files.  Uninitialised heap memory is
@@ -86,7 +88,10 @@ static int count_snaps(int nt, int jsnap)
     return n;
 }
 
-/* afd2d nz nx nt ns nbt ct dt dx dz jsnap tri vel src out [verb] */
+/* the pseudo-spectral entry points (aps2dc/aps3dc) share the afd set-up and call psm2d/psm */
+static int g_ps = 0;
+
+/* afd2d|aps2d nz nx nt ns nbt ct dt dx dz jsnap tri vel src out [verb] */
 static int run_afd2d(int argc, char **argv)
 {
     if (argc < 15) { fprintf(stderr, "afd2d: bad args\n"); return 2; }
@@ -142,7 +147,8 @@ static int run_afd2d(int argc, char **argv)
     par->verb = verb; par->ps = 1; par->vref = 1500;
 
     double t_start = now_s();
-    fdm2d(wvfld, dat, NULL, img, vel2, par, tri);
+    if (g_ps) psm2d(wvfld, dat, NULL, img, vel2, par, tri);
+    else fdm2d(wvfld, dat, NULL, img, vel2, par, tri);
     double t_el = now_s() - t_start;
     fprintf(stderr, "PWREF_ELAPSED_S=%.6f\n", t_el);
 
@@ -154,7 +160,7 @@ static int run_afd2d(int argc, char **argv)
     return 0;
 }
 
-/* afd3d nz nx ny nt ns nbt ct dt dx dy dz jsnap tri vel src out [verb] */
+/* afd3d|aps3d nz nx ny nt ns nbt ct dt dx dy dz jsnap tri vel src out [verb] */
 static int run_afd3d(int argc, char **argv)
 {
     if (argc < 18) { fprintf(stderr, "afd3d: bad args\n"); return 2; }
@@ -214,7 +220,8 @@ static int run_afd3d(int argc, char **argv)
     par->verb = verb; par->ps = 1; par->vref = 1500;
 
     double t_start = now_s();
-    fdm(wvfld, dat, NULL, img, vel2, par, tri);
+    if (g_ps) psm(wvfld, dat, NULL, img, vel2, par, tri);
+    else fdm(wvfld, dat, NULL, img, vel2, par, tri);
     double t_el = now_s() - t_start;
     fprintf(stderr, "PWREF_ELAPSED_S=%.6f\n", t_el);
 
@@ -515,11 +522,13 @@ int main(int argc, char **argv)
     /* every allocation through mmap: fresh, zero-filled pages */
     mallopt(M_MMAP_THRESHOLD, 0);
     mallopt(M_MMAP_MAX, 1 << 30);
-    if (argc < 2) { fprintf(stderr, "usage: pwref {afd2d|afd3d|ricker|fwimod|fwigrad|fwi|pfwigrad|pfwi|lstri} ...\n"); return 2; }
+    if (argc < 2) { fprintf(stderr, "usage: pwref {afd2d|afd3d|aps2d|aps3d|ricker|fwimod|fwigrad|fwi|pfwigrad|pfwi|lstri} ...\n"); return 2; }
     /* the engines print progress on stdout; keep it off the caller's stdout */
     if (!getenv("PWREF_KEEP_STDOUT")) { if (!freopen("/dev/null", "w", stdout)) return 4; }
     if (!strcmp(argv[1], "afd2d")) return run_afd2d(argc, argv);
     if (!strcmp(argv[1], "afd3d")) return run_afd3d(argc, argv);
+    if (!strcmp(argv[1], "aps2d")) { g_ps = 1; return run_afd2d(argc, argv); }
+    if (!strcmp(argv[1], "aps3d")) { g_ps = 1; return run_afd3d(argc, argv); }
     if (!strcmp(argv[1], "ricker")) return run_ricker(argc, argv);
     if (!strcmp(argv[1], "fwimod")) return run_fwimod(argc, argv);
     if (!strcmp(argv[1], "fwigrad")) return run_fwigrad(argc, argv);

@@ -34,9 +34,9 @@ def _put(tmp, name, arr):
     return p
 
 
-def afd2d(vel, nt, dt, dx, dz, nbt, ct, src_pack=None, ns=1, tri=False, dat=None, jsnap=0):
+def afd2d(vel, nt, dt, dx, dz, nbt, ct, src_pack=None, ns=1, tri=False, dat=None, jsnap=0, ps=False):
     """Returns (flat output exactly as afd2dc returns it, elapsed seconds).
-    vel (nz,nx); src_pack = [sx..,sz..,f..,t..,A..]; dat (nt,nx) for tri."""
+    vel (nz,nx); src_pack = [sx..,sz..,f..,t..,A..]; dat (nt,nx) for tri.  ps: aps2dc (psm2d)."""
     vel = np.asarray(vel, np.float32)
     nz, nx = vel.shape
     nsnap = sum(1 for it in range(nt) if jsnap and it % jsnap == 0)
@@ -47,16 +47,16 @@ def afd2d(vel, nt, dt, dx, dz, nbt, ct, src_pack=None, ns=1, tri=False, dat=None
         else:
             fs = _put(tmp, "src.f32", np.asarray(src_pack, np.float32))
         fo = os.path.join(tmp, "out.f32")
-        el = _run(["afd2d", nz, nx, nt, ns, nbt, repr(float(ct)), repr(float(dt)), repr(float(dx)),
+        el = _run(["aps2d" if ps else "afd2d", nz, nx, nt, ns, nbt, repr(float(ct)), repr(float(dt)), repr(float(dx)),
                    repr(float(dz)), jsnap, int(bool(tri)), fv, fs, fo], tmp)
         out = np.fromfile(fo, np.float32)
     assert out.size == nt * nx + (nz * nx * nsnap if jsnap > 0 else 0)
     return out, el
 
 
-def afd3d(vel, nt, dt, dx, dy, dz, nbt, ct, src_pack=None, ns=1, tri=False, dat=None, jsnap=0):
+def afd3d(vel, nt, dt, dx, dy, dz, nbt, ct, src_pack=None, ns=1, tri=False, dat=None, jsnap=0, ps=False):
     """Returns (flat output exactly as afd3dc returns it, elapsed seconds).
-    vel (nz,nx,ny); src_pack = [sx..,sy..,sz..,f..,t..,A..]; dat (nt,nx,ny) for tri."""
+    vel (nz,nx,ny); src_pack = [sx..,sy..,sz..,f..,t..,A..]; dat (nt,nx,ny) for tri.  ps: aps3dc (psm)."""
     vel = np.asarray(vel, np.float32)
     nz, nx, ny = vel.shape
     with tempfile.TemporaryDirectory() as tmp:
@@ -66,7 +66,7 @@ def afd3d(vel, nt, dt, dx, dy, dz, nbt, ct, src_pack=None, ns=1, tri=False, dat=
         else:
             fs = _put(tmp, "src.f32", np.asarray(src_pack, np.float32))
         fo = os.path.join(tmp, "out.f32")
-        el = _run(["afd3d", nz, nx, ny, nt, ns, nbt, repr(float(ct)), repr(float(dt)), repr(float(dx)),
+        el = _run(["aps3d" if ps else "afd3d", nz, nx, ny, nt, ns, nbt, repr(float(ct)), repr(float(dt)), repr(float(dx)),
                    repr(float(dy)), repr(float(dz)), jsnap, int(bool(tri)), fv, fs, fo], tmp)
         out = np.fromfile(fo, np.float32)
     return out, el

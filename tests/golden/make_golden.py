@@ -87,6 +87,37 @@ def afd3d_cases():
          jsnap=30, out=out)
 
 
+def aps_cases():
+    """Pseudo-spectral engine (apscfun: pywave/src/aps.c -> psm2d/psm, pywave/src/wave_psp.c).
+    Padded sizes with radix-2, 3, 4 and 5 factors on every axis."""
+    rng = np.random.default_rng(13)
+    # 46x57 + 20: 86x97 -> padded 90x100 (2*3*3*5 x 2*2*5*5); two sources, snapshots
+    vel = layered2d(46, 57, [1600, 2100, 2700]).astype(np.float32)
+    vel = (vel * (1 + 0.05 * rng.standard_normal(vel.shape))).astype(np.float32)
+    src = np.array([20.4, 40.0, 15.0, 30.6, 22, 16, 0.03, 0.06, 1.0, -1.5], np.float32)
+    nt = 400
+    out, _ = refrun.afd2d(vel, nt, 0.001, 10, 10, 20, 0.02, src, 2, jsnap=50, ps=True)
+    rick = np.stack([refrun.ricker(nt, 0.001, 22, 0.03, 1.0), refrun.ricker(nt, 0.001, 16, 0.06, -1.5)])
+    save("aps2d_two_sources", vel=vel, src=src, nt=nt, dt=0.001, dx=10., dz=10., nbt=20, ct=0.02,
+         jsnap=50, out=out, rick=rick)
+
+    # time reversal through the spectral engine
+    vel = (1700 + 700 * rng.random((40, 50))).astype(np.float32)
+    dat = rng.standard_normal((250, 50)).astype(np.float32)
+    out, _ = refrun.afd2d(vel, 250, 0.001, 10, 10, 20, 0.02, tri=True, dat=dat, jsnap=60, ps=True)
+    save("aps2d_tri", vel=vel, dat=dat, nt=250, dt=0.001, dx=10., dz=10., nbt=20, ct=0.02, jsnap=60,
+         out=out)
+
+    # 3D: 20x22x18 + 8 -> 36x38x34 -> padded 36x40x36
+    vel = (1800 + 600 * rng.random((20, 22, 18))).astype(np.float32)
+    src = np.array([10, 9, 8, 30, 0.03, 1.0], np.float32)
+    nt = 100
+    out, _ = refrun.afd3d(vel, nt, 0.001, 10, 12, 9, 8, 0.03, src, 1, jsnap=40, ps=True)
+    rick = refrun.ricker(nt, 0.001, 30, 0.03, 1.0)[None, :]
+    save("aps3d_small", vel=vel, src=src, nt=nt, dt=0.001, dx=10., dy=12., dz=9., nbt=8, ct=0.03,
+         jsnap=40, out=out, rick=rick)
+
+
 def ricker_cases():
     cases = [(1000, 0.001, 20, 0.1, 1), (1500, 0.001, 15, 0.1, 1), (2000, 0.001, 10, 0.25, 1),
              (3000, 0.001, 8, 0.15, 1e6), (333, 0.002, 12, 0.35, -3), (1501, 0.001, 10, 0.2, 2)]
@@ -201,3 +232,4 @@ if __name__ == "__main__":
     fwi_cases()
     passive_cases()
     lstri_cases()
+    aps_cases()

@@ -337,3 +337,99 @@ def test_lstri_inversion_reduces_residual(golden):
     d_fit = O.lstri(g["par"], g["vel"], g["src_inv"])
     r = np.linalg.norm(d_fit - g["data"]) / np.linalg.norm(g["data"])
     assert r < 0.5
+
+
+# ------------------------------------------------------------------ pseudo-spectral engine
+
+@pytest.mark.parametrize("n", [2, 3, 4, 5, 6, 8, 9, 12, 15, 16, 20, 25, 36, 45, 60, 72, 90, 100, 7, 14])
+def test_cfft_against_numpy(n):
+    # the KISS-algorithm transform agrees with numpy's to f32 round-off; radix 2, 3, 4, 5 and a
+    # generic prime (7) all occur
+    rng = np.random.default_rng(n)
+    x = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    ref = np.fft.fft(x.astype(np.complex128))
+    got = O.cfft(x)
+    assert np.abs(got - ref).max() <= 2e-6 * np.abs(ref).max() * max(1, np.log2(n))
+    back = O.cfft(got, inverse=True)  # unscaled: n x
+    assert np.abs(back / n - x).max() <= 1e-5 * np.abs(x).max()
+    # a planted transposition of one twiddle sign would show here
+    assert not np.allclose(got, np.fft.ifft(x) * n, atol=1e-3) or n <= 2
+
+
+@pytest.mark.parametrize("n", [4, 10, 18, 36, 80, 90, 144])
+def test_rfft_against_numpy(n):
+    rng = np.random.default_rng(n + 1)
+    x = rng.standard_normal(n).astype(np.float32)
+    X = O.rfft(x)
+    ref = np.fft.rfft(x.astype(np.float64))
+    assert np.abs(X - ref).max() <= 2e-6 * np.abs(ref).max() * np.log2(n)
+    assert np.abs(O.irfft(X, n) / n - x).max() <= 1e-5 * np.abs(x).max()
+
+
+def test_spectral_laplacian_of_a_plane_wave():
+    # through the 2D engine's operator: the Laplacian of cos(2 pi (kx x + kz z)) on the periodic
+    # padded grid is -(2 pi)^2 |k|^2 times it (kx, kz on the grid's wavenumbers), which pins the
+    # wavenumber table, the centering and the scaling
+    import ctypes
+    n1, n2, dz, dx = 36, 40, 10.0, 12.0
+    L = O.lib()
+    L.orc_psl_new.restype = ctypes.c_void_p
+    L.orc_psl_new.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float,
+                              ctypes.c_float]
+    L.orc_psl_apply.argtypes = [ctypes.c_void_p, np.ctypeslib.ndpointer(np.float32),
+                                np.ctypeslib.ndpointer(np.float32)]
+    L.orc_psl_free.argtypes = [ctypes.c_void_p]
+    P = L.orc_psl_new(n1, n2, 1, dz, dx, 1.0)
+    z = np.arange(n1) * dz
+    x = np.arange(n2) * dx
+    kz, kx = 3 / (n1 * dz), 5 / (n2 * dx)
+    f = np.cos(2 * np.pi * (kx * x[:, None] + kz * z[None, :])).astype(np.float32)  # [x][z]
+    out = np.zeros_like(f)
+    L.orc_psl_apply(P, np.ascontiguousarray(f), out)
+    L.orc_psl_free(P)
+    expect = -(2 * np.pi) ** 2 * (kx * kx + kz * kz) * f
+    assert np.abs(out - expect).max() <= 1e-4 * np.abs(expect).max()
+
+
+def _aps_src(g, dims):
+    src = g["src"]
+    ns = len(src) // (dims + 3)
+    return [src[k * ns:(k + 1) * ns] for k in range(dims)]
+
+
+def test_aps2d_bit_exact(golden):
+    g = golden("aps2d_two_sources")
+    nt = int(g["nt"])
+    nz, nx = g["vel"].shape
+    sx, sz = _aps_src(g, 2)
+    d, snaps, _ = O.afd2d(g["vel"], nt, float(g["dt"]), float(g["dx"]), float(g["dz"]), int(g["nbt"]),
+                          float(g["ct"]), sx, sz, g["rick"], jsnap=int(g["jsnap"]), ps=True)
+    out = g["out"]
+    assert np.array_equal(out[:nt * nx].reshape(nt, nx, order="F"), d.T)
+    w = out[nt * nx:].reshape(nz, nx, -1, order="F")
+    assert all(np.array_equal(w[..., k], snaps[k].T) for k in range(w.shape[-1]))
+
+
+def test_aps2d_tri_bit_exact(golden):
+    g = golden("aps2d_tri")
+    nt = int(g["nt"])
+    nz, nx = g["vel"].shape
+    _, snaps, img = O.afd2d(g["vel"], nt, float(g["dt"]), float(g["dx"]), float(g["dz"]), int(g["nbt"]),
+                            float(g["ct"]), [], [], None, tri=True, dat=np.ascontiguousarray(g["dat"].T),
+                            jsnap=int(g["jsnap"]), ps=True)
+    w = g["out"][nt * nx:].reshape(nz, nx, -1, order="F")
+    assert all(np.array_equal(w[..., k], snaps[k].T) for k in range(w.shape[-1]))
+    assert np.array_equal(img.T, w[..., 0])
+
+
+def test_aps3d_bit_exact(golden):
+    g = golden("aps3d_small")
+    nt = int(g["nt"])
+    nz, nx, ny = g["vel"].shape
+    sx, sy, sz = _aps_src(g, 3)
+    d, snaps, _ = O.afd3d(g["vel"], nt, float(g["dt"]), float(g["dx"]), float(g["dy"]), float(g["dz"]),
+                          int(g["nbt"]), float(g["ct"]), sx, sy, sz, g["rick"], jsnap=int(g["jsnap"]), ps=True)
+    out = g["out"]
+    assert np.array_equal(out[:nt * nx * ny].reshape(nt, nx, ny, order="F"), d.transpose(2, 1, 0))
+    w = out[nt * nx * ny:].reshape(nz, nx, ny, -1, order="F")
+    assert all(np.array_equal(w[..., k], snaps[k].transpose(2, 1, 0)) for k in range(w.shape[-1]))
