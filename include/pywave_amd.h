@@ -66,6 +66,18 @@ int pw_afd3d(pw_ctx *ctx, const float *vel, const float *src, int tri, int nt, i
              float dt, float ox, float dx, float oy, float dy, float oz, float dz, float *out,
              long long out_len);
 
+/* apscfun.aps2dc / aps3dc (pywave/src/aps.c; pywave/aps2d.py:64, pywave/aps3d.py): the afd
+ * arguments and return with the pseudo-spectral engine psm2d / psm (pywave/src/wave_psp.c:367,
+ * :25), whose Laplacian is IFFT(-(2 pi |k|)^2 FFT(u)) over the padded grid (the reference's KISS
+ * FFT arithmetic reproduced on the device). */
+int pw_aps2d(pw_ctx *ctx, const float *vel, const float *src, int tri, int nt, int nx, int nz,
+             int ns, int verb, int jsnap, int ifsnaps, int abc, int nbt, float ct, float dt,
+             float ox, float dx, float oz, float dz, float *out, long long out_len);
+int pw_aps3d(pw_ctx *ctx, const float *vel, const float *src, int tri, int nt, int nx, int ny,
+             int nz, int ns, int verb, int jsnap, int ifsnaps, int abc, int nbt, float ct,
+             float dt, float ox, float dx, float oy, float dy, float oz, float dz, float *out,
+             long long out_len);
+
 /* length of the afd return vector: nt*nx*ny + nz*nx*ny*ntsnap (ny = 1 in 2D) */
 long long pw_afd_output_len(int nt, int nz, int nx, int ny, int jsnap);
 
@@ -91,6 +103,9 @@ int pw_afd_output(pw_afd *p, float *out, long long out_len);
 int pw_afd_final_field(pw_afd *p, float *out, long long out_len);
 /* Kernel tile choice for the 3D step (0 = default); returns PW_ERR_ARG for unknown ids. */
 int pw_afd_set_variant(pw_afd *p, int variant, int ny_chunk);
+/* on != 0: the plan's time loop uses the pseudo-spectral Laplacian of psm2d/psm
+ * (pywave/src/wave_psp.c) instead of the finite-difference stencil (set before running) */
+int pw_afd_set_spectral(pw_afd *p, int on);
 int pw_afd_variant_count(void);
 /* info[4] = {rows per block (step-1 rows for two-step kernels), waves per block, kind (0 general,
  * 1 lean, 2 pipelined, 3 pipelined non-temporal, 4 pipelined x-major, 5 two steps per launch),

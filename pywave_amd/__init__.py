@@ -2,6 +2,7 @@
 
 Drop-ins for the reference's hot path (github.com/chenyk1990/pywave):
   afd2d, afd3d   2D/3D acoustic FD modelling and time-reversal imaging (pywave/afd2d.py, afd3d.py)
+  aps2d, aps3d   the same with the pseudo-spectral Laplacian (pywave/aps2d.py, aps3d.py)
   pfwi, fillpar  FWI forward modelling (mode 1), L-BFGS FWI (mode 2), passive modelling, LS-TRI
                  source inversion and passive velocity FWI (mode 4) (pywave/pfwi.py)
 Extensions beyond the reference's Python surface:
@@ -13,6 +14,7 @@ Extensions beyond the reference's Python surface:
 All numerics run in hand-written HIP kernels (pywave_amd/csrc); there is no CPU fallback.
 """
 from .afd import afd2d, afd3d, AfdPlan, count_snaps  # noqa: F401
+from .aps import aps2d, aps3d  # noqa: F401
 from .pfwi import pfwi, fillpar  # noqa: F401
 from .fwi import FwiProblem, FwiForward, fwi_traces, make_pararray  # noqa: F401
 from .tri import tri3d_image, group_masks  # noqa: F401

@@ -38,6 +38,11 @@ SIGNATURES = {
     "pw_afd3d": (c_int, [c_vp, c_vp, c_vp, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int,
                          c_int, c_int, c_float, c_float, c_float, c_float, c_float, c_float, c_float,
                          c_float, c_vp, c_ll]),
+    "pw_aps2d": (c_int, [c_vp, c_vp, c_vp, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int,
+                         c_int, c_float, c_float, c_float, c_float, c_float, c_float, c_vp, c_ll]),
+    "pw_aps3d": (c_int, [c_vp, c_vp, c_vp, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int,
+                         c_int, c_int, c_float, c_float, c_float, c_float, c_float, c_float, c_float,
+                         c_float, c_vp, c_ll]),
     "pw_afd_output_len": (c_ll, [c_int, c_int, c_int, c_int, c_int]),
     "pw_afd_create": (c_vp, [c_vp, c_int, c_int, c_int, c_int, c_float, c_float, c_float, c_int, c_float,
                              c_int, c_float, c_int]),
@@ -49,6 +54,7 @@ SIGNATURES = {
     "pw_afd_output": (c_int, [c_vp, c_vp, c_ll]),
     "pw_afd_final_field": (c_int, [c_vp, c_vp, c_ll]),
     "pw_afd_set_variant": (c_int, [c_vp, c_int, c_int]),
+    "pw_afd_set_spectral": (c_int, [c_vp, c_int]),
     "pw_afd_variant_count": (c_int, []),
     "pw_afd_variant_info": (c_int, [c_int, c_vp]),
     "pw_afd_autotune": (c_int, [c_vp, c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int)]),

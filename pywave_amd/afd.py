@@ -38,30 +38,35 @@ def afd2d(vel, nt=1501, dt=0.001, ax=[0, 20, 81], az=[0, 20, 81], ns=2, sx=[30, 
           f=[10, 10], t=[0.2, 0.35], A=[1, 2], nbt=30, ct=0.01, jsnap=4, abc=True, ifsnaps=False,
           ps=True, tri=False, dat=None, verb=True):
     """2D acoustic finite-difference modelling / time-reversal imaging (see module docstring)."""
+    return _run2d("pw_afd2d", "afd2d", vel, nt, dt, ax, az, ns, sx, sz, f, t, A, nbt, ct, jsnap, abc, ifsnaps,
+                  tri, dat, verb)
+
+
+def _run2d(cfun, what, vel, nt, dt, ax, az, ns, sx, sz, f, t, A, nbt, ct, jsnap, abc, ifsnaps, tri, dat, verb):
     vel = np.asarray(vel).flatten(order="F").astype(np.float32)
     ox, dx, nx = ax[0], ax[1], int(ax[2])
     oz, dz, nz = az[0], az[1], int(az[2])
     nt, jsnap, nbt = int(nt), int(jsnap), int(nbt)
-    _check_len(vel, nx * nz, "afd2d")
+    _check_len(vel, nx * nz, what)
     if tri:
         source = np.asarray(dat).flatten(order="F").astype(np.float32)
         if source.size != nt * nx:
-            raise ValueError("afd2d: dat must hold nt*nx = %d samples, got %d" % (nt * nx, source.size))
+            raise ValueError("%s: dat must hold nt*nx = %d samples, got %d" % (what, nt * nx, source.size))
         tri = 1
     else:
         source = np.concatenate([np.array(sx), np.array(sz), np.array(f), np.array(t), np.array(A)],
                                 axis=0, dtype="float32")
         if source.size < 5 * int(ns):
-            raise ValueError("afd2d: the source pack holds fewer than 5*ns values")
+            raise ValueError("%s: the source pack holds fewer than 5*ns values" % what)
         tri = 0
     n_out = nt * nx + (nz * nx * count_snaps(nt, jsnap) if jsnap > 0 else 0)
     dout = np.empty(n_out, np.float32)
     L = N.lib()
     ctx = N.context()
-    rc = L.pw_afd2d(ctx, N.ptr(vel), N.ptr(source), tri, nt, nx, nz, int(ns), int(bool(verb)), jsnap,
-                    int(bool(ifsnaps)), int(bool(abc)), nbt, float(ct), float(dt), float(ox), float(dx),
-                    float(oz), float(dz), N.ptr(dout), n_out)
-    N.check(rc, ctx, "afd2d")
+    rc = getattr(L, cfun)(ctx, N.ptr(vel), N.ptr(source), tri, nt, nx, nz, int(ns), int(bool(verb)), jsnap,
+                          int(bool(ifsnaps)), int(bool(abc)), nbt, float(ct), float(dt), float(ox), float(dx),
+                          float(oz), float(dz), N.ptr(dout), n_out)
+    N.check(rc, ctx, what)
     if jsnap > 0:
         ntsnap = count_snaps(nt, jsnap)
         wfd = dout[nt * nx:].reshape(nz, nx, ntsnap, order="F")
@@ -75,31 +80,37 @@ def afd3d(vel, nt=1501, dt=0.001, ax=[0, 20, 81], ay=[0, 20, 81], az=[0, 20, 81]
           A=[1, 2, 2], nbt=30, ct=0.01, jsnap=4, abc=True, ifsnaps=False, ps=True, tri=False, dat=None,
           verb=True):
     """3D acoustic finite-difference modelling / time-reversal imaging (see module docstring)."""
+    return _run3d("pw_afd3d", "afd3d", vel, nt, dt, ax, ay, az, ns, sx, sy, sz, f, t, A, nbt, ct, jsnap, abc,
+                  ifsnaps, tri, dat, verb)
+
+
+def _run3d(cfun, what, vel, nt, dt, ax, ay, az, ns, sx, sy, sz, f, t, A, nbt, ct, jsnap, abc, ifsnaps, tri, dat,
+           verb):
     vel = np.asarray(vel).flatten(order="F").astype(np.float32)
     ox, dx, nx = ax[0], ax[1], int(ax[2])
     oy, dy, ny = ay[0], ay[1], int(ay[2])
     oz, dz, nz = az[0], az[1], int(az[2])
     nt, jsnap, nbt = int(nt), int(jsnap), int(nbt)
-    _check_len(vel, nx * ny * nz, "afd3d")
+    _check_len(vel, nx * ny * nz, what)
     if tri:
         source = np.asarray(dat).flatten(order="F").astype(np.float32)
         if source.size != nt * nx * ny:
-            raise ValueError("afd3d: dat must hold nt*nx*ny samples")
+            raise ValueError("%s: dat must hold nt*nx*ny samples" % what)
         tri = 1
     else:
         source = np.concatenate([np.array(sx), np.array(sy), np.array(sz), np.array(f), np.array(t),
                                  np.array(A)], axis=0, dtype="float32")
         if source.size < 6 * int(ns):
-            raise ValueError("afd3d: the source pack holds fewer than 6*ns values")
+            raise ValueError("%s: the source pack holds fewer than 6*ns values" % what)
         tri = 0
     n_out = nt * nx * ny + (nz * nx * ny * count_snaps(nt, jsnap) if jsnap > 0 else 0)
     dout = np.empty(n_out, np.float32)
     L = N.lib()
     ctx = N.context()
-    rc = L.pw_afd3d(ctx, N.ptr(vel), N.ptr(source), tri, nt, nx, ny, nz, int(ns), int(bool(verb)), jsnap,
-                    int(bool(ifsnaps)), int(bool(abc)), nbt, float(ct), float(dt), float(ox), float(dx),
-                    float(oy), float(dy), float(oz), float(dz), N.ptr(dout), n_out)
-    N.check(rc, ctx, "afd3d")
+    rc = getattr(L, cfun)(ctx, N.ptr(vel), N.ptr(source), tri, nt, nx, ny, nz, int(ns), int(bool(verb)), jsnap,
+                          int(bool(ifsnaps)), int(bool(abc)), nbt, float(ct), float(dt), float(ox), float(dx),
+                          float(oy), float(dy), float(oz), float(dz), N.ptr(dout), n_out)
+    N.check(rc, ctx, what)
     if jsnap > 0:
         ntsnap = count_snaps(nt, jsnap)
         wfd = dout[nt * nx * ny:].reshape(nz, nx, ny, ntsnap, order="F")
@@ -112,10 +123,11 @@ class AfdPlan:
     """A modelling / time-reversal problem kept resident in HBM (pw_afd_* entry points).
 
     vel: (nz, nx) or (nz, nx, ny) velocity; d = (dz, dx[, dy]).  The step loop runs with
-    run(); output() returns the reference's flat return vector.
+    run(); output() returns the reference's flat return vector.  spectral=True uses the
+    pseudo-spectral Laplacian of aps2d/aps3d (pywave/src/wave_psp.c) instead of the stencil.
     """
 
-    def __init__(self, vel, d, nt, dt, nbt=30, ct=0.01, jsnap=0, device=0):
+    def __init__(self, vel, d, nt, dt, nbt=30, ct=0.01, jsnap=0, device=0, spectral=False):
         vel = np.asarray(vel, np.float32)
         self.ndim = vel.ndim
         if self.ndim not in (2, 3):
@@ -133,6 +145,8 @@ class AfdPlan:
                                         int(nbt), float(ct), self.jsnap)
         if not self._p:
             N.check(-1, self._ctx, "AfdPlan")
+        if spectral:  # the pseudo-spectral Laplacian of aps2d/aps3d
+            N.check(self._L.pw_afd_set_spectral(self._p, 1), self._ctx, "set_spectral")
         v = np.ascontiguousarray(vel.ravel(order="F"))
         N.check(self._L.pw_afd_set_velocity(self._p, N.ptr(v)), self._ctx, "set_velocity")
 

@@ -875,6 +875,38 @@ __global__ __launch_bounds__(64 * WAVES) void k_afd2d(Geo g, Step a)
         a.snap[(size_t)(x - g.nb) * g.nz1 + (z - g.nb)] = c;
 }
 
+// ---------------------------------------------------------------- pseudo-spectral step
+// psm2d/psm's update (pywave/src/wave_psp.c:518-581, :193-256) with the spectral Laplacian 'wave'
+// computed beforehand over the whole padded grid: the same leapfrog, injection, damping of both
+// levels, receivers and snapshots as the finite-difference step, on every cell of the updated
+// domain.  One thread per cell; D3 selects the y faces.
+template <bool D3>
+__global__ __launch_bounds__(256) void k_ps_update(Geo g, Step a, const float* __restrict__ wave)
+{
+    const int z = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int x = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int y = blockIdx.z;
+    if (z >= g.nz || x >= g.nx) return;
+    const long long j = (long long)y * g.ldxy + (long long)x * g.ldz + z;
+    const float u = a.cur[j];
+    const float vc = a.vv[j];
+    const float2 fzz = a.fz[z], fxx = a.fx[x];
+    const float2 fyy = D3 ? a.fy[y] : make_float2(1.f, 1.f);
+    const float p = D3 ? damp3(a.pv[j], fzz, fxx, fyy) : damp2(a.pv[j], fzz, fxx);
+    float c = u + (u - p);
+    c = c + wave[j] * vc;
+    c = inject(c, vc, a, z, x, y);
+    const int nby = D3 ? g.nb : 0;
+    const bool onrec = z == g.nb && x >= g.nb && x < g.nb + g.nx1 && y >= nby && y < nby + g.ny1;
+    const long long r = (long long)(y - nby) * g.nx1 + (x - g.nb);
+    if (a.tri && onrec) c = c + vc * a.tri[r];
+    c = D3 ? damp3(c, fzz, fxx, fyy) : damp2(c, fzz, fxx);
+    a.pv[j] = c;
+    if (a.rec && onrec) a.rec[r] = c;
+    if (a.snap && z >= g.nb && z < g.nb + g.nz1 && x >= g.nb && x < g.nb + g.nx1 && y >= nby && y < nby + g.ny1)
+        a.snap[((long long)(y - nby) * g.nx1 + (x - g.nb)) * g.nz1 + (z - g.nb)] = c;
+}
+
 // ---------------------------------------------------------------- helpers
 // vv = ((v*v)*dt)*dt on the padded layout from the clamp-expanded model velocity
 // (vel_expand, pywave/src/wave_abc.c:14-125, replicates the model edges into the boundary)
@@ -1301,6 +1333,9 @@ struct pw_afd {
     int variant = -1, ny_chunk = 0;
     size_t field_elems = 0;
     long long resident[kNumVariants3] = {};  // co-resident blocks per 3D variant (0: not queried)
+    float dz = 0.f, dx = 0.f, dy = 0.f;
+    PsLap* ps = nullptr;      // pseudo-spectral Laplacian (aps2d/aps3d) instead of the stencil
+    float* d_wave = nullptr;  // its output, field layout
 };
 
 namespace {
@@ -1473,6 +1508,8 @@ void afd_free(pw_afd* p)
     if (p->d_fx) hipFree(p->d_fx);
     if (p->d_fy) hipFree(p->d_fy);
     if (p->d_src) hipFree(p->d_src);
+    if (p->d_wave) hipFree(p->d_wave);
+    ps_destroy(p->ps);
 }
 
 int afd_check(pw_afd* p) { return p && p->ctx ? PW_OK : PW_ERR_ARG; }
@@ -1523,6 +1560,7 @@ pw_afd* pw_afd_create(pw_ctx* ctx, int ndim, int nz, int nx, int ny, float dz, f
     p->ct = ct;
     p->jsnap = jsnap;
     p->ntsnap = jsnap > 0 ? (nt - 1) / jsnap + 1 : 0;
+    p->dz = dz; p->dx = dx; p->dy = ndim == 3 ? dy : 1.f;
     Geo& g = p->g;
     g.nz1 = nz; g.nx1 = nx; g.ny1 = ndim == 3 ? ny : 1;
     g.nb = nbt;
@@ -1723,7 +1761,7 @@ static int afd_enqueue(pw_afd* p, int tri, hipEvent_t* evs, int nsteps = 0, bool
     if (tri && !p->have_tri) PW_FAIL(ctx, PW_ERR_STATE, "run: time-reversal data not set");
     if (!tri && !p->have_src) PW_FAIL(ctx, PW_ERR_STATE, "run: sources not set");
     hipStream_t s = ctx->s();
-    const int vi = p->ndim == 3 ? (p->variant >= 0 ? p->variant : default_variant(p)) : -1;
+    const int vi = p->ndim == 3 && !p->ps ? (p->variant >= 0 ? p->variant : default_variant(p)) : -1;
     const bool two = vi >= 0 && kVariants3[vi].lean == 5;
     if (two && !variant_ok(p, vi)) PW_FAIL(ctx, PW_ERR_ARG, "two-step 3D kernel not applicable to this grid");
     if (two && !p->d_c) {
@@ -1782,10 +1820,24 @@ static int afd_enqueue(pw_afd* p, int tri, hipEvent_t* evs, int nsteps = 0, bool
             a.tri = tri_of(it);
             a.rec = rec_of(it);
             a.snap = snap_of(it);
-            const int saved = p->variant;
-            if (two) p->variant = kLeanDefault;  // odd remainder: one single step
-            int rc = afd_launch_step(p, a, evs ? evs[2 * k] : nullptr, evs ? evs[2 * k + 1] : nullptr);
-            p->variant = saved;
+            int rc = PW_OK;
+            if (p->ps) {
+                // spectral Laplacian of C_n, then the update (the pair of launches is one step)
+                PW_HIP(ctx, ps_apply(p->ps, A, p->d_wave, s));
+                dim3 grid(pw::ceil_div(g.nz, 64), pw::ceil_div(g.nx, 4), g.ny);
+                if (p->ndim == 3)
+                    hipExtLaunchKernelGGL(k_ps_update<true>, grid, dim3(256), 0, s, evs ? evs[2 * k] : nullptr,
+                                          evs ? evs[2 * k + 1] : nullptr, 0, g, a, (const float*)p->d_wave);
+                else
+                    hipExtLaunchKernelGGL(k_ps_update<false>, grid, dim3(256), 0, s, evs ? evs[2 * k] : nullptr,
+                                          evs ? evs[2 * k + 1] : nullptr, 0, g, a, (const float*)p->d_wave);
+                PW_HIP(ctx, hipGetLastError());
+            } else {
+                const int saved = p->variant;
+                if (two) p->variant = kLeanDefault;  // odd remainder: one single step
+                rc = afd_launch_step(p, a, evs ? evs[2 * k] : nullptr, evs ? evs[2 * k + 1] : nullptr);
+                p->variant = saved;
+            }
             if (rc) return rc;
             std::swap(A, B);
             it += its;
@@ -1953,6 +2005,28 @@ int pw_afd_autotune(pw_afd* p, int steps, int* variant_out, int* chunk_out)
     return PW_OK;
 }
 
+// the pseudo-spectral Laplacian of aps2d/aps3d (on != 0) instead of the finite-difference stencil
+int pw_afd_set_spectral(pw_afd* p, int on)
+{
+    if (afd_check(p)) return PW_ERR_ARG;
+    pw_ctx* ctx = p->ctx;
+    if (!on) {
+        ps_destroy(p->ps);
+        p->ps = nullptr;
+        return PW_OK;
+    }
+    if (p->ps) return PW_OK;
+    const Geo& g = p->g;
+    std::string err;
+    p->ps = ps_create(g.nz2, g.nx2, g.ny2, g.ldz, g.ldxy, p->dz, p->dx, p->dy, err);
+    if (!p->ps) PW_FAIL(ctx, PW_ERR_ARG, "set_spectral: " + err);
+    if (!p->d_wave) {
+        PW_HIP(ctx, hipMalloc(&p->d_wave, p->field_elems * 4));
+        PW_HIP(ctx, hipMemsetAsync(p->d_wave, 0, p->field_elems * 4, ctx->s()));
+    }
+    return PW_OK;
+}
+
 int pw_afd_set_variant(pw_afd* p, int variant, int ny_chunk)
 {
     if (afd_check(p)) return PW_ERR_ARG;
@@ -1965,7 +2039,7 @@ int pw_afd_set_variant(pw_afd* p, int variant, int ny_chunk)
 
 static int afd_oneshot(pw_ctx* ctx, int ndim, const float* vel, const float* src, int tri, int nt,
                        int nx, int ny, int nz, int ns, int jsnap, int nbt, float ct, float dt,
-                       float dx, float dy, float dz, float* out, long long out_len)
+                       float dx, float dy, float dz, float* out, long long out_len, int spectral = 0)
 {
     if (!ctx) return PW_ERR_ARG;
     if (!vel || !src || !out) PW_FAIL(ctx, PW_ERR_ARG, "afd: null array");
@@ -1975,7 +2049,8 @@ static int afd_oneshot(pw_ctx* ctx, int ndim, const float* vel, const float* src
         PW_FAIL(ctx, PW_ERR_ARG, "afd: output length mismatch");
     pw_afd* p = pw_afd_create(ctx, ndim, nz, nx, ny, dz, dx, dy, nt, dt, nbt, ct, jsnap);
     if (!p) return PW_ERR_ARG;
-    int rc = pw_afd_set_velocity(p, vel);
+    int rc = spectral ? pw_afd_set_spectral(p, 1) : PW_OK;
+    if (!rc) rc = pw_afd_set_velocity(p, vel);
     if (!rc) rc = tri ? pw_afd_set_tri_data(p, src) : pw_afd_set_sources(p, ns, src, nullptr);
     if (!rc) rc = pw_afd_run(p, tri);
     if (!rc) rc = pw_afd_output(p, out, out_len);
@@ -2000,6 +2075,26 @@ int pw_afd3d(pw_ctx* ctx, const float* vel, const float* src, int tri, int nt, i
     (void)verb; (void)ifsnaps; (void)abc; (void)ox; (void)oy; (void)oz;
     return afd_oneshot(ctx, 3, vel, src, tri, nt, nx, ny, nz, ns, jsnap, nbt, ct, dt, dx, dy, dz, out,
                        out_len);
+}
+
+// apscfun.aps2dc / aps3dc (pywave/src/aps.c): the afd entry points with the pseudo-spectral engine
+int pw_aps2d(pw_ctx* ctx, const float* vel, const float* src, int tri, int nt, int nx, int nz,
+             int ns, int verb, int jsnap, int ifsnaps, int abc, int nbt, float ct, float dt,
+             float ox, float dx, float oz, float dz, float* out, long long out_len)
+{
+    (void)verb; (void)ifsnaps; (void)abc; (void)ox; (void)oz;
+    return afd_oneshot(ctx, 2, vel, src, tri, nt, nx, 1, nz, ns, jsnap, nbt, ct, dt, dx, 1.f, dz, out,
+                       out_len, 1);
+}
+
+int pw_aps3d(pw_ctx* ctx, const float* vel, const float* src, int tri, int nt, int nx, int ny,
+             int nz, int ns, int verb, int jsnap, int ifsnaps, int abc, int nbt, float ct,
+             float dt, float ox, float dx, float oy, float dy, float oz, float dz, float* out,
+             long long out_len)
+{
+    (void)verb; (void)ifsnaps; (void)abc; (void)ox; (void)oy; (void)oz;
+    return afd_oneshot(ctx, 3, vel, src, tri, nt, nx, ny, nz, ns, jsnap, nbt, ct, dt, dx, dy, dz, out,
+                       out_len, 1);
 }
 
 }  // extern "C"

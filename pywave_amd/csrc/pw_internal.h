@@ -65,3 +65,11 @@ template <typename T>
 inline T ceil_div(T a, T b) { return (a + b - 1) / b; }
 
 }  // namespace pw
+
+// Pseudo-spectral Laplacian of padded fields (pw_psp.hip): n1 (z, even, real transform), n2 (x),
+// n3 (y; 1 in 2D) are the padded extents; fields use the plans' layout (z stride ldz, plane
+// stride ldxy).  ps_apply enqueues wave = IFFT(-(2 pi |k|)^2 FFT(field)) on stream s.
+struct PsLap;
+PsLap* ps_create(int n1, int n2, int n3, int ldz, long long ldxy, float dz, float dx, float dy, std::string& err);
+hipError_t ps_apply(PsLap* P, const float* field, float* wave, hipStream_t s);
+void ps_destroy(PsLap* P);
