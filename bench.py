@@ -264,6 +264,56 @@ def lstri_bench():
     return out
 
 
+def aps3d_bench(cpu_steps=20):
+    """Pseudo-spectral 3D modelling on the reference demo's configuration (demos/test_first.py:
+    81^3 at 20 m, v = 1500 + 24 z m/s, nbt=30 -> 141^3 updated, 144^3 padded FFT grid, nt=1501,
+    aps3d's default three sources).  Per step: real z transforms, x and y transforms (the y pass
+    fused with the wavenumber multiply and its inverse), inverse x and z, then the leapfrog update.
+    Bytes per step and padded cell as the passes move them: z forward 8, x forward 16, y fused 20,
+    x inverse 16, z inverse 12 (f32 in, complex half-spectrum out and back), update 20 per updated
+    cell (u_n, u_{n-1}, vv, Laplacian read; u_{n+1} written)."""
+    from pywave_amd import AfdPlan
+
+    n, nbt, nt, dt, d = 81, 30, 1501, 0.001, 20.0
+    v = (np.arange(n) * 20 * 1.2 + 1500).astype(np.float32)
+    vel = np.ascontiguousarray(np.broadcast_to(v[:, None, None], (n, n, n)).astype(np.float32))
+    pack = np.array([30, 40, 50, 30, 40, 50, 30, 40, 50, 10, 10, 10, 0.2, 0.35, 0.5, 1, 2, 2], np.float32)
+    plan = AfdPlan(vel, (d, d, d), nt, dt, nbt, 0.01, 0, spectral=True)
+    plan.set_sources(pack, 3)
+    plan.run()
+    ms, _ = plan.run_timed()
+    out = plan.output()
+    assert np.isfinite(out).all()
+    plan.close()
+    m = n + 2 * nbt
+    n1 = n2 = n3 = 144
+    nk = n1 // 2 + 1
+    cells_p = n1 * n2 * n3
+    spec = nk * n2 * n3 * 8.0  # bytes of the half spectrum
+    step_bytes = (4.0 * cells_p + spec) + 2 * spec + (2 * spec + 4.0 * nk * n2 * n3) + 2 * spec + \
+                 (spec + 4.0 * cells_p) + 20.0 * m ** 3
+    step_us = ms * 1e3 / nt
+    res = {"workload": "aps3d 81^3 at 20 m (+30 ABC: 141^3, FFT grid 144^3), nt=1501, 3 sources (demos/test_first.py)",
+           "run_ms": round(ms, 2), "step_us": round(step_us, 2),
+           "Gcell_per_s": round(m ** 3 * nt / (ms * 1e-3) / 1e9, 2),
+           "algorithmic_bytes_per_step": step_bytes,
+           "achieved_GBs": round(step_bytes / (step_us * 1e-6) / 1e9, 1)}
+    res["frac_hbm"] = round(res["achieved_GBs"] / PEAK_HBM_GBS, 4)
+    try:
+        from oracle import refrun
+        if cpu_steps > 0 and refrun.available():
+            _, el = refrun.afd3d(vel, cpu_steps, dt, d, d, d, nbt, 0.01, pack, 3, ps=True)
+            per = el / cpu_steps
+            res["cpu_baseline"] = dict(value=round(per * nt, 1), unit="s per 1501-step run", cores=1,
+                                       kind="reference",
+                                       sample="reference C psm (pywave/src/wave_psp.c:25) on the same model, %d of "
+                                              "1501 steps in %.2f s, extrapolated; single thread" % (cpu_steps, el))
+            res["speedup_vs_cpu"] = round(per * nt / (ms * 1e-3), 1)
+    except Exception as e:
+        res["cpu_baseline"] = dict(value=None, unit="s per run", cores=1, kind="reference", sample="failed: %s" % e)
+    return res
+
+
 def tri_bench():
     """Time-reversal imaging on the 128^3 workload (+30-cell ABC: 188^3 updated, nt=2000): the
     forward modelling of three sources, then the four interleaved receiver groups back-propagated
@@ -377,10 +427,11 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-small", action="store_true")
     ap.add_argument("--no-lstri", action="store_true")
+    ap.add_argument("--no-aps", action="store_true")
     ap.add_argument("--fwi-plain-shots", type=int, default=4)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (HBM counter calibration)")
-    ap.add_argument("--only", choices=["fwi", "tri", "batch", "small", "lstri"],
+    ap.add_argument("--only", choices=["fwi", "tri", "batch", "small", "lstri", "aps"],
                     help="run one secondary workload alone and print its JSON object (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -393,7 +444,8 @@ def main():
         return
     if args.only:
         leg = {"fwi": lambda: fwi_bench(args.fwi_plain_shots), "tri": tri_bench, "batch": batch_bench,
-               "small": afd2d_small_bench, "lstri": lstri_bench}[args.only]
+               "small": afd2d_small_bench, "lstri": lstri_bench,
+               "aps": lambda: aps3d_bench(0 if args.no_cpu else 20)}[args.only]
         print(json.dumps({args.only: leg()}))
         return
 
@@ -450,6 +502,7 @@ def main():
     batch = None if args.no_batch else batch_bench()
     small = None if args.no_small else afd2d_small_bench()
     lstri = None if args.no_lstri else lstri_bench()
+    aps = None if args.no_aps else aps3d_bench()
     line = {
         "metric": "Gcell-updates/s 3D FP32 FD (% HBM roofline); FWI grad shots/s batched vs plain",
         "value": round(value, 3),
@@ -488,6 +541,7 @@ def main():
         "batch2d": batch,
         "afd2d_small": small,
         "lstri2d": lstri,
+        "aps3d": aps,
     }
     print(json.dumps(line))
 

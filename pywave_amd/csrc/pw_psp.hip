@@ -107,19 +107,38 @@ __device__ __forceinline__ void bf5(cf* F, int u, int m, int fs, const cf* tw)
     F[3 * m + u] = csub(s11, s12);
 }
 
-// the butterfly passes of one transform over LDS data already in digit-reversed order
-__device__ void run_levels(cf* F, const FftDev& d, int tid, int T)
+// a / b for 0 <= a < 2^22 and b >= 1 through the f32 reciprocal rb = 1/b (the product is within
+// one of the quotient; one correction step each way)
+__device__ __forceinline__ int qdiv(int a, int b, float rb)
+{
+    int q = (int)((float)a * rb);
+    const int r = a - q * b;
+    q += (r >= b) - (r < 0);
+    return q;
+}
+
+// twiddles of a descriptor into LDS
+__device__ __forceinline__ void stage_tw(cf* dst, const FftDev& d, int tid, int T)
+{
+    for (int i = tid; i < d.n; i += T) dst[i] = d.tw[i];
+}
+
+// the butterfly passes of nl transforms (line l at F + l*ld) over LDS data already in
+// digit-reversed order; tw: the descriptor's twiddles staged in LDS
+__device__ void run_levels(cf* F, const FftDev& d, const cf* tw, int tid, int T, int nl = 1, int ld = 0)
 {
     for (int L = d.nlev - 1; L >= 0; --L) {
         const int p = d.p[L], m = d.m[L], fs = d.fs[L];
         const int nbf = d.n / p;
-        for (int b = tid; b < nbf; b += T) {
-            const int c = b / m, k = b - c * m;
-            cf* G = F + c * p * m;
-            if (p == 4) bf4(G, k, m, fs, d.tw, d.inverse);
-            else if (p == 2) bf2(G, k, m, fs, d.tw);
-            else if (p == 3) bf3(G, k, m, fs, d.tw);
-            else bf5(G, k, m, fs, d.tw);
+        const float rn = 1.f / (float)nbf, rm = 1.f / (float)m;
+        for (int bb = tid; bb < nbf * nl; bb += T) {
+            const int l = nl > 1 ? qdiv(bb, nbf, rn) : 0, b = bb - l * nbf;
+            const int c = qdiv(b, m, rm), k = b - c * m;
+            cf* G = F + l * ld + c * p * m;
+            if (p == 4) bf4(G, k, m, fs, tw, d.inverse);
+            else if (p == 2) bf2(G, k, m, fs, tw);
+            else if (p == 3) bf3(G, k, m, fs, tw);
+            else bf5(G, k, m, fs, tw);
         }
         __syncthreads();
     }
@@ -132,32 +151,82 @@ struct PsGeo {
     float wwt;              // 1/(n1 n2 n3)
 };
 
-// forward: centering sign, real transform of each z column -> tmp [n3][n2][nk]
-__global__ void k_ps_rz(PsGeo g, const float* __restrict__ field, cf* __restrict__ tmp, FftDev d,
-                        const cf* __restrict__ sup)
+// Global-to-LDS copy of 'total' elements: each thread issues its (up to KR per round) loads
+// before the first LDS store, so the rounds' memory latencies overlap instead of queueing one
+// behind the other.  ld(e) reads element e, st(e, v) stores it.
+template <int KR, typename V, typename LD, typename ST>
+__device__ __forceinline__ void gather(int total, int tid, int T, LD ld, ST st)
+{
+    for (int e0 = 0; e0 < total; e0 += KR * T) {
+        V r[KR];
+#pragma unroll
+        for (int q = 0; q < KR; ++q) {
+            const int e = e0 + tid + q * T;
+            if (e < total) r[q] = ld(e);
+        }
+#pragma unroll
+        for (int q = 0; q < KR; ++q) {
+            const int e = e0 + tid + q * T;
+            if (e < total) st(e, r[q]);
+        }
+    }
+}
+
+__device__ __forceinline__ void stage_perm(int* dst, const FftDev& d, int tid, int T)
+{
+    for (int i = tid; i < d.n; i += T) dst[i] = d.perm[i];
+}
+
+// forward: centering sign, real transform of each z column -> tmp [n3][n2][nk]; cpb neighbouring
+// columns (x0 .. x0+cpb-1 of plane y) per block.  Columns are read whole and contiguously, the
+// digit-reversed order is formed in LDS.
+__global__ __launch_bounds__(256) void k_ps_rz(PsGeo g, const float* __restrict__ field, cf* __restrict__ tmp,
+                                               FftDev d, const cf* __restrict__ sup, int cpb)
 {
     extern __shared__ cf sh[];
-    const int x = blockIdx.x, y = blockIdx.y, tid = threadIdx.x, T = blockDim.x;
-    const float* col = field + (long long)y * g.ldxy + (long long)x * g.ldz;
-    const bool neg = g.n3 > 1 ? ((y & 1) != (x & 1)) : (x & 1);
-    for (int pos = tid; pos < g.h; pos += T) {
-        const int j = d.perm[pos];
-        float a = col[2 * j], b = col[2 * j + 1];
-        if (neg) { a = -a; b = -b; }
-        sh[pos] = {a, b};
+    const int h = g.h, tid = threadIdx.x, T = blockDim.x;
+    const int x0 = blockIdx.x * cpb, y = blockIdx.y;
+    const int nc = min(cpb, g.n2 - x0);
+    cf* A = sh;                             // [cpb][h] natural order
+    cf* B = sh + cpb * h;                   // [cpb][h] digit-reversed
+    cf* tw = sh + 2 * cpb * h;
+    int* perm = (int*)(tw + h);
+    stage_tw(tw, d, tid, T);
+    stage_perm(perm, d, tid, T);
+    float* Af = (float*)A;
+    const float r2h = 1.f / (float)(2 * h);
+    gather<8, float>(nc * 2 * h, tid, T,
+        [&](int e) {
+            const int c = qdiv(e, 2 * h, r2h), f = e - c * 2 * h;
+            return field[(long long)y * g.ldxy + (long long)(x0 + c) * g.ldz + f];
+        },
+        [&](int e, float v) {
+            const int c = qdiv(e, 2 * h, r2h);
+            const int x = x0 + c;
+            const bool neg = g.n3 > 1 ? ((y & 1) != (x & 1)) : (x & 1);
+            Af[e] = neg ? -v : v;
+        });
+    __syncthreads();
+    const float rh = 1.f / (float)h;
+    for (int e = tid; e < nc * h; e += T) {
+        const int c = qdiv(e, h, rh), pos = e - c * h;
+        B[c * h + pos] = A[c * h + perm[pos]];
     }
     __syncthreads();
-    run_levels(sh, d, tid, T);
-    cf* X = tmp + ((long long)y * g.n2 + x) * g.nk;
-    const int h = g.h;
-    for (int k = tid; k <= h / 2; k += T) {
+    run_levels(B, d, tw, tid, T, nc, h);
+    const int nkk = h / 2 + 1;
+    const float rk = 1.f / (float)nkk;
+    for (int e = tid; e < nc * nkk; e += T) {
+        const int c = qdiv(e, nkk, rk), k = e - c * nkk;
+        const cf* S = B + c * h;
+        cf* X = tmp + ((long long)y * g.n2 + x0 + c) * g.nk;
         if (k == 0) {
-            const cf t0 = sh[0];
+            const cf t0 = S[0];
             X[0] = {t0.r + t0.i, 0.f};
             X[h] = {t0.r - t0.i, 0.f};
             continue;
         }
-        const cf fpk = sh[k], q = sh[h - k];
+        const cf fpk = S[k], q = S[h - k];
         const cf fpnk = {q.r, -q.i};
         const cf f1k = cadd(fpk, fpnk), f2k = csub(fpk, fpnk);
         const cf w = cmul(f2k, sup[k - 1]);
@@ -168,74 +237,130 @@ __global__ void k_ps_rz(PsGeo g, const float* __restrict__ field, cf* __restrict
     }
 }
 
-// complex transforms along x (stride nk, n2 points) or y (stride nk*n2, n3 points).
-// mode 0: forward; 1: inverse; 2: forward, multiply by the wavenumber operator, inverse.
-__global__ void k_ps_line(PsGeo g, cf* __restrict__ tmp, int axis, int mode, FftDev df, FftDev di,
-                          const float* __restrict__ lapl)
+// complex transforms along x (stride nk, n2 points) or y (stride nk*n2, n3 points), lpb
+// neighbouring wavenumber lines (consecutive i1, adjacent in memory) per block so that every
+// load and store moves lpb contiguous elements; loads in natural order, digit reversal in LDS.
+// MODE 0: forward; 1: inverse; 2: forward, multiply by the wavenumber operator, inverse.  (A
+// compile-time mode keeps the descriptors in kernel arguments instead of private memory.)
+template <int MODE>
+__global__ __launch_bounds__(256) void k_ps_line(PsGeo g, cf* __restrict__ tmp, int axis, FftDev df, FftDev di,
+                                                 const float* __restrict__ lapl, int lpb)
 {
     extern __shared__ cf sh[];
-    const int l = blockIdx.x, tid = threadIdx.x, T = blockDim.x;
-    const int i1 = l % g.nk, o = l / g.nk;
+    const int tid = threadIdx.x, T = blockDim.x;
+    const int ngrp = (g.nk + lpb - 1) / lpb;
+    const int i10 = (blockIdx.x % ngrp) * lpb, o = blockIdx.x / ngrp;
+    const int nl = min(lpb, g.nk - i10);
     long long base, stride;
     int n;
-    if (axis == 0) { base = (long long)o * g.n2 * g.nk + i1; stride = g.nk; n = g.n2; }
-    else { base = (long long)o * g.nk + i1; stride = (long long)g.nk * g.n2; n = g.n3; }
-    cf* A = sh;
-    cf* B = sh + n;
-    const FftDev& d0 = mode == 1 ? di : df;
-    for (int pos = tid; pos < n; pos += T) A[pos] = tmp[base + (long long)d0.perm[pos] * stride];
+    if (axis == 0) { base = (long long)o * g.n2 * g.nk + i10; stride = g.nk; n = g.n2; }
+    else { base = (long long)o * g.nk + i10; stride = (long long)g.nk * g.n2; n = g.n3; }
+    // lines in LDS at a stride of n + 1 elements: the lanes that touch element i of neighbouring
+    // lines fall in different banks
+    const int L = n + 1;
+    cf* A = sh;                 // [lpb][L]
+    cf* B = sh + lpb * L;       // [lpb][L]
+    cf* tf = sh + 2 * lpb * L;  // twiddles of the first transform
+    cf* ti = tf + n;            // and of the inverse (MODE 2)
+    int* perm = (int*)(ti + n); // the same digit reversal for both directions
+    const FftDev& d0 = MODE == 1 ? di : df;
+    stage_tw(tf, d0, tid, T);
+    if (MODE == 2) stage_tw(ti, di, tid, T);
+    stage_perm(perm, d0, tid, T);
+    const float rl = 1.f / (float)nl, rn = 1.f / (float)n;
+    gather<8, cf>(n * nl, tid, T,
+        [&](int e) { const int i = qdiv(e, nl, rl), l = e - i * nl; return tmp[base + l + (long long)i * stride]; },
+        [&](int e, cf v) { const int i = qdiv(e, nl, rl), l = e - i * nl; A[l * L + i] = v; });
     __syncthreads();
-    run_levels(A, d0, tid, T);
-    if (mode == 2) {
-        for (int i = tid; i < n; i += T) {
-            const float w = lapl[base + (long long)i * stride];
-            A[i] = {A[i].r * w, A[i].i * w};
+    for (int e = tid; e < n * nl; e += T) {
+        const int l = qdiv(e, n, rn), pos = e - l * n;
+        B[l * L + pos] = A[l * L + perm[pos]];
+    }
+    __syncthreads();
+    run_levels(B, d0, tf, tid, T, nl, L);
+    cf* R = B;
+    if (MODE == 2) {
+        gather<8, float>(n * nl, tid, T,
+            [&](int e) { const int i = qdiv(e, nl, rl), l = e - i * nl; return lapl[base + l + (long long)i * stride]; },
+            [&](int e, float w) {
+                const int i = qdiv(e, nl, rl), l = e - i * nl;
+                const cf a = B[l * L + i];
+                B[l * L + i] = {a.r * w, a.i * w};
+            });
+        __syncthreads();
+        for (int e = tid; e < n * nl; e += T) {
+            const int l = qdiv(e, n, rn), pos = e - l * n;
+            A[l * L + pos] = B[l * L + perm[pos]];
         }
         __syncthreads();
-        for (int pos = tid; pos < n; pos += T) B[pos] = A[di.perm[pos]];
-        __syncthreads();
-        run_levels(B, di, tid, T);
-        A = B;
+        run_levels(A, di, ti, tid, T, nl, L);
+        R = A;
     }
-    for (int i = tid; i < n; i += T) tmp[base + (long long)i * stride] = A[i];
+    for (int e = tid; e < n * nl; e += T) {
+        const int i = qdiv(e, nl, rl), l = e - i * nl;
+        tmp[base + l + (long long)i * stride] = R[l * L + i];
+    }
 }
 
 // inverse: split of each z column's bins, half-length inverse transform, centering sign and
-// 1/(n1 n2 n3) -> wave (field layout)
-__global__ void k_ps_iz(PsGeo g, const cf* __restrict__ tmp, float* __restrict__ wave, FftDev d,
-                        const cf* __restrict__ sup)
+// 1/(n1 n2 n3) -> wave (field layout); cpb neighbouring columns per block
+__global__ __launch_bounds__(256) void k_ps_iz(PsGeo g, const cf* __restrict__ tmp, float* __restrict__ wave,
+                                               FftDev d, const cf* __restrict__ sup, int cpb)
 {
     extern __shared__ cf sh[];
-    const int x = blockIdx.x, y = blockIdx.y, tid = threadIdx.x, T = blockDim.x;
-    const int h = g.h;
+    const int h = g.h, tid = threadIdx.x, T = blockDim.x;
+    const int x0 = blockIdx.x * cpb, y = blockIdx.y;
+    const int nc = min(cpb, g.n2 - x0);
     cf* A = sh;
-    cf* B = sh + h;
-    const cf* X = tmp + ((long long)y * g.n2 + x) * g.nk;
-    for (int k = tid; k <= h / 2; k += T) {
-        if (k == 0) {
-            A[0] = {X[0].r + X[h].r, X[0].r - X[h].r};
-            continue;
+    cf* B = sh + cpb * h;
+    cf* tw = sh + 2 * cpb * h;
+    int* perm = (int*)(tw + h);
+    stage_tw(tw, d, tid, T);
+    stage_perm(perm, d, tid, T);
+    const int nkk = h / 2 + 1;
+    const float rk = 1.f / (float)nkk;
+    struct Pair {
+        cf a, b;
+    };
+    gather<4, Pair>(nc * nkk, tid, T,
+        [&](int e) {
+            const int c = qdiv(e, nkk, rk), k = e - c * nkk;
+            const cf* X = tmp + ((long long)y * g.n2 + x0 + c) * g.nk;
+            return Pair{X[k], X[h - k]};
+        },
+        [&](int e, Pair pr) {
+        const int c = qdiv(e, nkk, rk), k = e - c * nkk;
+        cf* S = A + c * h;
+        if (k == 0) {  // pr = (X[0], X[h])
+            S[0] = {pr.a.r + pr.b.r, pr.a.r - pr.b.r};
+            return;
         }
-        const cf fk = X[k], q = X[h - k];
+        const cf fk = pr.a, q = pr.b;
         const cf fnkc = {q.r, -q.i};
         const cf fek = cadd(fk, fnkc), dd = csub(fk, fnkc);
         const cf fok = cmul(dd, sup[k - 1]);
         const cf lo = cadd(fek, fok);
         cf hi = csub(fek, fok);
         hi.i = hi.i * -1.f;
-        if (k != h - k) A[k] = lo;
-        A[h - k] = hi;
+        if (k != h - k) S[k] = lo;
+        S[h - k] = hi;
+        });
+    __syncthreads();
+    const float rh = 1.f / (float)h;
+    for (int e = tid; e < nc * h; e += T) {
+        const int c = qdiv(e, h, rh), pos = e - c * h;
+        B[c * h + pos] = A[c * h + perm[pos]];
     }
     __syncthreads();
-    for (int pos = tid; pos < h; pos += T) B[pos] = A[d.perm[pos]];
-    __syncthreads();
-    run_levels(B, d, tid, T);
-    const bool neg = g.n3 > 1 ? ((y & 1) != (x & 1)) : (x & 1);
-    const float w = neg ? -g.wwt : g.wwt;
-    float* col = wave + (long long)y * g.ldxy + (long long)x * g.ldz;
-    for (int j = tid; j < h; j += T) {
-        col[2 * j] = w * B[j].r;
-        col[2 * j + 1] = w * B[j].i;
+    run_levels(B, d, tw, tid, T, nc, h);
+    const float* Bf = (const float*)B;
+    const float r2h = 1.f / (float)(2 * h);
+    for (int e = tid; e < nc * 2 * h; e += T) {
+        const int c = qdiv(e, 2 * h, r2h), f = e - c * 2 * h;
+        const int x = x0 + c;
+        const bool neg = g.n3 > 1 ? ((y & 1) != (x & 1)) : (x & 1);
+        const float w = neg ? -g.wwt : g.wwt;
+        wave[(long long)y * g.ldxy + (long long)x * g.ldz + f] = w * Bf[c * 2 * h + f];
     }
 }
 
@@ -301,7 +426,8 @@ struct PsLap {
     std::vector<void*> bufs;
     FftDev rf{}, ri{}, xf{}, xi{}, yf{}, yi{};
     cf *sup_f = nullptr, *sup_i = nullptr;
-    int thr_z = 64, thr_x = 64, thr_y = 64;
+    int cpb = 1;               // z columns per block of the real transforms
+    int lpb_x = 1, lpb_y = 1;  // wavenumber lines per block of the x and y passes
 };
 
 namespace {
@@ -345,7 +471,6 @@ int make_dev(PsLap* P, int n, int inverse, FftDev& d, std::string& err)
     return rc;
 }
 
-int threads_for(int n) { return std::min(256, std::max(64, (n / 2 + 63) / 64 * 64)); }
 
 }  // namespace
 
@@ -364,6 +489,11 @@ PsLap* ps_create(int n1, int n2, int n3, int ldz, long long ldxy, float dz, floa
 {
     if (n1 < 2 || n1 % 2 || n2 < 1 || n3 < 1 || ldz < n1) {
         err = "spectral Laplacian: bad padded extents";
+        return nullptr;
+    }
+    if (3 * (size_t)(n1 / 2) * sizeof(cf) + (size_t)n1 * 2 > 64 * 1024 ||
+        (4 * (size_t)std::max(n2, n3) + 2) * sizeof(cf) + (size_t)std::max(n2, n3) * 4 > 64 * 1024) {
+        err = "spectral Laplacian: a transform line exceeds the 64 KB of LDS per workgroup";
         return nullptr;
     }
     PsLap* P = new PsLap;
@@ -415,9 +545,17 @@ PsLap* ps_create(int n1, int n2, int n3, int ldz, long long ldxy, float dz, floa
         ps_destroy(P);
         return nullptr;
     }
-    P->thr_z = threads_for(g.h);
-    P->thr_x = threads_for(n2);
-    P->thr_y = threads_for(n3);
+    // neighbouring z columns per block: about 256 samples' worth of butterflies per block
+    P->cpb = std::max(1, std::min(8, 512 / g.h));
+    while (P->cpb > 1 && ((size_t)(2 * P->cpb + 1) * g.h * sizeof(cf) + g.h * 4) > 48 * 1024) P->cpb /= 2;
+    // 16 adjacent lines per block (128 B per load), within 48 KB of LDS for both line buffers
+    auto lpb_for = [&](int n) {
+        int l = 16;
+        while (l > 1 && ((size_t)2 * l * (n + 1) + 2 * (size_t)n) * sizeof(cf) + (size_t)n * 4 > 48 * 1024) l /= 2;
+        return std::min(l, nk);
+    };
+    P->lpb_x = lpb_for(n2);
+    P->lpb_y = lpb_for(n3);
     return P;
 }
 
@@ -425,19 +563,22 @@ PsLap* ps_create(int n1, int n2, int n3, int ldz, long long ldxy, float dz, floa
 hipError_t ps_apply(PsLap* P, const float* field, float* wave, hipStream_t s)
 {
     const PsGeo& g = P->g;
-    const size_t shz = 2 * (size_t)g.h * sizeof(cf);
-    hipLaunchKernelGGL(k_ps_rz, dim3(g.n2, g.n3), dim3(P->thr_z), shz, s, g, field, P->tmp, P->rf, P->sup_f);
+    const size_t shz = (2 * (size_t)P->cpb + 1) * g.h * sizeof(cf) + (size_t)g.h * 4;
+    const int gz = (g.n2 + P->cpb - 1) / P->cpb;
+    hipLaunchKernelGGL(k_ps_rz, dim3(gz, g.n3), dim3(256), shz, s, g, field, P->tmp, P->rf, P->sup_f, P->cpb);
+    const int gx = (g.nk + P->lpb_x - 1) / P->lpb_x, gy = (g.nk + P->lpb_y - 1) / P->lpb_y;
+    const size_t shx = (2 * (size_t)P->lpb_x * (g.n2 + 1) + 2 * (size_t)g.n2) * sizeof(cf) + (size_t)g.n2 * 4,
+                 shy = (2 * (size_t)P->lpb_y * (g.n3 + 1) + 2 * (size_t)g.n3) * sizeof(cf) + (size_t)g.n3 * 4;
     if (g.n3 == 1) {
-        hipLaunchKernelGGL(k_ps_line, dim3(g.nk), dim3(P->thr_x), 2 * (size_t)g.n2 * sizeof(cf), s, g, P->tmp, 0, 2,
-                           P->xf, P->xi, P->lapl);
+        hipLaunchKernelGGL(k_ps_line<2>, dim3(gx), dim3(256), shx, s, g, P->tmp, 0, P->xf, P->xi, P->lapl, P->lpb_x);
     } else {
-        hipLaunchKernelGGL(k_ps_line, dim3(g.nk * g.n3), dim3(P->thr_x), 2 * (size_t)g.n2 * sizeof(cf), s, g, P->tmp,
-                           0, 0, P->xf, P->xi, P->lapl);
-        hipLaunchKernelGGL(k_ps_line, dim3(g.nk * g.n2), dim3(P->thr_y), 2 * (size_t)g.n3 * sizeof(cf), s, g, P->tmp,
-                           1, 2, P->yf, P->yi, P->lapl);
-        hipLaunchKernelGGL(k_ps_line, dim3(g.nk * g.n3), dim3(P->thr_x), 2 * (size_t)g.n2 * sizeof(cf), s, g, P->tmp,
-                           0, 1, P->xf, P->xi, P->lapl);
+        hipLaunchKernelGGL(k_ps_line<0>, dim3(gx * g.n3), dim3(256), shx, s, g, P->tmp, 0, P->xf, P->xi, P->lapl,
+                           P->lpb_x);
+        hipLaunchKernelGGL(k_ps_line<2>, dim3(gy * g.n2), dim3(256), shy, s, g, P->tmp, 1, P->yf, P->yi, P->lapl,
+                           P->lpb_y);
+        hipLaunchKernelGGL(k_ps_line<1>, dim3(gx * g.n3), dim3(256), shx, s, g, P->tmp, 0, P->xf, P->xi, P->lapl,
+                           P->lpb_x);
     }
-    hipLaunchKernelGGL(k_ps_iz, dim3(g.n2, g.n3), dim3(P->thr_z), shz, s, g, P->tmp, wave, P->ri, P->sup_i);
+    hipLaunchKernelGGL(k_ps_iz, dim3(gz, g.n3), dim3(256), shz, s, g, P->tmp, wave, P->ri, P->sup_i, P->cpb);
     return hipGetLastError();
 }
