@@ -314,6 +314,67 @@ def aps3d_bench(cpu_steps=20):
     return res
 
 
+def visco_bench(cpu_steps=300):
+    """Visco-acoustic forward modelling (forward_modeling, pywave/src/wave_fwimodeling.c:116-220,
+    the media=2 modelling of pfwi) on the FWI workload's 400x800, 32-shot, nt=3000 acquisition
+    with a seeded Q of 20-100: all 32 shots per launch.  Per cell and step the kernel reads
+    p_n, p_{n-1}, v^2 dt^2, tau, taus, r_n and writes p_{n+1}, r_{n+1}: 32 algorithmic bytes.
+    The acoustic forward of the same plan is timed beside it."""
+    from pywave_amd import FwiProblem
+    from pywave_amd.workloads import fwi2d
+
+    w = fwi2d()
+    par = w["par"]
+    nt, ns, nz, nx, nb = par["nt"], par["ns"], par["nz"], par["nx"], par["nb"]
+    rng = np.random.default_rng(12)
+    q = (20.0 + 80.0 * rng.random((nz, nx))).astype(np.float32)
+    wav = (w["amp"] * ricker_wavelet(nt, par["dt"], w["f0"], w["t0"])).astype(np.float32)
+    prob = FwiProblem(par, wav, batch=ns)
+    prob.forward(w["vtrue"])  # warm-up, acoustic
+    t = time.perf_counter()
+    prob.forward(w["vtrue"])
+    ac_s = time.perf_counter() - t
+    ac_dev_ms = prob.timing()[0]
+    prob.set_visco(q)
+    prob.forward(w["vtrue"])  # warm-up, visco
+    t = time.perf_counter()
+    d = prob.forward(w["vtrue"])
+    vs_s = time.perf_counter() - t
+    vs_dev_ms = prob.timing()[0]
+    prob.close()
+    assert np.isfinite(d).all()
+    cells = (nz + 2 * nb) * (nx + 2 * nb) * ns
+    step_us = vs_dev_ms * 1e3 / nt
+    achieved = 32.0 * cells / (step_us * 1e-6) / 1e9
+    out = {"workload": "visco-acoustic forward modelling on fwi2d 400x800 at 10 m (+60-cell sponge: 520x920), "
+                       "32 shots in one launch, nt=3000, dt=1 ms, 8 Hz, Q 20-100 seeded",
+           "shots_per_s": round(ns / vs_s, 3), "forward_s": round(vs_s, 4), "device_ms": round(vs_dev_ms, 2),
+           "step_us": round(step_us, 2), "Gcell_per_s": round(cells * nt / (vs_dev_ms * 1e-3) / 1e9, 2),
+           "algorithmic_bytes_per_step": 32.0 * cells, "achieved_GBs": round(achieved, 1),
+           "frac_hbm": round(achieved / PEAK_HBM_GBS, 4),
+           "acoustic_shots_per_s": round(ns / ac_s, 3), "acoustic_device_ms": round(ac_dev_ms, 2)}
+    out["cpu_baseline"] = None
+    if cpu_steps > 0:
+        try:
+            from oracle import refrun
+            if refrun.available():
+                from pywave_amd.fwi import make_pararray
+                p1 = dict(par)
+                p1["ns"], p1["nt"] = 1, cpu_steps
+                wav1 = (w["amp"] * ricker_wavelet(cpu_steps, par["dt"], w["f0"], w["t0"])).astype(np.float32)
+                _, el = refrun.fwimod(make_pararray(p1), w["vtrue"], q, wav1, visco=True)
+                per_shot = el * nt / cpu_steps
+                out["cpu_baseline"] = dict(
+                    value=round(1.0 / per_shot, 5), unit="shots/s", cores=1, kind="reference",
+                    sample="reference C forward_modeling (visco), one shot of the same geometry, %d of %d steps in "
+                           "%.2f s, extrapolated to %.1f s per shot; single thread, -O2 -fwrapv"
+                           % (cpu_steps, nt, el, per_shot))
+                out["speedup_vs_cpu"] = round(out["shots_per_s"] * per_shot, 1)
+        except Exception as e:  # reported, never required
+            out["cpu_baseline"] = dict(value=None, unit="shots/s", cores=1, kind="reference", sample="failed: %s" % e)
+    return out
+
+
 def tri_bench():
     """Time-reversal imaging on the 128^3 workload (+30-cell ABC: 188^3 updated, nt=2000): the
     forward modelling of three sources, then the four interleaved receiver groups back-propagated
@@ -428,10 +489,11 @@ def main():
     ap.add_argument("--no-small", action="store_true")
     ap.add_argument("--no-lstri", action="store_true")
     ap.add_argument("--no-aps", action="store_true")
+    ap.add_argument("--no-visco", action="store_true")
     ap.add_argument("--fwi-plain-shots", type=int, default=4)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (HBM counter calibration)")
-    ap.add_argument("--only", choices=["fwi", "tri", "batch", "small", "lstri", "aps"],
+    ap.add_argument("--only", choices=["fwi", "tri", "batch", "small", "lstri", "aps", "visco"],
                     help="run one secondary workload alone and print its JSON object (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -445,7 +507,8 @@ def main():
     if args.only:
         leg = {"fwi": lambda: fwi_bench(args.fwi_plain_shots), "tri": tri_bench, "batch": batch_bench,
                "small": afd2d_small_bench, "lstri": lstri_bench,
-               "aps": lambda: aps3d_bench(0 if args.no_cpu else 20)}[args.only]
+               "aps": lambda: aps3d_bench(0 if args.no_cpu else 20),
+               "visco": lambda: visco_bench(0 if args.no_cpu else 300)}[args.only]
         print(json.dumps({args.only: leg()}))
         return
 
@@ -503,6 +566,7 @@ def main():
     small = None if args.no_small else afd2d_small_bench()
     lstri = None if args.no_lstri else lstri_bench()
     aps = None if args.no_aps else aps3d_bench()
+    visco = None if args.no_visco else visco_bench()
     line = {
         "metric": "Gcell-updates/s 3D FP32 FD (% HBM roofline); FWI grad shots/s batched vs plain",
         "value": round(value, 3),
@@ -542,6 +606,7 @@ def main():
         "afd2d_small": small,
         "lstri2d": lstri,
         "aps3d": aps,
+        "visco2d": visco,
     }
     print(json.dumps(line))
 

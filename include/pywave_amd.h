@@ -140,6 +140,11 @@ int pw_fwi_set_observed(pw_fwi *p, const float *data);
 int pw_fwi_set_source_field(pw_fwi *p, const float *src);
 /* the same from device memory of the plan's precision, [ns][nt][nx][nz] */
 int pw_fwi_set_source_field_dev(pw_fwi *p, const void *d_src);
+/* visco-acoustic modelling (forward_modeling, pywave/src/wave_fwimodeling.c:116-220): memory
+ * variable with tau/taus from q (nz,nx Fortran) and the pararray's f0 (preparation,
+ * pywave/src/wave_fwiutil.c:91-97).  f32 plans; pw_fwi_forward follows it, gradient and device
+ * entry points refuse a visco plan.  NULL returns to acoustic. */
+int pw_fwi_set_visco(pw_fwi *p, const float *q);
 /* forward_modeling_a (pywave/src/wave_fwimodeling.c:21-114) */
 int pw_fwi_forward(pw_fwi *p, const float *vel, float *data_out);
 /* one gradient over all shots: mode 0 = gradient_av (pywave/src/wave_fwigradient.c:93-262:
@@ -154,7 +159,8 @@ int pw_fwi_misfit_f64(pw_fwi *p, const double *vel, double *fcost);
  * traces / cotangent [ns][nt][nr], gradient (nz*nx) */
 int pw_fwi_forward_dev(pw_fwi *p, const void *d_vel, void *d_traces);
 int pw_fwi_adjoint_dev(pw_fwi *p, const void *d_vel, const void *d_gtraces, void *d_gvel);
-/* event-timed milliseconds of the last gradient: forward sweeps, backward sweeps */
+/* event-timed milliseconds of the last gradient (forward sweeps, backward sweeps) or of the last
+ * pw_fwi_forward (its sweeps, 0) */
 int pw_fwi_timing(pw_fwi *p, double *forward_ms, double *backward_ms);
 /* fwi() L-BFGS (pywave/src/wave_fwi.c:24-119, wave_fwilbfgs.c:9-165) with f64 dot products;
  * vinv: niter*nz*nx, *nstored = iterates written (the rest untouched) */
@@ -163,6 +169,10 @@ int pw_fwi_lbfgs(pw_fwi *p, const float *vel0, int niter, float *vinv, int *nsto
 /* pfwicfun.forward_modeling_ac (pywave/src/pfwi.c:27-495): out = nt*nx*ns, (nt,nx,ns) Fortran */
 int pw_forward_modeling_ac(pw_ctx *ctx, const float *vel, const float *q, const float *wav,
                            const float *pararray, float *out, long long out_len);
+/* the forward_modelingc that pywave/pfwi.py:72 calls for media=2 (undefined in the reference's
+ * extension): visco-acoustic forward_modeling, arguments and output as pw_forward_modeling_ac */
+int pw_forward_modeling_c(pw_ctx *ctx, const float *vel, const float *q, const float *wav,
+                          const float *pararray, float *out, long long out_len);
 /* pfwicfun.fwic (pywave/src/pfwi.c:497-1028): out = nz*nx*niter, (nz,nx,niter) Fortran */
 int pw_fwic(pw_ctx *ctx, const float *vel, const float *q, const float *wav, const float *data,
             const float *pararray, float *out, long long out_len);

@@ -81,6 +81,7 @@ def lib():
         L.orc_gradient_av.argtypes = [ctypes.POINTER(OrcAcq), _f32p, _f32p, _f32p, _f32p,
                                       ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
         L.orc_fwi.argtypes = [ctypes.POINTER(OrcAcq), _f32p, _f32p, _f32p, ctypes.c_int, _f32p]
+        L.orc_fwi_forward_visco.argtypes = [ctypes.POINTER(OrcAcq), _f32p, _f32p, ctypes.c_float, _f32p, _f32p]
         L.orc_fwi.restype = ctypes.c_int
         L.orc_source_map.argtypes = [ctypes.c_int] * 6 + [_f32p]
         L.orc_gradient_smooth2.argtypes = [ctypes.c_int] * 5 + [ctypes.c_float, _f32p]
@@ -343,3 +344,16 @@ def irfft(X, n):
     out = np.zeros(n, np.float32)
     lib().orc_rfft(n, 1, X.view(np.float32), out)
     return out
+
+
+def fwi_forward_visco(pararray, vel, q, wav):
+    """visco-acoustic forward_modeling: data (nt, nx, ns); q (nz, nx) quality factor, f0 = pararray[16]."""
+    a = make_acq(pararray)
+    p = np.asarray(pararray, np.float32)
+    velf = np.ascontiguousarray(np.asarray(vel, np.float32).ravel(order="F"))
+    qf = np.ascontiguousarray(np.asarray(q, np.float32).ravel(order="F"))
+    out = np.zeros(a.ns * a.nr * a.nt, np.float32)
+    lib().orc_fwi_forward_visco(ctypes.byref(a), velf, qf, float(p[16]), np.ascontiguousarray(wav, np.float32), out)
+    res = out.reshape(a.nt, a.nr, a.ns, order="F")
+    lib().orc_fwi_release(ctypes.byref(a))
+    return res

@@ -94,6 +94,11 @@ void orc_fwi_release(orc_acq *a);
 /* forward_modeling_a: data[is][ir][it] (ns*nr*nt) */
 int orc_fwi_forward(const orc_acq *a, const float *vel, const float *wav, float *data);
 
+/* visco-acoustic forward_modeling (pywave/src/wave_fwimodeling.c:116-220): data[is][ir][it];
+ * q: quality factor (nz*nx), f0: reference frequency of the tau model */
+int orc_fwi_forward_visco(const orc_acq *a, const float *vel, const float *q, float f0, const float *wav,
+                          float *data);
+
 /* One gradient_av call.  'state' carries the first-call scaling across calls:
  * *scaling <= 0 on entry means "first call" (compute 0.1/max, store it). */
 int orc_gradient_av(const orc_acq *a, const float *vel, const float *wav, const float *data,

@@ -234,6 +234,64 @@ int orc_fwi_forward(const orc_acq *a, const float *vel, const float *wav, float 
     return 0;
 }
 
+/* visco-acoustic modelling (pywave/src/wave_fwimodeling.c:116-220): a memory variable r with
+ * tau, taus from Q and f0 (preparation, wave_fwiutil.c:91-97); r and the source term are formed in
+ * double where the reference's expressions promote, then stored in f32; r is damped once per step,
+ * the pressure levels as in the acoustic case. */
+int orc_fwi_forward_visco(const orc_acq *a, const float *vel, const float *q, float f0, const float *wav,
+                          float *data)
+{
+    const double PI = 3.14159265358979323846264338328;
+    int pnz = a->padnz, pnx = a->padnx, nt = a->nt, nr = a->nr;
+    size_t n = (size_t)pnz * pnx, nzx = (size_t)a->nz * a->nx;
+    float dx2 = a->dx * a->dx, dz2 = a->dz * a->dz, dt2 = a->dt * a->dt;
+    float idt = 1. / a->dt;
+    float *tau1 = malloc(sizeof(float) * nzx), *taus1 = malloc(sizeof(float) * nzx);
+    for (size_t i = 0; i < nzx; i++) {
+        float tmp = sqrtf(q[i] * q[i] + 1);
+        float taue = (tmp + 1) / (2. * PI * f0 * q[i]);
+        taus1[i] = (tmp - 1) / (2. * PI * f0 * q[i]);
+        tau1[i] = taue / taus1[i] - 1;
+    }
+    float *vv = malloc(sizeof(float) * n), *tau = malloc(sizeof(float) * n), *taus = malloc(sizeof(float) * n);
+    float *rr = malloc(sizeof(float) * n);
+    float *p0 = malloc(sizeof(float) * n), *p1 = malloc(sizeof(float) * n), *p2 = malloc(sizeof(float) * n);
+    float *r1 = malloc(sizeof(float) * n), *r2 = malloc(sizeof(float) * n), *term = calloc(n, sizeof(float));
+    pad2d(vel, vv, a->nz, a->nx, a->nb);
+    pad2d(tau1, tau, a->nz, a->nx, a->nb);
+    pad2d(taus1, taus, a->nz, a->nx, a->nb);
+    for (int is = 0; is < a->ns; is++) {
+        float *dd = data + (size_t)is * nr * nt;
+        memset(dd, 0, sizeof(float) * (size_t)nr * nt);
+        memset(p0, 0, sizeof(float) * n); memset(p1, 0, sizeof(float) * n); memset(p2, 0, sizeof(float) * n);
+        memset(r1, 0, sizeof(float) * n); memset(r2, 0, sizeof(float) * n);
+        int sx = a->s0_v + is * a->ds_v;
+        orc_source_map(sx, a->sz, 2, 2, pnx, pnz, rr);
+        for (int it = 0; it < nt; it++) {
+            for (int ir = 0; ir < a->nr2[is]; ir++) {
+                int rx = a->r0_v[is] + ir * a->dr_v;
+                dd[(size_t)(a->r02[is] + ir) * nt + it] = p1[(size_t)rx * pnz + a->rz];
+            }
+            laplace8(p1, term, pnx, pnz, dx2, dz2);
+            for (size_t j = 0; j < n; j++) {
+                double num = (double)(tau[j] / taus[j] * term[j]) + ((double)idt - 0.5 / (double)taus[j]) * r1[j];
+                r2[j] = (float)(num / ((double)idt + 0.5 / (double)taus[j]));
+                double t = (double)term[j] * (1. + (double)tau[j]) - (double)(r2[j] + r1[j]) * 0.5;
+                term[j] = (float)(t + (double)(rr[j] * wav[it]));
+                p2[j] = 2 * p1[j] - p0[j] + vv[j] * vv[j] * dt2 * term[j];
+            }
+            float *t = p0; p0 = p1; p1 = p2; p2 = t;
+            t = r1; r1 = r2; r2 = t;
+            sponge(p0, a->bc, pnx, pnz, a->nb);
+            sponge(p1, a->bc, pnx, pnz, a->nb);
+            sponge(r1, a->bc, pnx, pnz, a->nb);
+        }
+    }
+    free(tau1); free(taus1); free(vv); free(tau); free(taus); free(rr);
+    free(p0); free(p1); free(p2); free(r1); free(r2); free(term);
+    return 0;
+}
+
 static void residual_weights(float *w, int nt, int nr, int wtn1, int wtn2, int woffn1, int woffn2)
 {
     float sw[10];

@@ -12,6 +12,8 @@
  *   afd2d : pywave/src/afd.c:386-546  -> fdm2d  (pywave/src/wave_fdm.c:397)
  *   afd3d : pywave/src/afd.c:95-284   -> fdm    (pywave/src/wave_fdm.c:28)
  *   fwimod: pywave/src/pfwi.c:27-495  -> forward_modeling_a (wave_fwimodeling.c:21)
+ *   fwimodv: the same set-up -> forward_modeling (wave_fwimodeling.c:116), the visco-acoustic
+ *            modelling pywave/pfwi.py:72 means to reach (media=2)
  *   fwigrad: pywave/src/pfwi.c:497-1028 setup -> gradient_init/gradient_av
  *            (wave_fwigradient.c:20,93), called 1 or 2 times
  *   fwi   : pywave/src/pfwi.c:497-1028 -> fwi (wave_fwi.c:24)
@@ -303,7 +305,8 @@ static int run_fwimod(int argc, char **argv)
     fill_acpar(&a, soupar, acpar);
     preparation(vel, q, wav, acpar, soupar, array);
     double t_start = now_s();
-    forward_modeling_a(data, soupar, acpar, array, false);
+    if (!strcmp(argv[1], "fwimodv")) forward_modeling(data, soupar, acpar, array, false);
+    else forward_modeling_a(data, soupar, acpar, array, false);
     fprintf(stderr, "PWREF_ELAPSED_S=%.6f\n", now_s() - t_start);
     write_f32(argv[6], data[0][0], (size_t)a.nt * a.nx * a.ns);
     return 0;
@@ -522,7 +525,7 @@ int main(int argc, char **argv)
     /* every allocation through mmap: fresh, zero-filled pages */
     mallopt(M_MMAP_THRESHOLD, 0);
     mallopt(M_MMAP_MAX, 1 << 30);
-    if (argc < 2) { fprintf(stderr, "usage: pwref {afd2d|afd3d|aps2d|aps3d|ricker|fwimod|fwigrad|fwi|pfwigrad|pfwi|lstri} ...\n"); return 2; }
+    if (argc < 2) { fprintf(stderr, "usage: pwref {afd2d|afd3d|aps2d|aps3d|ricker|fwimod|fwimodv|fwigrad|fwi|pfwigrad|pfwi|lstri} ...\n"); return 2; }
     /* the engines print progress on stdout; keep it off the caller's stdout */
     if (!getenv("PWREF_KEEP_STDOUT")) { if (!freopen("/dev/null", "w", stdout)) return 4; }
     if (!strcmp(argv[1], "afd2d")) return run_afd2d(argc, argv);
@@ -530,7 +533,7 @@ int main(int argc, char **argv)
     if (!strcmp(argv[1], "aps2d")) { g_ps = 1; return run_afd2d(argc, argv); }
     if (!strcmp(argv[1], "aps3d")) { g_ps = 1; return run_afd3d(argc, argv); }
     if (!strcmp(argv[1], "ricker")) return run_ricker(argc, argv);
-    if (!strcmp(argv[1], "fwimod")) return run_fwimod(argc, argv);
+    if (!strcmp(argv[1], "fwimod") || !strcmp(argv[1], "fwimodv")) return run_fwimod(argc, argv);
     if (!strcmp(argv[1], "fwigrad")) return run_fwigrad(argc, argv);
     if (!strcmp(argv[1], "fwi")) return run_fwi(argc, argv);
     if (!strcmp(argv[1], "pfwigrad")) return run_pfwigrad(argc, argv);

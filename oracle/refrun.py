@@ -79,12 +79,13 @@ def ricker(nt, dt, f0, t0, A):
         return np.fromfile(fo, np.float32)
 
 
-def fwimod(pararray, vel, q, wav):
-    """forward_modeling_ac: returns (data (nt,nx,ns), elapsed)."""
+def fwimod(pararray, vel, q, wav, visco=False):
+    """forward_modeling_ac (visco: the visco-acoustic forward_modeling): returns
+    (data (nt,nx,ns), elapsed)."""
     p = np.asarray(pararray, np.float32)
     nz, nx, nt, ns = int(p[0]), int(p[1]), int(p[6]), int(p[10])
     with tempfile.TemporaryDirectory() as tmp:
-        args = ["fwimod", _put(tmp, "par.f32", p), _put(tmp, "vel.f32", np.asarray(vel).ravel(order="F")),
+        args = ["fwimodv" if visco else "fwimod", _put(tmp, "par.f32", p), _put(tmp, "vel.f32", np.asarray(vel).ravel(order="F")),
                 _put(tmp, "q.f32", np.asarray(q).ravel(order="F")), _put(tmp, "wav.f32", wav),
                 os.path.join(tmp, "out.f32")]
         el = _run(args, tmp)

@@ -66,6 +66,8 @@ SIGNATURES = {
     "pw_fwi_set_wavelet": (c_int, [c_vp, c_vp]),
     "pw_fwi_set_observed": (c_int, [c_vp, c_vp]),
     "pw_fwi_set_source_field": (c_int, [c_vp, c_vp]),
+    "pw_fwi_set_visco": (c_int, [c_vp, c_vp]),
+    "pw_forward_modeling_c": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_ll]),
     "pw_fwi_set_source_field_dev": (c_int, [c_vp, c_vp]),
     "pw_fwi_forward": (c_int, [c_vp, c_vp, c_vp]),
     "pw_fwi_gradient": (c_int, [c_vp, c_vp, c_int, c_vp, ctypes.POINTER(c_double), ctypes.POINTER(c_float)]),

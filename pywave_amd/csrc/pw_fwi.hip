@@ -80,6 +80,13 @@ struct StepArgs {
     long long fulls;
     const T* fsrc;      // passive source field of this step on the interior, [slot][nx][nz], or null
     long long fss;      // its slot stride (nt*nz*nx)
+    // visco-acoustic modelling: tau, taus on the padded grid, memory variable r (damped once per
+    // step) read from r1 and written to r2, per slot [slot][fs]; idt = 1/dt in f32
+    const float* tau;
+    const float* taus;
+    const float* r1;
+    float* r2;
+    float idt;
     int it;
 };
 
@@ -105,7 +112,7 @@ using pwbuf::bsto;
 // One 64 x XC tile of one shot per block (one wave), streamed along x with a 9-column register
 // queue (radius 4) and an LDS z row; the loads of column x+1 (previous level, model, z halo,
 // sponge pair) and the centre of column x+5 are in flight while column x is computed.
-template <typename T>
+template <typename T, bool VISCO = false>
 __global__ __launch_bounds__(64) void k_fwd(FGeo<T> g, StepArgs<T> a, int ntz)
 {
     using C = Coef<T>;
@@ -142,6 +149,15 @@ __global__ __launch_bounds__(64) void k_fwd(FGeo<T> g, StepArgs<T> a, int ntz)
     const unsigned ofz = fzin ? (unsigned)(z - g.nb) * E : kOOB;
     auto fxin = [&](int x) { return x >= g.nb && x < g.nb + g.nx; };
     auto ldf = [&](int x) { return bldo<T>(rS, fxin(x) ? ofz : kOOB, fxin(x) ? (unsigned)((x - g.nb) * g.nz) * E : 0u); };
+    // visco-acoustic fields (f32 only)
+    const auto rTa = win(VISCO ? a.tau - bias : nullptr, VISCO ? wb / sizeof(T) * 4 : 0u);
+    const auto rTs = win(VISCO ? a.taus - bias : nullptr, VISCO ? wb / sizeof(T) * 4 : 0u);
+    const auto rR1 = win(VISCO ? a.r1 + slot * g.fs - bias : nullptr, VISCO ? wb / sizeof(T) * 4 : 0u);
+    const auto rR2 = win(VISCO ? a.r2 + slot * g.fs - bias : nullptr, VISCO ? wb / sizeof(T) * 4 : 0u);
+    const unsigned oz4 = zok ? (unsigned)(bias + z) * 4 : kOOB, oc4 = (unsigned)L * 4;
+    auto ldv = [&](__amdgpu_buffer_rsrc_t r, int x, bool ok) {
+        return bldo<float>(r, ok ? oz4 : kOOB, ok ? (unsigned)x * oc4 : 0u);
+    };
     T q[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
@@ -152,6 +168,12 @@ __global__ __launch_bounds__(64) void k_fwd(FGeo<T> g, StepArgs<T> a, int ntz)
     T mv = ldc(rM, oz, x0, true);
     T hz = ldc(rX, ozh, x0, true);
     T fv = ldf(x0);
+    float tav = 0.f, tsv = 1.f, r1v = 0.f;
+    if constexpr (VISCO) {
+        tav = ldv(rTa, x0, true);
+        tsv = ldv(rTs, x0, true);
+        r1v = ldv(rR1, x0, true);
+    }
     T xl = a.sfx[2 * x0], xh = a.sfx[2 * x0 + 1];
     int buf = 0;
     for (int x = x0; x < x1; ++x) {
@@ -162,6 +184,12 @@ __global__ __launch_bounds__(64) void k_fwd(FGeo<T> g, StepArgs<T> a, int ntz)
         const T nmv = ldc(rM, oz, xn, nv);
         const T nhz = ldc(rX, ozh, xn, nv);
         const T nfv = ldf(xn);
+        float ntav = 0.f, ntsv = 1.f, nr1v = 0.f;
+        if constexpr (VISCO) {
+            ntav = ldv(rTa, xn, nv);
+            ntsv = ldv(rTs, xn, nv);
+            nr1v = ldv(rR1, xn, nv);
+        }
         const int xs = xn < g.pnx ? xn : g.pnx - 1;
         const T nxl = a.sfx[2 * xs], nxh = a.sfx[2 * xs + 1];
         row[buf][lane + 4] = q[4];
@@ -195,8 +223,20 @@ __global__ __launch_bounds__(64) void k_fwd(FGeo<T> g, StepArgs<T> a, int ntz)
                         C::K4 * (R[4] + R[-4])) / g.dz2;
             }
             const int px = x - sh.sx0, pz = z - sh.sz0;
-            if (px >= 0 && px < sh.pw && pz >= 0 && pz < sh.ph)
-                term = term + a.patch[slot * PMAX * PMAX + px * PMAX + pz] * wv;
+            const bool inp = px >= 0 && px < sh.pw && pz >= 0 && pz < sh.ph;
+            if constexpr (VISCO) {
+                // forward_modeling (wave_fwimodeling.c:186-192): r2 and the source term formed in
+                // double where the reference's expressions promote, stored in f32
+                const float tf = (float)term;
+                const double num = (double)(tav / tsv * tf) + ((double)a.idt - 0.5 / (double)tsv) * (double)r1v;
+                const float r2v = (float)(num / ((double)a.idt + 0.5 / (double)tsv));
+                const float sv = (inp ? (float)a.patch[slot * PMAX * PMAX + px * PMAX + pz] : 0.f) * (float)wv;
+                const double tt = (double)tf * (1. + (double)tav) - (double)(r2v + r1v) * 0.5;
+                term = (T)(float)(tt + (double)sv);
+                bsto(rR2, oz4, (unsigned)x * oc4, (float)sponge((T)r2v, zl, zh, xl, xh));
+            } else {
+                if (inp) term = term + a.patch[slot * PMAX * PMAX + px * PMAX + pz] * wv;
+            }
             if (fzin && fxin(x)) term = term + fv;
             const T p0 = sponge(yv, zl, zh, xl, xh);
             T p2 = 2 * p1 - p0 + mv * term;
@@ -206,6 +246,7 @@ __global__ __launch_bounds__(64) void k_fwd(FGeo<T> g, StepArgs<T> a, int ntz)
         for (int k = 0; k < 8; ++k) q[k] = q[k + 1];
         q[8] = nq;
         yv = nyv; mv = nmv; hz = nhz; fv = nfv; xl = nxl; xh = nxh;
+        tav = ntav; tsv = ntsv; r1v = nr1v;
         buf ^= 1;
     }
 }
@@ -841,6 +882,8 @@ struct pw_fwi {
     bool passive = false;   // shots are the source field (gradient_pas_av), not point sources
     int cur_b0 = 0;         // first shot of the batch set by set_batch
     std::string active_err; // why the point-source geometry is unusable (empty: usable)
+    bool visco = false;     // visco-acoustic modelling (forward only)
+    float *tau_pad = nullptr, *taus_pad = nullptr, *R0 = nullptr, *R1 = nullptr;
     std::vector<ShotInfo> shot_active;
     float* tmpv = nullptr;
     float* work = nullptr;  // smoothing workspace
@@ -906,7 +949,8 @@ void fwi_free(pw_fwi* p)
 {
     void* bufs[] = {p->vel, p->vpad, p->m, p->sfz, p->sfx, p->wav, p->X, p->Y, p->A, p->B, p->rec, p->res,
                     p->obs, p->W, p->ckpt, p->ckpt_snap, p->seg, p->gm, p->gslot, p->gtot, p->patch,
-                    p->tmpv, p->work, p->shots, p->slot_shot, p->partial, p->dscalar, p->gfold, p->fsrc};
+                    p->tmpv, p->work, p->shots, p->slot_shot, p->partial, p->dscalar, p->gfold, p->fsrc,
+                    p->tau_pad, p->taus_pad, p->R0, p->R1};
     for (void* b : bufs)
         if (b) hipFree(b);
 }
@@ -1054,6 +1098,22 @@ int fill(pw_fwi* p, void* ptr, long long n, T v)
     return PW_OK;
 }
 
+// three timing events, released on every return path
+struct EventSet {
+    hipEvent_t e[3] = {nullptr, nullptr, nullptr};
+    hipError_t create()
+    {
+        for (auto& x : e)
+            if (hipError_t r = hipEventCreate(&x)) return r;
+        return hipSuccess;
+    }
+    ~EventSet()
+    {
+        for (auto x : e)
+            if (x) hipEventDestroy(x);
+    }
+};
+
 // forward sweep over one batch: traces into rec; checkpoints (and the last segment's
 // snapshots when 'store_last') when 'ckpt'
 template <typename T>
@@ -1070,6 +1130,12 @@ int forward_batch(pw_fwi* p, int nb_, bool ckpt, int mode, bool store_last)
     if (rc) return rc;
     T* X = (T*)p->X;
     T* Y = (T*)p->Y;
+    const bool visco = p->visco && sizeof(T) == 4;
+    float* Ra = p->R0;
+    float* Rb = p->R1;
+    if (visco) {
+        if ((rc = fill<float>(p, Ra, fld, 0.f)) || (rc = fill<float>(p, Rb, fld, 0.f))) return rc;
+    }
     const int I = a.interval, K = p->K;
     const int last = p->nseg - 1, a_last = last * K;
     for (int it = 0; it < a.nt; ++it) {
@@ -1077,6 +1143,10 @@ int forward_batch(pw_fwi* p, int nb_, bool ckpt, int mode, bool store_last)
         s_.X = X; s_.Y = Y; s_.it = it;
         s_.fsrc = step_src<T>(p, it);
         s_.rec = (T*)p->rec;
+        if (visco) {
+            s_.tau = p->tau_pad; s_.taus = p->taus_pad; s_.r1 = Ra; s_.r2 = Rb;
+            s_.idt = (float)(1. / (double)p->acq.dt);
+        }
         if (ckpt) {
             const int sgi = it / K;
             if (it % K == 0) {
@@ -1091,9 +1161,11 @@ int forward_batch(pw_fwi* p, int nb_, bool ckpt, int mode, bool store_last)
                 else if (it % I == 0) s_.snap = (T*)p->seg + (size_t)((it - a_last) / I + 1) * p->nslot * p->nzx;
             }
         }
-        hipLaunchKernelGGL(k_fwd<T>, dim3(ntz * nxc, nb_), dim3(64), 0, s, g, s_, ntz);
+        if (visco) hipLaunchKernelGGL((k_fwd<T, true>), dim3(ntz * nxc, nb_), dim3(64), 0, s, g, s_, ntz);
+        else hipLaunchKernelGGL((k_fwd<T, false>), dim3(ntz * nxc, nb_), dim3(64), 0, s, g, s_, ntz);
         PW_HIPS(hipGetLastError());
         std::swap(X, Y);
+        if (visco) std::swap(Ra, Rb);
     }
     return PW_OK;
 }
@@ -1222,14 +1294,16 @@ int gradient_all(pw_fwi* p, int mode, double* fcost)
     hipStream_t s = p->ctx->s();
     if (!p->have_wav || !p->have_obs) PW_FAIL(p->ctx, PW_ERR_STATE, "gradient: wavelet or observed data not set");
     if (mode == 0 && p->prec != 4) PW_FAIL(p->ctx, PW_ERR_ARG, "gradient: the reference mode runs in f32");
+    if (p->visco) PW_FAIL(p->ctx, PW_ERR_ARG, "gradient: visco-acoustic plans model only (pw_fwi_set_visco(p, NULL))");
     const long long fld = (long long)p->nslot * p->fs;
     int rc = 0;
     double fc = 0.0;
     if (mode == 0) rc = fill<float>(p, p->gtot, p->nzx, 0.f);
     else rc = fill<T>(p, p->gtot, p->nzx, T(0));
     if (rc) return rc;
-    hipEvent_t e0, e1, e2;
-    PW_HIPS(hipEventCreate(&e0)); PW_HIPS(hipEventCreate(&e1)); PW_HIPS(hipEventCreate(&e2));
+    EventSet ev;
+    PW_HIPS(ev.create());
+    hipEvent_t e0 = ev.e[0], e1 = ev.e[1], e2 = ev.e[2];
     float fwd_ms = 0, bwd_ms = 0;
     const bool store_last = true;
     for (int b0 = 0; b0 < a.ns; b0 += p->nslot) {
@@ -1279,7 +1353,6 @@ int gradient_all(pw_fwi* p, int mode, double* fcost)
         fwd_ms += m1;
         bwd_ms += m2;
     }
-    hipEventDestroy(e0); hipEventDestroy(e1); hipEventDestroy(e2);
     p->t_forward_ms = fwd_ms;
     p->t_backward_ms = bwd_ms;
     *fcost = fc;
@@ -1600,6 +1673,54 @@ int pw_fwi_set_source_field_dev(pw_fwi* p, const void* d_src)
     return use_source_field(p, true);
 }
 
+// visco-acoustic modelling (forward_modeling, pywave/src/wave_fwimodeling.c:116-220) with the
+// quality factor q (nz, nx) Fortran order and the pararray's f0; NULL returns to acoustic.  Only
+// pw_fwi_forward follows it (f32 plans); gradients refuse a visco plan.
+int pw_fwi_set_visco(pw_fwi* p, const float* q)
+{
+    if (!p) return PW_ERR_ARG;
+    p->dev_ckpt_valid = false;
+    if (!q) {
+        p->visco = false;
+        return PW_OK;
+    }
+    if (p->prec != 4) PW_FAIL(p->ctx, PW_ERR_ARG, "pw_fwi_set_visco: visco-acoustic modelling runs in f32");
+    const Acq& a = p->acq;
+    const long long nzx = p->nzx;
+    // tau, taus (preparation, pywave/src/wave_fwiutil.c:91-97): f32 with the 2 pi f0 q product in double
+    const double PI = 3.14159265358979323846264338328;
+    std::vector<float> ta(nzx), ts(nzx);
+    for (long long i = 0; i < nzx; ++i) {
+        const float tmp = sqrtf(q[i] * q[i] + 1);
+        const float taue = (tmp + 1) / (2. * PI * a.f0 * q[i]);
+        ts[i] = (tmp - 1) / (2. * PI * a.f0 * q[i]);
+        ta[i] = taue / ts[i] - 1;
+    }
+    // padded with the edge values (pad2d, wave_fwiutil.c:123-147), z stride ldz
+    std::vector<float> tap((size_t)p->fs, 0.f), tsp((size_t)p->fs, 1.f);
+    for (int ix = 0; ix < a.padnx; ++ix) {
+        const int sx = std::min(std::max(ix - a.nb, 0), a.nx - 1);
+        for (int iz = 0; iz < a.padnz; ++iz) {
+            const int sz = std::min(std::max(iz - a.nb, 0), a.nz - 1);
+            tap[(size_t)ix * p->ldz + iz] = ta[(size_t)sx * a.nz + sz];
+            tsp[(size_t)ix * p->ldz + iz] = ts[(size_t)sx * a.nz + sz];
+        }
+    }
+    int rc = PW_OK;
+    const long long fld = (long long)p->nslot * p->fs;
+    if (!p->tau_pad) rc = dmalloc(p, (void**)&p->tau_pad, p->fs * 4);
+    if (!rc && !p->taus_pad) rc = dmalloc(p, (void**)&p->taus_pad, p->fs * 4);
+    if (!rc && !p->R0) rc = dmalloc(p, (void**)&p->R0, fld * 4);
+    if (!rc && !p->R1) rc = dmalloc(p, (void**)&p->R1, fld * 4);
+    if (rc) return rc;
+    hipStream_t s = p->ctx->s();
+    PW_HIPS(hipMemcpyAsync(p->tau_pad, tap.data(), tap.size() * 4, hipMemcpyHostToDevice, s));
+    PW_HIPS(hipMemcpyAsync(p->taus_pad, tsp.data(), tsp.size() * 4, hipMemcpyHostToDevice, s));
+    PW_HIPS(hipStreamSynchronize(s));
+    p->visco = true;
+    return PW_OK;
+}
+
 // forward_modeling_a: data_out (nt, nr, ns) Fortran order
 int pw_fwi_forward(pw_fwi* p, const float* vel, float* data_out)
 {
@@ -1612,11 +1733,20 @@ int pw_fwi_forward(pw_fwi* p, const float* vel, float* data_out)
     const long long per = (long long)a.nt * a.nr;
     std::vector<float> t((size_t)per * p->nslot);
     std::vector<double> td;
+    EventSet ev;
+    PW_HIPS(ev.create());
+    float fwd_ms = 0;
     for (int b0 = 0; b0 < a.ns; b0 += p->nslot) {
         const int nb_ = std::min(p->nslot, a.ns - b0);
         if ((rc = set_batch(p, b0, nb_))) return rc;
+        PW_HIPS(hipEventRecord(ev.e[0], s));
         rc = p->prec == 4 ? forward_batch<float>(p, nb_, false, 0, false) : forward_batch<double>(p, nb_, false, 0, false);
         if (rc) return rc;
+        PW_HIPS(hipEventRecord(ev.e[1], s));
+        PW_HIPS(hipEventSynchronize(ev.e[1]));
+        float m = 0;
+        PW_HIPS(hipEventElapsedTime(&m, ev.e[0], ev.e[1]));
+        fwd_ms += m;
         if (p->prec == 4) {
             PW_HIPS(hipMemcpyAsync(t.data(), p->rec, (size_t)nb_ * per * 4, hipMemcpyDeviceToHost, s));
             PW_HIPS(hipStreamSynchronize(s));
@@ -1633,6 +1763,8 @@ int pw_fwi_forward(pw_fwi* p, const float* vel, float* data_out)
                 for (int ir = 0; ir < a.nr; ++ir) d[(size_t)ir * a.nt + it] = src[(size_t)it * a.nr + ir];
         }
     }
+    p->t_forward_ms = fwd_ms;
+    p->t_backward_ms = 0;
     return PW_OK;
 }
 
@@ -1685,6 +1817,7 @@ int pw_fwi_gradient_f64(pw_fwi* p, const double* vel, int mode, double* grad, do
 int pw_fwi_misfit_f64(pw_fwi* p, const double* vel, double* fcost)
 {
     if (!p || !vel || !fcost) return PW_ERR_ARG;
+    if (p->visco) PW_FAIL(p->ctx, PW_ERR_ARG, "misfit: visco-acoustic plans model only");
     if (!p->have_wav || !p->have_obs) PW_FAIL(p->ctx, PW_ERR_STATE, "misfit: wavelet or observed data not set");
     hipStream_t s = p->ctx->s();
     int rc;
@@ -1713,6 +1846,7 @@ int pw_fwi_misfit_f64(pw_fwi* p, const double* vel, double* fcost)
 int pw_fwi_forward_dev(pw_fwi* p, const void* d_vel, void* d_traces)
 {
     if (!p || !d_vel || !d_traces) return PW_ERR_ARG;
+    if (p->visco) PW_FAIL(p->ctx, PW_ERR_ARG, "forward_dev: visco-acoustic modelling is host-only (pw_fwi_forward)");
     if (!p->have_wav) PW_FAIL(p->ctx, PW_ERR_STATE, "forward: wavelet not set");
     const Acq& a = p->acq;
     hipStream_t s = p->ctx->s();
@@ -1976,6 +2110,26 @@ int pw_forward_modeling_ac(pw_ctx* ctx, const float* vel, const float* q, const 
     pw_fwi* p = pw_fwi_create(ctx, pararray, 0, 4);
     if (!p) return PW_ERR_ARG;
     int rc = pw_fwi_set_wavelet(p, wav);
+    if (!rc) rc = pw_fwi_forward(p, vel, out);
+    pw_fwi_destroy(p);
+    return rc;
+}
+
+// visco-acoustic counterpart of forward_modeling_ac: the forward_modelingc that pywave/pfwi.py:72
+// calls for media=2 (the reference's extension does not define it) on forward_modeling
+// (pywave/src/wave_fwimodeling.c:116-220); q is the quality factor
+int pw_forward_modeling_c(pw_ctx* ctx, const float* vel, const float* q, const float* wav,
+                          const float* pararray, float* out, long long out_len)
+{
+    if (!ctx || !vel || !q || !wav || !pararray || !out) return PW_ERR_ARG;
+    Acq a;
+    std::string err;
+    if (acq_from_pararray(pararray, a, err)) PW_FAIL(ctx, PW_ERR_ARG, err);
+    if (out_len != (long long)a.nt * a.nx * a.ns) PW_FAIL(ctx, PW_ERR_ARG, "forward_modelingc: output length");
+    pw_fwi* p = pw_fwi_create(ctx, pararray, 0, 4);
+    if (!p) return PW_ERR_ARG;
+    int rc = pw_fwi_set_wavelet(p, wav);
+    if (!rc) rc = pw_fwi_set_visco(p, q);
     if (!rc) rc = pw_fwi_forward(p, vel, out);
     pw_fwi_destroy(p);
     return rc;

@@ -100,6 +100,20 @@ class FwiProblem:
             raise ValueError("source field must hold nz*nx*nt*ns samples")
         N.check(self._L.pw_fwi_set_source_field(self._p, N.ptr(f)), self._ctx, "set_source_field")
 
+    def set_visco(self, q):
+        """Visco-acoustic modelling (forward_modeling, pywave/src/wave_fwimodeling.c:116-220) with the
+        quality factor q (nz, nx) and the pararray's f0: forward() then carries the memory variable.
+        f32 plans only; gradient(), misfit() and the device entry points refuse a visco plan.  None
+        returns to acoustic modelling."""
+        if q is None:
+            N.check(self._L.pw_fwi_set_visco(self._p, None), self._ctx, "set_visco")
+            return
+        qq = np.asarray(q)
+        if qq.shape != (self.nz, self.nx):
+            raise ValueError("q must be (nz, nx) = (%d, %d)" % (self.nz, self.nx))
+        qq = np.ascontiguousarray(qq.astype(np.float32).ravel(order="F"))
+        N.check(self._L.pw_fwi_set_visco(self._p, N.ptr(qq)), self._ctx, "set_visco")
+
     def set_checkpointing(self, steps):
         N.check(self._L.pw_fwi_set_checkpointing(self._p, int(steps)), self._ctx, "set_checkpointing")
 

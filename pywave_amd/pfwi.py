@@ -17,8 +17,10 @@ mode 4  inv and not onlysrc: passive FWI for velocity with the known source fiel
         returns (data (nt, nx, ns), [], [], [], [])
         The branch tests compare with True as the reference does (par['inv'] == True).
 other   prints "Wrong mode" and returns 0
-media != 1 in mode 1 reaches an undefined function in the reference (NameError); the same error
-is raised here.  Units follow the reference demos: km and km/s (the optimiser clips to [0, 10]).
+media != 1 in mode 1 calls forward_modelingc (pywave/pfwi.py:72), which the reference's extension
+does not define (NameError there); here it is the visco-acoustic forward_modeling
+(pywave/src/wave_fwimodeling.c:116-220) with q the quality factor and f0 the reference
+frequency, returning data (nt, nx, ns) as media=1 does.  Units follow the reference demos: km and km/s (the optimiser clips to [0, 10]).
 Slots of vinv after an early stop (line-search failure or convergence) are zero; the reference
 leaves them uninitialised.
 """
@@ -38,13 +40,15 @@ def pfwi(vel, q, wav, src, data=None, mode=1, media=1, inv=0, verb=1, nb=60, coe
         q = q.flatten(order="F").astype(np.float32)
         wav = wav.flatten(order="F").astype(np.float32)
         pararray = make_pararray(par)
-        if media != 1:
-            raise NameError("name 'forward_modelingc' is not defined")
         nt, nx, ns = int(par["nt"]), int(par["nx"]), int(par["ns"])
         out = np.empty(nt * nx * ns, np.float32)
         L, ctx = N.lib(), N.context()
-        N.check(L.pw_forward_modeling_ac(ctx, N.ptr(vel), N.ptr(q), N.ptr(wav), N.ptr(pararray), N.ptr(out),
-                                         out.size), ctx, "forward_modeling_ac")
+        if media == 1:
+            N.check(L.pw_forward_modeling_ac(ctx, N.ptr(vel), N.ptr(q), N.ptr(wav), N.ptr(pararray), N.ptr(out),
+                                             out.size), ctx, "forward_modeling_ac")
+        else:
+            N.check(L.pw_forward_modeling_c(ctx, N.ptr(vel), N.ptr(q), N.ptr(wav), N.ptr(pararray), N.ptr(out),
+                                            out.size), ctx, "forward_modelingc")
         data = out.reshape(par["nt"], par["nx"], par["ns"], order="F")
         vinv = []; grad = []; mwt = []; src = []
     elif mode == 2:

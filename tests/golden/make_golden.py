@@ -154,6 +154,11 @@ def fwi_cases():
     save("fwi_small", par=par, par4=par4, vtrue=vtrue, v0=v0, q=q, wav=wav, data=data,
          grad1=grads[0], grad2=grads[1], fcost=fc, grad_int4=grads4[0], fcost_int4=fc4, vinv=vinv)
 
+    # visco-acoustic modelling (forward_modeling, the media=2 path) on the same acquisition
+    qv = (20 + 40 * np.random.default_rng(17).random((nz, nx))).astype(np.float32)
+    data_visco, _ = refrun.fwimod(par, vtrue, qv, wav, visco=True)
+    save("fwi_visco", par=par, vtrue=vtrue, q=qv, wav=wav, data=data_visco)
+
     # marine-style acquisition (acqui_type 2), sources marching across the model
     par2 = np.array([30, 48, 0.01, 0.01, 0, 0, 300, 0.001, 0, 0, 2, 0.15, 0.0, 5, 16, 0.01, 10, 2, 1, 1],
                     np.float32)

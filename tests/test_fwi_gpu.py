@@ -71,8 +71,34 @@ def test_pfwi_dropin_modes(small):
     ref = small["vinv"]
     # f64 dot products in the optimiser: same trajectory to f32 round-off of the models
     assert np.abs(vinv - ref).max() < 1e-4
-    with pytest.raises(NameError):
-        pywave_amd.pfwi(small["vtrue"], q, small["wav"], None, mode=1, media=2, par=par)
+
+
+def test_visco_forward_bit_exact(golden):
+    # forward_modeling (wave_fwimodeling.c:116-220): memory variable r, tau/taus from Q and f0
+    g = golden("fwi_visco")
+    for batch in (1, 0):
+        p = FwiProblem(g["par"], g["wav"], data=g["data"], batch=batch)
+        p.set_visco(g["q"])
+        assert np.array_equal(p.forward(g["vtrue"]), g["data"]), batch
+        # back to acoustic on the same plan, and visco again
+        p.set_visco(None)
+        ac = p.forward(g["vtrue"])
+        assert rel_l2(ac, g["data"]) > 0.05
+        p.set_visco(g["q"])
+        assert np.array_equal(p.forward(g["vtrue"]), g["data"]), batch
+    with pytest.raises(ValueError, match="visco"):
+        p.gradient(g["vtrue"])
+
+
+def test_pfwi_media2_forward_modelingc(golden):
+    # the media=2 branch of mode 1 (pywave/pfwi.py:72) runs the visco-acoustic modelling
+    g = golden("fwi_visco")
+    par = dict(zip(["nz", "nx", "dz", "dx", "oz", "ox", "nt", "dt", "ot", "inv", "ns", "ds", "os", "sz", "nb",
+                    "coef", "f0", "acqui_type", "interval", "niter"], [float(x) for x in g["par"]]))
+    for k in ["nz", "nx", "nt", "ns", "nb", "acqui_type", "interval", "niter"]:
+        par[k] = int(par[k])
+    data, vinv, grad, src, mwt = pywave_amd.pfwi(g["vtrue"], g["q"], g["wav"], None, mode=1, media=2, par=par)
+    assert np.array_equal(data, g["data"]) and vinv == [] and grad == []
 
 
 def test_lbfgs_decreases_misfit(small):

@@ -433,3 +433,23 @@ def test_aps3d_bit_exact(golden):
     assert np.array_equal(out[:nt * nx * ny].reshape(nt, nx, ny, order="F"), d.transpose(2, 1, 0))
     w = out[nt * nx * ny:].reshape(nz, nx, ny, -1, order="F")
     assert all(np.array_equal(w[..., k], snaps[k].transpose(2, 1, 0)) for k in range(w.shape[-1]))
+
+
+# ------------------------------------------------------------------ visco-acoustic modelling
+
+def test_visco_forward_bit_exact(golden):
+    g = golden("fwi_visco")
+    assert np.array_equal(O.fwi_forward_visco(g["par"], g["vtrue"], g["q"], g["wav"]), g["data"])
+
+
+def test_visco_forward_tends_to_acoustic_at_high_q(golden):
+    # tau = taue/taus - 1 = 2/(sqrt(Q^2+1) - 1) -> 0 as Q grows: the memory variable vanishes and
+    # the visco-acoustic data approach the acoustic data; at Q = 20-60 they differ by ~10%
+    g = golden("fwi_small")
+    q_hi = np.full(g["vtrue"].shape, 1e5, np.float32)
+    d_hi = O.fwi_forward_visco(g["par"], g["vtrue"], q_hi, g["wav"])
+    assert rel_l2(d_hi, g["data"]) < 1e-3
+    v = golden("fwi_visco")
+    assert rel_l2(v["data"], g["data"]) > 0.05
+    # attenuation: the low-Q data carry less energy than the acoustic data
+    assert np.linalg.norm(v["data"]) < np.linalg.norm(g["data"])
