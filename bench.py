@@ -93,6 +93,19 @@ def fwi_bench(plain_shots):
     out["batched_s_per_gradient"] = round(tb, 4)
     out["misfit"] = f
     out["gradient_norm"] = float(torch.linalg.vector_norm(g.double()).item())
+    # the reference's own gradient (gradient_av: imaging condition, windowed residual, first-call
+    # scaling, smoothing) that pfwi mode 2 (fwic) iterates on, every shot in one batch
+    ref = FwiProblem(par, wav, data=obs, batch=ns, checkpoint=50)
+    ref.gradient(w["v0"], "reference")  # warm-up (sets the first-call scaling)
+    t = time.perf_counter()
+    ref.gradient(w["v0"], "reference")
+    tr = time.perf_counter() - t
+    rf, rb = ref.timing()
+    ref.close()
+    out["reference_gradient_shots_per_s"] = round(ns / tr, 3)
+    out["reference_gradient_s"] = round(tr, 4)
+    out["reference_gradient_forward_ms"] = round(rf, 1)
+    out["reference_gradient_backward_ms"] = round(rb, 1)
     out["cpu_baseline"] = fwi_cpu_baseline(par, w)
     if plain_shots <= 0:
         return out
@@ -319,7 +332,9 @@ def visco_bench(cpu_steps=300):
     the media=2 modelling of pfwi) on the FWI workload's 400x800, 32-shot, nt=3000 acquisition
     with a seeded Q of 20-100: all 32 shots per launch.  Per cell and step the kernel reads
     p_n, p_{n-1}, v^2 dt^2, tau, taus, r_n and writes p_{n+1}, r_{n+1}: 32 algorithmic bytes.
-    The acoustic forward of the same plan is timed beside it."""
+    The acoustic forward of the same plan is timed beside it, and the visco-acoustic imaging
+    gradient (gradient_v, pywave/src/wave_fwigradient.c:264-452) of all 32 shots with checkpoints
+    every 50 steps."""
     from pywave_amd import FwiProblem
     from pywave_amd.workloads import fwi2d
 
@@ -343,6 +358,16 @@ def visco_bench(cpu_steps=300):
     vs_dev_ms = prob.timing()[0]
     prob.close()
     assert np.isfinite(d).all()
+    # gradient_v of the smooth start model against the visco data, every shot in one batch
+    gp = FwiProblem(par, wav, data=d, batch=ns, checkpoint=50)
+    gp.set_visco(q)
+    gp.gradient(w["v0"], "reference")  # warm-up (sets the first-call scaling)
+    t = time.perf_counter()
+    gv, fv, _ = gp.gradient(w["v0"], "reference")
+    gr_s = time.perf_counter() - t
+    gr_f, gr_b = gp.timing()
+    gp.close()
+    assert np.isfinite(gv).all()
     cells = (nz + 2 * nb) * (nx + 2 * nb) * ns
     step_us = vs_dev_ms * 1e3 / nt
     achieved = 32.0 * cells / (step_us * 1e-6) / 1e9
@@ -352,7 +377,9 @@ def visco_bench(cpu_steps=300):
            "step_us": round(step_us, 2), "Gcell_per_s": round(cells * nt / (vs_dev_ms * 1e-3) / 1e9, 2),
            "algorithmic_bytes_per_step": 32.0 * cells, "achieved_GBs": round(achieved, 1),
            "frac_hbm": round(achieved / PEAK_HBM_GBS, 4),
-           "acoustic_shots_per_s": round(ns / ac_s, 3), "acoustic_device_ms": round(ac_dev_ms, 2)}
+           "acoustic_shots_per_s": round(ns / ac_s, 3), "acoustic_device_ms": round(ac_dev_ms, 2),
+           "gradient_v_shots_per_s": round(ns / gr_s, 3), "gradient_v_s": round(gr_s, 4),
+           "gradient_v_forward_ms": round(gr_f, 1), "gradient_v_backward_ms": round(gr_b, 1), "gradient_v_misfit": fv}
     out["cpu_baseline"] = None
     if cpu_steps > 0:
         try:
@@ -370,6 +397,14 @@ def visco_bench(cpu_steps=300):
                            "%.2f s, extrapolated to %.1f s per shot; single thread, -O2 -fwrapv"
                            % (cpu_steps, nt, el, per_shot))
                 out["speedup_vs_cpu"] = round(out["shots_per_s"] * per_shot, 1)
+                d1 = np.zeros((cpu_steps, nx, 1), np.float32)
+                _, _, elg = refrun.fwigrad(make_pararray(p1), w["v0"], q, wav1, d1, visco=True)
+                per_shot_g = elg * nt / cpu_steps
+                out["gradient_v_cpu_baseline"] = dict(
+                    value=round(1.0 / per_shot_g, 5), unit="shots/s", cores=1, kind="reference",
+                    sample="reference C gradient_v, one shot of the same geometry, %d of %d steps in %.2f s, "
+                           "extrapolated to %.1f s per shot; single thread, -O2 -fwrapv"
+                           % (cpu_steps, nt, elg, per_shot_g))
         except Exception as e:  # reported, never required
             out["cpu_baseline"] = dict(value=None, unit="shots/s", cores=1, kind="reference", sample="failed: %s" % e)
     return out

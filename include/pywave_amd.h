@@ -140,10 +140,12 @@ int pw_fwi_set_observed(pw_fwi *p, const float *data);
 int pw_fwi_set_source_field(pw_fwi *p, const float *src);
 /* the same from device memory of the plan's precision, [ns][nt][nx][nz] */
 int pw_fwi_set_source_field_dev(pw_fwi *p, const void *d_src);
-/* visco-acoustic modelling (forward_modeling, pywave/src/wave_fwimodeling.c:116-220): memory
- * variable with tau/taus from q (nz,nx Fortran) and the pararray's f0 (preparation,
- * pywave/src/wave_fwiutil.c:91-97).  f32 plans; pw_fwi_forward follows it, gradient and device
- * entry points refuse a visco plan.  NULL returns to acoustic. */
+/* visco-acoustic plan: memory variable with tau/taus from q (nz,nx Fortran) and the pararray's f0
+ * (preparation, pywave/src/wave_fwiutil.c:91-97).  pw_fwi_forward then follows forward_modeling
+ * (pywave/src/wave_fwimodeling.c:116-220); pw_fwi_gradient mode 0, pw_fwi_misfit_f64 and
+ * pw_fwi_lbfgs follow gradient_v (pywave/src/wave_fwigradient.c:264-452), the gradient fwi()
+ * (pywave/src/wave_fwi.c:40-41) takes for media != 1.  f32 plans, point sources only; the exact
+ * gradient (mode 1) and the device entry points refuse a visco plan.  NULL returns to acoustic. */
 int pw_fwi_set_visco(pw_fwi *p, const float *q);
 /* forward_modeling_a (pywave/src/wave_fwimodeling.c:21-114) */
 int pw_fwi_forward(pw_fwi *p, const float *vel, float *data_out);

@@ -47,7 +47,7 @@ class OrcAcq(ctypes.Structure):
         ("dr_v", ctypes.c_int),
         ("r0_v", ctypes.POINTER(ctypes.c_int)), ("r02", ctypes.POINTER(ctypes.c_int)),
         ("nr2", ctypes.POINTER(ctypes.c_int)), ("bc", ctypes.POINTER(ctypes.c_float)),
-        ("srcf", ctypes.c_void_p),
+        ("srcf", ctypes.c_void_p), ("q", ctypes.c_void_p), ("f0", ctypes.c_float),
     ]
 
 
@@ -195,17 +195,23 @@ def afd3d(vel, nt, dt, dx, dy, dz, nbt, ct, sx, sy, sz, rick, tri=False, dat=Non
 
 # ------------------------------------------------------------------ FWI
 
-def make_acq(pararray, src=None):
+def make_acq(pararray, src=None, q=None):
     """orc_acq from the 20-float pararray of pywave/pfwi.py:40-61 (fields as pfwi.c:697-716,
     with the hard-coded sz=5, rz=1, nr=nx, dr=dx, r0=ox of pfwi.c:781-806).  src: the passive
-    source field (nz, nx, nt, ns) of pfwi mode 4, or None; the caller keeps a reference to the
-    returned float32 copy (a._src) alive while a is used."""
+    source field (nz, nx, nt, ns) of pfwi mode 4, or None; q: the quality factor (nz, nx) of the
+    visco-acoustic gradient (gradient_v, f0 = pararray[16]), or None.  The returned object keeps
+    its float32 copies (a._src, a._q) alive while it is used."""
     p = np.asarray(pararray, np.float32)
     a = OrcAcq()
     a._src = None
+    a._q = None
     if src is not None:
         a._src = np.ascontiguousarray(np.asarray(src, np.float32).ravel(order="F"))
         a.srcf = a._src.ctypes.data
+    if q is not None:
+        a._q = np.ascontiguousarray(np.asarray(q, np.float32).ravel(order="F"))
+        a.q = a._q.ctypes.data
+        a.f0 = p[16]
     a.nz, a.nx = int(p[0]), int(p[1])
     a.dz, a.dx = p[2], p[3]
     a.z0, a.x0 = p[4], p[5]
@@ -237,10 +243,10 @@ def fwi_forward(pararray, vel, wav, src=None):
     return res
 
 
-def gradient_av(pararray, vel, wav, data, scaling=0.0, src=None):
-    """One gradient_av call (gradient_pas_av with src).  data (nt, nx, ns).  Returns (grad (nz,nx),
-    fcost, scaling)."""
-    a = make_acq(pararray, src)
+def gradient_av(pararray, vel, wav, data, scaling=0.0, src=None, q=None):
+    """One gradient_av call (gradient_pas_av with src, gradient_v with the quality factor q).
+    data (nt, nx, ns).  Returns (grad (nz,nx), fcost, scaling)."""
+    a = make_acq(pararray, src, q)
     velf = np.ascontiguousarray(np.asarray(vel, np.float32).ravel(order="F"))
     dflat = np.ascontiguousarray(np.asarray(data, np.float32).ravel(order="F"))
     g = np.zeros(a.nz * a.nx, np.float32)
@@ -253,9 +259,10 @@ def gradient_av(pararray, vel, wav, data, scaling=0.0, src=None):
     return g.reshape(nz, nx, order="F"), fc.value, sc.value
 
 
-def fwi(pararray, vel, wav, data, src=None):
-    """fwi() (pfwi() with src): returns vinv (nz, nx, niter) and the number of iterates stored."""
-    a = make_acq(pararray, src)
+def fwi(pararray, vel, wav, data, src=None, q=None):
+    """fwi() (pfwi() with src; media = 2, i.e. gradient_v, with the quality factor q): returns vinv
+    (nz, nx, niter) and the number of iterates stored."""
+    a = make_acq(pararray, src, q)
     niter = int(np.float32(pararray[19]))
     velf = np.ascontiguousarray(np.asarray(vel, np.float32).ravel(order="F"))
     dflat = np.ascontiguousarray(np.asarray(data, np.float32).ravel(order="F"))

@@ -86,6 +86,11 @@ typedef struct {
      * space-time source field [is][it][ix][iz] (nz*nx*nt*ns) replaces the point sources, all
      * shots record at r0_v[0] + ir*dr_v, ir < nr, and fwi resets the step every 5 iterations */
     const float *srcf;
+    /* visco-acoustic path (gradient_v, pywave/src/wave_fwigradient.c:264-452; fwi() with media != 1):
+     * when set, the quality factor q (nz*nx, z fastest) and the reference frequency f0 give tau and
+     * taus (preparation, wave_fwiutil.c:91-97) and orc_gradient_av / orc_fwi follow gradient_v */
+    const float *q;
+    float f0;
 } orc_acq;
 
 int orc_fwi_prepare(orc_acq *a);

@@ -11,6 +11,8 @@
  *   triangle smoothing pywave/src/wave_triangle.c:18-41,71-97,121-142,177-193,229-238
  *   forward_modeling_a pywave/src/wave_fwimodeling.c:21-114
  *   gradient_av        pywave/src/wave_fwigradient.c:93-262
+ *   forward_modeling   pywave/src/wave_fwimodeling.c:116-220  (visco-acoustic)
+ *   gradient_v         pywave/src/wave_fwigradient.c:264-452  (visco-acoustic, fwi() with media != 1)
  *   fwi + L-BFGS       pywave/src/wave_fwi.c:24-119, pywave/src/wave_fwilbfgs.c:9-165
  * Per-value arithmetic (float vs double sub-expressions, association, accumulation order)
  * follows the reference so that the results agree bit for bit.
@@ -234,18 +236,12 @@ int orc_fwi_forward(const orc_acq *a, const float *vel, const float *wav, float 
     return 0;
 }
 
-/* visco-acoustic modelling (pywave/src/wave_fwimodeling.c:116-220): a memory variable r with
- * tau, taus from Q and f0 (preparation, wave_fwiutil.c:91-97); r and the source term are formed in
- * double where the reference's expressions promote, then stored in f32; r is damped once per step,
- * the pressure levels as in the acoustic case. */
-int orc_fwi_forward_visco(const orc_acq *a, const float *vel, const float *q, float f0, const float *wav,
-                          float *data)
+/* tau and taus (preparation, pywave/src/wave_fwiutil.c:91-97) of the quality factor q, padded by
+ * edge replication onto the (padnx, padnz) grid */
+static void visco_tables(const orc_acq *a, const float *q, float f0, float *tau, float *taus)
 {
     const double PI = 3.14159265358979323846264338328;
-    int pnz = a->padnz, pnx = a->padnx, nt = a->nt, nr = a->nr;
-    size_t n = (size_t)pnz * pnx, nzx = (size_t)a->nz * a->nx;
-    float dx2 = a->dx * a->dx, dz2 = a->dz * a->dz, dt2 = a->dt * a->dt;
-    float idt = 1. / a->dt;
+    size_t nzx = (size_t)a->nz * a->nx;
     float *tau1 = malloc(sizeof(float) * nzx), *taus1 = malloc(sizeof(float) * nzx);
     for (size_t i = 0; i < nzx; i++) {
         float tmp = sqrtf(q[i] * q[i] + 1);
@@ -253,13 +249,28 @@ int orc_fwi_forward_visco(const orc_acq *a, const float *vel, const float *q, fl
         taus1[i] = (tmp - 1) / (2. * PI * f0 * q[i]);
         tau1[i] = taue / taus1[i] - 1;
     }
+    pad2d(tau1, tau, a->nz, a->nx, a->nb);
+    pad2d(taus1, taus, a->nz, a->nx, a->nb);
+    free(tau1); free(taus1);
+}
+
+/* visco-acoustic modelling (pywave/src/wave_fwimodeling.c:116-220): a memory variable r with
+ * tau, taus from Q and f0 (preparation, wave_fwiutil.c:91-97); r and the source term are formed in
+ * double where the reference's expressions promote, then stored in f32; r is damped once per step,
+ * the pressure levels as in the acoustic case. */
+int orc_fwi_forward_visco(const orc_acq *a, const float *vel, const float *q, float f0, const float *wav,
+                          float *data)
+{
+    int pnz = a->padnz, pnx = a->padnx, nt = a->nt, nr = a->nr;
+    size_t n = (size_t)pnz * pnx;
+    float dx2 = a->dx * a->dx, dz2 = a->dz * a->dz, dt2 = a->dt * a->dt;
+    float idt = 1. / a->dt;
     float *vv = malloc(sizeof(float) * n), *tau = malloc(sizeof(float) * n), *taus = malloc(sizeof(float) * n);
     float *rr = malloc(sizeof(float) * n);
     float *p0 = malloc(sizeof(float) * n), *p1 = malloc(sizeof(float) * n), *p2 = malloc(sizeof(float) * n);
     float *r1 = malloc(sizeof(float) * n), *r2 = malloc(sizeof(float) * n), *term = calloc(n, sizeof(float));
     pad2d(vel, vv, a->nz, a->nx, a->nb);
-    pad2d(tau1, tau, a->nz, a->nx, a->nb);
-    pad2d(taus1, taus, a->nz, a->nx, a->nb);
+    visco_tables(a, q, f0, tau, taus);
     for (int is = 0; is < a->ns; is++) {
         float *dd = data + (size_t)is * nr * nt;
         memset(dd, 0, sizeof(float) * (size_t)nr * nt);
@@ -287,7 +298,7 @@ int orc_fwi_forward_visco(const orc_acq *a, const float *vel, const float *q, fl
             sponge(r1, a->bc, pnx, pnz, a->nb);
         }
     }
-    free(tau1); free(taus1); free(vv); free(tau); free(taus); free(rr);
+    free(vv); free(tau); free(taus); free(rr);
     free(p0); free(p1); free(p2); free(r1); free(r2); free(term);
     return 0;
 }
@@ -335,12 +346,22 @@ int orc_gradient_av(const orc_acq *a, const float *vel, const float *wav, const 
     float *pp = malloc(sizeof(float) * (size_t)nr * nt);
     if (!wave) return -1;
     pad2d(vel, vv, nz, nx, nb);
+    /* gradient_v: memory variable r1/r2, its scratch tmp, tau/taus; updates on [4, pad-4) only */
+    const int visco = a->q != NULL;
+    float idt = 1. / dt;
+    float *tau = NULL, *taus = NULL, *r1 = NULL, *r2 = NULL, *tmp = NULL;
+    if (visco) {
+        tau = malloc(sizeof(float) * n); taus = malloc(sizeof(float) * n);
+        r1 = malloc(sizeof(float) * n); r2 = malloc(sizeof(float) * n); tmp = calloc(n, sizeof(float));
+        visco_tables(a, a->q, a->f0, tau, taus);
+    }
     *fcost = 0.;
     memset(grad, 0, sizeof(float) * nzx);
 
     for (int is = 0; is < a->ns; is++) {
         memset(p0, 0, sizeof(float) * n); memset(p1, 0, sizeof(float) * n); memset(p2, 0, sizeof(float) * n);
         memset(pp, 0, sizeof(float) * (size_t)nr * nt);
+        if (visco) { memset(r1, 0, sizeof(float) * n); memset(r2, 0, sizeof(float) * n); }
         int sx = a->s0_v + is * a->ds_v;
         orc_source_map(sx, a->sz, 2, 2, pnx, pnz, rr);
         int r02, nr2, r0v;
@@ -357,11 +378,26 @@ int orc_gradient_av(const orc_acq *a, const float *vel, const float *wav, const 
                         wave[(size_t)wit * nzx + (size_t)ix * nz + iz] = p1[(size_t)(ix + nb) * pnz + iz + nb];
                 wit++;
             }
-            const float *fs = a->srcf ? a->srcf + ((size_t)is * nt + it) * nzx : NULL;
-            fwi_step(p0, p1, p2, term, vv, a->srcf ? NULL : rr, wav[it], fs, nz, nx, nb, pnx, pnz, dx2, dz2, dt2);
+            if (visco) {  /* wave_fwigradient.c:318-341 */
+                laplace8(p1, term, pnx, pnz, dx2, dz2);
+                for (int ix = 4; ix < pnx - 4; ix++)
+                    for (int iz = 4; iz < pnz - 4; iz++) {
+                        size_t j = (size_t)ix * pnz + iz;
+                        double num = (double)(tau[j] / taus[j] * term[j]) + ((double)idt - 0.5 / (double)taus[j]) * r1[j];
+                        r2[j] = (float)(num / ((double)idt + 0.5 / (double)taus[j]));
+                        double t = (double)term[j] * (1. + (double)tau[j]) - (double)(r2[j] + r1[j]) * 0.5;
+                        term[j] = (float)(t + (double)(rr[j] * wav[it]));
+                        p2[j] = 2 * p1[j] - p0[j] + vv[j] * vv[j] * dt2 * term[j];
+                    }
+                float *t = r1; r1 = r2; r2 = t;
+            } else {
+                const float *fs = a->srcf ? a->srcf + ((size_t)is * nt + it) * nzx : NULL;
+                fwi_step(p0, p1, p2, term, vv, a->srcf ? NULL : rr, wav[it], fs, nz, nx, nb, pnx, pnz, dx2, dz2, dt2);
+            }
             float *t = p0; p0 = p1; p1 = p2; p2 = t;
             sponge(p0, a->bc, pnx, pnz, nb);
             sponge(p1, a->bc, pnx, pnz, nb);
+            if (visco) sponge(r1, a->bc, pnx, pnz, nb);
         }
         wit--;
         const float *dd = data + (size_t)is * nr * nt;
@@ -375,13 +411,35 @@ int orc_gradient_av(const orc_acq *a, const float *vel, const float *wav, const 
 
         memset(p0, 0, sizeof(float) * n); memset(p1, 0, sizeof(float) * n);
         memset(p2, 0, sizeof(float) * n); memset(term, 0, sizeof(float) * n);
+        if (visco) {
+            memset(r1, 0, sizeof(float) * n); memset(r2, 0, sizeof(float) * n); memset(tmp, 0, sizeof(float) * n);
+        }
         for (int it = nt - 1; it >= 0; it--) {
-            laplace8(p1, term, pnx, pnz, dx2, dz2);
+            if (visco) {  /* wave_fwigradient.c:375-388: the r term, then the Laplacian of tmp */
+                for (int ix = 4; ix < pnx - 4; ix++)
+                    for (int iz = 4; iz < pnz - 4; iz++) {
+                        size_t j = (size_t)ix * pnz + iz;
+                        double num = (double)(-tau[j] / taus[j] * p1[j]) + (-(double)idt + 0.5 / (double)taus[j]) * r1[j];
+                        r2[j] = (float)(num / (-(double)idt - 0.5 / (double)taus[j]));
+                        tmp[j] = (float)((double)p1[j] * (1. + (double)tau[j]) - 0.5 * (double)(r2[j] + r1[j]));
+                    }
+                laplace8(tmp, term, pnx, pnz, dx2, dz2);
+            } else {
+                laplace8(p1, term, pnx, pnz, dx2, dz2);
+            }
             for (int ir = 0; ir < nr2; ir++) {
                 int rx = r0v + ir * a->dr_v;
                 term[(size_t)rx * pnz + a->rz] += pp[(size_t)(r02 + ir) * nt + it];
             }
-            for (size_t j = 0; j < n; j++) p2[j] = 2 * p1[j] - p0[j] + vv[j] * vv[j] * dt2 * term[j];
+            if (visco) {
+                for (int ix = 4; ix < pnx - 4; ix++)
+                    for (int iz = 4; iz < pnz - 4; iz++) {
+                        size_t j = (size_t)ix * pnz + iz;
+                        p2[j] = 2 * p1[j] - p0[j] + vv[j] * vv[j] * dt2 * term[j];
+                    }
+            } else {
+                for (size_t j = 0; j < n; j++) p2[j] = 2 * p1[j] - p0[j] + vv[j] * vv[j] * dt2 * term[j];
+            }
             if (it % interval == 0) {
                 if (wit != wnt - 1 && wit != 0) {
                     for (int ix = 0; ix < nx; ix++)
@@ -400,6 +458,10 @@ int orc_gradient_av(const orc_acq *a, const float *vel, const float *wav, const 
             float *t = p0; p0 = p1; p1 = p2; p2 = t;
             sponge(p0, a->bc, pnx, pnz, nb);
             sponge(p1, a->bc, pnx, pnz, nb);
+            if (visco) {
+                t = r1; r1 = r2; r2 = t;
+                sponge(r1, a->bc, pnx, pnz, nb);
+            }
         }
     }
     if (*scaling <= 0) {
@@ -410,6 +472,7 @@ int orc_gradient_av(const orc_acq *a, const float *vel, const float *wav, const 
     }
     orc_gradient_smooth2(3, 3, nx, nz, 0, *scaling, grad);
     free(weight); free(vv); free(rr); free(p0); free(p1); free(p2); free(term); free(wave); free(pp);
+    free(tau); free(taus); free(r1); free(r2); free(tmp);
     return 0;
 }
 

@@ -15,7 +15,8 @@
  *   fwimodv: the same set-up -> forward_modeling (wave_fwimodeling.c:116), the visco-acoustic
  *            modelling pywave/pfwi.py:72 means to reach (media=2)
  *   fwigrad: pywave/src/pfwi.c:497-1028 setup -> gradient_init/gradient_av
- *            (wave_fwigradient.c:20,93), called 1 or 2 times
+ *            (wave_fwigradient.c:20,93), called 1 or 2 times; fwigradv the same with the
+ *            visco-acoustic gradient_v (wave_fwigradient.c:264) that fwi() takes for media != 1
  *   fwi   : pywave/src/pfwi.c:497-1028 -> fwi (wave_fwi.c:24)
  *   pfwigrad: pywave/src/pfwi.c:1507-2034 setup (passive, only velocity, known source field)
  *            -> gradient_pas_init/gradient_pas_av (wave_fwigradient.c:528,672), 1 or 2 calls
@@ -342,10 +343,12 @@ static int run_fwigrad(int argc, char **argv)
     int ncall = vel_b ? 2 : 1;
     float *grads = (float *)malloc(sizeof(float) * nzx * ncall);
     float fc[2] = {0, 0};
+    /* fwigradv: the visco-acoustic gradient fwi() selects for media != 1 */
+    void (*grad_fn)(float *, float *, float *) = strcmp(argv[1], "fwigradv") ? gradient_av : gradient_v;
     double t_start = now_s();
-    gradient_av(vel, &fc[0], grads);
+    grad_fn(vel, &fc[0], grads);
     fprintf(stderr, "PWREF_ELAPSED_S=%.6f\n", now_s() - t_start);
-    if (vel_b) gradient_av(vel_b, &fc[1], grads + nzx);
+    if (vel_b) grad_fn(vel_b, &fc[1], grads + nzx);
     write_f32(argv[7], grads, nzx * ncall);
     write_f32(argv[8], fc, ncall);
     return 0;
@@ -381,8 +384,8 @@ static int run_fwi(int argc, char **argv)
     fill_fwipar(acpar, fwipar);
 
     double t_start = now_s();
-    /* fwic passes its paspar pointer as 'verb' and its verb (=1) as 'media' */
-    fwi(data, vinv, vgrad, soupar, acpar, array, fwipar, optpar, true, 1);
+    /* fwic passes its paspar pointer as 'verb' and its verb (=1) as 'media'; fwiv: media = 2 */
+    fwi(data, vinv, vgrad, soupar, acpar, array, fwipar, optpar, true, strcmp(argv[1], "fwiv") ? 1 : 2);
     fprintf(stderr, "PWREF_ELAPSED_S=%.6f\n", now_s() - t_start);
     write_f32(argv[7], vinv[0], nzx * a.niter);
     return 0;
@@ -525,7 +528,7 @@ int main(int argc, char **argv)
     /* every allocation through mmap: fresh, zero-filled pages */
     mallopt(M_MMAP_THRESHOLD, 0);
     mallopt(M_MMAP_MAX, 1 << 30);
-    if (argc < 2) { fprintf(stderr, "usage: pwref {afd2d|afd3d|aps2d|aps3d|ricker|fwimod|fwimodv|fwigrad|fwi|pfwigrad|pfwi|lstri} ...\n"); return 2; }
+    if (argc < 2) { fprintf(stderr, "usage: pwref {afd2d|afd3d|aps2d|aps3d|ricker|fwimod|fwimodv|fwigrad|fwigradv|fwi|fwiv|pfwigrad|pfwi|lstri} ...\n"); return 2; }
     /* the engines print progress on stdout; keep it off the caller's stdout */
     if (!getenv("PWREF_KEEP_STDOUT")) { if (!freopen("/dev/null", "w", stdout)) return 4; }
     if (!strcmp(argv[1], "afd2d")) return run_afd2d(argc, argv);
@@ -534,8 +537,8 @@ int main(int argc, char **argv)
     if (!strcmp(argv[1], "aps3d")) { g_ps = 1; return run_afd3d(argc, argv); }
     if (!strcmp(argv[1], "ricker")) return run_ricker(argc, argv);
     if (!strcmp(argv[1], "fwimod") || !strcmp(argv[1], "fwimodv")) return run_fwimod(argc, argv);
-    if (!strcmp(argv[1], "fwigrad")) return run_fwigrad(argc, argv);
-    if (!strcmp(argv[1], "fwi")) return run_fwi(argc, argv);
+    if (!strcmp(argv[1], "fwigrad") || !strcmp(argv[1], "fwigradv")) return run_fwigrad(argc, argv);
+    if (!strcmp(argv[1], "fwi") || !strcmp(argv[1], "fwiv")) return run_fwi(argc, argv);
     if (!strcmp(argv[1], "pfwigrad")) return run_pfwigrad(argc, argv);
     if (!strcmp(argv[1], "pfwi")) return run_pfwi(argc, argv);
     if (!strcmp(argv[1], "lstri")) return run_lstri(argc, argv);

@@ -93,12 +93,13 @@ def fwimod(pararray, vel, q, wav, visco=False):
     return out.reshape(nt, nx, ns, order="F"), el
 
 
-def fwigrad(pararray, vel, q, wav, data, vel_second=None):
-    """gradient_init + gradient_av (x1 or x2).  Returns (grads [(nz,nx)...], fcosts, elapsed)."""
+def fwigrad(pararray, vel, q, wav, data, vel_second=None, visco=False):
+    """gradient_init + gradient_av (visco: gradient_v) (x1 or x2).  Returns (grads [(nz,nx)...],
+    fcosts, elapsed)."""
     p = np.asarray(pararray, np.float32)
     nz, nx = int(p[0]), int(p[1])
     with tempfile.TemporaryDirectory() as tmp:
-        args = ["fwigrad", _put(tmp, "par.f32", p), _put(tmp, "vel.f32", np.asarray(vel).ravel(order="F")),
+        args = ["fwigradv" if visco else "fwigrad", _put(tmp, "par.f32", p), _put(tmp, "vel.f32", np.asarray(vel).ravel(order="F")),
                 _put(tmp, "q.f32", np.asarray(q).ravel(order="F")), _put(tmp, "wav.f32", wav),
                 _put(tmp, "data.f32", np.asarray(data).ravel(order="F")),
                 os.path.join(tmp, "grad.f32"), os.path.join(tmp, "fc.f32")]
@@ -111,12 +112,12 @@ def fwigrad(pararray, vel, q, wav, data, vel_second=None):
     return [g[i * nz * nx:(i + 1) * nz * nx].reshape(nz, nx, order="F") for i in range(k)], fc, el
 
 
-def fwi(pararray, vel, q, wav, data):
-    """fwic: returns (vinv (nz,nx,niter), elapsed)."""
+def fwi(pararray, vel, q, wav, data, visco=False):
+    """fwic (visco: fwi() with media = 2, i.e. gradient_v): returns (vinv (nz,nx,niter), elapsed)."""
     p = np.asarray(pararray, np.float32)
     nz, nx, niter = int(p[0]), int(p[1]), int(p[19])
     with tempfile.TemporaryDirectory() as tmp:
-        args = ["fwi", _put(tmp, "par.f32", p), _put(tmp, "vel.f32", np.asarray(vel).ravel(order="F")),
+        args = ["fwiv" if visco else "fwi", _put(tmp, "par.f32", p), _put(tmp, "vel.f32", np.asarray(vel).ravel(order="F")),
                 _put(tmp, "q.f32", np.asarray(q).ravel(order="F")), _put(tmp, "wav.f32", wav),
                 _put(tmp, "data.f32", np.asarray(data).ravel(order="F")), os.path.join(tmp, "vinv.f32")]
         el = _run(args, tmp)

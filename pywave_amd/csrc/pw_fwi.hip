@@ -263,78 +263,203 @@ struct ImgArgs {
     long long gstride;
 };
 
-template <typename T>
-__global__ __launch_bounds__(64) void k_adj_img(FGeo<T> g, StepArgs<T> a, ImgArgs<T> im, int ntz)
+// ---------------------------------------------------------------- visco-acoustic imaging adjoint
+// gradient_v's backward step (pywave/src/wave_fwigradient.c:372-420): on [4, pad-4) the memory
+// variable r2 = ((-tau)/taus p1 + (-idt + 0.5/taus) r1) / (-idt - 0.5/taus) and the Laplacian's
+// argument tmp = p1 (1 + tau) - 0.5 (r2 + r1), both in double where C promotes and stored in f32;
+// zero outside.  r2 is damped once, the pressure levels as in gradient_av.
+struct ViscoBack {
+    const float* tau;  // padded [fs]
+    const float* taus;
+    const float* r1;   // backward memory variable [slot][fs], read
+    float* r2;         // written (damped)
+    float idt;
+};
+
+__device__ __forceinline__ float visco_back(float p1, float r1, float ta, float ts, float idt, float& r2)
 {
-    using C = Coef<T>;
+    const double num = (double)(-ta / ts * p1) + (-(double)idt + 0.5 / (double)ts) * (double)r1;
+    r2 = (float)(num / (-(double)idt - 0.5 / (double)ts));
+    return (float)((double)p1 * (1. + (double)ta) - 0.5 * (double)(r2 + r1));
+}
+
+// The reference imaging-condition adjoint step, pipelined as k_fwd.  gradient_av
+// (pywave/src/wave_fwigradient.c:201-241): term = L p1 + residual at the receivers,
+// p2 = 2 p1 - p0 + v^2 dt^2 term, and at imaging steps grad += (u[w+1] - 2u[w] + u[w-1])/wdt2 * p1
+// * (-2/v^3) in double.  VISCO: gradient_v (:372-420), the Laplacian of tmp instead of p1 and the
+// memory variable r2 written.  The column entering the 9-column queue (its p1, and for VISCO r1,
+// tau, taus, forming tmp and r2 once), the z halo, Y, m and the imaging operands of the next
+// column are loaded one iteration ahead through buffer resources.
+template <bool VISCO>
+__global__ __launch_bounds__(64) void k_adj_ref(FGeo<float> g, StepArgs<float> a, ImgArgs<float> im, ViscoBack vb,
+                                                int ntz)
+{
+    using C = Coef<float>;
     const int slot = blockIdx.y;
     const int tz = blockIdx.x % ntz, xc = blockIdx.x / ntz;
     const int lane = threadIdx.x, z0 = tz * 64, z = z0 + lane;
     const int x0 = xc * XC, x1 = min(x0 + XC, g.pnx);
-    __shared__ T row[2][72];
-    const T* __restrict__ X = a.X + slot * g.fs;
-    T* __restrict__ Y = a.Y + slot * g.fs;
+    __shared__ float row[2][72];
     const ShotInfo sh = a.shots[slot];
+    const int L = g.ldz;
+    const int bias = 4 * L + 4;  // windows start 4 columns + 4 cells before the field
+    const unsigned wb = (unsigned)((g.fs + 2 * bias) * 4LL);
+    const auto rX = win(a.X + slot * g.fs - bias, wb), rY = win(a.Y + slot * g.fs - bias, wb),
+               rM = win(a.m - bias, wb);
+    const auto rR1 = win(VISCO ? vb.r1 + slot * g.fs - bias : nullptr, VISCO ? wb : 0u);
+    const auto rR2 = win(VISCO ? vb.r2 + slot * g.fs - bias : nullptr, VISCO ? wb : 0u);
+    const auto rTa = win(VISCO ? vb.tau - bias : nullptr, VISCO ? wb : 0u);
+    const auto rTs = win(VISCO ? vb.taus - bias : nullptr, VISCO ? wb : 0u);
     const bool zok = z < g.pnz;
     const bool zin = z >= 4 && z < g.pnz - 4;
-    const T zl = zok ? a.sfz[2 * z] : T(1), zh = zok ? a.sfz[2 * z + 1] : T(1);
-    const T* inj = a.inj + slot * a.trs + (long long)a.it * g.nr;
-    T q[9];
+    const float zl = zok ? a.sfz[2 * z] : 1.f, zh = zok ? a.sfz[2 * z + 1] : 1.f;
+    const int zhz = lane < 4 ? z0 - 4 + lane : z0 + 60 + lane;  // z halo of lanes 0..7
+    const int lzh = lane < 4 ? lane : 64 + lane;
+    const bool zhv = lane < 8 && zhz >= 0 && zhz < g.pnz;
+    const bool zhin = zhv && zhz >= 4 && zhz < g.pnz - 4;
+    // lane offsets (out-of-range offsets read 0 and drop stores): every lane of the grid, lanes
+    // where tmp is formed (z in [4, pad-4)), halo lanes (VISCO: where tmp is formed)
+    const unsigned oz = zok ? (unsigned)(bias + z) * 4u : kOOB;
+    const unsigned ozi = (zok && zin) ? oz : kOOB;
+    const unsigned ozh = (VISCO ? zhin : zhv) ? (unsigned)(bias + zhz) * 4u : kOOB;
+    const unsigned oc = (unsigned)L * 4u;
+    auto xv = [&](int x) { return x >= 0 && x < g.pnx; };
+    auto xin = [&](int x) { return x >= 4 && x < g.pnx - 4; };
+    auto ld = [&](__amdgpu_buffer_rsrc_t r, unsigned voff, int x, bool ok) {
+        return bldo<float>(r, ok ? voff : kOOB, ok ? (unsigned)x * oc : 0u);
+    };
+    const bool lin = zok && zin;
+    // tmp and r2 of a cell from its raw values; zero where the reference leaves them unwritten
+    auto form = [&](bool ok, float p1, float r1, float ta, float ts, float& r2) -> float {
+        r2 = 0.f;
+        return ok ? visco_back(p1, r1, ta, ts, vb.idt, r2) : 0.f;
+    };
+    // imaging operands on the model interior [slot][nx][nz] (windows empty when this step does
+    // not image)
+    const bool img = im.w_cur != nullptr;
+    const unsigned nzx4 = img ? (unsigned)(g.nz * g.nx * 4) : 0u;
+    const auto rWp = win(img ? im.w_prev + slot * im.snaps : nullptr, nzx4);
+    const auto rWc = win(img ? im.w_cur + slot * im.snaps : nullptr, nzx4);
+    const auto rWn = win(img ? im.w_next + slot * im.snaps : nullptr, nzx4);
+    const auto rTv = win(img ? im.tmpv : nullptr, nzx4);
+    const auto rG = win(img ? im.grad + slot * im.gstride : nullptr, nzx4);
+    const bool gz = zok && z >= g.nb && z < g.nb + g.nz;
+    const unsigned og = gz ? (unsigned)(z - g.nb) * 4u : kOOB;
+    auto gxin = [&](int x) { return x >= g.nb && x < g.nb + g.nx; };
+    auto ldi = [&](__amdgpu_buffer_rsrc_t r, int x) {
+        const bool ok = img && gxin(x);
+        return bldo<float>(r, ok ? og : kOOB, ok ? (unsigned)((x - g.nb) * g.nz) * 4u : 0u);
+    };
+    const float* inj = a.inj + slot * a.trs + (long long)a.it * g.nr;
+    float tq[9], pq[5], rq[5];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        int xx = x0 - 4 + k;
-        q[k] = (zok && xx >= 0 && xx < g.pnx) ? X[(long long)xx * g.ldz + z] : T(0);
+    for (int k = 0; k < 9; ++k) {
+        const int xx = x0 - 4 + k;
+        const float p1 = ld(rX, oz, xx, xv(xx));
+        float r2 = 0.f;
+        if constexpr (VISCO) {
+            const bool ci = xin(xx);
+            const float r1 = ld(rR1, ozi, xx, ci), ta = ld(rTa, ozi, xx, ci), ts = ld(rTs, ozi, xx, ci);
+            tq[k] = form(lin && ci, p1, r1, ta, ts, r2);
+        } else {
+            tq[k] = p1;
+        }
+        if (k >= 4) {
+            pq[k - 4] = p1;
+            rq[k - 4] = r2;
+        }
     }
+    float hp, hr = 0.f, hta = 0.f, hts = 0.f;  // halo raw values of the centre column
+    if constexpr (VISCO) {
+        const bool ci = xin(x0);
+        hp = ld(rX, ozh, x0, ci); hr = ld(rR1, ozh, x0, ci);
+        hta = ld(rTa, ozh, x0, ci); hts = ld(rTs, ozh, x0, ci);
+    } else {
+        hp = ld(rX, ozh, x0, true);
+    }
+    float yv = ld(rY, oz, x0, true), mv = ld(rM, oz, x0, true);
+    float wp = ldi(rWp, x0), wc = ldi(rWc, x0), wn = ldi(rWn, x0), tv = ldi(rTv, x0), gv = ldi(rG, x0);
     int buf = 0;
     for (int x = x0; x < x1; ++x) {
-        const long long col = (long long)x * g.ldz;
-        q[8] = (zok && x + 4 < g.pnx) ? X[col + 4LL * g.ldz + z] : T(0);
-        const T yv = zok ? Y[col + z] : T(0);
-        const T mv = zok ? a.m[col + z] : T(0);
-        row[buf][lane + 4] = q[4];
-        if (lane < 4) {
-            int za = z0 - 4 + lane, zb = z0 + 64 + lane;
-            row[buf][lane] = za >= 0 ? X[col + za] : T(0);
-            row[buf][68 + lane] = zb < g.pnz ? X[col + zb] : T(0);
+        const int xn = x + 1, xq = x + 5;
+        const bool nv = xn < x1;
+        // next iteration's loads
+        const bool qv = nv && xq < g.pnx, qi = nv && xin(xq);
+        const float np1 = ld(rX, oz, xq, qv);
+        float nr1 = 0.f, nta = 0.f, nts = 0.f, nhr = 0.f, nhta = 0.f, nhts = 0.f, nhp;
+        if constexpr (VISCO) {
+            nr1 = ld(rR1, ozi, xq, qi); nta = ld(rTa, ozi, xq, qi); nts = ld(rTs, ozi, xq, qi);
+            const bool hi = nv && xin(xn);
+            nhp = ld(rX, ozh, xn, hi); nhr = ld(rR1, ozh, xn, hi);
+            nhta = ld(rTa, ozh, xn, hi); nhts = ld(rTs, ozh, xn, hi);
+        } else {
+            nhp = ld(rX, ozh, xn, nv);
         }
+        const float nyv = ld(rY, oz, xn, nv), nmv = ld(rM, oz, xn, nv);
+        float nwp = 0.f, nwc = 0.f, nwn = 0.f, ntv = 0.f, ngv = 0.f;
+        if (nv) {
+            nwp = ldi(rWp, xn); nwc = ldi(rWc, xn); nwn = ldi(rWn, xn); ntv = ldi(rTv, xn); ngv = ldi(rG, xn);
+        }
+        float ht;
+        if constexpr (VISCO) {
+            float hr2;
+            ht = form(zhin && xin(x), hp, hr, hta, hts, hr2);
+        } else {
+            ht = hp;
+        }
+        row[buf][lane + 4] = tq[4];
+        if (lane < 8) row[buf][lzh] = ht;
         __syncthreads();
         if (zok) {
-            const T p1 = q[4];
-            T term = T(0);
-            if (zin && x >= 4 && x < g.pnx - 4) {
-                const T* R = &row[buf][lane + 4];
-                term = (C::K0 * p1 + C::K1 * (q[5] + q[3]) + C::K2 * (q[6] + q[2]) + C::K3 * (q[7] + q[1]) +
-                        C::K4 * (q[8] + q[0])) / g.dx2 +
-                       (C::K0 * p1 + C::K1 * (R[1] + R[-1]) + C::K2 * (R[2] + R[-2]) + C::K3 * (R[3] + R[-3]) +
+            const float p1 = pq[0];
+            float term = 0.f;
+            if (zin && xin(x)) {
+                const float* R = &row[buf][lane + 4];
+                term = (C::K0 * tq[4] + C::K1 * (tq[5] + tq[3]) + C::K2 * (tq[6] + tq[2]) + C::K3 * (tq[7] + tq[1]) +
+                        C::K4 * (tq[8] + tq[0])) / g.dx2 +
+                       (C::K0 * tq[4] + C::K1 * (R[1] + R[-1]) + C::K2 * (R[2] + R[-2]) + C::K3 * (R[3] + R[-3]) +
                         C::K4 * (R[4] + R[-4])) / g.dz2;
             }
-            if (z == g.rz) {  // load the windowed residual at the receivers (:208-211)
+            if (z == g.rz) {  // the windowed residual at the receivers (:208-211, :390-394)
                 if (g.dr_v > 0) {
                     int d = x - sh.r0_v;
                     if (d >= 0 && d % g.dr_v == 0 && d / g.dr_v < sh.nr2) {
-                        int k = sh.r02 + d / g.dr_v;
-                        if (k >= 0 && k < g.nr) term = term + inj[k];
+                        int kk = sh.r02 + d / g.dr_v;
+                        if (kk >= 0 && kk < g.nr) term = term + inj[kk];
                     }
                 } else if (x == sh.r0_v) {
                     for (int ir = 0; ir < sh.nr2; ++ir)
                         if (sh.r02 + ir >= 0 && sh.r02 + ir < g.nr) term = term + inj[sh.r02 + ir];
                 }
             }
-            const T xl = a.sfx[2 * x], xh = a.sfx[2 * x + 1];
-            const T p0 = sponge(yv, zl, zh, xl, xh);
-            T p2 = 2 * p1 - p0 + mv * term;
-            Y[col + z] = sponge(p2, zl, zh, xl, xh);
-            if (im.w_cur && z >= g.nb && z < g.nb + g.nz && x >= g.nb && x < g.nb + g.nx) {
-                const long long k = (long long)(x - g.nb) * g.nz + (z - g.nb);
-                const long long ks = slot * im.snaps + k;
-                double e = ((double)im.w_next[ks] - 2. * (double)im.w_cur[ks] + (double)im.w_prev[ks]) / g.wdt2 *
-                           (double)p1 * (double)im.tmpv[k];
-                T* gp = im.grad + slot * im.gstride + k;
-                *gp = (T)((double)*gp + e);
+            const float xl = a.sfx[2 * x], xh = a.sfx[2 * x + 1];
+            const float p0 = sponge(yv, zl, zh, xl, xh);
+            const float p2 = 2 * p1 - p0 + mv * term;
+            bsto(rY, oz, (unsigned)x * oc, sponge(p2, zl, zh, xl, xh));
+            if constexpr (VISCO) bsto(rR2, oz, (unsigned)x * oc, sponge(rq[0], zl, zh, xl, xh));
+            if (img && gz && gxin(x)) {
+                const double e = ((double)wn - 2. * (double)wc + (double)wp) / g.wdt2 * (double)p1 * (double)tv;
+                bsto(rG, og, (unsigned)((x - g.nb) * g.nz) * 4u, (float)((double)gv + e));
             }
         }
 #pragma unroll
-        for (int k = 0; k < 8; ++k) q[k] = q[k + 1];
+        for (int k = 0; k < 8; ++k) tq[k] = tq[k + 1];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            pq[k] = pq[k + 1];
+            rq[k] = rq[k + 1];
+        }
+        if constexpr (VISCO) {
+            float r2;
+            tq[8] = form(lin && qi, np1, nr1, nta, nts, r2);
+            rq[4] = r2;
+        } else {
+            tq[8] = np1;
+        }
+        pq[4] = np1;
+        hp = nhp; hr = nhr; hta = nhta; hts = nhts;
+        yv = nyv; mv = nmv;
+        wp = nwp; wc = nwc; wn = nwn; tv = ntv; gv = ngv;
         buf ^= 1;
     }
 }
@@ -882,8 +1007,11 @@ struct pw_fwi {
     bool passive = false;   // shots are the source field (gradient_pas_av), not point sources
     int cur_b0 = 0;         // first shot of the batch set by set_batch
     std::string active_err; // why the point-source geometry is unusable (empty: usable)
-    bool visco = false;     // visco-acoustic modelling (forward only)
+    bool visco = false;     // visco-acoustic modelling and the gradient_v imaging gradient
+    // tau, taus on the padded grid; forward memory variable R0/R1 and backward RB0/RB1 per slot;
+    // the forward one at every checkpoint [segment][slot][fs]
     float *tau_pad = nullptr, *taus_pad = nullptr, *R0 = nullptr, *R1 = nullptr;
+    float *RB0 = nullptr, *RB1 = nullptr, *ckpt_r = nullptr;
     std::vector<ShotInfo> shot_active;
     float* tmpv = nullptr;
     float* work = nullptr;  // smoothing workspace
@@ -950,7 +1078,7 @@ void fwi_free(pw_fwi* p)
     void* bufs[] = {p->vel, p->vpad, p->m, p->sfz, p->sfx, p->wav, p->X, p->Y, p->A, p->B, p->rec, p->res,
                     p->obs, p->W, p->ckpt, p->ckpt_snap, p->seg, p->gm, p->gslot, p->gtot, p->patch,
                     p->tmpv, p->work, p->shots, p->slot_shot, p->partial, p->dscalar, p->gfold, p->fsrc,
-                    p->tau_pad, p->taus_pad, p->R0, p->R1};
+                    p->tau_pad, p->taus_pad, p->R0, p->R1, p->RB0, p->RB1, p->ckpt_r};
     for (void* b : bufs)
         if (b) hipFree(b);
 }
@@ -990,7 +1118,9 @@ int plan_storage(pw_fwi* p, int K)
     if (p->ckpt) { hipFree(p->ckpt); p->ckpt = nullptr; }
     if (p->ckpt_snap) { hipFree(p->ckpt_snap); p->ckpt_snap = nullptr; }
     if (p->seg) { hipFree(p->seg); p->seg = nullptr; }
+    if (p->ckpt_r) { hipFree(p->ckpt_r); p->ckpt_r = nullptr; }
     int rc = dmalloc(p, &p->ckpt, (size_t)nseg * 2 * p->nslot * p->fs * e);
+    if (!rc && p->visco) rc = dmalloc(p, (void**)&p->ckpt_r, (size_t)nseg * p->nslot * p->fs * 4);
     if (!rc) rc = dmalloc(p, &p->ckpt_snap, (size_t)nseg * p->nslot * p->nzx * e);
     // segment buffer: full fields for the exact adjoint (K of them) or interior snapshots for
     // the imaging condition (K/I + 2); the larger of the two
@@ -1153,6 +1283,8 @@ int forward_batch(pw_fwi* p, int nb_, bool ckpt, int mode, bool store_last)
                 T* c = (T*)p->ckpt + (size_t)sgi * 2 * fld;
                 PW_HIPS(hipMemcpyAsync(c, X, fld * sizeof(T), hipMemcpyDeviceToDevice, s));
                 PW_HIPS(hipMemcpyAsync(c + fld, Y, fld * sizeof(T), hipMemcpyDeviceToDevice, s));
+                if (visco)  // the memory variable of the same level as X
+                    PW_HIPS(hipMemcpyAsync(p->ckpt_r + (size_t)sgi * fld, Ra, fld * 4, hipMemcpyDeviceToDevice, s));
             }
             if (mode == 0 && (it + I) % K == 0 && it + I < a.nt)  // snapshot just before the next segment
                 s_.snap = (T*)p->ckpt_snap + (size_t)((it + I) / K) * p->nslot * p->nzx;
@@ -1186,6 +1318,10 @@ int recompute_segment(pw_fwi* p, int nb_, int sg, int mode)
     PW_HIPS(hipMemcpyAsync(p->Y, c + fld, fld * sizeof(T), hipMemcpyDeviceToDevice, s));
     T* X = (T*)p->X;
     T* Y = (T*)p->Y;
+    const bool visco = p->visco && sizeof(T) == 4;
+    float* Ra = p->R0;
+    float* Rb = p->R1;
+    if (visco) PW_HIPS(hipMemcpyAsync(Ra, p->ckpt_r + (size_t)sg * fld, fld * 4, hipMemcpyDeviceToDevice, s));
     // imaging needs snapshots up to b0 (inclusive) for segments that are not the last
     const int t_end = (mode == 0 && b0 < a.nt) ? b0 + 1 : b0;
     for (int it = a0; it < t_end; ++it) {
@@ -1194,7 +1330,14 @@ int recompute_segment(pw_fwi* p, int nb_, int sg, int mode)
         s_.fsrc = step_src<T>(p, it);
         if (mode == 1) s_.full = (T*)p->seg + (size_t)(it - a0) * fld;
         else if (it % I == 0) s_.snap = (T*)p->seg + (size_t)((it - a0) / I + 1) * p->nslot * p->nzx;
-        hipLaunchKernelGGL(k_fwd<T>, dim3(ntz * nxc, nb_), dim3(64), 0, s, g, s_, ntz);
+        if (visco) {
+            s_.tau = p->tau_pad; s_.taus = p->taus_pad; s_.r1 = Ra; s_.r2 = Rb;
+            s_.idt = (float)(1. / (double)a.dt);
+            hipLaunchKernelGGL((k_fwd<T, true>), dim3(ntz * nxc, nb_), dim3(64), 0, s, g, s_, ntz);
+            std::swap(Ra, Rb);
+        } else {
+            hipLaunchKernelGGL((k_fwd<T, false>), dim3(ntz * nxc, nb_), dim3(64), 0, s, g, s_, ntz);
+        }
         PW_HIPS(hipGetLastError());
         std::swap(X, Y);
     }
@@ -1238,7 +1381,18 @@ int adjoint_segment(pw_fwi* p, int nb_, int sg, int mode, T** Ap, T** Bp, T* gra
                 im.grad = grad_ref;
                 im.gstride = gstride;
             }
-            hipLaunchKernelGGL(k_adj_img<T>, dim3(ntz * nxc, nb_), dim3(64), 0, s, g, s_, im, ntz);
+            if constexpr (sizeof(T) == 4) {  // the reference mode runs in f32 (checked by the caller)
+                if (p->visco) {
+                    ViscoBack vb{p->tau_pad, p->taus_pad, p->RB0, p->RB1, (float)(1. / (double)a.dt)};
+                    hipLaunchKernelGGL(k_adj_ref<true>, dim3(ntz * nxc, nb_), dim3(64), 0, s, g, s_, im, vb, ntz);
+                    std::swap(p->RB0, p->RB1);
+                } else {
+                    hipLaunchKernelGGL(k_adj_ref<false>, dim3(ntz * nxc, nb_), dim3(64), 0, s, g, s_, im, ViscoBack{},
+                                       ntz);
+                }
+            } else {
+                return PW_ERR_ARG;
+            }
         } else {
             ExArgs<T> ex{};
             ex.F = (const T*)p->seg + (size_t)(it - a0) * fld;
@@ -1294,7 +1448,8 @@ int gradient_all(pw_fwi* p, int mode, double* fcost)
     hipStream_t s = p->ctx->s();
     if (!p->have_wav || !p->have_obs) PW_FAIL(p->ctx, PW_ERR_STATE, "gradient: wavelet or observed data not set");
     if (mode == 0 && p->prec != 4) PW_FAIL(p->ctx, PW_ERR_ARG, "gradient: the reference mode runs in f32");
-    if (p->visco) PW_FAIL(p->ctx, PW_ERR_ARG, "gradient: visco-acoustic plans model only (pw_fwi_set_visco(p, NULL))");
+    if (p->visco && mode != 0)
+        PW_FAIL(p->ctx, PW_ERR_ARG, "gradient: visco-acoustic plans take the reference gradient (gradient_v) only");
     const long long fld = (long long)p->nslot * p->fs;
     int rc = 0;
     double fc = 0.0;
@@ -1314,6 +1469,9 @@ int gradient_all(pw_fwi* p, int mode, double* fcost)
         if ((rc = residual_batch<T>(p, nb_, mode, &fc))) return rc;
         PW_HIPS(hipEventRecord(e1, s));
         if ((rc = fill<T>(p, p->A, fld, T(0))) || (rc = fill<T>(p, p->B, fld, T(0)))) return rc;
+        if (p->visco && sizeof(T) == 4 &&
+            ((rc = fill<float>(p, p->RB0, fld, 0.f)) || (rc = fill<float>(p, p->RB1, fld, 0.f))))
+            return rc;
         T* grad_ref = nullptr;
         long long gstride = 0;
         if (mode == 0) {
@@ -1610,6 +1768,7 @@ int use_source_field(pw_fwi* p, bool on)
         p->shot_host = p->shot_active;
         return PW_OK;
     }
+    if (p->visco) PW_FAIL(p->ctx, PW_ERR_ARG, "source field: the passive path is acoustic (pw_fwi_set_visco(p, NULL))");
     // gradient_pas_av (pywave/src/wave_fwigradient.c:710-713, :807-810): every shot records and
     // injects at r0_v[0] + ir*dr_v, ir < nr; no point source
     const int rlast = a.r0_v[0] + (a.nr - 1) * a.dr_v;
@@ -1673,9 +1832,10 @@ int pw_fwi_set_source_field_dev(pw_fwi* p, const void* d_src)
     return use_source_field(p, true);
 }
 
-// visco-acoustic modelling (forward_modeling, pywave/src/wave_fwimodeling.c:116-220) with the
-// quality factor q (nz, nx) Fortran order and the pararray's f0; NULL returns to acoustic.  Only
-// pw_fwi_forward follows it (f32 plans); gradients refuse a visco plan.
+// visco-acoustic plan with the quality factor q (nz, nx) Fortran order and the pararray's f0: the
+// forward follows forward_modeling (pywave/src/wave_fwimodeling.c:116-220), the reference gradient
+// and L-BFGS follow gradient_v (pywave/src/wave_fwigradient.c:264-452, fwi() with media != 1);
+// f32 plans; the exact gradient and the device entry points refuse it.  NULL returns to acoustic.
 int pw_fwi_set_visco(pw_fwi* p, const float* q)
 {
     if (!p) return PW_ERR_ARG;
@@ -1685,6 +1845,7 @@ int pw_fwi_set_visco(pw_fwi* p, const float* q)
         return PW_OK;
     }
     if (p->prec != 4) PW_FAIL(p->ctx, PW_ERR_ARG, "pw_fwi_set_visco: visco-acoustic modelling runs in f32");
+    if (p->passive) PW_FAIL(p->ctx, PW_ERR_ARG, "pw_fwi_set_visco: the passive path is acoustic (drop the source field)");
     const Acq& a = p->acq;
     const long long nzx = p->nzx;
     // tau, taus (preparation, pywave/src/wave_fwiutil.c:91-97): f32 with the 2 pi f0 q product in double
@@ -1712,6 +1873,9 @@ int pw_fwi_set_visco(pw_fwi* p, const float* q)
     if (!rc && !p->taus_pad) rc = dmalloc(p, (void**)&p->taus_pad, p->fs * 4);
     if (!rc && !p->R0) rc = dmalloc(p, (void**)&p->R0, fld * 4);
     if (!rc && !p->R1) rc = dmalloc(p, (void**)&p->R1, fld * 4);
+    if (!rc && !p->RB0) rc = dmalloc(p, (void**)&p->RB0, fld * 4);
+    if (!rc && !p->RB1) rc = dmalloc(p, (void**)&p->RB1, fld * 4);
+    if (!rc && !p->ckpt_r) rc = dmalloc(p, (void**)&p->ckpt_r, (size_t)p->nseg * fld * 4);
     if (rc) return rc;
     hipStream_t s = p->ctx->s();
     PW_HIPS(hipMemcpyAsync(p->tau_pad, tap.data(), tap.size() * 4, hipMemcpyHostToDevice, s));
@@ -1817,7 +1981,6 @@ int pw_fwi_gradient_f64(pw_fwi* p, const double* vel, int mode, double* grad, do
 int pw_fwi_misfit_f64(pw_fwi* p, const double* vel, double* fcost)
 {
     if (!p || !vel || !fcost) return PW_ERR_ARG;
-    if (p->visco) PW_FAIL(p->ctx, PW_ERR_ARG, "misfit: visco-acoustic plans model only");
     if (!p->have_wav || !p->have_obs) PW_FAIL(p->ctx, PW_ERR_STATE, "misfit: wavelet or observed data not set");
     hipStream_t s = p->ctx->s();
     int rc;
@@ -1874,6 +2037,7 @@ int pw_fwi_adjoint_dev(pw_fwi* p, const void* d_vel, const void* d_gtraces, void
 {
     if (!p || !d_vel || !d_gtraces || !d_gvel) return PW_ERR_ARG;
     if (!p->have_wav) PW_FAIL(p->ctx, PW_ERR_STATE, "adjoint: wavelet not set");
+    if (p->visco) PW_FAIL(p->ctx, PW_ERR_ARG, "adjoint_dev: the exact adjoint is acoustic (visco plans: reference gradient)");
     const Acq& a = p->acq;
     hipStream_t s = p->ctx->s();
     // checkpoints of this very model from the preceding forward_dev (one batch): no forward sweep

@@ -101,10 +101,11 @@ class FwiProblem:
         N.check(self._L.pw_fwi_set_source_field(self._p, N.ptr(f)), self._ctx, "set_source_field")
 
     def set_visco(self, q):
-        """Visco-acoustic modelling (forward_modeling, pywave/src/wave_fwimodeling.c:116-220) with the
-        quality factor q (nz, nx) and the pararray's f0: forward() then carries the memory variable.
-        f32 plans only; gradient(), misfit() and the device entry points refuse a visco plan.  None
-        returns to acoustic modelling."""
+        """Visco-acoustic plan with the quality factor q (nz, nx) and the pararray's f0: forward()
+        then follows forward_modeling (pywave/src/wave_fwimodeling.c:116-220), gradient(mode=
+        "reference"), misfit() and lbfgs() follow gradient_v (pywave/src/wave_fwigradient.c:264-452),
+        i.e. fwi() with media != 1 (pywave/src/wave_fwi.c:40-41).  f32 plans with point sources;
+        the exact gradient and the torch path refuse a visco plan.  None returns to acoustic."""
         if q is None:
             N.check(self._L.pw_fwi_set_visco(self._p, None), self._ctx, "set_visco")
             return

@@ -157,7 +157,14 @@ def fwi_cases():
     # visco-acoustic modelling (forward_modeling, the media=2 path) on the same acquisition
     qv = (20 + 40 * np.random.default_rng(17).random((nz, nx))).astype(np.float32)
     data_visco, _ = refrun.fwimod(par, vtrue, qv, wav, visco=True)
-    save("fwi_visco", par=par, vtrue=vtrue, q=qv, wav=wav, data=data_visco)
+    # its gradient (gradient_v: the gradient fwi() takes for media != 1), two calls, interval 1 and
+    # 4, and the fwi() trajectory with it
+    gv, fcv, _ = refrun.fwigrad(par, v0, qv, wav, data_visco, vel_second=(v0 * 1.01).astype(np.float32),
+                                visco=True)
+    gv4, fcv4, _ = refrun.fwigrad(par4, v0, qv, wav, data_visco, visco=True)
+    vinv_v, _ = refrun.fwi(par, v0, qv, wav, data_visco, visco=True)
+    save("fwi_visco", par=par, par4=par4, vtrue=vtrue, v0=v0, q=qv, wav=wav, data=data_visco, grad1=gv[0],
+         grad2=gv[1], fcost=fcv, grad_int4=gv4[0], fcost_int4=fcv4, vinv=vinv_v)
 
     # marine-style acquisition (acqui_type 2), sources marching across the model
     par2 = np.array([30, 48, 0.01, 0.01, 0, 0, 300, 0.001, 0, 0, 2, 0.15, 0.0, 5, 16, 0.01, 10, 2, 1, 1],

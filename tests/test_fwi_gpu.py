@@ -87,7 +87,49 @@ def test_visco_forward_bit_exact(golden):
         p.set_visco(g["q"])
         assert np.array_equal(p.forward(g["vtrue"]), g["data"]), batch
     with pytest.raises(ValueError, match="visco"):
-        p.gradient(g["vtrue"])
+        p.gradient(g["vtrue"], "exact")
+
+
+@pytest.mark.parametrize("ckpt", [0, 7, 130])
+def test_visco_gradient_bit_exact(golden, ckpt):
+    # gradient_v (the gradient fwi() takes for media != 1), one shot per launch: bit-exact, also
+    # when the forward memory variable is restored from checkpoints for recomputed segments
+    g = golden("fwi_visco")
+    p = FwiProblem(g["par"], g["wav"], data=g["data"], batch=1, checkpoint=ckpt)
+    p.set_visco(g["q"])
+    g1, f1, sc = p.gradient(g["v0"], "reference")
+    assert np.array_equal(g1, g["grad1"])
+    assert abs(f1 - float(g["fcost"][0])) <= 1e-3 * abs(f1)
+    g2, _, _ = p.gradient((g["v0"] * 1.01).astype(np.float32), "reference", scaling=sc)
+    assert np.array_equal(g2, g["grad2"])
+
+
+def test_visco_gradient_interval4_batched_and_misfit(golden):
+    g = golden("fwi_visco")
+    p = FwiProblem(g["par4"], g["wav"], data=g["data"], batch=1, checkpoint=12)
+    p.set_visco(g["q"])
+    g4, _, _ = p.gradient(g["v0"], "reference")
+    assert np.array_equal(g4, g["grad_int4"])
+    b = FwiProblem(g["par"], g["wav"], data=g["data"], batch=3, checkpoint=40)
+    b.set_visco(g["q"])
+    gb, fb, _ = b.gradient(g["v0"], "reference")
+    assert rel_l2(gb, g["grad1"]) <= 1e-5
+    assert abs(b.misfit(g["v0"]) - fb) <= 1e-6 * fb
+    # switching the plan back to acoustic gives the acoustic gradient of the same data
+    b.set_visco(None)
+    ga, _, _ = b.gradient(g["v0"], "reference")
+    assert rel_l2(ga, gb) > 0.05
+
+
+def test_visco_lbfgs_matches_reference(golden):
+    # fwi() with media = 2: the reference's trajectory to f32 round-off of the models (the
+    # optimiser's dot products are f64 here)
+    g = golden("fwi_visco")
+    p = FwiProblem(g["par"], g["wav"], data=g["data"], batch=3)
+    p.set_visco(g["q"])
+    vinv, k = p.lbfgs(g["v0"])
+    assert k == g["vinv"].shape[2]
+    assert np.abs(vinv - g["vinv"]).max() < 1e-4
 
 
 def test_pfwi_media2_forward_modelingc(golden):

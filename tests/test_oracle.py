@@ -453,3 +453,36 @@ def test_visco_forward_tends_to_acoustic_at_high_q(golden):
     assert rel_l2(v["data"], g["data"]) > 0.05
     # attenuation: the low-Q data carry less energy than the acoustic data
     assert np.linalg.norm(v["data"]) < np.linalg.norm(g["data"])
+
+
+def test_visco_gradient_bit_exact(golden):
+    # gradient_v (the gradient fwi() takes for media != 1): two calls sharing the first call's
+    # scaling, and interval 4, against the compiled reference
+    g = golden("fwi_visco")
+    g1, f1, sc = O.gradient_av(g["par"], g["v0"], g["wav"], g["data"], q=g["q"])
+    assert np.array_equal(g1, g["grad1"]) and np.float32(f1) == g["fcost"][0]
+    g2, f2, _ = O.gradient_av(g["par"], (g["v0"] * 1.01).astype(np.float32), g["wav"], g["data"], scaling=sc,
+                              q=g["q"])
+    assert np.array_equal(g2, g["grad2"]) and np.float32(f2) == g["fcost"][1]
+    g4, f4, _ = O.gradient_av(g["par4"], g["v0"], g["wav"], g["data"], q=g["q"])
+    assert np.array_equal(g4, g["grad_int4"]) and np.float32(f4) == g["fcost_int4"][0]
+
+
+def test_visco_fwi_trajectory_bit_exact(golden):
+    g = golden("fwi_visco")
+    vinv, k = O.fwi(g["par"], g["v0"], g["wav"], g["data"], q=g["q"])
+    assert k == int(g["par"][19])
+    assert np.array_equal(vinv, g["vinv"])
+
+
+def test_visco_gradient_tends_to_acoustic_at_high_q(golden):
+    # with tau -> 0 the backward memory term vanishes and tmp -> p1: gradient_v approaches
+    # gradient_av on the same data; at Q 20-60 the two differ
+    g = golden("fwi_small")
+    q_hi = np.full(g["v0"].shape, 1e5, np.float32)
+    gv, fv, _ = O.gradient_av(g["par"], g["v0"], g["wav"], g["data"], q=q_hi)
+    ga, fa, _ = O.gradient_av(g["par"], g["v0"], g["wav"], g["data"])
+    assert rel_l2(gv, ga) < 2e-3 and abs(fv - fa) < 2e-3 * fa
+    v = golden("fwi_visco")
+    gl, _, _ = O.gradient_av(g["par"], g["v0"], g["wav"], g["data"], q=v["q"])
+    assert rel_l2(gl, ga) > 0.05
