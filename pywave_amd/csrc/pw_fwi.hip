@@ -39,6 +39,40 @@ constexpr double PI_D = 3.14159265358979323846264338328;
 constexpr int PMAX = 7;  // largest source-map patch edge (rect <= 4)
 constexpr int XC = 16;   // x columns streamed per block
 
+// Tiles of the step kernels: 64 z cells x XC columns of one shot slot, one wave each.
+struct TileMap {
+    int ntz, nxc;  // z tiles and column groups per slot
+    int nblk;      // tiles over all slots of the launch
+};
+
+TileMap tile_map(int padnz, int padnx, int nslots)
+{
+    TileMap t;
+    t.ntz = (padnz + 63) / 64;
+    t.nxc = (padnx + XC - 1) / XC;
+    t.nblk = t.ntz * t.nxc * nslots;
+    return t;
+}
+
+// blocks to launch for a tile map: a multiple of the 8 XCDs
+inline dim3 tile_grid(const TileMap& t) { return dim3((unsigned)((t.nblk + 7) / 8 * 8)); }
+
+// XCD-aware tile order: the hardware deals blocks round-robin over the 8 XCDs; each XCD takes a
+// contiguous range of tiles (z tile fastest, then column group, then slot), so column neighbours,
+// which read each other's halo columns, share an L2.  False for the padding blocks.
+__device__ __forceinline__ bool tile_of(const TileMap& t, int& slot, int& tz, int& xc)
+{
+    const int per = gridDim.x >> 3;
+    const int lb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (lb >= t.nblk) return false;
+    const int per_slot = t.ntz * t.nxc;
+    slot = lb / per_slot;
+    const int r = lb - slot * per_slot;
+    tz = r % t.ntz;
+    xc = r / t.ntz;
+    return true;
+}
+
 template <typename T>
 struct Coef {
     // pywave/src/wave_fwiutil.c:17 (f32 constants from double literals; f64 for T = double)
@@ -110,14 +144,14 @@ using pwbuf::bsto;
 
 // ---------------------------------------------------------------- forward step (+ modelling)
 // One 64 x XC tile of one shot per block (one wave), streamed along x with a 9-column register
-// queue (radius 4) and an LDS z row; the loads of column x+1 (previous level, model, z halo,
-// sponge pair) and the centre of column x+5 are in flight while column x is computed.
+// queue (radius 4) and an LDS z row; the operands of column x+1 (previous level, model, z halo,
+// sponge pair, and the queue entry x+5) are in flight while column x is computed.
 template <typename T, bool VISCO = false>
-__global__ __launch_bounds__(64) void k_fwd(FGeo<T> g, StepArgs<T> a, int ntz)
+__global__ __launch_bounds__(64) void k_fwd(FGeo<T> g, StepArgs<T> a, TileMap tm)
 {
     using C = Coef<T>;
-    const int slot = blockIdx.y;
-    const int tz = blockIdx.x % ntz, xc = blockIdx.x / ntz;
+    int slot, tz, xc;
+    if (!tile_of(tm, slot, tz, xc)) return;
     const int lane = threadIdx.x, z0 = tz * 64, z = z0 + lane;
     const int x0 = xc * XC, x1 = min(x0 + XC, g.pnx);
     __shared__ T row[2][72];
@@ -158,95 +192,105 @@ __global__ __launch_bounds__(64) void k_fwd(FGeo<T> g, StepArgs<T> a, int ntz)
     auto ldv = [&](__amdgpu_buffer_rsrc_t r, int x, bool ok) {
         return bldo<float>(r, ok ? oz4 : kOOB, ok ? (unsigned)x * oc4 : 0u);
     };
+    // operands of one column: its previous level, model, z halo, passive source and visco
+    // fields, sponge pair, and the queue entry four columns ahead; fetched one column early
+    struct Col {
+        T q, y, m, h, f, xl, xh;
+        float ta, ts, r1;
+    };
+    auto fetch = [&](int xx) -> Col {
+        const bool v = xx < x1;
+        Col c;
+        c.q = ldc(rX, oz, xx + 4, v && xx + 4 < g.pnx);
+        c.y = ldc(rY, oz, xx, v);
+        c.m = ldc(rM, oz, xx, v);
+        c.h = ldc(rX, ozh, xx, v);
+        c.f = (v && a.fsrc) ? ldf(xx) : T(0);
+        c.ta = 0.f; c.ts = 1.f; c.r1 = 0.f;
+        if constexpr (VISCO) {
+            c.ta = ldv(rTa, xx, v);
+            c.ts = ldv(rTs, xx, v);
+            c.r1 = ldv(rR1, xx, v);
+        }
+        const int xs = xx < g.pnx ? xx : g.pnx - 1;
+        c.xl = a.sfx[2 * xs];
+        c.xh = a.sfx[2 * xs + 1];
+        return c;
+    };
+    // tile-uniform flags: the per-lane work below runs only in tiles that hold the receiver row,
+    // snapshot rows or source rows, behind scalar branches
+    const bool rec_rows = a.rec && g.rz >= z0 && g.rz < z0 + 64;
+    const bool snap_rows = a.snap && z0 < g.nb + g.nz && z0 + 64 > g.nb;
+    const bool src_rows = sh.ph > 0 && sh.sz0 < z0 + 64 && sh.sz0 + sh.ph > z0;
+    const bool fsrc_rows = a.fsrc && z0 < g.nb + g.nz && z0 + 64 > g.nb;
     T q[9];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) {
+    for (int k = 0; k < 8; ++k) {
         const int xx = x0 - 4 + k;
         q[k] = ldc(rX, oz, xx, xv(xx));
     }
-    T yv = ldc(rY, oz, x0, true);
-    T mv = ldc(rM, oz, x0, true);
-    T hz = ldc(rX, ozh, x0, true);
-    T fv = ldf(x0);
-    float tav = 0.f, tsv = 1.f, r1v = 0.f;
-    if constexpr (VISCO) {
-        tav = ldv(rTa, x0, true);
-        tsv = ldv(rTs, x0, true);
-        r1v = ldv(rR1, x0, true);
-    }
-    T xl = a.sfx[2 * x0], xh = a.sfx[2 * x0 + 1];
+    Col pend = fetch(x0);
     int buf = 0;
     for (int x = x0; x < x1; ++x) {
-        const int xn = x + 1;
-        const bool nv = xn < x1;
-        const T nq = ldc(rX, oz, x + 5, nv && x + 5 < g.pnx);
-        const T nyv = ldc(rY, oz, xn, nv);
-        const T nmv = ldc(rM, oz, xn, nv);
-        const T nhz = ldc(rX, ozh, xn, nv);
-        const T nfv = ldf(xn);
-        float ntav = 0.f, ntsv = 1.f, nr1v = 0.f;
-        if constexpr (VISCO) {
-            ntav = ldv(rTa, xn, nv);
-            ntsv = ldv(rTs, xn, nv);
-            nr1v = ldv(rR1, xn, nv);
-        }
-        const int xs = xn < g.pnx ? xn : g.pnx - 1;
-        const T nxl = a.sfx[2 * xs], nxh = a.sfx[2 * xs + 1];
+        const Col cur = pend;
+        pend = fetch(x + 1);
+        q[8] = cur.q;
+        const T yv = cur.y, mv = cur.m, hz = cur.h, xl = cur.xl, xh = cur.xh;
         row[buf][lane + 4] = q[4];
         if (lane < 8) row[buf][lzh] = hz;
         __syncthreads();
-        if (zok) {
-            const long long col = (long long)x * g.ldz;
-            const T p1 = q[4];
-            if (a.rec && z == g.rz) {  // record before the update (wave_fwigradient.c:136-139)
-                T* r = a.rec + slot * a.trs + (long long)a.it * g.nr;
-                if (g.dr_v > 0) {
-                    int d = x - sh.r0_v;
-                    if (d >= 0 && d % g.dr_v == 0 && d / g.dr_v < sh.nr2) {
-                        int k = sh.r02 + d / g.dr_v;
-                        if (k >= 0 && k < g.nr) r[k] = p1;
-                    }
-                } else if (x == sh.r0_v) {
-                    for (int ir = 0; ir < sh.nr2; ++ir)
-                        if (sh.r02 + ir >= 0 && sh.r02 + ir < g.nr) r[sh.r02 + ir] = p1;
+        const T p1 = q[4];
+        if (rec_rows && z == g.rz) {  // record before the update (wave_fwigradient.c:136-139)
+            T* r = a.rec + slot * a.trs + (long long)a.it * g.nr;
+            if (g.dr_v > 0) {
+                int d = x - sh.r0_v;
+                if (d >= 0 && d % g.dr_v == 0 && d / g.dr_v < sh.nr2) {
+                    int k = sh.r02 + d / g.dr_v;
+                    if (k >= 0 && k < g.nr) r[k] = p1;
                 }
+            } else if (x == sh.r0_v) {
+                for (int ir = 0; ir < sh.nr2; ++ir)
+                    if (sh.r02 + ir >= 0 && sh.r02 + ir < g.nr) r[sh.r02 + ir] = p1;
             }
-            if (a.snap && z >= g.nb && z < g.nb + g.nz && x >= g.nb && x < g.nb + g.nx)
-                a.snap[slot * a.snaps + (long long)(x - g.nb) * g.nz + (z - g.nb)] = p1;
-            if (a.full) a.full[slot * a.fulls + col + z] = p1;
-            T term = T(0);
-            if (zin && x >= 4 && x < g.pnx - 4) {
-                const T* R = &row[buf][lane + 4];
-                term = (C::K0 * p1 + C::K1 * (q[5] + q[3]) + C::K2 * (q[6] + q[2]) + C::K3 * (q[7] + q[1]) +
-                        C::K4 * (q[8] + q[0])) / g.dx2 +
-                       (C::K0 * p1 + C::K1 * (R[1] + R[-1]) + C::K2 * (R[2] + R[-2]) + C::K3 * (R[3] + R[-3]) +
-                        C::K4 * (R[4] + R[-4])) / g.dz2;
-            }
-            const int px = x - sh.sx0, pz = z - sh.sz0;
-            const bool inp = px >= 0 && px < sh.pw && pz >= 0 && pz < sh.ph;
-            if constexpr (VISCO) {
-                // forward_modeling (wave_fwimodeling.c:186-192): r2 and the source term formed in
-                // double where the reference's expressions promote, stored in f32
-                const float tf = (float)term;
-                const double num = (double)(tav / tsv * tf) + ((double)a.idt - 0.5 / (double)tsv) * (double)r1v;
-                const float r2v = (float)(num / ((double)a.idt + 0.5 / (double)tsv));
-                const float sv = (inp ? (float)a.patch[slot * PMAX * PMAX + px * PMAX + pz] : 0.f) * (float)wv;
-                const double tt = (double)tf * (1. + (double)tav) - (double)(r2v + r1v) * 0.5;
-                term = (T)(float)(tt + (double)sv);
-                bsto(rR2, oz4, (unsigned)x * oc4, (float)sponge((T)r2v, zl, zh, xl, xh));
-            } else {
-                if (inp) term = term + a.patch[slot * PMAX * PMAX + px * PMAX + pz] * wv;
-            }
-            if (fzin && fxin(x)) term = term + fv;
-            const T p0 = sponge(yv, zl, zh, xl, xh);
-            T p2 = 2 * p1 - p0 + mv * term;
-            bsto(rY, oz, (unsigned)x * oc, sponge(p2, zl, zh, xl, xh));
         }
+        if (snap_rows && x >= g.nb && x < g.nb + g.nx && z >= g.nb && z < g.nb + g.nz)
+            a.snap[slot * a.snaps + (long long)(x - g.nb) * g.nz + (z - g.nb)] = p1;
+        if (a.full && zok) a.full[slot * a.fulls + (long long)x * g.ldz + z] = p1;
+        // the Laplacian on [4, pad-4); formed on every lane (its operands are in range) and
+        // selected
+        const T* R = &row[buf][lane + 4];
+        const T lap = (C::K0 * p1 + C::K1 * (q[5] + q[3]) + C::K2 * (q[6] + q[2]) + C::K3 * (q[7] + q[1]) +
+                       C::K4 * (q[8] + q[0])) / g.dx2 +
+                      (C::K0 * p1 + C::K1 * (R[1] + R[-1]) + C::K2 * (R[2] + R[-2]) + C::K3 * (R[3] + R[-3]) +
+                       C::K4 * (R[4] + R[-4])) / g.dz2;
+        T term = (zin && x >= 4 && x < g.pnx - 4) ? lap : T(0);
+        bool inp = false;
+        int px = 0, pz = 0;
+        if (src_rows && x >= sh.sx0 && x < sh.sx0 + sh.pw) {
+            px = x - sh.sx0;
+            pz = z - sh.sz0;
+            inp = pz >= 0 && pz < sh.ph;
+        }
+        if constexpr (VISCO) {
+            // forward_modeling (wave_fwimodeling.c:186-192): r2 and the source term formed in
+            // double where the reference's expressions promote, stored in f32
+            const float tf = (float)term;
+            const float tav = cur.ta, tsv = cur.ts, r1v = cur.r1;
+            const double num = (double)(tav / tsv * tf) + ((double)a.idt - 0.5 / (double)tsv) * (double)r1v;
+            const float r2v = (float)(num / ((double)a.idt + 0.5 / (double)tsv));
+            const float sv = (inp ? (float)a.patch[slot * PMAX * PMAX + px * PMAX + pz] : 0.f) * (float)wv;
+            const double tt = (double)tf * (1. + (double)tav) - (double)(r2v + r1v) * 0.5;
+            term = (T)(float)(tt + (double)sv);
+            bsto(rR2, oz4, (unsigned)x * oc4, (float)sponge((T)r2v, zl, zh, xl, xh));
+        } else {
+            if (inp) term = term + a.patch[slot * PMAX * PMAX + px * PMAX + pz] * wv;
+        }
+        if (fsrc_rows && fzin && fxin(x)) term = term + cur.f;
+        const T p0 = sponge(yv, zl, zh, xl, xh);
+        T p2 = 2 * p1 - p0 + mv * term;
+        bsto(rY, oz, (unsigned)x * oc, sponge(p2, zl, zh, xl, xh));
 #pragma unroll
         for (int k = 0; k < 8; ++k) q[k] = q[k + 1];
-        q[8] = nq;
-        yv = nyv; mv = nmv; hz = nhz; fv = nfv; xl = nxl; xh = nxh;
-        tav = ntav; tsv = ntsv; r1v = nr1v;
         buf ^= 1;
     }
 }
@@ -292,11 +336,11 @@ __device__ __forceinline__ float visco_back(float p1, float r1, float ta, float 
 // column are loaded one iteration ahead through buffer resources.
 template <bool VISCO>
 __global__ __launch_bounds__(64) void k_adj_ref(FGeo<float> g, StepArgs<float> a, ImgArgs<float> im, ViscoBack vb,
-                                                int ntz)
+                                                TileMap tm)
 {
     using C = Coef<float>;
-    const int slot = blockIdx.y;
-    const int tz = blockIdx.x % ntz, xc = blockIdx.x / ntz;
+    int slot, tz, xc;
+    if (!tile_of(tm, slot, tz, xc)) return;
     const int lane = threadIdx.x, z0 = tz * 64, z = z0 + lane;
     const int x0 = xc * XC, x1 = min(x0 + XC, g.pnx);
     __shared__ float row[2][72];
@@ -477,11 +521,11 @@ struct ExArgs {
 // with dJ/dm += (L X_t + source) q_{t+1}.  A = q_{t+1}, B = q_{t+2} (overwritten by q_t), F = the
 // forward field X_t of this step; the m*A product is zero outside the Laplacian's rows [4, n-4).
 template <typename T>
-__global__ __launch_bounds__(64) void k_adj_exact(FGeo<T> g, StepArgs<T> a, ExArgs<T> ex, int ntz)
+__global__ __launch_bounds__(64) void k_adj_exact(FGeo<T> g, StepArgs<T> a, ExArgs<T> ex, TileMap tm)
 {
     using C = Coef<T>;
-    const int slot = blockIdx.y;
-    const int tz = blockIdx.x % ntz, xc = blockIdx.x / ntz;
+    int slot, tz, xc;
+    if (!tile_of(tm, slot, tz, xc)) return;
     const int lane = threadIdx.x, z0 = tz * 64, z = z0 + lane;
     const int x0 = xc * XC, x1 = min(x0 + XC, g.pnx);
     __shared__ T rowa[2][72];
@@ -1252,7 +1296,7 @@ int forward_batch(pw_fwi* p, int nb_, bool ckpt, int mode, bool store_last)
     const Acq& a = p->acq;
     hipStream_t s = p->ctx->s();
     FGeo<T> g = geo<T>(p);
-    const int ntz = (a.padnz + 63) / 64, nxc = (a.padnx + XC - 1) / XC;
+    const TileMap tm = tile_map(a.padnz, a.padnx, nb_);
     const long long fld = (long long)p->nslot * p->fs;
     int rc = fill<T>(p, p->X, fld, T(0));
     if (!rc) rc = fill<T>(p, p->Y, fld, T(0));
@@ -1293,8 +1337,8 @@ int forward_batch(pw_fwi* p, int nb_, bool ckpt, int mode, bool store_last)
                 else if (it % I == 0) s_.snap = (T*)p->seg + (size_t)((it - a_last) / I + 1) * p->nslot * p->nzx;
             }
         }
-        if (visco) hipLaunchKernelGGL((k_fwd<T, true>), dim3(ntz * nxc, nb_), dim3(64), 0, s, g, s_, ntz);
-        else hipLaunchKernelGGL((k_fwd<T, false>), dim3(ntz * nxc, nb_), dim3(64), 0, s, g, s_, ntz);
+        if (visco) hipLaunchKernelGGL((k_fwd<T, true>), tile_grid(tm), dim3(64), 0, s, g, s_, tm);
+        else hipLaunchKernelGGL((k_fwd<T, false>), tile_grid(tm), dim3(64), 0, s, g, s_, tm);
         PW_HIPS(hipGetLastError());
         std::swap(X, Y);
         if (visco) std::swap(Ra, Rb);
@@ -1309,7 +1353,7 @@ int recompute_segment(pw_fwi* p, int nb_, int sg, int mode)
     const Acq& a = p->acq;
     hipStream_t s = p->ctx->s();
     FGeo<T> g = geo<T>(p);
-    const int ntz = (a.padnz + 63) / 64, nxc = (a.padnx + XC - 1) / XC;
+    const TileMap tm = tile_map(a.padnz, a.padnx, nb_);
     const long long fld = (long long)p->nslot * p->fs;
     const int I = a.interval, K = p->K;
     const int a0 = sg * K, b0 = std::min(a0 + K, a.nt);
@@ -1333,10 +1377,10 @@ int recompute_segment(pw_fwi* p, int nb_, int sg, int mode)
         if (visco) {
             s_.tau = p->tau_pad; s_.taus = p->taus_pad; s_.r1 = Ra; s_.r2 = Rb;
             s_.idt = (float)(1. / (double)a.dt);
-            hipLaunchKernelGGL((k_fwd<T, true>), dim3(ntz * nxc, nb_), dim3(64), 0, s, g, s_, ntz);
+            hipLaunchKernelGGL((k_fwd<T, true>), tile_grid(tm), dim3(64), 0, s, g, s_, tm);
             std::swap(Ra, Rb);
         } else {
-            hipLaunchKernelGGL((k_fwd<T, false>), dim3(ntz * nxc, nb_), dim3(64), 0, s, g, s_, ntz);
+            hipLaunchKernelGGL((k_fwd<T, false>), tile_grid(tm), dim3(64), 0, s, g, s_, tm);
         }
         PW_HIPS(hipGetLastError());
         std::swap(X, Y);
@@ -1351,7 +1395,7 @@ int adjoint_segment(pw_fwi* p, int nb_, int sg, int mode, T** Ap, T** Bp, T* gra
     const Acq& a = p->acq;
     hipStream_t s = p->ctx->s();
     FGeo<T> g = geo<T>(p);
-    const int ntz = (a.padnz + 63) / 64, nxc = (a.padnx + XC - 1) / XC;
+    const TileMap tm = tile_map(a.padnz, a.padnx, nb_);
     const long long fld = (long long)p->nslot * p->fs;
     const int I = a.interval, K = p->K;
     const int a0 = sg * K, b0 = std::min(a0 + K, a.nt);
@@ -1384,11 +1428,10 @@ int adjoint_segment(pw_fwi* p, int nb_, int sg, int mode, T** Ap, T** Bp, T* gra
             if constexpr (sizeof(T) == 4) {  // the reference mode runs in f32 (checked by the caller)
                 if (p->visco) {
                     ViscoBack vb{p->tau_pad, p->taus_pad, p->RB0, p->RB1, (float)(1. / (double)a.dt)};
-                    hipLaunchKernelGGL(k_adj_ref<true>, dim3(ntz * nxc, nb_), dim3(64), 0, s, g, s_, im, vb, ntz);
+                    hipLaunchKernelGGL(k_adj_ref<true>, tile_grid(tm), dim3(64), 0, s, g, s_, im, vb, tm);
                     std::swap(p->RB0, p->RB1);
                 } else {
-                    hipLaunchKernelGGL(k_adj_ref<false>, dim3(ntz * nxc, nb_), dim3(64), 0, s, g, s_, im, ViscoBack{},
-                                       ntz);
+                    hipLaunchKernelGGL(k_adj_ref<false>, tile_grid(tm), dim3(64), 0, s, g, s_, im, ViscoBack{}, tm);
                 }
             } else {
                 return PW_ERR_ARG;
@@ -1398,7 +1441,7 @@ int adjoint_segment(pw_fwi* p, int nb_, int sg, int mode, T** Ap, T** Bp, T* gra
             ex.F = (const T*)p->seg + (size_t)(it - a0) * fld;
             ex.fsl = p->fs;
             ex.gm = (T*)p->gm;
-            hipLaunchKernelGGL(k_adj_exact<T>, dim3(ntz * nxc, nb_), dim3(64), 0, s, g, s_, ex, ntz);
+            hipLaunchKernelGGL(k_adj_exact<T>, tile_grid(tm), dim3(64), 0, s, g, s_, ex, tm);
         }
         PW_HIPS(hipGetLastError());
         std::swap(A, B);
