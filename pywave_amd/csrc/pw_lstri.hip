@@ -449,6 +449,8 @@ void launch_step(pw_lstri* p, bool adj, int nslot, const TriArgs& a)
 void step(pw_lstri* p, bool adj, int nslot, const TriArgs& a)
 {
     switch (p->xct) {
+        case 1: launch_step<1>(p, adj, nslot, a); break;
+        case 2: launch_step<2>(p, adj, nslot, a); break;
         case 4: launch_step<4>(p, adj, nslot, a); break;
         case 8: launch_step<8>(p, adj, nslot, a); break;
         default: launch_step<16>(p, adj, nslot, a); break;
@@ -622,7 +624,7 @@ pw_lstri* pw_lstri_create(pw_ctx* ctx, const float* pararray)
     p->ntz = (g.pz + 63) / 64;
     // narrow column chunks when the grid is small, so that a step still has several blocks per CU
     p->xct = 16;
-    while (p->xct > 4 && (long long)p->ntz * ((g.px + p->xct - 1) / p->xct) < 1024) p->xct /= 2;
+    while (p->xct > 1 && (long long)p->ntz * ((g.px + p->xct - 1) / p->xct) < 1024) p->xct /= 2;
     int rc = ls_alloc(p, &p->v2, g.fs);
     if (!rc) rc = ls_alloc(p, &p->sfz, 2LL * g.pz);
     if (!rc) rc = ls_alloc(p, &p->sfx, 2LL * g.px);
