@@ -173,7 +173,10 @@ def fwi_cases():
     q2 = np.full((30, 48), 1e4, np.float32)
     wav2 = (1e6 * refrun.ricker(300, 0.001, 12, 0.06, 1)).astype(np.float32)
     data2, _ = refrun.fwimod(par2, v2, q2, wav2)
-    save("fwi_marine", par=par2, vel=v2, wav=wav2, data=data2)
+    # and its visco-acoustic modelling (forward_modeling) with a Q of 15-45
+    q2v = (15 + 30 * np.random.default_rng(23).random((30, 48))).astype(np.float32)
+    data2v, _ = refrun.fwimod(par2, v2, q2v, wav2, visco=True)
+    save("fwi_marine", par=par2, vel=v2, wav=wav2, data=data2, q_visco=q2v, data_visco=data2v)
 
 
 def passive_field(nz, nx, nt, events, dt=0.001):

@@ -32,6 +32,9 @@ def test_forward_marine_geometry(golden):
     m = golden("fwi_marine")
     p = FwiProblem(m["par"], m["wav"], batch=2)
     assert np.array_equal(p.forward(m["vel"]), m["data"])
+    # visco-acoustic modelling on the same moving-spread acquisition
+    p.set_visco(m["q_visco"])
+    assert np.array_equal(p.forward(m["vel"]), m["data_visco"])
 
 
 @pytest.mark.parametrize("ckpt", [0, 7, 50, 130])

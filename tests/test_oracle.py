@@ -455,6 +455,14 @@ def test_visco_forward_tends_to_acoustic_at_high_q(golden):
     assert np.linalg.norm(v["data"]) < np.linalg.norm(g["data"])
 
 
+def test_visco_forward_marine_bit_exact(golden):
+    # moving-spread acquisition (acqui_type 2: per-shot receiver windows) with Q 15-45
+    g = golden("fwi_marine")
+    d = O.fwi_forward_visco(g["par"], g["vel"], g["q_visco"], g["wav"])
+    assert np.array_equal(d, g["data_visco"])
+    assert rel_l2(g["data_visco"], g["data"]) > 0.05
+
+
 def test_visco_gradient_bit_exact(golden):
     # gradient_v (the gradient fwi() takes for media != 1): two calls sharing the first call's
     # scaling, and interval 4, against the compiled reference
