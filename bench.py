@@ -21,6 +21,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0  # MI355X HBM3E
+CPU_PROCS = 16  # worker processes of the multi-process CPU baselines: the host-core share of one GPU
 
 
 def cpu_baseline(w, nt_cpu):
@@ -125,6 +126,26 @@ def fwi_bench(plain_shots):
     return out
 
 
+def cpu_model():
+    """The host CPU's model name (first /proc/cpuinfo entry)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def run_concurrently(fn, procs):
+    """Run fn() procs times at once (each call runs one reference process and returns its engine
+    seconds); returns the slowest engine time."""
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(procs) as ex:
+        return max(ex.map(lambda _: fn(), range(procs)))
+
+
 def fwi_cpu_baseline(par, w, nt_cpu=300):
     """Reference C gradient_av (pywave/src/wave_fwigradient.c:93-262, single thread) for one shot
     of the same 400x800 geometry over the first nt_cpu of 3000 steps (its per-step cost does not
@@ -143,10 +164,18 @@ def fwi_cpu_baseline(par, w, nt_cpu=300):
         q = np.ones_like(w["v0"])
         _, _, el = refrun.fwigrad(pa, w["v0"], q, wav, data)
         per_shot = el * par["nt"] / nt_cpu
-        return dict(value=round(1.0 / per_shot, 5), unit="shots/s", cores=1, kind="reference",
-                    sample="reference C gradient_av, one shot of the 400x800 (+60 sponge) geometry, %d of %d steps "
-                           "in %.1f s, extrapolated to %.1f s per shot; single thread, -O2 -fwrapv"
-                           % (nt_cpu, par["nt"], el, per_shot))
+        out = dict(value=round(1.0 / per_shot, 5), unit="shots/s", cores=1, kind="reference",
+                   sample="reference C gradient_av, one shot of the 400x800 (+60 sponge) geometry, %d of %d steps "
+                          "in %.1f s, extrapolated to %.1f s per shot; single thread, -O2 -fwrapv"
+                          % (nt_cpu, par["nt"], el, per_shot))
+        # shots are independent: the same sample in CPU_PROCS processes at once
+        procs = min(CPU_PROCS, os.cpu_count() or 1)
+        elp = run_concurrently(lambda: refrun.fwigrad(pa, w["v0"], q, wav, data)[2], procs)
+        out["parallel"] = dict(value=round(procs / (elp * par["nt"] / nt_cpu), 5), unit="shots/s", procs=procs,
+                               cpu_model=cpu_model(),
+                               sample="%d concurrent reference processes, each the same one-shot sample; slowest "
+                                      "%.1f s" % (procs, elp))
+        return out
     except Exception as e:  # reported, never required
         return dict(value=None, unit="shots/s", cores=1, kind="reference", sample="failed: %s" % e)
 
@@ -164,9 +193,19 @@ def batch_cpu_baseline(shots, total_cu, nt_cpu=400):
         _, el = refrun.afd2d(s0["vel"], nt_cpu, s0["dt"], s0["d"][1], s0["d"][0], 30, 0.01, src, 1)
         cells = (nz + 60) * (nx + 60) * nt_cpu
         rate = cells / el
-        return dict(value=round(len(shots) / (total_cu / rate), 5), unit="shots/s", cores=1, kind="reference",
-                    sample="reference C afd2d on shot 0 (%dx%d +30 ABC), %d steps in %.2f s = %.3f Gcell/s, "
-                           "extrapolated to the 96 shots; single thread" % (nz, nx, nt_cpu, el, rate / 1e9))
+        out = dict(value=round(len(shots) / (total_cu / rate), 5), unit="shots/s", cores=1, kind="reference",
+                   sample="reference C afd2d on shot 0 (%dx%d +30 ABC), %d steps in %.2f s = %.3f Gcell/s, "
+                          "extrapolated to the 96 shots; single thread" % (nz, nx, nt_cpu, el, rate / 1e9))
+        # shots are independent: the same sample in CPU_PROCS processes at once
+        procs = min(CPU_PROCS, os.cpu_count() or 1)
+        elp = run_concurrently(
+            lambda: refrun.afd2d(s0["vel"], nt_cpu, s0["dt"], s0["d"][1], s0["d"][0], 30, 0.01, src, 1)[1], procs)
+        prate = procs * cells / elp
+        out["parallel"] = dict(value=round(len(shots) / (total_cu / prate), 5), unit="shots/s", procs=procs,
+                               cpu_model=cpu_model(),
+                               sample="%d concurrent reference processes, each the same shot-0 sample; slowest %.2f s "
+                                      "= %.3f Gcell/s aggregate" % (procs, elp, prate / 1e9))
+        return out
     except Exception as e:
         return dict(value=None, unit="shots/s", cores=1, kind="reference", sample="failed: %s" % e)
 

@@ -210,6 +210,10 @@ int pw_lstri_invert(pw_lstri *p, const float *data, int niter, float *src, int *
 /* on != 0 (default): the inversion replays its two sweeps from hipGraphs captured on the first
  * call; 0 launches them step by step.  Results are identical.  No counterpart in the reference. */
 int pw_lstri_use_graph(pw_lstri *p, int on);
+/* time steps per sweep launch: 1, 2, or 0 for the plan's choice (two steps per launch on grids of at
+ * most 2^20 padded cells, where a step is latency-bound); identical results either way.  No
+ * counterpart in the reference (a scheduling choice of this engine). */
+int pw_lstri_steps_per_launch(pw_lstri *p, int n);
 /* device milliseconds of the last pw_lstri_model / pw_lstri_invert call */
 int pw_lstri_timing(pw_lstri *p, double *ms);
 /* pfwicfun.lstric (called by pywave/pfwi.py:160 and :247): pararray[9] (inv) != 0 -> datasrc =

@@ -88,6 +88,7 @@ SIGNATURES = {
     "pw_lstri_invert": (c_int, [c_vp, c_vp, c_int, c_vp, c_vp]),
     "pw_lstri_timing": (c_int, [c_vp, c_vp]),
     "pw_lstri_use_graph": (c_int, [c_vp, c_int]),
+    "pw_lstri_steps_per_launch": (c_int, [c_vp, c_int]),
     "pw_lstric": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_ll, c_ll]),
     "pw_tri_create": (c_vp, [c_vp, c_int, c_int, c_int, c_float, c_float, c_float, c_int, c_float, c_int, c_float,
                              c_int, c_int]),

@@ -235,6 +235,226 @@ __global__ __launch_bounds__(64) void k_tra(TriGeo g, TriArgs a, int ntz)
     }
 }
 
+// ---------------------------------------------------------------- two time steps per launch
+// On the small grids of the passive workloads a step launch is latency-bound, so the sweeps can
+// advance two steps per launch: a block owns an OZ2 x OX2 tile, stages the current level on the
+// tile grown by 8 cells in LDS, forms the first step on the tile grown by 4 (LDS) and the second
+// step on the tile, each cell with the single-step kernels' expressions, so every value equals
+// that of two single steps bit for bit.  Both new levels go to two spare fields, since the
+// neighbouring blocks still read the current and previous levels during the launch.
+constexpr int OZ2 = 56, OX2 = 4;             // output tile
+constexpr int Z1 = OZ2 + 8, X1 = OX2 + 8;    // first-step region
+constexpr int Z0 = OZ2 + 16, X0 = OX2 + 16;  // staged current level
+constexpr int T2 = 256;                      // threads per block
+
+struct TriArgs2 {
+    const float* X;  // current level (modelling: u1; adjoint: A)
+    const float* Y;  // previous level (decayed once)
+    float* C;        // out: the first step's level (the next launch's previous level)
+    float* D;        // out: the second step's level (the next launch's current level)
+    const float* v2;
+    const float* sfz;
+    const float* sfx;
+    float* plane0;   // plane of the first step (modelling: source, read; adjoint: image, written)
+    float* plane1;   // plane of the second step
+    long long pss;
+    float* row0;     // row of the first step (modelling: recorded; adjoint: injected)
+    float* row1;     // row of the second step
+    long long rss;
+};
+
+template <bool ADJ>
+__global__ __launch_bounds__(T2) void k_tr2(TriGeo g, TriArgs2 a, int ntz2)
+{
+    __shared__ float s0[X0][Z0 + 1];            // current level
+    __shared__ float p0[ADJ ? X0 : 1][Z0 + 1];  // adjoint: current level x v^2dt^2
+    __shared__ float s1[X1][Z1 + 1];            // first step's level
+    __shared__ float p1[ADJ ? X1 : 1][Z1 + 1];  // adjoint: first step's level x v^2dt^2
+    __shared__ float yv[X1][Z1 + 1];            // previous level on the first-step region
+    __shared__ float vr[X1][Z1 + 1];            // v^2dt^2 there
+    __shared__ float sp[ADJ ? 1 : X1][Z1 + 1];  // modelling: first step's source plane there
+    __shared__ float fz[Z0][2], fx[X0][2];      // decay factor pairs of the staged rows and columns
+    __shared__ float rd0[X1];                   // adjoint: first step's injected row there
+    const int slot = blockIdx.y;
+    const int tz = blockIdx.x % ntz2, tx = blockIdx.x / ntz2;
+    const int zo = tz * OZ2, xo = tx * OX2;
+    const long long so = slot * g.fs;
+    const float* __restrict__ X = a.X + so;
+    const float* __restrict__ Y = a.Y + so;
+    auto ing = [&](int z, int x) { return z >= 0 && z < g.pz && x >= 0 && x < g.px; };
+    auto inr = [&](int z, int x) { return z >= 4 && z < g.pz - 4 && x >= 4 && x < g.px - 4; };
+    auto mz = [&](int z) { return z >= g.nb && z < g.nb + g.nz; };
+    auto mx = [&](int x) { return x >= g.nb && x < g.nb + g.nx; };
+    auto mi = [&](int z, int x) { return slot * a.pss + (long long)(x - g.nb) * g.nz + (z - g.nb); };
+    // every global operand of both steps is loaded into registers up front (all loads in flight
+    // together), then written to LDS, so the launch waits on memory once
+    constexpr int N0 = (Z0 * X0 + T2 - 1) / T2, N1 = (Z1 * X1 + T2 - 1) / T2, N2 = (OZ2 * OX2 + T2 - 1) / T2;
+    float r0[N0], rv0[ADJ ? N0 : 1], ry[N1], rvv[N1], rsp[ADJ ? 1 : N1], pl1[N2];
+#pragma unroll
+    for (int j = 0; j < N0; ++j) {
+        const int e = threadIdx.x + j * T2;
+        const int zz = e % Z0, xx = e / Z0;
+        const int z = zo - 8 + zz, x = xo - 8 + xx;
+        const bool in = e < Z0 * X0 && ing(z, x);
+        const long long k = (long long)x * g.ldz + z;
+        r0[j] = in ? X[k] : 0.f;
+        if constexpr (ADJ) rv0[j] = in ? a.v2[k] : 0.f;
+    }
+#pragma unroll
+    for (int j = 0; j < N1; ++j) {
+        const int e = threadIdx.x + j * T2;
+        const int zz = e % Z1, xx = e / Z1;
+        const int z = zo - 4 + zz, x = xo - 4 + xx;
+        const bool in = e < Z1 * X1 && ing(z, x);
+        const long long k = (long long)x * g.ldz + z;
+        ry[j] = in ? Y[k] : 0.f;
+        rvv[j] = in ? a.v2[k] : 0.f;
+        if constexpr (!ADJ) rsp[j] = (in && mz(z) && mx(x)) ? a.plane0[mi(z, x)] : 0.f;
+    }
+    // the second step's own operands for this thread's output cells
+#pragma unroll
+    for (int j = 0; j < N2; ++j) {
+        const int e = threadIdx.x + j * T2;
+        const int zz = e % OZ2, xx = e / OZ2;
+        const int z = zo + zz, x = xo + xx;
+        pl1[j] = 0.f;
+        if constexpr (!ADJ) {
+            if (e < OZ2 * OX2 && ing(z, x) && mz(z) && mx(x)) pl1[j] = a.plane1[mi(z, x)];
+        } else {
+            if (e < OZ2 * OX2 && ing(z, x) && z == g.depth && mx(x)) pl1[j] = a.row1[slot * a.rss + (x - g.nb)];
+        }
+    }
+    {
+        // decay pairs (row zo-8+t, column xo-8+t) and the adjoint's injected row
+        const int t = threadIdx.x;
+        const int z = zo - 8 + t, x = xo - 8 + t;
+        const bool zv = t < Z0 && z >= 0 && z < g.pz, xv = t < X0 && x >= 0 && x < g.px;
+        const float a0 = zv ? a.sfz[2 * z] : 1.f, a1 = zv ? a.sfz[2 * z + 1] : 1.f;
+        const float b0 = xv ? a.sfx[2 * x] : 1.f, b1 = xv ? a.sfx[2 * x + 1] : 1.f;
+        const int xr = xo - 4 + t;
+        float d0 = 0.f;
+        if constexpr (ADJ)
+            if (t < X1 && xr >= g.nb && xr < g.nb + g.nx) d0 = a.row0[slot * a.rss + (xr - g.nb)];
+        if (t < Z0) { fz[t][0] = a0; fz[t][1] = a1; }
+        if (t < X0) { fx[t][0] = b0; fx[t][1] = b1; }
+        if constexpr (ADJ)
+            if (t < X1) rd0[t] = d0;
+    }
+#pragma unroll
+    for (int j = 0; j < N0; ++j) {
+        const int e = threadIdx.x + j * T2;
+        if (e < Z0 * X0) {
+            const int zz = e % Z0, xx = e / Z0;
+            s0[xx][zz] = r0[j];
+            if constexpr (ADJ) p0[xx][zz] = r0[j] * rv0[j];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < N1; ++j) {
+        const int e = threadIdx.x + j * T2;
+        if (e < Z1 * X1) {
+            const int zz = e % Z1, xx = e / Z1;
+            yv[xx][zz] = ry[j];
+            vr[xx][zz] = rvv[j];
+            if constexpr (!ADJ) sp[xx][zz] = rsp[j];
+        }
+    }
+    __syncthreads();
+    // first step on the tile grown by 4
+    for (int e = threadIdx.x; e < Z1 * X1; e += T2) {
+        const int zz = e % Z1, xx = e / Z1;
+        const int z = zo - 4 + zz, x = xo - 4 + xx;
+        const int cz = zz + 4, cx = xx + 4;  // the cell in s0
+        float v = 0.f;
+        if (ing(z, x)) {
+            const float zl = fz[cz][0], zh = fz[cz][1], xl = fx[cx][0], xh = fx[cx][1];
+            const float u1 = s0[cx][cz];
+            const float vv = vr[xx][zz];
+            float u2 = 0.f;  // the 4-cell rim stays zero
+            if constexpr (!ADJ) {
+                if (inr(z, x)) {
+                    const float lx = W4 * (s0[cx + 4][cz] + s0[cx - 4][cz]) + W3 * (s0[cx + 3][cz] + s0[cx - 3][cz]) +
+                                     W2 * (s0[cx + 2][cz] + s0[cx - 2][cz]) + W1 * (s0[cx + 1][cz] + s0[cx - 1][cz]) +
+                                     W0 * u1;
+                    const float lz = W4 * (s0[cx][cz + 4] + s0[cx][cz - 4]) + W3 * (s0[cx][cz + 3] + s0[cx][cz - 3]) +
+                                     W2 * (s0[cx][cz + 2] + s0[cx][cz - 2]) + W1 * (s0[cx][cz + 1] + s0[cx][cz - 1]) +
+                                     W0 * u1;
+                    u2 = (lx * g.sx + lz * g.sz) * vv + 2.0f * u1 - decay(yv[xx][zz], zl, zh, xl, xh);
+                }
+                if (mz(z) && mx(x)) u2 = u2 + sp[xx][zz] * vv;
+            } else {
+                if (inr(z, x)) {
+                    const float c = p0[cx][cz];
+                    const float lx = W4 * (p0[cx + 4][cz] + p0[cx - 4][cz]) + W3 * (p0[cx + 3][cz] + p0[cx - 3][cz]) +
+                                     W2 * (p0[cx + 2][cz] + p0[cx - 2][cz]) + W1 * (p0[cx + 1][cz] + p0[cx - 1][cz]) +
+                                     W0 * c;
+                    const float lz = W4 * (p0[cx][cz + 4] + p0[cx][cz - 4]) + W3 * (p0[cx][cz + 3] + p0[cx][cz - 3]) +
+                                     W2 * (p0[cx][cz + 2] + p0[cx][cz - 2]) + W1 * (p0[cx][cz + 1] + p0[cx][cz - 1]) +
+                                     W0 * c;
+                    u2 = (lx * g.sx + lz * g.sz) + 2.0f * u1 - decay(yv[xx][zz], zl, zh, xl, xh);
+                }
+                if (z == g.depth && mx(x)) u2 = u2 + rd0[xx];
+            }
+            v = decay(u2, zl, zh, xl, xh);
+            if (zz >= 4 && zz < 4 + OZ2 && xx >= 4 && xx < 4 + OX2) a.C[so + (long long)x * g.ldz + z] = v;
+            if constexpr (ADJ) p1[xx][zz] = v * vv;
+        } else if constexpr (ADJ) {
+            p1[xx][zz] = 0.f;
+        }
+        s1[xx][zz] = v;
+    }
+    __syncthreads();
+    // second step on the tile
+#pragma unroll
+    for (int j = 0; j < (OZ2 * OX2 + T2 - 1) / T2; ++j) {
+        const int e = threadIdx.x + j * T2;
+        const int zz = e % OZ2, xx = e / OZ2;
+        const int z = zo + zz, x = xo + xx;
+        if (e >= OZ2 * OX2 || !ing(z, x)) continue;
+        const int cz = zz + 4, cx = xx + 4;  // the cell in s1
+        const float zl = fz[zz + 8][0], zh = fz[zz + 8][1], xl = fx[xx + 8][0], xh = fx[xx + 8][1];
+        const float u1 = s1[cx][cz];
+        const float prev = s0[xx + 8][zz + 8];  // the launch's current level is this step's previous one
+        const bool rcell = z == g.depth && mx(x);
+        float u2 = 0.f;
+        if constexpr (!ADJ) {
+            if (rcell) {  // record both steps' levels before they are stepped
+                a.row0[slot * a.rss + (x - g.nb)] = 0.f + prev;
+                a.row1[slot * a.rss + (x - g.nb)] = 0.f + u1;
+            }
+            const float vv = vr[cx][cz];
+            if (inr(z, x)) {
+                const float lx = W4 * (s1[cx + 4][cz] + s1[cx - 4][cz]) + W3 * (s1[cx + 3][cz] + s1[cx - 3][cz]) +
+                                 W2 * (s1[cx + 2][cz] + s1[cx - 2][cz]) + W1 * (s1[cx + 1][cz] + s1[cx - 1][cz]) +
+                                 W0 * u1;
+                const float lz = W4 * (s1[cx][cz + 4] + s1[cx][cz - 4]) + W3 * (s1[cx][cz + 3] + s1[cx][cz - 3]) +
+                                 W2 * (s1[cx][cz + 2] + s1[cx][cz - 2]) + W1 * (s1[cx][cz + 1] + s1[cx][cz - 1]) +
+                                 W0 * u1;
+                u2 = (lx * g.sx + lz * g.sz) * vv + 2.0f * u1 - decay(prev, zl, zh, xl, xh);
+            }
+            if (mz(z) && mx(x)) u2 = u2 + pl1[j] * vv;
+        } else {
+            if (mz(z) && mx(x)) {  // images of both steps, u1 v^2dt^2 on the model
+                const long long m = mi(z, x);
+                a.plane0[m] = 0.f + p0[xx + 8][zz + 8];
+                a.plane1[m] = 0.f + p1[cx][cz];
+            }
+            if (inr(z, x)) {
+                const float c = p1[cx][cz];
+                const float lx = W4 * (p1[cx + 4][cz] + p1[cx - 4][cz]) + W3 * (p1[cx + 3][cz] + p1[cx - 3][cz]) +
+                                 W2 * (p1[cx + 2][cz] + p1[cx - 2][cz]) + W1 * (p1[cx + 1][cz] + p1[cx - 1][cz]) +
+                                 W0 * c;
+                const float lz = W4 * (p1[cx][cz + 4] + p1[cx][cz - 4]) + W3 * (p1[cx][cz + 3] + p1[cx][cz - 3]) +
+                                 W2 * (p1[cx][cz + 2] + p1[cx][cz - 2]) + W1 * (p1[cx][cz + 1] + p1[cx][cz - 1]) +
+                                 W0 * c;
+                u2 = (lx * g.sx + lz * g.sz) + 2.0f * u1 - decay(prev, zl, zh, xl, xh);
+            }
+            if (rcell) u2 = u2 + pl1[j];
+        }
+        a.D[so + (long long)x * g.ldz + z] = decay(u2, zl, zh, xl, xh);
+    }
+}
+
 // ---------------------------------------------------------------- small kernels
 __global__ void k_tri_pad_v2(TriGeo g, const float* __restrict__ vel, float dt2, float* __restrict__ v2)
 {
@@ -373,8 +593,9 @@ struct pw_lstri {
     long long nzx = 0, nm = 0, nd = 0;  // model plane, model (per shot), data (per shot) sizes
     float* v2 = nullptr;
     float *sfz = nullptr, *sfx = nullptr;
-    float* fields = nullptr;  // 2 * cap_slots fields
+    float* fields = nullptr;  // 4 * cap_slots fields (two levels, and two for the two-step launches)
     int cap_slots = 0;
+    int two_step = 0;         // sweeps advance two steps per launch (small grids)
     float *src = nullptr, *dat = nullptr, *stage = nullptr;  // batch buffers (modelling / imaging)
     int cap_src_slots = 0;
     float *x = nullptr, *gvec = nullptr, *S = nullptr, *rr = nullptr, *gg = nullptr, *Ss = nullptr;
@@ -433,7 +654,7 @@ int ensure_fields(pw_lstri* p, int slots)
     if (p->fields) hipFree(p->fields);
     p->fields = nullptr;
     p->cap_slots = 0;
-    int rc = ls_alloc(p, &p->fields, 2LL * slots * p->g.fs);
+    int rc = ls_alloc(p, &p->fields, 4LL * slots * p->g.fs);
     if (!rc) p->cap_slots = slots;
     return rc;
 }
@@ -444,6 +665,15 @@ void launch_step(pw_lstri* p, bool adj, int nslot, const TriArgs& a)
     dim3 grid(p->ntz * ((p->g.px + XCT - 1) / XCT), nslot);
     if (adj) hipLaunchKernelGGL(k_tra<XCT>, grid, dim3(64), 0, p->ctx->s(), p->g, a, p->ntz);
     else hipLaunchKernelGGL(k_trm<XCT>, grid, dim3(64), 0, p->ctx->s(), p->g, a, p->ntz);
+}
+
+// two steps per launch (k_tr2): a.X, a.Y in; a.C, a.D out
+void step2(pw_lstri* p, bool adj, int nslot, const TriArgs2& a)
+{
+    const int ntz2 = (p->g.pz + OZ2 - 1) / OZ2, ntx2 = (p->g.px + OX2 - 1) / OX2;
+    dim3 grid(ntz2 * ntx2, nslot);
+    if (adj) hipLaunchKernelGGL(k_tr2<true>, grid, dim3(T2), 0, p->ctx->s(), p->g, a, ntz2);
+    else hipLaunchKernelGGL(k_tr2<false>, grid, dim3(T2), 0, p->ctx->s(), p->g, a, ntz2);
 }
 
 void step(pw_lstri* p, bool adj, int nslot, const TriArgs& a)
@@ -463,9 +693,28 @@ int sweep_model(pw_lstri* p, int nslot, const float* w, float* d)
     int rc = ensure_fields(p, nslot);
     if (!rc) rc = zero(p, p->fields, 2LL * nslot * p->g.fs);
     if (rc) return rc;
+    const long long fl = (long long)nslot * p->g.fs;
     float* X = p->fields;
-    float* Y = p->fields + (long long)nslot * p->g.fs;
-    for (int it = 0; it < p->nt; ++it) {
+    float* Y = p->fields + fl;
+    float* C = p->fields + 2 * fl;
+    float* D = p->fields + 3 * fl;
+    int it0 = 0;
+    if (p->two_step) {
+        for (; it0 + 1 < p->nt; it0 += 2) {
+            TriArgs2 a{};
+            a.X = X; a.Y = Y; a.C = C; a.D = D; a.v2 = p->v2; a.sfz = p->sfz; a.sfx = p->sfx;
+            a.plane0 = const_cast<float*>(w) + (long long)it0 * p->nzx;
+            a.plane1 = const_cast<float*>(w) + (long long)(it0 + 1) * p->nzx;
+            a.pss = p->nm;
+            a.row0 = d + (long long)it0 * p->nx;
+            a.row1 = d + (long long)(it0 + 1) * p->nx;
+            a.rss = p->nd;
+            step2(p, false, nslot, a);
+            std::swap(X, D);  // the second step's level is current, the first step's previous
+            std::swap(Y, C);
+        }
+    }
+    for (int it = it0; it < p->nt; ++it) {
         TriArgs a{};
         a.X = X; a.Y = Y; a.v2 = p->v2; a.sfz = p->sfz; a.sfx = p->sfx;
         a.plane = const_cast<float*>(w) + (long long)it * p->nzx;
@@ -485,9 +734,28 @@ int sweep_adjoint(pw_lstri* p, int nslot, const float* d, float* w)
     int rc = ensure_fields(p, nslot);
     if (!rc) rc = zero(p, p->fields, 2LL * nslot * p->g.fs);
     if (rc) return rc;
+    const long long fl = (long long)nslot * p->g.fs;
     float* A = p->fields;
-    float* B = p->fields + (long long)nslot * p->g.fs;
-    for (int it = p->nt - 1; it >= 0; --it) {
+    float* B = p->fields + fl;
+    float* C = p->fields + 2 * fl;
+    float* D = p->fields + 3 * fl;
+    int it0 = p->nt - 1;
+    if (p->two_step) {
+        for (; it0 >= 1; it0 -= 2) {
+            TriArgs2 a{};
+            a.X = A; a.Y = B; a.C = C; a.D = D; a.v2 = p->v2; a.sfz = p->sfz; a.sfx = p->sfx;
+            a.plane0 = w + (long long)it0 * p->nzx;
+            a.plane1 = w + (long long)(it0 - 1) * p->nzx;
+            a.pss = p->nm;
+            a.row0 = const_cast<float*>(d) + (long long)it0 * p->nx;
+            a.row1 = const_cast<float*>(d) + (long long)(it0 - 1) * p->nx;
+            a.rss = p->nd;
+            step2(p, true, nslot, a);
+            std::swap(A, D);
+            std::swap(B, C);
+        }
+    }
+    for (int it = it0; it >= 0; --it) {
         TriArgs a{};
         a.X = A; a.Y = B; a.v2 = p->v2; a.sfz = p->sfz; a.sfx = p->sfx;
         a.plane = w + (long long)it * p->nzx;
@@ -625,6 +893,8 @@ pw_lstri* pw_lstri_create(pw_ctx* ctx, const float* pararray)
     // narrow column chunks when the grid is small, so that a step still has several blocks per CU
     p->xct = 16;
     while (p->xct > 1 && (long long)p->ntz * ((g.px + p->xct - 1) / p->xct) < 1024) p->xct /= 2;
+    // and two steps per launch when even one-column tiles leave a step latency-bound
+    p->two_step = (long long)g.pz * g.px <= (1LL << 20);
     int rc = ls_alloc(p, &p->v2, g.fs);
     if (!rc) rc = ls_alloc(p, &p->sfz, 2LL * g.pz);
     if (!rc) rc = ls_alloc(p, &p->sfx, 2LL * g.px);
@@ -855,6 +1125,17 @@ int pw_lstri_use_graph(pw_lstri* p, int on)
 {
     if (!p) return PW_ERR_ARG;
     p->use_graph = on != 0;
+    return PW_OK;
+}
+
+// time steps per launch of the sweeps: 1, 2, or 0 for the plan's choice (two on grids of at most
+// 2^20 padded cells); the results are the same bit for bit
+int pw_lstri_steps_per_launch(pw_lstri* p, int n)
+{
+    if (!p || n < 0 || n > 2) return PW_ERR_ARG;
+    const int two = n == 0 ? ((long long)p->g.pz * p->g.px <= (1LL << 20)) : (n == 2);
+    if (two != p->two_step) drop_graphs(p);  // captured sweeps hold the other launch sequence
+    p->two_step = two;
     return PW_OK;
 }
 

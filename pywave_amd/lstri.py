@@ -90,6 +90,11 @@ class LstriProblem:
         """Replay the inversion's sweeps from captured hipGraphs (default) or launch step by step."""
         N.check(self._L.pw_lstri_use_graph(self._p, int(bool(on))), self._ctx, "use_graph")
 
+    def steps_per_launch(self, n=0):
+        """Time steps per sweep launch: 1, 2, or 0 for the plan's choice (two on small grids, where a
+        step launch is latency-bound).  The results do not depend on it."""
+        N.check(self._L.pw_lstri_steps_per_launch(self._p, int(n)), self._ctx, "steps_per_launch")
+
     def timing(self):
         """Device milliseconds of the last model() / invert() call."""
         ms = ctypes.c_double()
