@@ -32,15 +32,24 @@ from .fwi import make_pararray
 __all__ = ["pfwi", "fillpar"]
 
 
+def _col(a, n, name, at_least=False):
+    """a flattened in Fortran order to f32, checked to hold n samples (at least n when at_least):
+    the engine entry points read exactly that many (the reference reads them unchecked)."""
+    v = np.asarray(a).flatten(order="F").astype(np.float32)
+    if (v.size < n) if at_least else (v.size != n):
+        raise ValueError("pfwi: %s holds %d samples, %s%d expected" % (name, v.size, "at least " if at_least else "", n))
+    return v
+
+
 def pfwi(vel, q, wav, src, data=None, mode=1, media=1, inv=0, verb=1, nb=60, coef=0.005, acq=1, ss=[0, 0.2, 1],
          f0=10, waterz=-30, niter=30, conv_e=0.01, par=None):
     par = fillpar(par)
     if mode == 1:
-        vel = vel.flatten(order="F").astype(np.float32)
-        q = q.flatten(order="F").astype(np.float32)
-        wav = wav.flatten(order="F").astype(np.float32)
+        nz, nt, nx, ns = int(par["nz"]), int(par["nt"]), int(par["nx"]), int(par["ns"])
+        vel = _col(vel, nz * nx, "vel")
+        q = _col(q, nz * nx, "q") if media != 1 else np.asarray(q).flatten(order="F").astype(np.float32)
+        wav = _col(wav, nt, "wav", at_least=True)
         pararray = make_pararray(par)
-        nt, nx, ns = int(par["nt"]), int(par["nx"]), int(par["ns"])
         out = np.empty(nt * nx * ns, np.float32)
         L, ctx = N.lib(), N.context()
         if media == 1:
@@ -52,12 +61,13 @@ def pfwi(vel, q, wav, src, data=None, mode=1, media=1, inv=0, verb=1, nb=60, coe
         data = out.reshape(par["nt"], par["nx"], par["ns"], order="F")
         vinv = []; grad = []; mwt = []; src = []
     elif mode == 2:
-        vel = vel.flatten(order="F").astype(np.float32)
-        q = q.flatten(order="F").astype(np.float32)
-        wav = wav.flatten(order="F").astype(np.float32)
-        data = data.flatten(order="F").astype(np.float32)
-        pararray = make_pararray(par)
         nz, nx, niter = int(par["nz"]), int(par["nx"]), int(par["niter"])
+        nt, ns = int(par["nt"]), int(par["ns"])
+        vel = _col(vel, nz * nx, "vel")
+        q = np.asarray(q).flatten(order="F").astype(np.float32)
+        wav = _col(wav, nt, "wav", at_least=True)
+        data = _col(data, nt * nx * ns, "data")
+        pararray = make_pararray(par)
         out = np.empty(nz * nx * niter, np.float32)
         L, ctx = N.lib(), N.context()
         N.check(L.pw_fwic(ctx, N.ptr(vel), N.ptr(q), N.ptr(wav), N.ptr(data), N.ptr(pararray), N.ptr(out),
@@ -73,15 +83,16 @@ def pfwi(vel, q, wav, src, data=None, mode=1, media=1, inv=0, verb=1, nb=60, coe
             return _lstric_inversion(vel, q, wav, data, par)
         if par["onlyvel"] != True:  # noqa: E712
             raise UnboundLocalError("local variable 'grad' referenced before assignment")
-        vel = vel.flatten(order="F").astype(np.float32)
-        q = q.flatten(order="F").astype(np.float32)
-        wav = wav.flatten(order="F").astype(np.float32)
+        nz, nx, niter = int(par["nz"]), int(par["nx"]), int(par["niter"])
+        nt, ns = int(par["nt"]), int(par["ns"])
+        vel = _col(vel, nz * nx, "vel")
+        q = np.asarray(q).flatten(order="F").astype(np.float32)
+        wav = _col(wav, nt, "wav", at_least=True)
         par["inv"] = 1
         pararray = make_pararray(par)
-        data = data.flatten(order="F").astype(np.float32)
-        src = src.flatten(order="F").astype(np.float32)
+        data = _col(data, nt * nx * ns, "data")
+        src = _col(src, nz * nx * nt * ns, "src")
         datasrc = np.concatenate([data, src], axis=0)
-        nz, nx, niter = int(par["nz"]), int(par["nx"]), int(par["niter"])
         out = np.empty(nz * nx * niter, np.float32)
         L, ctx = N.lib(), N.context()
         N.check(L.pw_pfwic(ctx, N.ptr(vel), N.ptr(q), N.ptr(wav), N.ptr(datasrc), N.ptr(pararray), N.ptr(out),
@@ -95,7 +106,7 @@ def pfwi(vel, q, wav, src, data=None, mode=1, media=1, inv=0, verb=1, nb=60, coe
 
 
 def _lstric(vel, q, wav, datasrc, par, out_len):
-    vel = vel.flatten(order="F").astype(np.float32)
+    vel = _col(vel, int(par["nz"]) * int(par["nx"]), "vel")
     q = q.flatten(order="F").astype(np.float32)
     wav = wav.flatten(order="F").astype(np.float32)
     datasrc = np.ascontiguousarray(datasrc.flatten(order="F").astype(np.float32))

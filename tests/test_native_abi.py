@@ -104,3 +104,23 @@ def test_pfwi_mode4_branches_without_device():
     with pytest.raises(UnboundLocalError):  # joint inversion: the reference's return fails
         pywave_amd.pfwi(v, v, w, src, data=np.zeros((10, 8, 1)), mode=4,
                         par=dict(par, inv=True, onlysrc=False, onlyvel=False))
+
+
+def test_pfwi_checks_input_lengths_before_device_work():
+    # the engine entry points read nz*nx model samples, nt wavelet samples and nt*nx*ns data
+    # samples unchecked (as the reference does); the wrapper rejects other lengths first
+    par = {"nz": 8, "nx": 8, "dz": 0.01, "dx": 0.01, "oz": 0, "ox": 0, "nt": 10, "dt": 0.001, "ot": 0,
+           "ns": 1, "nb": 8, "niter": 1}
+    v = np.full((8, 8), 2.0, np.float32)
+    w = np.zeros(10, np.float32)
+    with pytest.raises(ValueError, match="vel"):
+        pywave_amd.pfwi(v[:, :7], v, w, None, mode=1, par=dict(par))
+    with pytest.raises(ValueError, match="q"):
+        pywave_amd.pfwi(v, v[:4], w, None, mode=1, media=2, par=dict(par))
+    with pytest.raises(ValueError, match="wav"):
+        pywave_amd.pfwi(v, v, w[:9], None, mode=1, par=dict(par))
+    with pytest.raises(ValueError, match="data"):
+        pywave_amd.pfwi(v, v, w, None, data=np.zeros((10, 8, 2)), mode=2, par=dict(par))
+    with pytest.raises(ValueError, match="src"):
+        pywave_amd.pfwi(v, v, w, np.zeros((8, 8, 10, 2)), data=np.zeros((10, 8, 1)), mode=4,
+                        par=dict(par, inv=True, onlysrc=False, onlyvel=True))
