@@ -2042,8 +2042,9 @@ static int afd_oneshot(pw_ctx* ctx, int ndim, const float* vel, const float* src
                        float dx, float dy, float dz, float* out, long long out_len, int spectral = 0)
 {
     if (!ctx) return PW_ERR_ARG;
-    if (!vel || !src || !out) PW_FAIL(ctx, PW_ERR_ARG, "afd: null array");
-    if (!tri && ns < 1) PW_FAIL(ctx, PW_ERR_ARG, "afd: ns must be >= 1 for modelling");
+    if (!vel || !out || (!src && (tri || ns > 0))) PW_FAIL(ctx, PW_ERR_ARG, "afd: null array");
+    // ns = 0 models no source: zero traces and snapshots, as the reference's loops over no sources
+    if (!tri && ns < 0) PW_FAIL(ctx, PW_ERR_ARG, "afd: ns must be >= 0 for modelling");
     if (jsnap < 0) jsnap = 0;
     if (out_len != pw_afd_output_len(nt, nz, nx, ny, jsnap))
         PW_FAIL(ctx, PW_ERR_ARG, "afd: output length mismatch");
