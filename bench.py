@@ -366,6 +366,26 @@ def aps3d_bench(cpu_steps=20):
     return res
 
 
+def fwi_forward_bench(nt=300):
+    """The acoustic forward of the FWI workload (400x800 +60 sponge, 32 shots in one launch per
+    step) over its first nt steps, twice: a short run of the FWI step kernel for counter passes."""
+    from pywave_amd import FwiProblem
+    from pywave_amd.workloads import fwi2d
+
+    w = fwi2d(nt=nt)
+    par = w["par"]
+    wav = (w["amp"] * ricker_wavelet(nt, par["dt"], w["f0"], w["t0"])).astype(np.float32)
+    p = FwiProblem(par, wav, batch=par["ns"])
+    p.forward(w["vtrue"])
+    p.forward(w["vtrue"])
+    ms = p.timing()[0]
+    p.close()
+    cells = (par["nz"] + 2 * par["nb"]) * (par["nx"] + 2 * par["nb"]) * par["ns"]
+    return {"workload": "fwi2d acoustic forward, 32 shots, first %d steps" % nt, "device_ms": round(ms, 2),
+            "step_us": round(ms * 1e3 / nt, 2),
+            "achieved_GBs": round(16.0 * cells / (ms * 1e-3 / nt) / 1e9, 1)}
+
+
 def visco_bench(cpu_steps=300):
     """Visco-acoustic forward modelling (forward_modeling, pywave/src/wave_fwimodeling.c:116-220,
     the media=2 modelling of pfwi) on the FWI workload's 400x800, 32-shot, nt=3000 acquisition
@@ -567,7 +587,7 @@ def main():
     ap.add_argument("--fwi-plain-shots", type=int, default=4)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (HBM counter calibration)")
-    ap.add_argument("--only", choices=["fwi", "tri", "batch", "small", "lstri", "aps", "visco"],
+    ap.add_argument("--only", choices=["fwi", "tri", "batch", "small", "lstri", "aps", "visco", "fwifwd"],
                     help="run one secondary workload alone and print its JSON object (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -582,7 +602,8 @@ def main():
         leg = {"fwi": lambda: fwi_bench(args.fwi_plain_shots), "tri": tri_bench, "batch": batch_bench,
                "small": afd2d_small_bench, "lstri": lstri_bench,
                "aps": lambda: aps3d_bench(0 if args.no_cpu else 20),
-               "visco": lambda: visco_bench(0 if args.no_cpu else 300)}[args.only]
+               "visco": lambda: visco_bench(0 if args.no_cpu else 300),
+               "fwifwd": fwi_forward_bench}[args.only]
         print(json.dumps({args.only: leg()}))
         return
 
