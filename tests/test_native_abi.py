@@ -124,3 +124,21 @@ def test_pfwi_checks_input_lengths_before_device_work():
     with pytest.raises(ValueError, match="src"):
         pywave_amd.pfwi(v, v, w, np.zeros((8, 8, 10, 2)), data=np.zeros((10, 8, 1)), mode=4,
                         par=dict(par, inv=True, onlysrc=False, onlyvel=True))
+
+
+def test_cfun_length_checks_before_device_work():
+    # afdcfun's NULL-without-exception on a velocity length mismatch surfaces as SystemError
+    with pytest.raises(SystemError):
+        pywave_amd.afd2dc(np.zeros(10, np.float32), np.zeros(5, np.float32), 0, 10, 4, 3, 1, 0, 0, 0, 1, 2, 0.01,
+                          0.001, 0, 10, 0, 10)
+    with pytest.raises(SystemError):
+        pywave_amd.aps3dc(np.zeros(10, np.float32), np.zeros(6, np.float32), 0, 10, 2, 2, 2, 1, 0, 0, 0, 1, 2, 0.01,
+                          0.001, 0, 10, 0, 10, 0, 10)
+    with pytest.raises(ValueError):  # a source pack shorter than 5*ns
+        pywave_amd.afd2dc(np.zeros(12, np.float32), np.zeros(4, np.float32), 0, 10, 4, 3, 1, 0, 0, 0, 1, 2, 0.01,
+                          0.001, 0, 10, 0, 10)
+    par = np.array([8, 8, 0.01, 0.01, 0, 0, 10, 0.001, 0, 0, 1, 0.2, 0, 5, 8, 0.01, 10, 1, 1, 1], np.float32)
+    with pytest.raises(ValueError):
+        pywave_amd.forward_modeling_ac(np.zeros(63, np.float32), np.zeros(64, np.float32), np.zeros(10), par)
+    with pytest.raises(ValueError):
+        pywave_amd.fwic(np.zeros(64, np.float32), np.zeros(64), np.zeros(10), np.zeros(79), par)
