@@ -1,0 +1,47 @@
+"""The FWI gradient check of the benchmark configuration at its full size: the 400x800 model
+(+60-cell sponge), 32 shots in one batch, nt = 3000, checkpoints every 50 steps.  The exact-adjoint
+gradient (f64 plan) is compared with central finite differences of the f64 misfit at 20 interior
+cells drawn with seed 4 at least 10 cells from the edges, h = 1e-3 v; bar: 1e-3 relative."""
+import numpy as np
+import pytest
+
+from pywave_amd import FwiProblem
+from pywave_amd import _native as N
+from pywave_amd.workloads import fwi2d
+
+pytestmark = pytest.mark.gpu
+
+
+def _wavelet(nt, dt, f0, t0, amp):
+    w = np.zeros(nt, np.float32)
+    N.check(N.lib().pw_ricker(nt, dt, f0, t0, 1.0, N.ptr(w)), None, "ricker")
+    return (amp * w).astype(np.float32)
+
+
+@pytest.mark.timeout(600)
+def test_full_size_gradient_vs_finite_differences():
+    w = fwi2d()
+    par = w["par"]
+    nt, ns, nz, nx = par["nt"], par["ns"], par["nz"], par["nx"]
+    wav = _wavelet(nt, par["dt"], w["f0"], w["t0"], w["amp"])
+    gen = FwiProblem(par, wav, batch=ns)
+    obs = gen.forward(w["vtrue"])
+    gen.close()
+    p = FwiProblem(par, wav, data=obs, batch=ns, checkpoint=50, precision="f64")
+    v0 = w["v0"].astype(np.float64)
+    g, f, _ = p.gradient(v0, "exact")
+    assert np.isfinite(g).all() and f > 0
+    rng = np.random.default_rng(4)
+    cells = [(int(rng.integers(10, nz - 10)), int(rng.integers(10, nx - 10))) for _ in range(20)]
+    gmax = np.abs(g).max()
+    worst = 0.0
+    for iz, ix in cells:
+        h = 1e-3 * v0[iz, ix]
+        vp = v0.copy(); vp[iz, ix] += h
+        vm = v0.copy(); vm[iz, ix] -= h
+        fd = (p.misfit(vp) - p.misfit(vm)) / (2 * h)
+        scale = max(abs(fd), 1e-3 * gmax)
+        assert abs(fd - g[iz, ix]) <= 1e-3 * scale, (iz, ix, fd, g[iz, ix])
+        worst = max(worst, abs(fd - g[iz, ix]) / scale)
+    print("full-size FD check: 20 cells, worst relative difference %.2e" % worst)
+    p.close()
