@@ -1,7 +1,8 @@
 """The FWI gradient check of the benchmark configuration at its full size: the 400x800 model
 (+60-cell sponge), 32 shots in one batch, nt = 3000, checkpoints every 50 steps.  The exact-adjoint
 gradient (f64 plan) is compared with central finite differences of the f64 misfit at 20 interior
-cells drawn with seed 4 at least 10 cells from the edges, h = 1e-3 v; bar: 1e-3 relative."""
+cells drawn with seed 4 at least 10 cells from the edges, h = 1e-3 v; bar: 1e-3 relative.  The
+f32 gradient of the same batched, checkpointed path is then compared with the f64 one (rel-L2 1e-3)."""
 import numpy as np
 import pytest
 
@@ -45,3 +46,10 @@ def test_full_size_gradient_vs_finite_differences():
         worst = max(worst, abs(fd - g[iz, ix]) / scale)
     print("full-size FD check: 20 cells, worst relative difference %.2e" % worst)
     p.close()
+    # the f32 gradient of the benchmark's path (same batching and checkpoints) against the f64 one
+    q = FwiProblem(par, wav, data=obs, batch=ns, checkpoint=50)
+    g32, _, _ = q.gradient(w["v0"], "exact")
+    q.close()
+    rel = float(np.linalg.norm(g32.astype(np.float64) - g) / np.linalg.norm(g))
+    print("full-size f32 gradient vs f64: rel-L2 %.2e" % rel)
+    assert rel <= 1e-3
