@@ -50,3 +50,28 @@ def test_tri_128_image_focuses_on_a_source():
     # the product of the four groups' back-propagated fields peaks where they all focus (observed
     # one cell above a source); allowed: 2 cells laterally, 4 in depth
     assert np.any((d[:, 1] <= 2) & (d[:, 2] <= 2) & (d[:, 0] <= 4)), peak
+
+
+@pytest.mark.timeout(600)
+def test_plain3d_201_is_linear_in_the_source_amplitude():
+    # the headline configuration (201^3 + 30-cell ABC, nt = 1500): doubling the amplitude doubles
+    # every wavelet sample exactly; the traces double to within the f32 scheme's own departure
+    # from exact linearity, which is the reference's too: its operation sequence (the oracle's,
+    # bit-identical to the compiled reference) gives max |t(2A) - 2 t(A)| / max |t(A)| = 2e-6 on a
+    # 41^3 model, from 1-ulp differences that first appear next to the absorbing corners
+    from pywave_amd.workloads import plain3d
+    w = plain3d()
+    runs = []
+    for amp in (1.0, 2.0):
+        src = w["src"].copy()
+        src[5] = amp
+        p = AfdPlan(w["vel"], w["d"], w["nt"], w["dt"], w["nbt"], w["ct"], 0)
+        p.set_sources(src, w["ns"])
+        p.run()
+        runs.append(p.output().astype(np.float64))
+        p.close()
+    t1, t2 = runs
+    assert np.isfinite(t1).all() and np.abs(t1).max() > 0
+    err = np.abs(t2 - 2.0 * t1).max() / np.abs(t1).max()
+    print("plain3d 201^3: max |t(2A) - 2 t(A)| / max |t(A)| = %.2e" % err)
+    assert err <= 1e-5  # observed 4.4e-6
