@@ -28,6 +28,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "pw_buffer.h"
@@ -124,6 +125,17 @@ struct StepArgs {
     int it;
 };
 
+// v where k holds, +0 elsewhere, as a bit mask (a select the compiler keeps branch-free)
+__device__ __forceinline__ float keep_if(float v, bool k)
+{
+    return __builtin_bit_cast(float, __builtin_bit_cast(unsigned, v) & (k ? ~0u : 0u));
+}
+__device__ __forceinline__ double keep_if(double v, bool k)
+{
+    using u64 = unsigned long long;
+    return __builtin_bit_cast(double, __builtin_bit_cast(u64, v) & (k ? ~(u64)0 : (u64)0));
+}
+
 template <typename T>
 __device__ __forceinline__ T sponge(T v, T zl, T zh, T xl, T xh)
 {
@@ -154,7 +166,7 @@ __global__ __launch_bounds__(64) void k_fwd(FGeo<T> g, StepArgs<T> a, TileMap tm
     if (!tile_of(tm, slot, tz, xc)) return;
     const int lane = threadIdx.x, z0 = tz * 64, z = z0 + lane;
     const int x0 = xc * XC, x1 = min(x0 + XC, g.pnx);
-    __shared__ T row[2][72];
+    __shared__ T row[2][128];  // z row (4..67) and halo (0..3, 68..71); lanes 8..63 may write 72..127
     const ShotInfo sh = a.shots[slot];
     const int L = g.ldz;
     const int bias = 4 * L + 4;  // windows start 4 columns + 4 cells before the field
@@ -223,6 +235,84 @@ __global__ __launch_bounds__(64) void k_fwd(FGeo<T> g, StepArgs<T> a, TileMap tm
     const bool snap_rows = a.snap && z0 < g.nb + g.nz && z0 + 64 > g.nb;
     const bool src_rows = sh.ph > 0 && sh.sz0 < z0 + 64 && sh.sz0 + sh.ph > z0;
     const bool fsrc_rows = a.fsrc && z0 < g.nb + g.nz && z0 + 64 > g.nb;
+    // Lean tiles: full-width columns inside [4, pnx-4) with no receiver, source, passive-source or
+    // full-copy rows.  Every fetch is in range (the last column fetches nothing), the Laplacian
+    // applies on every column, the tile's sponge pairs are staged in LDS and the loop is unrolled,
+    // so the per-column work carries no scalar branches.  The snapshot store, the one optional
+    // part, is a buffer store whose out-of-range offset drops it outside the model interior.
+    if (!rec_rows && !src_rows && !fsrc_rows && !a.full && x0 >= 4 && x0 + XC + 4 <= g.pnx) {
+        __shared__ T sxp[2 * XC];
+        if (lane < 2 * XC) sxp[lane] = a.sfx[2 * x0 + lane];
+        auto lean = [&](auto snap_c) {
+            constexpr bool SNAP = decltype(snap_c)::value;
+            const bool szin = z >= g.nb && z < g.nb + g.nz;
+            const auto rSn = win(SNAP ? a.snap + slot * a.snaps : nullptr,
+                                 SNAP ? (unsigned)((long long)g.nz * g.nx * E) : 0u);
+            const unsigned osn = szin ? (unsigned)(z - g.nb) * E : kOOB;
+            auto get = [&](int xx) -> Col {
+                Col c;
+                c.q = bldo<T>(rX, oz, (unsigned)(xx + 4) * oc);
+                c.y = bldo<T>(rY, oz, (unsigned)xx * oc);
+                c.m = bldo<T>(rM, oz, (unsigned)xx * oc);
+                c.h = bldo<T>(rX, ozh, (unsigned)xx * oc);
+                c.f = T(0);
+                c.ta = 0.f; c.ts = 1.f; c.r1 = 0.f;
+                if constexpr (VISCO) {
+                    c.ta = bldo<float>(rTa, oz4, (unsigned)xx * oc4);
+                    c.ts = bldo<float>(rTs, oz4, (unsigned)xx * oc4);
+                    c.r1 = bldo<float>(rR1, oz4, (unsigned)xx * oc4);
+                }
+                c.xl = sxp[2 * (xx - x0)];
+                c.xh = sxp[2 * (xx - x0) + 1];
+                return c;
+            };
+            T q[9];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) q[k] = bldo<T>(rX, oz, (unsigned)(x0 - 4 + k) * oc);
+            __syncthreads();
+            Col pend = get(x0);
+#pragma unroll
+            for (int i = 0; i < XC; ++i) {
+                const int x = x0 + i;
+                const Col cur = pend;
+                if (i + 1 < XC) pend = get(x + 1);
+                q[8] = cur.q;
+                row[i & 1][lane + 4] = q[4];
+                row[i & 1][lzh] = cur.h;
+                __syncthreads();
+                const T p1 = q[4];
+                if constexpr (SNAP) {
+                    const bool sx = x >= g.nb && x < g.nb + g.nx;
+                    bsto(rSn, sx ? osn : kOOB, sx ? (unsigned)((x - g.nb) * g.nz) * E : 0u, p1);
+                }
+                const T* R = &row[i & 1][lane + 4];
+                const T lap = (C::K0 * p1 + C::K1 * (q[5] + q[3]) + C::K2 * (q[6] + q[2]) + C::K3 * (q[7] + q[1]) +
+                               C::K4 * (q[8] + q[0])) / g.dx2 +
+                              (C::K0 * p1 + C::K1 * (R[1] + R[-1]) + C::K2 * (R[2] + R[-2]) +
+                               C::K3 * (R[3] + R[-3]) + C::K4 * (R[4] + R[-4])) / g.dz2;
+                T term = keep_if(lap, zin);
+                const T xl = cur.xl, xh = cur.xh;
+                if constexpr (VISCO) {
+                    const float tf = (float)term;
+                    const float tav = cur.ta, tsv = cur.ts, r1v = cur.r1;
+                    const double num = (double)(tav / tsv * tf) + ((double)a.idt - 0.5 / (double)tsv) * (double)r1v;
+                    const float r2v = (float)(num / ((double)a.idt + 0.5 / (double)tsv));
+                    const float sv = 0.f * (float)wv;
+                    const double tt = (double)tf * (1. + (double)tav) - (double)(r2v + r1v) * 0.5;
+                    term = (T)(float)(tt + (double)sv);
+                    bsto(rR2, oz4, (unsigned)x * oc4, (float)sponge((T)r2v, zl, zh, xl, xh));
+                }
+                const T p0 = sponge(cur.y, zl, zh, xl, xh);
+                T p2 = 2 * p1 - p0 + cur.m * term;
+                bsto(rY, oz, (unsigned)x * oc, sponge(p2, zl, zh, xl, xh));
+#pragma unroll
+                for (int k = 0; k < 8; ++k) q[k] = q[k + 1];
+            }
+        };
+        if (snap_rows) lean(std::true_type{});
+        else lean(std::false_type{});
+        return;
+    }
     T q[9];
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
@@ -343,7 +433,7 @@ __global__ __launch_bounds__(64) void k_adj_ref(FGeo<float> g, StepArgs<float> a
     if (!tile_of(tm, slot, tz, xc)) return;
     const int lane = threadIdx.x, z0 = tz * 64, z = z0 + lane;
     const int x0 = xc * XC, x1 = min(x0 + XC, g.pnx);
-    __shared__ float row[2][72];
+    __shared__ float row[2][128];  // z row, halo, spare slots for lanes 8..63
     const ShotInfo sh = a.shots[slot];
     const int L = g.ldz;
     const int bias = 4 * L + 4;  // windows start 4 columns + 4 cells before the field
@@ -395,6 +485,78 @@ __global__ __launch_bounds__(64) void k_adj_ref(FGeo<float> g, StepArgs<float> a
         return bldo<float>(r, ok ? og : kOOB, ok ? (unsigned)((x - g.nb) * g.nz) * 4u : 0u);
     };
     const float* inj = a.inj + slot * a.trs + (long long)a.it * g.nr;
+    // Lean tiles (acoustic): full-width columns inside [4, pnx-4) and no receiver row, as in
+    // k_fwd: in-range fetches, LDS sponge pairs, an unrolled loop without scalar branches; the
+    // imaging update is a buffer store dropped outside the model interior.
+    if constexpr (!VISCO) {
+        if (!(g.rz >= z0 && g.rz < z0 + 64) && x0 >= 4 && x0 + XC + 4 <= g.pnx) {
+            __shared__ float sxp[2 * XC];
+            if (lane < 2 * XC) sxp[lane] = a.sfx[2 * x0 + lane];
+            auto lean = [&](auto img_c) {
+                constexpr bool IMG = decltype(img_c)::value;
+                struct ACol {
+                    float q, h, y, m, wp, wc, wn, tv, gv, xl, xh;
+                };
+                auto get = [&](int xx) -> ACol {
+                    ACol c;
+                    c.q = bldo<float>(rX, oz, (unsigned)(xx + 4) * oc);
+                    c.h = bldo<float>(rX, ozh, (unsigned)xx * oc);
+                    c.y = bldo<float>(rY, oz, (unsigned)xx * oc);
+                    c.m = bldo<float>(rM, oz, (unsigned)xx * oc);
+                    c.wp = c.wc = c.wn = c.tv = c.gv = 0.f;
+                    if constexpr (IMG) {
+                        const bool ok = gxin(xx);
+                        const unsigned vo = ok ? og : kOOB, so = ok ? (unsigned)((xx - g.nb) * g.nz) * 4u : 0u;
+                        c.wp = bldo<float>(rWp, vo, so);
+                        c.wc = bldo<float>(rWc, vo, so);
+                        c.wn = bldo<float>(rWn, vo, so);
+                        c.tv = bldo<float>(rTv, vo, so);
+                        c.gv = bldo<float>(rG, vo, so);
+                    }
+                    c.xl = sxp[2 * (xx - x0)];
+                    c.xh = sxp[2 * (xx - x0) + 1];
+                    return c;
+                };
+                float q[9];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) q[k] = bldo<float>(rX, oz, (unsigned)(x0 - 4 + k) * oc);
+                __syncthreads();
+                ACol pend = get(x0);
+#pragma unroll
+                for (int i = 0; i < XC; ++i) {
+                    const int x = x0 + i;
+                    const ACol cur = pend;
+                    if (i + 1 < XC) pend = get(x + 1);
+                    q[8] = cur.q;
+                    row[i & 1][lane + 4] = q[4];
+                    row[i & 1][lzh] = cur.h;
+                    __syncthreads();
+                    const float p1 = q[4];
+                    const float* R = &row[i & 1][lane + 4];
+                    const float lap = (C::K0 * p1 + C::K1 * (q[5] + q[3]) + C::K2 * (q[6] + q[2]) +
+                                       C::K3 * (q[7] + q[1]) + C::K4 * (q[8] + q[0])) / g.dx2 +
+                                      (C::K0 * p1 + C::K1 * (R[1] + R[-1]) + C::K2 * (R[2] + R[-2]) +
+                                       C::K3 * (R[3] + R[-3]) + C::K4 * (R[4] + R[-4])) / g.dz2;
+                    const float term = keep_if(lap, zin);
+                    const float p0 = sponge(cur.y, zl, zh, cur.xl, cur.xh);
+                    const float p2 = 2 * p1 - p0 + cur.m * term;
+                    bsto(rY, oz, (unsigned)x * oc, sponge(p2, zl, zh, cur.xl, cur.xh));
+                    if constexpr (IMG) {
+                        const bool ok = gxin(x);
+                        const double e = ((double)cur.wn - 2. * (double)cur.wc + (double)cur.wp) / g.wdt2 *
+                                         (double)p1 * (double)cur.tv;
+                        bsto(rG, ok ? og : kOOB, ok ? (unsigned)((x - g.nb) * g.nz) * 4u : 0u,
+                             (float)((double)cur.gv + e));
+                    }
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) q[k] = q[k + 1];
+                }
+            };
+            if (img) lean(std::true_type{});
+            else lean(std::false_type{});
+            return;
+        }
+    }
     float tq[9], pq[5], rq[5];
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
@@ -528,8 +690,8 @@ __global__ __launch_bounds__(64) void k_adj_exact(FGeo<T> g, StepArgs<T> a, ExAr
     if (!tile_of(tm, slot, tz, xc)) return;
     const int lane = threadIdx.x, z0 = tz * 64, z = z0 + lane;
     const int x0 = xc * XC, x1 = min(x0 + XC, g.pnx);
-    __shared__ T rowa[2][72];
-    __shared__ T rowf[2][72];
+    __shared__ T rowa[2][128];  // z rows, halo, spare slots for lanes 8..63
+    __shared__ T rowf[2][128];
     const ShotInfo sh = a.shots[slot];
     const int L = g.ldz;
     const int bias = 4 * L + 4;
@@ -571,6 +733,72 @@ __global__ __launch_bounds__(64) void k_adj_exact(FGeo<T> g, StepArgs<T> a, ExAr
     const unsigned ofz = fzin ? (unsigned)(z - g.nb) * E : kOOB;
     auto fxin = [&](int x) { return x >= g.nb && x < g.nb + g.nx; };
     auto ldf = [&](int x) { return bldo<T>(rS, fxin(x) ? ofz : kOOB, fxin(x) ? (unsigned)((x - g.nb) * g.nz) * E : 0u); };
+    // Lean tiles: every column of the stencil reach inside [4, pnx-4) (x0 >= 8, x0 + XC + 8 <=
+    // pnx) and no receiver, source or passive-source rows: in-range fetches without column masks,
+    // LDS sponge pairs and an unrolled loop without scalar branches.
+    const bool src_rows = sh.ph > 0 && sh.sz0 < z0 + 64 && sh.sz0 + sh.ph > z0;
+    const bool fsrc_rows = a.fsrc && z0 < g.nb + g.nz && z0 + 64 > g.nb;
+    if (!(g.rz >= z0 && g.rz < z0 + 64) && !src_rows && !fsrc_rows && x0 >= 8 && x0 + XC + 8 <= g.pnx) {
+        __shared__ T sxp[2 * XC];
+        if (lane < 2 * XC) sxp[lane] = a.sfx[2 * x0 + lane];
+        struct XCol {
+            T a5, m5, f5, a, b, g, ha, hf, xl, xh;
+        };
+        auto get = [&](int xx) -> XCol {
+            XCol c;
+            c.a5 = bldo<T>(rA, ozi, (unsigned)(xx + 4) * oc);
+            c.m5 = bldo<T>(rM, ozi, (unsigned)(xx + 4) * oc);
+            c.f5 = bldo<T>(rF, oz, (unsigned)(xx + 4) * oc);
+            c.a = bldo<T>(rA, oz, (unsigned)xx * oc);
+            c.b = bldo<T>(rB, oz, (unsigned)xx * oc);
+            c.g = bldo<T>(rG, oz, (unsigned)xx * oc);
+            c.ha = bldo<T>(rM, ozhi, (unsigned)xx * oc) * bldo<T>(rA, ozhi, (unsigned)xx * oc);
+            c.hf = bldo<T>(rF, ozh, (unsigned)xx * oc);
+            c.xl = sxp[2 * (xx - x0)];
+            c.xh = sxp[2 * (xx - x0) + 1];
+            return c;
+        };
+        T qa[9], qf[9];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const unsigned so = (unsigned)(x0 - 4 + k) * oc;
+            qa[k] = bldo<T>(rM, ozi, so) * bldo<T>(rA, ozi, so);
+            qf[k] = bldo<T>(rF, oz, so);
+        }
+        __syncthreads();
+        XCol pend = get(x0);
+#pragma unroll
+        for (int i = 0; i < XC; ++i) {
+            const int x = x0 + i;
+            const XCol cur = pend;
+            if (i + 1 < XC) pend = get(x + 1);
+            qa[8] = cur.m5 * cur.a5;
+            qf[8] = cur.f5;
+            rowa[i & 1][lane + 4] = qa[4];
+            rowf[i & 1][lane + 4] = qf[4];
+            rowa[i & 1][lzh] = cur.ha;
+            rowf[i & 1][lzh] = cur.hf;
+            __syncthreads();
+            const T* Ra = &rowa[i & 1][lane + 4];
+            const T lt = (C::K0 * qa[4] + C::K1 * (qa[5] + qa[3]) + C::K2 * (qa[6] + qa[2]) + C::K3 * (qa[7] + qa[1]) +
+                          C::K4 * (qa[8] + qa[0])) / g.dx2 +
+                         (C::K0 * Ra[0] + C::K1 * (Ra[1] + Ra[-1]) + C::K2 * (Ra[2] + Ra[-2]) +
+                          C::K3 * (Ra[3] + Ra[-3]) + C::K4 * (Ra[4] + Ra[-4])) / g.dz2;
+            const T src = T(0);
+            const T qn = 2 * cur.a - sponge(cur.b, zl, zh, cur.xl, cur.xh) + lt + src;
+            bsto(rB, oz, (unsigned)x * oc, sponge(qn, zl, zh, cur.xl, cur.xh));
+            const T* Rf = &rowf[i & 1][lane + 4];
+            const T lfv = (C::K0 * qf[4] + C::K1 * (qf[5] + qf[3]) + C::K2 * (qf[6] + qf[2]) +
+                           C::K3 * (qf[7] + qf[1]) + C::K4 * (qf[8] + qf[0])) / g.dx2 +
+                          (C::K0 * Rf[0] + C::K1 * (Rf[1] + Rf[-1]) + C::K2 * (Rf[2] + Rf[-2]) +
+                           C::K3 * (Rf[3] + Rf[-3]) + C::K4 * (Rf[4] + Rf[-4])) / g.dz2;
+            const T lf = keep_if(lfv, zin);
+            bsto(rG, oz, (unsigned)x * oc, cur.g + lf * cur.a);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) { qa[k] = qa[k + 1]; qf[k] = qf[k + 1]; }
+        }
+        return;
+    }
     T qa[9], qf[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
