@@ -99,6 +99,53 @@ __global__ __launch_bounds__(64) void k_afd2d_batch(BatchArgs a)
         blk_src |= cz[0] >= z0 - 1 && cz[0] <= z0 + 64 && cz[1] >= x0 - 1 && cz[1] <= x1;
     }
     const bool rtile = sh.nb >= z0 && sh.nb < z0 + 64;  // the receiver row is in this tile
+    // Lean tiles: no source or receiver rows and every column of the stencil reach inside the
+    // updated domain (x0 >= 2, x0 + XC + 2 <= nx): in-range fetches, the tile's damping pairs in
+    // LDS and an unrolled loop without scalar branches.
+    if (!blk_src && !rtile && x0 >= 2 && x0 + XC + 2 <= sh.nx) {
+        __shared__ float2 sxp[XC];
+        if (lane < XC) sxp[lane] = dx_[x0 + lane];
+        float q[5];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) q[k] = ld(rc, oz, (unsigned)(x0 - 2 + k) * ocol);
+        __syncthreads();
+        float nq = ld(rc, oz, (unsigned)(x0 + 2) * ocol);
+        float pr = ld(rp, oz, (unsigned)x0 * ocol), vc = ld(rv, oz, (unsigned)x0 * ocol);
+        float hz = ld(rc, ozh, (unsigned)x0 * ocol);
+#pragma unroll
+        for (int i = 0; i < XC; ++i) {
+            const int x = x0 + i;
+            q[4] = nq;
+            const float cpr = pr, cvc = vc, chz = hz;
+            if (i + 1 < XC) {
+                const unsigned sn = (unsigned)(x + 1) * ocol;
+                nq = ld(rc, oz, (unsigned)(x + 3) * ocol);
+                pr = ld(rp, oz, sn);
+                vc = ld(rv, oz, sn);
+                hz = ld(rc, ozh, sn);
+            }
+            row[i & 1][lane + 2] = q[2];
+            row[i & 1][lzh] = lane < 4 ? chz : q[2];
+            __syncthreads();
+            const float u = q[2];
+            const float* R = &row[i & 1][lane + 2];
+            float lap = C0 * u * sh.cc
+                      + C1 * (q[1] + q[3]) * sh.idx
+                      + C2 * (q[0] + q[4]) * sh.idx
+                      + C1 * (R[-1] + R[1]) * sh.idz
+                      + C2 * (R[-2] + R[2]) * sh.idz;
+            lap = __builtin_bit_cast(float, __builtin_bit_cast(unsigned, lap) & (zin ? ~0u : 0u));
+            const float2 fx = sxp[i];
+            float p = cpr;
+            p = p * fzz.x; p = p * fzz.y; p = p * fx.x; p = p * fx.y;
+            float c = u + (u - p);
+            c = c + lap * cvc;
+            c = c * fzz.x; c = c * fzz.y; c = c * fx.x; c = c * fx.y;
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, c), rp, oz, (unsigned)x * ocol, 0);
+            q[0] = q[1]; q[1] = q[2]; q[2] = q[3]; q[3] = q[4];
+        }
+        return;
+    }
     float q[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
