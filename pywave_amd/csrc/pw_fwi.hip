@@ -235,16 +235,22 @@ __global__ __launch_bounds__(64) void k_fwd(FGeo<T> g, StepArgs<T> a, TileMap tm
     const bool snap_rows = a.snap && z0 < g.nb + g.nz && z0 + 64 > g.nb;
     const bool src_rows = sh.ph > 0 && sh.sz0 < z0 + 64 && sh.sz0 + sh.ph > z0;
     const bool fsrc_rows = a.fsrc && z0 < g.nb + g.nz && z0 + 64 > g.nb;
-    // Lean tiles: full-width columns inside [4, pnx-4) with no receiver, source, passive-source or
-    // full-copy rows.  Every fetch is in range (the last column fetches nothing), the Laplacian
-    // applies on every column, the tile's sponge pairs are staged in LDS and the loop is unrolled,
-    // so the per-column work carries no scalar branches.  The snapshot store, the one optional
-    // part, is a buffer store whose out-of-range offset drops it outside the model interior.
-    if (!rec_rows && !src_rows && !fsrc_rows && !a.full && x0 >= 4 && x0 + XC + 4 <= g.pnx) {
+    // Lean tiles: full-width columns inside [4, pnx-4), no source-map cell, no passive-source or
+    // full-copy rows, and receivers (if any) at every grid point.  Every fetch is in range (the
+    // last column fetches nothing), the Laplacian applies on every column, the tile's sponge pairs
+    // are staged in LDS and the loop is unrolled, so the per-column work carries no scalar
+    // branches.  The snapshot and receiver stores, the optional parts, are buffer stores whose
+    // out-of-range offset drops them off the model interior and off the receiver line.
+    const bool src_tile = src_rows && sh.sx0 < x0 + XC && sh.sx0 + sh.pw > x0;
+    if (!src_tile && !fsrc_rows && !a.full && (!rec_rows || g.dr_v == 1) && x0 >= 4 && x0 + XC + 4 <= g.pnx) {
         __shared__ T sxp[2 * XC];
         if (lane < 2 * XC) sxp[lane] = a.sfx[2 * x0 + lane];
-        auto lean = [&](auto snap_c) {
-            constexpr bool SNAP = decltype(snap_c)::value;
+        auto lean = [&](auto snap_c, auto rec_c) {
+            constexpr bool SNAP = decltype(snap_c)::value, REC = decltype(rec_c)::value;
+            // receiver k = r02 + (x - r0_v) records lane z == rz (wave_fwigradient.c:136-139)
+            const auto rRc = win(REC ? a.rec + slot * a.trs + (long long)a.it * g.nr : nullptr,
+                                 REC ? (unsigned)(g.nr * E) : 0u);
+            const unsigned orz = z == g.rz ? 0u : kOOB;
             const bool szin = z >= g.nb && z < g.nb + g.nz;
             const auto rSn = win(SNAP ? a.snap + slot * a.snaps : nullptr,
                                  SNAP ? (unsigned)((long long)g.nz * g.nx * E) : 0u);
@@ -285,6 +291,11 @@ __global__ __launch_bounds__(64) void k_fwd(FGeo<T> g, StepArgs<T> a, TileMap tm
                     const bool sx = x >= g.nb && x < g.nb + g.nx;
                     bsto(rSn, sx ? osn : kOOB, sx ? (unsigned)((x - g.nb) * g.nz) * E : 0u, p1);
                 }
+                if constexpr (REC) {
+                    const int d = x - sh.r0_v, k = sh.r02 + d;
+                    const bool rx = d >= 0 && d < sh.nr2 && k >= 0 && k < g.nr;
+                    bsto(rRc, rx ? orz : kOOB, rx ? (unsigned)k * E : 0u, p1);
+                }
                 const T* R = &row[i & 1][lane + 4];
                 const T lap = (C::K0 * p1 + C::K1 * (q[5] + q[3]) + C::K2 * (q[6] + q[2]) + C::K3 * (q[7] + q[1]) +
                                C::K4 * (q[8] + q[0])) / g.dx2 +
@@ -309,8 +320,13 @@ __global__ __launch_bounds__(64) void k_fwd(FGeo<T> g, StepArgs<T> a, TileMap tm
                 for (int k = 0; k < 8; ++k) q[k] = q[k + 1];
             }
         };
-        if (snap_rows) lean(std::true_type{});
-        else lean(std::false_type{});
+        if (rec_rows) {
+            if (snap_rows) lean(std::true_type{}, std::true_type{});
+            else lean(std::false_type{}, std::true_type{});
+        } else {
+            if (snap_rows) lean(std::true_type{}, std::false_type{});
+            else lean(std::false_type{}, std::false_type{});
+        }
         return;
     }
     T q[9];
