@@ -501,15 +501,19 @@ __global__ __launch_bounds__(64) void k_adj_ref(FGeo<float> g, StepArgs<float> a
         return bldo<float>(r, ok ? og : kOOB, ok ? (unsigned)((x - g.nb) * g.nz) * 4u : 0u);
     };
     const float* inj = a.inj + slot * a.trs + (long long)a.it * g.nr;
-    // Lean tiles (acoustic): full-width columns inside [4, pnx-4) and no receiver row, as in
-    // k_fwd: in-range fetches, LDS sponge pairs, an unrolled loop without scalar branches; the
-    // imaging update is a buffer store dropped outside the model interior.
+    // Lean tiles (acoustic): full-width columns inside [4, pnx-4) and receivers (if the tile holds
+    // the receiver row) at every grid point, as in k_fwd: in-range fetches, LDS sponge pairs, an
+    // unrolled loop without scalar branches; the imaging update is a buffer store dropped outside
+    // the model interior, the receiver sample a load answered with 0 off the receiver line.
     if constexpr (!VISCO) {
-        if (!(g.rz >= z0 && g.rz < z0 + 64) && x0 >= 4 && x0 + XC + 4 <= g.pnx) {
+        const bool rec_rows = g.rz >= z0 && g.rz < z0 + 64;
+        if ((!rec_rows || g.dr_v == 1) && x0 >= 4 && x0 + XC + 4 <= g.pnx) {
             __shared__ float sxp[2 * XC];
             if (lane < 2 * XC) sxp[lane] = a.sfx[2 * x0 + lane];
-            auto lean = [&](auto img_c) {
-                constexpr bool IMG = decltype(img_c)::value;
+            auto lean = [&](auto img_c, auto rec_c) {
+                constexpr bool IMG = decltype(img_c)::value, REC = decltype(rec_c)::value;
+                const auto rIj = win(REC ? inj : nullptr, REC ? (unsigned)(g.nr * 4) : 0u);
+                const bool rl = z == g.rz;
                 struct ACol {
                     float q, h, y, m, wp, wc, wn, tv, gv, xl, xh;
                 };
@@ -553,7 +557,13 @@ __global__ __launch_bounds__(64) void k_adj_ref(FGeo<float> g, StepArgs<float> a
                                        C::K3 * (q[7] + q[1]) + C::K4 * (q[8] + q[0])) / g.dx2 +
                                       (C::K0 * p1 + C::K1 * (R[1] + R[-1]) + C::K2 * (R[2] + R[-2]) +
                                        C::K3 * (R[3] + R[-3]) + C::K4 * (R[4] + R[-4])) / g.dz2;
-                    const float term = keep_if(lap, zin);
+                    float term = keep_if(lap, zin);
+                    if constexpr (REC) {  // the windowed residual at the receivers (:208-211)
+                        const int d = x - sh.r0_v, kk = sh.r02 + d;
+                        const bool rx = d >= 0 && d < sh.nr2 && kk >= 0 && kk < g.nr;
+                        const float ij = bldo<float>(rIj, (rx && rl) ? 0u : kOOB, rx ? (unsigned)kk * 4u : 0u);
+                        term = (rx && rl) ? term + ij : term;
+                    }
                     const float p0 = sponge(cur.y, zl, zh, cur.xl, cur.xh);
                     const float p2 = 2 * p1 - p0 + cur.m * term;
                     bsto(rY, oz, (unsigned)x * oc, sponge(p2, zl, zh, cur.xl, cur.xh));
@@ -568,8 +578,13 @@ __global__ __launch_bounds__(64) void k_adj_ref(FGeo<float> g, StepArgs<float> a
                     for (int k = 0; k < 8; ++k) q[k] = q[k + 1];
                 }
             };
-            if (img) lean(std::true_type{});
-            else lean(std::false_type{});
+            if (rec_rows) {
+                if (img) lean(std::true_type{}, std::true_type{});
+                else lean(std::false_type{}, std::true_type{});
+            } else {
+                if (img) lean(std::true_type{}, std::false_type{});
+                else lean(std::false_type{}, std::false_type{});
+            }
             return;
         }
     }
@@ -752,11 +767,14 @@ __global__ __launch_bounds__(64) void k_adj_exact(FGeo<T> g, StepArgs<T> a, ExAr
     // Lean tiles: every column of the stencil reach inside [4, pnx-4) (x0 >= 8, x0 + XC + 8 <=
     // pnx) and no receiver, source or passive-source rows: in-range fetches without column masks,
     // LDS sponge pairs and an unrolled loop without scalar branches.
-    const bool src_rows = sh.ph > 0 && sh.sz0 < z0 + 64 && sh.sz0 + sh.ph > z0;
+    const bool src_tile = sh.ph > 0 && sh.sz0 < z0 + 64 && sh.sz0 + sh.ph > z0 && sh.sx0 < x0 + XC &&
+                          sh.sx0 + sh.pw > x0;
     const bool fsrc_rows = a.fsrc && z0 < g.nb + g.nz && z0 + 64 > g.nb;
-    if (!(g.rz >= z0 && g.rz < z0 + 64) && !src_rows && !fsrc_rows && x0 >= 8 && x0 + XC + 8 <= g.pnx) {
+    const bool rec_rows = g.rz >= z0 && g.rz < z0 + 64;
+    if ((!rec_rows || g.dr_v == 1) && !src_tile && !fsrc_rows && x0 >= 8 && x0 + XC + 8 <= g.pnx) {
         __shared__ T sxp[2 * XC];
         if (lane < 2 * XC) sxp[lane] = a.sfx[2 * x0 + lane];
+        const unsigned orz = rlane ? 0u : kOOB;
         struct XCol {
             T a5, m5, f5, a, b, g, ha, hf, xl, xh;
         };
@@ -800,7 +818,10 @@ __global__ __launch_bounds__(64) void k_adj_exact(FGeo<T> g, StepArgs<T> a, ExAr
                           C::K4 * (qa[8] + qa[0])) / g.dx2 +
                          (C::K0 * Ra[0] + C::K1 * (Ra[1] + Ra[-1]) + C::K2 * (Ra[2] + Ra[-2]) +
                           C::K3 * (Ra[3] + Ra[-3]) + C::K4 * (Ra[4] + Ra[-4])) / g.dz2;
-            const T src = T(0);
+            // R^T g_t: the receiver sample on lane z == rz (0 elsewhere and off the receivers)
+            const int d = x - sh.r0_v, kk = sh.r02 + d;
+            const bool rx = rec_rows && d >= 0 && d < sh.nr2 && kk >= 0 && kk < g.nr;
+            const T src = T(0) + bldo<T>(rI, rx ? orz : kOOB, rx ? (unsigned)kk * E : 0u);
             const T qn = 2 * cur.a - sponge(cur.b, zl, zh, cur.xl, cur.xh) + lt + src;
             bsto(rB, oz, (unsigned)x * oc, sponge(qn, zl, zh, cur.xl, cur.xh));
             const T* Rf = &rowf[i & 1][lane + 4];
