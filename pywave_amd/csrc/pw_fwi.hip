@@ -588,6 +588,121 @@ __global__ __launch_bounds__(64) void k_adj_ref(FGeo<float> g, StepArgs<float> a
             return;
         }
     }
+    // Lean tiles (visco-acoustic): every column of the stencil reach inside [4, pnx-4) (x0 >= 8,
+    // x0 + XC + 8 <= pnx), so tmp and r2 are formed on every lane of the Laplacian rows without
+    // column masks; receivers and imaging as in the acoustic lean loop.
+    if constexpr (VISCO) {
+        const bool rec_rows = g.rz >= z0 && g.rz < z0 + 64;
+        if ((!rec_rows || g.dr_v == 1) && x0 >= 8 && x0 + XC + 8 <= g.pnx) {
+            __shared__ float sxv[2 * XC];
+            if (lane < 2 * XC) sxv[lane] = a.sfx[2 * x0 + lane];
+            auto lean = [&](auto img_c, auto rec_c) {
+                constexpr bool IMG = decltype(img_c)::value, REC = decltype(rec_c)::value;
+                const auto rIj = win(REC ? inj : nullptr, REC ? (unsigned)(g.nr * 4) : 0u);
+                const bool rl = z == g.rz;
+                struct VCol {
+                    float p5, r5, ta5, ts5, hp, hr, hta, hts, y, m, wp, wc, wn, tv, gv, xl, xh;
+                };
+                // operands of column xx (and the queue entry xx + 4)
+                auto get = [&](int xx) -> VCol {
+                    VCol c;
+                    const unsigned s4 = (unsigned)(xx + 4) * oc, s0 = (unsigned)xx * oc;
+                    c.p5 = bldo<float>(rX, oz, s4);
+                    c.r5 = bldo<float>(rR1, ozi, s4);
+                    c.ta5 = bldo<float>(rTa, ozi, s4);
+                    c.ts5 = bldo<float>(rTs, ozi, s4);
+                    c.hp = bldo<float>(rX, ozh, s0);
+                    c.hr = bldo<float>(rR1, ozh, s0);
+                    c.hta = bldo<float>(rTa, ozh, s0);
+                    c.hts = bldo<float>(rTs, ozh, s0);
+                    c.y = bldo<float>(rY, oz, s0);
+                    c.m = bldo<float>(rM, oz, s0);
+                    c.wp = c.wc = c.wn = c.tv = c.gv = 0.f;
+                    if constexpr (IMG) {
+                        const bool ok = gxin(xx);
+                        const unsigned vo = ok ? og : kOOB, so = ok ? (unsigned)((xx - g.nb) * g.nz) * 4u : 0u;
+                        c.wp = bldo<float>(rWp, vo, so);
+                        c.wc = bldo<float>(rWc, vo, so);
+                        c.wn = bldo<float>(rWn, vo, so);
+                        c.tv = bldo<float>(rTv, vo, so);
+                        c.gv = bldo<float>(rG, vo, so);
+                    }
+                    c.xl = sxv[2 * (xx - x0)];
+                    c.xh = sxv[2 * (xx - x0) + 1];
+                    return c;
+                };
+                float tq[9], pq[5], rq[5];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const unsigned so = (unsigned)(x0 - 4 + k) * oc;
+                    const float p1 = bldo<float>(rX, oz, so);
+                    float r2;
+                    tq[k] = form(lin, p1, bldo<float>(rR1, ozi, so), bldo<float>(rTa, ozi, so),
+                                 bldo<float>(rTs, ozi, so), r2);
+                    if (k >= 4) {
+                        pq[k - 4] = p1;
+                        rq[k - 4] = r2;
+                    }
+                }
+                __syncthreads();
+                VCol pend = get(x0);
+#pragma unroll
+                for (int i = 0; i < XC; ++i) {
+                    const int x = x0 + i;
+                    const VCol cur = pend;
+                    if (i + 1 < XC) pend = get(x + 1);
+                    float r2n;
+                    tq[8] = form(lin, cur.p5, cur.r5, cur.ta5, cur.ts5, r2n);
+                    pq[4] = cur.p5;
+                    rq[4] = r2n;
+                    float hr2;
+                    const float ht = form(zhin, cur.hp, cur.hr, cur.hta, cur.hts, hr2);
+                    row[i & 1][lane + 4] = tq[4];
+                    row[i & 1][lzh] = ht;
+                    __syncthreads();
+                    const float p1 = pq[0];
+                    const float* R = &row[i & 1][lane + 4];
+                    const float lap = (C::K0 * tq[4] + C::K1 * (tq[5] + tq[3]) + C::K2 * (tq[6] + tq[2]) +
+                                       C::K3 * (tq[7] + tq[1]) + C::K4 * (tq[8] + tq[0])) / g.dx2 +
+                                      (C::K0 * tq[4] + C::K1 * (R[1] + R[-1]) + C::K2 * (R[2] + R[-2]) +
+                                       C::K3 * (R[3] + R[-3]) + C::K4 * (R[4] + R[-4])) / g.dz2;
+                    float term = keep_if(lap, zin);
+                    if constexpr (REC) {
+                        const int d = x - sh.r0_v, kk = sh.r02 + d;
+                        const bool rx = d >= 0 && d < sh.nr2 && kk >= 0 && kk < g.nr;
+                        const float ij = bldo<float>(rIj, (rx && rl) ? 0u : kOOB, rx ? (unsigned)kk * 4u : 0u);
+                        term = (rx && rl) ? term + ij : term;
+                    }
+                    const float p0 = sponge(cur.y, zl, zh, cur.xl, cur.xh);
+                    const float p2 = 2 * p1 - p0 + cur.m * term;
+                    bsto(rY, oz, (unsigned)x * oc, sponge(p2, zl, zh, cur.xl, cur.xh));
+                    bsto(rR2, oz, (unsigned)x * oc, sponge(rq[0], zl, zh, cur.xl, cur.xh));
+                    if constexpr (IMG) {
+                        const bool ok = gxin(x);
+                        const double e = ((double)cur.wn - 2. * (double)cur.wc + (double)cur.wp) / g.wdt2 *
+                                         (double)p1 * (double)cur.tv;
+                        bsto(rG, ok ? og : kOOB, ok ? (unsigned)((x - g.nb) * g.nz) * 4u : 0u,
+                             (float)((double)cur.gv + e));
+                    }
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) tq[k] = tq[k + 1];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        pq[k] = pq[k + 1];
+                        rq[k] = rq[k + 1];
+                    }
+                }
+            };
+            if (rec_rows) {
+                if (img) lean(std::true_type{}, std::true_type{});
+                else lean(std::false_type{}, std::true_type{});
+            } else {
+                if (img) lean(std::true_type{}, std::false_type{});
+                else lean(std::false_type{}, std::false_type{});
+            }
+            return;
+        }
+    }
     float tq[9], pq[5], rq[5];
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
