@@ -16,7 +16,7 @@
 namespace {
 
 constexpr float C0 = -30. / 12., C1 = +16. / 12., C2 = -1. / 12.;  // pywave/src/wave_fdm.c:15
-constexpr int XC = 16;  // x columns streamed per tile
+constexpr int XC = 4;  // x columns streamed per tile
 
 struct Shot {
     int nz, nx, nz2, nx2, ldz, nb, nz1, nx1;

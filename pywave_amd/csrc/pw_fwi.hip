@@ -38,7 +38,7 @@ namespace {
 
 constexpr double PI_D = 3.14159265358979323846264338328;
 constexpr int PMAX = 7;  // largest source-map patch edge (rect <= 4)
-constexpr int XC = 16;   // x columns streamed per block
+constexpr int XC = 8;   // x columns streamed per block
 
 // Tiles of the step kernels: 64 z cells x XC columns of one shot slot, one wave each.
 struct TileMap {
