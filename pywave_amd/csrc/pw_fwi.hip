@@ -39,6 +39,7 @@ namespace {
 constexpr double PI_D = 3.14159265358979323846264338328;
 constexpr int PMAX = 7;  // largest source-map patch edge (rect <= 4)
 constexpr int XC = 8;   // x columns streamed per block
+constexpr int XCV = 16;  // the visco-acoustic imaging adjoint's tiles (it forms 8 halo columns per tile)
 
 // Tiles of the step kernels: 64 z cells x XC columns of one shot slot, one wave each.
 struct TileMap {
@@ -46,11 +47,11 @@ struct TileMap {
     int nblk;      // tiles over all slots of the launch
 };
 
-TileMap tile_map(int padnz, int padnx, int nslots)
+TileMap tile_map(int padnz, int padnx, int nslots, int width = XC)
 {
     TileMap t;
     t.ntz = (padnz + 63) / 64;
-    t.nxc = (padnx + XC - 1) / XC;
+    t.nxc = (padnx + width - 1) / width;
     t.nblk = t.ntz * t.nxc * nslots;
     return t;
 }
@@ -440,7 +441,7 @@ __device__ __forceinline__ float visco_back(float p1, float r1, float ta, float 
 // memory variable r2 written.  The column entering the 9-column queue (its p1, and for VISCO r1,
 // tau, taus, forming tmp and r2 once), the z halo, Y, m and the imaging operands of the next
 // column are loaded one iteration ahead through buffer resources.
-template <bool VISCO>
+template <bool VISCO, int W>
 __global__ __launch_bounds__(64) void k_adj_ref(FGeo<float> g, StepArgs<float> a, ImgArgs<float> im, ViscoBack vb,
                                                 TileMap tm)
 {
@@ -448,7 +449,7 @@ __global__ __launch_bounds__(64) void k_adj_ref(FGeo<float> g, StepArgs<float> a
     int slot, tz, xc;
     if (!tile_of(tm, slot, tz, xc)) return;
     const int lane = threadIdx.x, z0 = tz * 64, z = z0 + lane;
-    const int x0 = xc * XC, x1 = min(x0 + XC, g.pnx);
+    const int x0 = xc * W, x1 = min(x0 + W, g.pnx);
     __shared__ float row[2][128];  // z row, halo, spare slots for lanes 8..63
     const ShotInfo sh = a.shots[slot];
     const int L = g.ldz;
@@ -507,9 +508,9 @@ __global__ __launch_bounds__(64) void k_adj_ref(FGeo<float> g, StepArgs<float> a
     // the model interior, the receiver sample a load answered with 0 off the receiver line.
     if constexpr (!VISCO) {
         const bool rec_rows = g.rz >= z0 && g.rz < z0 + 64;
-        if ((!rec_rows || g.dr_v == 1) && x0 >= 4 && x0 + XC + 4 <= g.pnx) {
-            __shared__ float sxp[2 * XC];
-            if (lane < 2 * XC) sxp[lane] = a.sfx[2 * x0 + lane];
+        if ((!rec_rows || g.dr_v == 1) && x0 >= 4 && x0 + W + 4 <= g.pnx) {
+            __shared__ float sxp[2 * W];
+            if (lane < 2 * W) sxp[lane] = a.sfx[2 * x0 + lane];
             auto lean = [&](auto img_c, auto rec_c) {
                 constexpr bool IMG = decltype(img_c)::value, REC = decltype(rec_c)::value;
                 const auto rIj = win(REC ? inj : nullptr, REC ? (unsigned)(g.nr * 4) : 0u);
@@ -543,10 +544,10 @@ __global__ __launch_bounds__(64) void k_adj_ref(FGeo<float> g, StepArgs<float> a
                 __syncthreads();
                 ACol pend = get(x0);
 #pragma unroll
-                for (int i = 0; i < XC; ++i) {
+                for (int i = 0; i < W; ++i) {
                     const int x = x0 + i;
                     const ACol cur = pend;
-                    if (i + 1 < XC) pend = get(x + 1);
+                    if (i + 1 < W) pend = get(x + 1);
                     q[8] = cur.q;
                     row[i & 1][lane + 4] = q[4];
                     row[i & 1][lzh] = cur.h;
@@ -589,13 +590,13 @@ __global__ __launch_bounds__(64) void k_adj_ref(FGeo<float> g, StepArgs<float> a
         }
     }
     // Lean tiles (visco-acoustic): every column of the stencil reach inside [4, pnx-4) (x0 >= 8,
-    // x0 + XC + 8 <= pnx), so tmp and r2 are formed on every lane of the Laplacian rows without
+    // x0 + W + 8 <= pnx), so tmp and r2 are formed on every lane of the Laplacian rows without
     // column masks; receivers and imaging as in the acoustic lean loop.
     if constexpr (VISCO) {
         const bool rec_rows = g.rz >= z0 && g.rz < z0 + 64;
-        if ((!rec_rows || g.dr_v == 1) && x0 >= 8 && x0 + XC + 8 <= g.pnx) {
-            __shared__ float sxv[2 * XC];
-            if (lane < 2 * XC) sxv[lane] = a.sfx[2 * x0 + lane];
+        if ((!rec_rows || g.dr_v == 1) && x0 >= 8 && x0 + W + 8 <= g.pnx) {
+            __shared__ float sxv[2 * W];
+            if (lane < 2 * W) sxv[lane] = a.sfx[2 * x0 + lane];
             auto lean = [&](auto img_c, auto rec_c) {
                 constexpr bool IMG = decltype(img_c)::value, REC = decltype(rec_c)::value;
                 const auto rIj = win(REC ? inj : nullptr, REC ? (unsigned)(g.nr * 4) : 0u);
@@ -647,10 +648,10 @@ __global__ __launch_bounds__(64) void k_adj_ref(FGeo<float> g, StepArgs<float> a
                 __syncthreads();
                 VCol pend = get(x0);
 #pragma unroll
-                for (int i = 0; i < XC; ++i) {
+                for (int i = 0; i < W; ++i) {
                     const int x = x0 + i;
                     const VCol cur = pend;
-                    if (i + 1 < XC) pend = get(x + 1);
+                    if (i + 1 < W) pend = get(x + 1);
                     float r2n;
                     tq[8] = form(lin, cur.p5, cur.r5, cur.ta5, cur.ts5, r2n);
                     pq[4] = cur.p5;
@@ -1776,6 +1777,7 @@ int adjoint_segment(pw_fwi* p, int nb_, int sg, int mode, T** Ap, T** Bp, T* gra
     hipStream_t s = p->ctx->s();
     FGeo<T> g = geo<T>(p);
     const TileMap tm = tile_map(a.padnz, a.padnx, nb_);
+    const TileMap tmv = tile_map(a.padnz, a.padnx, nb_, XCV);
     const long long fld = (long long)p->nslot * p->fs;
     const int I = a.interval, K = p->K;
     const int a0 = sg * K, b0 = std::min(a0 + K, a.nt);
@@ -1808,10 +1810,10 @@ int adjoint_segment(pw_fwi* p, int nb_, int sg, int mode, T** Ap, T** Bp, T* gra
             if constexpr (sizeof(T) == 4) {  // the reference mode runs in f32 (checked by the caller)
                 if (p->visco) {
                     ViscoBack vb{p->tau_pad, p->taus_pad, p->RB0, p->RB1, (float)(1. / (double)a.dt)};
-                    hipLaunchKernelGGL(k_adj_ref<true>, tile_grid(tm), dim3(64), 0, s, g, s_, im, vb, tm);
+                    hipLaunchKernelGGL((k_adj_ref<true, XCV>), tile_grid(tmv), dim3(64), 0, s, g, s_, im, vb, tmv);
                     std::swap(p->RB0, p->RB1);
                 } else {
-                    hipLaunchKernelGGL(k_adj_ref<false>, tile_grid(tm), dim3(64), 0, s, g, s_, im, ViscoBack{}, tm);
+                    hipLaunchKernelGGL((k_adj_ref<false, XC>), tile_grid(tm), dim3(64), 0, s, g, s_, im, ViscoBack{}, tm);
                 }
             } else {
                 return PW_ERR_ARG;
