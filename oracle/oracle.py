@@ -9,6 +9,10 @@ What pins each function (tests/test_oracle.py):
                                  zero-phase symmetry, peak at the onset index
   afd2d, afd3d                   golden traces/snapshots from the compiled reference, bit-exact
                                  when fed the reference's wavelet; linearity in amplitude
+  afd_f64, lap_f64               the FP64 twin: golden traces of the compiled reference within f32
+                                 rounding (tests/test_oracle_f64.py); its Laplacian exact on
+                                 quintics and equal to the 4th-order symbol on plane waves; linear
+                                 in amplitude to 1e-13 where the f32 scheme is not
   fwi_prepare geometry           closed-form integer formulas (pywave/src/wave_fwiutil.c:45-60)
   fwi_forward, gradient_av, fwi  golden outputs from the compiled reference, bit-exact
   the same with src= (passive)   compiled reference's gradient_pas_av / pfwi, bit-exact; a field
@@ -71,6 +75,13 @@ def lib():
                                 ctypes.c_int, ctypes.c_float, ctypes.c_int, _i32p, _i32p, _i32p,
                                 _f32p, ctypes.c_int, _f32p, ctypes.c_int, ctypes.c_void_p,
                                 ctypes.c_void_p]
+        _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+        L.orc_afd_f64.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_int,
+                                  ctypes.c_float, ctypes.c_int, _i32p, _i32p, _i32p, _f32p, _f64p, ctypes.c_int,
+                                  ctypes.c_void_p]
+        L.orc_lap_f64.argtypes = [_f64p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                  ctypes.c_double, ctypes.c_double, _f64p]
         L.orc_aps2d.argtypes = L.orc_afd2d.argtypes
         L.orc_aps3d.argtypes = L.orc_afd3d.argtypes
         L.orc_cfft.argtypes = [ctypes.c_int, ctypes.c_int, _f32p, _f32p]
@@ -191,6 +202,57 @@ def afd3d(vel, nt, dt, dx, dy, dz, nbt, ct, sx, sy, sz, rick, tri=False, dat=Non
                     snaps.ctypes.data if snaps is not None else None,
                     img.ctypes.data if img is not None else None)
     return d, snaps, img
+
+
+def afd_f64(vel, nt, dt, d, nbt, ct, src, rick, jsnap=0):
+    """FP64 twin of afd2d/afd3d modelling (oracle/orc_afd64.c).  vel (nz,nx) or (nz,nx,ny) f32;
+    d = (dz, dx) or (dz, dx, dy); src: the reference's source pack ([sx..,sz..,...] in 2D,
+    [sx..,sy..,sz..,...] in 3D, truncated to int as afd2dc/afd3dc do); rick (ns, nt) f32.
+    Returns (traces (nt, nx[, ny]) float64 in the reference's F order, snaps (nz, nx[, ny],
+    ntsnap) float64 or None)."""
+    vel = np.asarray(vel, np.float32)
+    is3d = vel.ndim == 3
+    nz1, nx1 = vel.shape[:2]
+    ny1 = vel.shape[2] if is3d else 1
+    velf = np.ascontiguousarray(vel.ravel(order="F"))
+    rick = np.ascontiguousarray(np.atleast_2d(rick), np.float32)
+    ns = rick.shape[0]
+    src = np.asarray(src, np.float32)
+    spx = np.ascontiguousarray(src[0:ns].astype(np.int32))
+    if is3d:
+        spy = np.ascontiguousarray(src[ns:2 * ns].astype(np.int32))
+        spz = np.ascontiguousarray(src[2 * ns:3 * ns].astype(np.int32))
+        dz, dx, dy = d
+    else:
+        spy = np.zeros(max(ns, 1), np.int32)
+        spz = np.ascontiguousarray(src[ns:2 * ns].astype(np.int32))
+        (dz, dx), dy = d, 1.0
+    dat = np.zeros(ny1 * nx1 * nt, np.float64)
+    nsnap = count_snaps(nt, jsnap)
+    snaps = np.zeros(nsnap * nz1 * nx1 * ny1, np.float64) if nsnap else None
+    rc = lib().orc_afd_f64(velf, nz1, nx1, ny1, int(is3d), int(nt), dt, dx, dy, dz, int(nbt), ct, ns,
+                           spx if ns else np.zeros(1, np.int32), spy, spz if ns else np.zeros(1, np.int32), rick,
+                           dat, int(jsnap), snaps.ctypes.data if snaps is not None else None)
+    if rc:
+        raise MemoryError("orc_afd_f64: allocation failed")
+    shp = (nt, nx1, ny1) if is3d else (nt, nx1)
+    tr = dat.reshape(shp, order="F")
+    if snaps is not None:
+        snaps = snaps.reshape(((nz1, nx1, ny1) if is3d else (nz1, nx1)) + (nsnap,), order="F")
+    return tr, snaps
+
+
+def lap_f64(u, d):
+    """The FP64 twin's 4th-order Laplacian of a padded field u (nz2, nx2[, ny2]) with spacings
+    d = (dz, dx[, dy]); zero on the two-cell rim."""
+    u = np.asarray(u, np.float64)
+    nz2, nx2 = u.shape[:2]
+    ny2 = u.shape[2] if u.ndim == 3 else 1
+    dz, dx = d[0], d[1]
+    dy = d[2] if len(d) > 2 else 1.0
+    out = np.zeros(u.size, np.float64)
+    lib().orc_lap_f64(np.ascontiguousarray(u.ravel(order="F")), nz2, nx2, ny2, dz, dx, dy, out)
+    return out.reshape(u.shape, order="F")
 
 
 # ------------------------------------------------------------------ FWI

@@ -52,6 +52,15 @@ int orc_aps3d(const float *vel, int nz1, int nx1, int ny1, int nt, float dt, flo
               const int *spz, const float *rick, int tri, float *dat, int jsnap, float *snaps,
               float *img);
 
+/* FP64 twin of orc_afd2d / orc_afd3d (modelling only): the same scheme in double precision from
+ * the same f32 inputs (rick: ns*nt f32 wavelets).  is3d = 0 gives the 2D engine (ny1 = 1).
+ * dat: ny1*nx1*nt [iy][ix][it]; snaps: ntsnap*ny1*nx1*nz1 or NULL. */
+int orc_afd_f64(const float *vel, int nz1, int nx1, int ny1, int is3d, int nt, float dt, float dx,
+                float dy, float dz, int nbt, float ct, int ns, const int *spx, const int *spy,
+                const int *spz, const float *rick, double *dat, int jsnap, double *snaps);
+/* its 4th-order Laplacian of a padded [ny2][nx2][nz2] field (ny2 = 1: 2D), zero on the rim */
+void orc_lap_f64(const double *u, int nz2, int nx2, int ny2, double dz, double dx, double dy, double *out);
+
 /* ---- transforms (pywave/src/wave_kissfft.c, wave_fft2.c, wave_fft3.c) ---- */
 
 /* complex transform of n points (interleaved re, im); inverse = 1 for the positive exponent
