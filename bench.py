@@ -36,20 +36,25 @@ def cpu_baseline(w, nt_cpu):
     # onset at step 0 so the truncated run injects a non-trivial source
     src = np.array([c, c, c, 15.0, 0.0, 1.0], np.float32)
     try:
-        from oracle import refrun
-        if refrun.available():
-            _, el = refrun.afd3d(vel, nt_cpu, w["dt"], *w["d"][1:], w["d"][0], nb, w["ct"], src, 1)
-            return dict(value=cells * nt_cpu / el / 1e9, unit="Gcell-updates/s", cores=1, kind="reference",
-                        sample="reference C afd3d (fdm, pywave/src/wave_fdm.c:28) on the same 201^3 model, "
-                               "%d of 1500 steps, %.1f s, single thread, -O2 -fwrapv" % (nt_cpu, el))
         from oracle import oracle as O
+        from oracle import refrun
         rick = O.ricker_bank(nt_cpu, w["dt"], [15.0], [0.0], [1.0])
         t = time.perf_counter()
         O.afd3d(vel, nt_cpu, w["dt"], 10, 10, 10, nb, w["ct"], [c], [c], [c], rick)
-        el = time.perf_counter() - t
-        return dict(value=cells * nt_cpu / el / 1e9, unit="Gcell-updates/s", cores=1, kind="port",
-                    sample="oracle C restatement of afd3d on the same 201^3 model, %d of 1500 steps, %.1f s"
-                           % (nt_cpu, el))
+        el_port = time.perf_counter() - t
+        port = dict(value=cells * nt_cpu / el_port / 1e9, unit="Gcell-updates/s", cores=1, kind="port",
+                    sample="the oracle's bit-exact C restatement of afd3d (oracle/orc_afd.c, z-innermost loops) on "
+                           "the same 201^3 model, %d of 1500 steps, %.1f s, single thread" % (nt_cpu, el_port))
+        if not refrun.available():
+            return port
+        _, el = refrun.afd3d(vel, nt_cpu, w["dt"], *w["d"][1:], w["d"][0], nb, w["ct"], src, 1)
+        out = dict(value=cells * nt_cpu / el / 1e9, unit="Gcell-updates/s", cores=1, kind="reference",
+                   sample="reference C afd3d (fdm, pywave/src/wave_fdm.c:28) compiled from its sources "
+                          "(oracle/Makefile), on the same 201^3 model, %d of 1500 steps, %.1f s, single thread, "
+                          "-O2 -fwrapv" % (nt_cpu, el))
+        port["reference_over_port_time"] = round(el / el_port, 3)
+        out["port"] = port
+        return out
     except Exception as e:  # the baseline is reported, never required
         return dict(value=None, unit="Gcell-updates/s", cores=1, kind="reference", sample="failed: %s" % e)
 
@@ -107,6 +112,16 @@ def fwi_bench(plain_shots):
     out["reference_gradient_s"] = round(tr, 4)
     out["reference_gradient_forward_ms"] = round(rf, 1)
     out["reference_gradient_backward_ms"] = round(rb, 1)
+    # fwi() (pfwi mode 2's loop, pywave/src/wave_fwi.c:24-119): 5 L-BFGS iterations of the reference
+    # gradient with Wolfe line searches, every shot in one batch; misfit of each stored model (f64)
+    lb = FwiProblem(par, wav, data=obs, batch=ns, checkpoint=50)
+    t = time.perf_counter()
+    vinv, k = lb.lbfgs(w["v0"], 5)
+    tl = time.perf_counter() - t
+    mis = [lb.misfit(w["v0"])] + [lb.misfit(vinv[:, :, i]) for i in range(k)]
+    lb.close()
+    out["lbfgs"] = {"iterations": k, "s": round(tl, 3), "s_per_iteration": round(tl / max(k, 1), 3),
+                    "misfit_start_then_per_iterate": [round(m, 6) for m in mis]}
     out["cpu_baseline"] = fwi_cpu_baseline(par, w)
     if plain_shots <= 0:
         return out
@@ -386,6 +401,30 @@ def fwi_forward_bench(nt=300):
             "achieved_GBs": round(16.0 * cells / (ms * 1e-3 / nt) / 1e9, 1)}
 
 
+def fwi_gradient_bench(nt=300):
+    """The FWI workload's 32 shots (400x800 +60 sponge, all shots per launch, checkpoints every 50
+    steps) over its first nt steps: the exact-adjoint gradient and the reference gradient
+    (gradient_av), two of each -- short runs of the adjoint kernels for counter passes."""
+    from pywave_amd import FwiProblem
+    from pywave_amd.workloads import fwi2d
+
+    w = fwi2d(nt=nt)
+    par = w["par"]
+    wav = (w["amp"] * ricker_wavelet(nt, par["dt"], w["f0"], w["t0"])).astype(np.float32)
+    gen = FwiProblem(par, wav, batch=par["ns"])
+    obs = gen.forward(w["vtrue"])
+    gen.close()
+    p = FwiProblem(par, wav, data=obs, batch=par["ns"], checkpoint=50)
+    out = {"workload": "fwi2d gradients, 32 shots, first %d steps, checkpoints every 50 steps" % nt}
+    for mode in ("exact", "reference"):
+        for _ in range(2):
+            p.gradient(w["v0"], mode)
+        f, b = p.timing()
+        out[mode] = {"forward_ms": round(f, 2), "backward_ms": round(b, 2)}
+    p.close()
+    return out
+
+
 def visco_bench(cpu_steps=300):
     """Visco-acoustic forward modelling (forward_modeling, pywave/src/wave_fwimodeling.c:116-220,
     the media=2 modelling of pfwi) on the FWI workload's 400x800, 32-shot, nt=3000 acquisition
@@ -469,7 +508,7 @@ def visco_bench(cpu_steps=300):
     return out
 
 
-def tri_bench():
+def tri_bench(cpu_steps=12):
     """Time-reversal imaging on the 128^3 workload (+30-cell ABC: 188^3 updated, nt=2000): the
     forward modelling of three sources, then the four interleaved receiver groups back-propagated
     together with the image sum_snap prod_g u_g^2 (jsnap=4) accumulated on the GPU.  Bytes per
@@ -495,11 +534,50 @@ def tri_bench():
     cells = (n + 2 * nb) ** 3
     nsnap = (w["nt"] - 1) // w["jsnap"] + 1
     alg = w["nt"] * cells * (12.0 * G + 4.0) + nsnap * 16.0 * n ** 3
-    return {"workload": "tri3d 128^3 (+30 ABC: 188^3), nt=2000, 3 sources, 4 interleaved receiver groups, jsnap=4",
-            "forward_ms": round(fwd_ms, 2), "backprop_image_ms": round(ms, 2),
-            "field_cell_updates_per_s_G": round(G * cells * w["nt"] / (ms * 1e-3) / 1e9, 2),
-            "achieved_GBs": round(alg / (ms * 1e-3) / 1e9, 1),
-            "frac_hbm": round(alg / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
+    out = {"workload": "tri3d 128^3 (+30 ABC: 188^3), nt=2000, 3 sources, 4 interleaved receiver groups, jsnap=4",
+           "forward_ms": round(fwd_ms, 2), "backprop_image_ms": round(ms, 2),
+           "field_cell_updates_per_s_G": round(G * cells * w["nt"] / (ms * 1e-3) / 1e9, 2),
+           "achieved_GBs": round(alg / (ms * 1e-3) / 1e9, 1),
+           "frac_hbm": round(alg / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
+    if cpu_steps > 0:
+        out["cpu_baseline"] = tri_cpu_baseline(w, dat, cpu_steps, ms)
+    return out
+
+
+def tri_cpu_baseline(w, dat, nt_cpu, gpu_ms):
+    """The reference recipe (demos/test_third_tri.py:57-77) on the CPU: one group's time reversal
+    through the reference C afd3d (tri=True, jsnap=4; fdm, pywave/src/wave_fdm.c:28-381) over
+    nt_cpu of the 2000 steps, and the image formula sum_snap prod_g u_g^2 in numpy on that run's
+    snapshots; both extrapolated to 4 groups x 2000 steps (the per-step cost does not depend on
+    the step).  Single thread."""
+    try:
+        from oracle import refrun
+        if not refrun.available():
+            return None
+        vel, n, nb, js = w["vel"], w["vel"].shape[0], w["nbt"], w["jsnap"]
+        G, nt = len(w["groups"]), w["nt"]
+        g0 = np.zeros((nt_cpu, n, n), np.float32)
+        g0[:, w["groups"][0], :] = dat[nt - nt_cpu:, w["groups"][0], :]  # the group's last samples
+        out, el = refrun.afd3d(vel, nt_cpu, w["dt"], *w["d"][1:], w["d"][0], nb, w["ct"], tri=True, dat=g0, jsnap=js)
+        wfd = out[nt_cpu * n * n:].reshape(n, n, n, -1, order="F")
+        t = time.perf_counter()
+        img = np.zeros((n, n, n), np.float64)
+        for k in range(wfd.shape[3]):
+            prod = np.ones((n, n, n), np.float64)
+            for _ in range(G):  # the same group's field stands in for each of the G groups
+                prod *= wfd[:, :, :, k].astype(np.float64) ** 2
+            img += prod
+        el_img = time.perf_counter() - t
+        nsnap_cpu = wfd.shape[3]
+        total = G * el * nt / nt_cpu + el_img * ((nt - 1) // js + 1) / nsnap_cpu
+        return dict(value=round(total, 1), unit="s per image (4 group back-propagations + image)", cores=1,
+                    kind="reference",
+                    sample="reference C afd3d(tri=True, jsnap=4) of one group on the 128^3 model, %d of %d steps in "
+                           "%.2f s, and the image formula on its %d snapshots in %.3f s (numpy), extrapolated to %d "
+                           "groups x %d steps; single thread, -O2 -fwrapv" % (nt_cpu, nt, el, nsnap_cpu, el_img, G, nt),
+                    speedup_vs_gpu_backprop_image=round(total / (gpu_ms * 1e-3), 1))
+    except Exception as e:  # reported, never required
+        return dict(value=None, unit="s per image", cores=1, kind="reference", sample="failed: %s" % e)
 
 
 def batch_bench():
@@ -523,6 +601,47 @@ def batch_bench():
             "achieved_GBs": round(alg / (ms * 1e-3) / 1e9, 1),
             "frac_hbm": round(alg / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
             "cpu_baseline": cpu}
+
+
+def big3d_bench(n=261, nt=200):
+    """The 3D step on a grid whose fields do not fit the 256 MB Infinity Cache: the headline model
+    recipe at 261^3 + 30-cell ABC = 321^3 updated (324^3 padded, 3 fields = 408 MB), a 15 Hz source
+    at the centre, surface receivers, nt steps.  The autotuned kernel, the fastest single-step and
+    each two-step (temporally blocked) variant, by dispatch-stamped HIP events per launch."""
+    from pywave_amd import AfdPlan
+    from pywave_amd import _native as N
+    from pywave_amd.workloads import plain3d_model
+
+    vel = plain3d_model(n)
+    c = n // 2
+    nb = 30
+    plan = AfdPlan(vel, (10.0, 10.0, 10.0), nt, 0.001, nb, 0.01, 0)
+    plan.set_sources(np.array([c, c, c, 15.0, 0.05, 1.0], np.float32), 1)
+    L = N.lib()
+    tuned = plan.autotune(steps=6)
+    cells = (n + 2 * nb) ** 3
+    alg_step = 16.0 * cells + 4.0 * n * n
+    res = {"workload": "afd3d %d^3 model + 30-cell ABC = %d^3 updated (324^3 padded: 3 fields of %.0f MB), nt=%d"
+                       % (n, n + 2 * nb, 324 ** 3 * 4 / 1e6, nt),
+           "autotuned": list(tuned), "variants": []}
+    info = np.zeros(4, np.int32)
+    for v in range(L.pw_afd_variant_count()):
+        N.check(L.pw_afd_variant_info(v, N.ptr(info)), None, "variant_info")
+        spl = int(info[3])
+        if v != tuned[0] and spl == 1:
+            continue  # single-step variants: the tuned one only
+        try:
+            plan.set_variant(v, tuned[1] if v == tuned[0] else 0)
+            plan.run()
+            total_ms, step_ms = plan.run_timed(per_step=True)
+        except (ValueError, N.PywaveError):
+            continue  # not applicable to this grid
+        ach = alg_step / (step_ms * 1e-3) / 1e9
+        res["variants"].append({"variant": v, "time_steps_per_launch": spl, "us_per_time_step": round(step_ms * 1e3, 2),
+                                "run_ms": round(total_ms, 2), "Gcell_per_s": round(cells / (step_ms * 1e-3) / 1e9, 1),
+                                "achieved_GBs": round(ach, 1), "frac_hbm": round(ach / PEAK_HBM_GBS, 4)})
+    plan.close()
+    return res
 
 
 def ricker_wavelet(nt, dt, f0, t0):
@@ -584,10 +703,12 @@ def main():
     ap.add_argument("--no-lstri", action="store_true")
     ap.add_argument("--no-aps", action="store_true")
     ap.add_argument("--no-visco", action="store_true")
+    ap.add_argument("--no-big", action="store_true")
     ap.add_argument("--fwi-plain-shots", type=int, default=4)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (HBM counter calibration)")
-    ap.add_argument("--only", choices=["fwi", "tri", "batch", "small", "lstri", "aps", "visco", "fwifwd"],
+    ap.add_argument("--only", choices=["fwi", "tri", "batch", "small", "lstri", "aps", "visco", "fwifwd", "fwigrad",
+                                       "big3d"],
                     help="run one secondary workload alone and print its JSON object (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -599,11 +720,12 @@ def main():
         calibrate_copy()
         return
     if args.only:
-        leg = {"fwi": lambda: fwi_bench(args.fwi_plain_shots), "tri": tri_bench, "batch": batch_bench,
+        leg = {"fwi": lambda: fwi_bench(args.fwi_plain_shots), "tri": lambda: tri_bench(0 if args.no_cpu else 12),
+               "batch": batch_bench,
                "small": afd2d_small_bench, "lstri": lstri_bench,
                "aps": lambda: aps3d_bench(0 if args.no_cpu else 20),
                "visco": lambda: visco_bench(0 if args.no_cpu else 300),
-               "fwifwd": fwi_forward_bench}[args.only]
+               "fwifwd": fwi_forward_bench, "fwigrad": fwi_gradient_bench, "big3d": big3d_bench}[args.only]
         print(json.dumps({args.only: leg()}))
         return
 
@@ -656,12 +778,13 @@ def main():
 
     cpu = None if args.no_cpu else cpu_baseline(w, args.cpu_steps)
     fwi = None if args.no_fwi else fwi_bench(args.fwi_plain_shots)
-    tri = None if args.no_tri else tri_bench()
+    tri = None if args.no_tri else tri_bench(0 if args.no_cpu else 12)
     batch = None if args.no_batch else batch_bench()
     small = None if args.no_small else afd2d_small_bench()
     lstri = None if args.no_lstri else lstri_bench()
     aps = None if args.no_aps else aps3d_bench()
     visco = None if args.no_visco else visco_bench()
+    big = None if args.no_big else big3d_bench()
     line = {
         "metric": "Gcell-updates/s 3D FP32 FD (% HBM roofline); FWI grad shots/s batched vs plain",
         "value": round(value, 3),
@@ -669,7 +792,8 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(sec * 1e3, 3),
+        "ms_per_step": round(sec * 1e3, 3),  # one step = one whole nt-step modelling run
+        "ms_per_time_step": round(sec * 1e3 / w["nt"], 5),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -702,6 +826,7 @@ def main():
         "lstri2d": lstri,
         "aps3d": aps,
         "visco2d": visco,
+        "plain3d_321": big,
     }
     print(json.dumps(line))
 

@@ -24,6 +24,8 @@ extern "C" {
 #define PW_ERR_HIP -2      /* HIP runtime error */
 #define PW_ERR_NOMEM -3    /* device or host allocation failed */
 #define PW_ERR_STATE -4    /* call order (e.g. run before velocity was set) */
+#define PW_ERR_NONFINITE -5 /* a result (traces, wavefield, image, gradient, misfit) holds Inf or NaN:
+                               the run blew up, usually a time step beyond the CFL limit */
 
 typedef struct pw_ctx pw_ctx; /* one device + one stream; owns no global state */
 typedef struct pw_afd pw_afd; /* one afd modeling / time-reversal problem resident in HBM */
@@ -158,7 +160,10 @@ int pw_fwi_gradient(pw_fwi *p, const float *vel, int mode, float *grad, double *
 int pw_fwi_gradient_f64(pw_fwi *p, const double *vel, int mode, double *grad, double *fcost);
 int pw_fwi_misfit_f64(pw_fwi *p, const double *vel, double *fcost);
 /* device pointers (element type = plan precision) on the context's stream: model (nz*nx),
- * traces / cotangent [ns][nt][nr], gradient (nz*nx) */
+ * traces / cotangent [ns][nt][nr], gradient (nz*nx).  When one batch holds every shot,
+ * pw_fwi_forward_dev keeps its checkpoints, and the next pw_fwi_adjoint_dev reuses them if it is
+ * given the same model pointer holding the same values (a 64-bit fingerprint of the model is
+ * compared, so a model changed in place between the two calls is swept forward again). */
 int pw_fwi_forward_dev(pw_fwi *p, const void *d_vel, void *d_traces);
 int pw_fwi_adjoint_dev(pw_fwi *p, const void *d_vel, const void *d_gtraces, void *d_gvel);
 /* event-timed milliseconds of the last gradient (forward sweeps, backward sweeps) or of the last
@@ -223,10 +228,12 @@ int pw_lstric(pw_ctx *ctx, const float *vel, const float *q, const float *wav, c
               const float *pararray, float *out, long long out_len, long long in_len);
 
 /* ---------------------------------------------------------------- time-reversal imaging of groups
- * Back-propagates ngroups (1..4) receiver groups of one data set together (one launch per step,
- * the model read once for all groups) and accumulates the source-location image
+ * Back-propagates ngroups (1..64) receiver groups of one data set, up to 4 together (one launch per
+ * step, the model read once for those groups), and accumulates the source-location image
  * sum over snapshots (it % jsnap == 0) of prod_g u_g^2 in f64 -- the image the reference's demo
- * forms from per-group afd3d(tri=True) runs (demos/test_third_tri.py:57-77). */
+ * forms from per-group afd3d(tri=True) runs (demos/test_third_tri.py:57-77).  With more than 4
+ * groups the groups run in passes of 4 and the plan keeps each snapshot's running product in HBM
+ * (8 bytes per model cell and snapshot). */
 typedef struct pw_tri pw_tri;
 pw_tri *pw_tri_create(pw_ctx *ctx, int nz, int nx, int ny, float dz, float dx, float dy, int nt,
                       float dt, int nbt, float ct, int ngroups, int jsnap);

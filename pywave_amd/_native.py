@@ -111,6 +111,11 @@ class PywaveError(RuntimeError):
     """A HIP-side failure reported by libpywave_amd."""
 
 
+class NonFiniteError(PywaveError):
+    """A result (traces, wavefield, image, gradient or misfit) holds Inf or NaN: the run blew up,
+    usually because the time step exceeds the scheme's CFL limit (PW_ERR_NONFINITE)."""
+
+
 def _bind_torch_runtime():
     """One HIP runtime per process: torch ships its own libamdhip64 with the same soname as
     ROCm's, and whichever is loaded first serves both.  When torch is installed it is imported
@@ -165,6 +170,8 @@ def check(rc, ctx, what):
         msg = lib().pw_last_error(ctx).decode(errors="replace") if ctx else ""
         if rc == -1:
             raise ValueError("%s: %s" % (what, msg))
+        if rc == -5:
+            raise NonFiniteError("%s: %s" % (what, msg))
         raise PywaveError("%s failed (%d): %s" % (what, rc, msg))
 
 

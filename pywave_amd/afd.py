@@ -18,6 +18,7 @@ import ctypes
 import numpy as np
 
 from . import _native as N
+from .stability import check_cfl
 
 __all__ = ["afd2d", "afd3d", "AfdPlan", "count_snaps"]
 
@@ -48,6 +49,7 @@ def _run2d(cfun, what, vel, nt, dt, ax, az, ns, sx, sz, f, t, A, nbt, ct, jsnap,
     oz, dz, nz = az[0], az[1], int(az[2])
     nt, jsnap, nbt = int(nt), int(jsnap), int(nbt)
     _check_len(vel, nx * nz, what)
+    check_cfl(vel, dt, (dz, dx), "spectral" if cfun.startswith("pw_aps") else "afd", what)
     if tri:
         source = np.asarray(dat).flatten(order="F").astype(np.float32)
         if source.size != nt * nx:
@@ -92,6 +94,7 @@ def _run3d(cfun, what, vel, nt, dt, ax, ay, az, ns, sx, sy, sz, f, t, A, nbt, ct
     oz, dz, nz = az[0], az[1], int(az[2])
     nt, jsnap, nbt = int(nt), int(jsnap), int(nbt)
     _check_len(vel, nx * ny * nz, what)
+    check_cfl(vel, dt, (dz, dx, dy), "spectral" if cfun.startswith("pw_aps") else "afd", what)
     if tri:
         source = np.asarray(dat).flatten(order="F").astype(np.float32)
         if source.size != nt * nx * ny:
@@ -139,6 +142,7 @@ class AfdPlan:
         dy = float(d[2]) if self.ndim == 3 else 1.0
         self.nt, self.jsnap = int(nt), int(jsnap)
         self.nz, self.nx, self.ny = nz, nx, ny
+        check_cfl(vel, dt, (dz, dx, dy)[:self.ndim], "spectral" if spectral else "afd", "AfdPlan")
         self._L = N.lib()
         self._ctx = N.context(device)
         self._p = self._L.pw_afd_create(self._ctx, self.ndim, nz, nx, ny, dz, dx, dy, self.nt, float(dt),

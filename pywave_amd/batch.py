@@ -10,6 +10,7 @@ import ctypes
 import numpy as np
 
 from . import _native as N
+from .stability import check_cfl
 
 __all__ = ["Afd2dBatch", "afd2d_batch"]
 
@@ -30,6 +31,7 @@ class Afd2dBatch:
         if not self._p:
             N.check(-1, self._ctx, "Afd2dBatch")
         for i, s in enumerate(shots):
+            check_cfl(s["vel"], s["dt"], s["d"][:2], "afd", "Afd2dBatch shot %d" % i)
             v = np.ascontiguousarray(np.asarray(s["vel"], np.float32).ravel(order="F"))
             sp = N.f32c(s["src"])
             wv = N.f32c(s["wavelet"]) if s.get("wavelet") is not None else None

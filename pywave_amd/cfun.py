@@ -23,6 +23,7 @@ reads the arrays (the reference reads them unchecked).
 import numpy as np
 
 from . import _native as N
+from .stability import check_cfl
 from .afd import count_snaps
 
 __all__ = ["afd2dc", "afd3dc", "aps2dc", "aps3dc", "forward_modeling_ac", "forward_modelingc", "fwic", "lstric",
@@ -38,6 +39,7 @@ def _fd2(cfun, name, vel, src, tri, nt, nx, nz, ns, verb, jsnap, ifsnaps, abc, n
     nt, nx, nz, ns, jsnap = int(nt), int(nx), int(nz), int(ns), int(jsnap)
     if vel.size != nx * nz:
         raise SystemError("%s: Dimension mismatch, N_input = %d, N_data = %d" % (name, vel.size, nx * nz))
+    check_cfl(vel, dt, (dz, dx), "spectral" if cfun.startswith("pw_aps") else "afd", name)
     need = nt * nx if tri else 5 * ns
     if src.size < need:
         raise ValueError("%s: the second argument holds %d values, %d expected" % (name, src.size, need))
@@ -56,6 +58,7 @@ def _fd3(cfun, name, vel, src, tri, nt, nx, ny, nz, ns, verb, jsnap, ifsnaps, ab
     nt, nx, ny, nz, ns, jsnap = int(nt), int(nx), int(ny), int(nz), int(ns), int(jsnap)
     if vel.size != nx * ny * nz:
         raise SystemError("%s: Dimension mismatch, N_input = %d, N_data = %d" % (name, vel.size, nx * ny * nz))
+    check_cfl(vel, dt, (dz, dx, dy), "spectral" if cfun.startswith("pw_aps") else "afd", name)
     need = nt * nx * ny if tri else 6 * ns
     if src.size < need:
         raise ValueError("%s: the second argument holds %d values, %d expected" % (name, src.size, need))

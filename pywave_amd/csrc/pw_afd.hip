@@ -975,6 +975,7 @@ struct TriStep {
     const float* pool;                       // the 2G fields back to back (pipelined kernel)
     unsigned pool_bytes;
     int off_cur[G], off_pv[G];               // element offsets of cur[f], pv[f] in the pool
+    int img_mul;                             // 0: img += prod; 1: img *= prod (a later pass of groups)
 };
 
 template <int G>
@@ -1087,8 +1088,10 @@ __global__ __launch_bounds__(256) void k_tri3d(Geo g, TriStep<G> a, int ntz, int
                 a.pv[f][poff + xoff[r] + z] = c;
                 prod = f == 0 ? c * c : prod * c * c;
             }
-            if (a.img && inside)
-                a.img[((size_t)(y - g.nb) * g.nx1 + (x - g.nb)) * g.nz1 + (z - g.nb)] += (double)prod;
+            if (a.img && inside) {
+                double& im = a.img[((size_t)(y - g.nb) * g.nx1 + (x - g.nb)) * g.nz1 + (z - g.nb)];
+                im = a.img_mul ? im * (double)prod : im + (double)prod;
+            }
         }
 #pragma unroll
         for (int f = 0; f < G; ++f)
@@ -1261,7 +1264,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_tri3d_pipe(Geo g, TriStep<G> a, 
             prod = f == 0 ? c[f] * c[f] : prod * c[f] * c[f];
         }
         if (a.img) {
-            const double v = im + (double)prod;
+            const double v = a.img_mul ? im * (double)prod : im + (double)prod;
             using u32x2 = unsigned __attribute__((ext_vector_type(2)));
             __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), img_rsrc(y), oim, 0, 0);
         }
@@ -1907,6 +1910,10 @@ int pw_afd_output(pw_afd* p, float* out, long long out_len)
     if (out_len != need) PW_FAIL(ctx, PW_ERR_ARG, "output: length mismatch");
     if (p->last_mode < 0) PW_FAIL(ctx, PW_ERR_STATE, "output: nothing has run");
     hipStream_t s = ctx->s();
+    // result watchdog: a run that blew up (CFL) leaves Inf/NaN in the last wavefield and traces
+    if (int rc = pw::check_finite(ctx, p->final_field, p->field_elems, 4, "afd: final wavefield")) return rc;
+    if (p->last_mode == 0)
+        if (int rc = pw::check_finite(ctx, p->d_rec, ntr, 4, "afd: receiver traces")) return rc;
     if (p->last_mode == 1) {
         std::memcpy(out, p->tri_host.data(), ntr * 4);  // TRI echoes its input data
     } else {
@@ -1931,6 +1938,7 @@ int pw_afd_final_field(pw_afd* p, float* out, long long out_len)
     if ((size_t)out_len != n) PW_FAIL(ctx, PW_ERR_ARG, "final_field: length mismatch");
     if (!p->final_field) PW_FAIL(ctx, PW_ERR_STATE, "final_field: nothing has run");
     hipStream_t s = ctx->s();
+    if (int rc = pw::check_finite(ctx, p->final_field, p->field_elems, 4, "afd: final wavefield")) return rc;
     dim3 grid(pw::ceil_div(g.nz1, 256), g.nx1, g.ny1);
     hipLaunchKernelGGL(k_interior, grid, dim3(256), 0, s, g, p->final_field, p->d_tmp);
     PW_HIP(ctx, hipGetLastError());
@@ -2101,10 +2109,21 @@ int pw_aps3d(pw_ctx* ctx, const float* vel, const float* src, int tri, int nt, i
 }  // extern "C"
 
 // ==================================================================================== TRI plans
+// Receiver groups are back-propagated up to kTriPassGroups at a time.  With more groups the
+// image needs every group's field at every snapshot step, so the passes keep the running product
+// of the groups done so far per snapshot (f64, [snapshot][interior cell]): the first pass stores
+// its f32 group product, later passes multiply theirs in, and the image is the sum over
+// snapshots at the end.
+constexpr int kTriPassGroups = 4;
+constexpr int kTriMaxGroups = 64;
+
 struct pw_tri {
     pw_ctx* ctx = nullptr;
     Geo g{};
     int nt = 0, G = 0, jsnap = 0;
+    int npass = 1;            // passes of at most kTriPassGroups groups
+    int nsnap = 0;
+    double* d_part = nullptr;  // [nsnap][ny1][nx1][nz1] running products (npass > 1)
     float dt = 0.f;
     size_t field_elems = 0;
     float* d_f[8] = {nullptr};  // 2*G fields (inside d_pool when the pipelined kernel runs)
@@ -2133,7 +2152,7 @@ void tri_free(pw_tri* p)
             if (p->d_f[i]) hipFree(p->d_f[i]);
     }
     void* b[] = {p->d_vv, p->d_fz, p->d_fx, p->d_fy, p->d_dat, p->d_tmp, p->d_mask, p->d_img,
-                 p->d_mbits, p->d_fz1, p->d_fx1, p->d_fy1};
+                 p->d_mbits, p->d_fz1, p->d_fx1, p->d_fy1, p->d_part};
     for (void* x : b)
         if (x) hipFree(x);
 }
@@ -2168,15 +2187,29 @@ void tri_launch(pw_tri* p, const TriStep<G>& a, hipStream_t s)
     hipLaunchKernelGGL(k_tri3d<G>, dim3(grid), dim3(256), 0, s, g, a, ntz, ntx, nyc, nblk);
 }
 
+// image = sum over snapshots of the running products (latest snapshot first, the order in which a
+// single pass adds them)
+__global__ void k_tri_sum_snaps(size_t ncell, int nsnap, const double* __restrict__ part, double* __restrict__ img)
+{
+    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += (size_t)gridDim.x * blockDim.x) {
+        double v = 0.0;
+        for (int k = nsnap - 1; k >= 0; --k) v += part[(size_t)k * ncell + c];
+        img[c] = v;
+    }
+}
+
+// one back-propagation of G groups, the pass-th group of passes (groups pass*kTriPassGroups ...)
 template <int G>
-int tri_enqueue(pw_tri* p)
+int tri_enqueue(pw_tri* p, int pass)
 {
     pw_ctx* ctx = p->ctx;
     hipStream_t s = ctx->s();
     const Geo& g = p->g;
     const size_t nrec = (size_t)g.nx1 * g.ny1;
+    const size_t ncell = (size_t)g.nz1 * nrec;
     for (int i = 0; i < 2 * G; ++i) PW_HIP(ctx, hipMemsetAsync(p->d_f[i], 0, p->field_elems * 4, s));
-    PW_HIP(ctx, hipMemsetAsync(p->d_img, 0, (size_t)g.nz1 * nrec * 8, s));
+    if (p->npass == 1) PW_HIP(ctx, hipMemsetAsync(p->d_img, 0, ncell * 8, s));
+    else if (pass == 0) PW_HIP(ctx, hipMemsetAsync(p->d_part, 0, (size_t)p->nsnap * ncell * 8, s));
     float* A[G];
     float* B[G];
     for (int f = 0; f < G; ++f) { A[f] = p->d_f[2 * f]; B[f] = p->d_f[2 * f + 1]; }
@@ -2185,9 +2218,12 @@ int tri_enqueue(pw_tri* p)
         for (int f = 0; f < G; ++f) { a.cur[f] = A[f]; a.pv[f] = B[f]; }
         a.vv = p->d_vv; a.fz = p->d_fz; a.fx = p->d_fx; a.fy = p->d_fy;
         a.dat = p->d_dat + (size_t)it * nrec;
-        a.mask = p->d_mask;
-        a.img = (p->jsnap > 0 && it % p->jsnap == 0) ? p->d_img : nullptr;
-        a.mbits = p->d_mbits;
+        a.mask = p->d_mask + (size_t)pass * kTriPassGroups * nrec;
+        const bool snap = p->jsnap > 0 && it % p->jsnap == 0;
+        if (p->npass == 1) a.img = snap ? p->d_img : nullptr;
+        else a.img = snap ? p->d_part + (size_t)(it / p->jsnap) * ncell : nullptr;
+        a.img_mul = pass > 0;
+        a.mbits = p->d_mbits ? p->d_mbits + (size_t)pass * nrec : nullptr;
         a.fz1 = p->d_fz1; a.fx1 = p->d_fx1; a.fy1 = p->d_fy1;
         a.pool = p->d_pool;
         a.pool_bytes = p->pool_bytes;
@@ -2212,15 +2248,19 @@ pw_tri* pw_tri_create(pw_ctx* ctx, int nz, int nx, int ny, float dz, float dx, f
                       int nbt, float ct, int ngroups, int jsnap)
 {
     if (!ctx) return nullptr;
-    if (nz < 1 || nx < 1 || ny < 1 || nt < 1 || nbt < 0 || jsnap < 1 || ngroups < 1 || ngroups > 4 ||
+    if (nz < 1 || nx < 1 || ny < 1 || nt < 1 || nbt < 0 || jsnap < 1 || ngroups < 1 || ngroups > kTriMaxGroups ||
         !(dz > 0.f) || !(dx > 0.f) || !(dy > 0.f) || !(dt > 0.f)) {
-        ctx->err = "pw_tri_create: invalid arguments (1 <= ngroups <= 4, jsnap >= 1)";
+        ctx->err = "pw_tri_create: invalid arguments (1 <= ngroups <= " + std::to_string(kTriMaxGroups) +
+                   ", jsnap >= 1)";
         return nullptr;
     }
     if (hipSetDevice(ctx->device) != hipSuccess) { ctx->err = "hipSetDevice failed"; return nullptr; }
     pw_tri* p = new pw_tri;
     p->ctx = ctx;
     p->nt = nt; p->dt = dt; p->G = ngroups; p->jsnap = jsnap;
+    p->npass = pw::ceil_div(ngroups, kTriPassGroups);
+    p->nsnap = (nt - 1) / jsnap + 1;
+    const int gp = std::min(ngroups, kTriPassGroups);  // fields held at once: two per group of a pass
     Geo& g = p->g;
     g.nz1 = nz; g.nx1 = nx; g.ny1 = ny; g.nb = nbt;
     g.nz = nz + 2 * nbt; g.nx = nx + 2 * nbt; g.ny = ny + 2 * nbt;
@@ -2254,19 +2294,22 @@ pw_tri* pw_tri_create(pw_ctx* ctx, int nz, int nx, int ny, float dz, float dx, f
     p->pipe_ok = pipe;
     bool ok = true;
     // the pipelined kernel addresses all 2G fields through one buffer resource (< 2 GiB)
-    const double pool_b = ((double)2 * ngroups * p->field_elems + 4.0 * g.ldz + 4) * 4.0;
+    const double pool_b = ((double)2 * gp * p->field_elems + 4.0 * g.ldz + 4) * 4.0;
     pipe = pipe && pool_b < 2147483648.0;
     p->pipe_ok = pipe;
     if (pipe) {
-        ok = hipMalloc(&p->d_pool, (size_t)2 * ngroups * p->field_elems * 4) == hipSuccess;
-        for (int i = 0; i < 2 * ngroups && ok; ++i) p->d_f[i] = p->d_pool + (size_t)i * p->field_elems;
-        p->pool_bytes = (unsigned)((2LL * ngroups * (long long)p->field_elems + 2LL * (2 * g.ldz + 2)) * 4);
+        ok = hipMalloc(&p->d_pool, (size_t)2 * gp * p->field_elems * 4) == hipSuccess;
+        for (int i = 0; i < 2 * gp && ok; ++i) p->d_f[i] = p->d_pool + (size_t)i * p->field_elems;
+        p->pool_bytes = (unsigned)((2LL * gp * (long long)p->field_elems + 2LL * (2 * g.ldz + 2)) * 4);
     } else {
-        for (int i = 0; i < 2 * ngroups && ok; ++i) ok = hipMalloc(&p->d_f[i], p->field_elems * 4) == hipSuccess;
+        for (int i = 0; i < 2 * gp && ok; ++i) ok = hipMalloc(&p->d_f[i], p->field_elems * 4) == hipSuccess;
     }
+    if (ok && p->npass > 1)
+        ok = hipMalloc(&p->d_part, (size_t)p->nsnap * nz * nrec * 8) == hipSuccess;
     if (ok && pipe)
         ok = hipMalloc(&p->d_fz1, fz1.size() * 4) == hipSuccess && hipMalloc(&p->d_fx1, fx1.size() * 4) == hipSuccess &&
-             hipMalloc(&p->d_fy1, fy1.size() * 4) == hipSuccess && hipMalloc(&p->d_mbits, nrec) == hipSuccess &&
+             hipMalloc(&p->d_fy1, fy1.size() * 4) == hipSuccess &&
+             hipMalloc(&p->d_mbits, (size_t)p->npass * nrec) == hipSuccess &&
              hipMemcpy(p->d_fz1, fz1.data(), fz1.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
              hipMemcpy(p->d_fx1, fx1.data(), fx1.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
              hipMemcpy(p->d_fy1, fy1.data(), fy1.size() * 4, hipMemcpyHostToDevice) == hipSuccess;
@@ -2335,11 +2378,13 @@ int pw_tri_set_data(pw_tri* p, const float* dat, const unsigned char* masks)
     hipLaunchKernelGGL(k_transpose, grid, dim3(32, 8), 0, s, p->d_tmp, p->d_dat, (int)nrec, p->nt);
     PW_HIP(ctx, hipGetLastError());
     PW_HIP(ctx, hipMemcpyAsync(p->d_mask, masks, nrec * p->G, hipMemcpyHostToDevice, s));
-    std::vector<unsigned char> bits(nrec, 0);
+    // per pass, bit f of a receiver's byte: it belongs to the pass's f-th group
+    std::vector<unsigned char> bits((size_t)p->npass * nrec, 0);
     for (int f = 0; f < p->G; ++f)
         for (size_t k = 0; k < nrec; ++k)
-            if (masks[(size_t)f * nrec + k]) bits[k] |= (unsigned char)(1u << f);
-    if (p->d_mbits) PW_HIP(ctx, hipMemcpyAsync(p->d_mbits, bits.data(), nrec, hipMemcpyHostToDevice, s));
+            if (masks[(size_t)f * nrec + k])
+                bits[(size_t)(f / kTriPassGroups) * nrec + k] |= (unsigned char)(1u << (f % kTriPassGroups));
+    if (p->d_mbits) PW_HIP(ctx, hipMemcpyAsync(p->d_mbits, bits.data(), bits.size(), hipMemcpyHostToDevice, s));
     PW_HIP(ctx, hipStreamSynchronize(s));
     p->have_dat = true;
     return PW_OK;
@@ -2349,13 +2394,26 @@ int pw_tri_run(pw_tri* p)
 {
     if (!p) return PW_ERR_ARG;
     if (!p->have_vel || !p->have_dat) PW_FAIL(p->ctx, PW_ERR_STATE, "tri run: velocity or data not set");
-    switch (p->G) {
-        case 1: return tri_enqueue<1>(p);
-        case 2: return tri_enqueue<2>(p);
-        case 3: return tri_enqueue<3>(p);
-        case 4: return tri_enqueue<4>(p);
+    for (int pass = 0; pass < p->npass; ++pass) {
+        const int gk = std::min(kTriPassGroups, p->G - pass * kTriPassGroups);
+        int rc = PW_ERR_ARG;
+        switch (gk) {
+            case 1: rc = tri_enqueue<1>(p, pass); break;
+            case 2: rc = tri_enqueue<2>(p, pass); break;
+            case 3: rc = tri_enqueue<3>(p, pass); break;
+            case 4: rc = tri_enqueue<4>(p, pass); break;
+        }
+        if (rc) return rc;
     }
-    return PW_ERR_ARG;
+    if (p->npass > 1) {
+        const Geo& g = p->g;
+        const size_t ncell = (size_t)g.nz1 * g.nx1 * g.ny1;
+        const unsigned blocks = (unsigned)std::min<size_t>(4096, pw::ceil_div<size_t>(ncell, 256));
+        hipLaunchKernelGGL(k_tri_sum_snaps, dim3(blocks), dim3(256), 0, p->ctx->s(), ncell, p->nsnap,
+                           (const double*)p->d_part, p->d_img);
+        PW_HIP(p->ctx, hipGetLastError());
+    }
+    return PW_OK;
 }
 
 int pw_tri_run_timed(pw_tri* p, double* total_ms)
@@ -2387,6 +2445,7 @@ int pw_tri_image(pw_tri* p, double* img, long long len)
     if (len != (long long)g.nz1 * g.nx1 * g.ny1) PW_FAIL(p->ctx, PW_ERR_ARG, "tri image: length mismatch");
     if (!p->ran) PW_FAIL(p->ctx, PW_ERR_STATE, "tri image: nothing has run");
     hipStream_t s = p->ctx->s();
+    if (int rc = pw::check_finite(p->ctx, p->d_img, (size_t)len, 8, "tri: image")) return rc;
     PW_HIP(p->ctx, hipMemcpyAsync(img, p->d_img, len * 8, hipMemcpyDeviceToHost, s));
     PW_HIP(p->ctx, hipStreamSynchronize(s));
     return PW_OK;

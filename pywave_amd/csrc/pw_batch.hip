@@ -498,6 +498,7 @@ int pw_batch2d_traces(pw_batch* p, int i, float* out, long long len)
     if (len != (long long)s.nt * s.nx1) PW_FAIL(p->ctx, PW_ERR_ARG, "batch traces: length mismatch");
     std::vector<float> t((size_t)len);
     hipStream_t st = p->ctx->s();
+    if (int rc = pw::check_finite(p->ctx, p->d_rec + s.rec_off, (size_t)len, 4, "batch2d: receiver traces")) return rc;
     PW_HIP(p->ctx, hipMemcpyAsync(t.data(), p->d_rec + s.rec_off, len * 4, hipMemcpyDeviceToHost, st));
     PW_HIP(p->ctx, hipStreamSynchronize(st));
     for (int it = 0; it < s.nt; ++it)

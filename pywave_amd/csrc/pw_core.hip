@@ -1,4 +1,5 @@
 // pw_core.hip -- context, error reporting and host-side set-up (geometry, damping, wavelet).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -290,6 +291,52 @@ void damping_pairs(int n2, int nb_lo, int nb_hi, const float* w_lo, const float*
 
 }  // namespace pw
 
+namespace {
+
+template <typename U>
+__global__ void k_count_nonfinite(size_t n, const U* __restrict__ a, U expo, unsigned* __restrict__ count)
+{
+    unsigned c = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        c += (a[i] & expo) == expo;  // exponent all ones: Inf or NaN
+    if (c) atomicAdd(count, c);
+}
+
+__global__ void k_zero_u32(unsigned* p) { *p = 0; }
+
+}  // namespace
+
+int pw::count_nonfinite(pw_ctx* ctx, const void* d, size_t n, int elem_bytes, unsigned* count)
+{
+    *count = 0;
+    if (!n) return PW_OK;
+    hipStream_t s = ctx->s();
+    if (!ctx->d_count) PW_HIP(ctx, hipMalloc((void**)&ctx->d_count, sizeof(unsigned)));
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, ctx->d_count);
+    const unsigned blocks = (unsigned)std::min<size_t>(2048, (n + 255) / 256);
+    if (elem_bytes == 8)
+        hipLaunchKernelGGL(k_count_nonfinite<unsigned long long>, dim3(blocks), dim3(256), 0, s, n,
+                           (const unsigned long long*)d, 0x7ff0000000000000ull, ctx->d_count);
+    else
+        hipLaunchKernelGGL(k_count_nonfinite<unsigned>, dim3(blocks), dim3(256), 0, s, n, (const unsigned*)d,
+                           0x7f800000u, ctx->d_count);
+    PW_HIP(ctx, hipGetLastError());
+    PW_HIP(ctx, hipMemcpyAsync(count, ctx->d_count, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    PW_HIP(ctx, hipStreamSynchronize(s));
+    return PW_OK;
+}
+
+int pw::check_finite(pw_ctx* ctx, const void* d, size_t n, int elem_bytes, const char* what)
+{
+    unsigned c = 0;
+    if (int rc = count_nonfinite(ctx, d, n, elem_bytes, &c)) return rc;
+    if (c)
+        PW_FAIL(ctx, PW_ERR_NONFINITE,
+                std::string(what) + ": " + std::to_string(c) + " of " + std::to_string(n) +
+                    " values are Inf or NaN (the run blew up; is the time step within the CFL limit?)");
+    return PW_OK;
+}
+
 extern "C" {
 
 pw_ctx* pw_create(int device)
@@ -314,6 +361,7 @@ void pw_destroy(pw_ctx* ctx)
         hipStreamSynchronize(ctx->stream);
         hipStreamDestroy(ctx->stream);
     }
+    if (ctx->d_count) hipFree(ctx->d_count);
     delete ctx;
 }
 

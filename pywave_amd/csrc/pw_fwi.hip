@@ -112,8 +112,8 @@ struct StepArgs {
     long long trs;      // trace stride per slot (nt*nr)
     T* snap;            // interior snapshot of p1 for this step: [slot][nx][nz] or null
     long long snaps;    // snapshot stride per slot
-    T* full;            // full-field copy of p1 for this step [slot][fs] or null
-    long long fulls;
+    T* full;            // the step's term (masked L p1 + sources) on the full field [slot][fs], or
+    long long fulls;    // null: what the exact adjoint's dJ/dm needs of the forward step
     const T* fsrc;      // passive source field of this step on the interior, [slot][nx][nz], or null
     long long fss;      // its slot stride (nt*nz*nx)
     // visco-acoustic modelling: tau, taus on the padded grid, memory variable r (damped once per
@@ -236,18 +236,21 @@ __global__ __launch_bounds__(64) void k_fwd(FGeo<T> g, StepArgs<T> a, TileMap tm
     const bool snap_rows = a.snap && z0 < g.nb + g.nz && z0 + 64 > g.nb;
     const bool src_rows = sh.ph > 0 && sh.sz0 < z0 + 64 && sh.sz0 + sh.ph > z0;
     const bool fsrc_rows = a.fsrc && z0 < g.nb + g.nz && z0 + 64 > g.nb;
-    // Lean tiles: full-width columns inside [4, pnx-4), no source-map cell, no passive-source or
-    // full-copy rows, and receivers (if any) at every grid point.  Every fetch is in range (the
+    // Lean tiles: full-width columns inside [4, pnx-4), no source-map cell, no passive-source
+    // rows, and receivers (if any) at every grid point.  Every fetch is in range (the
     // last column fetches nothing), the Laplacian applies on every column, the tile's sponge pairs
     // are staged in LDS and the loop is unrolled, so the per-column work carries no scalar
     // branches.  The snapshot and receiver stores, the optional parts, are buffer stores whose
     // out-of-range offset drops them off the model interior and off the receiver line.
     const bool src_tile = src_rows && sh.sx0 < x0 + XC && sh.sx0 + sh.pw > x0;
-    if (!src_tile && !fsrc_rows && !a.full && (!rec_rows || g.dr_v == 1) && x0 >= 4 && x0 + XC + 4 <= g.pnx) {
+    if (!src_tile && !fsrc_rows && !(a.full && snap_rows) && (!rec_rows || g.dr_v == 1) && x0 >= 4 &&
+        x0 + XC + 4 <= g.pnx) {
         __shared__ T sxp[2 * XC];
         if (lane < 2 * XC) sxp[lane] = a.sfx[2 * x0 + lane];
-        auto lean = [&](auto snap_c, auto rec_c) {
+        auto lean = [&](auto snap_c, auto rec_c, auto term_c) {
             constexpr bool SNAP = decltype(snap_c)::value, REC = decltype(rec_c)::value;
+            constexpr bool TERM = !VISCO && decltype(term_c)::value;
+            const auto rTo = win(TERM ? a.full + slot * a.fulls - bias : nullptr, TERM ? wb : 0u);
             // receiver k = r02 + (x - r0_v) records lane z == rz (wave_fwigradient.c:136-139)
             const auto rRc = win(REC ? a.rec + slot * a.trs + (long long)a.it * g.nr : nullptr,
                                  REC ? (unsigned)(g.nr * E) : 0u);
@@ -303,6 +306,7 @@ __global__ __launch_bounds__(64) void k_fwd(FGeo<T> g, StepArgs<T> a, TileMap tm
                               (C::K0 * p1 + C::K1 * (R[1] + R[-1]) + C::K2 * (R[2] + R[-2]) +
                                C::K3 * (R[3] + R[-3]) + C::K4 * (R[4] + R[-4])) / g.dz2;
                 T term = keep_if(lap, zin);
+                if constexpr (TERM) bsto(rTo, oz, (unsigned)x * oc, term);
                 const T xl = cur.xl, xh = cur.xh;
                 if constexpr (VISCO) {
                     const float tf = (float)term;
@@ -321,12 +325,17 @@ __global__ __launch_bounds__(64) void k_fwd(FGeo<T> g, StepArgs<T> a, TileMap tm
                 for (int k = 0; k < 8; ++k) q[k] = q[k + 1];
             }
         };
-        if (rec_rows) {
-            if (snap_rows) lean(std::true_type{}, std::true_type{});
-            else lean(std::false_type{}, std::true_type{});
+        using Y_ = std::true_type;
+        using N_ = std::false_type;
+        if (a.full) {  // terms for the exact adjoint (never with snapshots: the general loop has both)
+            if (rec_rows) lean(N_{}, Y_{}, Y_{});
+            else lean(N_{}, N_{}, Y_{});
+        } else if (rec_rows) {
+            if (snap_rows) lean(Y_{}, Y_{}, N_{});
+            else lean(N_{}, Y_{}, N_{});
         } else {
-            if (snap_rows) lean(std::true_type{}, std::false_type{});
-            else lean(std::false_type{}, std::false_type{});
+            if (snap_rows) lean(Y_{}, N_{}, N_{});
+            else lean(N_{}, N_{}, N_{});
         }
         return;
     }
@@ -362,7 +371,6 @@ __global__ __launch_bounds__(64) void k_fwd(FGeo<T> g, StepArgs<T> a, TileMap tm
         }
         if (snap_rows && x >= g.nb && x < g.nb + g.nx && z >= g.nb && z < g.nb + g.nz)
             a.snap[slot * a.snaps + (long long)(x - g.nb) * g.nz + (z - g.nb)] = p1;
-        if (a.full && zok) a.full[slot * a.fulls + (long long)x * g.ldz + z] = p1;
         // the Laplacian on [4, pad-4); formed on every lane (its operands are in range) and
         // selected
         const T* R = &row[buf][lane + 4];
@@ -393,6 +401,8 @@ __global__ __launch_bounds__(64) void k_fwd(FGeo<T> g, StepArgs<T> a, TileMap tm
             if (inp) term = term + a.patch[slot * PMAX * PMAX + px * PMAX + pz] * wv;
         }
         if (fsrc_rows && fzin && fxin(x)) term = term + cur.f;
+        if constexpr (!VISCO)
+            if (a.full && zok) a.full[slot * a.fulls + (long long)x * g.ldz + z] = term;
         const T p0 = sponge(yv, zl, zh, xl, xh);
         T p2 = 2 * p1 - p0 + mv * term;
         bsto(rY, oz, (unsigned)x * oc, sponge(p2, zl, zh, xl, xh));
@@ -821,24 +831,27 @@ __global__ __launch_bounds__(64) void k_adj_ref(FGeo<float> g, StepArgs<float> a
 // q_t = S(2 q_{t+1} - S q_{t+2} + L^T(m q_{t+1}) + R^T g_t);  dJ/dm += (L X_t + r w_t) q_{t+1}
 template <typename T>
 struct ExArgs {
-    const T* F;      // forward p1 at this step, full field [slot][fs]
+    const T* F;      // the forward step's term (masked L X_t + sources) on the full field [slot][fs]
     long long fsl;   // slot stride of F
     T* gm;           // dJ/dm accumulator [slot][fs]
 };
 
 // Exact adjoint step (pipelined like k_fwd): q_t = S(2 q_{t+1} - S q_{t+2} + L^T(m q_{t+1}) + R^T g_t)
-// with dJ/dm += (L X_t + source) q_{t+1}.  A = q_{t+1}, B = q_{t+2} (overwritten by q_t), F = the
-// forward field X_t of this step; the m*A product is zero outside the Laplacian's rows [4, n-4).
+// with dJ/dm += term_t q_{t+1}, term_t = L X_t + source (what the forward step multiplied by m,
+// stored by the forward sweep or its recomputation).  A = q_{t+1}, B = q_{t+2} (overwritten by
+// q_t); the m*A product is zero outside the Laplacian's rows [4, n-4).  The adjoint has no
+// counterpart in the reference whose rounding it must follow, so it contracts to FMAs and divides
+// by the squared spacings through reciprocals.
 template <typename T>
 __global__ __launch_bounds__(64) void k_adj_exact(FGeo<T> g, StepArgs<T> a, ExArgs<T> ex, TileMap tm)
 {
+#pragma clang fp contract(fast)
     using C = Coef<T>;
     int slot, tz, xc;
     if (!tile_of(tm, slot, tz, xc)) return;
     const int lane = threadIdx.x, z0 = tz * 64, z = z0 + lane;
     const int x0 = xc * XC, x1 = min(x0 + XC, g.pnx);
     __shared__ T rowa[2][128];  // z rows, halo, spare slots for lanes 8..63
-    __shared__ T rowf[2][128];
     const ShotInfo sh = a.shots[slot];
     const int L = g.ldz;
     const int bias = 4 * L + 4;
@@ -846,6 +859,7 @@ __global__ __launch_bounds__(64) void k_adj_exact(FGeo<T> g, StepArgs<T> a, ExAr
     const auto rA = win(a.X + slot * g.fs - bias, wb), rB = win(a.Y + slot * g.fs - bias, wb),
                rM = win(a.m - bias, wb), rF = win(ex.F + slot * ex.fsl - bias, wb),
                rG = win(ex.gm + slot * g.fs - bias, wb);
+    const T rdx = T(1) / g.dx2, rdz = T(1) / g.dz2;
     const bool zok = z < g.pnz;
     const bool zin = z >= 4 && z < g.pnz - 4;
     const T zl = zok ? a.sfz[2 * z] : T(1), zh = zok ? a.sfz[2 * z + 1] : T(1);
@@ -863,6 +877,12 @@ __global__ __launch_bounds__(64) void k_adj_exact(FGeo<T> g, StepArgs<T> a, ExAr
     auto ldc = [&](__amdgpu_buffer_rsrc_t r, unsigned voff, int x, bool ok) {  // column x of a field
         return bldo<T>(r, ok ? voff : kOOB, ok ? (unsigned)x * oc : 0u);
     };
+    auto lap = [&](const T* q, const T* R) {
+        return (C::K0 * q[4] + C::K1 * (q[5] + q[3]) + C::K2 * (q[6] + q[2]) + C::K3 * (q[7] + q[1]) +
+                C::K4 * (q[8] + q[0])) * rdx +
+               (C::K0 * R[0] + C::K1 * (R[1] + R[-1]) + C::K2 * (R[2] + R[-2]) + C::K3 * (R[3] + R[-3]) +
+                C::K4 * (R[4] + R[-4])) * rdz;
+    };
     // receiver injection: lane z == rz of column x takes trace sample k(x) (regular spread)
     const T* inj = a.inj + slot * a.trs + (long long)a.it * g.nr;
     const auto rI = win(inj, (unsigned)(g.nr * E));
@@ -874,46 +894,35 @@ __global__ __launch_bounds__(64) void k_adj_exact(FGeo<T> g, StepArgs<T> a, ExAr
         const int k = sh.r02 + d / g.dr_v;
         return (k >= 0 && k < g.nr) ? k : -1;
     };
-    const T wv = a.wav[a.it];
-    const bool fzin = a.fsrc && z >= g.nb && z < g.nb + g.nz;
-    const auto rS = win(a.fsrc ? a.fsrc + slot * a.fss : nullptr, a.fsrc ? (unsigned)(g.nz * g.nx * sizeof(T)) : 0u);
-    const unsigned ofz = fzin ? (unsigned)(z - g.nb) * E : kOOB;
-    auto fxin = [&](int x) { return x >= g.nb && x < g.nb + g.nx; };
-    auto ldf = [&](int x) { return bldo<T>(rS, fxin(x) ? ofz : kOOB, fxin(x) ? (unsigned)((x - g.nb) * g.nz) * E : 0u); };
-    // Lean tiles: every column of the stencil reach inside [4, pnx-4) (x0 >= 8, x0 + XC + 8 <=
-    // pnx) and no receiver, source or passive-source rows: in-range fetches without column masks,
-    // LDS sponge pairs and an unrolled loop without scalar branches.
-    const bool src_tile = sh.ph > 0 && sh.sz0 < z0 + 64 && sh.sz0 + sh.ph > z0 && sh.sx0 < x0 + XC &&
-                          sh.sx0 + sh.pw > x0;
-    const bool fsrc_rows = a.fsrc && z0 < g.nb + g.nz && z0 + 64 > g.nb;
     const bool rec_rows = g.rz >= z0 && g.rz < z0 + 64;
-    if ((!rec_rows || g.dr_v == 1) && !src_tile && !fsrc_rows && x0 >= 8 && x0 + XC + 8 <= g.pnx) {
+    // Lean tiles: every column of the stencil reach inside [4, pnx-4) (x0 >= 8, x0 + XC + 8 <=
+    // pnx) and receivers (if the tile holds their row) at every grid point: in-range fetches
+    // without column masks, LDS sponge pairs and an unrolled loop without scalar branches.
+    if ((!rec_rows || g.dr_v == 1) && x0 >= 8 && x0 + XC + 8 <= g.pnx) {
         __shared__ T sxp[2 * XC];
         if (lane < 2 * XC) sxp[lane] = a.sfx[2 * x0 + lane];
         const unsigned orz = rlane ? 0u : kOOB;
         struct XCol {
-            T a5, m5, f5, a, b, g, ha, hf, xl, xh;
+            T a5, m5, a, b, g, t, ha, xl, xh;
         };
         auto get = [&](int xx) -> XCol {
             XCol c;
             c.a5 = bldo<T>(rA, ozi, (unsigned)(xx + 4) * oc);
             c.m5 = bldo<T>(rM, ozi, (unsigned)(xx + 4) * oc);
-            c.f5 = bldo<T>(rF, oz, (unsigned)(xx + 4) * oc);
             c.a = bldo<T>(rA, oz, (unsigned)xx * oc);
             c.b = bldo<T>(rB, oz, (unsigned)xx * oc);
             c.g = bldo<T>(rG, oz, (unsigned)xx * oc);
+            c.t = bldo<T>(rF, oz, (unsigned)xx * oc);
             c.ha = bldo<T>(rM, ozhi, (unsigned)xx * oc) * bldo<T>(rA, ozhi, (unsigned)xx * oc);
-            c.hf = bldo<T>(rF, ozh, (unsigned)xx * oc);
             c.xl = sxp[2 * (xx - x0)];
             c.xh = sxp[2 * (xx - x0) + 1];
             return c;
         };
-        T qa[9], qf[9];
+        T qa[9];
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             const unsigned so = (unsigned)(x0 - 4 + k) * oc;
             qa[k] = bldo<T>(rM, ozi, so) * bldo<T>(rA, ozi, so);
-            qf[k] = bldo<T>(rF, oz, so);
         }
         __syncthreads();
         XCol pend = get(x0);
@@ -923,55 +932,39 @@ __global__ __launch_bounds__(64) void k_adj_exact(FGeo<T> g, StepArgs<T> a, ExAr
             const XCol cur = pend;
             if (i + 1 < XC) pend = get(x + 1);
             qa[8] = cur.m5 * cur.a5;
-            qf[8] = cur.f5;
             rowa[i & 1][lane + 4] = qa[4];
-            rowf[i & 1][lane + 4] = qf[4];
             rowa[i & 1][lzh] = cur.ha;
-            rowf[i & 1][lzh] = cur.hf;
             __syncthreads();
-            const T* Ra = &rowa[i & 1][lane + 4];
-            const T lt = (C::K0 * qa[4] + C::K1 * (qa[5] + qa[3]) + C::K2 * (qa[6] + qa[2]) + C::K3 * (qa[7] + qa[1]) +
-                          C::K4 * (qa[8] + qa[0])) / g.dx2 +
-                         (C::K0 * Ra[0] + C::K1 * (Ra[1] + Ra[-1]) + C::K2 * (Ra[2] + Ra[-2]) +
-                          C::K3 * (Ra[3] + Ra[-3]) + C::K4 * (Ra[4] + Ra[-4])) / g.dz2;
+            const T lt = lap(qa, &rowa[i & 1][lane + 4]);
             // R^T g_t: the receiver sample on lane z == rz (0 elsewhere and off the receivers)
             const int d = x - sh.r0_v, kk = sh.r02 + d;
             const bool rx = rec_rows && d >= 0 && d < sh.nr2 && kk >= 0 && kk < g.nr;
-            const T src = T(0) + bldo<T>(rI, rx ? orz : kOOB, rx ? (unsigned)kk * E : 0u);
+            const T src = bldo<T>(rI, rx ? orz : kOOB, rx ? (unsigned)kk * E : 0u);
             const T qn = 2 * cur.a - sponge(cur.b, zl, zh, cur.xl, cur.xh) + lt + src;
             bsto(rB, oz, (unsigned)x * oc, sponge(qn, zl, zh, cur.xl, cur.xh));
-            const T* Rf = &rowf[i & 1][lane + 4];
-            const T lfv = (C::K0 * qf[4] + C::K1 * (qf[5] + qf[3]) + C::K2 * (qf[6] + qf[2]) +
-                           C::K3 * (qf[7] + qf[1]) + C::K4 * (qf[8] + qf[0])) / g.dx2 +
-                          (C::K0 * Rf[0] + C::K1 * (Rf[1] + Rf[-1]) + C::K2 * (Rf[2] + Rf[-2]) +
-                           C::K3 * (Rf[3] + Rf[-3]) + C::K4 * (Rf[4] + Rf[-4])) / g.dz2;
-            const T lf = keep_if(lfv, zin);
-            bsto(rG, oz, (unsigned)x * oc, cur.g + lf * cur.a);
+            bsto(rG, oz, (unsigned)x * oc, cur.g + cur.t * cur.a);
 #pragma unroll
-            for (int k = 0; k < 8; ++k) { qa[k] = qa[k + 1]; qf[k] = qf[k + 1]; }
+            for (int k = 0; k < 8; ++k) qa[k] = qa[k + 1];
         }
         return;
     }
-    T qa[9], qf[9];
+    T qa[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
         const int xx = x0 - 4 + k;
         const bool okp = xv(xx) && xin(xx);
         qa[k] = ldc(rM, ozi, xx, okp) * ldc(rA, ozi, xx, okp);
-        qf[k] = ldc(rF, oz, xx, xv(xx));
     }
     T av = ldc(rA, oz, x0, true), bv = ldc(rB, oz, x0, true);
-    T gv = ldc(rG, oz, x0, true);
-    T ha, hf;
+    T gv = ldc(rG, oz, x0, true), tv = ldc(rF, oz, x0, true);
+    T ha;
     {
         const bool okp = xin(x0);
         ha = ldc(rM, ozhi, x0, okp) * ldc(rA, ozhi, x0, okp);
-        hf = ldc(rF, ozh, x0, true);
     }
     T xl = a.sfx[2 * x0], xh = a.sfx[2 * x0 + 1];
     int k0 = rk(x0);
     T iv = bld<T>(rI, (unsigned)(k0 < 0 ? 0 : k0) * E, k0 >= 0);
-    T fv = ldf(x0);
     int buf = 0;
     for (int x = x0; x < x1; ++x) {
         const int xn = x + 1, x5 = x + 5;
@@ -979,32 +972,22 @@ __global__ __launch_bounds__(64) void k_adj_exact(FGeo<T> g, StepArgs<T> a, ExAr
         const bool ok5 = nv && x5 < g.pnx;
         const bool okp5 = ok5 && xin(x5);
         const T na5 = ldc(rA, ozi, x5, okp5), nm5 = ldc(rM, ozi, x5, okp5);
-        const T nf5 = ldc(rF, oz, x5, ok5);
         const T nav = ldc(rA, oz, xn, nv);
         const T nbv = ldc(rB, oz, xn, nv);
         const T ngv = ldc(rG, oz, xn, nv);
+        const T ntv = ldc(rF, oz, xn, nv);
         const bool okh = nv && xin(xn);
         const T nha = ldc(rM, ozhi, xn, okh) * ldc(rA, ozhi, xn, okh);
-        const T nhf = ldc(rF, ozh, xn, nv);
         const int xs = xn < g.pnx ? xn : g.pnx - 1;
         const T nxl = a.sfx[2 * xs], nxh = a.sfx[2 * xs + 1];
         const int kn = nv ? rk(xn) : -1;
         const T niv = bld<T>(rI, (unsigned)(kn < 0 ? 0 : kn) * E, kn >= 0);
-        const T nfv = ldf(xn);
         rowa[buf][lane + 4] = qa[4];
-        rowf[buf][lane + 4] = qf[4];
-        if (lane < 8) {
-            rowa[buf][lzh] = ha;
-            rowf[buf][lzh] = hf;
-        }
+        if (lane < 8) rowa[buf][lzh] = ha;
         __syncthreads();
         if (zok) {
             // L^T (m q): the symmetric stencil over the masked product, at every cell
-            const T* Ra = &rowa[buf][lane + 4];
-            T lt = (C::K0 * qa[4] + C::K1 * (qa[5] + qa[3]) + C::K2 * (qa[6] + qa[2]) + C::K3 * (qa[7] + qa[1]) +
-                    C::K4 * (qa[8] + qa[0])) / g.dx2 +
-                   (C::K0 * Ra[0] + C::K1 * (Ra[1] + Ra[-1]) + C::K2 * (Ra[2] + Ra[-2]) + C::K3 * (Ra[3] + Ra[-3]) +
-                    C::K4 * (Ra[4] + Ra[-4])) / g.dz2;
+            const T lt = lap(qa, &rowa[buf][lane + 4]);
             T src = T(0);
             if (rlane) {
                 if (g.dr_v > 0) {
@@ -1016,26 +999,12 @@ __global__ __launch_bounds__(64) void k_adj_exact(FGeo<T> g, StepArgs<T> a, ExAr
             }
             T qn = 2 * av - sponge(bv, zl, zh, xl, xh) + lt + src;
             bsto(rB, oz, (unsigned)x * oc, sponge(qn, zl, zh, xl, xh));
-            // dJ/dm accumulation with the forward field's Laplacian (rim rows are zero)
-            T lf = T(0);
-            if (zin && xin(x)) {
-                const T* Rf = &rowf[buf][lane + 4];
-                lf = (C::K0 * qf[4] + C::K1 * (qf[5] + qf[3]) + C::K2 * (qf[6] + qf[2]) + C::K3 * (qf[7] + qf[1]) +
-                      C::K4 * (qf[8] + qf[0])) / g.dx2 +
-                     (C::K0 * Rf[0] + C::K1 * (Rf[1] + Rf[-1]) + C::K2 * (Rf[2] + Rf[-2]) + C::K3 * (Rf[3] + Rf[-3]) +
-                      C::K4 * (Rf[4] + Rf[-4])) / g.dz2;
-            }
-            const int px = x - sh.sx0, pz = z - sh.sz0;
-            if (px >= 0 && px < sh.pw && pz >= 0 && pz < sh.ph)
-                lf = lf + a.patch[slot * PMAX * PMAX + px * PMAX + pz] * wv;
-            if (fzin && fxin(x)) lf = lf + fv;
-            bsto(rG, oz, (unsigned)x * oc, gv + lf * av);
+            bsto(rG, oz, (unsigned)x * oc, gv + tv * av);
         }
 #pragma unroll
-        for (int k = 0; k < 8; ++k) { qa[k] = qa[k + 1]; qf[k] = qf[k + 1]; }
+        for (int k = 0; k < 8; ++k) qa[k] = qa[k + 1];
         qa[8] = nm5 * na5;
-        qf[8] = nf5;
-        av = nav; bv = nbv; gv = ngv; ha = nha; hf = nhf; xl = nxl; xh = nxh; iv = niv; fv = nfv;
+        av = nav; bv = nbv; gv = ngv; tv = ntv; ha = nha; xl = nxl; xh = nxh; iv = niv;
         buf ^= 1;
     }
 }
@@ -1147,6 +1116,34 @@ __global__ void k_sum_partials(int n, const double* __restrict__ partial, double
     double s = 0.0;
     for (int i = 0; i < n; ++i) s = is_max ? fmax(s, partial[i]) : s + partial[i];
     *out = s;
+}
+
+// An order-independent 64-bit fingerprint of n values (one block of 1024 threads): the sum over i
+// of the splitmix64 finaliser of (bits_i xor i * golden-ratio constant).  Any change to a value
+// changes it except with probability about 2^-64.
+template <typename T>
+__global__ __launch_bounds__(1024) void k_fingerprint(long long n, const T* __restrict__ v, unsigned long long* out)
+{
+    using u64 = unsigned long long;
+    __shared__ u64 red[1024];
+    u64 h = 0;
+    for (long long i = threadIdx.x; i < n; i += blockDim.x) {
+        u64 b;
+        if constexpr (sizeof(T) == 8) b = __builtin_bit_cast(u64, v[i]);
+        else b = (u64)__builtin_bit_cast(unsigned, v[i]);
+        u64 x = b ^ ((u64)i * 0x9E3779B97F4A7C15ull);
+        x ^= x >> 30; x *= 0xBF58476D1CE4E5B9ull;
+        x ^= x >> 27; x *= 0x94D049BB133111EBull;
+        x ^= x >> 31;
+        h += x;
+    }
+    red[threadIdx.x] = h;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
 }
 
 // sum per-slot interior gradients into the total, in slot order
@@ -1457,6 +1454,7 @@ struct pw_fwi {
     // uploads a model or changes the set-up clears the flag
     bool dev_ckpt_valid = false;
     const void* dev_ckpt_vel = nullptr;
+    unsigned long long* dhash = nullptr;  // fingerprints of the forward_dev and adjoint_dev models
     size_t budget = 0;  // bytes allowed for checkpoints + segments
     double t_forward_ms = 0, t_backward_ms = 0;
 };
@@ -1503,7 +1501,7 @@ void fwi_free(pw_fwi* p)
     void* bufs[] = {p->vel, p->vpad, p->m, p->sfz, p->sfx, p->wav, p->X, p->Y, p->A, p->B, p->rec, p->res,
                     p->obs, p->W, p->ckpt, p->ckpt_snap, p->seg, p->gm, p->gslot, p->gtot, p->patch,
                     p->tmpv, p->work, p->shots, p->slot_shot, p->partial, p->dscalar, p->gfold, p->fsrc,
-                    p->tau_pad, p->taus_pad, p->R0, p->R1, p->RB0, p->RB1, p->ckpt_r};
+                    p->tau_pad, p->taus_pad, p->R0, p->R1, p->RB0, p->RB1, p->ckpt_r, p->dhash};
     for (void* b : bufs)
         if (b) hipFree(b);
 }
@@ -1971,6 +1969,17 @@ int finish_reference_gradient(pw_fwi* p, float* scaling_io)
     return PW_OK;
 }
 
+// Result watchdog of the gradient entry points: a blown-up run (time step beyond the CFL limit)
+// leaves an Inf/NaN misfit or gradient.  The L-BFGS line search calls gradient_all directly and
+// handles such trial models as the reference does (its Wolfe tests fail on them).
+int finite_result(pw_fwi* p, double fc, const char* what)
+{
+    if (!std::isfinite(fc))
+        PW_FAIL(p->ctx, PW_ERR_NONFINITE,
+                std::string(what) + ": the misfit is Inf or NaN (the run blew up; is the time step within the CFL limit?)");
+    return pw::check_finite(p->ctx, p->gtot, (size_t)p->nzx, (int)esz(p), what);
+}
+
 template <typename T>
 int dot_dev(pw_fwi* p, const void* x, const void* y, double* out)
 {
@@ -2067,6 +2076,7 @@ pw_fwi* pw_fwi_create(pw_ctx* ctx, const float* pararray, int batch, int precisi
     rc = rc ? rc : dmalloc(p, (void**)&p->slot_shot, sizeof(int) * p->nslot);
     rc = rc ? rc : dmalloc(p, (void**)&p->partial, 1024 * 8);
     rc = rc ? rc : dmalloc(p, (void**)&p->dscalar, 8);
+    rc = rc ? rc : dmalloc(p, (void**)&p->dhash, 16);
     if (!rc) rc = plan_storage(p, 0);
     if (rc) {
         fwi_free(p);
@@ -2336,6 +2346,8 @@ int pw_fwi_forward(pw_fwi* p, const float* vel, float* data_out)
         float m = 0;
         PW_HIPS(hipEventElapsedTime(&m, ev.e[0], ev.e[1]));
         fwd_ms += m;
+        if ((rc = pw::check_finite(p->ctx, p->rec, (size_t)nb_ * per, (int)esz(p), "fwi forward: modelled data")))
+            return rc;
         if (p->prec == 4) {
             PW_HIPS(hipMemcpyAsync(t.data(), p->rec, (size_t)nb_ * per * 4, hipMemcpyDeviceToHost, s));
             PW_HIPS(hipStreamSynchronize(s));
@@ -2370,6 +2382,7 @@ int pw_fwi_gradient(pw_fwi* p, const float* vel, int mode, float* grad, double* 
     if (p->prec == 4) rc = gradient_all<float>(p, mode, &fc);
     else rc = gradient_all<double>(p, mode, &fc);
     if (rc) return rc;
+    if ((rc = finite_result(p, fc, "fwi gradient"))) return rc;
     if (mode == 0) {
         if ((rc = finish_reference_gradient(p, scaling_io))) return rc;
         PW_HIPS(hipMemcpyAsync(grad, p->gtot, p->nzx * 4, hipMemcpyDeviceToHost, s));
@@ -2395,6 +2408,7 @@ int pw_fwi_gradient_f64(pw_fwi* p, const double* vel, int mode, double* grad, do
     int rc = upload_model<double>(p, nullptr, false);
     double fc = 0;
     if (!rc) rc = gradient_all<double>(p, mode, &fc);
+    if (!rc) rc = finite_result(p, fc, "fwi gradient");
     if (rc) return rc;
     PW_HIPS(hipMemcpyAsync(grad, p->gtot, p->nzx * 8, hipMemcpyDeviceToHost, s));
     PW_HIPS(hipStreamSynchronize(s));
@@ -2426,6 +2440,8 @@ int pw_fwi_misfit_f64(pw_fwi* p, const double* vel, double* fcost)
         if (!rc) rc = p->prec == 4 ? residual_batch<float>(p, nb_, 1, &fc) : residual_batch<double>(p, nb_, 1, &fc);
         if (rc) return rc;
     }
+    if (!std::isfinite(fc))
+        PW_FAIL(p->ctx, PW_ERR_NONFINITE, "fwi misfit: Inf or NaN (the run blew up; is the time step within the CFL limit?)");
     *fcost = fc;
     return PW_OK;
 }
@@ -2452,6 +2468,13 @@ int pw_fwi_forward_dev(pw_fwi* p, const void* d_vel, void* d_traces)
         PW_HIPS(hipMemcpyAsync((char*)d_traces + (size_t)b0 * per * e, p->rec, (size_t)nb_ * per * e,
                                hipMemcpyDeviceToDevice, s));
     }
+    if (keep) {  // fingerprint of the model the checkpoints belong to
+        if (p->prec == 4)
+            hipLaunchKernelGGL(k_fingerprint<float>, dim3(1), dim3(1024), 0, s, p->nzx, (const float*)d_vel, p->dhash);
+        else
+            hipLaunchKernelGGL(k_fingerprint<double>, dim3(1), dim3(1024), 0, s, p->nzx, (const double*)d_vel, p->dhash);
+        PW_HIPS(hipGetLastError());
+    }
     p->dev_ckpt_valid = keep;
     p->dev_ckpt_vel = d_vel;
     return PW_OK;
@@ -2465,8 +2488,22 @@ int pw_fwi_adjoint_dev(pw_fwi* p, const void* d_vel, const void* d_gtraces, void
     if (p->visco) PW_FAIL(p->ctx, PW_ERR_ARG, "adjoint_dev: the exact adjoint is acoustic (visco plans: reference gradient)");
     const Acq& a = p->acq;
     hipStream_t s = p->ctx->s();
-    // checkpoints of this very model from the preceding forward_dev (one batch): no forward sweep
-    const bool reuse = p->dev_ckpt_valid && p->dev_ckpt_vel == d_vel && p->nslot >= a.ns;
+    // checkpoints of this very model from the preceding forward_dev (one batch): no forward sweep.
+    // The same pointer is not enough (a caller may have changed the model in place): the model's
+    // fingerprint must equal the one taken by forward_dev.
+    bool reuse = p->dev_ckpt_valid && p->dev_ckpt_vel == d_vel && p->nslot >= a.ns;
+    if (reuse) {
+        if (p->prec == 4)
+            hipLaunchKernelGGL(k_fingerprint<float>, dim3(1), dim3(1024), 0, s, p->nzx, (const float*)d_vel, p->dhash + 1);
+        else
+            hipLaunchKernelGGL(k_fingerprint<double>, dim3(1), dim3(1024), 0, s, p->nzx, (const double*)d_vel,
+                               p->dhash + 1);
+        PW_HIPS(hipGetLastError());
+        unsigned long long h[2] = {0, 1};
+        PW_HIPS(hipMemcpyAsync(h, p->dhash, sizeof(h), hipMemcpyDeviceToHost, s));
+        PW_HIPS(hipStreamSynchronize(s));
+        reuse = h[0] == h[1];
+    }
     int rc = 0;
     if (!reuse)
         rc = p->prec == 4 ? upload_model<float>(p, (const float*)d_vel, true)

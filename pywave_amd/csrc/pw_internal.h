@@ -20,6 +20,7 @@ struct pw_ctx {
     std::string err;
     // autotuned 3D step choice per padded grid shape: (nz2, nx2, ny2) -> (variant, y chunk)
     std::map<std::tuple<int, int, int>, std::pair<int, int>> tuned;
+    unsigned* d_count = nullptr;  // device counter of pw::count_nonfinite (allocated on first use)
     hipStream_t s() const { return use_ext ? ext_stream : stream; }
 };
 
@@ -63,6 +64,13 @@ void damping_pairs(int n2, int nb_lo, int nb_hi, const float* w_lo, const float*
 
 template <typename T>
 inline T ceil_div(T a, T b) { return (a + b - 1) / b; }
+
+// Number of Inf/NaN values among n elements (elem_bytes 4 = f32, 8 = f64) of device array d,
+// counted on the context's stream; synchronises the stream.  The result watchdog of every plan.
+int count_nonfinite(pw_ctx* ctx, const void* d, size_t n, int elem_bytes, unsigned* count);
+
+// PW_ERR_NONFINITE with a message naming what blew up, if count_nonfinite finds any
+int check_finite(pw_ctx* ctx, const void* d, size_t n, int elem_bytes, const char* what);
 
 }  // namespace pw
 

@@ -22,6 +22,7 @@ import ctypes
 import numpy as np
 
 from . import _native as N
+from .stability import check_cfl
 
 __all__ = ["FwiProblem", "FwiForward", "fwi_traces", "make_pararray"]
 
@@ -133,6 +134,7 @@ class FwiProblem:
         v = np.asarray(vel)
         if v.shape != (self.nz, self.nx):
             raise ValueError("velocity must be (nz, nx) = (%d, %d)" % (self.nz, self.nx))
+        check_cfl(v, float(self.par[7]), (float(self.par[2]), float(self.par[3])), "fwi", "FwiProblem")
         return np.ascontiguousarray(v.astype(np.float32).ravel(order="F"))
 
     # ------------------------------------------------------------------ work

@@ -18,6 +18,7 @@ import ctypes
 import numpy as np
 
 from . import _native as N
+from .stability import check_cfl
 from .fwi import make_pararray
 
 __all__ = ["LstriProblem"]
@@ -47,6 +48,7 @@ class LstriProblem:
         v = np.asarray(vel)
         if v.shape != (self.nz, self.nx):
             raise ValueError("velocity must be (nz, nx) = (%d, %d)" % (self.nz, self.nx))
+        check_cfl(v, float(self.par[7]), (float(self.par[2]), float(self.par[3])), "fwi", "LstriProblem")
         v = np.ascontiguousarray(v.astype(np.float32).ravel(order="F"))
         N.check(self._L.pw_lstri_set_model(self._p, N.ptr(v)), self._ctx, "set_model")
 

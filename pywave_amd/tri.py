@@ -11,6 +11,7 @@ import ctypes
 import numpy as np
 
 from . import _native as N
+from .stability import check_cfl
 
 __all__ = ["tri3d_image", "group_masks"]
 
@@ -30,8 +31,8 @@ def group_masks(nx, ny, groups):
 def tri3d_image(vel, dat, groups, nt=None, dt=0.001, ax=[0, 20, 81], ay=[0, 20, 81], az=[0, 20, 81], nbt=30,
                 ct=0.01, jsnap=4, normalize=True, return_plan_time=False):
     """vel (nz, nx, ny); dat (nt, nx, ny) surface data; groups: list of x-index arrays or (nx, ny)
-    boolean masks (at most 4).  Returns the image (nz, nx, ny) (divided by its maximum when
-    normalize)."""
+    boolean masks (up to 64; more than 4 run in passes of 4, with each snapshot's running product
+    kept in HBM).  Returns the image (nz, nx, ny) (divided by its maximum when normalize)."""
     vel = np.asarray(vel, np.float32)
     nz, nx, ny = int(az[2]), int(ax[2]), int(ay[2])
     if vel.shape != (nz, nx, ny):
@@ -41,6 +42,7 @@ def tri3d_image(vel, dat, groups, nt=None, dt=0.001, ax=[0, 20, 81], ay=[0, 20, 
     if dat.shape != (nt, nx, ny):
         raise ValueError("dat must be (nt, nx, ny)")
     masks = group_masks(nx, ny, groups)
+    check_cfl(vel, dt, (float(az[1]), float(ax[1]), float(ay[1])), "afd", "tri3d_image")
     L = N.lib()
     ctx = N.context()
     p = L.pw_tri_create(ctx, nz, nx, ny, float(az[1]), float(ax[1]), float(ay[1]), nt, float(dt), int(nbt),

@@ -82,3 +82,36 @@ def test_single_shot_lstri_plan(golden):
     p = LstriProblem(par, g["vel"])
     d = p.model(g["src"][..., :1])
     assert np.array_equal(d[:, :, 0], O.timerev(par, g["vel"], g["src"][..., 0], adj=False))
+
+
+def test_cfl_violation_warns_and_the_watchdog_raises():
+    # v dt sqrt(2)/h = 1.2 x the 4th-order scheme's limit: a CFLWarning before the run, and the
+    # result watchdog reports the blown-up traces and wavefield instead of returning Inf/NaN
+    from pywave_amd._native import NonFiniteError
+    from pywave_amd.stability import CFLWarning, cfl_limit
+    h, v = 10.0, 3000.0
+    dt = 1.2 * cfl_limit("afd") * h / (v * np.sqrt(2))
+    vel = np.full((40, 50), v, np.float32)
+    with pytest.warns(CFLWarning):
+        with pytest.raises(NonFiniteError, match="Inf or NaN"):
+            pywave_amd.afd2d(vel, nt=600, dt=dt, ax=[0, h, 50], az=[0, h, 40], ns=1, sx=[25], sz=[20], f=[20],
+                             t=[0.01], A=[1], nbt=10, ct=0.01, jsnap=0, verb=False)
+    # the same model within the limit runs
+    dt_ok = 0.9 * cfl_limit("afd") * h / (v * np.sqrt(2))
+    out = pywave_amd.afd2d(vel, nt=600, dt=dt_ok, ax=[0, h, 50], az=[0, h, 40], ns=1, sx=[25], sz=[20], f=[20],
+                           t=[0.01], A=[1], nbt=10, ct=0.01, jsnap=0, verb=False)
+    assert np.isfinite(out).all()
+
+
+def test_fwi_watchdog_raises_on_a_blown_up_forward():
+    from pywave_amd._native import NonFiniteError
+    from pywave_amd.stability import CFLWarning, cfl_limit
+    nz, nx, nt, h, v = 30, 40, 400, 0.01, 3.0
+    dt = 1.2 * cfl_limit("fwi") * h / (v * np.sqrt(2))
+    par = np.array([nz, nx, h, h, 0, 0, nt, dt, 0, 0, 2, 0.1, 0.1, 5, 8, 0.01, 10, 1, 1, 1], np.float32)
+    wav = (1e3 * _ricker(nt, dt, 10.0, 0.02, 1.0)).astype(np.float32)
+    p = FwiProblem(par, wav, batch=2)
+    with pytest.warns(CFLWarning):
+        with pytest.raises(NonFiniteError):
+            p.forward(np.full((nz, nx), v, np.float32))
+    p.close()

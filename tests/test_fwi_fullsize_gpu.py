@@ -53,3 +53,25 @@ def test_full_size_gradient_vs_finite_differences():
     rel = float(np.linalg.norm(g32.astype(np.float64) - g) / np.linalg.norm(g))
     print("full-size f32 gradient vs f64: rel-L2 %.2e" % rel)
     assert rel <= 1e-3
+
+
+@pytest.mark.timeout(600)
+def test_full_size_lbfgs_five_iterations_lower_the_misfit():
+    # fwi() (pywave/src/wave_fwi.c:24-119) on the benchmark configuration: 5 L-BFGS iterations of
+    # the reference gradient (gradient_av) with Wolfe line searches; every stored model lowers the
+    # misfit and stays inside the reference's clip [0, 10] km/s
+    w = fwi2d()
+    par = dict(w["par"])
+    nt, ns = par["nt"], par["ns"]
+    wav = _wavelet(nt, par["dt"], w["f0"], w["t0"], w["amp"])
+    gen = FwiProblem(par, wav, batch=ns)
+    obs = gen.forward(w["vtrue"])
+    gen.close()
+    p = FwiProblem(par, wav, data=obs, batch=ns, checkpoint=50)
+    vinv, k = p.lbfgs(w["v0"], 5)
+    assert k == 5 and np.isfinite(vinv).all()
+    assert vinv.min() >= 0.0 and vinv.max() <= 10.0
+    f = [p.misfit(w["v0"])] + [p.misfit(vinv[:, :, i]) for i in range(5)]
+    p.close()
+    print("full-size L-BFGS misfits:", ["%.6g" % x for x in f])
+    assert all(f[i + 1] < f[i] for i in range(5))

@@ -257,3 +257,36 @@ def test_autograd_checkpoint_reuse_and_repeat():
         g2 = v.grad
         assert torch.equal(g1, g2), batch
         assert rel_l2(g1.cpu().numpy(), g_direct) <= 1e-5, batch
+
+
+def test_adjoint_dev_recomputes_after_in_place_model_change():
+    # pw_fwi_forward_dev keeps its checkpoints for the next pw_fwi_adjoint_dev on the same model
+    # pointer; a model changed in place between the two calls must not reuse them
+    torch = pytest.importorskip("torch")
+    par, wav, vt, v0 = _fd_case()
+    obs = FwiProblem(par, wav).forward(vt)
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    g = torch.tensor(np.ascontiguousarray((0.01 * obs).transpose(2, 0, 1)), device=dev)  # cotangent
+    nz, nx = v0.shape
+    v1 = (1.02 * v0).astype(np.float32)
+
+    def adjoint(p, vz):
+        gv = torch.empty_like(vz)
+        p.adjoint_dev(vz.data_ptr(), g.data_ptr(), gv.data_ptr(), stream)
+        torch.cuda.synchronize()
+        return gv.cpu().numpy()
+
+    # the gradient at v1 from a plan that never saw another model
+    fresh = FwiProblem(par, wav, batch=2, checkpoint=30)
+    vz1 = torch.tensor(np.ascontiguousarray(v1.T), device=dev)  # z fastest
+    g_ref = adjoint(fresh, vz1)
+    fresh.close()
+    p = FwiProblem(par, wav, batch=2, checkpoint=30)
+    vz = torch.tensor(np.ascontiguousarray(v0.astype(np.float32).T), device=dev)
+    tr = torch.empty((p.ns, p.nt, p.nr), dtype=torch.float32, device=dev)
+    p.forward_dev(vz.data_ptr(), tr.data_ptr(), stream)
+    vz.copy_(vz1)  # same pointer, new values
+    g_got = adjoint(p, vz)
+    p.close()
+    assert np.array_equal(g_got, g_ref)

@@ -89,3 +89,32 @@ def test_tri_image_focuses_on_sources():
     peak = np.array(np.unravel_index(np.argmax(img), img.shape))
     d = np.abs(pos - peak)
     assert np.any((d[:, 1] <= 2) & (d[:, 2] <= 2) & (d[:, 0] <= 7)), peak
+
+
+def _recipe_image_f64(vel, dat, groups, jsnap):
+    """The demo's recipe from the oracle's per-group back-propagations, products in f64."""
+    nt, nx, ny = dat.shape
+    img = None
+    for gsel in groups:
+        dg = np.zeros_like(dat)
+        dg[:, gsel, :] = dat[:, gsel, :]
+        _, snaps, _ = O.afd3d(vel, nt, 0.001, 10, 10, 10, 8, 0.04, [], [], [], None, tri=True,
+                              dat=np.ascontiguousarray(dg.transpose(2, 1, 0)), jsnap=jsnap)
+        u2 = snaps.transpose(3, 2, 1, 0).astype(np.float64) ** 2  # (nz, nx, ny, nsnap)
+        img = u2 if img is None else img * u2
+    return img.sum(axis=3)
+
+
+@pytest.mark.parametrize("ng", [5, 6, 9])
+def test_tri_image_more_than_four_groups(ng):
+    # groups beyond four run in passes of four with each snapshot's running product kept in HBM;
+    # the data are scaled so that the 2*ng-th powers stay far from f32 underflow
+    vel, dat, groups = _case(seed=7, nt=120, ng=ng)
+    dat = (dat / np.abs(dat).max()).astype(np.float32)
+    ref = _recipe_image_f64(vel, dat, groups, 4)
+    nz, nx, ny = vel.shape
+    img = pywave_amd.tri3d_image(vel, dat, groups, dt=0.001, ax=[0, 10, nx], ay=[0, 10, ny], az=[0, 10, nz], nbt=8,
+                                 ct=0.04, jsnap=4, normalize=False)
+    print("tri %d groups: rel-L2 against the f64 recipe %.2e" % (ng, rel_l2(img, ref)))
+    assert np.isfinite(img).all() and img.max() > 0
+    assert rel_l2(img, ref) <= 1e-5
