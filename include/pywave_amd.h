@@ -15,6 +15,8 @@
 #ifndef PYWAVE_AMD_H
 #define PYWAVE_AMD_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -61,6 +63,21 @@ int pw_ricker(int nt, float dt, float f0, float t0, float A, float *out);
 int pw_afd2d(pw_ctx *ctx, const float *vel, const float *src, int tri, int nt, int nx, int nz,
              int ns, int verb, int jsnap, int ifsnaps, int abc, int nbt, float ct, float dt,
              float ox, float dx, float oz, float dz, float *out, long long out_len);
+
+/* Allocation-free 2D modelling (no counterpart in the reference): for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The same run as pw_afd2d's modelling (tri = 0, jsnap = 0) on device
+ * buffers: d_vel nz*nx (z fastest), src_xz host [sx.., sz..] (2*ns grid indices, as afd2dc
+ * truncates them), d_wav ns*nt wavelets (row per source, e.g. from pw_ricker), d_traces nx*nt
+ * (the reference's [ix][it]).  Every field of the run lives in one file-scope device array of
+ * pw_afd2d_noalloc_capacity() bytes (64 MiB) per GPU, held until the process ends; a run that
+ * needs more returns PW_ERR_ARG.  Calls by any thread on one GPU share that array: calls that
+ * overlap in time on one GPU race.  Enqueued on the context's stream; nothing is allocated,
+ * freed or synchronised. */
+size_t pw_afd2d_noalloc_capacity(void);
+int pw_afd2d_dev_noalloc(pw_ctx *ctx, const float *d_vel, int nz, int nx, float dz, float dx, int nt,
+                         float dt, int nbt, float ct, int ns, const int *src_xz, const float *d_wav,
+                         float *d_traces);
 
 /* afdcfun.afd3dc (pywave/src/afd.c:23-314); src is [sx sy sz f t A] (6*ns) or nt*nx*ny data. */
 int pw_afd3d(pw_ctx *ctx, const float *vel, const float *src, int tri, int nt, int nx, int ny,

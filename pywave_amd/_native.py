@@ -43,6 +43,9 @@ SIGNATURES = {
     "pw_aps3d": (c_int, [c_vp, c_vp, c_vp, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int,
                          c_int, c_int, c_float, c_float, c_float, c_float, c_float, c_float, c_float,
                          c_float, c_vp, c_ll]),
+    "pw_afd2d_noalloc_capacity": (ctypes.c_size_t, []),
+    "pw_afd2d_dev_noalloc": (c_int, [c_vp, c_vp, c_int, c_int, c_float, c_float, c_int, c_float, c_int, c_float,
+                                     c_int, c_vp, c_vp, c_vp]),
     "pw_afd_output_len": (c_ll, [c_int, c_int, c_int, c_int, c_int]),
     "pw_afd_create": (c_vp, [c_vp, c_int, c_int, c_int, c_int, c_float, c_float, c_float, c_int, c_float,
                              c_int, c_float, c_int]),

@@ -18,6 +18,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 
 #include <hip/hip_ext.h>
 
@@ -2074,6 +2076,139 @@ int pw_afd2d(pw_ctx* ctx, const float* vel, const float* src, int tri, int nt, i
     (void)verb; (void)ifsnaps; (void)abc; (void)ox; (void)oz;
     return afd_oneshot(ctx, 2, vel, src, tri, nt, nx, 1, nz, ns, jsnap, nbt, ct, dt, dx, 1.f, dz, out,
                        out_len);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- allocation-free 2D modelling
+// pw_afd2d_dev_noalloc keeps every field of its run in one file-scope device array instead of
+// memory it allocates: callers that cannot call hipMalloc (code under HIP graph capture, tight
+// serving loops, harnesses that forbid it) pass device inputs and outputs only.  The array holds
+// kNoAllocBytes per GPU until the process ends; each GPU has its own copy at its own address,
+// found once per device by hipGetSymbolAddress.  In host code the name 'scratch' is a host-side
+// stand-in, never a device address: only the looked-up pointer is passed to kernels.
+static constexpr size_t kNoAllocBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+
+std::mutex g_scratch_mu;
+std::map<int, unsigned char*> g_scratch_ptr;  // device -> the array's device address
+
+int scratch_on_device(pw_ctx* ctx, unsigned char** out)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_ptr.find(ctx->device);
+    if (it == g_scratch_ptr.end()) {
+        PW_HIP(ctx, hipSetDevice(ctx->device));
+        void* ptr = nullptr;
+        PW_HIP(ctx, hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) PW_FAIL(ctx, PW_ERR_HIP, "noalloc: no device address for the scratch array");
+        it = g_scratch_ptr.emplace(ctx->device, (unsigned char*)ptr).first;
+    }
+    *out = it->second;
+    return PW_OK;
+}
+
+__global__ void k_fill_f32(float* __restrict__ p, size_t n, float v)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t pw_afd2d_noalloc_capacity(void) { return kNoAllocBytes; }
+
+int pw_afd2d_dev_noalloc(pw_ctx* ctx, const float* d_vel, int nz, int nx, float dz, float dx, int nt, float dt,
+                         int nbt, float ct, int ns, const int* src_xz, const float* d_wav, float* d_traces)
+{
+    if (!ctx) return PW_ERR_ARG;
+    if (!d_vel || !d_traces || nz < 1 || nx < 1 || nt < 1 || nbt < 0 || ns < 0 || !(dz > 0.f) || !(dx > 0.f) ||
+        !(dt > 0.f) || (ns > 0 && (!src_xz || !d_wav)))
+        PW_FAIL(ctx, PW_ERR_ARG, "afd2d_dev_noalloc: invalid dimensions, spacings or pointers");
+    unsigned char* base = nullptr;
+    if (int rc = scratch_on_device(ctx, &base)) return rc;
+    Geo g{};
+    g.nz1 = nz; g.nx1 = nx; g.ny1 = 1;
+    g.nb = nbt;
+    g.nz = nz + 2 * nbt; g.nx = nx + 2 * nbt; g.ny = 1;
+    pw::padded_sizes(g.nz, g.nx, 1, &g.nz2, &g.nx2, &g.ny2);
+    g.ny2 = 1;
+    g.ldz = pw::ceil_div(g.nz2, LDZ_ALIGN) * LDZ_ALIGN;
+    g.ldxy = (long long)g.ldz * g.nx2;
+    g.idz = 1 / (dz * dz);
+    g.idx = 1 / (dx * dx);
+    g.idy = 0.f;
+    g.cc = g.idx + g.idz;
+    // carve the array: three fields, damping pairs, source cells, time-major receiver rows
+    const size_t field = (size_t)g.ldxy;
+    size_t off = 0;
+    auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) / 256 * 256; return o; };
+    const size_t oA = take(field * 4), oB = take(field * 4), oV = take(field * 4);
+    const size_t oFz = take((size_t)g.nz2 * 8), oFx = take((size_t)g.nx2 * 8), oFy = take(8);
+    const size_t oS = take((size_t)3 * std::max(ns, 1) * 4), oR = take((size_t)nt * nx * 4);
+    if (off > kNoAllocBytes)
+        PW_FAIL(ctx, PW_ERR_ARG,
+                "afd2d_dev_noalloc: the run needs " + std::to_string(off) + " bytes of device scratch, more than the " +
+                    std::to_string(kNoAllocBytes) + " the allocation-free entry point holds");
+    float* A = (float*)(base + oA);
+    float* B = (float*)(base + oB);
+    float* V = (float*)(base + oV);
+    float2* fz = (float2*)(base + oFz);
+    float2* fx = (float2*)(base + oFx);
+    float2* fy = (float2*)(base + oFy);
+    int* src = (int*)(base + oS);
+    float* rec = (float*)(base + oR);
+    // damping pairs (pywave/src/afd.c:392-402: every face nbt cells with ct) and source cells,
+    // checked as pw_afd_set_sources checks them
+    std::vector<float> w(std::max(nbt, 1));
+    pw::abc_weights(nbt, ct, w.data());
+    std::vector<float> pz, px, py(2, 1.0f);
+    pw::damping_pairs(g.nz2, nbt, nbt, w.data(), w.data(), pz);
+    pw::damping_pairs(g.nx2, nbt, nbt, w.data(), w.data(), px);
+    std::vector<int> cen(3 * std::max(ns, 1), 0);
+    for (int i = 0; i < ns; ++i) {
+        const int gz = src_xz[ns + i] + nbt, gx = src_xz[i] + nbt;
+        if (!(gz - 1 >= 0 && gz + 1 < g.nz && gx - 1 >= 0 && gx + 1 < g.nx))
+            PW_FAIL(ctx, PW_ERR_ARG, "afd2d_dev_noalloc: source " + std::to_string(i) +
+                                         " stencil leaves the padded domain");
+        cen[3 * i] = gz; cen[3 * i + 1] = gx; cen[3 * i + 2] = 0;
+    }
+    hipStream_t s = ctx->s();
+    PW_HIP(ctx, hipSetDevice(ctx->device));
+    // the array keeps the previous call's contents: clear the fields (a fill kernel, so that the
+    // sequence also records into a graph)
+    const unsigned fb = (unsigned)std::min<size_t>(4096, pw::ceil_div<size_t>(3 * field, 256));
+    hipLaunchKernelGGL(k_fill_f32, dim3(fb), dim3(256), 0, s, A, (size_t)(oV + field * 4 - oA) / 4, 0.f);
+    PW_HIP(ctx, hipGetLastError());
+    PW_HIP(ctx, hipMemcpyAsync(fz, pz.data(), pz.size() * 4, hipMemcpyHostToDevice, s));
+    PW_HIP(ctx, hipMemcpyAsync(fx, px.data(), px.size() * 4, hipMemcpyHostToDevice, s));
+    PW_HIP(ctx, hipMemcpyAsync(fy, py.data(), py.size() * 4, hipMemcpyHostToDevice, s));
+    PW_HIP(ctx, hipMemcpyAsync(src, cen.data(), cen.size() * 4, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_expand_vv, dim3(pw::ceil_div(g.nz, 256), g.nx, 1), dim3(256), 0, s, g, d_vel, V, dt);
+    PW_HIP(ctx, hipGetLastError());
+    Step a{};
+    a.vv = V; a.fz = fz; a.fx = fx; a.fy = fy;
+    a.src = src; a.wav = d_wav; a.ns = ns; a.nt = nt;
+    constexpr int W = 4;
+    const dim3 grid(pw::ceil_div(g.nz, 64), pw::ceil_div(g.nx, W));
+    float* cur = A;
+    float* pv = B;
+    for (int it = 0; it < nt; ++it) {
+        a.cur = cur; a.pv = pv; a.it = it;
+        a.rec = rec + (size_t)it * nx;
+        hipLaunchKernelGGL(k_afd2d<W>, grid, dim3(64 * W), 0, s, g, a);
+        std::swap(cur, pv);
+    }
+    PW_HIP(ctx, hipGetLastError());
+    // time-major rows -> the reference's time-fastest traces [ix][it]
+    hipLaunchKernelGGL(k_transpose, dim3(pw::ceil_div(nx, 32), pw::ceil_div(nt, 32)), dim3(32, 8), 0, s, rec, d_traces,
+                       nt, nx);
+    PW_HIP(ctx, hipGetLastError());
+    return PW_OK;
 }
 
 int pw_afd3d(pw_ctx* ctx, const float* vel, const float* src, int tri, int nt, int nx, int ny,
