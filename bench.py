@@ -90,10 +90,11 @@ def fwi_bench(plain_shots):
         torch.cuda.synchronize()
         return v.grad, loss.item()
 
-    grad_autograd()  # warm-up
+    grad_autograd()  # warm-up (also times the step kernels' tile widths)
     t = time.perf_counter()
     g, f = grad_autograd()
     tb = time.perf_counter() - t
+    out["batched_tile_widths"] = prob.widths()
     prob.close()
     out["batched_shots_per_s"] = round(ns / tb, 3)
     out["batched_s_per_gradient"] = round(tb, 4)
@@ -134,6 +135,7 @@ def fwi_bench(plain_shots):
     t = time.perf_counter()
     plain.gradient(w["v0"], "exact")
     tp = time.perf_counter() - t
+    out["plain_tile_widths"] = plain.widths()
     plain.close()
     out["plain_shots_per_s"] = round(plain_shots / tp, 3)
     out["plain_sample_shots"] = plain_shots

@@ -183,6 +183,10 @@ int pw_fwi_misfit_f64(pw_fwi *p, const double *vel, double *fcost);
  * compared, so a model changed in place between the two calls is swept forward again). */
 int pw_fwi_forward_dev(pw_fwi *p, const void *d_vel, void *d_traces);
 int pw_fwi_adjoint_dev(pw_fwi *p, const void *d_vel, const void *d_gtraces, void *d_gvel);
+/* tile widths (x columns per wave) of the step kernels: w[0] forward, w[1] visco-acoustic forward,
+ * w[2] exact adjoint, w[3] reference adjoint, w[4] visco-acoustic reference adjoint; w[5] = 1 once
+ * the plan has timed the widths on its grid and batch (its first sweep does), 0 before */
+int pw_fwi_widths(pw_fwi *p, int *w);
 /* event-timed milliseconds of the last gradient (forward sweeps, backward sweeps) or of the last
  * pw_fwi_forward (its sweeps, 0) */
 int pw_fwi_timing(pw_fwi *p, double *forward_ms, double *backward_ms);

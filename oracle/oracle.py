@@ -94,6 +94,8 @@ def lib():
         L.orc_fwi.argtypes = [ctypes.POINTER(OrcAcq), _f32p, _f32p, _f32p, ctypes.c_int, _f32p]
         L.orc_fwi_forward_visco.argtypes = [ctypes.POINTER(OrcAcq), _f32p, _f32p, ctypes.c_float, _f32p, _f32p]
         L.orc_fwi.restype = ctypes.c_int
+        L.orc_fcost_f32.argtypes = [_f32p, _f32p, ctypes.c_longlong]
+        L.orc_fcost_f32.restype = ctypes.c_float
         L.orc_source_map.argtypes = [ctypes.c_int] * 6 + [_f32p]
         L.orc_gradient_smooth2.argtypes = [ctypes.c_int] * 5 + [ctypes.c_float, _f32p]
         L.orc_timerev.argtypes = [ctypes.c_int] * 4 + [ctypes.c_float] * 4 + [_f32p, ctypes.c_int, ctypes.c_int,
@@ -303,6 +305,14 @@ def fwi_forward(pararray, vel, wav, src=None):
     res = out.reshape(a.nt, a.nr, a.ns, order="F")
     lib().orc_fwi_release(ctypes.byref(a))
     return res
+
+
+def fcost_f32(data, pred):
+    """gradient_av's misfit as the reference accumulates it (f32 accumulator, f64 terms) over data
+    (nt, nr, ns) in its loop order."""
+    d = np.ascontiguousarray(np.asarray(data, np.float32).ravel(order="F"))
+    p = np.ascontiguousarray(np.asarray(pred, np.float32).ravel(order="F"))
+    return float(lib().orc_fcost_f32(d, p, d.size))
 
 
 def gradient_av(pararray, vel, wav, data, scaling=0.0, src=None, q=None):

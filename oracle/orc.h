@@ -123,6 +123,10 @@ int orc_gradient_av(const orc_acq *a, const float *vel, const float *wav, const 
 int orc_fwi(const orc_acq *a, const float *vel0, const float *wav, const float *data,
             int niter, float *vinv);
 
+/* gradient_av's f32 misfit accumulation over n residual samples in the reference's order
+ * (shot, receiver, sample: the F order of data (nt, nr, ns)) */
+float orc_fcost_f32(const float *d, const float *p, long long n);
+
 /* Triangle smoothing pieces used by source_map / gradient_smooth2 */
 void orc_source_map(int sx, int sz, int rectx, int rectz, int padnx, int padnz, float *rr);
 void orc_gradient_smooth2(int rectx, int rectz, int nx, int nz, int waterz, float scaling,

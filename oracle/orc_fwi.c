@@ -607,3 +607,16 @@ int orc_fwi(const orc_acq *a, const float *vel0, const float *wav, const float *
     free(x); free(g); free(dir); free(xk); free(o.sk); free(o.yk);
     return stored;
 }
+
+float orc_fcost_f32(const float *d, const float *p, long long n)
+{
+    /* gradient_av's misfit (pywave/src/wave_fwigradient.c:179-184): the residual is stored in
+     * f32, 0.5*r*r is formed in double and added to the f32 accumulator, shot by shot, receiver
+     * by receiver, sample by sample */
+    float f = 0.f;
+    for (long long i = 0; i < n; i++) {
+        float r = d[i] - p[i];
+        f += 0.5 * r * r;
+    }
+    return f;
+}

@@ -38,16 +38,18 @@ namespace {
 
 constexpr double PI_D = 3.14159265358979323846264338328;
 constexpr int PMAX = 7;  // largest source-map patch edge (rect <= 4)
-constexpr int XC = 8;   // x columns streamed per block
-constexpr int XCV = 16;  // the visco-acoustic imaging adjoint's tiles (it forms 8 halo columns per tile)
+// Tile widths (x columns streamed per wave) the step kernels are built for; the plan picks one per
+// kernel by timing them on its own grid and batch (tune_widths), 8 until then.
+constexpr int kWidths[] = {4, 8, 16};
+constexpr int kDefaultWidth = 8;
 
-// Tiles of the step kernels: 64 z cells x XC columns of one shot slot, one wave each.
+// Tiles of the step kernels: 64 z cells x W columns of one shot slot, one wave each.
 struct TileMap {
     int ntz, nxc;  // z tiles and column groups per slot
     int nblk;      // tiles over all slots of the launch
 };
 
-TileMap tile_map(int padnz, int padnx, int nslots, int width = XC)
+TileMap tile_map(int padnz, int padnx, int nslots, int width)
 {
     TileMap t;
     t.ntz = (padnz + 63) / 64;
@@ -156,10 +158,10 @@ using pwbuf::bsto;
 
 
 // ---------------------------------------------------------------- forward step (+ modelling)
-// One 64 x XC tile of one shot per block (one wave), streamed along x with a 9-column register
+// One 64 x W tile of one shot per block (one wave), streamed along x with a 9-column register
 // queue (radius 4) and an LDS z row; the operands of column x+1 (previous level, model, z halo,
 // sponge pair, and the queue entry x+5) are in flight while column x is computed.
-template <typename T, bool VISCO = false>
+template <typename T, bool VISCO, int XC>
 __global__ __launch_bounds__(64) void k_fwd(FGeo<T> g, StepArgs<T> a, TileMap tm)
 {
     using C = Coef<T>;
@@ -842,7 +844,7 @@ struct ExArgs {
 // q_t); the m*A product is zero outside the Laplacian's rows [4, n-4).  The adjoint has no
 // counterpart in the reference whose rounding it must follow, so it contracts to FMAs and divides
 // by the squared spacings through reciprocals.
-template <typename T>
+template <typename T, int XC>
 __global__ __launch_bounds__(64) void k_adj_exact(FGeo<T> g, StepArgs<T> a, ExArgs<T> ex, TileMap tm)
 {
 #pragma clang fp contract(fast)
@@ -1118,16 +1120,19 @@ __global__ void k_sum_partials(int n, const double* __restrict__ partial, double
     *out = s;
 }
 
-// An order-independent 64-bit fingerprint of n values (one block of 1024 threads): the sum over i
-// of the splitmix64 finaliser of (bits_i xor i * golden-ratio constant).  Any change to a value
-// changes it except with probability about 2^-64.
+// An order-independent 64-bit fingerprint of n values: the sum (mod 2^64) over i of the splitmix64
+// finaliser of (bits_i xor i * golden-ratio constant), added up by the blocks in any order into
+// *out, which the caller clears first.  Any change to a value changes it except with probability
+// about 2^-64.
+__global__ void k_zero_u64(unsigned long long* p) { *p = 0; }
+
 template <typename T>
-__global__ __launch_bounds__(1024) void k_fingerprint(long long n, const T* __restrict__ v, unsigned long long* out)
+__global__ __launch_bounds__(256) void k_fingerprint(long long n, const T* __restrict__ v, unsigned long long* out)
 {
     using u64 = unsigned long long;
-    __shared__ u64 red[1024];
+    __shared__ u64 red[256];
     u64 h = 0;
-    for (long long i = threadIdx.x; i < n; i += blockDim.x) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
         u64 b;
         if constexpr (sizeof(T) == 8) b = __builtin_bit_cast(u64, v[i]);
         else b = (u64)__builtin_bit_cast(unsigned, v[i]);
@@ -1143,8 +1148,10 @@ __global__ __launch_bounds__(1024) void k_fingerprint(long long n, const T* __re
         if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
         __syncthreads();
     }
-    if (threadIdx.x == 0) *out = red[0];
+    if (threadIdx.x == 0) atomicAdd(out, red[0]);
 }
+
+int fingerprint(pw_fwi* p, const void* d_vel, unsigned long long* out);
 
 // sum per-slot interior gradients into the total, in slot order
 template <typename T>
@@ -1456,6 +1463,11 @@ struct pw_fwi {
     const void* dev_ckpt_vel = nullptr;
     unsigned long long* dhash = nullptr;  // fingerprints of the forward_dev and adjoint_dev models
     size_t budget = 0;  // bytes allowed for checkpoints + segments
+    // tile widths of the step kernels (forward, visco forward, exact adjoint, reference adjoint,
+    // visco reference adjoint), picked by tune_widths for the plan's grid and batch
+    int w_fwd = kDefaultWidth, w_fwdv = kDefaultWidth, w_exact = kDefaultWidth, w_ref = kDefaultWidth,
+        w_refv = kDefaultWidth;
+    bool widths_tuned = false;
     double t_forward_ms = 0, t_backward_ms = 0;
 };
 
@@ -1651,6 +1663,36 @@ int fill(pw_fwi* p, void* ptr, long long n, T v)
     return PW_OK;
 }
 
+// launches of the step kernels at a tile width (one of kWidths)
+template <typename T, bool VISCO>
+void launch_fwd(int W, const FGeo<T>& g, const StepArgs<T>& a, int pnz, int pnx, int nb_, hipStream_t s)
+{
+    const TileMap tm = tile_map(pnz, pnx, nb_, W);
+    if (W == 4) hipLaunchKernelGGL((k_fwd<T, VISCO, 4>), tile_grid(tm), dim3(64), 0, s, g, a, tm);
+    else if (W == 16) hipLaunchKernelGGL((k_fwd<T, VISCO, 16>), tile_grid(tm), dim3(64), 0, s, g, a, tm);
+    else hipLaunchKernelGGL((k_fwd<T, VISCO, 8>), tile_grid(tm), dim3(64), 0, s, g, a, tm);
+}
+
+template <typename T>
+void launch_exact(int W, const FGeo<T>& g, const StepArgs<T>& a, const ExArgs<T>& ex, int pnz, int pnx, int nb_,
+                  hipStream_t s)
+{
+    const TileMap tm = tile_map(pnz, pnx, nb_, W);
+    if (W == 4) hipLaunchKernelGGL((k_adj_exact<T, 4>), tile_grid(tm), dim3(64), 0, s, g, a, ex, tm);
+    else if (W == 16) hipLaunchKernelGGL((k_adj_exact<T, 16>), tile_grid(tm), dim3(64), 0, s, g, a, ex, tm);
+    else hipLaunchKernelGGL((k_adj_exact<T, 8>), tile_grid(tm), dim3(64), 0, s, g, a, ex, tm);
+}
+
+template <bool VISCO>
+void launch_ref(int W, const FGeo<float>& g, const StepArgs<float>& a, const ImgArgs<float>& im, const ViscoBack& vb,
+                int pnz, int pnx, int nb_, hipStream_t s)
+{
+    const TileMap tm = tile_map(pnz, pnx, nb_, W);
+    if (W == 4) hipLaunchKernelGGL((k_adj_ref<VISCO, 4>), tile_grid(tm), dim3(64), 0, s, g, a, im, vb, tm);
+    else if (W == 16) hipLaunchKernelGGL((k_adj_ref<VISCO, 16>), tile_grid(tm), dim3(64), 0, s, g, a, im, vb, tm);
+    else hipLaunchKernelGGL((k_adj_ref<VISCO, 8>), tile_grid(tm), dim3(64), 0, s, g, a, im, vb, tm);
+}
+
 // three timing events, released on every return path
 struct EventSet {
     hipEvent_t e[3] = {nullptr, nullptr, nullptr};
@@ -1667,6 +1709,96 @@ struct EventSet {
     }
 };
 
+// Milliseconds per launch of fn (two warm-up launches, then `reps` timed ones).
+template <typename F>
+int time_launches(pw_fwi* p, F fn, int reps, float* ms)
+{
+    hipStream_t s = p->ctx->s();
+    EventSet ev;
+    PW_HIPS(ev.create());
+    fn();
+    fn();
+    PW_HIPS(hipEventRecord(ev.e[0], s));
+    for (int k = 0; k < reps; ++k) fn();
+    PW_HIPS(hipEventRecord(ev.e[1], s));
+    PW_HIPS(hipGetLastError());
+    PW_HIPS(hipEventSynchronize(ev.e[1]));
+    PW_HIPS(hipEventElapsedTime(ms, ev.e[0], ev.e[1]));
+    *ms /= reps;
+    return PW_OK;
+}
+
+// Tile-width autotuner of the step kernels: times each width in kWidths for the kernels this plan
+// runs, on its own grid, batch and buffers, and keeps the fastest per kernel.  It runs before a
+// sweep sets its fields up (every sweep clears what the timing launches write: the fields, traces,
+// adjoint and gradient accumulators), on the first batch's shots.
+template <typename T>
+int tune_widths(pw_fwi* p)
+{
+    const Acq& a = p->acq;
+    hipStream_t s = p->ctx->s();
+    const int nb_ = std::min(p->nslot, a.ns);
+    int rc = set_batch(p, 0, nb_);
+    if (rc) return rc;
+    const FGeo<T> g = geo<T>(p);
+    const int reps = 8;
+    auto pick = [&](auto launch, int* best) -> int {
+        float tbest = 0.f;
+        for (int W : kWidths) {
+            float ms = 0.f;
+            if (int r = time_launches(p, [&] { launch(W); }, reps, &ms)) return r;
+            if (W == kWidths[0] || ms < tbest) { tbest = ms; *best = W; }
+        }
+        return PW_OK;
+    };
+    const bool visco = p->visco && sizeof(T) == 4;
+    StepArgs<T> f = base_args<T>(p);
+    f.X = (const T*)p->X; f.Y = (T*)p->Y; f.it = 0;
+    f.fsrc = step_src<T>(p, 0);
+    f.rec = (T*)p->rec;
+    if (visco) {
+        f.tau = p->tau_pad; f.taus = p->taus_pad; f.r1 = p->R0; f.r2 = p->R1;
+        f.idt = (float)(1. / (double)a.dt);
+        rc = pick([&](int W) { launch_fwd<T, true>(W, g, f, a.padnz, a.padnx, nb_, s); }, &p->w_fwdv);
+    } else {
+        rc = pick([&](int W) { launch_fwd<T, false>(W, g, f, a.padnz, a.padnx, nb_, s); }, &p->w_fwd);
+    }
+    if (rc) return rc;
+    StepArgs<T> b = base_args<T>(p);
+    b.X = (const T*)p->A; b.Y = (T*)p->B; b.it = a.nt - 1;
+    b.fsrc = step_src<T>(p, a.nt - 1);
+    b.inj = (const T*)p->res;
+    if (!visco) {
+        ExArgs<T> ex{(const T*)p->seg, p->fs, (T*)p->gm};
+        rc = pick([&](int W) { launch_exact<T>(W, g, b, ex, a.padnz, a.padnx, nb_, s); }, &p->w_exact);
+        if (rc) return rc;
+    }
+    if constexpr (sizeof(T) == 4) {
+        const long long ss = (long long)p->nslot * p->nzx;
+        ImgArgs<T> im{};  // an imaging step: three snapshots, the -2/v^3 field and a gradient
+        im.w_prev = (const T*)p->seg; im.w_cur = (const T*)p->seg + ss; im.w_next = (const T*)p->seg + 2 * ss;
+        im.snaps = p->nzx;
+        im.tmpv = p->tmpv;
+        im.grad = (T*)p->gslot;
+        im.gstride = p->nzx;
+        if (visco) {
+            ViscoBack vb{p->tau_pad, p->taus_pad, p->RB0, p->RB1, (float)(1. / (double)a.dt)};
+            rc = pick([&](int W) { launch_ref<true>(W, g, b, im, vb, a.padnz, a.padnx, nb_, s); }, &p->w_refv);
+        } else {
+            rc = pick([&](int W) { launch_ref<false>(W, g, b, im, ViscoBack{}, a.padnz, a.padnx, nb_, s); }, &p->w_ref);
+        }
+        if (rc) return rc;
+    }
+    p->widths_tuned = true;
+    return PW_OK;
+}
+
+int ensure_widths(pw_fwi* p)
+{
+    if (p->widths_tuned) return PW_OK;
+    return p->prec == 4 ? tune_widths<float>(p) : tune_widths<double>(p);
+}
+
 // forward sweep over one batch: traces into rec; checkpoints (and the last segment's
 // snapshots when 'store_last') when 'ckpt'
 template <typename T>
@@ -1675,7 +1807,6 @@ int forward_batch(pw_fwi* p, int nb_, bool ckpt, int mode, bool store_last)
     const Acq& a = p->acq;
     hipStream_t s = p->ctx->s();
     FGeo<T> g = geo<T>(p);
-    const TileMap tm = tile_map(a.padnz, a.padnx, nb_);
     const long long fld = (long long)p->nslot * p->fs;
     int rc = fill<T>(p, p->X, fld, T(0));
     if (!rc) rc = fill<T>(p, p->Y, fld, T(0));
@@ -1716,8 +1847,8 @@ int forward_batch(pw_fwi* p, int nb_, bool ckpt, int mode, bool store_last)
                 else if (it % I == 0) s_.snap = (T*)p->seg + (size_t)((it - a_last) / I + 1) * p->nslot * p->nzx;
             }
         }
-        if (visco) hipLaunchKernelGGL((k_fwd<T, true>), tile_grid(tm), dim3(64), 0, s, g, s_, tm);
-        else hipLaunchKernelGGL((k_fwd<T, false>), tile_grid(tm), dim3(64), 0, s, g, s_, tm);
+        if (visco) launch_fwd<T, true>(p->w_fwdv, g, s_, a.padnz, a.padnx, nb_, s);
+        else launch_fwd<T, false>(p->w_fwd, g, s_, a.padnz, a.padnx, nb_, s);
         PW_HIPS(hipGetLastError());
         std::swap(X, Y);
         if (visco) std::swap(Ra, Rb);
@@ -1732,7 +1863,6 @@ int recompute_segment(pw_fwi* p, int nb_, int sg, int mode)
     const Acq& a = p->acq;
     hipStream_t s = p->ctx->s();
     FGeo<T> g = geo<T>(p);
-    const TileMap tm = tile_map(a.padnz, a.padnx, nb_);
     const long long fld = (long long)p->nslot * p->fs;
     const int I = a.interval, K = p->K;
     const int a0 = sg * K, b0 = std::min(a0 + K, a.nt);
@@ -1756,10 +1886,10 @@ int recompute_segment(pw_fwi* p, int nb_, int sg, int mode)
         if (visco) {
             s_.tau = p->tau_pad; s_.taus = p->taus_pad; s_.r1 = Ra; s_.r2 = Rb;
             s_.idt = (float)(1. / (double)a.dt);
-            hipLaunchKernelGGL((k_fwd<T, true>), tile_grid(tm), dim3(64), 0, s, g, s_, tm);
+            launch_fwd<T, true>(p->w_fwdv, g, s_, a.padnz, a.padnx, nb_, s);
             std::swap(Ra, Rb);
         } else {
-            hipLaunchKernelGGL((k_fwd<T, false>), tile_grid(tm), dim3(64), 0, s, g, s_, tm);
+            launch_fwd<T, false>(p->w_fwd, g, s_, a.padnz, a.padnx, nb_, s);
         }
         PW_HIPS(hipGetLastError());
         std::swap(X, Y);
@@ -1774,8 +1904,6 @@ int adjoint_segment(pw_fwi* p, int nb_, int sg, int mode, T** Ap, T** Bp, T* gra
     const Acq& a = p->acq;
     hipStream_t s = p->ctx->s();
     FGeo<T> g = geo<T>(p);
-    const TileMap tm = tile_map(a.padnz, a.padnx, nb_);
-    const TileMap tmv = tile_map(a.padnz, a.padnx, nb_, XCV);
     const long long fld = (long long)p->nslot * p->fs;
     const int I = a.interval, K = p->K;
     const int a0 = sg * K, b0 = std::min(a0 + K, a.nt);
@@ -1808,10 +1936,10 @@ int adjoint_segment(pw_fwi* p, int nb_, int sg, int mode, T** Ap, T** Bp, T* gra
             if constexpr (sizeof(T) == 4) {  // the reference mode runs in f32 (checked by the caller)
                 if (p->visco) {
                     ViscoBack vb{p->tau_pad, p->taus_pad, p->RB0, p->RB1, (float)(1. / (double)a.dt)};
-                    hipLaunchKernelGGL((k_adj_ref<true, XCV>), tile_grid(tmv), dim3(64), 0, s, g, s_, im, vb, tmv);
+                    launch_ref<true>(p->w_refv, g, s_, im, vb, a.padnz, a.padnx, nb_, s);
                     std::swap(p->RB0, p->RB1);
                 } else {
-                    hipLaunchKernelGGL((k_adj_ref<false, XC>), tile_grid(tm), dim3(64), 0, s, g, s_, im, ViscoBack{}, tm);
+                    launch_ref<false>(p->w_ref, g, s_, im, ViscoBack{}, a.padnz, a.padnx, nb_, s);
                 }
             } else {
                 return PW_ERR_ARG;
@@ -1821,7 +1949,7 @@ int adjoint_segment(pw_fwi* p, int nb_, int sg, int mode, T** Ap, T** Bp, T* gra
             ex.F = (const T*)p->seg + (size_t)(it - a0) * fld;
             ex.fsl = p->fs;
             ex.gm = (T*)p->gm;
-            hipLaunchKernelGGL(k_adj_exact<T>, tile_grid(tm), dim3(64), 0, s, g, s_, ex, tm);
+            launch_exact<T>(p->w_exact, g, s_, ex, a.padnz, a.padnx, nb_, s);
         }
         PW_HIPS(hipGetLastError());
         std::swap(A, B);
@@ -1978,6 +2106,19 @@ int finite_result(pw_fwi* p, double fc, const char* what)
         PW_FAIL(p->ctx, PW_ERR_NONFINITE,
                 std::string(what) + ": the misfit is Inf or NaN (the run blew up; is the time step within the CFL limit?)");
     return pw::check_finite(p->ctx, p->gtot, (size_t)p->nzx, (int)esz(p), what);
+}
+
+int fingerprint(pw_fwi* p, const void* d_vel, unsigned long long* out)
+{
+    hipStream_t s = p->ctx->s();
+    const unsigned blocks = (unsigned)std::min<long long>(256, (p->nzx + 255) / 256);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, out);
+    if (p->prec == 4)
+        hipLaunchKernelGGL(k_fingerprint<float>, dim3(blocks), dim3(256), 0, s, p->nzx, (const float*)d_vel, out);
+    else
+        hipLaunchKernelGGL(k_fingerprint<double>, dim3(blocks), dim3(256), 0, s, p->nzx, (const double*)d_vel, out);
+    PW_HIPS(hipGetLastError());
+    return PW_OK;
 }
 
 template <typename T>
@@ -2197,6 +2338,7 @@ namespace {
 int use_source_field(pw_fwi* p, bool on)
 {
     p->dev_ckpt_valid = false;
+    p->widths_tuned = false;  // the tiles that run the general loop change
     const Acq& a = p->acq;
     if (!on) {
         p->passive = false;
@@ -2275,6 +2417,7 @@ int pw_fwi_set_visco(pw_fwi* p, const float* q)
 {
     if (!p) return PW_ERR_ARG;
     p->dev_ckpt_valid = false;
+    p->widths_tuned = false;  // other kernels run
     if (!q) {
         p->visco = false;
         return PW_OK;
@@ -2328,6 +2471,7 @@ int pw_fwi_forward(pw_fwi* p, const float* vel, float* data_out)
     const Acq& a = p->acq;
     hipStream_t s = p->ctx->s();
     int rc = upload_vel_host(p, vel);
+    if (!rc) rc = ensure_widths(p);
     if (rc) return rc;
     const long long per = (long long)a.nt * a.nr;
     std::vector<float> t((size_t)per * p->nslot);
@@ -2377,6 +2521,7 @@ int pw_fwi_gradient(pw_fwi* p, const float* vel, int mode, float* grad, double* 
     if (!p || !vel || !grad || (mode != 0 && mode != 1)) return PW_ERR_ARG;
     hipStream_t s = p->ctx->s();
     int rc = upload_vel_host(p, vel);
+    if (!rc) rc = ensure_widths(p);
     if (rc) return rc;
     double fc = 0;
     if (p->prec == 4) rc = gradient_all<float>(p, mode, &fc);
@@ -2406,6 +2551,7 @@ int pw_fwi_gradient_f64(pw_fwi* p, const double* vel, int mode, double* grad, do
     hipStream_t s = p->ctx->s();
     PW_HIPS(hipMemcpyAsync(p->vel, vel, p->nzx * 8, hipMemcpyHostToDevice, s));
     int rc = upload_model<double>(p, nullptr, false);
+    if (!rc) rc = ensure_widths(p);
     double fc = 0;
     if (!rc) rc = gradient_all<double>(p, mode, &fc);
     if (!rc) rc = finite_result(p, fc, "fwi gradient");
@@ -2430,6 +2576,7 @@ int pw_fwi_misfit_f64(pw_fwi* p, const double* vel, double* fcost)
         std::vector<float> v(vel, vel + p->nzx);
         rc = upload_vel_host(p, v.data());
     }
+    if (!rc) rc = ensure_widths(p);
     if (rc) return rc;
     double fc = 0;
     const Acq& a = p->acq;
@@ -2455,6 +2602,7 @@ int pw_fwi_forward_dev(pw_fwi* p, const void* d_vel, void* d_traces)
     const Acq& a = p->acq;
     hipStream_t s = p->ctx->s();
     int rc = p->prec == 4 ? upload_model<float>(p, (const float*)d_vel, true) : upload_model<double>(p, (const double*)d_vel, true);
+    if (!rc) rc = ensure_widths(p);
     if (rc) return rc;
     const long long per = (long long)a.nt * a.nr;
     const size_t e = esz(p);
@@ -2468,13 +2616,7 @@ int pw_fwi_forward_dev(pw_fwi* p, const void* d_vel, void* d_traces)
         PW_HIPS(hipMemcpyAsync((char*)d_traces + (size_t)b0 * per * e, p->rec, (size_t)nb_ * per * e,
                                hipMemcpyDeviceToDevice, s));
     }
-    if (keep) {  // fingerprint of the model the checkpoints belong to
-        if (p->prec == 4)
-            hipLaunchKernelGGL(k_fingerprint<float>, dim3(1), dim3(1024), 0, s, p->nzx, (const float*)d_vel, p->dhash);
-        else
-            hipLaunchKernelGGL(k_fingerprint<double>, dim3(1), dim3(1024), 0, s, p->nzx, (const double*)d_vel, p->dhash);
-        PW_HIPS(hipGetLastError());
-    }
+    if (keep && (rc = fingerprint(p, d_vel, p->dhash))) return rc;  // of the model the checkpoints belong to
     p->dev_ckpt_valid = keep;
     p->dev_ckpt_vel = d_vel;
     return PW_OK;
@@ -2493,12 +2635,7 @@ int pw_fwi_adjoint_dev(pw_fwi* p, const void* d_vel, const void* d_gtraces, void
     // fingerprint must equal the one taken by forward_dev.
     bool reuse = p->dev_ckpt_valid && p->dev_ckpt_vel == d_vel && p->nslot >= a.ns;
     if (reuse) {
-        if (p->prec == 4)
-            hipLaunchKernelGGL(k_fingerprint<float>, dim3(1), dim3(1024), 0, s, p->nzx, (const float*)d_vel, p->dhash + 1);
-        else
-            hipLaunchKernelGGL(k_fingerprint<double>, dim3(1), dim3(1024), 0, s, p->nzx, (const double*)d_vel,
-                               p->dhash + 1);
-        PW_HIPS(hipGetLastError());
+        if (int r = fingerprint(p, d_vel, p->dhash + 1)) return r;
         unsigned long long h[2] = {0, 1};
         PW_HIPS(hipMemcpyAsync(h, p->dhash, sizeof(h), hipMemcpyDeviceToHost, s));
         PW_HIPS(hipStreamSynchronize(s));
@@ -2508,6 +2645,7 @@ int pw_fwi_adjoint_dev(pw_fwi* p, const void* d_vel, const void* d_gtraces, void
     if (!reuse)
         rc = p->prec == 4 ? upload_model<float>(p, (const float*)d_vel, true)
                           : upload_model<double>(p, (const double*)d_vel, true);
+    if (!rc && !reuse) rc = ensure_widths(p);
     if (rc) return rc;
     const size_t e = esz(p);
     const long long per = (long long)a.nt * a.nr;
@@ -2564,6 +2702,14 @@ int pw_fwi_adjoint_dev(pw_fwi* p, const void* d_vel, const void* d_gtraces, void
     return PW_OK;
 }
 
+int pw_fwi_widths(pw_fwi* p, int* w)
+{
+    if (!p || !w) return PW_ERR_ARG;
+    w[0] = p->w_fwd; w[1] = p->w_fwdv; w[2] = p->w_exact; w[3] = p->w_ref; w[4] = p->w_refv;
+    w[5] = p->widths_tuned ? 1 : 0;
+    return PW_OK;
+}
+
 int pw_fwi_timing(pw_fwi* p, double* forward_ms, double* backward_ms)
 {
     if (!p) return PW_ERR_ARG;
@@ -2595,6 +2741,7 @@ int pw_fwi_lbfgs(pw_fwi* p, const float* vel0, int niter, float* vinv, int* nsto
     rc = rc ? rc : dmalloc(p, (void**)&q, bytes);
     rc = rc ? rc : dmalloc(p, (void**)&sk, bytes * npair);
     rc = rc ? rc : dmalloc(p, (void**)&yk, bytes * npair);
+    rc = rc ? rc : ensure_widths(p);
     auto cleanup = [&]() {
         float* bs[] = {x, g, dir, xk, q, sk, yk};
         for (float* b : bs)

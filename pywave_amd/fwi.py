@@ -125,6 +125,14 @@ class FwiProblem:
         keys = ["padnz", "padnx", "batch", "checkpoint_steps", "segments", "ns", "nt", "nr"]
         return dict(zip(keys, (int(x) for x in v)))
 
+    def widths(self):
+        """Tile widths of the step kernels picked by the plan's timing sweep (its first sweep runs
+        it): forward, visco forward, exact adjoint, reference adjoint, visco reference adjoint."""
+        w = np.zeros(6, np.int32)
+        N.check(self._L.pw_fwi_widths(self._p, N.ptr(w)), self._ctx, "widths")
+        keys = ["forward", "visco_forward", "exact_adjoint", "reference_adjoint", "visco_reference_adjoint"]
+        return dict(zip(keys, (int(x) for x in w[:5])), tuned=bool(w[5]))
+
     def timing(self):
         f, b = ctypes.c_double(), ctypes.c_double()
         self._L.pw_fwi_timing(self._p, ctypes.byref(f), ctypes.byref(b))

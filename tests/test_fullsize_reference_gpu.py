@@ -17,6 +17,7 @@ import numpy as np
 import pytest
 
 from conftest import GOLDEN, rel_l2
+from oracle import oracle as O
 from conftest import load_golden as _load
 from pywave_amd import AfdPlan, Afd2dBatch, FwiProblem
 from pywave_amd import _native as N
@@ -121,11 +122,20 @@ def test_fwi2d_forward_and_gradient_against_reference():
     g1, f1, _ = p.gradient(w["v0"], "reference")
     p.close()
     e1 = rel_l2(g1, ref["grad"])
-    print("fwi2d gradient_av, one shot per launch: rel-L2 %.2e, sha256 %s, misfit %.6g (reference %.6g)" %
-          (e1, "equal" if sha(g1) == str(ref["sha256_grad"]) else "differs", f1,
-           float(ref["fcost"][0])))
+    # the reference's misfit is an f32 accumulation over 77 M samples: reproduced bit for bit by
+    # the oracle's restatement of that sum from the GPU's predicted data, while the engine's f64
+    # sum is the accurate value (about 2 % above it here)
+    gen = FwiProblem(par, wav, batch=ns)
+    pred = gen.forward(w["v0"])
+    gen.close()
+    f32 = O.fcost_f32(data, pred)
+    print("fwi2d gradient_av, one shot per launch: rel-L2 %.2e, sha256 %s; misfit f64 %.6f, reference %.6f (its f32 "
+          "accumulation restated: %.6f)" % (e1, "equal" if sha(g1) == str(ref["sha256_grad"]) else "differs", f1,
+                                            float(ref["fcost"][0]), f32))
     assert sha(g1) == str(ref["sha256_grad"])
-    assert abs(f1 - float(ref["fcost"][0])) <= 1e-3 * float(ref["fcost"][0])
+    assert np.float32(f32) == ref["fcost"][0]
+    r = (data - pred).astype(np.float64)  # the residual in f32, as the engine forms it
+    assert abs(f1 - 0.5 * float(np.sum(r * r))) <= 1e-7 * f1
     # all 32 shots in one launch, checkpoints every 50 steps
     q = FwiProblem(par, wav, data=data, batch=ns, checkpoint=50)
     g32, _, _ = q.gradient(w["v0"], "reference")

@@ -494,3 +494,16 @@ def test_visco_gradient_tends_to_acoustic_at_high_q(golden):
     v = golden("fwi_visco")
     gl, _, _ = O.gradient_av(g["par"], g["v0"], g["wav"], g["data"], q=v["q"])
     assert rel_l2(gl, ga) > 0.05
+
+
+def test_fcost_f32_reproduces_the_reference_misfit(golden):
+    # gradient_av accumulates 0.5 r^2 into an f32 scalar (pywave/src/wave_fwigradient.c:179-184);
+    # the oracle's restatement of that accumulation gives the golden misfit bit for bit, and the
+    # exact (f64) sum differs from it by the accumulator's rounding
+    g = golden("fwi_small")
+    pred = O.fwi_forward(g["par"], g["v0"], g["wav"])
+    f32 = O.fcost_f32(g["data"], pred)
+    assert np.float32(f32) == g["fcost"][0]
+    r = g["data"].astype(np.float64) - pred.astype(np.float32).astype(np.float64)
+    exact = 0.5 * float(np.sum(r * r))
+    assert abs(exact - f32) <= 1e-4 * exact
