@@ -95,6 +95,7 @@ def fwi_bench(plain_shots):
     g, f = grad_autograd()
     tb = time.perf_counter() - t
     out["batched_tile_widths"] = prob.widths()
+    out["batched_tuning_ms_per_launch"] = prob.tuning_log()
     prob.close()
     out["batched_shots_per_s"] = round(ns / tb, 3)
     out["batched_s_per_gradient"] = round(tb, 4)
@@ -131,6 +132,7 @@ def fwi_bench(plain_shots):
     par1["ns"] = plain_shots
     obs1 = np.ascontiguousarray(obs[:, :, :plain_shots])
     plain = FwiProblem(par1, wav, data=obs1, batch=1, checkpoint=50)
+    plain.set_steps_per_launch(1)  # the plain method: one kernel per time step
     plain.gradient(w["v0"], "exact")
     t = time.perf_counter()
     plain.gradient(w["v0"], "exact")

@@ -185,8 +185,18 @@ int pw_fwi_forward_dev(pw_fwi *p, const void *d_vel, void *d_traces);
 int pw_fwi_adjoint_dev(pw_fwi *p, const void *d_vel, const void *d_gtraces, void *d_gvel);
 /* tile widths (x columns per wave) of the step kernels: w[0] forward, w[1] visco-acoustic forward,
  * w[2] exact adjoint, w[3] reference adjoint, w[4] visco-acoustic reference adjoint; w[5] = 1 once
- * the plan has timed the widths on its grid and batch (its first sweep does), 0 before */
+ * the plan has timed the widths on its grid and batch (its first sweep does), 0 before; w[6] the
+ * two-step forward's width, 0 when forward sweeps run one step per launch; w[7] the same for the
+ * exact adjoint (8 ints) */
 int pw_fwi_widths(pw_fwi *p, int *w);
+/* forward time steps one launch may advance: 2 (default; the plan's timing sweep keeps two steps
+ * per launch where they are faster), 1 (one kernel per time step, the plain method) or -2 (two
+ * steps per launch whatever the timing; acoustic point-source plans) */
+int pw_fwi_set_steps_per_launch(pw_fwi *p, int n);
+/* the plan's last width-timing sweep: triples (kernel, width, ms per launch) written to out (up to
+ * cap of them); kernel 0 forward, 1 visco forward, 2 exact adjoint, 3 reference adjoint, 4 visco
+ * reference adjoint, 5 two-step forward, 6 two-step exact adjoint.  Returns the number of triples. */
+int pw_fwi_tuning_log(pw_fwi *p, float *out, int cap);
 /* event-timed milliseconds of the last gradient (forward sweeps, backward sweeps) or of the last
  * pw_fwi_forward (its sweeps, 0) */
 int pw_fwi_timing(pw_fwi *p, double *forward_ms, double *backward_ms);

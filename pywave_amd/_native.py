@@ -80,6 +80,8 @@ SIGNATURES = {
     "pw_fwi_adjoint_dev": (c_int, [c_vp, c_vp, c_vp, c_vp]),
     "pw_fwi_timing": (c_int, [c_vp, ctypes.POINTER(c_double), ctypes.POINTER(c_double)]),
     "pw_fwi_widths": (c_int, [c_vp, c_vp]),
+    "pw_fwi_set_steps_per_launch": (c_int, [c_vp, c_int]),
+    "pw_fwi_tuning_log": (c_int, [c_vp, c_vp, c_int]),
     "pw_fwi_lbfgs": (c_int, [c_vp, c_vp, c_int, c_vp, ctypes.POINTER(c_int)]),
     "pw_forward_modeling_ac": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_ll]),
     "pw_fwic": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_ll]),

@@ -414,6 +414,193 @@ __global__ __launch_bounds__(64) void k_fwd(FGeo<T> g, StepArgs<T> a, TileMap tm
     }
 }
 
+// ---------------------------------------------------------------- two forward steps per launch
+// Temporal blocking of k_fwd for batched sweeps: one launch advances (X_t, X_{t-1}) to
+// (X_{t+2}, X_{t+1}), written to two other buffers (neighbouring tiles still read X_t and X_{t-1}).
+// A tile owns 56 z rows x W columns.  Its wave's 64 lanes hold the owned rows and 4 rows on either
+// side; streaming along x, it forms step 1 (X_{t+1}) on all 64 lanes at column c, four columns
+// ahead of step 2 (X_{t+2}) at column c - 4, which takes its x neighbours from a 9-column register
+// queue of step-1 values and its z neighbours from LDS.  Step 1 therefore covers the tile grown by
+// 4 cells on every side (recomputed by the neighbouring tiles rather than exchanged).  Every cell
+// of both steps uses k_fwd's arithmetic in the same order (records before the update, the second
+// sponge of the previous level on read), so the results equal two single-step launches bit for
+// bit.  HBM traffic per two steps: X_t, X_{t-1} and v^2 dt^2 read once (plus halos), X_{t+1} and
+// X_{t+2} written once: 20 B per cell for two updates against 32 B for two single steps.
+constexpr int kOwnZ = 56;  // owned z rows of a two-step tile
+
+template <typename T>
+struct Fwd2Args {
+    const T* X;        // X_t (sponged)
+    const T* Y;        // X_{t-1} as stored
+    T* C;              // out: X_{t+1}
+    T* D;              // out: X_{t+2}
+    const T* m;        // (v*v)*dt^2 on the padded grid
+    const T* sfz;
+    const T* sfx;
+    const ShotInfo* shots;
+    const T* patch;
+    const T* wav;
+    T* rec;            // traces [slot][nt][nr] or null
+    long long trs;
+    T* snap1;          // interior snapshots of X_t and X_{t+1} [slot][nx][nz], or null
+    T* snap2;
+    long long snaps;
+    T* term1;          // the terms of steps t and t+1 on the full field [slot][fs], or null
+    T* term2;
+    long long terms;
+    int it;            // step t; the launch makes steps t and t+1
+};
+
+template <typename T, int W>
+__global__ __launch_bounds__(64) void k_fwd2(FGeo<T> g, Fwd2Args<T> a, TileMap tm)
+{
+    using C = Coef<T>;
+    int slot, tz, xc;
+    if (!tile_of(tm, slot, tz, xc)) return;
+    const int lane = threadIdx.x;
+    const int zo = tz * kOwnZ;   // owned rows [zo, zo + 56)
+    const int z0 = zo - 4;       // row of lane 0
+    const int z = z0 + lane;
+    const int x0 = xc * W;       // owned columns [x0, x0 + W) within the grid
+    __shared__ T rowA[2][128];   // X_t column with its z halo (lanes 8..63 may write 72..127)
+    __shared__ T rowB[2][72];    // X_{t+1} column, lane l at l + 4
+    const ShotInfo sh = a.shots[slot];
+    const int L = g.ldz;
+    const int bias = 4 * L + 4;
+    const unsigned wb = (unsigned)((g.fs + 2 * bias) * (long long)sizeof(T));
+    constexpr unsigned E = sizeof(T);
+    const auto rX = win(a.X + slot * g.fs - bias, wb), rY = win(a.Y + slot * g.fs - bias, wb),
+               rM = win(a.m - bias, wb), rC = win(a.C + slot * g.fs - bias, wb),
+               rD = win(a.D + slot * g.fs - bias, wb);
+    const auto rT1 = win(a.term1 ? a.term1 + slot * a.terms - bias : nullptr, a.term1 ? wb : 0u);
+    const auto rT2 = win(a.term2 ? a.term2 + slot * a.terms - bias : nullptr, a.term2 ? wb : 0u);
+    const bool zok = z >= 0 && z < g.pnz;
+    const bool zin = z >= 4 && z < g.pnz - 4;
+    const bool own = zok && lane >= 4 && lane < 4 + kOwnZ;
+    const T zl = zok ? a.sfz[2 * z] : T(1), zh = zok ? a.sfz[2 * z + 1] : T(1);
+    const int zhz = lane < 4 ? z0 - 4 + lane : z0 + 60 + lane;  // z halo of lanes 0..7
+    const int lzh = lane < 4 ? lane : 64 + lane;
+    const bool zhv = lane < 8 && zhz >= 0 && zhz < g.pnz;
+    const unsigned oz = zok ? (unsigned)(bias + z) * E : kOOB, ozh = zhv ? (unsigned)(bias + zhz) * E : kOOB;
+    const unsigned oo = own ? oz : kOOB;  // stores: owned rows only
+    const unsigned oc = (unsigned)L * E;
+    auto xv = [&](int x) { return x >= 0 && x < g.pnx; };
+    auto ld = [&](__amdgpu_buffer_rsrc_t r, unsigned voff, int x) {  // column x of a field (0 off the grid)
+        return bldo<T>(r, xv(x) ? voff : kOOB, xv(x) ? (unsigned)x * oc : 0u);
+    };
+    const T wv1 = a.wav[a.it], wv2 = a.wav[a.it + 1];
+    const bool rec_rows = a.rec && g.rz >= zo && g.rz < zo + kOwnZ;
+    const bool src_rows = sh.ph > 0 && sh.sz0 < z0 + 64 && sh.sz0 + sh.ph > z0;
+    const bool snap_rows = (a.snap1 || a.snap2) && zo < g.nb + g.nz && zo + kOwnZ > g.nb;
+    auto source = [&](int x, T wv) -> T {  // source-map term of cell (z, x)
+        const int px = x - sh.sx0, pz = z - sh.sz0;
+        return (px >= 0 && px < sh.pw && pz >= 0 && pz < sh.ph) ? a.patch[slot * PMAX * PMAX + px * PMAX + pz] * wv
+                                                                : T(0);
+    };
+    // records before the update (wave_fwigradient.c:136-139), snapshots of the interior: owned cells
+    auto record = [&](int x, int it, T p1) {
+        if (!(rec_rows && own && z == g.rz)) return;
+        T* r = a.rec + slot * a.trs + (long long)it * g.nr;
+        if (g.dr_v > 0) {
+            const int d = x - sh.r0_v;
+            if (d >= 0 && d % g.dr_v == 0 && d / g.dr_v < sh.nr2) {
+                const int k = sh.r02 + d / g.dr_v;
+                if (k >= 0 && k < g.nr) r[k] = p1;
+            }
+        } else if (x == sh.r0_v) {
+            for (int ir = 0; ir < sh.nr2; ++ir)
+                if (sh.r02 + ir >= 0 && sh.r02 + ir < g.nr) r[sh.r02 + ir] = p1;
+        }
+    };
+    auto snapshot = [&](T* snap, int x, T p1) {
+        if (snap && snap_rows && own && x >= g.nb && x < g.nb + g.nx && z >= g.nb && z < g.nb + g.nz)
+            snap[slot * a.snaps + (long long)(x - g.nb) * g.nz + (z - g.nb)] = p1;
+    };
+    // X_t queue: columns c-4 .. c+4 while step 1 runs at column c
+    T qx[9], q1[9], mq[5];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) qx[k] = ld(rX, oz, x0 - 8 + k);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) q1[k] = T(0);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) mq[k] = T(0);
+    struct Col {
+        T q, h, y, m;
+    };
+    auto fetch = [&](int c) -> Col {  // the operands step 1 needs at column c
+        Col o;
+        o.q = ld(rX, oz, c + 4);
+        o.h = ld(rX, ozh, c);
+        o.y = ld(rY, oz, c);
+        o.m = ld(rM, oz, c);
+        return o;
+    };
+    Col pend = fetch(x0 - 4);
+#pragma unroll 1
+    for (int i = 0; i < W + 8; ++i) {
+        const int c = x0 - 4 + i;  // step-1 column
+        const Col cur = pend;
+        if (i + 1 < W + 8) pend = fetch(c + 1);
+        qx[8] = cur.q;
+        mq[4] = cur.m;
+        // ---- step 1 at column c, every lane
+        rowA[i & 1][lane + 4] = qx[4];
+        rowA[i & 1][lzh] = cur.h;
+        __syncthreads();
+        const T p1 = qx[4];
+        const bool cv = xv(c);
+        const bool cown = c >= x0 && c < x0 + W && cv;
+        const T* R = &rowA[i & 1][lane + 4];
+        const T lap1 = (C::K0 * p1 + C::K1 * (qx[5] + qx[3]) + C::K2 * (qx[6] + qx[2]) + C::K3 * (qx[7] + qx[1]) +
+                        C::K4 * (qx[8] + qx[0])) / g.dx2 +
+                       (C::K0 * p1 + C::K1 * (R[1] + R[-1]) + C::K2 * (R[2] + R[-2]) + C::K3 * (R[3] + R[-3]) +
+                        C::K4 * (R[4] + R[-4])) / g.dz2;
+        T term1 = (zin && c >= 4 && c < g.pnx - 4) ? lap1 : T(0);
+        if (src_rows) term1 = term1 + source(c, wv1);
+        const T cx = cv ? a.sfx[2 * c] : T(1), cxh = cv ? a.sfx[2 * c + 1] : T(1);
+        if (cown) {
+            record(c, a.it, p1);
+            snapshot(a.snap1, c, p1);
+            if (a.term1) bsto(rT1, oo, (unsigned)c * oc, term1);
+        }
+        const T p0 = sponge(cur.y, zl, zh, cx, cxh);
+        const T p2 = 2 * p1 - p0 + cur.m * term1;
+        const T x1 = (zok && cv) ? sponge(p2, zl, zh, cx, cxh) : T(0);
+        if (cown) bsto(rC, oo, (unsigned)c * oc, x1);
+        q1[8] = x1;
+        // ---- step 2 at column s = c - 4, once step 1 has covered s - 4 .. s + 4
+        if (i >= 8) {
+            const int s = c - 4;
+            rowB[i & 1][lane + 4] = q1[4];
+            __syncthreads();
+            const T u = q1[4];
+            const bool sv = xv(s);
+            const bool sown = s < x0 + W && sv;
+            const T* R2 = &rowB[i & 1][lane + 4];
+            const T lap2 = (C::K0 * u + C::K1 * (q1[5] + q1[3]) + C::K2 * (q1[6] + q1[2]) + C::K3 * (q1[7] + q1[1]) +
+                            C::K4 * (q1[8] + q1[0])) / g.dx2 +
+                           (C::K0 * u + C::K1 * (R2[1] + R2[-1]) + C::K2 * (R2[2] + R2[-2]) +
+                            C::K3 * (R2[3] + R2[-3]) + C::K4 * (R2[4] + R2[-4])) / g.dz2;
+            T term2 = (zin && s >= 4 && s < g.pnx - 4) ? lap2 : T(0);
+            if (src_rows) term2 = term2 + source(s, wv2);
+            if (sown) {
+                record(s, a.it + 1, u);
+                snapshot(a.snap2, s, u);
+                if (a.term2) bsto(rT2, oo, (unsigned)s * oc, term2);
+            }
+            const T sx = sv ? a.sfx[2 * s] : T(1), sxh = sv ? a.sfx[2 * s + 1] : T(1);
+            const T p0b = sponge(qx[0], zl, zh, sx, sxh);  // X_t at s, its second sponge
+            const T p2b = 2 * u - p0b + mq[0] * term2;
+            if (sown) bsto(rD, oo, (unsigned)s * oc, sponge(p2b, zl, zh, sx, sxh));
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) { qx[k] = qx[k + 1]; q1[k] = q1[k + 1]; }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) mq[k] = mq[k + 1];
+        __builtin_amdgcn_sched_barrier(0);  // keep each column's loads one column ahead, no further
+    }
+}
+
 // ---------------------------------------------------------------- reference imaging-condition adjoint
 template <typename T>
 struct ImgArgs {
@@ -1011,6 +1198,256 @@ __global__ __launch_bounds__(64) void k_adj_exact(FGeo<T> g, StepArgs<T> a, ExAr
     }
 }
 
+// ---------------------------------------------------------------- exact adjoint, two steps per launch
+// Temporal blocking of k_adj_exact for batched sweeps (f32): one launch takes (q_{t+1}, q_{t+2}) to
+// (q_{t-1}, q_t), written to two other buffers, and adds both steps' dJ/dm terms in one
+// read-modify-write of the accumulator.  The tiles are k_fwd2's: 56 owned z rows x W columns;
+// step 1 (q_t) runs on all 64 lanes four columns ahead of step 2 (q_{t-1}) on the owned rows, and
+// covers the tile grown by 4 cells on every side (receiver injection included).  Every cell of
+// both steps uses k_adj_exact's operations in the same order.  HBM traffic per two steps: q_{t+1},
+// q_{t+2}, m, the two forward terms and dJ/dm read, q_t, q_{t-1} and dJ/dm written: 36 B per
+// cell against 56 B for two single steps.
+template <typename T>
+struct Adj2Args {
+    const T* A;        // q_{t+1}
+    const T* B;        // q_{t+2}
+    T* C;              // out: q_t
+    T* D;              // out: q_{t-1}
+    const T* m;
+    const T* sfz;
+    const T* sfx;
+    const ShotInfo* shots;
+    const T* inj;      // residual traces [slot][nt][nr]
+    long long trs;
+    const T* F1;       // the forward terms of steps t and t-1 on the full field [slot][fs]
+    const T* F2;
+    long long fsl;
+    T* gm;             // dJ/dm accumulator [slot][fs]
+    int it;            // step t; the launch makes steps t and t-1
+};
+
+template <typename T, int W>
+__global__ __launch_bounds__(64) void k_adj2(FGeo<T> g, Adj2Args<T> a, TileMap tm)
+{
+#pragma clang fp contract(fast)
+    using C = Coef<T>;
+    int slot, tz, xc;
+    if (!tile_of(tm, slot, tz, xc)) return;
+    const int lane = threadIdx.x;
+    const int zo = tz * kOwnZ;
+    const int z0 = zo - 4;
+    const int z = z0 + lane;
+    const int x0 = xc * W;
+    __shared__ T rowP[2][128];  // m q_{t+1} column with its z halo (step 1)
+    __shared__ T rowQ[2][72];   // m q_t column, lane l at l + 4 (step 2)
+    const ShotInfo sh = a.shots[slot];
+    const int L = g.ldz;
+    const int bias = 4 * L + 4;
+    const unsigned wb = (unsigned)((g.fs + 2 * bias) * (long long)sizeof(T));
+    constexpr unsigned E = sizeof(T);
+    const auto rA = win(a.A + slot * g.fs - bias, wb), rB = win(a.B + slot * g.fs - bias, wb),
+               rM = win(a.m - bias, wb), rC = win(a.C + slot * g.fs - bias, wb),
+               rD = win(a.D + slot * g.fs - bias, wb), rF1 = win(a.F1 + slot * a.fsl - bias, wb),
+               rF2 = win(a.F2 + slot * a.fsl - bias, wb), rG = win(a.gm + slot * g.fs - bias, wb);
+    const T rdx = T(1) / g.dx2, rdz = T(1) / g.dz2;
+    const bool zok = z >= 0 && z < g.pnz;
+    const bool zin = z >= 4 && z < g.pnz - 4;
+    const bool own = zok && lane >= 4 && lane < 4 + kOwnZ;
+    const T zl = zok ? a.sfz[2 * z] : T(1), zh = zok ? a.sfz[2 * z + 1] : T(1);
+    const int zhz = lane < 4 ? z0 - 4 + lane : z0 + 60 + lane;  // z halo of lanes 0..7
+    const int lzh = lane < 4 ? lane : 64 + lane;
+    const bool zhv = lane < 8 && zhz >= 0 && zhz < g.pnz;
+    const bool zhin = zhv && zhz >= 4 && zhz < g.pnz - 4;
+    const unsigned oz = zok ? (unsigned)(bias + z) * E : kOOB;
+    const unsigned ozi = zin ? oz : kOOB;
+    const unsigned ozhi = zhin ? (unsigned)(bias + zhz) * E : kOOB;
+    const unsigned oo = own ? oz : kOOB;
+    const unsigned oc = (unsigned)L * E;
+    auto xv = [&](int x) { return x >= 0 && x < g.pnx; };
+    auto xin = [&](int x) { return x >= 4 && x < g.pnx - 4; };
+    auto ld = [&](__amdgpu_buffer_rsrc_t r, unsigned voff, int x, bool ok) {
+        return bldo<T>(r, ok ? voff : kOOB, ok ? (unsigned)x * oc : 0u);
+    };
+    auto lap = [&](const T* q, const T* R) {
+        return (C::K0 * q[4] + C::K1 * (q[5] + q[3]) + C::K2 * (q[6] + q[2]) + C::K3 * (q[7] + q[1]) +
+                C::K4 * (q[8] + q[0])) * rdx +
+               (C::K0 * R[0] + C::K1 * (R[1] + R[-1]) + C::K2 * (R[2] + R[-2]) + C::K3 * (R[3] + R[-3]) +
+                C::K4 * (R[4] + R[-4])) * rdz;
+    };
+    // R^T g at lane z == rz of column x for step it (0 elsewhere)
+    const bool rlane = z == g.rz;
+    auto inj = [&](int x, int it) -> T {
+        if (!rlane || !xv(x)) return T(0);
+        const T* r = a.inj + slot * a.trs + (long long)it * g.nr;
+        T v = T(0);
+        if (g.dr_v > 0) {
+            const int d = x - sh.r0_v;
+            if (d >= 0 && d % g.dr_v == 0 && d / g.dr_v < sh.nr2) {
+                const int k = sh.r02 + d / g.dr_v;
+                if (k >= 0 && k < g.nr) v = v + r[k];
+            }
+        } else if (x == sh.r0_v) {
+            for (int ir = 0; ir < sh.nr2; ++ir)
+                if (sh.r02 + ir >= 0 && sh.r02 + ir < g.nr) v = v + r[sh.r02 + ir];
+        }
+        return v;
+    };
+    // Lean tiles: every column the step-1 stencil reaches inside [4, pnx-4) and receivers (if the
+    // window holds their row) at every grid point: unmasked fetches, LDS sponge pairs, the receiver
+    // samples through masked buffer loads, no scalar branches in the unrolled loop
+    const bool rwin = g.rz >= z0 && g.rz < z0 + 64;
+    if (x0 >= 12 && x0 + W + 12 <= g.pnx && (!rwin || g.dr_v == 1)) {
+        __shared__ T sxp[2 * (W + 8)];  // sponge pairs of columns x0-4 .. x0+W+3
+        if (lane < 2 * (W + 8) / 2) {
+            sxp[2 * lane] = a.sfx[2 * (x0 - 4 + lane)];
+            sxp[2 * lane + 1] = a.sfx[2 * (x0 - 4 + lane) + 1];
+        }
+        const unsigned orz = rlane ? 0u : kOOB;
+        const auto rI1 = win(a.inj + slot * a.trs + (long long)a.it * g.nr, (unsigned)(g.nr * E));
+        const auto rI2 = win(a.inj + slot * a.trs + (long long)(a.it - 1) * g.nr, (unsigned)(g.nr * E));
+        auto src = [&](__amdgpu_buffer_rsrc_t r, int x) -> T {  // receiver k = r02 + (x - r0_v)
+            const int d = x - sh.r0_v, k = sh.r02 + d;
+            const bool ok = rwin && d >= 0 && d < sh.nr2 && k >= 0 && k < g.nr;
+            return T(0) + bldo<T>(r, ok ? orz : kOOB, ok ? (unsigned)k * E : 0u);
+        };
+        T ra[9], mq[9], P[9], Q[9], qt[5];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const unsigned so = (unsigned)(x0 - 8 + k) * oc;
+            ra[k] = bldo<T>(rA, oz, so);
+            mq[k] = bldo<T>(rM, oz, so);
+            P[k] = keep_if(mq[k] * ra[k], zin);
+        }
+#pragma unroll
+        for (int k = 0; k < 9; ++k) Q[k] = T(0);
+#pragma unroll
+        for (int k = 0; k < 5; ++k) qt[k] = T(0);
+        struct LCol {
+            T a4, m4, b, ha, i1;
+        };
+        auto get = [&](int c) -> LCol {
+            LCol o;
+            const unsigned so = (unsigned)c * oc, s4 = (unsigned)(c + 4) * oc;
+            o.a4 = bldo<T>(rA, oz, s4);
+            o.m4 = bldo<T>(rM, oz, s4);
+            o.b = bldo<T>(rB, oz, so);
+            o.ha = bldo<T>(rM, ozhi, so) * bldo<T>(rA, ozhi, so);
+            o.i1 = src(rI1, c);
+            return o;
+        };
+        __syncthreads();
+        LCol pend = get(x0 - 4);
+#pragma unroll 1
+        for (int i = 0; i < W + 8; ++i) {
+            const int c = x0 - 4 + i;
+            const LCol cur = pend;
+            if (i + 1 < W + 8) pend = get(c + 1);
+            ra[8] = cur.a4;
+            mq[8] = cur.m4;
+            P[8] = keep_if(cur.m4 * cur.a4, zin);
+            rowP[i & 1][lane + 4] = P[4];
+            rowP[i & 1][lzh] = cur.ha;
+            __syncthreads();
+            const T cx = sxp[2 * i], cxh = sxp[2 * i + 1];
+            const T lt = lap(P, &rowP[i & 1][lane + 4]);
+            const T q1 = 2 * ra[4] - sponge(cur.b, zl, zh, cx, cxh) + lt + cur.i1;
+            const T x1 = keep_if(sponge(q1, zl, zh, cx, cxh), zok);
+            if (i >= 4 && i < W + 4) bsto(rC, oo, (unsigned)c * oc, x1);
+            qt[4] = x1;
+            Q[8] = keep_if(mq[4] * x1, zin);
+            if (i >= 8) {
+                const int s = c - 4;
+                rowQ[i & 1][lane + 4] = Q[4];
+                __syncthreads();
+                const T sx = sxp[2 * (i - 4)], sxh = sxp[2 * (i - 4) + 1];
+                const T lt2 = lap(Q, &rowQ[i & 1][lane + 4]);
+                const T q2 = 2 * qt[0] - sponge(ra[0], zl, zh, sx, sxh) + lt2 + src(rI2, s);
+                const unsigned so = (unsigned)s * oc;
+                bsto(rD, oo, so, sponge(q2, zl, zh, sx, sxh));
+                const T gv = bldo<T>(rG, oo, so), f1 = bldo<T>(rF1, oo, so), f2 = bldo<T>(rF2, oo, so);
+                const T g1 = gv + f1 * ra[0];
+                bsto(rG, oo, so, g1 + f2 * qt[0]);
+            }
+#pragma unroll
+            for (int k = 0; k < 8; ++k) { ra[k] = ra[k + 1]; mq[k] = mq[k + 1]; P[k] = P[k + 1]; Q[k] = Q[k + 1]; }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) qt[k] = qt[k + 1];
+            __builtin_amdgcn_sched_barrier(0);  // keep each column's loads one column ahead, no further
+        }
+        return;
+    }
+    // queues while step 1 runs at column c: raw q_{t+1} (ra), m (mq) and the masked product m q_{t+1}
+    // (P) at c-4 .. c+4; the masked product m q_t (Q) at c-8 .. c and raw q_t (qt) at c-4 .. c
+    T ra[9], mq[9], P[9], Q[9], qt[5];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int x = x0 - 8 + k;
+        ra[k] = ld(rA, oz, x, xv(x));
+        mq[k] = ld(rM, oz, x, xv(x));
+        P[k] = (zin && xin(x)) ? mq[k] * ra[k] : T(0);
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Q[k] = T(0);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) qt[k] = T(0);
+    struct Col {
+        T a4, m4, b, ha;
+    };
+    auto fetch = [&](int c) -> Col {
+        Col o;
+        o.a4 = ld(rA, oz, c + 4, xv(c + 4));
+        o.m4 = ld(rM, oz, c + 4, xv(c + 4));
+        o.b = ld(rB, oz, c, xv(c));
+        o.ha = ld(rM, ozhi, c, xin(c)) * ld(rA, ozhi, c, xin(c));
+        return o;
+    };
+    Col pend = fetch(x0 - 4);
+#pragma unroll 1
+    for (int i = 0; i < W + 8; ++i) {
+        const int c = x0 - 4 + i;
+        const Col cur = pend;
+        if (i + 1 < W + 8) pend = fetch(c + 1);
+        ra[8] = cur.a4;
+        mq[8] = cur.m4;
+        P[8] = (zin && xin(c + 4)) ? cur.m4 * cur.a4 : T(0);
+        // ---- step 1 (q_t) at column c, every lane
+        rowP[i & 1][lane + 4] = P[4];
+        rowP[i & 1][lzh] = cur.ha;
+        __syncthreads();
+        const bool cv = xv(c);
+        const T cx = cv ? a.sfx[2 * c] : T(1), cxh = cv ? a.sfx[2 * c + 1] : T(1);
+        const T lt = lap(P, &rowP[i & 1][lane + 4]);
+        const T q1 = 2 * ra[4] - sponge(cur.b, zl, zh, cx, cxh) + lt + inj(c, a.it);
+        const T x1 = (zok && cv) ? sponge(q1, zl, zh, cx, cxh) : T(0);
+        const bool cown = c >= x0 && c < x0 + W && cv;
+        if (cown) bsto(rC, oo, (unsigned)c * oc, x1);
+        qt[4] = x1;
+        Q[8] = (zin && xin(c)) ? mq[4] * x1 : T(0);
+        // ---- step 2 (q_{t-1}) at column s = c - 4 on the owned rows, with both dJ/dm terms
+        if (i >= 8) {
+            const int s = c - 4;
+            rowQ[i & 1][lane + 4] = Q[4];
+            __syncthreads();
+            const bool sv = xv(s);
+            const T sx = sv ? a.sfx[2 * s] : T(1), sxh = sv ? a.sfx[2 * s + 1] : T(1);
+            const T lt2 = lap(Q, &rowQ[i & 1][lane + 4]);
+            const T q2 = 2 * qt[0] - sponge(ra[0], zl, zh, sx, sxh) + lt2 + inj(s, a.it - 1);
+            if (sv) {
+                const unsigned so = (unsigned)s * oc;
+                bsto(rD, oo, so, sponge(q2, zl, zh, sx, sxh));
+                const T gv = bldo<T>(rG, oo, so), f1 = bldo<T>(rF1, oo, so), f2 = bldo<T>(rF2, oo, so);
+                const T g1 = gv + f1 * ra[0];
+                bsto(rG, oo, so, g1 + f2 * qt[0]);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) { ra[k] = ra[k + 1]; mq[k] = mq[k + 1]; P[k] = P[k + 1]; Q[k] = Q[k + 1]; }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) qt[k] = qt[k + 1];
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
 // ---------------------------------------------------------------- small kernels
 template <typename T>
 __global__ void k_fill(T* p, long long n, T v)
@@ -1467,7 +1904,15 @@ struct pw_fwi {
     // visco reference adjoint), picked by tune_widths for the plan's grid and batch
     int w_fwd = kDefaultWidth, w_fwdv = kDefaultWidth, w_exact = kDefaultWidth, w_ref = kDefaultWidth,
         w_refv = kDefaultWidth;
+    int w_fwd2 = 0;       // two-step forward tiles' width (0: single steps are faster or not allowed)
+    int w_adj2 = 0;       // two-step exact adjoint's width (0: single steps)
+    int max_fused = 2;    // forward steps one launch may advance (1: one kernel per time step)
+    bool force_fused = false;  // two steps per launch whether or not they time faster
+    std::vector<float> tune_log;  // (kernel, width, ms per launch) of the last timing sweep
     bool widths_tuned = false;
+    void *X2 = nullptr, *Y2 = nullptr;  // the two-step forward's output fields (allocated on first use)
+    void *A2 = nullptr, *B2 = nullptr;  // the two-step adjoint's output fields (allocated on first use)
+    void *adj_sa = nullptr, *adj_sb = nullptr;  // the adjoint fields free during a backward sweep
     double t_forward_ms = 0, t_backward_ms = 0;
 };
 
@@ -1513,7 +1958,8 @@ void fwi_free(pw_fwi* p)
     void* bufs[] = {p->vel, p->vpad, p->m, p->sfz, p->sfx, p->wav, p->X, p->Y, p->A, p->B, p->rec, p->res,
                     p->obs, p->W, p->ckpt, p->ckpt_snap, p->seg, p->gm, p->gslot, p->gtot, p->patch,
                     p->tmpv, p->work, p->shots, p->slot_shot, p->partial, p->dscalar, p->gfold, p->fsrc,
-                    p->tau_pad, p->taus_pad, p->R0, p->R1, p->RB0, p->RB1, p->ckpt_r, p->dhash};
+                    p->tau_pad, p->taus_pad, p->R0, p->R1, p->RB0, p->RB1, p->ckpt_r, p->dhash, p->X2, p->Y2,
+                    p->A2, p->B2};
     for (void* b : bufs)
         if (b) hipFree(b);
 }
@@ -1673,6 +2119,30 @@ void launch_fwd(int W, const FGeo<T>& g, const StepArgs<T>& a, int pnz, int pnx,
     else hipLaunchKernelGGL((k_fwd<T, VISCO, 8>), tile_grid(tm), dim3(64), 0, s, g, a, tm);
 }
 
+// two-step forward tiles: 56 owned rows x W columns
+template <typename T>
+void launch_fwd2(int W, const FGeo<T>& g, const Fwd2Args<T>& a, int pnz, int pnx, int nb_, hipStream_t s)
+{
+    TileMap tm;
+    tm.ntz = (pnz + kOwnZ - 1) / kOwnZ;
+    tm.nxc = (pnx + W - 1) / W;
+    tm.nblk = tm.ntz * tm.nxc * nb_;
+    if (W == 8) hipLaunchKernelGGL((k_fwd2<T, 8>), tile_grid(tm), dim3(64), 0, s, g, a, tm);
+    else if (W == 32) hipLaunchKernelGGL((k_fwd2<T, 32>), tile_grid(tm), dim3(64), 0, s, g, a, tm);
+    else hipLaunchKernelGGL((k_fwd2<T, 16>), tile_grid(tm), dim3(64), 0, s, g, a, tm);
+}
+
+void launch_adj2(int W, const FGeo<float>& g, const Adj2Args<float>& a, int pnz, int pnx, int nb_, hipStream_t s)
+{
+    TileMap tm;
+    tm.ntz = (pnz + kOwnZ - 1) / kOwnZ;
+    tm.nxc = (pnx + W - 1) / W;
+    tm.nblk = tm.ntz * tm.nxc * nb_;
+    if (W == 8) hipLaunchKernelGGL((k_adj2<float, 8>), tile_grid(tm), dim3(64), 0, s, g, a, tm);
+    else if (W == 32) hipLaunchKernelGGL((k_adj2<float, 32>), tile_grid(tm), dim3(64), 0, s, g, a, tm);
+    else hipLaunchKernelGGL((k_adj2<float, 16>), tile_grid(tm), dim3(64), 0, s, g, a, tm);
+}
+
 template <typename T>
 void launch_exact(int W, const FGeo<T>& g, const StepArgs<T>& a, const ExArgs<T>& ex, int pnz, int pnx, int nb_,
                   hipStream_t s)
@@ -1709,6 +2179,49 @@ struct EventSet {
     }
 };
 
+// whether forward sweeps advance two steps per launch (acoustic point-source plans whose tuner
+// found the two-step kernel faster, unless limited to one kernel per time step)
+bool fused2(const pw_fwi* p) { return p->max_fused >= 2 && p->w_fwd2 > 0 && !p->visco && !p->passive && p->X2; }
+
+// whether exact-adjoint sweeps advance two steps per launch (f32 plans)
+bool fused2_adj(const pw_fwi* p) { return p->max_fused >= 2 && p->w_adj2 > 0 && p->prec == 4 && p->A2; }
+
+Adj2Args<float> adj2_args(pw_fwi* p, const float* A, const float* B, float* C, float* D, const float* F1,
+                          const float* F2, int it)
+{
+    Adj2Args<float> r{};
+    r.A = A; r.B = B; r.C = C; r.D = D;
+    r.m = (const float*)p->m;
+    r.sfz = (const float*)p->sfz;
+    r.sfx = (const float*)p->sfx;
+    r.shots = p->shots;
+    r.inj = (const float*)p->res;
+    r.trs = (long long)p->acq.nt * p->acq.nr;
+    r.F1 = F1; r.F2 = F2;
+    r.fsl = p->fs;
+    r.gm = (float*)p->gm;
+    r.it = it;
+    return r;
+}
+
+template <typename T>
+Fwd2Args<T> fwd2_args(pw_fwi* p, const T* X, const T* Y, T* C, T* D, int it)
+{
+    Fwd2Args<T> f{};
+    f.X = X; f.Y = Y; f.C = C; f.D = D;
+    f.m = (const T*)p->m;
+    f.sfz = (const T*)p->sfz;
+    f.sfx = (const T*)p->sfx;
+    f.shots = p->shots;
+    f.patch = (const T*)p->patch;
+    f.wav = (const T*)p->wav;
+    f.trs = (long long)p->acq.nt * p->acq.nr;
+    f.snaps = p->nzx;
+    f.terms = p->fs;
+    f.it = it;
+    return f;
+}
+
 // Milliseconds per launch of fn (two warm-up launches, then `reps` timed ones).
 template <typename F>
 int time_launches(pw_fwi* p, F fn, int reps, float* ms)
@@ -1742,11 +2255,20 @@ int tune_widths(pw_fwi* p)
     if (rc) return rc;
     const FGeo<T> g = geo<T>(p);
     const int reps = 8;
+    p->tune_log.clear();
+    auto note = [&](int kind, int W, float ms) {
+        p->tune_log.push_back((float)kind);
+        p->tune_log.push_back((float)W);
+        p->tune_log.push_back(ms);
+    };
+    int kind = 0;  // 0 forward, 1 visco forward, 2 exact adjoint, 3 reference adjoint, 4 visco reference adjoint,
+                   // 5 two-step forward, 6 two-step exact adjoint
     auto pick = [&](auto launch, int* best) -> int {
         float tbest = 0.f;
         for (int W : kWidths) {
             float ms = 0.f;
             if (int r = time_launches(p, [&] { launch(W); }, reps, &ms)) return r;
+            note(kind, W, ms);
             if (W == kWidths[0] || ms < tbest) { tbest = ms; *best = W; }
         }
         return PW_OK;
@@ -1759,19 +2281,74 @@ int tune_widths(pw_fwi* p)
     if (visco) {
         f.tau = p->tau_pad; f.taus = p->taus_pad; f.r1 = p->R0; f.r2 = p->R1;
         f.idt = (float)(1. / (double)a.dt);
+        kind = 1;
         rc = pick([&](int W) { launch_fwd<T, true>(W, g, f, a.padnz, a.padnx, nb_, s); }, &p->w_fwdv);
     } else {
+        kind = 0;
         rc = pick([&](int W) { launch_fwd<T, false>(W, g, f, a.padnz, a.padnx, nb_, s); }, &p->w_fwd);
     }
     if (rc) return rc;
+    // two steps per launch: kept when a launch takes less than two single steps
+    p->w_fwd2 = 0;
+    if (p->max_fused >= 2 && !visco && !p->passive && a.nt >= 2) {
+        const long long fld = (long long)p->nslot * p->fs;
+        if (!p->X2 && (rc = dmalloc(p, &p->X2, fld * sizeof(T)))) return rc;
+        if (!p->Y2 && (rc = dmalloc(p, &p->Y2, fld * sizeof(T)))) return rc;
+        float t1 = 0.f;
+        const int w1 = p->w_fwd;
+        if ((rc = time_launches(p, [&] { launch_fwd<T, false>(w1, g, f, a.padnz, a.padnx, nb_, s); }, reps, &t1)))
+            return rc;
+        Fwd2Args<T> f2 = fwd2_args<T>(p, (const T*)p->X, (const T*)p->Y, (T*)p->X2, (T*)p->Y2, 0);
+        f2.rec = (T*)p->rec;
+        float tbest = 0.f;
+        int wbest = 0;
+        for (int W : {8, 16, 32}) {
+            float ms = 0.f;
+            if ((rc = time_launches(p, [&] { launch_fwd2<T>(W, g, f2, a.padnz, a.padnx, nb_, s); }, reps, &ms)))
+                return rc;
+            note(5, W, ms);
+            if (!wbest || ms < tbest) { tbest = ms; wbest = W; }
+        }
+        if (tbest < 2.f * t1 || p->force_fused) p->w_fwd2 = wbest;
+    }
     StepArgs<T> b = base_args<T>(p);
     b.X = (const T*)p->A; b.Y = (T*)p->B; b.it = a.nt - 1;
     b.fsrc = step_src<T>(p, a.nt - 1);
     b.inj = (const T*)p->res;
     if (!visco) {
         ExArgs<T> ex{(const T*)p->seg, p->fs, (T*)p->gm};
+        kind = 2;
         rc = pick([&](int W) { launch_exact<T>(W, g, b, ex, a.padnz, a.padnx, nb_, s); }, &p->w_exact);
         if (rc) return rc;
+        // two adjoint steps per launch (f32): kept when a launch takes less than two single steps
+        p->w_adj2 = 0;
+        if constexpr (sizeof(T) == 4) {
+            if (p->max_fused >= 2 && a.nt >= 2) {
+                const long long fld = (long long)p->nslot * p->fs;
+                if (!p->A2 && (rc = dmalloc(p, &p->A2, fld * 4))) return rc;
+                if (!p->B2 && (rc = dmalloc(p, &p->B2, fld * 4))) return rc;
+                float t1 = 0.f;
+                const int w1 = p->w_exact;
+                if ((rc = time_launches(p, [&] { launch_exact<T>(w1, g, b, ex, a.padnz, a.padnx, nb_, s); }, reps,
+                                        &t1)))
+                    return rc;
+                // the two terms: the first two fields of the segment buffer (it holds K >= 2 of them)
+                const float* F = (const float*)p->seg;
+                const bool two_terms = p->seg_cap >= 2 * fld;
+                Adj2Args<float> r = adj2_args(p, (const float*)p->A, (const float*)p->B, (float*)p->A2,
+                                              (float*)p->B2, F, two_terms ? F + fld : F, a.nt - 1);
+                float tbest = 0.f;
+                int wbest = 0;
+                for (int W : {8, 16, 32}) {
+                    float ms = 0.f;
+                    if ((rc = time_launches(p, [&] { launch_adj2(W, g, r, a.padnz, a.padnx, nb_, s); }, reps, &ms)))
+                        return rc;
+                    note(6, W, ms);
+                    if (!wbest || ms < tbest) { tbest = ms; wbest = W; }
+                }
+                if (tbest < 2.f * t1 || p->force_fused) p->w_adj2 = wbest;
+            }
+        }
     }
     if constexpr (sizeof(T) == 4) {
         const long long ss = (long long)p->nslot * p->nzx;
@@ -1783,8 +2360,10 @@ int tune_widths(pw_fwi* p)
         im.gstride = p->nzx;
         if (visco) {
             ViscoBack vb{p->tau_pad, p->taus_pad, p->RB0, p->RB1, (float)(1. / (double)a.dt)};
+            kind = 4;
             rc = pick([&](int W) { launch_ref<true>(W, g, b, im, vb, a.padnz, a.padnx, nb_, s); }, &p->w_refv);
         } else {
+            kind = 3;
             rc = pick([&](int W) { launch_ref<false>(W, g, b, im, ViscoBack{}, a.padnz, a.padnx, nb_, s); }, &p->w_ref);
         }
         if (rc) return rc;
@@ -1822,36 +2401,58 @@ int forward_batch(pw_fwi* p, int nb_, bool ckpt, int mode, bool store_last)
     }
     const int I = a.interval, K = p->K;
     const int last = p->nseg - 1, a_last = last * K;
-    for (int it = 0; it < a.nt; ++it) {
+    T* Sa = (T*)p->X2;  // the two-step launches' output fields
+    T* Sb = (T*)p->Y2;
+    // what step it stores besides its level: an interior snapshot, or its term (exact adjoint)
+    auto io = [&](int it, T** snap, T** full) {
+        *snap = nullptr;
+        *full = nullptr;
+        if (!ckpt) return;
+        if (mode == 0 && (it + I) % K == 0 && it + I < a.nt)  // snapshot just before the next segment
+            *snap = (T*)p->ckpt_snap + (size_t)((it + I) / K) * p->nslot * p->nzx;
+        if (store_last && it >= a_last) {
+            if (mode == 1) *full = (T*)p->seg + (size_t)(it - a_last) * fld;
+            else if (it % I == 0) *snap = (T*)p->seg + (size_t)((it - a_last) / I + 1) * p->nslot * p->nzx;
+        }
+    };
+    for (int it = 0; it < a.nt;) {
+        if (ckpt && it % K == 0) {
+            const int sgi = it / K;
+            T* c = (T*)p->ckpt + (size_t)sgi * 2 * fld;
+            PW_HIPS(hipMemcpyAsync(c, X, fld * sizeof(T), hipMemcpyDeviceToDevice, s));
+            PW_HIPS(hipMemcpyAsync(c + fld, Y, fld * sizeof(T), hipMemcpyDeviceToDevice, s));
+            if (visco)  // the memory variable of the same level as X
+                PW_HIPS(hipMemcpyAsync(p->ckpt_r + (size_t)sgi * fld, Ra, fld * 4, hipMemcpyDeviceToDevice, s));
+        }
+        // two steps in one launch unless a checkpoint falls between them
+        if (fused2(p) && it + 1 < a.nt && !(ckpt && (it + 1) % K == 0)) {
+            Fwd2Args<T> f = fwd2_args<T>(p, X, Y, Sa, Sb, it);
+            f.rec = (T*)p->rec;
+            io(it, &f.snap1, &f.term1);
+            io(it + 1, &f.snap2, &f.term2);
+            launch_fwd2<T>(p->w_fwd2, g, f, a.padnz, a.padnx, nb_, s);
+            PW_HIPS(hipGetLastError());
+            T* oX = X;
+            T* oY = Y;
+            X = Sb; Y = Sa; Sa = oX; Sb = oY;
+            it += 2;
+            continue;
+        }
         StepArgs<T> s_ = base_args<T>(p);
         s_.X = X; s_.Y = Y; s_.it = it;
         s_.fsrc = step_src<T>(p, it);
         s_.rec = (T*)p->rec;
+        io(it, &s_.snap, &s_.full);
         if (visco) {
             s_.tau = p->tau_pad; s_.taus = p->taus_pad; s_.r1 = Ra; s_.r2 = Rb;
             s_.idt = (float)(1. / (double)p->acq.dt);
-        }
-        if (ckpt) {
-            const int sgi = it / K;
-            if (it % K == 0) {
-                T* c = (T*)p->ckpt + (size_t)sgi * 2 * fld;
-                PW_HIPS(hipMemcpyAsync(c, X, fld * sizeof(T), hipMemcpyDeviceToDevice, s));
-                PW_HIPS(hipMemcpyAsync(c + fld, Y, fld * sizeof(T), hipMemcpyDeviceToDevice, s));
-                if (visco)  // the memory variable of the same level as X
-                    PW_HIPS(hipMemcpyAsync(p->ckpt_r + (size_t)sgi * fld, Ra, fld * 4, hipMemcpyDeviceToDevice, s));
-            }
-            if (mode == 0 && (it + I) % K == 0 && it + I < a.nt)  // snapshot just before the next segment
-                s_.snap = (T*)p->ckpt_snap + (size_t)((it + I) / K) * p->nslot * p->nzx;
-            if (store_last && it >= a_last) {
-                if (mode == 1) s_.full = (T*)p->seg + (size_t)(it - a_last) * fld;
-                else if (it % I == 0) s_.snap = (T*)p->seg + (size_t)((it - a_last) / I + 1) * p->nslot * p->nzx;
-            }
         }
         if (visco) launch_fwd<T, true>(p->w_fwdv, g, s_, a.padnz, a.padnx, nb_, s);
         else launch_fwd<T, false>(p->w_fwd, g, s_, a.padnz, a.padnx, nb_, s);
         PW_HIPS(hipGetLastError());
         std::swap(X, Y);
         if (visco) std::swap(Ra, Rb);
+        ++it;
     }
     return PW_OK;
 }
@@ -1877,12 +2478,31 @@ int recompute_segment(pw_fwi* p, int nb_, int sg, int mode)
     if (visco) PW_HIPS(hipMemcpyAsync(Ra, p->ckpt_r + (size_t)sg * fld, fld * 4, hipMemcpyDeviceToDevice, s));
     // imaging needs snapshots up to b0 (inclusive) for segments that are not the last
     const int t_end = (mode == 0 && b0 < a.nt) ? b0 + 1 : b0;
-    for (int it = a0; it < t_end; ++it) {
+    T* Sa = (T*)p->X2;
+    T* Sb = (T*)p->Y2;
+    auto io = [&](int it, T** snap, T** full) {
+        *snap = nullptr;
+        *full = nullptr;
+        if (mode == 1) *full = (T*)p->seg + (size_t)(it - a0) * fld;
+        else if (it % I == 0) *snap = (T*)p->seg + (size_t)((it - a0) / I + 1) * p->nslot * p->nzx;
+    };
+    for (int it = a0; it < t_end;) {
+        if (fused2(p) && it + 1 < t_end) {
+            Fwd2Args<T> f = fwd2_args<T>(p, X, Y, Sa, Sb, it);
+            io(it, &f.snap1, &f.term1);
+            io(it + 1, &f.snap2, &f.term2);
+            launch_fwd2<T>(p->w_fwd2, g, f, a.padnz, a.padnx, nb_, s);
+            PW_HIPS(hipGetLastError());
+            T* oX = X;
+            T* oY = Y;
+            X = Sb; Y = Sa; Sa = oX; Sb = oY;
+            it += 2;
+            continue;
+        }
         StepArgs<T> s_ = base_args<T>(p);
         s_.X = X; s_.Y = Y; s_.it = it;
         s_.fsrc = step_src<T>(p, it);
-        if (mode == 1) s_.full = (T*)p->seg + (size_t)(it - a0) * fld;
-        else if (it % I == 0) s_.snap = (T*)p->seg + (size_t)((it - a0) / I + 1) * p->nslot * p->nzx;
+        io(it, &s_.snap, &s_.full);
         if (visco) {
             s_.tau = p->tau_pad; s_.taus = p->taus_pad; s_.r1 = Ra; s_.r2 = Rb;
             s_.idt = (float)(1. / (double)a.dt);
@@ -1893,7 +2513,52 @@ int recompute_segment(pw_fwi* p, int nb_, int sg, int mode)
         }
         PW_HIPS(hipGetLastError());
         std::swap(X, Y);
+        ++it;
     }
+    return PW_OK;
+}
+
+// exact-adjoint sweep of segment sg with two steps per launch (an odd step runs alone); the two
+// free adjoint fields p->adj_sa, p->adj_sb rotate with A and B
+int adjoint_segment2(pw_fwi* p, int nb_, int sg, float** Ap, float** Bp)
+{
+    const Acq& a = p->acq;
+    hipStream_t s = p->ctx->s();
+    FGeo<float> g = geo<float>(p);
+    const long long fld = (long long)p->nslot * p->fs;
+    const int K = p->K;
+    const int a0 = sg * K, b0 = std::min(a0 + K, a.nt);
+    float* A = *Ap;
+    float* B = *Bp;
+    float* Sa = (float*)p->adj_sa;
+    float* Sb = (float*)p->adj_sb;
+    const float* seg = (const float*)p->seg;
+    for (int it = b0 - 1; it >= a0;) {
+        if (it - 1 >= a0) {
+            Adj2Args<float> r = adj2_args(p, A, B, Sa, Sb, seg + (size_t)(it - a0) * fld,
+                                          seg + (size_t)(it - 1 - a0) * fld, it);
+            launch_adj2(p->w_adj2, g, r, a.padnz, a.padnx, nb_, s);
+            PW_HIPS(hipGetLastError());
+            float* oA = A;
+            float* oB = B;
+            A = Sb; B = Sa; Sa = oA; Sb = oB;  // (q_{t-1}, q_t) lead the next launch
+            it -= 2;
+            continue;
+        }
+        StepArgs<float> s_ = base_args<float>(p);
+        s_.X = A; s_.Y = B; s_.it = it;
+        s_.fsrc = step_src<float>(p, it);
+        s_.inj = (const float*)p->res;
+        ExArgs<float> ex{seg + (size_t)(it - a0) * fld, p->fs, (float*)p->gm};
+        launch_exact<float>(p->w_exact, g, s_, ex, a.padnz, a.padnx, nb_, s);
+        PW_HIPS(hipGetLastError());
+        std::swap(A, B);
+        --it;
+    }
+    *Ap = A;
+    *Bp = B;
+    p->adj_sa = Sa;
+    p->adj_sb = Sb;
     return PW_OK;
 }
 
@@ -1901,6 +2566,9 @@ int recompute_segment(pw_fwi* p, int nb_, int sg, int mode)
 template <typename T>
 int adjoint_segment(pw_fwi* p, int nb_, int sg, int mode, T** Ap, T** Bp, T* grad_ref, long long gstride)
 {
+    if constexpr (sizeof(T) == 4) {
+        if (mode == 1 && fused2_adj(p)) return adjoint_segment2(p, nb_, sg, Ap, Bp);
+    }
     const Acq& a = p->acq;
     hipStream_t s = p->ctx->s();
     FGeo<T> g = geo<T>(p);
@@ -2038,6 +2706,8 @@ int gradient_all(pw_fwi* p, int mode, double* fcost)
         }
         T* A = (T*)p->A;
         T* B = (T*)p->B;
+        p->adj_sa = p->A2;
+        p->adj_sb = p->B2;
         for (int sg = p->nseg - 1; sg >= 0; --sg) {
             if (!(store_last && sg == p->nseg - 1))
                 if ((rc = recompute_segment<T>(p, nb_, sg, mode))) return rc;
@@ -2665,6 +3335,8 @@ int pw_fwi_adjoint_dev(pw_fwi* p, const void* d_vel, const void* d_gtraces, void
             if (!rc) rc = fill<float>(p, p->B, fld, 0.f);
             if (!rc) rc = fill<float>(p, p->gm, fld, 0.f);
             float *A = (float*)p->A, *B = (float*)p->B;
+            p->adj_sa = p->A2;
+            p->adj_sb = p->B2;
             for (int sg = p->nseg - 1; sg >= 0 && !rc; --sg) {
                 if (sg != p->nseg - 1) rc = recompute_segment<float>(p, nb_, sg, 1);
                 if (!rc) rc = adjoint_segment<float>(p, nb_, sg, 1, &A, &B, nullptr, 0);
@@ -2682,6 +3354,8 @@ int pw_fwi_adjoint_dev(pw_fwi* p, const void* d_vel, const void* d_gtraces, void
             if (!rc) rc = fill<double>(p, p->B, fld, 0.0);
             if (!rc) rc = fill<double>(p, p->gm, fld, 0.0);
             double *A = (double*)p->A, *B = (double*)p->B;
+            p->adj_sa = p->A2;
+            p->adj_sb = p->B2;
             for (int sg = p->nseg - 1; sg >= 0 && !rc; --sg) {
                 if (sg != p->nseg - 1) rc = recompute_segment<double>(p, nb_, sg, 1);
                 if (!rc) rc = adjoint_segment<double>(p, nb_, sg, 1, &A, &B, nullptr, 0);
@@ -2707,6 +3381,25 @@ int pw_fwi_widths(pw_fwi* p, int* w)
     if (!p || !w) return PW_ERR_ARG;
     w[0] = p->w_fwd; w[1] = p->w_fwdv; w[2] = p->w_exact; w[3] = p->w_ref; w[4] = p->w_refv;
     w[5] = p->widths_tuned ? 1 : 0;
+    w[6] = p->w_fwd2;
+    w[7] = p->w_adj2;
+    return PW_OK;
+}
+
+int pw_fwi_tuning_log(pw_fwi* p, float* out, int cap)
+{
+    if (!p) return PW_ERR_ARG;
+    const int n = (int)p->tune_log.size() / 3;
+    for (int i = 0; i < std::min(n, cap) * 3 && out; ++i) out[i] = p->tune_log[i];
+    return n;
+}
+
+int pw_fwi_set_steps_per_launch(pw_fwi* p, int n)
+{
+    if (!p || !(n == 1 || n == 2 || n == -2)) return PW_ERR_ARG;
+    p->max_fused = n < 0 ? -n : n;
+    p->force_fused = n < 0;
+    p->widths_tuned = false;
     return PW_OK;
 }
 

@@ -128,10 +128,26 @@ class FwiProblem:
     def widths(self):
         """Tile widths of the step kernels picked by the plan's timing sweep (its first sweep runs
         it): forward, visco forward, exact adjoint, reference adjoint, visco reference adjoint."""
-        w = np.zeros(6, np.int32)
+        w = np.zeros(8, np.int32)
         N.check(self._L.pw_fwi_widths(self._p, N.ptr(w)), self._ctx, "widths")
         keys = ["forward", "visco_forward", "exact_adjoint", "reference_adjoint", "visco_reference_adjoint"]
-        return dict(zip(keys, (int(x) for x in w[:5])), tuned=bool(w[5]))
+        return dict(zip(keys, (int(x) for x in w[:5])), tuned=bool(w[5]), two_step_forward=int(w[6]),
+                    two_step_exact_adjoint=int(w[7]))
+
+    def tuning_log(self):
+        """The width-timing sweep's measurements: [(kernel, width, ms per launch), ...]."""
+        names = ["forward", "visco_forward", "exact_adjoint", "reference_adjoint", "visco_reference_adjoint",
+                 "two_step_forward", "two_step_exact_adjoint"]
+        n = self._L.pw_fwi_tuning_log(self._p, None, 0)
+        buf = np.zeros(3 * max(n, 1), np.float32)
+        self._L.pw_fwi_tuning_log(self._p, N.ptr(buf), n)
+        return [(names[int(buf[3 * i])], int(buf[3 * i + 1]), round(float(buf[3 * i + 2]), 4)) for i in range(n)]
+
+    def set_steps_per_launch(self, n):
+        """Forward time steps one launch may advance: 2 (default: two-step launches where the plan's
+        timing sweep finds them faster), 1 (one kernel per time step, the plain method) or -2 (two
+        steps per launch whatever the timing, on acoustic point-source plans)."""
+        N.check(self._L.pw_fwi_set_steps_per_launch(self._p, int(n)), self._ctx, "set_steps_per_launch")
 
     def timing(self):
         f, b = ctypes.c_double(), ctypes.c_double()

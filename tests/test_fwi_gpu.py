@@ -290,3 +290,46 @@ def test_adjoint_dev_recomputes_after_in_place_model_change():
     g_got = adjoint(p, vz)
     p.close()
     assert np.array_equal(g_got, g_ref)
+
+
+@pytest.mark.parametrize("ckpt", [0, 17, 50])
+def test_two_step_forward_equals_single_steps(small, ckpt):
+    # the temporally blocked forward (two steps per launch, forced) against one kernel per time
+    # step: modelled data, the reference gradient (its recomputed snapshots) and the exact gradient
+    # (its recomputed terms) bit for bit, also with an odd checkpoint interval
+    out = {}
+    for n in (1, -2):
+        p = FwiProblem(small["par"], small["wav"], data=small["data"], batch=2, checkpoint=ckpt)
+        p.set_steps_per_launch(n)
+        d = p.forward(small["vtrue"])
+        w = p.widths()
+        g0, f0, _ = p.gradient(small["v0"], "reference")
+        g1, f1, _ = p.gradient(small["v0"], "exact")
+        p.close()
+        out[n] = (d, g0, f0, g1, f1, w)
+    assert out[-2][5]["two_step_forward"] > 0 and out[1][5]["two_step_forward"] == 0
+    assert out[-2][5]["two_step_exact_adjoint"] > 0 and out[1][5]["two_step_exact_adjoint"] == 0
+    for k in range(3):
+        assert np.array_equal(out[1][k], out[-2][k]), k
+    # the exact gradient: two-step forward terms and two-step adjoint launches; the adjoint
+    # contracts to FMAs, so its rounding may differ from single launches in the last bits
+    assert out[1][4] == out[-2][4]
+    e = rel_l2(out[-2][3], out[1][3])
+    print("two-step exact gradient vs single steps: rel-L2 %.2e" % e)
+    assert e <= 1e-6
+    # and against the compiled reference's forward (golden)
+    assert np.array_equal(out[-2][0], small["data"])
+
+
+def test_two_step_forward_odd_nt_and_marine_geometry(golden):
+    g = golden("fwi_marine")
+    par = g["par"].copy()
+    par[6] = 299  # odd nt: the last step runs alone
+    res = []
+    for n in (1, -2):
+        p = FwiProblem(par, g["wav"], batch=2)
+        p.set_steps_per_launch(n)
+        res.append(p.forward(g["vel"]))
+        p.close()
+    assert np.array_equal(res[0], res[1])
+    assert np.array_equal(res[1], g["data"][:299])
