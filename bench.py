@@ -295,7 +295,9 @@ def lstri_bench():
     src_inv, its = p.invert(data, return_iters=True)
     inv_s = time.perf_counter() - t
     inv_dev_ms = p.timing()
+    (inv_w, inv_spl), inv_log = p.launch_choice()
     fit = p.model(src_inv)
+    (mod_w, mod_spl), mod_log = p.launch_choice()
     resid = float(np.linalg.norm(fit - data) / np.linalg.norm(data))
     n_sweep_steps = 2 * nt * int(np.sum(its))
     out = {"workload": "lstri2d 130x334 (+60 decay: 250x454), nt=3001, 3 events x 6 buried sources, "
@@ -305,7 +307,11 @@ def lstri_bench():
            "invert_s": round(inv_s, 3), "invert_device_ms": round(inv_dev_ms, 1),
            "invert_ms_per_cg_iteration": round(inv_dev_ms / max(int(np.sum(its)), 1), 2),
            "invert_us_per_sweep_step": round(inv_dev_ms * 1e3 / max(n_sweep_steps, 1), 3),
-           "cg_iterations": [int(x) for x in its], "relative_residual": round(resid, 4)}
+           "cg_iterations": [int(x) for x in its], "relative_residual": round(resid, 4),
+           "launch_choice": {"invert_one_shot": {"tile_width": inv_w, "steps_per_launch": inv_spl,
+                                                 "timing_ms_per_launch": inv_log},
+                             "model_all_shots": {"tile_width": mod_w, "steps_per_launch": mod_spl,
+                                                 "timing_ms_per_launch": mod_log}}}
     p.close()
     try:
         from oracle import refrun
@@ -595,6 +601,7 @@ def batch_bench():
     b.run(use_graph=True)  # warm-up (graph capture)
     ms = b.run(use_graph=True)
     cu = b.cell_updates()
+    width, wlog = b.width()
     b.close()
     nrec = sum(s["vel"].shape[1] * s["nt"] for s in shots)
     alg = 16.0 * cu + 4.0 * nrec
@@ -604,6 +611,7 @@ def batch_bench():
             "cell_updates": cu, "Gcell_per_s": round(cu / (ms * 1e-3) / 1e9, 2),
             "achieved_GBs": round(alg / (ms * 1e-3) / 1e9, 1),
             "frac_hbm": round(alg / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+            "tile_width": width, "width_timing_ms_per_launch": {str(w): t for w, t in wlog},
             "cpu_baseline": cpu}
 
 

@@ -246,10 +246,19 @@ int pw_lstri_invert(pw_lstri *p, const float *data, int niter, float *src, int *
 /* on != 0 (default): the inversion replays its two sweeps from hipGraphs captured on the first
  * call; 0 launches them step by step.  Results are identical.  No counterpart in the reference. */
 int pw_lstri_use_graph(pw_lstri *p, int on);
-/* time steps per sweep launch: 1, 2, or 0 for the plan's choice (two steps per launch on grids of at
- * most 2^20 padded cells, where a step is latency-bound); identical results either way.  No
- * counterpart in the reference (a scheduling choice of this engine). */
+/* time steps per sweep launch: 1, 2, or 0 for the plan's choice (timed before its first sweep of
+ * each shot count: single-step tile widths 1-16 and two-step launches on the plan's own grid);
+ * identical results either way.  No counterpart in the reference (a scheduling choice of this
+ * engine). */
 int pw_lstri_steps_per_launch(pw_lstri *p, int n);
+/* single-step tile width (x columns per 64-cell tile): 1, 2, 4, 8, 16, or 0 for the plan's timed
+ * choice; identical results either way.  No counterpart in the reference. */
+int pw_lstri_set_tile_width(pw_lstri *p, int w);
+/* the last sweep's launch choice: w[0] single-step tile width, w[1] steps per launch; the timing
+ * sweep's (kernel, width, ms per launch) triples into log[3*cap] (kernel 0 modelling step, 1
+ * adjoint step, 2 two-step modelling launch, 3 two-step adjoint launch); returns their number.
+ * No counterpart in the reference. */
+int pw_lstri_launch_choice(pw_lstri *p, int *w, float *log, int cap);
 /* device milliseconds of the last pw_lstri_model / pw_lstri_invert call */
 int pw_lstri_timing(pw_lstri *p, double *ms);
 /* pfwicfun.lstric (called by pywave/pfwi.py:160 and :247): pararray[9] (inv) != 0 -> datasrc =
@@ -291,6 +300,13 @@ int pw_batch2d_run(pw_batch *p, int use_graph, double *ms);
 /* traces of shot i, (nt, nx) Fortran order (afd2d's dout) */
 int pw_batch2d_traces(pw_batch *p, int i, float *out, long long len);
 long long pw_batch2d_cell_updates(pw_batch *p);
+/* tile width (x columns per 64-cell tile): 0 (default) times widths 2, 4, 8 and 16 on the batch
+ * before its next run and keeps the fastest (once per batch geometry and context); 2, 4, 8 or 16
+ * uses that width.  Results are bit-identical at every width.  No counterpart in the reference. */
+int pw_batch2d_set_width(pw_batch *p, int xc);
+/* the width in use (returned); the timing sweep's (width, ms per launch) pairs into log[2*cap]
+ * and their number into *nlog (either may be NULL) */
+int pw_batch2d_width(pw_batch *p, float *log, int cap, int *nlog);
 void pw_batch2d_destroy(pw_batch *p);
 
 #ifdef __cplusplus

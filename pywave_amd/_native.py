@@ -95,6 +95,8 @@ SIGNATURES = {
     "pw_lstri_timing": (c_int, [c_vp, c_vp]),
     "pw_lstri_use_graph": (c_int, [c_vp, c_int]),
     "pw_lstri_steps_per_launch": (c_int, [c_vp, c_int]),
+    "pw_lstri_launch_choice": (c_int, [c_vp, c_vp, c_vp, c_int]),
+    "pw_lstri_set_tile_width": (c_int, [c_vp, c_int]),
     "pw_lstric": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_ll, c_ll]),
     "pw_tri_create": (c_vp, [c_vp, c_int, c_int, c_int, c_float, c_float, c_float, c_int, c_float, c_int, c_float,
                              c_int, c_int]),
@@ -109,6 +111,8 @@ SIGNATURES = {
     "pw_batch2d_run": (c_int, [c_vp, c_int, ctypes.POINTER(c_double)]),
     "pw_batch2d_traces": (c_int, [c_vp, c_int, c_vp, c_ll]),
     "pw_batch2d_cell_updates": (c_ll, [c_vp]),
+    "pw_batch2d_set_width": (c_int, [c_vp, c_int]),
+    "pw_batch2d_width": (c_int, [c_vp, c_vp, c_int, ctypes.POINTER(c_int)]),
     "pw_batch2d_destroy": (None, [c_vp]),
 }
 

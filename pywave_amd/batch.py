@@ -52,6 +52,20 @@ class Afd2dBatch:
     def cell_updates(self):
         return int(self._L.pw_batch2d_cell_updates(self._p))
 
+    def set_width(self, xc=0):
+        """Tile width: 0 times widths 2, 4, 8, 16 before the next run and keeps the fastest; else
+        that width.  Bit-identical results at every width."""
+        N.check(self._L.pw_batch2d_set_width(self._p, int(xc)), self._ctx, "set_width")
+
+    def width(self):
+        """(width in use, [(width, ms per launch), ...] of the timing sweep that chose it)."""
+        n = ctypes.c_int(0)
+        self._L.pw_batch2d_width(self._p, None, 0, ctypes.byref(n))
+        buf = np.zeros(2 * max(n.value, 1), np.float32)
+        w = self._L.pw_batch2d_width(self._p, N.ptr(buf), n.value, ctypes.byref(n))
+        N.check(min(w, 0), self._ctx, "width")
+        return int(w), [(int(buf[2 * i]), round(float(buf[2 * i + 1]), 4)) for i in range(n.value)]
+
     def close(self):
         if getattr(self, "_p", None):
             self._L.pw_batch2d_destroy(self._p)

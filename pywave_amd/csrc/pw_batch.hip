@@ -4,7 +4,8 @@
 // 397-686).  Here every shot keeps its own padded fields in one pooled allocation; shots are
 // ordered by decreasing nt so the shots still running at step it are a prefix, and one launch per
 // step covers the flat list of 64 x XC tiles of all running shots (a tile looks up its shot in a
-// table).  The step loop is captured once into a hipGraph and replayed.  Per cell the arithmetic
+// table; the tile width XC is timed on the batch itself before its first run).  The step loop is
+// captured once into a hipGraph and replayed.  Per cell the arithmetic
 // is that of the single-shot kernel (pw_afd.hip), so every shot is bit-identical to afd2d.
 #include <algorithm>
 #include <cmath>
@@ -16,7 +17,8 @@
 namespace {
 
 constexpr float C0 = -30. / 12., C1 = +16. / 12., C2 = -1. / 12.;  // pywave/src/wave_fdm.c:15
-constexpr int XC = 4;  // x columns streamed per tile
+constexpr int kBatchWidths[] = {2, 4, 8, 16};  // x columns streamed per tile, as timed
+constexpr int kBatchDefaultWidth = 4;
 
 struct Shot {
     int nz, nx, nz2, nx2, ldz, nb, nz1, nx1;
@@ -57,6 +59,7 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t field_rsrc(const float* p, uns
 // scalar offset; lanes outside the shot's updated domain (zero in every field), and every lane
 // of a column outside it, get an out-of-range offset, so such cells are neither read nor
 // written and need no branches.
+template <int XC>
 __global__ __launch_bounds__(64) void k_afd2d_batch(BatchArgs a)
 {
     const int tile = blockIdx.x;
@@ -243,17 +246,167 @@ struct pw_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool dirty = true;
+    int xc = kBatchDefaultWidth;  // tile width of the current layout
+    int xc_forced = 0;            // 0: timed before the first run; else this width
+    bool tuned = false;
+    int max_tiles = 0;            // tiles at the narrowest width (size of the tile -> shot table)
+    std::vector<float> tune_log;  // (width, ms per launch) of the timing sweep
 };
 
 namespace {
 
-void batch_free(pw_batch* p)
+void drop_graph(pw_batch* p)
 {
     if (p->exec) hipGraphExecDestroy(p->exec);
     if (p->graph) hipGraphDestroy(p->graph);
+    p->exec = nullptr;
+    p->graph = nullptr;
+}
+
+void batch_free(pw_batch* p)
+{
+    drop_graph(p);
     void* b[] = {p->d_a, p->d_b, p->d_vv, p->d_rec, p->d_wav, p->d_damp, p->d_src, p->d_tile_shot, p->d_shots};
     for (void* x : b)
         if (x) hipFree(x);
+}
+
+// Tile layout for width xc: per-shot tile counts and first tiles, the tile -> shot table and the
+// active tile count per step (shots sorted by decreasing nt, so the running shots are a prefix).
+void layout(pw_batch* p, int xc, std::vector<int>& tile_shot)
+{
+    int tile0 = 0;
+    for (Shot& s : p->shots) {
+        s.nxc = pw::ceil_div(s.nx, xc);
+        s.tile0 = tile0;
+        tile0 += s.ntz * s.nxc;
+    }
+    p->total_tiles = tile0;
+    tile_shot.assign(tile0, 0);
+    for (int k = 0; k < p->n; ++k)
+        for (int t = 0; t < p->shots[k].ntz * p->shots[k].nxc; ++t) tile_shot[p->shots[k].tile0 + t] = k;
+    // shots [0, k) run at step it; k shrinks as it grows (shot 0 runs to the last step)
+    p->tiles_active.assign(p->max_nt, 0);
+    for (int it = 0, k = p->n; it < p->max_nt; ++it) {
+        while (k > 1 && p->shots[k - 1].nt <= it) --k;
+        p->tiles_active[it] = k == p->n ? tile0 : p->shots[k].tile0;
+    }
+    p->xc = xc;
+}
+
+// every tile launched at step it belongs to a shot that has step it (its traces and wavelet reach it)
+bool layout_consistent(const pw_batch* p, const std::vector<int>& tile_shot)
+{
+    for (int it = 0; it < p->max_nt; ++it) {
+        const int n = p->tiles_active[it];
+        if (n < 1 || n > p->total_tiles || p->shots[tile_shot[n - 1]].nt <= it) return false;
+        if (n < p->total_tiles && p->shots[tile_shot[n]].nt > it) return false;
+    }
+    return true;
+}
+
+// switch the device tables to width xc (the captured step loop holds the old grid sizes)
+int apply_layout(pw_batch* p, int xc)
+{
+    std::vector<int> tile_shot;
+    layout(p, xc, tile_shot);
+    if (!layout_consistent(p, tile_shot)) PW_FAIL(p->ctx, PW_ERR_STATE, "batch2d: inconsistent tile layout");
+    drop_graph(p);
+    hipStream_t st = p->ctx->s();
+    PW_HIP(p->ctx, hipMemcpyAsync(p->d_tile_shot, tile_shot.data(), sizeof(int) * tile_shot.size(),
+                                  hipMemcpyHostToDevice, st));
+    PW_HIP(p->ctx, hipMemcpyAsync(p->d_shots, p->shots.data(), sizeof(Shot) * p->n, hipMemcpyHostToDevice, st));
+    PW_HIP(p->ctx, hipStreamSynchronize(st));
+    return PW_OK;
+}
+
+BatchArgs step_args(const pw_batch* p, int it)
+{
+    BatchArgs a{};
+    a.shots = p->d_shots; a.tile_shot = p->d_tile_shot;
+    a.pool_a = p->d_a; a.pool_b = p->d_b; a.vv = p->d_vv; a.damp = p->d_damp;
+    a.src = p->d_src; a.wav = p->d_wav; a.rec = p->d_rec; a.it = it;
+    return a;
+}
+
+void launch_step(int xc, int tiles, const BatchArgs& a, hipStream_t st)
+{
+    switch (xc) {
+        case 2: hipLaunchKernelGGL(k_afd2d_batch<2>, dim3(tiles), dim3(64), 0, st, a); break;
+        case 8: hipLaunchKernelGGL(k_afd2d_batch<8>, dim3(tiles), dim3(64), 0, st, a); break;
+        case 16: hipLaunchKernelGGL(k_afd2d_batch<16>, dim3(tiles), dim3(64), 0, st, a); break;
+        default: hipLaunchKernelGGL(k_afd2d_batch<4>, dim3(tiles), dim3(64), 0, st, a); break;
+    }
+}
+
+// What decides the timing sweep's outcome: the shots' grids and lengths, the absorbing width and
+// the source centres (tiles holding a source run the general loop).
+std::vector<long long> tune_key(const pw_batch* p)
+{
+    std::vector<long long> k = {1, p->n, p->shots.empty() ? 0 : p->shots[0].nb, p->xc_forced};
+    for (const Shot& s : p->shots) {
+        k.push_back(s.nz);
+        k.push_back(s.nx);
+        k.push_back(s.nt);
+        k.push_back(s.ns);
+    }
+    k.insert(k.end(), p->src_host.begin(), p->src_host.end());
+    return k;
+}
+
+// Tile-width sweep: each width in kBatchWidths laid out and timed on the first time step (every
+// shot running) after clearing the fields; the fastest is kept.  The run that follows clears
+// the fields again and rewrites every trace sample the timing launches wrote.
+int tune_width(pw_batch* p)
+{
+    const std::vector<long long> key = tune_key(p);
+    auto hit = p->ctx->tuned2d.find(key);
+    if (hit != p->ctx->tuned2d.end()) {
+        p->tune_log = hit->second.second;
+        p->tuned = true;
+        return apply_layout(p, hit->second.first[0]);
+    }
+    hipStream_t st = p->ctx->s();
+    p->tune_log.clear();
+    int best = p->xc_forced ? p->xc_forced : kBatchDefaultWidth;
+    if (!p->xc_forced) {
+        hipEvent_t e0, e1;
+        PW_HIP(p->ctx, hipEventCreate(&e0));
+        PW_HIP(p->ctx, hipEventCreate(&e1));
+        float tbest = 0.f;
+        int rc = PW_OK;
+        for (int W : kBatchWidths) {
+            if ((rc = apply_layout(p, W))) break;
+            hipLaunchKernelGGL(k_zero, dim3(2048), dim3(256), 0, st, p->d_a, p->pool);
+            hipLaunchKernelGGL(k_zero, dim3(2048), dim3(256), 0, st, p->d_b, p->pool);
+            const int reps = 8;
+            // every launch is time step 0, which every shot has (traces row 0, wavelet sample 0)
+            const BatchArgs a0 = step_args(p, 0);
+            for (int r = 0; r < 2; ++r) launch_step(W, p->tiles_active[0], a0, st);
+            hipEventRecord(e0, st);
+            for (int r = 0; r < reps; ++r) launch_step(W, p->tiles_active[0], a0, st);
+            hipEventRecord(e1, st);
+            float ms = 0.f;
+            if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                p->ctx->err = "batch2d: width timing launch failed";
+                rc = PW_ERR_HIP;
+                break;
+            }
+            ms /= reps;
+            p->tune_log.push_back((float)W);
+            p->tune_log.push_back(ms);
+            if (W == kBatchWidths[0] || ms < tbest) { tbest = ms; best = W; }
+        }
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        if (rc) return rc;
+    }
+    int rc = apply_layout(p, best);
+    if (rc) return rc;
+    p->tuned = true;
+    p->ctx->tuned2d[key] = {{best}, p->tune_log};
+    return PW_OK;
 }
 
 }  // namespace
@@ -277,7 +430,6 @@ pw_batch* pw_batch2d_create(pw_ctx* ctx, int n, const int* dims, const float* d,
     pw::abc_weights(nbt, ct, w.data());
     std::vector<float2> damp;
     long long off = 0, roff = 0;
-    int tile0 = 0;
     for (int k = 0; k < n; ++k) {
         const int i = p->order[k];
         const int nz1 = dims[3 * i], nx1 = dims[3 * i + 1], nt = dims[3 * i + 2];
@@ -306,37 +458,32 @@ pw_batch* pw_batch2d_create(pw_ctx* ctx, int n, const int* dims, const float* d,
         s.rec_off = roff;
         roff += (long long)nt * nx1;
         s.ntz = pw::ceil_div(s.nz, 64);
-        s.nxc = pw::ceil_div(s.nx, XC);
-        s.tile0 = tile0;
-        tile0 += s.ntz * s.nxc;
         p->shots.push_back(s);
         p->dts.push_back(dt);
         p->max_nt = std::max(p->max_nt, nt);
     }
-    p->total_tiles = tile0;
     p->pool = off;
     p->rec_pool = roff;
     p->damp_pool = (long long)damp.size();
     p->ns_of.assign(n, 0);
     p->have.assign(n, 0);
-    std::vector<int> tile_shot(tile0);
-    for (int k = 0; k < n; ++k)
-        for (int t = 0; t < p->shots[k].ntz * p->shots[k].nxc; ++t) tile_shot[p->shots[k].tile0 + t] = k;
-    // active tiles per step: shots sorted by nt, so running shots form a prefix
-    p->tiles_active.assign(p->max_nt, 0);
-    for (int it = 0; it < p->max_nt; ++it) {
-        int k = 0;
-        while (k < n && p->shots[k].nt > it) ++k;
-        p->tiles_active[it] = k == n ? tile0 : p->shots[k].tile0;
+    std::vector<int> tile_shot;
+    layout(p, kBatchWidths[0], tile_shot);  // the narrowest width has the most tiles
+    p->max_tiles = p->total_tiles;
+    layout(p, kBatchDefaultWidth, tile_shot);
+    if (!layout_consistent(p, tile_shot)) {
+        ctx->err = "pw_batch2d_create: inconsistent tile layout";
+        delete p;
+        return nullptr;
     }
     bool ok = hipMalloc(&p->d_a, p->pool * 4) == hipSuccess && hipMalloc(&p->d_b, p->pool * 4) == hipSuccess &&
               hipMalloc(&p->d_vv, p->pool * 4) == hipSuccess && hipMalloc(&p->d_rec, p->rec_pool * 4) == hipSuccess &&
               hipMalloc(&p->d_damp, p->damp_pool * sizeof(float2)) == hipSuccess &&
-              hipMalloc(&p->d_tile_shot, sizeof(int) * tile0) == hipSuccess &&
+              hipMalloc(&p->d_tile_shot, sizeof(int) * p->max_tiles) == hipSuccess &&
               hipMalloc(&p->d_shots, sizeof(Shot) * n) == hipSuccess;
     hipStream_t st = ctx->s();
     ok = ok && hipMemcpyAsync(p->d_damp, damp.data(), damp.size() * sizeof(float2), hipMemcpyHostToDevice, st) == hipSuccess &&
-         hipMemcpyAsync(p->d_tile_shot, tile_shot.data(), sizeof(int) * tile0, hipMemcpyHostToDevice, st) == hipSuccess &&
+         hipMemcpyAsync(p->d_tile_shot, tile_shot.data(), sizeof(int) * tile_shot.size(), hipMemcpyHostToDevice, st) == hipSuccess &&
          hipMemsetAsync(p->d_vv, 0, p->pool * 4, st) == hipSuccess && hipStreamSynchronize(st) == hipSuccess;
     if (!ok) {
         ctx->err = "pw_batch2d_create: device allocation failed";
@@ -435,8 +582,7 @@ static int batch_upload(pw_batch* p)
         PW_HIP(ctx, hipMemcpyAsync(p->d_wav, p->wav_host.data(), p->wav_host.size() * 4, hipMemcpyHostToDevice, st));
     PW_HIP(ctx, hipMemcpyAsync(p->d_shots, p->shots.data(), sizeof(Shot) * p->n, hipMemcpyHostToDevice, st));
     PW_HIP(ctx, hipStreamSynchronize(st));
-    if (p->exec) { hipGraphExecDestroy(p->exec); p->exec = nullptr; }
-    if (p->graph) { hipGraphDestroy(p->graph); p->graph = nullptr; }
+    drop_graph(p);
     p->dirty = false;
     return PW_OK;
 }
@@ -446,13 +592,7 @@ static void batch_enqueue_steps(pw_batch* p, hipStream_t st)
     // zero the fields with a fill kernel (graph-safe), then one launch per step
     hipLaunchKernelGGL(k_zero, dim3(2048), dim3(256), 0, st, p->d_a, p->pool);
     hipLaunchKernelGGL(k_zero, dim3(2048), dim3(256), 0, st, p->d_b, p->pool);
-    for (int it = 0; it < p->max_nt; ++it) {
-        BatchArgs a{};
-        a.shots = p->d_shots; a.tile_shot = p->d_tile_shot;
-        a.pool_a = p->d_a; a.pool_b = p->d_b; a.vv = p->d_vv; a.damp = p->d_damp;
-        a.src = p->d_src; a.wav = p->d_wav; a.rec = p->d_rec; a.it = it;
-        hipLaunchKernelGGL(k_afd2d_batch, dim3(p->tiles_active[it]), dim3(64), 0, st, a);
-    }
+    for (int it = 0; it < p->max_nt; ++it) launch_step(p->xc, p->tiles_active[it], step_args(p, it), st);
 }
 
 // use_graph != 0: capture the step loop once and replay it
@@ -464,6 +604,11 @@ int pw_batch2d_run(pw_batch* p, int use_graph, double* ms)
         if (!p->have[k]) PW_FAIL(ctx, PW_ERR_STATE, "batch run: a shot has no model/sources");
     if (p->dirty) {
         int rc = batch_upload(p);
+        if (rc) return rc;
+        p->tuned = false;  // sources moved: the tiles that run the general loop may differ
+    }
+    if (!p->tuned) {
+        int rc = tune_width(p);
         if (rc) return rc;
     }
     hipStream_t st = ctx->s();
@@ -504,6 +649,30 @@ int pw_batch2d_traces(pw_batch* p, int i, float* out, long long len)
     for (int it = 0; it < s.nt; ++it)
         for (int ix = 0; ix < s.nx1; ++ix) out[(size_t)ix * s.nt + it] = t[(size_t)it * s.nx1 + ix];
     return PW_OK;
+}
+
+// tile width: 0 (default) times each width on the batch before its next run and keeps the
+// fastest; 2, 4, 8 or 16 uses that width
+int pw_batch2d_set_width(pw_batch* p, int xc)
+{
+    if (!p) return PW_ERR_ARG;
+    bool ok = xc == 0;
+    for (int W : kBatchWidths) ok |= xc == W;
+    if (!ok) PW_FAIL(p->ctx, PW_ERR_ARG, "batch set_width: width must be 0, 2, 4, 8 or 16");
+    p->xc_forced = xc;
+    p->tuned = false;
+    return PW_OK;
+}
+
+// tile width of the last run (or of the next, before the first); log (width, ms per launch) pairs of
+// the timing sweep that chose it into out[2*cap]; returns the width
+int pw_batch2d_width(pw_batch* p, float* log, int cap, int* nlog)
+{
+    if (!p) return PW_ERR_ARG;
+    const int n = (int)p->tune_log.size() / 2;
+    for (int i = 0; i < std::min(n, cap) * 2 && log; ++i) log[i] = p->tune_log[i];
+    if (nlog) *nlog = n;
+    return p->xc;
 }
 
 // total cell-updates of one run (updated domains x nt, summed over shots)

@@ -2372,10 +2372,45 @@ int tune_widths(pw_fwi* p)
     return PW_OK;
 }
 
+// What decides the timing sweep's outcome: precision, grid, shots per launch, the kernels the
+// plan runs and the acquisition (which tiles hold sources and receivers and so run the general
+// loop).  Plans with equal keys on one context share one sweep's widths.
+std::vector<long long> tune_key(const pw_fwi* p)
+{
+    const Acq& a = p->acq;
+    std::vector<long long> k = {2, (long long)p->prec, p->nslot, a.padnz, a.padnx, a.nt, a.nb, a.sz, a.rz,
+                                a.ns, a.acqui_type, a.ds_v, a.s0_v, a.dr_v, p->visco, p->passive,
+                                p->max_fused, p->force_fused, p->seg_cap >= 2LL * p->nslot * p->fs};
+    k.insert(k.end(), a.r0_v.begin(), a.r0_v.end());
+    k.insert(k.end(), a.nr2.begin(), a.nr2.end());
+    return k;
+}
+
 int ensure_widths(pw_fwi* p)
 {
     if (p->widths_tuned) return PW_OK;
-    return p->prec == 4 ? tune_widths<float>(p) : tune_widths<double>(p);
+    const std::vector<long long> key = tune_key(p);
+    auto hit = p->ctx->tuned2d.find(key);
+    if (hit != p->ctx->tuned2d.end()) {
+        const std::vector<int>& w = hit->second.first;
+        p->w_fwd = w[0]; p->w_fwdv = w[1]; p->w_exact = w[2]; p->w_ref = w[3]; p->w_refv = w[4];
+        p->w_fwd2 = w[5]; p->w_adj2 = w[6];
+        const long long fld = p->nslot * p->fs;
+        int rc = PW_OK;
+        if (p->w_fwd2 && !p->X2) rc = dmalloc(p, &p->X2, fld * esz(p));
+        if (!rc && p->w_fwd2 && !p->Y2) rc = dmalloc(p, &p->Y2, fld * esz(p));
+        if (!rc && p->w_adj2 && !p->A2) rc = dmalloc(p, &p->A2, fld * 4);
+        if (!rc && p->w_adj2 && !p->B2) rc = dmalloc(p, &p->B2, fld * 4);
+        if (rc) return rc;
+        p->tune_log = hit->second.second;  // the sweep of the earlier plan that chose them
+        p->widths_tuned = true;
+        return PW_OK;
+    }
+    int rc = p->prec == 4 ? tune_widths<float>(p) : tune_widths<double>(p);
+    if (!rc)
+        p->ctx->tuned2d[key] = {{p->w_fwd, p->w_fwdv, p->w_exact, p->w_ref, p->w_refv, p->w_fwd2, p->w_adj2},
+                                p->tune_log};
+    return rc;
 }
 
 // forward sweep over one batch: traces into rec; checkpoints (and the last segment's

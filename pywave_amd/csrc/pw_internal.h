@@ -20,6 +20,10 @@ struct pw_ctx {
     std::string err;
     // autotuned 3D step choice per padded grid shape: (nz2, nx2, ny2) -> (variant, y chunk)
     std::map<std::tuple<int, int, int>, std::pair<int, int>> tuned;
+    // autotuned 2D tile widths per workload key (kind, padded shapes, shots per launch and what
+    // else decides which tiles run the general loop) -> the widths a plan's timing sweep chose,
+    // then that sweep's log (kernel, width, ms per launch)
+    std::map<std::vector<long long>, std::pair<std::vector<int>, std::vector<float>>> tuned2d;
     unsigned* d_count = nullptr;  // device counter of pw::count_nonfinite (allocated on first use)
     hipStream_t s() const { return use_ext ? ext_stream : stream; }
 };

@@ -595,7 +595,14 @@ struct pw_lstri {
     float *sfz = nullptr, *sfx = nullptr;
     float* fields = nullptr;  // 4 * cap_slots fields (two levels, and two for the two-step launches)
     int cap_slots = 0;
-    int two_step = 0;         // sweeps advance two steps per launch (small grids)
+    int two_step = 0;         // sweeps advance two steps per launch
+    // launch choice per shots-per-sweep count: (single-step tile width, two steps per launch),
+    // from timing both sweeps' launches on the plan's own grid and buffers
+    std::map<int, std::pair<int, int>> choice;
+    int spl_forced = 0;       // 0: the timed choice; 1 or 2 steps per launch
+    int xct_forced = 0;       // 0: the timed choice; else this single-step tile width
+    std::vector<float> tune_log;  // (kernel, width, ms per launch) of the last timing sweep
+    int graph_xct = 0, graph_two = -1;  // the launch choice the captured CG sweeps hold
     float *src = nullptr, *dat = nullptr, *stage = nullptr;  // batch buffers (modelling / imaging)
     int cap_src_slots = 0;
     float *x = nullptr, *gvec = nullptr, *S = nullptr, *rr = nullptr, *gg = nullptr, *Ss = nullptr;
@@ -685,6 +692,88 @@ void step(pw_lstri* p, bool adj, int nslot, const TriArgs& a)
         case 8: launch_step<8>(p, adj, nslot, a); break;
         default: launch_step<16>(p, adj, nslot, a); break;
     }
+}
+
+int ensure_batch(pw_lstri* p, int nslot);
+
+// Milliseconds per launch of fn (two warm-up launches, then `reps` timed ones).
+template <typename F>
+int time_launches(pw_lstri* p, F fn, float* ms)
+{
+    hipStream_t s = p->ctx->s();
+    const int reps = 10;
+    hipEvent_t e0, e1;
+    LS_HIP(hipEventCreate(&e0));
+    LS_HIP(hipEventCreate(&e1));
+    fn();
+    fn();
+    hipEventRecord(e0, s);
+    for (int k = 0; k < reps; ++k) fn();
+    hipEventRecord(e1, s);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    LS_HIP(e);
+    *ms /= reps;
+    return PW_OK;
+}
+
+// Launch choice for sweeps of nslot shots: each single-step tile width timed on a modelling and
+// an adjoint step, then the two-step launches against two single steps of the fastest width.  The
+// timing launches run on the sweep buffers before a sweep (or its graph capture): the sweep clears
+// the fields and rewrites the data row and image plane they wrote.  Plans of one geometry on one
+// context share the choice.
+int select_launch(pw_lstri* p, int nslot)
+{
+    auto own = p->choice.find(nslot);
+    if (own == p->choice.end()) {
+        const std::vector<long long> key = {3, p->g.pz, p->g.px, p->g.nb, p->g.depth, p->nz, p->nx, p->nt, nslot};
+        auto hit = p->ctx->tuned2d.find(key);
+        if (hit != p->ctx->tuned2d.end()) {
+            p->tune_log = hit->second.second;
+            own = p->choice.emplace(nslot, std::make_pair(hit->second.first[0], hit->second.first[1])).first;
+        } else {
+            int rc = ensure_fields(p, nslot);
+            if (!rc) rc = ensure_batch(p, nslot);
+            if (!rc) rc = zero(p, p->fields, 4LL * nslot * p->g.fs);
+            if (rc) return rc;
+            p->tune_log.clear();
+            const long long fl = (long long)nslot * p->g.fs;
+            TriArgs a{};
+            a.X = p->fields; a.Y = p->fields + fl; a.v2 = p->v2; a.sfz = p->sfz; a.sfx = p->sfx;
+            a.plane = p->src; a.pss = p->nm; a.row = p->dat; a.rss = p->nd;
+            float best = 0.f;
+            int wbest = 1;
+            for (int W : {1, 2, 4, 8, 16}) {
+                p->xct = W;
+                float tm = 0.f, ta = 0.f;
+                if ((rc = time_launches(p, [&] { step(p, false, nslot, a); }, &tm))) return rc;
+                if ((rc = time_launches(p, [&] { step(p, true, nslot, a); }, &ta))) return rc;
+                for (float v : {0.f, (float)W, tm, 1.f, (float)W, ta}) p->tune_log.push_back(v);
+                if (W == 1 || tm + ta < best) { best = tm + ta; wbest = W; }
+            }
+            int two = 0;
+            if (p->nt >= 2) {  // the two-step launch reads planes and rows it and it + 1
+                TriArgs2 b{};
+                b.X = p->fields; b.Y = p->fields + fl; b.C = p->fields + 2 * fl; b.D = p->fields + 3 * fl;
+                b.v2 = p->v2; b.sfz = p->sfz; b.sfx = p->sfx;
+                b.plane0 = p->src; b.plane1 = p->src + p->nzx; b.pss = p->nm;
+                b.row0 = p->dat; b.row1 = p->dat + p->nx; b.rss = p->nd;
+                float tm = 0.f, ta = 0.f;
+                if ((rc = time_launches(p, [&] { step2(p, false, nslot, b); }, &tm))) return rc;
+                if ((rc = time_launches(p, [&] { step2(p, true, nslot, b); }, &ta))) return rc;
+                for (float v : {2.f, (float)OX2, tm, 3.f, (float)OX2, ta}) p->tune_log.push_back(v);
+                two = tm + ta < 2.f * best;
+            }
+            own = p->choice.emplace(nslot, std::make_pair(wbest, two)).first;
+            p->ctx->tuned2d[key] = {{wbest, two}, p->tune_log};
+        }
+    }
+    p->xct = p->xct_forced ? p->xct_forced : own->second.first;
+    p->two_step = p->spl_forced ? p->spl_forced == 2 : own->second.second;
+    return PW_OK;
 }
 
 // modelling sweep: w [slot][nt][nx][nz] -> d [slot][nt][nx] (time-major), nslot shots at once
@@ -889,12 +978,7 @@ pw_lstri* pw_lstri_create(pw_ctx* ctx, const float* pararray)
         delete p;
         return nullptr;
     }
-    p->ntz = (g.pz + 63) / 64;
-    // narrow column chunks when the grid is small, so that a step still has several blocks per CU
-    p->xct = 16;
-    while (p->xct > 1 && (long long)p->ntz * ((g.px + p->xct - 1) / p->xct) < 1024) p->xct /= 2;
-    // and two steps per launch when even one-column tiles leave a step latency-bound
-    p->two_step = (long long)g.pz * g.px <= (1LL << 20);
+    p->ntz = (g.pz + 63) / 64;  // tile width and steps per launch: select_launch, before a sweep
     int rc = ls_alloc(p, &p->v2, g.fs);
     if (!rc) rc = ls_alloc(p, &p->sfz, 2LL * g.pz);
     if (!rc) rc = ls_alloc(p, &p->sfx, 2LL * g.px);
@@ -970,6 +1054,7 @@ int pw_lstri_model(pw_lstri* p, const float* src, float* data)
     float ms_tot = 0;
     for (int s0 = 0; s0 < p->ns; s0 += bs) {
         const int k = std::min(bs, p->ns - s0);
+        if ((rc = select_launch(p, k))) break;
         LS_HIP(hipMemcpyAsync(p->src, src + (size_t)s0 * p->nm, (size_t)k * p->nm * 4, hipMemcpyHostToDevice, s));
         LS_HIP(hipEventRecord(e0, s));
         if ((rc = sweep_model(p, k, p->src, p->dat))) break;
@@ -995,6 +1080,7 @@ int pw_lstri_image(pw_lstri* p, const float* data, float* img)
     if (rc) return rc;
     for (int s0 = 0; s0 < p->ns && !rc; s0 += bs) {
         const int k = std::min(bs, p->ns - s0);
+        if ((rc = select_launch(p, k))) break;
         if ((rc = upload_data(p, data, s0, k, p->dat))) break;
         if ((rc = sweep_adjoint(p, k, p->dat, p->src))) break;
         LS_HIP(hipMemcpyAsync(img + (size_t)s0 * p->nm, p->src, (size_t)k * p->nm * 4, hipMemcpyDeviceToHost, s));
@@ -1024,10 +1110,13 @@ int pw_lstri_invert(pw_lstri* p, const float* data, int niter, float* src, int* 
         if (!rc) rc = zero(p, p->Ss, p->nd);
         if (rc) return rc;
     }
-    if ((rc = ensure_batch(p, 1)) || (rc = ensure_fields(p, 1))) return rc;
+    if ((rc = ensure_batch(p, 1)) || (rc = ensure_fields(p, 1)) || (rc = select_launch(p, 1))) return rc;
+    if (p->graph_xct != p->xct || p->graph_two != p->two_step) drop_graphs(p);
     const long long nm = p->nm, nd = p->nd;
     const bool graph = p->use_graph && s != nullptr;
     if (graph && !p->x_adj) {
+        p->graph_xct = p->xct;
+        p->graph_two = p->two_step;
         LS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         rc = sweep_adjoint(p, 1, p->rr, p->gvec);
         hipError_t e = hipStreamEndCapture(s, &p->g_adj);
@@ -1128,15 +1217,33 @@ int pw_lstri_use_graph(pw_lstri* p, int on)
     return PW_OK;
 }
 
-// time steps per launch of the sweeps: 1, 2, or 0 for the plan's choice (two on grids of at most
-// 2^20 padded cells); the results are the same bit for bit
+// time steps per launch of the sweeps: 1, 2, or 0 for the plan's timed choice; the results are
+// the same bit for bit
 int pw_lstri_steps_per_launch(pw_lstri* p, int n)
 {
     if (!p || n < 0 || n > 2) return PW_ERR_ARG;
-    const int two = n == 0 ? ((long long)p->g.pz * p->g.px <= (1LL << 20)) : (n == 2);
-    if (two != p->two_step) drop_graphs(p);  // captured sweeps hold the other launch sequence
-    p->two_step = two;
+    p->spl_forced = n;
     return PW_OK;
+}
+
+// single-step tile width: 1, 2, 4, 8 or 16 columns, or 0 for the plan's timed choice
+int pw_lstri_set_tile_width(pw_lstri* p, int w)
+{
+    if (!p || !(w == 0 || w == 1 || w == 2 || w == 4 || w == 8 || w == 16)) return PW_ERR_ARG;
+    p->xct_forced = w;
+    return PW_OK;
+}
+
+// the launch choice of the last sweep: w[0] single-step tile width, w[1] steps per launch (1, 2);
+// the timing sweep's (kernel, width, ms per launch) triples into log[3*cap] (kernel 0 modelling
+// step, 1 adjoint step, 2 two-step modelling, 3 two-step adjoint); returns their number
+int pw_lstri_launch_choice(pw_lstri* p, int* w, float* log, int cap)
+{
+    if (!p) return PW_ERR_ARG;
+    if (w) { w[0] = p->xct; w[1] = p->two_step ? 2 : 1; }
+    const int n = (int)p->tune_log.size() / 3;
+    for (int i = 0; i < std::min(n, cap) * 3 && log; ++i) log[i] = p->tune_log[i];
+    return n;
 }
 
 // elapsed milliseconds of the last model / invert call (device work; invert includes its transfers)

@@ -76,3 +76,29 @@ def test_batch_matches_dropin_with_builtin_wavelet():
         ref = pywave_amd.afd2d(s["vel"], nt=s["nt"], dt=0.001, ax=[0, 10, nx], az=[0, 10, nz], ns=1,
                                sx=[nx // 2], sz=[3], f=[20], t=[0.03], A=[1], nbt=10, ct=0.02, jsnap=0)
         assert np.array_equal(tr, ref.reshape(s["nt"], nx, order="F"))
+
+
+def test_every_tile_width_bit_exact_and_tuning_recorded():
+    # shots wide enough that every width has lean interior tiles beside general ones
+    shots = _shots(5, 4) + [dict(_shots(6, 1)[0], vel=np.full((40, 150), 2500, np.float32))]
+    ref = [_oracle(s, 6, 0.03) for s in shots]
+    b = Afd2dBatch(shots, nbt=6, ct=0.03)
+    b.run(True)
+    w, log = b.width()
+    assert w in (2, 4, 8, 16) and sorted(x for x, _ in log) == [2, 4, 8, 16] and all(ms > 0 for _, ms in log)
+    for i in range(len(shots)):
+        assert np.array_equal(b.traces(i), ref[i]), ("tuned", w, i)
+    for xc in (2, 4, 8, 16):
+        b.set_width(xc)
+        b.run(True)
+        assert b.width()[0] == xc
+        for i in range(len(shots)):
+            assert np.array_equal(b.traces(i), ref[i]), (xc, i)
+    with pytest.raises(ValueError):
+        b.set_width(3)
+    b.close()
+    # a second batch of the same geometry on the context takes the first one's choice and log
+    b2 = Afd2dBatch(shots, nbt=6, ct=0.03)
+    b2.run(False)
+    assert b2.width() == (w, log)
+    b2.close()

@@ -129,3 +129,35 @@ def test_two_step_launches_equal_single_steps(ls, nt):
     assert np.array_equal(a.invert(d2), b.invert(d2))
     with pytest.raises(ValueError):
         b.steps_per_launch(3)
+
+
+def test_timed_launch_choice_and_every_tile_width_bit_exact(ls):
+    # the plan times single-step widths 1-16 and two-step launches before its first sweep; every
+    # width gives the same modelled data, image and inversion bit for bit
+    par = ls["par"].copy()
+    rng = np.random.default_rng(9)
+    nt = int(par[6])
+    src = rng.standard_normal((16, 24, nt, 2)).astype(np.float32)
+    dat = rng.standard_normal((nt, 24, 2)).astype(np.float32)
+    p = LstriProblem(par, ls["vel"])
+    d0, i0 = p.model(src), p.image(dat)
+    (w, spl), log = p.launch_choice()
+    assert w in (1, 2, 4, 8, 16) and spl in (1, 2)
+    kinds = [k for k, _, _ in log]
+    assert kinds.count("model_step") == 5 and kinds.count("adjoint_step") == 5 and all(ms > 0 for _, _, ms in log)
+    assert kinds.count("two_step_model") == 1 and kinds.count("two_step_adjoint") == 1
+    pi = par.copy()
+    pi[9], pi[19] = 1, 3
+    q = LstriProblem(pi, ls["vel"])
+    x0 = q.invert(d0)
+    for width in (1, 2, 4, 8, 16):
+        p.set_tile_width(width)
+        p.steps_per_launch(1)
+        assert np.array_equal(p.model(src), d0), width
+        assert np.array_equal(p.image(dat), i0), width
+        assert p.launch_choice()[0] == (width, 1)
+        q.set_tile_width(width)
+        q.steps_per_launch(1)
+        assert np.array_equal(q.invert(d0), x0), width
+    with pytest.raises(ValueError):
+        p.set_tile_width(3)
