@@ -547,8 +547,9 @@ __global__ __launch_bounds__(64 * WAVES) void k_afd3d_pipe(Geo g, Step a, int nt
 // A block owns an output tile of 60 z x (TX1 - 4) x rows over a y chunk.  Streaming along y, at
 // plane j it computes step 1 (C_{n+1}) on the tile grown by 2 cells in z and x (64 lanes x TX1
 // rows, from C_n staged in LDS with a further 2-cell halo) and, two planes behind, step 2
-// (C_{n+2}) at plane j-2 from the step-1 values kept in an LDS ring of three planes (z, x
-// neighbours) and in a 5-plane register queue (y neighbours).  The grown ring of step 1 is
+// (C_{n+2}) at plane j-2 from the step-1 values kept in an LDS ring of five planes: the y
+// neighbours are a lane's own ring entries, the z and x neighbours those of plane j-2, so one
+// barrier per plane suffices and no step-1 value occupies a register beyond its plane.  The grown ring of step 1 is
 // recomputed by neighbouring blocks instead of exchanged; every cell of both steps uses the
 // single-step arithmetic, so results are bit-identical.  C_{n+1} and C_{n+2} go to two other
 // buffers (neighbours still read C_n and C_{n-1} around the tile).  HBM traffic per two steps:
@@ -596,8 +597,8 @@ __global__ __launch_bounds__(64 * WAVES) void k_afd3d_2step(Geo g, Step2 a, int 
     constexpr int TXO = TX1 - 4;      // output rows
     constexpr int TZO = 60;           // output z per tile
     static_assert(TXO >= 4, "tile too narrow");
-    __shared__ float lc[2][TX1 + 4][68];  // C_n of plane j: z from zs-4, x from xs-4
-    __shared__ float l1[3][TX1][64];      // step-1 planes j-2, j-1, j (ring)
+    __shared__ float lc[2][TX1 + 4][68];  // C_n of plane j: z from zs-4, x from xs-4 (double buffer)
+    __shared__ float l1[5][TX1][64];      // C_{n+1} of planes j-4 .. j (ring, slot rb holds plane j)
     const int per = gridDim.x >> 3;
     const int lb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
     if (lb >= nblk) return;
@@ -610,31 +611,33 @@ __global__ __launch_bounds__(64 * WAVES) void k_afd3d_2step(Geo g, Step2 a, int 
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int z = zs - 2 + lane;
     const int ldz = g.ldz;
-    // whole-field resources reaching 4 rows + 4 cells around the grid; lane offsets are vector,
-    // the plane offset is the scalar soffset; planes outside the grid use a size-0 resource
+    // Whole-field resources from plane -2 (minus 4 rows + 4 cells) to plane ny2 (plus as much):
+    // plane y is at scalar offset (y + 2) * pstep, lane offsets are vector.  The planes before the
+    // field are its zero guard and every field is zero outside the updated domain, so planes
+    // -2 .. ny2 - 1 load without a mask; beyond the window a load returns 0.
     const int bias = 4 * ldz + 4;
-    const unsigned fbytes = (unsigned)((g.ldxy * g.ny2 + 2 * bias) * 4);
+    const long long gb = 2 * g.ldxy + bias;
+    const unsigned fbytes = (unsigned)((g.ldxy * (g.ny2 + 2) + 2 * bias) * 4);
     const unsigned pstep = (unsigned)(g.ldxy * 4);
-    const auto rC = plane_rsrc(a.cur - bias, fbytes), rP = plane_rsrc(a.pv - bias, fbytes),
-               rV = plane_rsrc(a.vv - bias, fbytes), rN = plane_rsrc(a.cur - bias, 0u),
-               rO1 = plane_rsrc(a.out1 - bias, fbytes), rO2 = plane_rsrc(a.out2 - bias, fbytes);
+    const auto rC = plane_rsrc(a.cur - gb, fbytes), rP = plane_rsrc(a.pv - gb, fbytes),
+               rV = plane_rsrc(a.vv - gb, fbytes), rO1 = plane_rsrc(a.out1 - gb, fbytes),
+               rO2 = plane_rsrc(a.out2 - gb, fbytes);
+    auto pof = [&](int yy) { return (unsigned)(yy + 2) * pstep; };
     const bool zact = z >= 0 && z < g.nz;
     const bool zout = lane >= 2 && lane < 62 && z < g.nz;
     const float fz = a.fz1[z < 0 ? 0 : z];
     const bool zin = z >= 2 && z < g.nz2 - 2;
-    // cur z halo: lanes 0..3 stage zs-4, zs-3, zs+62, zs+63; the others rewrite their centre
+    // C_n z halo: lanes 0..3 stage zs-4, zs-3, zs+62, zs+63; the others rewrite their centre
     const int zh = lane < 2 ? zs - 4 + lane : zs + 60 + lane;
     const int lzh = lane < 4 ? (lane < 2 ? lane : 64 + lane) : lane + 2;
     const int cl = lane < 2 ? 2 : (lane > 61 ? 61 : lane);  // step-2 z neighbours stay in the ring row
     unsigned oc[RPW], ohz[RPW], oown[RPW];
     float fx[RPW];
     bool xin[RPW];
-    int xr[RPW];
 #pragma unroll
     for (int k = 0; k < RPW; ++k) {
         const int r = w * RPW + k;
         const int x = xs - 2 + r;
-        xr[k] = x;
         const bool xact = x >= 0 && x < g.nx;
         oc[k] = (zact && xact) ? (unsigned)(bias + x * ldz + z) * 4u : kOOB;
         ohz[k] = (lane < 4 && zh >= 0 && zh < g.nz && xact) ? (unsigned)(bias + x * ldz + zh) * 4u : kOOB;
@@ -642,7 +645,8 @@ __global__ __launch_bounds__(64 * WAVES) void k_afd3d_2step(Geo g, Step2 a, int 
         fx[k] = a.fx1[x < 0 ? 0 : x];
         xin[k] = x >= 2 && x < g.nx2 - 2;
     }
-    const int xh = w < 2 ? xs - 4 + w : xs - 4 + TX1 + w;  // cur x halo row of waves 0..3
+    auto xrow = [&](int k) { return xs - 2 + w * RPW + k; };
+    const int xh = w < 2 ? xs - 4 + w : xs - 4 + TX1 + w;  // C_n x halo row of waves 0..3
     const int lrh = w < 2 ? w : TX1 + w;
     const unsigned ohx = (w < 4 && xh >= 0 && xh < g.nx && zact) ? (unsigned)(bias + xh * ldz + z) * 4u : kOOB;
     bool blk_src = false;
@@ -657,58 +661,48 @@ __global__ __launch_bounds__(64 * WAVES) void k_afd3d_2step(Geo g, Step2 a, int 
     auto st = [](float v, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
         __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
     };
-    auto yok = [&](int yy) { return yy >= 0 && yy < g.ny; };
     auto fyat = [&](int yy) { return a.fy1[yy < 0 ? 0 : (yy > g.ny2 ? g.ny2 : yy)]; };
 
     // step-1 plane j runs over [y0-2, y1+1]; step 2 trails it by two planes
     const int j0 = y0 - 2, j1 = y1 + 1;
-    float q[RPW][5], s1q[RPW][5], vq[RPW][3];
+    float q[RPW][5], vq[RPW][3];
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
-        const int yy = j0 - 2 + k;
-        const auto rs = yok(yy) ? rC : rN;
-        const unsigned so = yok(yy) ? (unsigned)yy * pstep : 0u;
+        const int yy = j0 - 2 + k;  // planes -4 and -3 lie before the guard: not loaded
+        const bool ok = yy >= -2;
 #pragma unroll
-        for (int r = 0; r < RPW; ++r) q[r][k] = ld(rs, oc[r], so);
+        for (int r = 0; r < RPW; ++r) q[r][k] = ld(rC, ok ? oc[r] : kOOB, ok ? pof(yy) : 0u);
     }
 #pragma unroll
-    for (int r = 0; r < RPW; ++r) {
-#pragma unroll
-        for (int k = 0; k < 5; ++k) s1q[r][k] = 0.f;
-        vq[r][0] = vq[r][1] = vq[r][2] = 0.f;
-    }
+    for (int r = 0; r < RPW; ++r) vq[r][0] = vq[r][1] = vq[r][2] = 0.f;
     float pr[RPW], vc[RPW], hz[RPW], hx;
     float fyq0 = 1.f, fyq1 = 1.f, fyv = fyat(j0);
     {
-        const bool v = yok(j0);
-        const unsigned so = v ? (unsigned)j0 * pstep : 0u;
-        const auto rc = v ? rC : rN, rp = v ? rP : rN, rv = v ? rV : rN;
+        const unsigned so = pof(j0);
 #pragma unroll
         for (int r = 0; r < RPW; ++r) {
-            pr[r] = ld(rp, oc[r], so);
-            vc[r] = ld(rv, oc[r], so);
-            hz[r] = ld(rc, ohz[r], so);
+            pr[r] = ld(rP, oc[r], so);
+            vc[r] = ld(rV, oc[r], so);
+            hz[r] = ld(rC, ohz[r], so);
         }
-        hx = ld(rc, ohx, so);
+        hx = ld(rC, ohx, so);
     }
-    int cb = 0, rb = 0;  // cur double buffer, step-1 ring slot of plane j
+    int cb = 0, rb = 0;  // C_n double buffer; ring slot of step-1 plane j
     for (int j = j0; j <= j1; ++j) {
-        // prefetch plane j+1 (and the centre of j+3)
+        // prefetch plane j+1 (and the centre of j+3): planes >= -1, unmasked (a plane past the
+        // last one needed, on the last iteration, is loaded and not used)
         const int jn = j + 1;
-        const bool nv = jn <= j1 && yok(jn);
-        const bool n3 = jn <= j1 && yok(j + 3);
-        const unsigned s1 = nv ? (unsigned)jn * pstep : 0u, s3 = n3 ? (unsigned)(j + 3) * pstep : 0u;
-        const auto rc1 = nv ? rC : rN, rp1 = nv ? rP : rN, rv1 = nv ? rV : rN, rc3 = n3 ? rC : rN;
+        const unsigned s1 = pof(jn), s3 = pof(j + 3);
         float npr[RPW], nvc[RPW], nhz[RPW], nq[RPW], nhx;
         const float nfy = fyat(jn);
 #pragma unroll
         for (int r = 0; r < RPW; ++r) {
-            nq[r] = ld(rc3, oc[r], s3);
-            npr[r] = ld(rp1, oc[r], s1);
-            nvc[r] = ld(rv1, oc[r], s1);
-            nhz[r] = ld(rc1, ohz[r], s1);
+            nq[r] = ld(rC, oc[r], s3);
+            npr[r] = ld(rP, oc[r], s1);
+            nvc[r] = ld(rV, oc[r], s1);
+            nhz[r] = ld(rC, ohz[r], s1);
         }
-        nhx = ld(rc1, ohx, s1);
+        nhx = ld(rC, ohx, s1);
         float(*C)[68] = lc[cb];
 #pragma unroll
         for (int r = 0; r < RPW; ++r) {
@@ -717,12 +711,11 @@ __global__ __launch_bounds__(64 * WAVES) void k_afd3d_2step(Geo g, Step2 a, int 
         }
         if (w < 4) C[lrh][lane + 2] = hx;
         __syncthreads();
-        // ---- step 1 at plane j
+        // ---- step 1 at plane j (the tile grown by 2 rows and 2 cells), into the ring slot rb
         const bool jown = j >= y0 && j < y1;
         const bool jint = j >= g.nb && j < g.nb + g.ny1;
         const bool yin = j >= 2 && j < g.ny2 - 2;
-        const unsigned sj = jown ? (unsigned)j * pstep : 0u;
-        const auto rw1 = jown ? rO1 : rN;
+        const unsigned sj = pof(j);
         float c1[RPW];
 #pragma unroll
         for (int r = 0; r < RPW; ++r) {
@@ -742,55 +735,64 @@ __global__ __launch_bounds__(64 * WAVES) void k_afd3d_2step(Geo g, Step2 a, int 
         }
         if (blk_src) {
 #pragma unroll
-            for (int r = 0; r < RPW; ++r) c1[r] = inject_at(c1[r], vc[r], a.src, a.wav, a.ns, a.nt, a.it1, z, xr[r], j);
+            for (int r = 0; r < RPW; ++r)
+                c1[r] = inject_at(c1[r], vc[r], a.src, a.wav, a.ns, a.nt, a.it1, z, xrow(r), j);
         }
         if (a.tri1 && rtile && jint) {
 #pragma unroll
-            for (int r = 0; r < RPW; ++r)
-                if (z == g.nb && xr[r] >= g.nb && xr[r] < g.nb + g.nx1)
-                    c1[r] = c1[r] + vc[r] * a.tri1[(j - g.nb) * g.nx1 + (xr[r] - g.nb)];
+            for (int r = 0; r < RPW; ++r) {
+                const int x = xrow(r);
+                if (z == g.nb && x >= g.nb && x < g.nb + g.nx1)
+                    c1[r] = c1[r] + vc[r] * a.tri1[(j - g.nb) * g.nx1 + (x - g.nb)];
+            }
         }
+        float(*R0)[64] = l1[rb];
 #pragma unroll
         for (int r = 0; r < RPW; ++r) {
             c1[r] = c1[r] * fz * fx[r] * fyv;
-            l1[rb][w * RPW + r][lane] = c1[r];
-            st(c1[r], rw1, oown[r], sj);
-            s1q[r][0] = s1q[r][1]; s1q[r][1] = s1q[r][2]; s1q[r][2] = s1q[r][3]; s1q[r][3] = s1q[r][4];
-            s1q[r][4] = c1[r];
+            R0[w * RPW + r][lane] = c1[r];
+            if (jown) st(c1[r], rO1, oown[r], sj);
             vq[r][0] = vq[r][1]; vq[r][1] = vq[r][2]; vq[r][2] = vc[r];
         }
         if (a.rec1 && rtile && jown && jint) {
 #pragma unroll
-            for (int r = 0; r < RPW; ++r)
-                if (oown[r] != kOOB && z == g.nb && xr[r] >= g.nb && xr[r] < g.nb + g.nx1)
-                    a.rec1[(j - g.nb) * g.nx1 + (xr[r] - g.nb)] = c1[r];
+            for (int r = 0; r < RPW; ++r) {
+                const int x = xrow(r);
+                if (oown[r] != kOOB && z == g.nb && x >= g.nb && x < g.nb + g.nx1)
+                    a.rec1[(j - g.nb) * g.nx1 + (x - g.nb)] = c1[r];
+            }
         }
         if (a.snap1 && jown && jint) {
 #pragma unroll
             for (int r = 0; r < RPW; ++r) {
-                const int x = xr[r];
+                const int x = xrow(r);
                 if (oown[r] != kOOB && z >= g.nb && z < g.nb + g.nz1 && x >= g.nb && x < g.nb + g.nx1)
                     a.snap1[((size_t)(j - g.nb) * g.nx1 + (x - g.nb)) * g.nz1 + (z - g.nb)] = c1[r];
             }
         }
-        // ---- step 2 at plane m = j-2 (step-1 planes m-2..m+2 are in the queue; the ring slot of
-        // plane m was written two iterations ago and published by the last barrier)
+        // ---- step 2 at plane m = j-2: its step-1 planes m-2 .. m+2 are ring slots rb+1 .. rb+5 (mod
+        // 5).  The y neighbours are this lane's own ring entries (written by this lane, in order);
+        // the z and x neighbours come from plane m, written two iterations ago by other waves and
+        // published by the last barrier.
         const int m = j - 2;
         if (m >= y0) {
             const float fym = fyq0;
             const bool yin2 = m >= 2 && m < g.ny2 - 2;
             const bool mint = m >= g.nb && m < g.nb + g.ny1;
-            const unsigned sm = (unsigned)m * pstep;
-            const float(*S)[64] = l1[rb == 0 ? 1 : (rb == 1 ? 2 : 0)];  // slot (j-2) % 3 == (rb + 1) % 3
+            const unsigned sm = pof(m);
+            const float(*Sm2)[64] = l1[rb >= 4 ? rb - 4 : rb + 1];  // plane m-2 = j-4
+            const float(*Sm1)[64] = l1[rb >= 3 ? rb - 3 : rb + 2];  // plane m-1
+            const float(*S)[64] = l1[rb >= 2 ? rb - 2 : rb + 3];    // plane m
+            const float(*Sp1)[64] = l1[rb >= 1 ? rb - 1 : rb + 4];  // plane m+1
             float c2[RPW];
 #pragma unroll
             for (int r = 0; r < RPW; ++r) {
                 const int lr = w * RPW + r;
                 if (lr < 2 || lr >= TX1 - 2) { c2[r] = 0.f; continue; }  // uniform per wave
-                const float u = s1q[r][2];
+                const float u = S[lr][lane];
                 float lap = C0 * u * g.cc
-                          + C1 * (s1q[r][1] + s1q[r][3]) * g.idy
-                          + C2 * (s1q[r][0] + s1q[r][4]) * g.idy
+                          + C1 * (Sm1[lr][lane] + Sp1[lr][lane]) * g.idy
+                          + C2 * (Sm2[lr][lane] + R0[lr][lane]) * g.idy
                           + C1 * (S[lr - 1][cl] + S[lr + 1][cl]) * g.idx
                           + C2 * (S[lr - 2][cl] + S[lr + 2][cl]) * g.idx
                           + C1 * (S[lr][cl - 1] + S[lr][cl + 1]) * g.idz
@@ -803,13 +805,15 @@ __global__ __launch_bounds__(64 * WAVES) void k_afd3d_2step(Geo g, Step2 a, int 
             if (blk_src) {
 #pragma unroll
                 for (int r = 0; r < RPW; ++r)
-                    c2[r] = inject_at(c2[r], vq[r][0], a.src, a.wav, a.ns, a.nt, a.it2, z, xr[r], m);
+                    c2[r] = inject_at(c2[r], vq[r][0], a.src, a.wav, a.ns, a.nt, a.it2, z, xrow(r), m);
             }
             if (a.tri2 && rtile && mint) {
 #pragma unroll
-                for (int r = 0; r < RPW; ++r)
-                    if (z == g.nb && xr[r] >= g.nb && xr[r] < g.nb + g.nx1)
-                        c2[r] = c2[r] + vq[r][0] * a.tri2[(m - g.nb) * g.nx1 + (xr[r] - g.nb)];
+                for (int r = 0; r < RPW; ++r) {
+                    const int x = xrow(r);
+                    if (z == g.nb && x >= g.nb && x < g.nb + g.nx1)
+                        c2[r] = c2[r] + vq[r][0] * a.tri2[(m - g.nb) * g.nx1 + (x - g.nb)];
+                }
             }
 #pragma unroll
             for (int r = 0; r < RPW; ++r) {
@@ -818,14 +822,16 @@ __global__ __launch_bounds__(64 * WAVES) void k_afd3d_2step(Geo g, Step2 a, int 
             }
             if (a.rec2 && rtile && mint) {
 #pragma unroll
-                for (int r = 0; r < RPW; ++r)
-                    if (oown[r] != kOOB && z == g.nb && xr[r] >= g.nb && xr[r] < g.nb + g.nx1)
-                        a.rec2[(m - g.nb) * g.nx1 + (xr[r] - g.nb)] = c2[r];
+                for (int r = 0; r < RPW; ++r) {
+                    const int x = xrow(r);
+                    if (oown[r] != kOOB && z == g.nb && x >= g.nb && x < g.nb + g.nx1)
+                        a.rec2[(m - g.nb) * g.nx1 + (x - g.nb)] = c2[r];
+                }
             }
             if (a.snap2 && mint) {
 #pragma unroll
                 for (int r = 0; r < RPW; ++r) {
-                    const int x = xr[r];
+                    const int x = xrow(r);
                     if (oown[r] != kOOB && z >= g.nb && z < g.nb + g.nz1 && x >= g.nb && x < g.nb + g.nx1)
                         a.snap2[((size_t)(m - g.nb) * g.nx1 + (x - g.nb)) * g.nz1 + (z - g.nb)] = c2[r];
                 }
@@ -839,7 +845,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_afd3d_2step(Geo g, Step2 a, int 
         hx = nhx;
         fyq0 = fyq1; fyq1 = fyv; fyv = nfy;
         cb ^= 1;
-        rb = rb == 2 ? 0 : rb + 1;
+        rb = rb == 4 ? 0 : rb + 1;
     }
 }
 
@@ -1295,7 +1301,7 @@ const Variant3 kVariants3[] = {{16, 4, 0}, {8, 4, 0},  {16, 8, 0},  {32, 8, 0}, 
                                {16, 8, 1}, {8, 8, 1},  {16, 4, 1},  {32, 8, 1},  {16, 16, 1}, {32, 16, 1},
                                {16, 8, 2}, {16, 4, 2}, {32, 8, 2},  {8, 4, 2},   {16, 8, 3}, {16, 4, 3},
                                {32, 16, 2}, {8, 8, 2}, {16, 4, 4}, {8, 4, 4}, {16, 8, 4}, {32, 8, 4},
-                               {16, 16, 5}, {16, 8, 5}, {32, 16, 5}, {24, 8, 5}};
+                               {16, 4, 5}, {16, 8, 5}, {24, 8, 5}, {32, 8, 5}};
 constexpr int kNumVariants3 = sizeof(kVariants3) / sizeof(kVariants3[0]);
 constexpr int kLeanDefault = 12;
 constexpr int kLeanOnlyDefault = 6;  // lean kernel when fields exceed the pipelined kernels' 2 GiB windows
@@ -1380,8 +1386,8 @@ long long resident_blocks(pw_afd* p, int vi)
         PW_OCC(18, (k_afd3d_pipe<2, 16, 0, 0>)) PW_OCC(19, (k_afd3d_pipe<1, 8, 0, 0>))
         PW_OCC(20, (k_afd3d_pipe<4, 4, 0, 1>)) PW_OCC(21, (k_afd3d_pipe<2, 4, 0, 1>))
         PW_OCC(22, (k_afd3d_pipe<2, 8, 0, 1>)) PW_OCC(23, (k_afd3d_pipe<4, 8, 0, 1>))
-        PW_OCC(24, (k_afd3d_2step<1, 16>)) PW_OCC(25, (k_afd3d_2step<2, 8>))
-        PW_OCC(26, (k_afd3d_2step<2, 16>)) PW_OCC(27, (k_afd3d_2step<3, 8>))
+        PW_OCC(24, (k_afd3d_2step<4, 4>)) PW_OCC(25, (k_afd3d_2step<2, 8>))
+        PW_OCC(26, (k_afd3d_2step<3, 8>)) PW_OCC(27, (k_afd3d_2step<4, 8>))
 #undef PW_OCC
         default: break;
     }
@@ -1491,10 +1497,10 @@ int afd_launch_2step(pw_afd* p, int vi, const Step2& a, hipEvent_t e0, hipEvent_
         hipExtLaunchKernelGGL((k_afd3d_2step<R_, W_>), dim3(grid), dim3(64 * W_), 0, s, e0, e1, 0, g, a, ntz, ntx, \
                               nyc, nblk);                                                                 \
         break;
-        PW_V2(24, 1, 16)
+        PW_V2(24, 4, 4)
         PW_V2(25, 2, 8)
-        PW_V2(26, 2, 16)
-        PW_V2(27, 3, 8)
+        PW_V2(26, 3, 8)
+        PW_V2(27, 4, 8)
 #undef PW_V2
         default:
             PW_FAIL(ctx, PW_ERR_ARG, "not a two-step 3D kernel variant");
@@ -1582,11 +1588,12 @@ pw_afd* pw_afd_create(pw_ctx* ctx, int ndim, int nz, int nx, int ny, float dz, f
     g.idy = ndim == 3 ? 1 / (dy * dy) : 0.f;
     g.cc = ndim == 3 ? g.idy + g.idx + g.idz : g.idx + g.idz;
     p->field_elems = (size_t)g.ldxy * g.ny2;
-    // 3D fields carry zero guard planes (2 before, 3 plus a tile of rows after) so the lean
-    // kernels can load the radius-2 neighbourhood of any tile cell unconditionally
+    // 3D fields carry zero guard planes (2 plus 4 rows before, 5 plus a tile of rows after) so the
+    // lean and two-step kernels can load the neighbourhood of any tile cell unconditionally (the
+    // two-step kernel reads planes -2 .. ny + 4)
     constexpr int kMaxTx = 32;
-    p->guard_lo = ndim == 3 ? (size_t)(2 * g.ldxy + 128) : 0;
-    const size_t guard_hi = ndim == 3 ? (size_t)(3 * g.ldxy + (kMaxTx + 4) * g.ldz + 128) : 0;
+    p->guard_lo = ndim == 3 ? (size_t)(2 * g.ldxy + 4 * g.ldz + 128) : 0;
+    const size_t guard_hi = ndim == 3 ? (size_t)(5 * g.ldxy + (kMaxTx + 4) * g.ldz + 128) : 0;
     p->raw_elems = p->field_elems + p->guard_lo + guard_hi;
 
     // damping pairs per axis; all six faces use nbt and ct (pywave/src/afd.c:101-116,392-402)
