@@ -77,12 +77,12 @@ def fwi_bench(plain_shots):
     obs = gen.forward(w["vtrue"])  # (nt, nr, ns)
     gen.close()
     out = {"workload": "fwi2d 400x800 at 10 m (+60-cell sponge: 520x920), 32 shots, nt=3000, dt=1 ms, 8 Hz, "
-                       "exact-adjoint L2 gradient, checkpoints every 50 steps"}
+                       "exact-adjoint L2 gradient; the forward fields of every step kept in HBM when they fit "
+                       "(no recomputation), else checkpoints every 50 steps"}
     dev = torch.device("cuda:0")
-    prob = FwiProblem(par, wav, batch=ns, checkpoint=50)
     d = torch.tensor(np.ascontiguousarray(obs.transpose(2, 0, 1)), device=dev)  # (ns, nt, nr)
 
-    def grad_autograd():
+    def grad_autograd(prob):
         v = torch.tensor(w["v0"], device=dev, requires_grad=True)
         tr = fwi_traces(v, prob)
         loss = 0.5 * ((tr.double() - d.double()) ** 2).sum()
@@ -90,20 +90,41 @@ def fwi_bench(plain_shots):
         torch.cuda.synchronize()
         return v.grad, loss.item()
 
-    grad_autograd()  # warm-up (also times the step kernels' tile widths)
-    t = time.perf_counter()
-    g, f = grad_autograd()
-    tb = time.perf_counter() - t
+    def timed_gradient(checkpoint):
+        prob = FwiProblem(par, wav, batch=ns, checkpoint=checkpoint)
+        grad_autograd(prob)  # warm-up (also times the step kernels' tile widths)
+        t = time.perf_counter()
+        g, f = grad_autograd(prob)
+        el = time.perf_counter() - t
+        fw, bw = prob.timing()
+        info = prob.info()
+        return prob, g, f, el, fw, bw, info
+
+    prob, g, f, tb, fw, bw, info = timed_gradient(0)
+    out["batched_checkpoint_steps"] = info["checkpoint_steps"]
+    out["batched_segments"] = info["segments"]
     out["batched_tile_widths"] = prob.widths()
     out["batched_tuning_ms_per_launch"] = prob.tuning_log()
     prob.close()
     out["batched_shots_per_s"] = round(ns / tb, 3)
     out["batched_s_per_gradient"] = round(tb, 4)
+    out["batched_forward_ms"] = round(fw, 1)
+    out["batched_backward_ms"] = round(bw, 1)
     out["misfit"] = f
     out["gradient_norm"] = float(torch.linalg.vector_norm(g.double()).item())
+    # the same gradient with checkpoints every 50 steps (59 of 60 segments recomputed)
+    prob, g50, _, t50, fw50, bw50, _ = timed_gradient(50)
+    prob.close()
+    out["checkpointed_50_shots_per_s"] = round(ns / t50, 3)
+    out["checkpointed_50_forward_ms"] = round(fw50, 1)
+    out["checkpointed_50_backward_ms"] = round(bw50, 1)
+    # the exact adjoint's FMA contraction can differ between tile loops, so the two gradients agree
+    # to f32 rounding rather than bit for bit
+    out["checkpointed_50_rel_l2_vs_stored"] = float(torch.linalg.vector_norm((g - g50).double()).item() /
+                                                    torch.linalg.vector_norm(g.double()).item())
     # the reference's own gradient (gradient_av: imaging condition, windowed residual, first-call
     # scaling, smoothing) that pfwi mode 2 (fwic) iterates on, every shot in one batch
-    ref = FwiProblem(par, wav, data=obs, batch=ns, checkpoint=50)
+    ref = FwiProblem(par, wav, data=obs, batch=ns)
     ref.gradient(w["v0"], "reference")  # warm-up (sets the first-call scaling)
     t = time.perf_counter()
     ref.gradient(w["v0"], "reference")
@@ -116,7 +137,7 @@ def fwi_bench(plain_shots):
     out["reference_gradient_backward_ms"] = round(rb, 1)
     # fwi() (pfwi mode 2's loop, pywave/src/wave_fwi.c:24-119): 5 L-BFGS iterations of the reference
     # gradient with Wolfe line searches, every shot in one batch; misfit of each stored model (f64)
-    lb = FwiProblem(par, wav, data=obs, batch=ns, checkpoint=50)
+    lb = FwiProblem(par, wav, data=obs, batch=ns)
     t = time.perf_counter()
     vinv, k = lb.lbfgs(w["v0"], 5)
     tl = time.perf_counter() - t
@@ -131,7 +152,7 @@ def fwi_bench(plain_shots):
     par1 = dict(par)
     par1["ns"] = plain_shots
     obs1 = np.ascontiguousarray(obs[:, :, :plain_shots])
-    plain = FwiProblem(par1, wav, data=obs1, batch=1, checkpoint=50)
+    plain = FwiProblem(par1, wav, data=obs1, batch=1)
     plain.set_steps_per_launch(1)  # the plain method: one kernel per time step
     plain.gradient(w["v0"], "exact")
     t = time.perf_counter()
@@ -538,8 +559,9 @@ def tri_bench(cpu_steps=12):
     dat = plan.output().reshape(w["nt"], n, n, order="F")
     plan.close()
     ax = [0, w["d"][1], n]
+    launch = {}
     _, ms = tri3d_image(vel, dat, w["groups"], dt=w["dt"], ax=ax, ay=ax, az=ax, nbt=nb, ct=w["ct"],
-                        jsnap=w["jsnap"], return_plan_time=True)
+                        jsnap=w["jsnap"], return_plan_time=True, launch_info=launch)
     G = len(w["groups"])
     cells = (n + 2 * nb) ** 3
     nsnap = (w["nt"] - 1) // w["jsnap"] + 1
@@ -548,7 +570,9 @@ def tri_bench(cpu_steps=12):
            "forward_ms": round(fwd_ms, 2), "backprop_image_ms": round(ms, 2),
            "field_cell_updates_per_s_G": round(G * cells * w["nt"] / (ms * 1e-3) / 1e9, 2),
            "achieved_GBs": round(alg / (ms * 1e-3) / 1e9, 1),
-           "frac_hbm": round(alg / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
+           "frac_hbm": round(alg / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+           "backprop_rows_per_block": launch.get("rows_per_block"), "backprop_y_chunk": launch.get("y_chunk"),
+           "backprop_timing_ms_per_launch": launch.get("timing_ms_per_launch")}
     if cpu_steps > 0:
         out["cpu_baseline"] = tri_cpu_baseline(w, dat, cpu_steps, ms)
     return out

@@ -2281,6 +2281,10 @@ struct pw_tri {
     float *d_fz1 = nullptr, *d_fx1 = nullptr, *d_fy1 = nullptr;
     bool pipe_ok = false;  // single damping factor per axis and 32-bit plane windows
     bool have_vel = false, have_dat = false, ran = false;
+    // pipelined kernel: rows per block (one per wave) and y-chunk length, timed before the first run
+    int tri_w = 8, tri_nyc = 0;
+    bool tuned = false;
+    std::vector<float> tune_log;  // (rows per block, y chunk, ms per launch) of the timing sweep
 };
 
 namespace {
@@ -2299,24 +2303,37 @@ void tri_free(pw_tri* p)
         if (x) hipFree(x);
 }
 
+template <int G, int W>
+void tri_launch_pipe(pw_tri* p, const TriStep<G>& a, hipStream_t s)
+{
+    const Geo& g = p->g;
+    const int ntz = pw::ceil_div(g.nz, 64), ntx = pw::ceil_div(g.nx, W);
+    const long long tiles = (long long)ntz * ntx;
+    int nyc = p->tri_nyc;
+    if (nyc <= 0) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->ctx->device) != hipSuccess || cus < 1)
+            cus = 256;
+        // y chunks of about 8 planes, at least two blocks per CU
+        const int nch = std::max(pw::ceil_div(g.ny, 8), (int)std::max(1LL, (2LL * cus + tiles - 1) / tiles));
+        nyc = std::max(1, pw::ceil_div(g.ny, nch));
+    }
+    const int nch = pw::ceil_div(g.ny, nyc);
+    const int nblk = (int)(tiles * nch);
+    const int grid = pw::ceil_div(nblk, 8) * 8;
+    hipLaunchKernelGGL((k_tri3d_pipe<G, W>), dim3(grid), dim3(64 * W), 0, s, g, a, ntz, ntx, nyc, nblk);
+}
+
 template <int G>
 void tri_launch(pw_tri* p, const TriStep<G>& a, hipStream_t s)
 {
     const Geo& g = p->g;
     if (p->pipe_ok) {
-        constexpr int W = 8;
-        const int ntz = pw::ceil_div(g.nz, 64), ntx = pw::ceil_div(g.nx, W);
-        const long long tiles = (long long)ntz * ntx;
-        int cus = 0;
-        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->ctx->device) != hipSuccess || cus < 1)
-            cus = 256;
-        // y chunks of about 8 planes, at least two blocks per CU
-        int nch = std::max(pw::ceil_div(g.ny, 8), (int)std::max(1LL, (2LL * cus + tiles - 1) / tiles));
-        const int nyc = std::max(1, pw::ceil_div(g.ny, nch));
-        nch = pw::ceil_div(g.ny, nyc);
-        const int nblk = (int)(tiles * nch);
-        const int grid = pw::ceil_div(nblk, 8) * 8;
-        hipLaunchKernelGGL((k_tri3d_pipe<G, W>), dim3(grid), dim3(64 * W), 0, s, g, a, ntz, ntx, nyc, nblk);
+        switch (p->tri_w) {
+            case 4: tri_launch_pipe<G, 4>(p, a, s); break;
+            case 16: tri_launch_pipe<G, 16>(p, a, s); break;
+            default: tri_launch_pipe<G, 8>(p, a, s); break;
+        }
         return;
     }
     const int ntz = pw::ceil_div(g.nz, 64), ntx = pw::ceil_div(g.nx, 16);
@@ -2340,6 +2357,90 @@ __global__ void k_tri_sum_snaps(size_t ncell, int nsnap, const double* __restric
     }
 }
 
+// the launch arguments of step it of pass `pass` (fields A = current, B = previous), no image
+template <int G>
+TriStep<G> tri_step(const pw_tri* p, int pass, int it, float* const* A, float* const* B)
+{
+    const size_t nrec = (size_t)p->g.nx1 * p->g.ny1;
+    TriStep<G> a{};
+    for (int f = 0; f < G; ++f) { a.cur[f] = A[f]; a.pv[f] = B[f]; }
+    a.vv = p->d_vv; a.fz = p->d_fz; a.fx = p->d_fx; a.fy = p->d_fy;
+    a.dat = p->d_dat + (size_t)it * nrec;
+    a.mask = p->d_mask + (size_t)pass * kTriPassGroups * nrec;
+    a.img = nullptr;
+    a.img_mul = pass > 0;
+    a.mbits = p->d_mbits ? p->d_mbits + (size_t)pass * nrec : nullptr;
+    a.fz1 = p->d_fz1; a.fx1 = p->d_fx1; a.fy1 = p->d_fy1;
+    a.pool = p->d_pool;
+    a.pool_bytes = p->pool_bytes;
+    if (p->d_pool)
+        for (int f = 0; f < G; ++f) {
+            a.off_cur[f] = (int)(A[f] - p->d_pool);
+            a.off_pv[f] = (int)(B[f] - p->d_pool);
+        }
+    return a;
+}
+
+// Launch-shape sweep of the pipelined kernel: rows per block 4, 8, 16 (one per wave) and y chunks
+// of 4 to 24 planes, each timed on the last time step of pass 0 (no image) on the plan's own
+// fields; the fastest is kept.  The run that follows clears the fields the timing launches wrote.
+template <int G>
+int tri_tune(pw_tri* p)
+{
+    pw_ctx* ctx = p->ctx;
+    hipStream_t s = ctx->s();
+    float* A[G];
+    float* B[G];
+    for (int f = 0; f < G; ++f) { A[f] = p->d_f[2 * f]; B[f] = p->d_f[2 * f + 1]; }
+    const TriStep<G> a = tri_step<G>(p, 0, p->nt - 1, A, B);
+    hipEvent_t e0, e1;
+    PW_HIP(ctx, hipEventCreate(&e0));
+    PW_HIP(ctx, hipEventCreate(&e1));
+    p->tune_log.clear();
+    float best = 0.f;
+    int bw = 8, bc = 0, rc = PW_OK;
+    for (int W : {4, 8, 16}) {
+        for (int nyc : {4, 6, 8, 12, 16, 24}) {
+            p->tri_w = W;
+            p->tri_nyc = std::min(nyc, p->g.ny);
+            const int reps = 6;
+            for (int r = 0; r < 2; ++r) tri_launch<G>(p, a, s);
+            hipEventRecord(e0, s);
+            for (int r = 0; r < reps; ++r) tri_launch<G>(p, a, s);
+            hipEventRecord(e1, s);
+            float ms = 0.f;
+            if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                ctx->err = "tri: launch-shape timing failed";
+                rc = PW_ERR_HIP;
+                break;
+            }
+            ms /= reps;
+            for (float v : {(float)W, (float)p->tri_nyc, ms}) p->tune_log.push_back(v);
+            if (best == 0.f || ms < best) { best = ms; bw = W; bc = p->tri_nyc; }
+        }
+        if (rc) break;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc) return rc;
+    p->tri_w = bw;
+    p->tri_nyc = bc;
+    p->tuned = true;
+    return PW_OK;
+}
+
+int tri_ensure_tuned(pw_tri* p)
+{
+    if (p->tuned || !p->pipe_ok) return PW_OK;
+    switch (std::min(kTriPassGroups, p->G)) {
+        case 1: return tri_tune<1>(p);
+        case 2: return tri_tune<2>(p);
+        case 3: return tri_tune<3>(p);
+        default: return tri_tune<4>(p);
+    }
+}
+
 // one back-propagation of G groups, the pass-th group of passes (groups pass*kTriPassGroups ...)
 template <int G>
 int tri_enqueue(pw_tri* p, int pass)
@@ -2356,24 +2457,10 @@ int tri_enqueue(pw_tri* p, int pass)
     float* B[G];
     for (int f = 0; f < G; ++f) { A[f] = p->d_f[2 * f]; B[f] = p->d_f[2 * f + 1]; }
     for (int it = p->nt - 1; it >= 0; --it) {
-        TriStep<G> a{};
-        for (int f = 0; f < G; ++f) { a.cur[f] = A[f]; a.pv[f] = B[f]; }
-        a.vv = p->d_vv; a.fz = p->d_fz; a.fx = p->d_fx; a.fy = p->d_fy;
-        a.dat = p->d_dat + (size_t)it * nrec;
-        a.mask = p->d_mask + (size_t)pass * kTriPassGroups * nrec;
+        TriStep<G> a = tri_step<G>(p, pass, it, A, B);
         const bool snap = p->jsnap > 0 && it % p->jsnap == 0;
         if (p->npass == 1) a.img = snap ? p->d_img : nullptr;
         else a.img = snap ? p->d_part + (size_t)(it / p->jsnap) * ncell : nullptr;
-        a.img_mul = pass > 0;
-        a.mbits = p->d_mbits ? p->d_mbits + (size_t)pass * nrec : nullptr;
-        a.fz1 = p->d_fz1; a.fx1 = p->d_fx1; a.fy1 = p->d_fy1;
-        a.pool = p->d_pool;
-        a.pool_bytes = p->pool_bytes;
-        if (p->d_pool)
-            for (int f = 0; f < G; ++f) {
-                a.off_cur[f] = (int)(A[f] - p->d_pool);
-                a.off_pv[f] = (int)(B[f] - p->d_pool);
-            }
         tri_launch<G>(p, a, s);
         PW_HIP(ctx, hipGetLastError());
         for (int f = 0; f < G; ++f) std::swap(A[f], B[f]);
@@ -2536,6 +2623,7 @@ int pw_tri_run(pw_tri* p)
 {
     if (!p) return PW_ERR_ARG;
     if (!p->have_vel || !p->have_dat) PW_FAIL(p->ctx, PW_ERR_STATE, "tri run: velocity or data not set");
+    if (int rc = tri_ensure_tuned(p)) return rc;
     for (int pass = 0; pass < p->npass; ++pass) {
         const int gk = std::min(kTriPassGroups, p->G - pass * kTriPassGroups);
         int rc = PW_ERR_ARG;
@@ -2563,6 +2651,8 @@ int pw_tri_run_timed(pw_tri* p, double* total_ms)
     if (!p) return PW_ERR_ARG;
     pw_ctx* ctx = p->ctx;
     hipStream_t s = ctx->s();
+    if (!p->have_vel || !p->have_dat) PW_FAIL(ctx, PW_ERR_STATE, "tri run: velocity or data not set");
+    if (int rc = tri_ensure_tuned(p)) return rc;  // the launch-shape sweep is not part of the timed run
     hipEvent_t e0, e1;
     PW_HIP(ctx, hipEventCreate(&e0));
     PW_HIP(ctx, hipEventCreate(&e1));
@@ -2577,6 +2667,36 @@ int pw_tri_run_timed(pw_tri* p, double* total_ms)
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     return PW_OK;
+}
+
+// launch shape of the pipelined kernel: rows per block 4, 8 or 16 and a y chunk of at least one
+// plane; rows 0 returns to the timed choice
+int pw_tri_set_launch(pw_tri* p, int rows, int y_chunk)
+{
+    if (!p) return PW_ERR_ARG;
+    if (rows == 0) {
+        p->tuned = false;
+        return PW_OK;
+    }
+    if (!(rows == 4 || rows == 8 || rows == 16) || y_chunk < 1)
+        PW_FAIL(p->ctx, PW_ERR_ARG, "tri set_launch: rows must be 4, 8 or 16 and the y chunk positive");
+    p->tri_w = rows;
+    p->tri_nyc = y_chunk;
+    p->tune_log.clear();
+    p->tuned = true;
+    return PW_OK;
+}
+
+// the pipelined kernel's launch shape: w[0] rows per block, w[1] y-chunk planes (0: not yet
+// chosen or the general kernel runs); the timing sweep's (rows, chunk, ms per launch) triples into
+// log[3*cap]; returns their number
+int pw_tri_launch_choice(pw_tri* p, int* w, float* log, int cap)
+{
+    if (!p) return PW_ERR_ARG;
+    if (w) { w[0] = p->pipe_ok ? p->tri_w : 0; w[1] = p->pipe_ok ? p->tri_nyc : 0; }
+    const int n = (int)p->tune_log.size() / 3;
+    for (int i = 0; i < std::min(n, cap) * 3 && log; ++i) log[i] = p->tune_log[i];
+    return n;
 }
 
 // image (nz*nx*ny, z fastest) in f64, unnormalised

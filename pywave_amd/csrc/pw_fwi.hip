@@ -1259,7 +1259,6 @@ __global__ __launch_bounds__(64) void k_adj2(FGeo<T> g, Adj2Args<T> a, TileMap t
     const bool zhv = lane < 8 && zhz >= 0 && zhz < g.pnz;
     const bool zhin = zhv && zhz >= 4 && zhz < g.pnz - 4;
     const unsigned oz = zok ? (unsigned)(bias + z) * E : kOOB;
-    const unsigned ozi = zin ? oz : kOOB;
     const unsigned ozhi = zhin ? (unsigned)(bias + zhz) * E : kOOB;
     const unsigned oo = own ? oz : kOOB;
     const unsigned oc = (unsigned)L * E;
@@ -1914,6 +1913,10 @@ struct pw_fwi {
     void *A2 = nullptr, *B2 = nullptr;  // the two-step adjoint's output fields (allocated on first use)
     void *adj_sa = nullptr, *adj_sb = nullptr;  // the adjoint fields free during a backward sweep
     double t_forward_ms = 0, t_backward_ms = 0;
+    // device-entry-point timing: events around the last forward_dev (0, 1) and adjoint_dev (2, 3),
+    // read by pw_fwi_timing
+    hipEvent_t ev_dev[4] = {nullptr, nullptr, nullptr, nullptr};
+    bool dev_timed_fwd = false, dev_timed_adj = false;
 };
 
 namespace {
@@ -1962,6 +1965,8 @@ void fwi_free(pw_fwi* p)
                     p->A2, p->B2};
     for (void* b : bufs)
         if (b) hipFree(b);
+    for (hipEvent_t ev : p->ev_dev)
+        if (ev) hipEventDestroy(ev);
 }
 
 #define PW_HIPS(call)                                                                                   \
@@ -1976,6 +1981,7 @@ void fwi_free(pw_fwi* p)
 int dmalloc(pw_fwi* p, void** ptr, size_t bytes)
 {
     if (hipMalloc(ptr, bytes ? bytes : 8) != hipSuccess) {
+        (void)hipGetLastError();  // an out-of-memory result must not surface as a later launch error
         *ptr = nullptr;
         p->ctx->err = "pw_fwi: device allocation of " + std::to_string(bytes) + " bytes failed";
         return PW_ERR_NOMEM;
@@ -1989,9 +1995,14 @@ int plan_storage(pw_fwi* p, int K)
     const Acq& a = p->acq;
     const size_t e = esz(p);
     if (K <= 0) {
-        // one segment when everything fits the budget (no recomputation), else 50 steps
+        // one segment when everything fits the budget (no recomputation), else 50 steps; if the
+        // one-segment storage cannot be allocated after all, 50-step segments
         size_t seg_all = (size_t)a.nt * p->nslot * p->fs * e;
-        K = seg_all + 2 * p->nslot * p->fs * e <= p->budget ? a.nt : 50;
+        const int k_all = seg_all + 2 * p->nslot * p->fs * e <= p->budget ? a.nt : 50;
+        const std::string err = p->ctx->err;
+        if (k_all != 50 && plan_storage(p, k_all) == PW_OK) return PW_OK;
+        p->ctx->err = err;
+        K = 50;
     }
     K = std::max(1, std::min(K, a.nt));
     if (K % a.interval) K += a.interval - K % a.interval;  // segments start at imaging steps
@@ -2768,6 +2779,7 @@ int gradient_all(pw_fwi* p, int mode, double* fcost)
         bwd_ms += m2;
     }
     p->t_forward_ms = fwd_ms;
+    p->dev_timed_fwd = p->dev_timed_adj = false;
     p->t_backward_ms = bwd_ms;
     *fcost = fc;
     p->last_fcost = fc;
@@ -2892,7 +2904,9 @@ pw_fwi* pw_fwi_create(pw_ctx* ctx, const float* pararray, int batch, int precisi
     p->shot_active = p->shot_host;
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = (size_t)8 << 30;
-    p->budget = free_b / 2;
+    // checkpoint and segment storage may take most of the device memory free at creation (the
+    // forward fields of every step stay in HBM when they fit: no recomputation)
+    p->budget = free_b / 10 * 8;
     const size_t e = precision;
     const long long fld = (long long)p->nslot * p->fs;
     const long long tr = (long long)a.nt * a.nr;
@@ -3214,6 +3228,7 @@ int pw_fwi_forward(pw_fwi* p, const float* vel, float* data_out)
         }
     }
     p->t_forward_ms = fwd_ms;
+    p->dev_timed_fwd = p->dev_timed_adj = false;
     p->t_backward_ms = 0;
     return PW_OK;
 }
@@ -3309,6 +3324,9 @@ int pw_fwi_forward_dev(pw_fwi* p, const void* d_vel, void* d_traces)
     int rc = p->prec == 4 ? upload_model<float>(p, (const float*)d_vel, true) : upload_model<double>(p, (const double*)d_vel, true);
     if (!rc) rc = ensure_widths(p);
     if (rc) return rc;
+    if (!p->ev_dev[0])
+        for (hipEvent_t& ev : p->ev_dev) PW_HIPS(hipEventCreate(&ev));
+    PW_HIPS(hipEventRecord(p->ev_dev[0], s));
     const long long per = (long long)a.nt * a.nr;
     const size_t e = esz(p);
     // one batch holds every shot: keep its checkpoints for the adjoint of the same model
@@ -3322,6 +3340,8 @@ int pw_fwi_forward_dev(pw_fwi* p, const void* d_vel, void* d_traces)
                                hipMemcpyDeviceToDevice, s));
     }
     if (keep && (rc = fingerprint(p, d_vel, p->dhash))) return rc;  // of the model the checkpoints belong to
+    PW_HIPS(hipEventRecord(p->ev_dev[1], s));
+    p->dev_timed_fwd = true;
     p->dev_ckpt_valid = keep;
     p->dev_ckpt_vel = d_vel;
     return PW_OK;
@@ -3335,6 +3355,9 @@ int pw_fwi_adjoint_dev(pw_fwi* p, const void* d_vel, const void* d_gtraces, void
     if (p->visco) PW_FAIL(p->ctx, PW_ERR_ARG, "adjoint_dev: the exact adjoint is acoustic (visco plans: reference gradient)");
     const Acq& a = p->acq;
     hipStream_t s = p->ctx->s();
+    if (!p->ev_dev[0])
+        for (hipEvent_t& ev : p->ev_dev) PW_HIPS(hipEventCreate(&ev));
+    PW_HIPS(hipEventRecord(p->ev_dev[2], s));
     // checkpoints of this very model from the preceding forward_dev (one batch): no forward sweep.
     // The same pointer is not enough (a caller may have changed the model in place): the model's
     // fingerprint must equal the one taken by forward_dev.
@@ -3407,6 +3430,8 @@ int pw_fwi_adjoint_dev(pw_fwi* p, const void* d_vel, const void* d_gtraces, void
         PW_HIPS(hipGetLastError());
     }
     PW_HIPS(hipMemcpyAsync(d_gvel, p->gtot, p->nzx * e, hipMemcpyDeviceToDevice, s));
+    PW_HIPS(hipEventRecord(p->ev_dev[3], s));
+    p->dev_timed_adj = true;
     p->dev_ckpt_valid = false;  // the sweep reused the segment buffer: a second adjoint recomputes
     return PW_OK;
 }
@@ -3441,6 +3466,20 @@ int pw_fwi_set_steps_per_launch(pw_fwi* p, int n)
 int pw_fwi_timing(pw_fwi* p, double* forward_ms, double* backward_ms)
 {
     if (!p) return PW_ERR_ARG;
+    // the device entry points' spans (waits for them), when they ran after the host-array calls
+    float ms = 0.f;
+    if (p->dev_timed_fwd) {
+        PW_HIPS(hipEventSynchronize(p->ev_dev[1]));
+        PW_HIPS(hipEventElapsedTime(&ms, p->ev_dev[0], p->ev_dev[1]));
+        p->t_forward_ms = ms;
+        p->dev_timed_fwd = false;
+    }
+    if (p->dev_timed_adj) {
+        PW_HIPS(hipEventSynchronize(p->ev_dev[3]));
+        PW_HIPS(hipEventElapsedTime(&ms, p->ev_dev[2], p->ev_dev[3]));
+        p->t_backward_ms = ms;
+        p->dev_timed_adj = false;
+    }
     if (forward_ms) *forward_ms = p->t_forward_ms;
     if (backward_ms) *backward_ms = p->t_backward_ms;
     return PW_OK;

@@ -29,10 +29,13 @@ def group_masks(nx, ny, groups):
 
 
 def tri3d_image(vel, dat, groups, nt=None, dt=0.001, ax=[0, 20, 81], ay=[0, 20, 81], az=[0, 20, 81], nbt=30,
-                ct=0.01, jsnap=4, normalize=True, return_plan_time=False):
+                ct=0.01, jsnap=4, normalize=True, return_plan_time=False, launch_info=None, launch=None):
     """vel (nz, nx, ny); dat (nt, nx, ny) surface data; groups: list of x-index arrays or (nx, ny)
     boolean masks (up to 64; more than 4 run in passes of 4, with each snapshot's running product
-    kept in HBM).  Returns the image (nz, nx, ny) (divided by its maximum when normalize)."""
+    kept in HBM).  Returns the image (nz, nx, ny) (divided by its maximum when normalize).  A dict
+    passed as launch_info receives the back-propagation kernel's launch shape (rows per block, y
+    chunk) and the timing sweep that chose it; launch=(rows, y_chunk) forces that shape (rows 4, 8 or
+    16; the image is the same bit for bit)."""
     vel = np.asarray(vel, np.float32)
     nz, nx, ny = int(az[2]), int(ax[2]), int(ay[2])
     if vel.shape != (nz, nx, ny):
@@ -55,10 +58,20 @@ def tri3d_image(vel, dat, groups, nt=None, dt=0.001, ax=[0, 20, 81], ay=[0, 20, 
         d = np.ascontiguousarray(dat.ravel(order="F"))
         mk = np.ascontiguousarray(masks)
         N.check(L.pw_tri_set_data(p, N.ptr(d), N.ptr(mk)), ctx, "tri set_data")
+        if launch is not None:
+            N.check(L.pw_tri_set_launch(p, int(launch[0]), int(launch[1])), ctx, "tri set_launch")
         ms = ctypes.c_double(0)
         N.check(L.pw_tri_run_timed(p, ctypes.byref(ms)), ctx, "tri run")
         img = np.empty(nz * nx * ny, np.float64)
         N.check(L.pw_tri_image(p, N.ptr(img), img.size), ctx, "tri image")
+        if launch_info is not None:
+            w = np.zeros(2, np.int32)
+            n = L.pw_tri_launch_choice(p, N.ptr(w), None, 0)
+            buf = np.zeros(3 * max(n, 1), np.float32)
+            L.pw_tri_launch_choice(p, N.ptr(w), N.ptr(buf), n)
+            launch_info.update(rows_per_block=int(w[0]), y_chunk=int(w[1]),
+                               timing_ms_per_launch=[(int(buf[3 * i]), int(buf[3 * i + 1]),
+                                                      round(float(buf[3 * i + 2]), 4)) for i in range(n)])
     finally:
         L.pw_tri_destroy(p)
     img = img.reshape(nz, nx, ny, order="F")

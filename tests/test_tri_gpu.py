@@ -118,3 +118,21 @@ def test_tri_image_more_than_four_groups(ng):
     print("tri %d groups: rel-L2 against the f64 recipe %.2e" % (ng, rel_l2(img, ref)))
     assert np.isfinite(img).all() and img.max() > 0
     assert rel_l2(img, ref) <= 1e-5
+
+
+def test_every_launch_shape_gives_the_same_image():
+    # rows per block 4, 8, 16 and several y chunks: the same per-cell arithmetic in the same order,
+    # so the images are equal bit for bit; the timed choice is one of the swept shapes
+    vel, dat, groups = _case(seed=11, n=(40, 36, 30), nt=60, ng=4)
+    nz, nx, ny = vel.shape
+    kw = dict(dt=0.001, ax=[0, 10, nx], ay=[0, 10, ny], az=[0, 10, nz], nbt=8, ct=0.04, jsnap=3, normalize=False)
+    info = {}
+    ref = pywave_amd.tri3d_image(vel, dat, groups, launch_info=info, **kw)
+    assert info["rows_per_block"] in (4, 8, 16) and 1 <= info["y_chunk"] <= 24
+    assert len(info["timing_ms_per_launch"]) == 18
+    assert (info["rows_per_block"], info["y_chunk"]) in [(r, c) for r, c, _ in info["timing_ms_per_launch"]]
+    for rows, chunk in [(4, 3), (8, 7), (16, 5), (16, 46)]:
+        img = pywave_amd.tri3d_image(vel, dat, groups, launch=(rows, chunk), **kw)
+        assert np.array_equal(img, ref), (rows, chunk)
+    with pytest.raises(ValueError):
+        pywave_amd.tri3d_image(vel, dat, groups, launch=(5, 4), **kw)
