@@ -59,7 +59,7 @@ def cpu_baseline(w, nt_cpu):
         return dict(value=None, unit="Gcell-updates/s", cores=1, kind="reference", sample="failed: %s" % e)
 
 
-def fwi_bench(plain_shots):
+def fwi_bench(plain_shots, widths=None):
     """FWI gradient throughput on the 400x800, 32-shot, nt=3000 workload (8 Hz Ricker, 60-cell
     sponge): the batched gradient -- all 32 shots per launch, checkpoints every 50 steps --
     taken through torch.autograd (fwi_traces + L2 misfit + backward, i.e. the hand-written
@@ -92,6 +92,7 @@ def fwi_bench(plain_shots):
 
     def timed_gradient(checkpoint):
         prob = FwiProblem(par, wav, batch=ns, checkpoint=checkpoint)
+        fwi_set_widths(prob, widths)
         grad_autograd(prob)  # warm-up (also times the step kernels' tile widths)
         t = time.perf_counter()
         g, f = grad_autograd(prob)
@@ -316,9 +317,9 @@ def lstri_bench():
     src_inv, its = p.invert(data, return_iters=True)
     inv_s = time.perf_counter() - t
     inv_dev_ms = p.timing()
-    (inv_w, inv_spl), inv_log = p.launch_choice()
+    (inv_w, inv_spl, inv_cols), inv_log = p.launch_choice()
     fit = p.model(src_inv)
-    (mod_w, mod_spl), mod_log = p.launch_choice()
+    (mod_w, mod_spl, mod_cols), mod_log = p.launch_choice()
     resid = float(np.linalg.norm(fit - data) / np.linalg.norm(data))
     n_sweep_steps = 2 * nt * int(np.sum(its))
     out = {"workload": "lstri2d 130x334 (+60 decay: 250x454), nt=3001, 3 events x 6 buried sources, "
@@ -330,9 +331,9 @@ def lstri_bench():
            "invert_us_per_sweep_step": round(inv_dev_ms * 1e3 / max(n_sweep_steps, 1), 3),
            "cg_iterations": [int(x) for x in its], "relative_residual": round(resid, 4),
            "launch_choice": {"invert_one_shot": {"tile_width": inv_w, "steps_per_launch": inv_spl,
-                                                 "timing_ms_per_launch": inv_log},
+                                                 "four_step_columns": inv_cols, "timing_ms_per_launch": inv_log},
                              "model_all_shots": {"tile_width": mod_w, "steps_per_launch": mod_spl,
-                                                 "timing_ms_per_launch": mod_log}}}
+                                                 "four_step_columns": mod_cols, "timing_ms_per_launch": mod_log}}}
     p.close()
     try:
         from oracle import refrun
@@ -412,7 +413,13 @@ def aps3d_bench(cpu_steps=20):
     return res
 
 
-def fwi_forward_bench(nt=300):
+def fwi_set_widths(prob, widths):
+    """widths: None (the plan's timing sweep) or (forward, exact adjoint, reference adjoint)."""
+    if widths:
+        prob.set_widths(forward=widths[0], exact_adjoint=widths[1], reference_adjoint=widths[2])
+
+
+def fwi_forward_bench(nt=300, widths=None):
     """The acoustic forward of the FWI workload (400x800 +60 sponge, 32 shots in one launch per
     step) over its first nt steps, twice: a short run of the FWI step kernel for counter passes."""
     from pywave_amd import FwiProblem
@@ -422,6 +429,7 @@ def fwi_forward_bench(nt=300):
     par = w["par"]
     wav = (w["amp"] * ricker_wavelet(nt, par["dt"], w["f0"], w["t0"])).astype(np.float32)
     p = FwiProblem(par, wav, batch=par["ns"])
+    fwi_set_widths(p, widths)
     p.forward(w["vtrue"])
     p.forward(w["vtrue"])
     ms = p.timing()[0]
@@ -432,10 +440,11 @@ def fwi_forward_bench(nt=300):
             "achieved_GBs": round(16.0 * cells / (ms * 1e-3 / nt) / 1e9, 1)}
 
 
-def fwi_gradient_bench(nt=300):
-    """The FWI workload's 32 shots (400x800 +60 sponge, all shots per launch, checkpoints every 50
-    steps) over its first nt steps: the exact-adjoint gradient and the reference gradient
-    (gradient_av), two of each -- short runs of the adjoint kernels for counter passes."""
+def fwi_gradient_bench(nt=300, widths=None):
+    """The FWI workload's 32 shots (400x800 +60 sponge, all shots per launch, every step's forward
+    fields kept in HBM as in the full workload) over its first nt steps: the exact-adjoint gradient
+    and the reference gradient (gradient_av), two of each -- short runs of the adjoint kernels for
+    counter passes."""
     from pywave_amd import FwiProblem
     from pywave_amd.workloads import fwi2d
 
@@ -445,8 +454,10 @@ def fwi_gradient_bench(nt=300):
     gen = FwiProblem(par, wav, batch=par["ns"])
     obs = gen.forward(w["vtrue"])
     gen.close()
-    p = FwiProblem(par, wav, data=obs, batch=par["ns"], checkpoint=50)
-    out = {"workload": "fwi2d gradients, 32 shots, first %d steps, checkpoints every 50 steps" % nt}
+    p = FwiProblem(par, wav, data=obs, batch=par["ns"])
+    fwi_set_widths(p, widths)
+    out = {"workload": "fwi2d gradients, 32 shots, first %d steps, forward fields kept in HBM" % nt,
+           "widths": p.widths()}
     for mode in ("exact", "reference"):
         for _ in range(2):
             p.gradient(w["v0"], mode)
@@ -741,6 +752,10 @@ def main():
     ap.add_argument("--no-visco", action="store_true")
     ap.add_argument("--no-big", action="store_true")
     ap.add_argument("--fwi-plain-shots", type=int, default=4)
+    ap.add_argument("--fwi-widths", type=lambda t: tuple(int(x) for x in t.split(",")), default=None,
+                    help="forward,exact-adjoint,reference-adjoint tile widths of the batched FWI plans "
+                         "(default: each plan's timing sweep); for counter runs, where the sweep sees "
+                         "serialised launches")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (HBM counter calibration)")
     ap.add_argument("--only", choices=["fwi", "tri", "batch", "small", "lstri", "aps", "visco", "fwifwd", "fwigrad",
@@ -756,12 +771,13 @@ def main():
         calibrate_copy()
         return
     if args.only:
-        leg = {"fwi": lambda: fwi_bench(args.fwi_plain_shots), "tri": lambda: tri_bench(0 if args.no_cpu else 12),
+        leg = {"fwi": lambda: fwi_bench(args.fwi_plain_shots, args.fwi_widths), "tri": lambda: tri_bench(0 if args.no_cpu else 12),
                "batch": batch_bench,
                "small": afd2d_small_bench, "lstri": lstri_bench,
                "aps": lambda: aps3d_bench(0 if args.no_cpu else 20),
                "visco": lambda: visco_bench(0 if args.no_cpu else 300),
-               "fwifwd": fwi_forward_bench, "fwigrad": fwi_gradient_bench, "big3d": big3d_bench}[args.only]
+               "fwifwd": lambda: fwi_forward_bench(widths=args.fwi_widths),
+               "fwigrad": lambda: fwi_gradient_bench(widths=args.fwi_widths), "big3d": big3d_bench}[args.only]
         print(json.dumps({args.only: leg()}))
         return
 

@@ -193,6 +193,11 @@ int pw_fwi_widths(pw_fwi *p, int *w);
  * per launch where they are faster), 1 (one kernel per time step, the plain method) or -2 (two
  * steps per launch whatever the timing; acoustic point-source plans) */
 int pw_fwi_set_steps_per_launch(pw_fwi *p, int n);
+/* force the step kernels' tile widths instead of timing them: w[0] forward, w[1] visco forward,
+ * w[2] exact adjoint, w[3] reference adjoint, w[4] visco reference adjoint, each 4, 8 or 16 (one
+ * step per launch); NULL returns to the timing sweep.  Identical results at every width except the
+ * exact adjoint's FMA contraction (f32 rounding).  No counterpart in the reference. */
+int pw_fwi_set_widths(pw_fwi *p, const int *w);
 /* the plan's last width-timing sweep: triples (kernel, width, ms per launch) written to out (up to
  * cap of them); kernel 0 forward, 1 visco forward, 2 exact adjoint, 3 reference adjoint, 4 visco
  * reference adjoint, 5 two-step forward, 6 two-step exact adjoint.  Returns the number of triples. */
@@ -246,18 +251,22 @@ int pw_lstri_invert(pw_lstri *p, const float *data, int niter, float *src, int *
 /* on != 0 (default): the inversion replays its two sweeps from hipGraphs captured on the first
  * call; 0 launches them step by step.  Results are identical.  No counterpart in the reference. */
 int pw_lstri_use_graph(pw_lstri *p, int on);
-/* time steps per sweep launch: 1, 2, or 0 for the plan's choice (timed before its first sweep of
- * each shot count: single-step tile widths 1-16 and two-step launches on the plan's own grid);
- * identical results either way.  No counterpart in the reference (a scheduling choice of this
+/* time steps per sweep launch: 1, 2, 4, or 0 for the plan's choice (timed before its first sweep
+ * of each shot count: single-step tile widths 1-16, two-step launches and four-step launches of
+ * 4- and 8-column tiles on the plan's own grid); identical results either way.  No counterpart in the reference (a scheduling choice of this
  * engine). */
 int pw_lstri_steps_per_launch(pw_lstri *p, int n);
 /* single-step tile width (x columns per 64-cell tile): 1, 2, 4, 8, 16, or 0 for the plan's timed
  * choice; identical results either way.  No counterpart in the reference. */
 int pw_lstri_set_tile_width(pw_lstri *p, int w);
-/* the last sweep's launch choice: w[0] single-step tile width, w[1] steps per launch; the timing
- * sweep's (kernel, width, ms per launch) triples into log[3*cap] (kernel 0 modelling step, 1
- * adjoint step, 2 two-step modelling launch, 3 two-step adjoint launch); returns their number.
- * No counterpart in the reference. */
+/* output columns of the four-step launches' tiles: 4, 8, or 0 for the timed choice (identical
+ * results).  No counterpart in the reference. */
+int pw_lstri_set_four_step_columns(pw_lstri *p, int c);
+/* the last sweep's launch choice: w[0] single-step tile width, w[1] steps per launch, w[2] the
+ * four-step tiles' output columns (3 ints); the timing sweep's (kernel, width, ms per launch)
+ * triples into log[3*cap] (kernel 0 modelling step, 1 adjoint step, 2 two-step modelling launch,
+ * 3 two-step adjoint launch, 4 four-step modelling launch, 5 four-step adjoint launch); returns
+ * their number.  No counterpart in the reference. */
 int pw_lstri_launch_choice(pw_lstri *p, int *w, float *log, int cap);
 /* device milliseconds of the last pw_lstri_model / pw_lstri_invert call */
 int pw_lstri_timing(pw_lstri *p, double *ms);

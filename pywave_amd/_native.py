@@ -81,6 +81,7 @@ SIGNATURES = {
     "pw_fwi_timing": (c_int, [c_vp, ctypes.POINTER(c_double), ctypes.POINTER(c_double)]),
     "pw_fwi_widths": (c_int, [c_vp, c_vp]),
     "pw_fwi_set_steps_per_launch": (c_int, [c_vp, c_int]),
+    "pw_fwi_set_widths": (c_int, [c_vp, c_vp]),
     "pw_fwi_tuning_log": (c_int, [c_vp, c_vp, c_int]),
     "pw_fwi_lbfgs": (c_int, [c_vp, c_vp, c_int, c_vp, ctypes.POINTER(c_int)]),
     "pw_forward_modeling_ac": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_ll]),
@@ -97,6 +98,7 @@ SIGNATURES = {
     "pw_lstri_steps_per_launch": (c_int, [c_vp, c_int]),
     "pw_lstri_launch_choice": (c_int, [c_vp, c_vp, c_vp, c_int]),
     "pw_lstri_set_tile_width": (c_int, [c_vp, c_int]),
+    "pw_lstri_set_four_step_columns": (c_int, [c_vp, c_int]),
     "pw_lstric": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_ll, c_ll]),
     "pw_tri_create": (c_vp, [c_vp, c_int, c_int, c_int, c_float, c_float, c_float, c_int, c_float, c_int, c_float,
                              c_int, c_int]),

@@ -44,6 +44,29 @@ __device__ __forceinline__ double bldo<double>(__amdgpu_buffer_rsrc_t r, unsigne
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
 }
 
+// non-temporal (streamed once: not kept in the caches for reuse) load and store of an f32 or f64
+template <typename T>
+__device__ __forceinline__ T bldo_nt(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff);
+template <>
+__device__ __forceinline__ float bldo_nt<float>(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 2));
+}
+template <>
+__device__ __forceinline__ double bldo_nt<double>(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 2));
+}
+__device__ __forceinline__ void bsto_nt(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, float v)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 2);
+}
+__device__ __forceinline__ void bsto_nt(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, double v)
+{
+    using u32x2 = unsigned __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, 2);
+}
+
 __device__ __forceinline__ void bsto(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, float v)
 {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);

@@ -155,6 +155,8 @@ using pwbuf::win;
 using pwbuf::bld;
 using pwbuf::bldo;
 using pwbuf::bsto;
+using pwbuf::bldo_nt;
+using pwbuf::bsto_nt;
 
 
 // ---------------------------------------------------------------- forward step (+ modelling)
@@ -308,7 +310,7 @@ __global__ __launch_bounds__(64) void k_fwd(FGeo<T> g, StepArgs<T> a, TileMap tm
                               (C::K0 * p1 + C::K1 * (R[1] + R[-1]) + C::K2 * (R[2] + R[-2]) +
                                C::K3 * (R[3] + R[-3]) + C::K4 * (R[4] + R[-4])) / g.dz2;
                 T term = keep_if(lap, zin);
-                if constexpr (TERM) bsto(rTo, oz, (unsigned)x * oc, term);
+                if constexpr (TERM) bsto_nt(rTo, oz, (unsigned)x * oc, term);  // read once, by the adjoint
                 const T xl = cur.xl, xh = cur.xh;
                 if constexpr (VISCO) {
                     const float tf = (float)term;
@@ -404,7 +406,7 @@ __global__ __launch_bounds__(64) void k_fwd(FGeo<T> g, StepArgs<T> a, TileMap tm
         }
         if (fsrc_rows && fzin && fxin(x)) term = term + cur.f;
         if constexpr (!VISCO)
-            if (a.full && zok) a.full[slot * a.fulls + (long long)x * g.ldz + z] = term;
+            if (a.full && zok) __builtin_nontemporal_store(term, &a.full[slot * a.fulls + (long long)x * g.ldz + z]);
         const T p0 = sponge(yv, zl, zh, xl, xh);
         T p2 = 2 * p1 - p0 + mv * term;
         bsto(rY, oz, (unsigned)x * oc, sponge(p2, zl, zh, xl, xh));
@@ -1101,7 +1103,7 @@ __global__ __launch_bounds__(64) void k_adj_exact(FGeo<T> g, StepArgs<T> a, ExAr
             c.a = bldo<T>(rA, oz, (unsigned)xx * oc);
             c.b = bldo<T>(rB, oz, (unsigned)xx * oc);
             c.g = bldo<T>(rG, oz, (unsigned)xx * oc);
-            c.t = bldo<T>(rF, oz, (unsigned)xx * oc);
+            c.t = bldo_nt<T>(rF, oz, (unsigned)xx * oc);  // the stored term, read once
             c.ha = bldo<T>(rM, ozhi, (unsigned)xx * oc) * bldo<T>(rA, ozhi, (unsigned)xx * oc);
             c.xl = sxp[2 * (xx - x0)];
             c.xh = sxp[2 * (xx - x0) + 1];
@@ -1909,6 +1911,7 @@ struct pw_fwi {
     bool force_fused = false;  // two steps per launch whether or not they time faster
     std::vector<float> tune_log;  // (kernel, width, ms per launch) of the last timing sweep
     bool widths_tuned = false;
+    bool widths_forced = false;  // pw_fwi_set_widths: no timing sweep
     void *X2 = nullptr, *Y2 = nullptr;  // the two-step forward's output fields (allocated on first use)
     void *A2 = nullptr, *B2 = nullptr;  // the two-step adjoint's output fields (allocated on first use)
     void *adj_sa = nullptr, *adj_sb = nullptr;  // the adjoint fields free during a backward sweep
@@ -3057,7 +3060,7 @@ namespace {
 int use_source_field(pw_fwi* p, bool on)
 {
     p->dev_ckpt_valid = false;
-    p->widths_tuned = false;  // the tiles that run the general loop change
+    if (!p->widths_forced) p->widths_tuned = false;  // the tiles that run the general loop change
     const Acq& a = p->acq;
     if (!on) {
         p->passive = false;
@@ -3136,7 +3139,7 @@ int pw_fwi_set_visco(pw_fwi* p, const float* q)
 {
     if (!p) return PW_ERR_ARG;
     p->dev_ckpt_valid = false;
-    p->widths_tuned = false;  // other kernels run
+    if (!p->widths_forced) p->widths_tuned = false;  // other kernels run
     if (!q) {
         p->visco = false;
         return PW_OK;
@@ -3454,12 +3457,34 @@ int pw_fwi_tuning_log(pw_fwi* p, float* out, int cap)
     return n;
 }
 
+// force the step kernels' tile widths: w[0] forward, w[1] visco forward, w[2] exact adjoint, w[3]
+// reference adjoint, w[4] visco reference adjoint (4, 8 or 16 each), one step per launch; NULL
+// returns to the plan's timing sweep
+int pw_fwi_set_widths(pw_fwi* p, const int* w)
+{
+    if (!p) return PW_ERR_ARG;
+    if (!w) {
+        p->widths_tuned = p->widths_forced = false;
+        return PW_OK;
+    }
+    for (int i = 0; i < 5; ++i) {
+        bool ok = false;
+        for (int W : kWidths) ok |= w[i] == W;
+        if (!ok) PW_FAIL(p->ctx, PW_ERR_ARG, "pw_fwi_set_widths: widths must be 4, 8 or 16");
+    }
+    p->w_fwd = w[0]; p->w_fwdv = w[1]; p->w_exact = w[2]; p->w_ref = w[3]; p->w_refv = w[4];
+    p->w_fwd2 = p->w_adj2 = 0;
+    p->tune_log.clear();
+    p->widths_tuned = p->widths_forced = true;
+    return PW_OK;
+}
+
 int pw_fwi_set_steps_per_launch(pw_fwi* p, int n)
 {
     if (!p || !(n == 1 || n == 2 || n == -2)) return PW_ERR_ARG;
     p->max_fused = n < 0 ? -n : n;
     p->force_fused = n < 0;
-    p->widths_tuned = false;
+    if (!p->widths_forced) p->widths_tuned = false;
     return PW_OK;
 }
 

@@ -455,6 +455,233 @@ __global__ __launch_bounds__(T2) void k_tr2(TriGeo g, TriArgs2 a, int ntz2)
     }
 }
 
+// ---------------------------------------------------------------- NS time steps per launch
+// The two-step scheme carried to NS steps: a block owns an OZN x OX tile and stages the current
+// level on the tile grown by 4*NS cells.  Step k (1..NS) forms its level on the tile grown by
+// 4*(NS-k), reading the level before it through the stencil and the one before that pointwise, so
+// two LDS level buffers suffice: A holds the even levels (the staged current level, then 2, ...),
+// B the odd ones (the previous level as staged, then 1, 3, ...), and a step overwrites, cell by
+// cell, the level it reads pointwise.  Every cell uses the single-step kernels' expressions, so each
+// level equals NS single steps bit for bit.  The last two levels go to the spare fields C (step
+// NS-1) and D (step NS).
+constexpr int OZN = 56;  // output z per tile
+constexpr int TN = 256;  // threads per block
+
+template <int NS>
+struct TriArgsN {
+    const float* X;       // current level (modelling: u1; adjoint: A)
+    const float* Y;       // previous level (decayed once)
+    float* C;             // out: level NS-1 (the next launch's previous level)
+    float* D;             // out: level NS (the next launch's current level)
+    const float* v2;
+    const float* sfz;
+    const float* sfx;
+    float* plane[NS];     // per step: modelling source plane (read); adjoint image plane (written)
+    long long pss;
+    float* row[NS];       // per step: modelling recorded row (written); adjoint injected row (read)
+    long long rss;
+};
+
+// cells of the region of step k (the tile grown by 4*(NS-k))
+template <int NS, int OX>
+__host__ __device__ constexpr int trn_zr(int k) { return OZN + 8 * (NS - k); }
+template <int NS, int OX>
+__host__ __device__ constexpr int trn_xr(int k) { return OX + 8 * (NS - k); }
+template <int NS, int OX>
+__host__ __device__ constexpr int trn_sp_off(int k)  // offset of step k's source plane in the staged planes
+{
+    return k <= 1 ? 0 : trn_sp_off<NS, OX>(k - 1) + trn_zr<NS, OX>(k - 1) * trn_xr<NS, OX>(k - 1);
+}
+
+template <bool ADJ, int NS, int OX>
+__global__ __launch_bounds__(TN) void k_trn(TriGeo g, TriArgsN<NS> a, int ntz)
+{
+    static_assert(NS >= 2, "at least two steps");
+    constexpr int G = 4 * NS;
+    constexpr int ZA = OZN + 2 * G, XA = OX + 2 * G;              // staged region R_0
+    constexpr int ZB = OZN + 2 * (G - 4), XB = OX + 2 * (G - 4);  // R_1
+    constexpr int NSP = ADJ ? 1 : trn_sp_off<NS, OX>(NS + 1);     // staged source planes
+    __shared__ float LA[XA][ZA + 1];                 // even levels, on R_0 coordinates
+    __shared__ float LB[XB][ZB + 1];                 // the staged previous level, then the odd levels (R_1)
+    __shared__ float PA[ADJ ? XA : 1][ZA + 1];       // adjoint: level x v^2dt^2
+    __shared__ float PB[ADJ ? XB : 1][ZB + 1];
+    __shared__ float VV[XA][ZA + 1];                 // v^2dt^2 on R_0
+    __shared__ float SP[NSP];                        // modelling: each step's source plane on its region
+    __shared__ float fz[ZA][2], fx[XA][2];           // decay factor pairs of R_0's rows and columns
+    __shared__ float rd[ADJ ? NS : 1][XB];           // adjoint: each step's injected row over R_1's columns
+    const int slot = blockIdx.y;
+    const int tz = blockIdx.x % ntz, tx = blockIdx.x / ntz;
+    const int zo = tz * OZN, xo = tx * OX;
+    const long long so = slot * g.fs;
+    const float* __restrict__ X = a.X + so;
+    const float* __restrict__ Y = a.Y + so;
+    auto ing = [&](int z, int x) { return z >= 0 && z < g.pz && x >= 0 && x < g.px; };
+    auto inr = [&](int z, int x) { return z >= 4 && z < g.pz - 4 && x >= 4 && x < g.px - 4; };
+    auto mz = [&](int z) { return z >= g.nb && z < g.nb + g.nz; };
+    auto mx = [&](int x) { return x >= g.nb && x < g.nb + g.nx; };
+    auto own = [&](int z, int x) { return z >= zo && z < zo + OZN && x >= xo && x < xo + OX; };
+    auto mi = [&](int z, int x) { return slot * a.pss + (long long)(x - g.nb) * g.nz + (z - g.nb); };
+    // ---- stage: current level, v^2dt^2 (and their product) on R_0, the previous level on R_1,
+    // the decay pairs, and each step's source plane (modelling) or injected row (adjoint).  Every
+    // global operand is loaded into registers first (all loads in flight together), then written
+    // to LDS, so the launch waits on memory once.
+    constexpr int N0 = (ZA * XA + TN - 1) / TN, N1 = (ZB * XB + TN - 1) / TN, NP = (NSP + TN - 1) / TN;
+    float r0[N0], rv[N0], r1[N1], rp[ADJ ? 1 : NP];
+#pragma unroll
+    for (int j = 0; j < N0; ++j) {
+        const int e = threadIdx.x + j * TN;
+        const int zz = e % ZA, xx = e / ZA;
+        const int z = zo - G + zz, x = xo - G + xx;
+        const bool in = e < ZA * XA && ing(z, x);
+        const long long k = (long long)x * g.ldz + z;
+        r0[j] = in ? X[k] : 0.f;
+        rv[j] = in ? a.v2[k] : 0.f;
+    }
+#pragma unroll
+    for (int j = 0; j < N1; ++j) {
+        const int e = threadIdx.x + j * TN;
+        const int zz = e % ZB, xx = e / ZB;
+        const int z = zo - G + 4 + zz, x = xo - G + 4 + xx;
+        r1[j] = (e < ZB * XB && ing(z, x)) ? Y[(long long)x * g.ldz + z] : 0.f;
+    }
+    if constexpr (!ADJ) {
+        // flat index over the steps' regions back to back: step k's region at trn_sp_off(k)
+#pragma unroll
+        for (int j = 0; j < NP; ++j) {
+            const int e = threadIdx.x + j * TN;
+            int k = 1;
+            while (k < NS && e >= trn_sp_off<NS, OX>(k + 1)) ++k;
+            const int zr = trn_zr<NS, OX>(k), el = e - trn_sp_off<NS, OX>(k);
+            const int zz = el % zr, xx = el / zr;
+            const int z = zo - 4 * (NS - k) + zz, x = xo - 4 * (NS - k) + xx;
+            rp[j] = (e < NSP && ing(z, x) && mz(z) && mx(x)) ? a.plane[k - 1][mi(z, x)] : 0.f;
+        }
+    } else {
+        for (int e = threadIdx.x; e < NS * XB; e += TN) {
+            const int k = e / XB, xx = e % XB;
+            const int x = xo - G + 4 + xx;
+            rd[k][xx] = (x >= 0 && x < g.px && mx(x)) ? a.row[k][slot * a.rss + (x - g.nb)] : 0.f;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < N0; ++j) {
+        const int e = threadIdx.x + j * TN;
+        if (e < ZA * XA) {
+            const int zz = e % ZA, xx = e / ZA;
+            LA[xx][zz] = r0[j];
+            VV[xx][zz] = rv[j];
+            if constexpr (ADJ) PA[xx][zz] = r0[j] * rv[j];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < N1; ++j) {
+        const int e = threadIdx.x + j * TN;
+        if (e < ZB * XB) LB[e / ZB][e % ZB] = r1[j];
+    }
+    if constexpr (!ADJ) {
+#pragma unroll
+        for (int j = 0; j < NP; ++j) {
+            const int e = threadIdx.x + j * TN;
+            if (e < NSP) SP[e] = rp[j];
+        }
+    }
+    for (int t = threadIdx.x; t < ZA + XA; t += TN) {
+        if (t < ZA) {
+            const int z = zo - G + t;
+            const bool zv = z >= 0 && z < g.pz;
+            fz[t][0] = zv ? a.sfz[2 * z] : 1.f;
+            fz[t][1] = zv ? a.sfz[2 * z + 1] : 1.f;
+        } else {
+            const int u = t - ZA, x = xo - G + u;
+            const bool xv = x >= 0 && x < g.px;
+            fx[u][0] = xv ? a.sfx[2 * x] : 1.f;
+            fx[u][1] = xv ? a.sfx[2 * x + 1] : 1.f;
+        }
+    }
+    __syncthreads();
+    // ---- steps 1..NS
+#pragma unroll
+    for (int k = 1; k <= NS; ++k) {
+        const int zr = trn_zr<NS, OX>(k), xr = trn_xr<NS, OX>(k);
+        const int da = 4 * k, db = 4 * (k - 1);  // R_k's offsets in R_0 and R_1 coordinates
+        const bool odd = k & 1;                  // level k goes to B (odd) or A (even)
+        for (int e = threadIdx.x; e < zr * xr; e += TN) {
+            const int zz = e % zr, xx = e / zr;
+            const int z = zo - 4 * (NS - k) + zz, x = xo - 4 * (NS - k) + xx;
+            const int za = zz + da, xa = xx + da, zb = zz + db, xb = xx + db;
+            float v = 0.f, pv = 0.f;
+            if (ing(z, x)) {
+                const float zl = fz[za][0], zh = fz[za][1], xl = fx[xa][0], xh = fx[xa][1];
+                const float vv = VV[xa][za];
+                // the level before this step (stencil) and the one before that (pointwise)
+                const float u1 = odd ? LA[xa][za] : LB[xb][zb];
+                const float prev = odd ? LB[xb][zb] : LA[xa][za];
+                float u2 = 0.f;  // the 4-cell rim stays zero
+                if constexpr (!ADJ) {
+                    if (g.depth == z && mx(x) && own(z, x))  // record the level before the step
+                        a.row[k - 1][slot * a.rss + (x - g.nb)] = 0.f + u1;
+                    if (inr(z, x)) {
+                        float lx, lz;
+                        if (odd) {
+                            lx = W4 * (LA[xa + 4][za] + LA[xa - 4][za]) + W3 * (LA[xa + 3][za] + LA[xa - 3][za]) +
+                                 W2 * (LA[xa + 2][za] + LA[xa - 2][za]) + W1 * (LA[xa + 1][za] + LA[xa - 1][za]) +
+                                 W0 * u1;
+                            lz = W4 * (LA[xa][za + 4] + LA[xa][za - 4]) + W3 * (LA[xa][za + 3] + LA[xa][za - 3]) +
+                                 W2 * (LA[xa][za + 2] + LA[xa][za - 2]) + W1 * (LA[xa][za + 1] + LA[xa][za - 1]) +
+                                 W0 * u1;
+                        } else {
+                            lx = W4 * (LB[xb + 4][zb] + LB[xb - 4][zb]) + W3 * (LB[xb + 3][zb] + LB[xb - 3][zb]) +
+                                 W2 * (LB[xb + 2][zb] + LB[xb - 2][zb]) + W1 * (LB[xb + 1][zb] + LB[xb - 1][zb]) +
+                                 W0 * u1;
+                            lz = W4 * (LB[xb][zb + 4] + LB[xb][zb - 4]) + W3 * (LB[xb][zb + 3] + LB[xb][zb - 3]) +
+                                 W2 * (LB[xb][zb + 2] + LB[xb][zb - 2]) + W1 * (LB[xb][zb + 1] + LB[xb][zb - 1]) +
+                                 W0 * u1;
+                        }
+                        u2 = (lx * g.sx + lz * g.sz) * vv + 2.0f * u1 - decay(prev, zl, zh, xl, xh);
+                    }
+                    if (mz(z) && mx(x)) u2 = u2 + SP[trn_sp_off<NS, OX>(k) + e] * vv;
+                } else {
+                    const float c = odd ? PA[xa][za] : PB[xb][zb];  // the level before, x v^2dt^2
+                    if (mz(z) && mx(x) && own(z, x)) a.plane[k - 1][mi(z, x)] = 0.f + c;  // this step's image
+                    if (inr(z, x)) {
+                        float lx, lz;
+                        if (odd) {
+                            lx = W4 * (PA[xa + 4][za] + PA[xa - 4][za]) + W3 * (PA[xa + 3][za] + PA[xa - 3][za]) +
+                                 W2 * (PA[xa + 2][za] + PA[xa - 2][za]) + W1 * (PA[xa + 1][za] + PA[xa - 1][za]) +
+                                 W0 * c;
+                            lz = W4 * (PA[xa][za + 4] + PA[xa][za - 4]) + W3 * (PA[xa][za + 3] + PA[xa][za - 3]) +
+                                 W2 * (PA[xa][za + 2] + PA[xa][za - 2]) + W1 * (PA[xa][za + 1] + PA[xa][za - 1]) +
+                                 W0 * c;
+                        } else {
+                            lx = W4 * (PB[xb + 4][zb] + PB[xb - 4][zb]) + W3 * (PB[xb + 3][zb] + PB[xb - 3][zb]) +
+                                 W2 * (PB[xb + 2][zb] + PB[xb - 2][zb]) + W1 * (PB[xb + 1][zb] + PB[xb - 1][zb]) +
+                                 W0 * c;
+                            lz = W4 * (PB[xb][zb + 4] + PB[xb][zb - 4]) + W3 * (PB[xb][zb + 3] + PB[xb][zb - 3]) +
+                                 W2 * (PB[xb][zb + 2] + PB[xb][zb - 2]) + W1 * (PB[xb][zb + 1] + PB[xb][zb - 1]) +
+                                 W0 * c;
+                        }
+                        u2 = (lx * g.sx + lz * g.sz) + 2.0f * u1 - decay(prev, zl, zh, xl, xh);
+                    }
+                    if (z == g.depth && mx(x)) u2 = u2 + rd[k - 1][xb];
+                }
+                v = decay(u2, zl, zh, xl, xh);
+                if constexpr (ADJ) pv = v * vv;
+                if (k >= NS - 1 && own(z, x)) (k == NS ? a.D : a.C)[so + (long long)x * g.ldz + z] = v;
+            }
+            if (k < NS) {  // over the level this cell read pointwise (no other cell reads it now)
+                if (odd) {
+                    LB[xb][zb] = v;
+                    if constexpr (ADJ) PB[xb][zb] = pv;
+                } else {
+                    LA[xa][za] = v;
+                    if constexpr (ADJ) PA[xa][za] = pv;
+                }
+            }
+        }
+        if (k < NS) __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------------- small kernels
 __global__ void k_tri_pad_v2(TriGeo g, const float* __restrict__ vel, float dt2, float* __restrict__ v2)
 {
@@ -595,14 +822,16 @@ struct pw_lstri {
     float *sfz = nullptr, *sfx = nullptr;
     float* fields = nullptr;  // 4 * cap_slots fields (two levels, and two for the two-step launches)
     int cap_slots = 0;
-    int two_step = 0;         // sweeps advance two steps per launch
-    // launch choice per shots-per-sweep count: (single-step tile width, two steps per launch),
-    // from timing both sweeps' launches on the plan's own grid and buffers
-    std::map<int, std::pair<int, int>> choice;
-    int spl_forced = 0;       // 0: the timed choice; 1 or 2 steps per launch
+    int spl = 1;              // time steps per sweep launch: 1, 2 or 4
+    int oxn = 4;              // output columns of the four-step launches' tiles (4 or 8)
+    // launch choice per shots-per-sweep count: {single-step tile width, steps per launch, four-step
+    // tile columns}, from timing both sweeps' launches on the plan's own grid and buffers
+    std::map<int, std::vector<int>> choice;
+    int spl_forced = 0;       // 0: the timed choice; 1, 2 or 4 steps per launch
     int xct_forced = 0;       // 0: the timed choice; else this single-step tile width
+    int oxn_forced = 0;       // 0: the timed choice; else the four-step tiles' output columns
     std::vector<float> tune_log;  // (kernel, width, ms per launch) of the last timing sweep
-    int graph_xct = 0, graph_two = -1;  // the launch choice the captured CG sweeps hold
+    int graph_xct = 0, graph_spl = -1, graph_oxn = 0;  // the launch choice the captured CG sweeps hold
     float *src = nullptr, *dat = nullptr, *stage = nullptr;  // batch buffers (modelling / imaging)
     int cap_src_slots = 0;
     float *x = nullptr, *gvec = nullptr, *S = nullptr, *rr = nullptr, *gg = nullptr, *Ss = nullptr;
@@ -683,6 +912,21 @@ void step2(pw_lstri* p, bool adj, int nslot, const TriArgs2& a)
     else hipLaunchKernelGGL(k_tr2<false>, grid, dim3(T2), 0, p->ctx->s(), p->g, a, ntz2);
 }
 
+// four steps per launch (k_trn): a.X, a.Y in; a.C, a.D out
+void step4(pw_lstri* p, bool adj, int nslot, const TriArgsN<4>& a)
+{
+    const int ntz = (p->g.pz + OZN - 1) / OZN, ntx = (p->g.px + p->oxn - 1) / p->oxn;
+    dim3 grid(ntz * ntx, nslot);
+    hipStream_t s = p->ctx->s();
+    if (p->oxn == 8) {
+        if (adj) hipLaunchKernelGGL((k_trn<true, 4, 8>), grid, dim3(TN), 0, s, p->g, a, ntz);
+        else hipLaunchKernelGGL((k_trn<false, 4, 8>), grid, dim3(TN), 0, s, p->g, a, ntz);
+    } else {
+        if (adj) hipLaunchKernelGGL((k_trn<true, 4, 4>), grid, dim3(TN), 0, s, p->g, a, ntz);
+        else hipLaunchKernelGGL((k_trn<false, 4, 4>), grid, dim3(TN), 0, s, p->g, a, ntz);
+    }
+}
+
 void step(pw_lstri* p, bool adj, int nslot, const TriArgs& a)
 {
     switch (p->xct) {
@@ -733,7 +977,7 @@ int select_launch(pw_lstri* p, int nslot)
         auto hit = p->ctx->tuned2d.find(key);
         if (hit != p->ctx->tuned2d.end()) {
             p->tune_log = hit->second.second;
-            own = p->choice.emplace(nslot, std::make_pair(hit->second.first[0], hit->second.first[1])).first;
+            own = p->choice.emplace(nslot, hit->second.first).first;
         } else {
             int rc = ensure_fields(p, nslot);
             if (!rc) rc = ensure_batch(p, nslot);
@@ -744,8 +988,8 @@ int select_launch(pw_lstri* p, int nslot)
             TriArgs a{};
             a.X = p->fields; a.Y = p->fields + fl; a.v2 = p->v2; a.sfz = p->sfz; a.sfx = p->sfx;
             a.plane = p->src; a.pss = p->nm; a.row = p->dat; a.rss = p->nd;
-            float best = 0.f;
-            int wbest = 1;
+            float best = 0.f;  // per time step
+            int wbest = 1, sbest = 1, obest = 4;
             for (int W : {1, 2, 4, 8, 16}) {
                 p->xct = W;
                 float tm = 0.f, ta = 0.f;
@@ -754,7 +998,6 @@ int select_launch(pw_lstri* p, int nslot)
                 for (float v : {0.f, (float)W, tm, 1.f, (float)W, ta}) p->tune_log.push_back(v);
                 if (W == 1 || tm + ta < best) { best = tm + ta; wbest = W; }
             }
-            int two = 0;
             if (p->nt >= 2) {  // the two-step launch reads planes and rows it and it + 1
                 TriArgs2 b{};
                 b.X = p->fields; b.Y = p->fields + fl; b.C = p->fields + 2 * fl; b.D = p->fields + 3 * fl;
@@ -765,14 +1008,33 @@ int select_launch(pw_lstri* p, int nslot)
                 if ((rc = time_launches(p, [&] { step2(p, false, nslot, b); }, &tm))) return rc;
                 if ((rc = time_launches(p, [&] { step2(p, true, nslot, b); }, &ta))) return rc;
                 for (float v : {2.f, (float)OX2, tm, 3.f, (float)OX2, ta}) p->tune_log.push_back(v);
-                two = tm + ta < 2.f * best;
+                if ((tm + ta) / 2.f < best) { best = (tm + ta) / 2.f; sbest = 2; }
             }
-            own = p->choice.emplace(nslot, std::make_pair(wbest, two)).first;
-            p->ctx->tuned2d[key] = {{wbest, two}, p->tune_log};
+            if (p->nt >= 4) {  // the four-step launches read planes and rows it .. it + 3
+                TriArgsN<4> b{};
+                b.X = p->fields; b.Y = p->fields + fl; b.C = p->fields + 2 * fl; b.D = p->fields + 3 * fl;
+                b.v2 = p->v2; b.sfz = p->sfz; b.sfx = p->sfx;
+                b.pss = p->nm; b.rss = p->nd;
+                for (int k = 0; k < 4; ++k) {
+                    b.plane[k] = p->src + k * p->nzx;
+                    b.row[k] = p->dat + k * p->nx;
+                }
+                for (int ox : {4, 8}) {
+                    p->oxn = ox;
+                    float tm = 0.f, ta = 0.f;
+                    if ((rc = time_launches(p, [&] { step4(p, false, nslot, b); }, &tm))) return rc;
+                    if ((rc = time_launches(p, [&] { step4(p, true, nslot, b); }, &ta))) return rc;
+                    for (float v : {4.f, (float)ox, tm, 5.f, (float)ox, ta}) p->tune_log.push_back(v);
+                    if ((tm + ta) / 4.f < best) { best = (tm + ta) / 4.f; sbest = 4; obest = ox; }
+                }
+            }
+            own = p->choice.emplace(nslot, std::vector<int>{wbest, sbest, obest}).first;
+            p->ctx->tuned2d[key] = {own->second, p->tune_log};
         }
     }
-    p->xct = p->xct_forced ? p->xct_forced : own->second.first;
-    p->two_step = p->spl_forced ? p->spl_forced == 2 : own->second.second;
+    p->xct = p->xct_forced ? p->xct_forced : own->second[0];
+    p->spl = p->spl_forced ? p->spl_forced : own->second[1];
+    p->oxn = p->oxn_forced ? p->oxn_forced : own->second[2];
     return PW_OK;
 }
 
@@ -788,7 +1050,22 @@ int sweep_model(pw_lstri* p, int nslot, const float* w, float* d)
     float* C = p->fields + 2 * fl;
     float* D = p->fields + 3 * fl;
     int it0 = 0;
-    if (p->two_step) {
+    if (p->spl == 4) {
+        for (; it0 + 3 < p->nt; it0 += 4) {
+            TriArgsN<4> a{};
+            a.X = X; a.Y = Y; a.C = C; a.D = D; a.v2 = p->v2; a.sfz = p->sfz; a.sfx = p->sfx;
+            a.pss = p->nm;
+            a.rss = p->nd;
+            for (int k = 0; k < 4; ++k) {
+                a.plane[k] = const_cast<float*>(w) + (long long)(it0 + k) * p->nzx;
+                a.row[k] = d + (long long)(it0 + k) * p->nx;
+            }
+            step4(p, false, nslot, a);
+            std::swap(X, D);  // the fourth step's level is current, the third step's previous
+            std::swap(Y, C);
+        }
+    }
+    if (p->spl >= 2) {
         for (; it0 + 1 < p->nt; it0 += 2) {
             TriArgs2 a{};
             a.X = X; a.Y = Y; a.C = C; a.D = D; a.v2 = p->v2; a.sfz = p->sfz; a.sfx = p->sfx;
@@ -829,7 +1106,22 @@ int sweep_adjoint(pw_lstri* p, int nslot, const float* d, float* w)
     float* C = p->fields + 2 * fl;
     float* D = p->fields + 3 * fl;
     int it0 = p->nt - 1;
-    if (p->two_step) {
+    if (p->spl == 4) {
+        for (; it0 >= 3; it0 -= 4) {
+            TriArgsN<4> a{};
+            a.X = A; a.Y = B; a.C = C; a.D = D; a.v2 = p->v2; a.sfz = p->sfz; a.sfx = p->sfx;
+            a.pss = p->nm;
+            a.rss = p->nd;
+            for (int k = 0; k < 4; ++k) {
+                a.plane[k] = w + (long long)(it0 - k) * p->nzx;
+                a.row[k] = const_cast<float*>(d) + (long long)(it0 - k) * p->nx;
+            }
+            step4(p, true, nslot, a);
+            std::swap(A, D);
+            std::swap(B, C);
+        }
+    }
+    if (p->spl >= 2) {
         for (; it0 >= 1; it0 -= 2) {
             TriArgs2 a{};
             a.X = A; a.Y = B; a.C = C; a.D = D; a.v2 = p->v2; a.sfz = p->sfz; a.sfx = p->sfx;
@@ -1111,12 +1403,13 @@ int pw_lstri_invert(pw_lstri* p, const float* data, int niter, float* src, int* 
         if (rc) return rc;
     }
     if ((rc = ensure_batch(p, 1)) || (rc = ensure_fields(p, 1)) || (rc = select_launch(p, 1))) return rc;
-    if (p->graph_xct != p->xct || p->graph_two != p->two_step) drop_graphs(p);
+    if (p->graph_xct != p->xct || p->graph_spl != p->spl || p->graph_oxn != p->oxn) drop_graphs(p);
     const long long nm = p->nm, nd = p->nd;
     const bool graph = p->use_graph && s != nullptr;
     if (graph && !p->x_adj) {
         p->graph_xct = p->xct;
-        p->graph_two = p->two_step;
+        p->graph_spl = p->spl;
+        p->graph_oxn = p->oxn;
         LS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         rc = sweep_adjoint(p, 1, p->rr, p->gvec);
         hipError_t e = hipStreamEndCapture(s, &p->g_adj);
@@ -1217,12 +1510,20 @@ int pw_lstri_use_graph(pw_lstri* p, int on)
     return PW_OK;
 }
 
-// time steps per launch of the sweeps: 1, 2, or 0 for the plan's timed choice; the results are
-// the same bit for bit
+// time steps per launch of the sweeps: 1, 2, 4, or 0 for the plan's timed choice; the results
+// are the same bit for bit
 int pw_lstri_steps_per_launch(pw_lstri* p, int n)
 {
-    if (!p || n < 0 || n > 2) return PW_ERR_ARG;
+    if (!p || !(n == 0 || n == 1 || n == 2 || n == 4)) return PW_ERR_ARG;
     p->spl_forced = n;
+    return PW_OK;
+}
+
+// output columns of the four-step launches' tiles: 4, 8, or 0 for the plan's timed choice
+int pw_lstri_set_four_step_columns(pw_lstri* p, int c)
+{
+    if (!p || !(c == 0 || c == 4 || c == 8)) return PW_ERR_ARG;
+    p->oxn_forced = c;
     return PW_OK;
 }
 
@@ -1234,13 +1535,14 @@ int pw_lstri_set_tile_width(pw_lstri* p, int w)
     return PW_OK;
 }
 
-// the launch choice of the last sweep: w[0] single-step tile width, w[1] steps per launch (1, 2);
-// the timing sweep's (kernel, width, ms per launch) triples into log[3*cap] (kernel 0 modelling
-// step, 1 adjoint step, 2 two-step modelling, 3 two-step adjoint); returns their number
+// the launch choice of the last sweep: w[0] single-step tile width, w[1] steps per launch (1, 2, 4),
+// w[2] the four-step tiles' output columns; the timing sweep's (kernel, width, ms per launch)
+// triples into log[3*cap] (kernel 0 modelling step, 1 adjoint step, 2 two-step modelling, 3
+// two-step adjoint, 4 four-step modelling, 5 four-step adjoint); returns their number
 int pw_lstri_launch_choice(pw_lstri* p, int* w, float* log, int cap)
 {
     if (!p) return PW_ERR_ARG;
-    if (w) { w[0] = p->xct; w[1] = p->two_step ? 2 : 1; }
+    if (w) { w[0] = p->xct; w[1] = p->spl; w[2] = p->oxn; }
     const int n = (int)p->tune_log.size() / 3;
     for (int i = 0; i < std::min(n, cap) * 3 && log; ++i) log[i] = p->tune_log[i];
     return n;

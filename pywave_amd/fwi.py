@@ -143,6 +143,16 @@ class FwiProblem:
         self._L.pw_fwi_tuning_log(self._p, N.ptr(buf), n)
         return [(names[int(buf[3 * i])], int(buf[3 * i + 1]), round(float(buf[3 * i + 2]), 4)) for i in range(n)]
 
+    def set_widths(self, forward=8, exact_adjoint=8, reference_adjoint=8, visco_forward=8,
+                   visco_reference_adjoint=8, timed=False):
+        """Force the step kernels' tile widths (4, 8 or 16 columns, one step per launch) instead of
+        the plan's timing sweep; timed=True returns to the sweep."""
+        if timed:
+            N.check(self._L.pw_fwi_set_widths(self._p, None), self._ctx, "set_widths")
+            return
+        w = np.array([forward, visco_forward, exact_adjoint, reference_adjoint, visco_reference_adjoint], np.int32)
+        N.check(self._L.pw_fwi_set_widths(self._p, N.ptr(w)), self._ctx, "set_widths")
+
     def set_steps_per_launch(self, n):
         """Forward time steps one launch may advance: 2 (default: two-step launches where the plan's
         timing sweep finds them faster), 1 (one kernel per time step, the plain method) or -2 (two

@@ -93,7 +93,7 @@ class LstriProblem:
         N.check(self._L.pw_lstri_use_graph(self._p, int(bool(on))), self._ctx, "use_graph")
 
     def steps_per_launch(self, n=0):
-        """Time steps per sweep launch: 1, 2, or 0 for the plan's choice (timed on the plan's grid
+        """Time steps per sweep launch: 1, 2, 4, or 0 for the plan's choice (timed on the plan's grid
         before its first sweep).  The results do not depend on it."""
         N.check(self._L.pw_lstri_steps_per_launch(self._p, int(n)), self._ctx, "steps_per_launch")
 
@@ -101,17 +101,22 @@ class LstriProblem:
         """Single-step tile width: 1, 2, 4, 8 or 16 columns, or 0 for the plan's timed choice."""
         N.check(self._L.pw_lstri_set_tile_width(self._p, int(w)), self._ctx, "set_tile_width")
 
+    def set_four_step_columns(self, c=0):
+        """Output columns of the four-step launches' tiles: 4, 8, or 0 for the timed choice."""
+        N.check(self._L.pw_lstri_set_four_step_columns(self._p, int(c)), self._ctx, "set_four_step_columns")
+
     def launch_choice(self):
-        """(single-step tile width, steps per launch) of the last sweep, and the timing sweep's
-        [(kernel, width, ms per launch), ...]."""
-        names = ["model_step", "adjoint_step", "two_step_model", "two_step_adjoint"]
-        w = np.zeros(2, np.int32)
+        """(single-step tile width, steps per launch, four-step tile columns) of the last sweep, and
+        the timing sweep's [(kernel, width, ms per launch), ...]."""
+        names = ["model_step", "adjoint_step", "two_step_model", "two_step_adjoint", "four_step_model",
+                 "four_step_adjoint"]
+        w = np.zeros(3, np.int32)
         n = self._L.pw_lstri_launch_choice(self._p, N.ptr(w), None, 0)
         N.check(min(n, 0), self._ctx, "launch_choice")
         buf = np.zeros(3 * max(n, 1), np.float32)
         self._L.pw_lstri_launch_choice(self._p, N.ptr(w), N.ptr(buf), n)
         log = [(names[int(buf[3 * i])], int(buf[3 * i + 1]), round(float(buf[3 * i + 2]), 4)) for i in range(n)]
-        return (int(w[0]), int(w[1])), log
+        return (int(w[0]), int(w[1]), int(w[2])), log
 
     def timing(self):
         """Device milliseconds of the last model() / invert() call."""

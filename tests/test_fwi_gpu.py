@@ -333,3 +333,28 @@ def test_two_step_forward_odd_nt_and_marine_geometry(golden):
         p.close()
     assert np.array_equal(res[0], res[1])
     assert np.array_equal(res[1], g["data"][:299])
+
+
+def test_forced_widths_give_the_reference_results(small):
+    # every forced tile width: forward data and the reference gradient bit-exact against the
+    # reference C, the exact gradient within f32 rounding of the timed widths' one
+    par, wav, vt, v0 = _fd_case()
+    obs = FwiProblem(par, wav).forward(vt)
+    v0 = v0.astype(np.float32)
+    ge, _, _ = FwiProblem(par, wav, data=obs, batch=2).gradient(v0, "exact")
+    for wd in (4, 8, 16):
+        p = FwiProblem(small["par"], small["wav"], data=small["data"], batch=1)
+        p.set_widths(forward=wd, exact_adjoint=wd, reference_adjoint=wd)
+        assert np.array_equal(p.forward(small["vtrue"]), small["data"]), wd
+        g1, _, _ = p.gradient(small["v0"], "reference")
+        assert np.array_equal(g1, small["grad1"]), wd
+        w = p.widths()
+        assert (w["forward"], w["exact_adjoint"], w["reference_adjoint"]) == (wd, wd, wd) and w["tuned"]
+        q = FwiProblem(par, wav, data=obs, batch=2)
+        q.set_widths(forward=wd, exact_adjoint=wd, reference_adjoint=wd)
+        gq, _, _ = q.gradient(v0, "exact")
+        assert rel_l2(gq, ge) <= 1e-6, wd
+        q.close()
+        p.close()
+    with pytest.raises(ValueError):
+        FwiProblem(small["par"], small["wav"], batch=1).set_widths(forward=5)
