@@ -583,6 +583,7 @@ def tri_bench(cpu_steps=12):
            "achieved_GBs": round(alg / (ms * 1e-3) / 1e9, 1),
            "frac_hbm": round(alg / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
            "backprop_rows_per_block": launch.get("rows_per_block"), "backprop_y_chunk": launch.get("y_chunk"),
+           "backprop_non_temporal": launch.get("non_temporal"),
            "backprop_timing_ms_per_launch": launch.get("timing_ms_per_launch")}
     if cpu_steps > 0:
         out["cpu_baseline"] = tri_cpu_baseline(w, dat, cpu_steps, ms)

@@ -292,13 +292,14 @@ int pw_tri_set_data(pw_tri *p, const float *dat, const unsigned char *masks);
 int pw_tri_run(pw_tri *p);
 int pw_tri_run_timed(pw_tri *p, double *total_ms);
 /* launch shape of the back-propagation kernel, timed before the first run (rows per block 4, 8 or
- * 16; y chunks of 4 to 24 planes): w[0] rows per block, w[1] y-chunk planes; the timing sweep's
- * (rows, chunk, ms per launch) triples into log[3*cap]; returns their number.  No counterpart in
- * the reference. */
+ * 16; y chunks of 4 to 24 planes; default or non-temporal cache policy on the streamed operands):
+ * w[0] rows per block, w[1] y-chunk planes, w[2] non-temporal (3 ints); the timing sweep's (rows,
+ * chunk, non-temporal, ms per launch) quadruples into log[4*cap]; returns their number.  No
+ * counterpart in the reference. */
 int pw_tri_launch_choice(pw_tri *p, int *w, float *log, int cap);
-/* force the launch shape (rows 4, 8 or 16, y chunk >= 1; identical results), or rows = 0 for the
- * timed choice.  No counterpart in the reference. */
-int pw_tri_set_launch(pw_tri *p, int rows, int y_chunk);
+/* force the launch shape (rows 4, 8 or 16, y chunk >= 1, non_temporal 0 or 1; identical results),
+ * or rows = 0 for the timed choice.  No counterpart in the reference. */
+int pw_tri_set_launch(pw_tri *p, int rows, int y_chunk, int non_temporal);
 /* unnormalised f64 image, nz*nx*ny z fastest */
 int pw_tri_image(pw_tri *p, double *img, long long len);
 void pw_tri_destroy(pw_tri *p);

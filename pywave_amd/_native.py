@@ -107,7 +107,7 @@ SIGNATURES = {
     "pw_tri_run": (c_int, [c_vp]),
     "pw_tri_run_timed": (c_int, [c_vp, ctypes.POINTER(c_double)]),
     "pw_tri_launch_choice": (c_int, [c_vp, c_vp, c_vp, c_int]),
-    "pw_tri_set_launch": (c_int, [c_vp, c_int, c_int]),
+    "pw_tri_set_launch": (c_int, [c_vp, c_int, c_int, c_int]),
     "pw_tri_image": (c_int, [c_vp, c_vp, c_ll]),
     "pw_tri_destroy": (None, [c_vp]),
     "pw_batch2d_create": (c_vp, [c_vp, c_int, c_vp, c_vp, c_int, c_float]),

@@ -1116,10 +1116,12 @@ __global__ __launch_bounds__(256) void k_tri3d(Geo g, TriStep<G> a, int ntz, int
 // cell) and the centre columns of plane y+3 are in flight while plane y is computed; cells outside
 // the updated domain, receivers off the row and the image outside the model are masked through the
 // buffer range check.  TX = WAVES rows per block, one per wave.
-template <int G, int WAVES>
+template <int G, int WAVES, int NT>
 __global__ __launch_bounds__(64 * WAVES) void k_tri3d_pipe(Geo g, TriStep<G> a, int ntz, int ntx, int nyc,
                                                            int nblk)
 {
+    constexpr int AUX = NT ? 2 : 0;  // non-temporal policy on the streamed-once operands (previous
+                                     // levels, vv, the new levels' stores) when NT
     static_assert(WAVES >= 4, "one x-halo row per wave for waves 0..3");
     constexpr int TZ = 64, TX = WAVES, LW = TZ + 4;
     __shared__ float lds[G][2][TX + 4][LW];
@@ -1221,11 +1223,11 @@ __global__ __launch_bounds__(64 * WAVES) void k_tri3d_pipe(Geo g, TriStep<G> a, 
         for (int f = 0; f < G; ++f) {
             const unsigned oc_f = (unsigned)a.off_cur[f] * 4u, op_f = (unsigned)a.off_pv[f] * 4u;
             nq[f] = ld(r3, oc, s3 + oc_f);
-            npr[f] = ld(r1, oc, s1 + op_f);
+            npr[f] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r1, oc, s1 + op_f, AUX));
             nhz[f] = ld(r1, ohz, s1 + oc_f);
             nhx[f] = ld(r1, ohx, s1 + oc_f);
         }
-        const float nvc = ld(nv ? rV : rN, oc, s1);
+        const float nvc = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(nv ? rV : rN, oc, s1, AUX));
         const float ndv = ld_dat(yn);
         const unsigned nbits = ld_bits(yn);
         const double nim = ld_img(yn);
@@ -1268,7 +1270,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_tri3d_pipe(Geo g, TriStep<G> a, 
         for (int f = 0; f < G; ++f) {
             c[f] = c[f] * fz * fx * fyv;
             __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, c[f]), rF, oc,
-                                                  s0 + (unsigned)a.off_pv[f] * 4u, 0);
+                                                  s0 + (unsigned)a.off_pv[f] * 4u, AUX);
             prod = f == 0 ? c[f] * c[f] : prod * c[f] * c[f];
         }
         if (a.img) {
@@ -2282,9 +2284,9 @@ struct pw_tri {
     bool pipe_ok = false;  // single damping factor per axis and 32-bit plane windows
     bool have_vel = false, have_dat = false, ran = false;
     // pipelined kernel: rows per block (one per wave) and y-chunk length, timed before the first run
-    int tri_w = 8, tri_nyc = 0;
+    int tri_w = 8, tri_nyc = 0, tri_nt = 0;  // tri_nt: non-temporal policy on the streamed operands
     bool tuned = false;
-    std::vector<float> tune_log;  // (rows per block, y chunk, ms per launch) of the timing sweep
+    std::vector<float> tune_log;  // (rows per block, y chunk, non-temporal, ms per launch) of the timing sweep
 };
 
 namespace {
@@ -2303,7 +2305,7 @@ void tri_free(pw_tri* p)
         if (x) hipFree(x);
 }
 
-template <int G, int W>
+template <int G, int W, int NT>
 void tri_launch_pipe(pw_tri* p, const TriStep<G>& a, hipStream_t s)
 {
     const Geo& g = p->g;
@@ -2321,7 +2323,7 @@ void tri_launch_pipe(pw_tri* p, const TriStep<G>& a, hipStream_t s)
     const int nch = pw::ceil_div(g.ny, nyc);
     const int nblk = (int)(tiles * nch);
     const int grid = pw::ceil_div(nblk, 8) * 8;
-    hipLaunchKernelGGL((k_tri3d_pipe<G, W>), dim3(grid), dim3(64 * W), 0, s, g, a, ntz, ntx, nyc, nblk);
+    hipLaunchKernelGGL((k_tri3d_pipe<G, W, NT>), dim3(grid), dim3(64 * W), 0, s, g, a, ntz, ntx, nyc, nblk);
 }
 
 template <int G>
@@ -2329,10 +2331,18 @@ void tri_launch(pw_tri* p, const TriStep<G>& a, hipStream_t s)
 {
     const Geo& g = p->g;
     if (p->pipe_ok) {
-        switch (p->tri_w) {
-            case 4: tri_launch_pipe<G, 4>(p, a, s); break;
-            case 16: tri_launch_pipe<G, 16>(p, a, s); break;
-            default: tri_launch_pipe<G, 8>(p, a, s); break;
+        if (p->tri_nt) {
+            switch (p->tri_w) {
+                case 4: tri_launch_pipe<G, 4, 1>(p, a, s); break;
+                case 16: tri_launch_pipe<G, 16, 1>(p, a, s); break;
+                default: tri_launch_pipe<G, 8, 1>(p, a, s); break;
+            }
+        } else {
+            switch (p->tri_w) {
+                case 4: tri_launch_pipe<G, 4, 0>(p, a, s); break;
+                case 16: tri_launch_pipe<G, 16, 0>(p, a, s); break;
+                default: tri_launch_pipe<G, 8, 0>(p, a, s); break;
+            }
         }
         return;
     }
@@ -2381,9 +2391,9 @@ TriStep<G> tri_step(const pw_tri* p, int pass, int it, float* const* A, float* c
     return a;
 }
 
-// Launch-shape sweep of the pipelined kernel: rows per block 4, 8, 16 (one per wave) and y chunks
-// of 4 to 24 planes, each timed on the last time step of pass 0 (no image) on the plan's own
-// fields; the fastest is kept.  The run that follows clears the fields the timing launches wrote.
+// Launch-shape sweep of the pipelined kernel: rows per block 4, 8, 16 (one per wave), y chunks of
+// 4 to 24 planes and the cache policy of the streamed operands (default or non-temporal), each
+// timed on the last time step of pass 0 (no image) on the plan's own fields; the fastest is kept.  The run that follows clears the fields the timing launches wrote.
 template <int G>
 int tri_tune(pw_tri* p)
 {
@@ -2398,11 +2408,15 @@ int tri_tune(pw_tri* p)
     PW_HIP(ctx, hipEventCreate(&e1));
     p->tune_log.clear();
     float best = 0.f;
-    int bw = 8, bc = 0, rc = PW_OK;
-    for (int W : {4, 8, 16}) {
-        for (int nyc : {4, 6, 8, 12, 16, 24}) {
+    int bw = 8, bc = 0, bn = 0, rc = PW_OK;
+    constexpr int kRows[] = {4, 8, 16}, kChunks[] = {4, 6, 8, 12, 16, 24};
+    for (int combo = 0; combo < 36 && !rc; ++combo) {
+        const int W = kRows[combo / 12], nyc = kChunks[combo % 6];
+        const int nt = (combo / 6) & 1;
+        {
             p->tri_w = W;
             p->tri_nyc = std::min(nyc, p->g.ny);
+            p->tri_nt = nt;
             const int reps = 6;
             for (int r = 0; r < 2; ++r) tri_launch<G>(p, a, s);
             hipEventRecord(e0, s);
@@ -2416,16 +2430,16 @@ int tri_tune(pw_tri* p)
                 break;
             }
             ms /= reps;
-            for (float v : {(float)W, (float)p->tri_nyc, ms}) p->tune_log.push_back(v);
-            if (best == 0.f || ms < best) { best = ms; bw = W; bc = p->tri_nyc; }
+            for (float v : {(float)W, (float)p->tri_nyc, (float)nt, ms}) p->tune_log.push_back(v);
+            if (best == 0.f || ms < best) { best = ms; bw = W; bc = p->tri_nyc; bn = nt; }
         }
-        if (rc) break;
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     if (rc) return rc;
     p->tri_w = bw;
     p->tri_nyc = bc;
+    p->tri_nt = bn;
     p->tuned = true;
     return PW_OK;
 }
@@ -2669,9 +2683,9 @@ int pw_tri_run_timed(pw_tri* p, double* total_ms)
     return PW_OK;
 }
 
-// launch shape of the pipelined kernel: rows per block 4, 8 or 16 and a y chunk of at least one
-// plane; rows 0 returns to the timed choice
-int pw_tri_set_launch(pw_tri* p, int rows, int y_chunk)
+// launch shape of the pipelined kernel: rows per block 4, 8 or 16, a y chunk of at least one
+// plane and the cache policy (non_temporal 0 or 1); rows 0 returns to the timed choice
+int pw_tri_set_launch(pw_tri* p, int rows, int y_chunk, int non_temporal)
 {
     if (!p) return PW_ERR_ARG;
     if (rows == 0) {
@@ -2682,20 +2696,25 @@ int pw_tri_set_launch(pw_tri* p, int rows, int y_chunk)
         PW_FAIL(p->ctx, PW_ERR_ARG, "tri set_launch: rows must be 4, 8 or 16 and the y chunk positive");
     p->tri_w = rows;
     p->tri_nyc = y_chunk;
+    p->tri_nt = non_temporal ? 1 : 0;
     p->tune_log.clear();
     p->tuned = true;
     return PW_OK;
 }
 
 // the pipelined kernel's launch shape: w[0] rows per block, w[1] y-chunk planes (0: not yet
-// chosen or the general kernel runs); the timing sweep's (rows, chunk, ms per launch) triples into
-// log[3*cap]; returns their number
+// chosen or the general kernel runs), w[2] 1 for the non-temporal policy; the timing sweep's
+// (rows, chunk, non-temporal, ms per launch) quadruples into log[4*cap]; returns their number
 int pw_tri_launch_choice(pw_tri* p, int* w, float* log, int cap)
 {
     if (!p) return PW_ERR_ARG;
-    if (w) { w[0] = p->pipe_ok ? p->tri_w : 0; w[1] = p->pipe_ok ? p->tri_nyc : 0; }
-    const int n = (int)p->tune_log.size() / 3;
-    for (int i = 0; i < std::min(n, cap) * 3 && log; ++i) log[i] = p->tune_log[i];
+    if (w) {
+        w[0] = p->pipe_ok ? p->tri_w : 0;
+        w[1] = p->pipe_ok ? p->tri_nyc : 0;
+        w[2] = p->pipe_ok ? p->tri_nt : 0;
+    }
+    const int n = (int)p->tune_log.size() / 4;
+    for (int i = 0; i < std::min(n, cap) * 4 && log; ++i) log[i] = p->tune_log[i];
     return n;
 }
 

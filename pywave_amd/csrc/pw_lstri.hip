@@ -465,7 +465,7 @@ __global__ __launch_bounds__(T2) void k_tr2(TriGeo g, TriArgs2 a, int ntz2)
 // level equals NS single steps bit for bit.  The last two levels go to the spare fields C (step
 // NS-1) and D (step NS).
 constexpr int OZN = 56;  // output z per tile
-constexpr int TN = 256;  // threads per block
+constexpr int TN = 1024;  // threads per block (16 waves: the LDS reads of the steps need them in flight)
 
 template <int NS>
 struct TriArgsN {

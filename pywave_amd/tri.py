@@ -34,8 +34,8 @@ def tri3d_image(vel, dat, groups, nt=None, dt=0.001, ax=[0, 20, 81], ay=[0, 20, 
     boolean masks (up to 64; more than 4 run in passes of 4, with each snapshot's running product
     kept in HBM).  Returns the image (nz, nx, ny) (divided by its maximum when normalize).  A dict
     passed as launch_info receives the back-propagation kernel's launch shape (rows per block, y
-    chunk) and the timing sweep that chose it; launch=(rows, y_chunk) forces that shape (rows 4, 8 or
-    16; the image is the same bit for bit)."""
+    chunk, non-temporal cache policy) and the timing sweep that chose it; launch=(rows, y_chunk[,
+    non_temporal]) forces that shape (rows 4, 8 or 16; the image is the same bit for bit)."""
     vel = np.asarray(vel, np.float32)
     nz, nx, ny = int(az[2]), int(ax[2]), int(ay[2])
     if vel.shape != (nz, nx, ny):
@@ -59,19 +59,20 @@ def tri3d_image(vel, dat, groups, nt=None, dt=0.001, ax=[0, 20, 81], ay=[0, 20, 
         mk = np.ascontiguousarray(masks)
         N.check(L.pw_tri_set_data(p, N.ptr(d), N.ptr(mk)), ctx, "tri set_data")
         if launch is not None:
-            N.check(L.pw_tri_set_launch(p, int(launch[0]), int(launch[1])), ctx, "tri set_launch")
+            nt_ = int(launch[2]) if len(launch) > 2 else 0
+            N.check(L.pw_tri_set_launch(p, int(launch[0]), int(launch[1]), nt_), ctx, "tri set_launch")
         ms = ctypes.c_double(0)
         N.check(L.pw_tri_run_timed(p, ctypes.byref(ms)), ctx, "tri run")
         img = np.empty(nz * nx * ny, np.float64)
         N.check(L.pw_tri_image(p, N.ptr(img), img.size), ctx, "tri image")
         if launch_info is not None:
-            w = np.zeros(2, np.int32)
+            w = np.zeros(3, np.int32)
             n = L.pw_tri_launch_choice(p, N.ptr(w), None, 0)
-            buf = np.zeros(3 * max(n, 1), np.float32)
+            buf = np.zeros(4 * max(n, 1), np.float32)
             L.pw_tri_launch_choice(p, N.ptr(w), N.ptr(buf), n)
-            launch_info.update(rows_per_block=int(w[0]), y_chunk=int(w[1]),
-                               timing_ms_per_launch=[(int(buf[3 * i]), int(buf[3 * i + 1]),
-                                                      round(float(buf[3 * i + 2]), 4)) for i in range(n)])
+            launch_info.update(rows_per_block=int(w[0]), y_chunk=int(w[1]), non_temporal=bool(w[2]),
+                               timing_ms_per_launch=[(int(buf[4 * i]), int(buf[4 * i + 1]), bool(buf[4 * i + 2]),
+                                                      round(float(buf[4 * i + 3]), 4)) for i in range(n)])
     finally:
         L.pw_tri_destroy(p)
     img = img.reshape(nz, nx, ny, order="F")

@@ -141,11 +141,12 @@ def test_timed_launch_choice_and_every_tile_width_bit_exact(ls):
     dat = rng.standard_normal((nt, 24, 2)).astype(np.float32)
     p = LstriProblem(par, ls["vel"])
     d0, i0 = p.model(src), p.image(dat)
-    (w, spl), log = p.launch_choice()
-    assert w in (1, 2, 4, 8, 16) and spl in (1, 2)
+    (w, spl, cols), log = p.launch_choice()
+    assert w in (1, 2, 4, 8, 16) and spl in (1, 2, 4) and cols in (4, 8)
     kinds = [k for k, _, _ in log]
     assert kinds.count("model_step") == 5 and kinds.count("adjoint_step") == 5 and all(ms > 0 for _, _, ms in log)
     assert kinds.count("two_step_model") == 1 and kinds.count("two_step_adjoint") == 1
+    assert kinds.count("four_step_model") == 2 and kinds.count("four_step_adjoint") == 2
     pi = par.copy()
     pi[9], pi[19] = 1, 3
     q = LstriProblem(pi, ls["vel"])
@@ -155,9 +156,46 @@ def test_timed_launch_choice_and_every_tile_width_bit_exact(ls):
         p.steps_per_launch(1)
         assert np.array_equal(p.model(src), d0), width
         assert np.array_equal(p.image(dat), i0), width
-        assert p.launch_choice()[0] == (width, 1)
+        assert p.launch_choice()[0][:2] == (width, 1)
         q.set_tile_width(width)
         q.steps_per_launch(1)
         assert np.array_equal(q.invert(d0), x0), width
     with pytest.raises(ValueError):
         p.set_tile_width(3)
+
+
+@pytest.mark.parametrize("nt", [120, 121, 122, 123])
+@pytest.mark.parametrize("cols", [4, 8])
+def test_four_step_launches_equal_single_steps(ls, nt, cols):
+    # four steps per launch (remainders of 1-3 steps in two-step and single-step launches): modelled
+    # data, images and the CG inversion equal the one-step launches bit for bit, and the modelling
+    # and adjoint equal the oracle's timerev_lop
+    from oracle import oracle as O
+    par = ls["par"].copy()
+    par[6] = nt
+    rng = np.random.default_rng(1000 + nt + cols)
+    src = rng.standard_normal((16, 24, nt, 2)).astype(np.float32)
+    dat = rng.standard_normal((nt, 24, 2)).astype(np.float32)
+    one = LstriProblem(par, ls["vel"])
+    one.steps_per_launch(1)
+    four = LstriProblem(par, ls["vel"])
+    four.steps_per_launch(4)
+    four.set_four_step_columns(cols)
+    d4 = four.model(src)
+    assert four.launch_choice()[0][1:] == (4, cols)
+    assert np.array_equal(one.model(src), d4)
+    i4 = four.image(dat)
+    assert np.array_equal(one.image(dat), i4)
+    for s in range(2):
+        assert np.array_equal(d4[:, :, s], O.timerev(par, ls["vel"], src[..., s], adj=False)), s
+        assert np.array_equal(i4[..., s], O.timerev(par, ls["vel"], dat[:, :, s], adj=True)), s
+    pi = par.copy()
+    pi[9], pi[19] = 1, 3
+    a = LstriProblem(pi, ls["vel"])
+    a.steps_per_launch(1)
+    b = LstriProblem(pi, ls["vel"])
+    b.steps_per_launch(4)
+    b.set_four_step_columns(cols)
+    assert np.array_equal(a.invert(d4), b.invert(d4))
+    with pytest.raises(ValueError):
+        b.set_four_step_columns(6)

@@ -129,10 +129,11 @@ def test_every_launch_shape_gives_the_same_image():
     info = {}
     ref = pywave_amd.tri3d_image(vel, dat, groups, launch_info=info, **kw)
     assert info["rows_per_block"] in (4, 8, 16) and 1 <= info["y_chunk"] <= 24
-    assert len(info["timing_ms_per_launch"]) == 18
-    assert (info["rows_per_block"], info["y_chunk"]) in [(r, c) for r, c, _ in info["timing_ms_per_launch"]]
-    for rows, chunk in [(4, 3), (8, 7), (16, 5), (16, 46)]:
-        img = pywave_amd.tri3d_image(vel, dat, groups, launch=(rows, chunk), **kw)
-        assert np.array_equal(img, ref), (rows, chunk)
+    assert len(info["timing_ms_per_launch"]) == 36
+    assert ((info["rows_per_block"], info["y_chunk"], info["non_temporal"]) in
+            [(r, c, n) for r, c, n, _ in info["timing_ms_per_launch"]])
+    for rows, chunk, nt_ in [(4, 3, 0), (8, 7, 1), (16, 5, 0), (16, 46, 1), (4, 6, 1)]:
+        img = pywave_amd.tri3d_image(vel, dat, groups, launch=(rows, chunk, nt_), **kw)
+        assert np.array_equal(img, ref), (rows, chunk, nt_)
     with pytest.raises(ValueError):
         pywave_amd.tri3d_image(vel, dat, groups, launch=(5, 4), **kw)
