@@ -46,6 +46,7 @@ constexpr int kDefaultWidth = 8;
 // Tiles of the step kernels: 64 z cells x W columns of one shot slot, one wave each.
 struct TileMap {
     int ntz, nxc;  // z tiles and column groups per slot
+    int ns;        // slots (shots) of the launch
     int nblk;      // tiles over all slots of the launch
 };
 
@@ -54,6 +55,7 @@ TileMap tile_map(int padnz, int padnx, int nslots, int width)
     TileMap t;
     t.ntz = (padnz + 63) / 64;
     t.nxc = (padnx + width - 1) / width;
+    t.ns = nslots;
     t.nblk = t.ntz * t.nxc * nslots;
     return t;
 }
@@ -62,16 +64,17 @@ TileMap tile_map(int padnz, int padnx, int nslots, int width)
 inline dim3 tile_grid(const TileMap& t) { return dim3((unsigned)((t.nblk + 7) / 8 * 8)); }
 
 // XCD-aware tile order: the hardware deals blocks round-robin over the 8 XCDs; each XCD takes a
-// contiguous range of tiles (z tile fastest, then column group, then slot), so column neighbours,
-// which read each other's halo columns, share an L2.  False for the padding blocks.
+// contiguous range of tiles, slot fastest, then z tile, then column group: the shots' tiles of one
+// grid tile run together on one XCD and read the fields the shots share (v^2dt^2, the sponge)
+// through one L2, and column neighbours, which read each other's halo columns, share it too.
+// False for the padding blocks.
 __device__ __forceinline__ bool tile_of(const TileMap& t, int& slot, int& tz, int& xc)
 {
     const int per = gridDim.x >> 3;
     const int lb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
     if (lb >= t.nblk) return false;
-    const int per_slot = t.ntz * t.nxc;
-    slot = lb / per_slot;
-    const int r = lb - slot * per_slot;
+    const int r = lb / t.ns;
+    slot = lb - r * t.ns;
     tz = r % t.ntz;
     xc = r / t.ntz;
     return true;
@@ -2140,6 +2143,7 @@ void launch_fwd2(int W, const FGeo<T>& g, const Fwd2Args<T>& a, int pnz, int pnx
     TileMap tm;
     tm.ntz = (pnz + kOwnZ - 1) / kOwnZ;
     tm.nxc = (pnx + W - 1) / W;
+    tm.ns = nb_;
     tm.nblk = tm.ntz * tm.nxc * nb_;
     if (W == 8) hipLaunchKernelGGL((k_fwd2<T, 8>), tile_grid(tm), dim3(64), 0, s, g, a, tm);
     else if (W == 32) hipLaunchKernelGGL((k_fwd2<T, 32>), tile_grid(tm), dim3(64), 0, s, g, a, tm);
@@ -2151,6 +2155,7 @@ void launch_adj2(int W, const FGeo<float>& g, const Adj2Args<float>& a, int pnz,
     TileMap tm;
     tm.ntz = (pnz + kOwnZ - 1) / kOwnZ;
     tm.nxc = (pnx + W - 1) / W;
+    tm.ns = nb_;
     tm.nblk = tm.ntz * tm.nxc * nb_;
     if (W == 8) hipLaunchKernelGGL((k_adj2<float, 8>), tile_grid(tm), dim3(64), 0, s, g, a, tm);
     else if (W == 32) hipLaunchKernelGGL((k_adj2<float, 32>), tile_grid(tm), dim3(64), 0, s, g, a, tm);
