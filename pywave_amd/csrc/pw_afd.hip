@@ -28,7 +28,7 @@
 namespace {
 
 constexpr float C0 = -30. / 12., C1 = +16. / 12., C2 = -1. / 12.;  // pywave/src/wave_fdm.c:15
-constexpr int LDZ_ALIGN = 32;
+constexpr int LDZ_ALIGN = 16;
 
 struct Geo {
     int nz, nx, ny;     // updated domain (model + 2*nb)
@@ -1580,7 +1580,8 @@ pw_afd* pw_afd_create(pw_ctx* ctx, int ndim, int nz, int nx, int ny, float dz, f
     g.nz = nz + 2 * nbt; g.nx = nx + 2 * nbt; g.ny = ndim == 3 ? ny + 2 * nbt : 1;
     pw::padded_sizes(g.nz, g.nx, g.ny, &g.nz2, &g.nx2, &g.ny2);
     if (ndim == 2) g.ny2 = 1;
-    // z stride: rows start on 128 B boundaries.  A 64-lane z tile may run past the end of a row
+    // z stride: rows start on 64 B boundaries (the padding after a row's last cell then shares a
+    // cache line with the next row).  A 64-lane z tile may run past the end of a row
     // (into the next row or the guard); such lanes lie outside the updated domain and are masked
     // or only feed other masked lanes.
     g.ldz = pw::ceil_div(g.nz2, LDZ_ALIGN) * LDZ_ALIGN;
