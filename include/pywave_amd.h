@@ -187,7 +187,8 @@ int pw_fwi_adjoint_dev(pw_fwi *p, const void *d_vel, const void *d_gtraces, void
  * w[2] exact adjoint, w[3] reference adjoint, w[4] visco-acoustic reference adjoint; w[5] = 1 once
  * the plan has timed the widths on its grid and batch (its first sweep does), 0 before; w[6] the
  * two-step forward's width, 0 when forward sweeps run one step per launch; w[7] the same for the
- * exact adjoint (8 ints) */
+ * exact adjoint; w[8] the width of forward steps that store their term for the exact adjoint
+ * (9 ints) */
 int pw_fwi_widths(pw_fwi *p, int *w);
 /* forward time steps one launch may advance: 2 (default; the plan's timing sweep keeps two steps
  * per launch where they are faster), 1 (one kernel per time step, the plain method) or -2 (two
@@ -195,12 +196,13 @@ int pw_fwi_widths(pw_fwi *p, int *w);
 int pw_fwi_set_steps_per_launch(pw_fwi *p, int n);
 /* force the step kernels' tile widths instead of timing them: w[0] forward, w[1] visco forward,
  * w[2] exact adjoint, w[3] reference adjoint, w[4] visco reference adjoint, each 4, 8 or 16 (one
- * step per launch); NULL returns to the timing sweep.  Identical results at every width except the
+ * step per launch; forward steps that store their term take w[0]); NULL returns to the timing sweep.  Identical results at every width except the
  * exact adjoint's FMA contraction (f32 rounding).  No counterpart in the reference. */
 int pw_fwi_set_widths(pw_fwi *p, const int *w);
 /* the plan's last width-timing sweep: triples (kernel, width, ms per launch) written to out (up to
  * cap of them); kernel 0 forward, 1 visco forward, 2 exact adjoint, 3 reference adjoint, 4 visco
- * reference adjoint, 5 two-step forward, 6 two-step exact adjoint.  Returns the number of triples. */
+ * reference adjoint, 5 two-step forward, 6 two-step exact adjoint, 7 forward storing its term.
+ * Returns the number of triples. */
 int pw_fwi_tuning_log(pw_fwi *p, float *out, int cap);
 /* event-timed milliseconds of the last gradient (forward sweeps, backward sweeps) or of the last
  * pw_fwi_forward (its sweeps, 0) */

@@ -1908,6 +1908,7 @@ struct pw_fwi {
     // visco reference adjoint), picked by tune_widths for the plan's grid and batch
     int w_fwd = kDefaultWidth, w_fwdv = kDefaultWidth, w_exact = kDefaultWidth, w_ref = kDefaultWidth,
         w_refv = kDefaultWidth;
+    int w_fwdt = kDefaultWidth;  // forward steps that store their term for the exact adjoint
     int w_fwd2 = 0;       // two-step forward tiles' width (0: single steps are faster or not allowed)
     int w_adj2 = 0;       // two-step exact adjoint's width (0: single steps)
     int max_fused = 2;    // forward steps one launch may advance (1: one kernel per time step)
@@ -2281,7 +2282,7 @@ int tune_widths(pw_fwi* p)
         p->tune_log.push_back(ms);
     };
     int kind = 0;  // 0 forward, 1 visco forward, 2 exact adjoint, 3 reference adjoint, 4 visco reference adjoint,
-                   // 5 two-step forward, 6 two-step exact adjoint
+                   // 5 two-step forward, 6 two-step exact adjoint, 7 forward storing its term
     auto pick = [&](auto launch, int* best) -> int {
         float tbest = 0.f;
         for (int W : kWidths) {
@@ -2305,6 +2306,15 @@ int tune_widths(pw_fwi* p)
     } else {
         kind = 0;
         rc = pick([&](int W) { launch_fwd<T, false>(W, g, f, a.padnz, a.padnx, nb_, s); }, &p->w_fwd);
+        // the exact gradient's forward steps also store their term (4 B more per cell, written
+        // once): timed on their own, as the extra stream moves the best width
+        if (!rc) {
+            StepArgs<T> ft = f;
+            ft.full = (T*)p->seg;
+            ft.fulls = p->fs;
+            kind = 7;
+            rc = pick([&](int W) { launch_fwd<T, false>(W, g, ft, a.padnz, a.padnx, nb_, s); }, &p->w_fwdt);
+        }
     }
     if (rc) return rc;
     // two steps per launch: kept when a launch takes less than two single steps
@@ -2413,7 +2423,7 @@ int ensure_widths(pw_fwi* p)
     if (hit != p->ctx->tuned2d.end()) {
         const std::vector<int>& w = hit->second.first;
         p->w_fwd = w[0]; p->w_fwdv = w[1]; p->w_exact = w[2]; p->w_ref = w[3]; p->w_refv = w[4];
-        p->w_fwd2 = w[5]; p->w_adj2 = w[6];
+        p->w_fwd2 = w[5]; p->w_adj2 = w[6]; p->w_fwdt = w[7];
         const long long fld = p->nslot * p->fs;
         int rc = PW_OK;
         if (p->w_fwd2 && !p->X2) rc = dmalloc(p, &p->X2, fld * esz(p));
@@ -2427,7 +2437,8 @@ int ensure_widths(pw_fwi* p)
     }
     int rc = p->prec == 4 ? tune_widths<float>(p) : tune_widths<double>(p);
     if (!rc)
-        p->ctx->tuned2d[key] = {{p->w_fwd, p->w_fwdv, p->w_exact, p->w_ref, p->w_refv, p->w_fwd2, p->w_adj2},
+        p->ctx->tuned2d[key] = {{p->w_fwd, p->w_fwdv, p->w_exact, p->w_ref, p->w_refv, p->w_fwd2, p->w_adj2,
+                                 p->w_fwdt},
                                 p->tune_log};
     return rc;
 }
@@ -2502,7 +2513,7 @@ int forward_batch(pw_fwi* p, int nb_, bool ckpt, int mode, bool store_last)
             s_.idt = (float)(1. / (double)p->acq.dt);
         }
         if (visco) launch_fwd<T, true>(p->w_fwdv, g, s_, a.padnz, a.padnx, nb_, s);
-        else launch_fwd<T, false>(p->w_fwd, g, s_, a.padnz, a.padnx, nb_, s);
+        else launch_fwd<T, false>(s_.full ? p->w_fwdt : p->w_fwd, g, s_, a.padnz, a.padnx, nb_, s);
         PW_HIPS(hipGetLastError());
         std::swap(X, Y);
         if (visco) std::swap(Ra, Rb);
@@ -2563,7 +2574,7 @@ int recompute_segment(pw_fwi* p, int nb_, int sg, int mode)
             launch_fwd<T, true>(p->w_fwdv, g, s_, a.padnz, a.padnx, nb_, s);
             std::swap(Ra, Rb);
         } else {
-            launch_fwd<T, false>(p->w_fwd, g, s_, a.padnz, a.padnx, nb_, s);
+            launch_fwd<T, false>(s_.full ? p->w_fwdt : p->w_fwd, g, s_, a.padnz, a.padnx, nb_, s);
         }
         PW_HIPS(hipGetLastError());
         std::swap(X, Y);
@@ -3451,6 +3462,7 @@ int pw_fwi_widths(pw_fwi* p, int* w)
     w[5] = p->widths_tuned ? 1 : 0;
     w[6] = p->w_fwd2;
     w[7] = p->w_adj2;
+    w[8] = p->w_fwdt;
     return PW_OK;
 }
 
@@ -3478,6 +3490,7 @@ int pw_fwi_set_widths(pw_fwi* p, const int* w)
         if (!ok) PW_FAIL(p->ctx, PW_ERR_ARG, "pw_fwi_set_widths: widths must be 4, 8 or 16");
     }
     p->w_fwd = w[0]; p->w_fwdv = w[1]; p->w_exact = w[2]; p->w_ref = w[3]; p->w_refv = w[4];
+    p->w_fwdt = w[0];
     p->w_fwd2 = p->w_adj2 = 0;
     p->tune_log.clear();
     p->widths_tuned = p->widths_forced = true;

@@ -127,17 +127,18 @@ class FwiProblem:
 
     def widths(self):
         """Tile widths of the step kernels picked by the plan's timing sweep (its first sweep runs
-        it): forward, visco forward, exact adjoint, reference adjoint, visco reference adjoint."""
-        w = np.zeros(8, np.int32)
+        it): forward, visco forward, exact adjoint, reference adjoint, visco reference adjoint, and
+        the forward steps that store their term for the exact adjoint."""
+        w = np.zeros(9, np.int32)
         N.check(self._L.pw_fwi_widths(self._p, N.ptr(w)), self._ctx, "widths")
         keys = ["forward", "visco_forward", "exact_adjoint", "reference_adjoint", "visco_reference_adjoint"]
         return dict(zip(keys, (int(x) for x in w[:5])), tuned=bool(w[5]), two_step_forward=int(w[6]),
-                    two_step_exact_adjoint=int(w[7]))
+                    two_step_exact_adjoint=int(w[7]), forward_storing_terms=int(w[8]))
 
     def tuning_log(self):
         """The width-timing sweep's measurements: [(kernel, width, ms per launch), ...]."""
         names = ["forward", "visco_forward", "exact_adjoint", "reference_adjoint", "visco_reference_adjoint",
-                 "two_step_forward", "two_step_exact_adjoint"]
+                 "two_step_forward", "two_step_exact_adjoint", "forward_storing_terms"]
         n = self._L.pw_fwi_tuning_log(self._p, None, 0)
         buf = np.zeros(3 * max(n, 1), np.float32)
         self._L.pw_fwi_tuning_log(self._p, N.ptr(buf), n)
