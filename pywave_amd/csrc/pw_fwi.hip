@@ -720,15 +720,22 @@ __global__ __launch_bounds__(64) void k_adj_ref(FGeo<float> g, StepArgs<float> a
                 const auto rIj = win(REC ? inj : nullptr, REC ? (unsigned)(g.nr * 4) : 0u);
                 const bool rl = z == g.rz;
                 struct ACol {
-                    float q, h, y, m, wp, wc, wn, tv, gv, xl, xh;
+                    float q, h, y, m, wp, wc, wn, tv, gv, xl, xh, ij;
                 };
+                // every operand of a column, the receiver sample included, is fetched one column
+                // ahead: a load issued after them would make each column wait for all of them
                 auto get = [&](int xx) -> ACol {
                     ACol c;
                     c.q = bldo<float>(rX, oz, (unsigned)(xx + 4) * oc);
                     c.h = bldo<float>(rX, ozh, (unsigned)xx * oc);
                     c.y = bldo<float>(rY, oz, (unsigned)xx * oc);
                     c.m = bldo<float>(rM, oz, (unsigned)xx * oc);
-                    c.wp = c.wc = c.wn = c.tv = c.gv = 0.f;
+                    c.wp = c.wc = c.wn = c.tv = c.gv = c.ij = 0.f;
+                    if constexpr (REC) {
+                        const int d = xx - sh.r0_v, kk = sh.r02 + d;
+                        const bool rx = d >= 0 && d < sh.nr2 && kk >= 0 && kk < g.nr;
+                        c.ij = bldo<float>(rIj, (rx && rl) ? 0u : kOOB, rx ? (unsigned)kk * 4u : 0u);
+                    }
                     if constexpr (IMG) {
                         const bool ok = gxin(xx);
                         const unsigned vo = ok ? og : kOOB, so = ok ? (unsigned)((xx - g.nb) * g.nz) * 4u : 0u;
@@ -766,8 +773,7 @@ __global__ __launch_bounds__(64) void k_adj_ref(FGeo<float> g, StepArgs<float> a
                     if constexpr (REC) {  // the windowed residual at the receivers (:208-211)
                         const int d = x - sh.r0_v, kk = sh.r02 + d;
                         const bool rx = d >= 0 && d < sh.nr2 && kk >= 0 && kk < g.nr;
-                        const float ij = bldo<float>(rIj, (rx && rl) ? 0u : kOOB, rx ? (unsigned)kk * 4u : 0u);
-                        term = (rx && rl) ? term + ij : term;
+                        term = (rx && rl) ? term + cur.ij : term;
                     }
                     const float p0 = sponge(cur.y, zl, zh, cur.xl, cur.xh);
                     const float p2 = 2 * p1 - p0 + cur.m * term;
@@ -806,9 +812,9 @@ __global__ __launch_bounds__(64) void k_adj_ref(FGeo<float> g, StepArgs<float> a
                 const auto rIj = win(REC ? inj : nullptr, REC ? (unsigned)(g.nr * 4) : 0u);
                 const bool rl = z == g.rz;
                 struct VCol {
-                    float p5, r5, ta5, ts5, hp, hr, hta, hts, y, m, wp, wc, wn, tv, gv, xl, xh;
+                    float p5, r5, ta5, ts5, hp, hr, hta, hts, y, m, wp, wc, wn, tv, gv, xl, xh, ij;
                 };
-                // operands of column xx (and the queue entry xx + 4)
+                // operands of column xx (and the queue entry xx + 4), the receiver sample included
                 auto get = [&](int xx) -> VCol {
                     VCol c;
                     const unsigned s4 = (unsigned)(xx + 4) * oc, s0 = (unsigned)xx * oc;
@@ -822,7 +828,12 @@ __global__ __launch_bounds__(64) void k_adj_ref(FGeo<float> g, StepArgs<float> a
                     c.hts = bldo<float>(rTs, ozh, s0);
                     c.y = bldo<float>(rY, oz, s0);
                     c.m = bldo<float>(rM, oz, s0);
-                    c.wp = c.wc = c.wn = c.tv = c.gv = 0.f;
+                    c.wp = c.wc = c.wn = c.tv = c.gv = c.ij = 0.f;
+                    if constexpr (REC) {
+                        const int d = xx - sh.r0_v, kk = sh.r02 + d;
+                        const bool rx = d >= 0 && d < sh.nr2 && kk >= 0 && kk < g.nr;
+                        c.ij = bldo<float>(rIj, (rx && rl) ? 0u : kOOB, rx ? (unsigned)kk * 4u : 0u);
+                    }
                     if constexpr (IMG) {
                         const bool ok = gxin(xx);
                         const unsigned vo = ok ? og : kOOB, so = ok ? (unsigned)((xx - g.nb) * g.nz) * 4u : 0u;
@@ -837,15 +848,22 @@ __global__ __launch_bounds__(64) void k_adj_ref(FGeo<float> g, StepArgs<float> a
                     return c;
                 };
                 float tq[9], pq[5], rq[5];
+                // the queue's first eight columns: every load issued before the first use
+                float p0[8], r0[8], ta0[8], ts0[8];
 #pragma unroll
                 for (int k = 0; k < 8; ++k) {
                     const unsigned so = (unsigned)(x0 - 4 + k) * oc;
-                    const float p1 = bldo<float>(rX, oz, so);
+                    p0[k] = bldo<float>(rX, oz, so);
+                    r0[k] = bldo<float>(rR1, ozi, so);
+                    ta0[k] = bldo<float>(rTa, ozi, so);
+                    ts0[k] = bldo<float>(rTs, ozi, so);
+                }
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
                     float r2;
-                    tq[k] = form(lin, p1, bldo<float>(rR1, ozi, so), bldo<float>(rTa, ozi, so),
-                                 bldo<float>(rTs, ozi, so), r2);
+                    tq[k] = form(lin, p0[k], r0[k], ta0[k], ts0[k], r2);
                     if (k >= 4) {
-                        pq[k - 4] = p1;
+                        pq[k - 4] = p0[k];
                         rq[k - 4] = r2;
                     }
                 }
@@ -875,8 +893,7 @@ __global__ __launch_bounds__(64) void k_adj_ref(FGeo<float> g, StepArgs<float> a
                     if constexpr (REC) {
                         const int d = x - sh.r0_v, kk = sh.r02 + d;
                         const bool rx = d >= 0 && d < sh.nr2 && kk >= 0 && kk < g.nr;
-                        const float ij = bldo<float>(rIj, (rx && rl) ? 0u : kOOB, rx ? (unsigned)kk * 4u : 0u);
-                        term = (rx && rl) ? term + ij : term;
+                        term = (rx && rl) ? term + cur.ij : term;
                     }
                     const float p0 = sponge(cur.y, zl, zh, cur.xl, cur.xh);
                     const float p2 = 2 * p1 - p0 + cur.m * term;
@@ -1097,49 +1114,60 @@ __global__ __launch_bounds__(64) void k_adj_exact(FGeo<T> g, StepArgs<T> a, ExAr
         if (lane < 2 * XC) sxp[lane] = a.sfx[2 * x0 + lane];
         const unsigned orz = rlane ? 0u : kOOB;
         struct XCol {
-            T a5, m5, a, b, g, t, ha, xl, xh;
+            T a5, m5, a, b, g, t, ha, xl, xh, s;
         };
-        auto get = [&](int xx) -> XCol {
-            XCol c;
-            c.a5 = bldo<T>(rA, ozi, (unsigned)(xx + 4) * oc);
-            c.m5 = bldo<T>(rM, ozi, (unsigned)(xx + 4) * oc);
-            c.a = bldo<T>(rA, oz, (unsigned)xx * oc);
-            c.b = bldo<T>(rB, oz, (unsigned)xx * oc);
-            c.g = bldo<T>(rG, oz, (unsigned)xx * oc);
-            c.t = bldo_nt<T>(rF, oz, (unsigned)xx * oc);  // the stored term, read once
-            c.ha = bldo<T>(rM, ozhi, (unsigned)xx * oc) * bldo<T>(rA, ozhi, (unsigned)xx * oc);
-            c.xl = sxp[2 * (xx - x0)];
-            c.xh = sxp[2 * (xx - x0) + 1];
-            return c;
-        };
-        T qa[9];
+        // REC: the tile holds the receiver row; R^T g_t, the receiver sample on lane z == rz (0
+        // elsewhere and off the receivers), is fetched with the column's other operands, one
+        // column ahead: a load issued after them would make every column wait for all of them
+        auto lean = [&](auto rec) {
+            constexpr bool REC = decltype(rec)::value;
+            auto get = [&](int xx) -> XCol {
+                XCol c;
+                c.a5 = bldo<T>(rA, ozi, (unsigned)(xx + 4) * oc);
+                c.m5 = bldo<T>(rM, ozi, (unsigned)(xx + 4) * oc);
+                c.a = bldo<T>(rA, oz, (unsigned)xx * oc);
+                c.b = bldo<T>(rB, oz, (unsigned)xx * oc);
+                c.g = bldo<T>(rG, oz, (unsigned)xx * oc);
+                c.t = bldo_nt<T>(rF, oz, (unsigned)xx * oc);  // the stored term, read once
+                c.ha = bldo<T>(rM, ozhi, (unsigned)xx * oc) * bldo<T>(rA, ozhi, (unsigned)xx * oc);
+                c.xl = sxp[2 * (xx - x0)];
+                c.xh = sxp[2 * (xx - x0) + 1];
+                if constexpr (REC) {
+                    const int d = xx - sh.r0_v, kk = sh.r02 + d;
+                    const bool rx = d >= 0 && d < sh.nr2 && kk >= 0 && kk < g.nr;
+                    c.s = bldo<T>(rI, rx ? orz : kOOB, rx ? (unsigned)kk * E : 0u);
+                } else {
+                    c.s = T(0);
+                }
+                return c;
+            };
+            T qa[9];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const unsigned so = (unsigned)(x0 - 4 + k) * oc;
-            qa[k] = bldo<T>(rM, ozi, so) * bldo<T>(rA, ozi, so);
-        }
-        __syncthreads();
-        XCol pend = get(x0);
-#pragma unroll
-        for (int i = 0; i < XC; ++i) {
-            const int x = x0 + i;
-            const XCol cur = pend;
-            if (i + 1 < XC) pend = get(x + 1);
-            qa[8] = cur.m5 * cur.a5;
-            rowa[i & 1][lane + 4] = qa[4];
-            rowa[i & 1][lzh] = cur.ha;
+            for (int k = 0; k < 8; ++k) {
+                const unsigned so = (unsigned)(x0 - 4 + k) * oc;
+                qa[k] = bldo<T>(rM, ozi, so) * bldo<T>(rA, ozi, so);
+            }
             __syncthreads();
-            const T lt = lap(qa, &rowa[i & 1][lane + 4]);
-            // R^T g_t: the receiver sample on lane z == rz (0 elsewhere and off the receivers)
-            const int d = x - sh.r0_v, kk = sh.r02 + d;
-            const bool rx = rec_rows && d >= 0 && d < sh.nr2 && kk >= 0 && kk < g.nr;
-            const T src = bldo<T>(rI, rx ? orz : kOOB, rx ? (unsigned)kk * E : 0u);
-            const T qn = 2 * cur.a - sponge(cur.b, zl, zh, cur.xl, cur.xh) + lt + src;
-            bsto(rB, oz, (unsigned)x * oc, sponge(qn, zl, zh, cur.xl, cur.xh));
-            bsto(rG, oz, (unsigned)x * oc, cur.g + cur.t * cur.a);
+            XCol pend = get(x0);
 #pragma unroll
-            for (int k = 0; k < 8; ++k) qa[k] = qa[k + 1];
-        }
+            for (int i = 0; i < XC; ++i) {
+                const int x = x0 + i;
+                const XCol cur = pend;
+                if (i + 1 < XC) pend = get(x + 1);
+                qa[8] = cur.m5 * cur.a5;
+                rowa[i & 1][lane + 4] = qa[4];
+                rowa[i & 1][lzh] = cur.ha;
+                __syncthreads();
+                const T lt = lap(qa, &rowa[i & 1][lane + 4]);
+                const T qn = 2 * cur.a - sponge(cur.b, zl, zh, cur.xl, cur.xh) + lt + cur.s;
+                bsto(rB, oz, (unsigned)x * oc, sponge(qn, zl, zh, cur.xl, cur.xh));
+                bsto(rG, oz, (unsigned)x * oc, cur.g + cur.t * cur.a);
+#pragma unroll
+                for (int k = 0; k < 8; ++k) qa[k] = qa[k + 1];
+            }
+        };
+        if (rec_rows) lean(std::true_type{});
+        else lean(std::false_type{});
         return;
     }
     T qa[9];
