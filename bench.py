@@ -571,7 +571,7 @@ def tri_bench(cpu_steps=12):
     nb = w["nbt"]
     plan = AfdPlan(vel, w["d"], w["nt"], w["dt"], nb, w["ct"], 0)
     plan.set_sources(w["src"], w["ns"])
-    plan.autotune(steps=5)
+    fwd_variant = list(plan.autotune(steps=5))
     fwd_ms, _ = plan.run_timed()
     dat = plan.output().reshape(w["nt"], n, n, order="F")
     plan.close()
@@ -584,7 +584,7 @@ def tri_bench(cpu_steps=12):
     nsnap = (w["nt"] - 1) // w["jsnap"] + 1
     alg = w["nt"] * cells * (12.0 * G + 4.0) + nsnap * 16.0 * n ** 3
     out = {"workload": "tri3d 128^3 (+30 ABC: 188^3), nt=2000, 3 sources, 4 interleaved receiver groups, jsnap=4",
-           "forward_ms": round(fwd_ms, 2), "backprop_image_ms": round(ms, 2),
+           "forward_ms": round(fwd_ms, 2), "forward_kernel_variant": fwd_variant, "backprop_image_ms": round(ms, 2),
            "field_cell_updates_per_s_G": round(G * cells * w["nt"] / (ms * 1e-3) / 1e9, 2),
            "achieved_GBs": round(alg / (ms * 1e-3) / 1e9, 1),
            "frac_hbm": round(alg / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
