@@ -723,7 +723,23 @@ def calibrate_copy():
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / 10
     print(json.dumps({"copy_bytes_read": a.numel() * 4, "copy_bytes_written": b.numel() * 4,
-                      "copy_ms": round(ms, 4), "copy_GBs": round(2 * a.numel() * 4 / (ms * 1e-3) / 1e9, 1)}))
+                      "copy_ms": round(ms, 4), "copy_GBs": round(2 * a.numel() * 4 / (ms * 1e-3) / 1e9, 1),
+                      "stream": stream_bandwidth()}))
+
+
+def stream_bandwidth(reps=10, n=1 << 27):
+    """The device's plain streaming rate (pw_stream_bandwidth: 1, 2 or 3 input streams and one
+    output stream of 2^27 floats, 512 MB each, more than the Infinity Cache holds): the practical
+    HBM ceiling against which the step kernels' actual traffic is read."""
+    import ctypes
+    from pywave_amd import _native as N
+    L, ctx = N.lib(), N.context()
+    out = {}
+    for reads in (1, 2, 3):
+        g = ctypes.c_double()
+        N.check(L.pw_stream_bandwidth(ctx, n, reads, reps, ctypes.byref(g)), ctx, "stream_bandwidth")
+        out["read%d_write1_GBs" % reads] = round(g.value, 1)
+    return out
 
 
 def load_traffic(name, variant):
@@ -878,6 +894,9 @@ def main():
             "algorithmic_bytes_per_launch": alg_bytes,
         },
         "cpu_baseline": cpu,
+        # the plain streaming ceiling of this device (three reads and one write, like the step
+        # kernel): the step kernel's actual traffic rate is read against it
+        "stream_GBs": stream_bandwidth(),
         "fwi": fwi,
         "tri3d": tri,
         "batch2d": batch,

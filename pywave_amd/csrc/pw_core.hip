@@ -304,6 +304,23 @@ __global__ void k_count_nonfinite(size_t n, const U* __restrict__ a, U expo, uns
 
 __global__ void k_zero_u32(unsigned* p) { *p = 0; }
 
+// Streaming probe: o = a (+ b (+ c)), R input streams and one output stream, 16 B per lane per
+// stream with the non-temporal cache policy (each byte is touched once), grid-stride: the plain
+// streaming rate against which the step kernels' traffic is read
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+template <int R>
+__global__ __launch_bounds__(256) void k_stream(long long n4, const f32x4* __restrict__ a,
+                                                const f32x4* __restrict__ b, const f32x4* __restrict__ c,
+                                                f32x4* __restrict__ o)
+{
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n4; i += (long long)gridDim.x * 256) {
+        f32x4 v = __builtin_nontemporal_load(&a[i]);
+        if constexpr (R > 1) v = v + __builtin_nontemporal_load(&b[i]);
+        if constexpr (R > 2) v = v + __builtin_nontemporal_load(&c[i]);
+        __builtin_nontemporal_store(v, &o[i]);
+    }
+}
+
 }  // namespace
 
 int pw::count_nonfinite(pw_ctx* ctx, const void* d, size_t n, int elem_bytes, unsigned* count)
@@ -394,6 +411,59 @@ int pw_synchronize(pw_ctx* ctx)
 {
     if (!ctx) return PW_ERR_ARG;
     PW_HIP(ctx, hipStreamSynchronize(ctx->s()));
+    return PW_OK;
+}
+
+// Streaming bandwidth of the device: `reads` input streams (1, 2 or 3) and one output stream of n
+// floats each (n a positive multiple of 4), `reps` timed launches after two untimed ones;
+// *gbs = bytes moved per second / 1e9.  Allocates its buffers for the call.
+int pw_stream_bandwidth(pw_ctx* ctx, long long n, int reads, int reps, double* gbs)
+{
+    if (!ctx || !gbs || n <= 0 || n % 4 || reads < 1 || reads > 3 || reps < 1)
+        PW_FAIL(ctx, PW_ERR_ARG, "pw_stream_bandwidth: n must be a positive multiple of 4, reads 1-3, reps >= 1");
+    PW_HIP(ctx, hipSetDevice(ctx->device));
+    hipStream_t s = ctx->s();
+    float* buf[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto release = [&] {
+        for (float* b : buf)
+            if (b) (void)hipFree(b);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    };
+    hipError_t r = hipSuccess;
+    for (int k = 0; k <= reads && r == hipSuccess; ++k) {
+        r = hipMalloc((void**)&buf[k], (size_t)n * 4);
+        if (r == hipSuccess) r = hipMemsetAsync(buf[k], 0, (size_t)n * 4, s);
+    }
+    if (r == hipSuccess) r = hipEventCreate(&e0);
+    if (r == hipSuccess) r = hipEventCreate(&e1);
+    if (r != hipSuccess) {
+        release();
+        PW_FAIL(ctx, PW_ERR_HIP, std::string("pw_stream_bandwidth: ") + hipGetErrorString(r));
+    }
+    const long long n4 = n / 4;
+    const unsigned blocks = (unsigned)std::min<long long>(32768, (n4 + 255) / 256);
+    const f32x4 *a = (const f32x4*)buf[0], *b = (const f32x4*)buf[reads > 1 ? 1 : 0],
+                *c = (const f32x4*)buf[reads > 2 ? 2 : 0];
+    f32x4* o = (f32x4*)buf[reads];
+    auto launch = [&] {
+        if (reads == 1) hipLaunchKernelGGL(k_stream<1>, dim3(blocks), dim3(256), 0, s, n4, a, b, c, o);
+        else if (reads == 2) hipLaunchKernelGGL(k_stream<2>, dim3(blocks), dim3(256), 0, s, n4, a, b, c, o);
+        else hipLaunchKernelGGL(k_stream<3>, dim3(blocks), dim3(256), 0, s, n4, a, b, c, o);
+    };
+    launch();
+    launch();
+    r = hipEventRecord(e0, s);
+    for (int k = 0; k < reps && r == hipSuccess; ++k) launch();
+    if (r == hipSuccess) r = hipGetLastError();
+    if (r == hipSuccess) r = hipEventRecord(e1, s);
+    if (r == hipSuccess) r = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (r == hipSuccess) r = hipEventElapsedTime(&ms, e0, e1);
+    release();
+    if (r != hipSuccess) PW_FAIL(ctx, PW_ERR_HIP, std::string("pw_stream_bandwidth: ") + hipGetErrorString(r));
+    *gbs = (double)(reads + 1) * (double)n * 4.0 * reps / ((double)ms * 1e-3) / 1e9;
     return PW_OK;
 }
 

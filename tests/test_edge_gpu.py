@@ -115,3 +115,19 @@ def test_fwi_watchdog_raises_on_a_blown_up_forward():
         with pytest.raises(NonFiniteError):
             p.forward(np.full((nz, nx), v, np.float32))
     p.close()
+
+
+def test_stream_bandwidth_probe():
+    # the streaming probe that bench.py reads the step kernels' traffic against: a plausible rate
+    # for each mix of streams (64 MB each), and argument checks before any device work
+    import ctypes
+    from pywave_amd import _native as N
+    L, ctx = N.lib(), N.context()
+    for reads in (1, 2, 3):
+        g = ctypes.c_double()
+        N.check(L.pw_stream_bandwidth(ctx, 1 << 24, reads, 3, ctypes.byref(g)), ctx, "stream_bandwidth")
+        assert 100.0 < g.value < 20000.0, (reads, g.value)
+    g = ctypes.c_double()
+    for n, reads, reps in ((1 << 24, 4, 3), ((1 << 24) + 2, 1, 3), (1 << 24, 1, 0), (0, 1, 3)):
+        with pytest.raises(ValueError):
+            N.check(L.pw_stream_bandwidth(ctx, n, reads, reps, ctypes.byref(g)), ctx, "stream_bandwidth")
