@@ -234,6 +234,37 @@ def test_cfg1_size_early_steps_vs_oracle(cfg1_early, variant):
     assert np.array_equal(p.output(), d.ravel())
 
 
+@pytest.fixture(scope="module")
+def large_grid():
+    # 241^3 + 30-cell ABC = 301^3 updated (304^3 padded): three fields of more than 240 MB, which
+    # the plan lays out with rows on 128 B boundaries (the 261^3 benchmark grid, whose fields fit
+    # the Infinity Cache, gets 64 B ones); the wavefield after a few steps around a central source
+    vel = plain3d_model(241)
+    nt = 12
+    # (a Ricker of twelve samples is all but zero: the wavelet is a smooth pulse of its own, which
+    # both the plan and the oracle take as given)
+    rick = O.ricker_bank(nt, 0.001, [15], [0.006], [1.0])
+    rick = (np.sin(np.arange(nt) * np.pi / nt) ** 2)[None, :].astype(np.float32).reshape(rick.shape)
+    d, snaps, _ = O.afd3d(vel, nt, 0.001, 10, 10, 10, 30, 0.01, [120], [120], [120], rick, jsnap=nt - 1)
+    return vel, nt, rick, d, snaps
+
+
+@pytest.mark.parametrize("variant", [-1, 15, 17, 24])
+def test_large_grid_layout_vs_oracle(large_grid, variant):
+    vel, nt, rick, d, snaps = large_grid
+    assert np.abs(snaps).max() > 1e-3
+    p = AfdPlan(vel, (10, 10, 10), nt, 0.001, 30, 0.01, nt - 1)
+    p.set_sources(np.array([120, 120, 120, 15, 0.006, 1.0], np.float32), 1, wavelet=rick)
+    if variant >= 0:
+        p.set_variant(variant, 0)
+    p.run()
+    out = p.output()
+    p.close()
+    n = vel.shape[0]
+    assert np.array_equal(out[:nt * n * n], d.ravel())
+    assert np.array_equal(out[nt * n * n:], snaps.ravel())
+
+
 def test_cfg1_full_run_linear_in_amplitude():
     # Linear in the source amplitude up to gradual underflow: ahead of the wavefront the field
     # passes through the f32 denormal range, where doubling is not exact (the CPU oracle shows
