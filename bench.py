@@ -675,8 +675,8 @@ def big3d_bench(n=261, nt=200):
     tuned = plan.autotune(steps=6)
     cells = (n + 2 * nb) ** 3
     alg_step = 16.0 * cells + 4.0 * n * n
-    res = {"workload": "afd3d %d^3 model + 30-cell ABC = %d^3 updated (324^3 padded: 3 fields of %.0f MB), nt=%d"
-                       % (n, n + 2 * nb, 324 ** 3 * 4 / 1e6, nt),
+    res = {"workload": "afd3d %d^3 model + 30-cell ABC = %d^3 updated (324^3 padded: 3 fields of more than "
+                       "136 MB, beyond the 256 MB Infinity Cache), nt=%d" % (n, n + 2 * nb, nt),
            "autotuned": list(tuned), "variants": []}
     info = np.zeros(4, np.int32)
     for v in range(L.pw_afd_variant_count()):
@@ -725,6 +725,38 @@ def calibrate_copy():
     print(json.dumps({"copy_bytes_read": a.numel() * 4, "copy_bytes_written": b.numel() * 4,
                       "copy_ms": round(ms, 4), "copy_GBs": round(2 * a.numel() * 4 / (ms * 1e-3) / 1e9, 1),
                       "stream": stream_bandwidth()}))
+
+
+def sizes_bench(sizes=(101, 141, 181, 201, 221, 241, 261), nt=200):
+    """The autotuned 3D step against the grid size (the headline model recipe at n^3 + 30-cell
+    ABC, a centred 15 Hz source, surface receivers, nt steps): where the three fields stop fitting
+    the 256 MB Infinity Cache, the rate falls towards the device's streaming rate."""
+    import ctypes
+    from pywave_amd import AfdPlan
+    from pywave_amd import _native as N
+    from pywave_amd.workloads import plain3d_model
+
+    out = []
+    for n in sizes:
+        nb = 30
+        m = n + 2 * nb
+        z2, x2, y2 = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        N.lib().pw_padded_sizes(m, m, m, ctypes.byref(z2), ctypes.byref(x2), ctypes.byref(y2))
+        c = n // 2
+        plan = AfdPlan(plain3d_model(n), (10.0, 10.0, 10.0), nt, 0.001, nb, 0.01, 0)
+        plan.set_sources(np.array([c, c, c, 15.0, 0.05, 1.0], np.float32), 1)
+        tuned = plan.autotune(steps=6)
+        plan.run()
+        _, step_ms = plan.run_timed(per_step=True)
+        plan.close()
+        cells = m ** 3
+        ach = (16.0 * cells + 4.0 * n * n) / (step_ms * 1e-3) / 1e9
+        out.append({"model": n, "updated": m, "padded": [z2.value, x2.value, y2.value],
+                    "three_fields_MB": round(3 * 4.0 * z2.value * x2.value * y2.value / 1e6, 1),
+                    "kernel_variant": list(tuned), "us_per_time_step": round(step_ms * 1e3, 2),
+                    "Gcell_per_s": round(cells / (step_ms * 1e-3) / 1e9, 1), "achieved_GBs": round(ach, 1),
+                    "frac_hbm": round(ach / PEAK_HBM_GBS, 4)})
+    return {"workload": "afd3d n^3 models + 30-cell ABC, nt=%d, autotuned kernel per grid" % nt, "grids": out}
 
 
 def stream_bandwidth(reps=10, n=1 << 27):
@@ -782,7 +814,7 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (HBM counter calibration)")
     ap.add_argument("--only", choices=["fwi", "tri", "batch", "small", "lstri", "aps", "visco", "fwifwd", "fwigrad",
-                                       "big3d"],
+                                       "big3d", "sizes"],
                     help="run one secondary workload alone and print its JSON object (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -800,7 +832,8 @@ def main():
                "aps": lambda: aps3d_bench(0 if args.no_cpu else 20),
                "visco": lambda: visco_bench(0 if args.no_cpu else 300),
                "fwifwd": lambda: fwi_forward_bench(widths=args.fwi_widths),
-               "fwigrad": lambda: fwi_gradient_bench(widths=args.fwi_widths), "big3d": big3d_bench}[args.only]
+               "fwigrad": lambda: fwi_gradient_bench(widths=args.fwi_widths), "big3d": big3d_bench,
+               "sizes": sizes_bench}[args.only]
         print(json.dumps({args.only: leg()}))
         return
 
