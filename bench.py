@@ -759,18 +759,22 @@ def sizes_bench(sizes=(101, 141, 181, 201, 221, 241, 261), nt=200):
     return {"workload": "afd3d n^3 models + 30-cell ABC, nt=%d, autotuned kernel per grid" % nt, "grids": out}
 
 
-def stream_bandwidth(reps=10, n=1 << 27):
+def stream_bandwidth(reps=10, n=1 << 27, n_small=12_500_000):
     """The device's plain streaming rate (pw_stream_bandwidth: 1, 2 or 3 input streams and one
-    output stream of 2^27 floats, 512 MB each, more than the Infinity Cache holds): the practical
-    HBM ceiling against which the step kernels' actual traffic is read."""
+    output stream of 2^27 floats, 512 MB each, more than the 256 MB Infinity Cache holds, with
+    non-temporal accesses): the practical HBM ceiling against which the step kernels' actual
+    traffic is read.  The same mixes with the default cache policy at 512 MB a stream (`cached`)
+    and at 50 MB a stream (100-200 MB in all, which the Infinity Cache holds between launches:
+    `in_cache`) give the ceiling of kernels whose fields fit that cache."""
     import ctypes
     from pywave_amd import _native as N
     L, ctx = N.lib(), N.context()
     out = {}
-    for reads in (1, 2, 3):
-        g = ctypes.c_double()
-        N.check(L.pw_stream_bandwidth(ctx, n, reads, reps, ctypes.byref(g)), ctx, "stream_bandwidth")
-        out["read%d_write1_GBs" % reads] = round(g.value, 1)
+    for key, size, cached in (("", n, 0), ("_cached", n, 1), ("_in_cache", n_small, 1)):
+        for reads in (1, 2, 3):
+            g = ctypes.c_double()
+            N.check(L.pw_stream_bandwidth(ctx, size, reads, reps, cached, ctypes.byref(g)), ctx, "stream_bandwidth")
+            out["read%d_write1%s_GBs" % (reads, key)] = round(g.value, 1)
     return out
 
 

@@ -45,9 +45,11 @@ int pw_use_own_stream(pw_ctx *ctx);
 int pw_synchronize(pw_ctx *ctx);
 /* streaming bandwidth of the device, for reading the step kernels' rooflines: `reads` input
  * streams (1-3) and one output stream of n floats each (n a positive multiple of 4), reps timed
- * launches on the context's stream; *gbs = bytes moved / s / 1e9.  Allocates (reads + 1) * n * 4
- * bytes for the call.  No counterpart in the reference. */
-int pw_stream_bandwidth(pw_ctx *ctx, long long n, int reads, int reps, double *gbs);
+ * launches on the context's stream; cached 0 uses non-temporal loads and stores, cached 1 the
+ * default policy (buffers that together fit the Infinity Cache then stay in it between launches);
+ * *gbs = bytes moved / s / 1e9.  Allocates (reads + 1) * n * 4 bytes for the call.  No counterpart
+ * in the reference. */
+int pw_stream_bandwidth(pw_ctx *ctx, long long n, int reads, int reps, int cached, double *gbs);
 
 /* ---------------------------------------------------------------- host setup helpers */
 /* kiss_fft_next_fast_size (pywave/src/wave_kissfft.c:412-424) */

@@ -29,7 +29,7 @@ SIGNATURES = {
     "pw_set_stream": (c_int, [c_vp, c_vp]),
     "pw_use_own_stream": (c_int, [c_vp]),
     "pw_synchronize": (c_int, [c_vp]),
-    "pw_stream_bandwidth": (c_int, [c_vp, c_ll, c_int, c_int, c_vp]),
+    "pw_stream_bandwidth": (c_int, [c_vp, c_ll, c_int, c_int, c_int, c_vp]),
     "pw_next_fast_size": (c_int, [c_int]),
     "pw_padded_sizes": (None, [c_int, c_int, c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int), ctypes.POINTER(c_int)]),
     "pw_abc_weights": (None, [c_int, c_float, c_vp]),

@@ -308,16 +308,20 @@ __global__ void k_zero_u32(unsigned* p) { *p = 0; }
 // stream with the non-temporal cache policy (each byte is touched once), grid-stride: the plain
 // streaming rate against which the step kernels' traffic is read
 typedef float f32x4 __attribute__((ext_vector_type(4)));
-template <int R>
+// (CACHED 0: non-temporal loads and stores; 1: the default policy, under which buffers that fit
+// the Infinity Cache stay there between launches)
+template <int R, int CACHED>
 __global__ __launch_bounds__(256) void k_stream(long long n4, const f32x4* __restrict__ a,
                                                 const f32x4* __restrict__ b, const f32x4* __restrict__ c,
                                                 f32x4* __restrict__ o)
 {
+    auto ld = [](const f32x4* p) { return CACHED ? *p : __builtin_nontemporal_load(p); };
     for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n4; i += (long long)gridDim.x * 256) {
-        f32x4 v = __builtin_nontemporal_load(&a[i]);
-        if constexpr (R > 1) v = v + __builtin_nontemporal_load(&b[i]);
-        if constexpr (R > 2) v = v + __builtin_nontemporal_load(&c[i]);
-        __builtin_nontemporal_store(v, &o[i]);
+        f32x4 v = ld(&a[i]);
+        if constexpr (R > 1) v = v + ld(&b[i]);
+        if constexpr (R > 2) v = v + ld(&c[i]);
+        if constexpr (CACHED) o[i] = v;
+        else __builtin_nontemporal_store(v, &o[i]);
     }
 }
 
@@ -417,10 +421,11 @@ int pw_synchronize(pw_ctx* ctx)
 // Streaming bandwidth of the device: `reads` input streams (1, 2 or 3) and one output stream of n
 // floats each (n a positive multiple of 4), `reps` timed launches after two untimed ones;
 // *gbs = bytes moved per second / 1e9.  Allocates its buffers for the call.
-int pw_stream_bandwidth(pw_ctx* ctx, long long n, int reads, int reps, double* gbs)
+int pw_stream_bandwidth(pw_ctx* ctx, long long n, int reads, int reps, int cached, double* gbs)
 {
-    if (!ctx || !gbs || n <= 0 || n % 4 || reads < 1 || reads > 3 || reps < 1)
-        PW_FAIL(ctx, PW_ERR_ARG, "pw_stream_bandwidth: n must be a positive multiple of 4, reads 1-3, reps >= 1");
+    if (!ctx || !gbs || n <= 0 || n % 4 || reads < 1 || reads > 3 || reps < 1 || cached < 0 || cached > 1)
+        PW_FAIL(ctx, PW_ERR_ARG,
+                "pw_stream_bandwidth: n must be a positive multiple of 4, reads 1-3, reps >= 1, cached 0 or 1");
     PW_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t s = ctx->s();
     float* buf[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -448,9 +453,15 @@ int pw_stream_bandwidth(pw_ctx* ctx, long long n, int reads, int reps, double* g
                 *c = (const f32x4*)buf[reads > 2 ? 2 : 0];
     f32x4* o = (f32x4*)buf[reads];
     auto launch = [&] {
-        if (reads == 1) hipLaunchKernelGGL(k_stream<1>, dim3(blocks), dim3(256), 0, s, n4, a, b, c, o);
-        else if (reads == 2) hipLaunchKernelGGL(k_stream<2>, dim3(blocks), dim3(256), 0, s, n4, a, b, c, o);
-        else hipLaunchKernelGGL(k_stream<3>, dim3(blocks), dim3(256), 0, s, n4, a, b, c, o);
+        const int k = 2 * (reads - 1) + cached;
+        switch (k) {
+            case 0: hipLaunchKernelGGL((k_stream<1, 0>), dim3(blocks), dim3(256), 0, s, n4, a, b, c, o); break;
+            case 1: hipLaunchKernelGGL((k_stream<1, 1>), dim3(blocks), dim3(256), 0, s, n4, a, b, c, o); break;
+            case 2: hipLaunchKernelGGL((k_stream<2, 0>), dim3(blocks), dim3(256), 0, s, n4, a, b, c, o); break;
+            case 3: hipLaunchKernelGGL((k_stream<2, 1>), dim3(blocks), dim3(256), 0, s, n4, a, b, c, o); break;
+            case 4: hipLaunchKernelGGL((k_stream<3, 0>), dim3(blocks), dim3(256), 0, s, n4, a, b, c, o); break;
+            default: hipLaunchKernelGGL((k_stream<3, 1>), dim3(blocks), dim3(256), 0, s, n4, a, b, c, o); break;
+        }
     };
     launch();
     launch();

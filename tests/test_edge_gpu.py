@@ -124,10 +124,12 @@ def test_stream_bandwidth_probe():
     from pywave_amd import _native as N
     L, ctx = N.lib(), N.context()
     for reads in (1, 2, 3):
-        g = ctypes.c_double()
-        N.check(L.pw_stream_bandwidth(ctx, 1 << 24, reads, 3, ctypes.byref(g)), ctx, "stream_bandwidth")
-        assert 100.0 < g.value < 20000.0, (reads, g.value)
+        for cached in (0, 1):
+            g = ctypes.c_double()
+            N.check(L.pw_stream_bandwidth(ctx, 1 << 24, reads, 3, cached, ctypes.byref(g)), ctx, "stream_bandwidth")
+            assert 100.0 < g.value < 40000.0, (reads, cached, g.value)
     g = ctypes.c_double()
-    for n, reads, reps in ((1 << 24, 4, 3), ((1 << 24) + 2, 1, 3), (1 << 24, 1, 0), (0, 1, 3)):
+    for n, reads, reps, cached in ((1 << 24, 4, 3, 0), ((1 << 24) + 2, 1, 3, 0), (1 << 24, 1, 0, 0), (0, 1, 3, 0),
+                                   (1 << 24, 1, 3, 2)):
         with pytest.raises(ValueError):
-            N.check(L.pw_stream_bandwidth(ctx, n, reads, reps, ctypes.byref(g)), ctx, "stream_bandwidth")
+            N.check(L.pw_stream_bandwidth(ctx, n, reads, reps, cached, ctypes.byref(g)), ctx, "stream_bandwidth")
