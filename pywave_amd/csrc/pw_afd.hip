@@ -20,6 +20,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <type_traits>
 
 #include <hip/hip_ext.h>
 
@@ -439,7 +440,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_afd3d_pipe(Geo g, Step a, int nt
     auto yok = [&](int yy) { return yy >= 0 && yy < g.ny; };
 
     // prologue: centre planes y0-2 .. y0+2, and plane y0's prev, vv and halos
-    float q[RPW][5];
+    float q[RPW][6];
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
         const int yy = y0 - 2 + k;
@@ -448,69 +449,79 @@ __global__ __launch_bounds__(64 * WAVES) void k_afd3d_pipe(Geo g, Step a, int nt
 #pragma unroll
         for (int r = 0; r < RPW; ++r) q[r][k] = ld(rs, oc[r], so);
     }
-    float pr[RPW], vc[RPW], hz[RPW], hx;
-    float fyv = a.fy1[y0];
+    // The operands of one plane are double-buffered by plane parity, and the centre planes sit in
+    // a ring of six slots indexed by plane mod 6: five in use and one free for the plane being
+    // prefetched.  The y loop is unrolled by 6 so that every ring slot and buffer is a fixed
+    // register: no value is copied from plane to plane, and a prefetch never targets a register
+    // whose previous load is still to be read (which would make the wave wait for all its loads in
+    // flight before issuing the next plane's).
+    float pr[2][RPW], vc[2][RPW], hz[2][RPW], hx[2], fyb[2];
+    fyb[0] = a.fy1[y0];
     {
         const unsigned so = (unsigned)y0 * pstep;
 #pragma unroll
         for (int r = 0; r < RPW; ++r) {
-            pr[r] = ldn(rP, oc[r], so);
-            vc[r] = ldn(rV, oc[r], so);
-            hz[r] = ld(rC, ohz[r], so);
+            pr[0][r] = ldn(rP, oc[r], so);
+            vc[0][r] = ldn(rV, oc[r], so);
+            hz[0][r] = ld(rC, ohz[r], so);
         }
-        hx = ld(rC, ohx, so);
+        hx[0] = ld(rC, ohx, so);
     }
     // block-uniform: does this z tile hold the receiver plane (z == nb)?
     const bool rtile = g.nb >= z0 && g.nb < z0 + TZ;
-    unsigned s0 = (unsigned)y0 * pstep;  // plane y
-    int buf = 0;
-    for (int y = y0; y < y1; ++y) {
-        // prefetch for the next iteration: plane y+1 (prev, vv, halos) and the centre of y+3
+    // plane y at phase K = (y - y0) mod 6: its operands in buffers [K & 1] (LDS buffer too), the
+    // centre planes y-2 .. y+3 in ring slots (K + j) mod 6, j = 0 .. 5 (plane y+3 being loaded)
+    auto plane = [&](auto Kc, int y) {
+        constexpr int K = decltype(Kc)::value;
+        constexpr int cu = K & 1, nu = cu ^ 1;
+        auto Q = [&](int r, int j) -> float& { return q[r][(K + j) % 6]; };
+        // prefetch for the next plane: plane y+1 (prev, vv, halos) and the centre of y+3
+        const unsigned s0 = (unsigned)y * pstep;
         const bool nv = y + 1 < y1;
         const bool n3 = nv && y + 3 < g.ny;
         const unsigned s1 = s0 + pstep, s3 = s0 + 3 * pstep;
         const auto rc1 = nv ? rC : rN, rp1 = nv ? rP : rN, rv1 = nv ? rV : rN, rc3 = n3 ? rC : rN;
-        float npr[RPW], nvc[RPW], nhz[RPW], nq[RPW], nhx;
         // vector load (the field stores in this kernel keep the compiler off the scalar cache),
         // so it is prefetched like the rest
-        const float nfy = a.fy1[y + 1];
+        fyb[nu] = a.fy1[y + 1];
 #pragma unroll
         for (int r = 0; r < RPW; ++r) {
-            nq[r] = ld(rc3, oc[r], s3);
-            npr[r] = ldn(rp1, oc[r], s1);
-            nvc[r] = ldn(rv1, oc[r], s1);
-            nhz[r] = ld(rc1, ohz[r], s1);
+            Q(r, 5) = ld(rc3, oc[r], s3);
+            pr[nu][r] = ldn(rp1, oc[r], s1);
+            vc[nu][r] = ldn(rv1, oc[r], s1);
+            hz[nu][r] = ld(rc1, ohz[r], s1);
         }
-        nhx = ld(rc1, ohx, s1);
-        float(*L)[LW] = lds[buf];
+        hx[nu] = ld(rc1, ohx, s1);
+        float(*L)[LW] = lds[cu];
 #pragma unroll
         for (int r = 0; r < RPW; ++r) {
-            L[w * RPW + r + 2][lane + 2] = q[r][2];
-            L[w * RPW + r + 2][lzh] = lane < 4 ? hz[r] : q[r][2];
+            L[w * RPW + r + 2][lane + 2] = Q(r, 2);
+            L[w * RPW + r + 2][lzh] = lane < 4 ? hz[cu][r] : Q(r, 2);
         }
-        if (w < 4) L[lrh][lane + 2] = hx;
+        if (w < 4) L[lrh][lane + 2] = hx[cu];
         __syncthreads();
         const bool yin = y >= 2 && y < g.ny2 - 2;
+        const float fyv = fyb[cu];
         float c[RPW];
 #pragma unroll
         for (int r = 0; r < RPW; ++r) {
             const int lr = w * RPW + r + 2;
-            const float u = q[r][2];
+            const float u = Q(r, 2);
             float lap = C0 * u * g.cc
-                      + C1 * (q[r][1] + q[r][3]) * g.idy
-                      + C2 * (q[r][0] + q[r][4]) * g.idy
+                      + C1 * (Q(r, 1) + Q(r, 3)) * g.idy
+                      + C2 * (Q(r, 0) + Q(r, 4)) * g.idy
                       + C1 * (L[lr - 1][lane + 2] + L[lr + 1][lane + 2]) * g.idx
                       + C2 * (L[lr - 2][lane + 2] + L[lr + 2][lane + 2]) * g.idx
                       + C1 * (L[lr][lane + 1] + L[lr][lane + 3]) * g.idz
                       + C2 * (L[lr][lane] + L[lr][lane + 4]) * g.idz;
             lap = (yin && zin && xin[r]) ? lap : 0.f;
-            const float p = pr[r] * fz * fx[r] * fyv;
+            const float p = pr[cu][r] * fz * fx[r] * fyv;
             c[r] = u + (u - p);
-            c[r] = c[r] + lap * vc[r];
+            c[r] = c[r] + lap * vc[cu][r];
         }
         if (blk_src) {
 #pragma unroll
-            for (int r = 0; r < RPW; ++r) c[r] = inject(c[r], vc[r], a, z, x0 + w * RPW + r, y);
+            for (int r = 0; r < RPW; ++r) c[r] = inject(c[r], vc[cu][r], a, z, x0 + w * RPW + r, y);
         }
         const bool yint = y >= g.nb && y < g.nb + g.ny1;
         if (a.tri && rtile && yint) {  // time-reversal injection on the receiver plane
@@ -518,7 +529,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_afd3d_pipe(Geo g, Step a, int nt
             for (int r = 0; r < RPW; ++r) {
                 const int x = x0 + w * RPW + r;
                 if (z == g.nb && x >= g.nb && x < g.nb + g.nx1)
-                    c[r] = c[r] + vc[r] * a.tri[(y - g.nb) * g.nx1 + (x - g.nb)];
+                    c[r] = c[r] + vc[cu][r] * a.tri[(y - g.nb) * g.nx1 + (x - g.nb)];
             }
         }
 #pragma unroll
@@ -541,15 +552,14 @@ __global__ __launch_bounds__(64 * WAVES) void k_afd3d_pipe(Geo g, Step a, int nt
                     a.snap[((size_t)(y - g.nb) * g.nx1 + (x - g.nb)) * g.nz1 + (z - g.nb)] = c[r];
             }
         }
-#pragma unroll
-        for (int r = 0; r < RPW; ++r) {
-            q[r][0] = q[r][1]; q[r][1] = q[r][2]; q[r][2] = q[r][3]; q[r][3] = q[r][4]; q[r][4] = nq[r];
-            pr[r] = npr[r]; vc[r] = nvc[r]; hz[r] = nhz[r];
-        }
-        hx = nhx;
-        fyv = nfy;
-        buf ^= 1;
-        s0 = s1;
+    };
+    for (int yb = y0; yb < y1; yb += 6) {
+        plane(std::integral_constant<int, 0>{}, yb);
+        if (yb + 1 < y1) plane(std::integral_constant<int, 1>{}, yb + 1);
+        if (yb + 2 < y1) plane(std::integral_constant<int, 2>{}, yb + 2);
+        if (yb + 3 < y1) plane(std::integral_constant<int, 3>{}, yb + 3);
+        if (yb + 4 < y1) plane(std::integral_constant<int, 4>{}, yb + 4);
+        if (yb + 5 < y1) plane(std::integral_constant<int, 5>{}, yb + 5);
     }
 }
 
