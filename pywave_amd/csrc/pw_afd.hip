@@ -395,15 +395,19 @@ __global__ __launch_bounds__(64 * WAVES) void k_afd3d_pipe(Geo g, Step a, int nt
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int z = z0 + lane;
     const int ldz = g.ldz;
-    // one resource per field from 2 rows + 2 cells before the grid to as far after it; lane
-    // offsets (within a plane, constant over the loop) are vector, the plane offset is the
-    // scalar soffset; a plane outside [0, ny) or outside this block's needs uses a resource of
-    // size 0, so none of its loads touches memory
+    // one resource per field over the planes this block touches (yb .. ye-1: its chunk and two
+    // planes either side, within [0, ny)), from 2 rows + 2 cells before plane yb; lane offsets
+    // (within a plane, constant over the loop) are vector, the plane offset from yb is the scalar
+    // soffset; a plane outside [0, ny) or outside this block's needs uses a resource of size 0, so
+    // none of its loads touches memory.  The window of a chunk stays below 2 GiB (the launch
+    // bounds the chunk length), whatever the size of the field.
     const int bias = 2 * ldz + 2;
-    const unsigned fbytes = (unsigned)((g.ldxy * g.ny2 + 2 * bias) * 4);
+    const int yb = y0 >= 2 ? y0 - 2 : 0, ye = min(y1 + 2, g.ny);
+    const long long base = (long long)yb * g.ldxy - bias;
+    const unsigned fbytes = (unsigned)(((long long)(ye - yb) * g.ldxy + 2 * bias) * 4);
     const unsigned pstep = (unsigned)(g.ldxy * 4);  // bytes per plane
-    const auto rC = plane_rsrc(a.cur - bias, fbytes), rP = plane_rsrc(a.pv - bias, fbytes),
-               rV = plane_rsrc(a.vv - bias, fbytes), rN = plane_rsrc(a.cur - bias, 0u);
+    const auto rC = plane_rsrc(a.cur + base, fbytes), rP = plane_rsrc(a.pv + base, fbytes),
+               rV = plane_rsrc(a.vv + base, fbytes), rN = plane_rsrc(a.cur - bias, 0u);
     bool blk_src = false;
     for (int s = 0; s < a.ns; ++s) {
         int gz = a.src[3 * s], gx = a.src[3 * s + 1], gy = a.src[3 * s + 2];
@@ -445,7 +449,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_afd3d_pipe(Geo g, Step a, int nt
     for (int k = 0; k < 5; ++k) {
         const int yy = y0 - 2 + k;
         const auto rs = yok(yy) ? rC : rN;
-        const unsigned so = yok(yy) ? (unsigned)yy * pstep : 0u;
+        const unsigned so = yok(yy) ? (unsigned)(yy - yb) * pstep : 0u;
 #pragma unroll
         for (int r = 0; r < RPW; ++r) q[r][k] = ld(rs, oc[r], so);
     }
@@ -458,7 +462,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_afd3d_pipe(Geo g, Step a, int nt
     float pr[2][RPW], vc[2][RPW], hz[2][RPW], hx[2], fyb[2];
     fyb[0] = a.fy1[y0];
     {
-        const unsigned so = (unsigned)y0 * pstep;
+        const unsigned so = (unsigned)(y0 - yb) * pstep;
 #pragma unroll
         for (int r = 0; r < RPW; ++r) {
             pr[0][r] = ldn(rP, oc[r], so);
@@ -476,7 +480,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_afd3d_pipe(Geo g, Step a, int nt
         constexpr int cu = K & 1, nu = cu ^ 1;
         auto Q = [&](int r, int j) -> float& { return q[r][(K + j) % 6]; };
         // prefetch for the next plane: plane y+1 (prev, vv, halos) and the centre of y+3
-        const unsigned s0 = (unsigned)y * pstep;
+        const unsigned s0 = (unsigned)(y - yb) * pstep;
         const bool nv = y + 1 < y1;
         const bool n3 = nv && y + 3 < g.ny;
         const unsigned s1 = s0 + pstep, s3 = s0 + 3 * pstep;
@@ -1359,7 +1363,9 @@ struct pw_afd {
     float* d_fx1 = nullptr;
     float* d_fy1 = nullptr;
     bool lean_ok = false;  // faces do not overlap, 32-bit offsets
-    bool pipe_ok = false;  // lean_ok and every field fits one 2 GiB buffer window
+    bool pipe_ok = false;  // lean_ok and every field fits one 2 GiB buffer window (two-step kernels)
+    bool chunk_ok = false;  // lean_ok and a y chunk of one plane fits a 2 GiB window (single-step
+                            // pipelined kernels, whose windows cover one chunk)
     std::vector<float> tri_host;  // echoed back as the trace part in TRI mode
     bool have_vel = false, have_src = false, have_tri = false;
     int last_mode = -1;  // 0 modelling, 1 tri
@@ -1374,17 +1380,25 @@ struct pw_afd {
 
 namespace {
 
+// longest y chunk whose planes (the chunk and two either side, plus the 2-row bias at each end)
+// fit the pipelined kernels' 2 GiB buffer windows
+int max_chunk_planes(const Geo& g)
+{
+    const double plane = 4.0 * g.ldxy, bias = 4.0 * (2.0 * g.ldz + 2.0);
+    return (int)std::min(1.0e6, std::floor((2147483648.0 - 2.0 * bias) / plane)) - 5;
+}
+
 // the 3D kernel used when none was chosen
 int default_variant(const pw_afd* p)
 {
-    return p->pipe_ok ? kLeanDefault : (p->lean_ok ? kLeanOnlyDefault : kGeneralDefault);
+    return p->chunk_ok ? kLeanDefault : (p->lean_ok ? kLeanOnlyDefault : kGeneralDefault);
 }
 
 // whether 3D variant vi can run on this plan's grid
 bool variant_ok(const pw_afd* p, int vi)
 {
     const int k = kVariants3[vi].lean;
-    return k == 0 || (k == 1 && p->lean_ok) || (k >= 2 && p->pipe_ok);
+    return k == 0 || (k == 1 && p->lean_ok) || (k >= 2 && k <= 4 && p->chunk_ok) || (k == 5 && p->pipe_ok);
 }
 
 // blocks of 3D variant vi that the device holds at once (CUs x blocks per CU), cached per plan
@@ -1443,6 +1457,7 @@ int afd_launch_step(pw_afd* p, const Step& a, hipEvent_t e0 = nullptr, hipEvent_
             const int nch = (int)std::max(1LL, cap / tiles);
             nyc = std::max(8, pw::ceil_div(g.ny, nch));
         }
+        if (v.lean >= 2 && v.lean <= 4) nyc = std::min(nyc, max_chunk_planes(g));  // chunk window < 2 GiB
         const int nch = pw::ceil_div(g.ny, nyc);
         const int nblk = ntz * ntx * nch;
         const int grid = pw::ceil_div(nblk, 8) * 8;
@@ -1646,6 +1661,7 @@ pw_afd* pw_afd_create(pw_ctx* ctx, int ndim, int nz, int nx, int ny, float dz, f
     lean = lean && (double)(g.ldxy + 8 * g.ldz + 8) * 4.0 < 2147483648.0;  // pipelined plane windows
     p->lean_ok = lean;
     p->pipe_ok = lean && (double)p->raw_elems * 4.0 < 2147483648.0;
+    p->chunk_ok = lean && max_chunk_planes(g) >= 1;
 
     const size_t nrec = (size_t)g.nx1 * g.ny1;
     bool ok = hipMalloc(&p->raw[0], p->raw_elems * 4) == hipSuccess &&

@@ -272,6 +272,33 @@ def test_large_grid_layout_vs_oracle(large_grid, variant):
     assert np.array_equal(out[nt * n * n:], snaps.ravel())
 
 
+def test_fields_beyond_2gib_pipelined_equals_lean():
+    # 781^3 + 30-cell ABC = 841^3 updated: each field holds more than 2 GiB, so the pipelined
+    # single-step kernels address one y chunk per buffer window (the two-step kernels, whose
+    # windows span the whole field, do not apply); their traces equal bit for bit those of the lean
+    # kernel (32-bit offsets over the whole field, pinned against the oracle on the small grids),
+    # with sources just below the surface at both y ends and in the middle
+    vel = plain3d_model(781)
+    nt = 8
+    pulse = (np.sin(np.arange(nt) * np.pi / nt) ** 2).astype(np.float32)
+    src = np.array([[3, 390, 1, 15, 0.0, 1.0], [3, 100, 779, 15, 0.0, 1.0], [4, 700, 390, 15, 0.0, 1.0]],
+                   np.float32)
+    p = AfdPlan(vel, (10, 10, 10), nt, 0.001, 30, 0.01, 0)
+    p.set_sources(src.ravel(), 3, wavelet=np.repeat(pulse[None, :], 3, axis=0))
+    outs = {}
+    for variant in (8, 15, 17):
+        p.set_variant(variant, 0)
+        p.run()
+        outs[variant] = p.output().copy()
+    with pytest.raises(ValueError):
+        p.set_variant(24, 0)
+        p.run()
+    p.close()
+    assert np.abs(outs[8]).max() > 1e-3
+    assert np.array_equal(outs[15], outs[8])
+    assert np.array_equal(outs[17], outs[8])
+
+
 def test_cfg1_full_run_linear_in_amplitude():
     # Linear in the source amplitude up to gradual underflow: ahead of the wavefront the field
     # passes through the f32 denormal range, where doubling is not exact (the CPU oracle shows
